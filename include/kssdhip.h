@@ -1,0 +1,155 @@
+/*
+ * kssdhip -- MI355X (gfx950) sketch-comparison kernels behind a plain C ABI.
+ *
+ * This is the drop-in boundary for the KSSD3A `ani` / `matrix` counting path.
+ * Every type below is byte-compatible with the reference's in-memory types,
+ * so reference-style host code can hand its own arrays straight in:
+ *
+ *   kssd_ctxgidobj_t    == ctxgidobj_t    sketch_rearrange.h:19 (packed, 12 B)
+ *   kssd_ctx_mut2_t     == ctx_mut2_t     command_ani.h:120-124
+ *   kssd_obj_section_t  == obj_section_t  command_ani.h:114-118
+ *   kssd_ani_features_t == ani_features_t model_ani.h:215-222
+ *
+ * Conventions
+ *   - Return value: 0 on success, a negative KSSDHIP_E* code on failure;
+ *     kssdhip_last_error() then returns a message (per calling thread).
+ *     The reference exits the process instead (err()/errx()); the host
+ *     front end of this package maps a nonzero return to that behaviour.
+ *   - Functions without a _dev suffix take HOST pointers and are synchronous.
+ *     Functions with a _dev suffix take DEVICE pointers and a hipStream_t
+ *     passed as void*; they are asynchronous on that stream.
+ *   - Outputs are OVERWRITTEN (the library zeroes them); the reference
+ *     accumulates into arrays the caller zeroed, which gives the same result.
+ *   - One kssdhip_index may be used by one host thread at a time; distinct
+ *     handles are independent.
+ */
+#ifndef KSSDHIP_H
+#define KSSDHIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct __attribute__((packed)) {
+	uint64_t ctxgid; /* ctx << 20 | gid */
+	uint32_t obj;
+} kssd_ctxgidobj_t;
+
+typedef struct {
+	uint32_t num_ctx;
+	uint32_t num_mut2_ctx;
+} kssd_ctx_mut2_t;
+
+typedef struct {
+	uint32_t diff_obj;
+	uint32_t diff_obj_section;
+} kssd_obj_section_t;
+
+typedef struct {
+	uint32_t XnY_ctx;
+	uint32_t X_ctx;
+	uint32_t N_diff_obj_section;
+	uint32_t N_mut2_ctx;
+	uint32_t N_diff_obj;
+} kssd_ani_features_t;
+
+typedef struct kssdhip_index kssdhip_index;
+
+/* count flags */
+#define KSSDHIP_LOWER 0x1u          /* only gid < qry_gid (command_ani.h:685-689) */
+#define KSSDHIP_UNIQUE 0x2u         /* also fill unique-owner counters (command_ani.h:564-574) */
+#define KSSDHIP_IGNORECONFLICT 0x4u /* skip reference runs with >1 object (command_ani.h:532) */
+
+/* error codes */
+#define KSSDHIP_EINVAL (-1)
+#define KSSDHIP_ENOMEM (-2)
+#define KSSDHIP_EHIP (-3)
+#define KSSDHIP_ENODEV (-4)
+
+/* Message for the last failure on this thread ("" if none). */
+const char *kssdhip_last_error(void);
+
+/* Select the HIP device used by handles created afterwards on this thread.
+ * No reference counterpart (the reference has no device). */
+int kssdhip_init(int device);
+
+/*
+ * Upload a sorted context index (the contents of sortedcomb_ctxgid64obj32)
+ * to HBM and build its bucket table there.
+ * Replaces: reading the index (command_ani.c:508-516, :1550-1575) and
+ * kssd_build_fenceposts_ctxgid (command_ani.c:3656-3678), which the
+ * reference redoes on every block (command_ani.h:476, :650).
+ * records must be sorted by (ctxgid, obj) (kssdlib_sort.c:145-148);
+ * n < 2^32; ref_gnum <= 2^20; obj_bits in 1..32.
+ */
+int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int ref_gnum,
+		       unsigned obj_bits, kssdhip_index **out);
+int kssdhip_index_close(kssdhip_index *index);
+/* Device bytes held by the handle (index arrays, bucket table, scratch). */
+size_t kssdhip_index_device_bytes(const kssdhip_index *index);
+
+/*
+ * Count one block of queries against the index.
+ * Replaces the counting part of
+ *   count_ctx_obj_frm_comb_sketch_section_with_unique (command_ani.h:463-575)
+ *   count_ctx_obj_frm_comb_sketch_section_lower       (command_ani.h:635-721)
+ * Arguments mirror the reference: section_sk are the block's query keys
+ * (each query sorted ascending), section_skidx are section_gnum+1 ABSOLUTE
+ * offsets into the query comblco (the block starts at section_skidx[0]),
+ * qry_gid_offset is the gid of the block's first query (LOWER only).
+ * ctx/obj (and uctx/uobj with KSSDHIP_UNIQUE) are dense row-major
+ * [section_gnum][ref_gnum] arrays (MCTX/MOBJ, command_ani.h:435-436).
+ * Ranking, filtering and printing stay with the caller (command_ani.h:577-621).
+ */
+int kssdhip_count_block(kssdhip_index *index, const uint64_t *section_sk,
+			const uint64_t *section_skidx, int section_gnum,
+			int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *ctx,
+			kssd_obj_section_t *obj, kssd_ctx_mut2_t *unique_ctx,
+			kssd_obj_section_t *unique_obj);
+
+/*
+ * Device-resident form of kssdhip_count_block for batched and serving use:
+ * d_keys/d_skidx/d_ctx/... are device pointers; d_skidx holds section_gnum+1
+ * offsets RELATIVE to d_keys.  Asynchronous on `stream` (a hipStream_t).
+ * No reference counterpart beyond kssdhip_count_block.
+ */
+int kssdhip_count_block_dev(kssdhip_index *index, const uint64_t *d_keys,
+			    const uint64_t *d_skidx, int section_gnum,
+			    uint64_t total_keys, int qry_gid_offset, uint32_t flags,
+			    kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
+			    kssd_ctx_mut2_t *d_unique_ctx,
+			    kssd_obj_section_t *d_unique_obj, void *stream);
+
+/*
+ * Leftmost index record whose context equals each query key's context, or
+ * SIZE_MAX.  Replaces kssd_find_first_occurrences_fenceposts
+ * (command_ani.c:3703-3738).  a: n sorted keys (host); idx: n results (host).
+ */
+int kssdhip_find_first(kssdhip_index *index, const uint64_t *a, size_t n,
+		       uint64_t *idx);
+
+/*
+ * Plain kernel: the features of ONE (query, reference) pair, one launch.
+ * Replaces get_ani_features_ctx_min_over_conflicts_both_filtered
+ * (command_ani.h:303-346).  a (n keys) and b (m keys) are host arrays, each
+ * sorted ascending; X_ctx of the result is 0, as in the reference.
+ */
+int kssdhip_pair_features(const uint64_t *a, size_t n, const uint64_t *b,
+			  size_t m, unsigned obj_bits, int ignore_b_conflict,
+			  kssd_ani_features_t *out);
+
+/*
+ * Device-resident form of the plain kernel (one launch per call):
+ * d_a/d_b device arrays, d_out a device kssd_ani_features_t that is
+ * overwritten.  Asynchronous on `stream`.
+ */
+int kssdhip_pair_features_dev(const uint64_t *d_a, size_t n, const uint64_t *d_b,
+			      size_t m, unsigned obj_bits, int ignore_b_conflict,
+			      kssd_ani_features_t *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

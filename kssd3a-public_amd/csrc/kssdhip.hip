@@ -1,0 +1,704 @@
+// kssdhip -- sketch-comparison kernels for MI355X (gfx950), C ABI in
+// include/kssdhip.h.
+//
+// Layout in HBM for a resident index of N records (sorted by ctxgid, obj):
+//   ctxgid : u64[N]   ctx << 20 | gid          (8 B/record)
+//   obj    : u32[N]   object bits              (4 B/record)
+//   bucket : u32[2^k + 1]  bucket[t] = first record whose context has
+//            top-k bits (of its actual bit width) >= t
+// Dense per-pair counters are u64 cells laid out exactly like the
+// reference's ctx_mut2_t / obj_section_t arrays (low word first), so a single
+// 64-bit atomic add updates both fields of a cell.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "kssdhip.h"
+#include "kssdhip_internal.h"
+
+// ---------------------------------------------------------------- errors
+static thread_local char g_err[1024];
+static thread_local int g_device = 0;
+
+int kssdhip_fail(int code, const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof g_err, fmt, ap);
+	va_end(ap);
+	return code;
+}
+
+extern "C" const char *kssdhip_last_error(void) { return g_err; }
+
+#define HIP_TRY(expr)                                                                  \
+	do {                                                                               \
+		hipError_t e_ = (expr);                                                        \
+		if (e_ != hipSuccess)                                                          \
+			return kssdhip_fail(KSSDHIP_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+	} while (0)
+
+extern "C" int kssdhip_init(int device)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+		return kssdhip_fail(KSSDHIP_ENODEV, "no HIP device available");
+	if (device < 0 || device >= n)
+		return kssdhip_fail(KSSDHIP_EINVAL, "device %d out of range (0..%d)", device, n - 1);
+	HIP_TRY(hipSetDevice(device));
+	g_device = device;
+	g_err[0] = '\0';
+	return 0;
+}
+
+// ---------------------------------------------------------------- device helpers
+namespace kssd {
+
+__device__ __forceinline__ int dna_popcount(uint32_t x)
+{
+	// dna_popcount.h:45-49: one bit per differing 2-bit base lane
+	x |= x >> 1;
+	x &= 0x55555555u;
+	return __popc(x);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (uint64_t)gridDim.x * blockDim.x)
+		p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (uint64_t)gridDim.x * blockDim.x)
+		p[i] = v;
+}
+
+// 12-byte packed records -> SoA.  A record starts at a 4-byte aligned offset.
+__global__ void k_unpack_records(const uint32_t *raw, uint64_t n, uint64_t *ctxgid, uint32_t *obj)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (uint64_t)gridDim.x * blockDim.x) {
+		const uint32_t *r = raw + 3 * i;
+		ctxgid[i] = (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+		obj[i] = r[2];
+	}
+}
+
+// bucket[t] = first record with (ctx >> shift) >= t, bucket[2^k] = N.
+__global__ void k_buckets(const uint64_t *ctxgid, uint64_t n, int k, int shift, uint32_t *bucket)
+{
+	const uint64_t nb = 1ull << k;
+	for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t <= nb;
+	     t += (uint64_t)gridDim.x * blockDim.x) {
+		if (t == nb) {
+			bucket[t] = (uint32_t)n;
+			continue;
+		}
+		uint64_t lo = 0, hi = n;
+		while (lo < hi) {
+			const uint64_t mid = lo + ((hi - lo) >> 1);
+			if ((ctxgid[mid] >> (GID_NBITS + shift)) < t)
+				lo = mid + 1;
+			else
+				hi = mid;
+		}
+		bucket[t] = (uint32_t)lo;
+	}
+}
+
+// First record with context == qctx, or n.  qctx must be <= ctx_max.
+__device__ __forceinline__ uint64_t lookup_ctx(const uint64_t *__restrict__ ctxgid,
+					       const uint32_t *__restrict__ bucket, int shift,
+					       uint64_t qctx)
+{
+	const uint32_t t = (uint32_t)(qctx >> shift);
+	uint64_t lo = bucket[t], hi = bucket[t + 1];
+	const uint64_t target = qctx << GID_NBITS;
+	const uint64_t end = hi;
+	while (lo < hi) {
+		const uint64_t mid = lo + ((hi - lo) >> 1);
+		if (ctxgid[mid] < target)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	if (lo < end && (ctxgid[lo] >> GID_NBITS) == qctx)
+		return lo;
+	return ~0ull;
+}
+
+// Largest q in [0, nq) with skidx[q] <= e (skidx[nq] > e holds).
+__device__ __forceinline__ int owner_query(const uint64_t *__restrict__ skidx, int nq, uint64_t e)
+{
+	int lo = 0, hi = nq;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (skidx[mid] <= e)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+// Index-join counting, one thread per query key; the first key of each
+// query context group does the work for the group (command_ani.h:506-575,
+// :668-716).
+__global__ __launch_bounds__(256) void k_join(JoinArgs a)
+{
+	const uint64_t objmask = a.obj_bits >= 64 ? ~0ull : ((1ull << a.obj_bits) - 1ull);
+	const bool lower = (a.flags & KSSDHIP_LOWER) != 0;
+	const bool unique = !lower && (a.flags & KSSDHIP_UNIQUE) != 0;
+	const bool ignore = (a.flags & KSSDHIP_IGNORECONFLICT) != 0;
+	for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < a.total;
+	     e += (uint64_t)gridDim.x * blockDim.x) {
+		const int q = owner_query(a.skidx, a.nq, e);
+		const uint64_t qb = a.skidx[q], qe = a.skidx[q + 1];
+		const uint64_t qctx = a.keys[e] >> a.obj_bits;
+		if (e > qb && (a.keys[e - 1] >> a.obj_bits) == qctx)
+			continue;
+		if (qctx > a.ctx_max)
+			continue;
+		uint64_t s_end = e + 1;
+		while (s_end < qe && (a.keys[s_end] >> a.obj_bits) == qctx)
+			++s_end;
+		uint64_t d = lookup_ctx(a.ctxgid, a.bucket, a.shift, qctx);
+		if (d == ~0ull)
+			continue;
+		const int qgid = a.qry_gid_offset + q;
+		int owners = 0, sole_diff = 0;
+		uint32_t sole_gid = 0;
+		bool sole_allowed = false;
+		const uint64_t rowbase = (uint64_t)q * (uint64_t)a.ref_gnum;
+		while (d < a.n) {
+			const uint64_t cg = a.ctxgid[d];
+			if ((cg >> GID_NBITS) != qctx)
+				break;
+			const uint32_t gid = (uint32_t)(cg & GID_MASK);
+			const uint64_t rb = d;
+			do {
+				++d;
+			} while (d < a.n && a.ctxgid[d] == cg);
+			const uint64_t re = d;
+			if (lower && (int)gid >= qgid) {
+				if ((int)gid > qgid)
+					break;
+				continue;
+			}
+			if (unique) {
+				++owners;
+				sole_gid = gid;
+			}
+			if (ignore && re - rb > 1)
+				continue;
+			int m = MIN_INIT;
+			for (uint64_t x = e; x < s_end && m; ++x) {
+				const uint32_t oq = (uint32_t)(a.keys[x] & objmask);
+				for (uint64_t r = rb; r < re; ++r) {
+					const uint32_t diff = oq ^ a.robj[r];
+					if (diff == 0) {
+						m = 0;
+						break;
+					}
+					const int c = dna_popcount(diff);
+					m = c < m ? c : m;
+				}
+			}
+			const uint64_t cell = rowbase + gid;
+			atomicAdd(&a.ctx[cell], 1ull | ((unsigned long long)(m > 1) << 32));
+			if (m)
+				atomicAdd(&a.objc[cell], 1ull | ((unsigned long long)m << 32));
+			if (unique) {
+				sole_diff = m;
+				sole_allowed = true;
+			}
+		}
+		if (unique && owners == 1 && sole_allowed) {
+			const uint64_t cell = rowbase + sole_gid;
+			atomicAdd(&a.uctx[cell], 1ull | ((unsigned long long)(sole_diff > 1) << 32));
+			if (sole_diff)
+				atomicAdd(&a.uobj[cell], 1ull | ((unsigned long long)sole_diff << 32));
+		}
+	}
+}
+
+// Leftmost matching index record per query key (or ~0).
+__global__ void k_find_first(const uint64_t *ctxgid, const uint32_t *bucket, int shift,
+			     uint64_t ctx_max, const uint64_t *keys, uint64_t n, unsigned obj_bits,
+			     uint64_t *idx)
+{
+	for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n;
+	     e += (uint64_t)gridDim.x * blockDim.x) {
+		const uint64_t qctx = keys[e] >> obj_bits;
+		idx[e] = qctx > ctx_max ? ~0ull : lookup_ctx(ctxgid, bucket, shift, qctx);
+	}
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+	for (int off = 32; off > 0; off >>= 1)
+		v += __shfl_xor(v, off, 64);
+	return v;
+}
+
+// Plain kernel: merge-join of one pair (command_ani.h:303-346).  One thread
+// per query key; the first key of each context group searches b.
+__global__ __launch_bounds__(256) void k_pair(const uint64_t *__restrict__ A, uint64_t n,
+					      const uint64_t *__restrict__ B, uint64_t m,
+					      unsigned obj_bits, int ignore_b,
+					      kssd_ani_features_t *out)
+{
+	const uint64_t objmask = obj_bits >= 64 ? ~0ull : ((1ull << obj_bits) - 1ull);
+	uint32_t xny = 0, dobj = 0, dsec = 0, mut2 = 0;
+	const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+	if (i < n) {
+		const uint64_t ca = A[i] >> obj_bits;
+		if (i == 0 || (A[i - 1] >> obj_bits) != ca) {
+			uint64_t ae = i + 1;
+			while (ae < n && (A[ae] >> obj_bits) == ca)
+				++ae;
+			uint64_t lo = 0, hi = m;
+			while (lo < hi) {
+				const uint64_t mid = lo + ((hi - lo) >> 1);
+				if ((B[mid] >> obj_bits) < ca)
+					lo = mid + 1;
+				else
+					hi = mid;
+			}
+			if (lo < m && (B[lo] >> obj_bits) == ca) {
+				uint64_t be = lo + 1;
+				while (be < m && (B[be] >> obj_bits) == ca)
+					++be;
+				if (!(ignore_b && be - lo > 1)) {
+					int mn = MIN_INIT;
+					for (uint64_t x = i; x < ae && mn; ++x) {
+						const uint32_t oa = (uint32_t)(A[x] & objmask);
+						for (uint64_t y = lo; y < be; ++y) {
+							const uint32_t d = oa ^ (uint32_t)(B[y] & objmask);
+							if (d == 0) {
+								mn = 0;
+								break;
+							}
+							const int c = dna_popcount(d);
+							mn = c < mn ? c : mn;
+						}
+					}
+					xny = 1;
+					if (mn > 0) {
+						dobj = 1;
+						dsec = (uint32_t)mn;
+						mut2 = mn > 1;
+					}
+				}
+			}
+		}
+	}
+	xny = wave_sum(xny);
+	dobj = wave_sum(dobj);
+	dsec = wave_sum(dsec);
+	mut2 = wave_sum(mut2);
+	if ((threadIdx.x & 63) == 0 && xny) {
+		atomicAdd(&out->XnY_ctx, xny);
+		if (dobj) {
+			atomicAdd(&out->N_diff_obj, dobj);
+			atomicAdd(&out->N_diff_obj_section, dsec);
+		}
+		if (mut2)
+			atomicAdd(&out->N_mut2_ctx, mut2);
+	}
+}
+
+static unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 1u << 20)
+{
+	uint64_t g = (n + block - 1) / block;
+	if (g == 0)
+		g = 1;
+	if (g > cap)
+		g = cap;
+	return (unsigned)g;
+}
+
+} // namespace kssd
+
+using namespace kssd;
+
+// ---------------------------------------------------------------- index handle
+struct kssdhip_index {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	uint64_t n = 0;
+	int ref_gnum = 0;
+	unsigned obj_bits = 0;
+	uint64_t *d_ctxgid = nullptr;
+	uint32_t *d_obj = nullptr;
+	uint32_t *d_bucket = nullptr;
+	int k = 0, shift = 0;
+	uint64_t ctx_max = 0;
+	// grow-only scratch for the host-pointer entry points
+	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx;
+};
+
+int DevBuf::reserve(size_t want)
+{
+	if (want <= bytes)
+		return 0;
+	if (ptr)
+		(void)hipFree(ptr);
+	ptr = nullptr;
+	bytes = 0;
+	HIP_TRY(hipMalloc(&ptr, want));
+	bytes = want;
+	return 0;
+}
+
+void DevBuf::release()
+{
+	if (ptr)
+		(void)hipFree(ptr);
+	ptr = nullptr;
+	bytes = 0;
+}
+
+static int bit_width_u64(uint64_t v)
+{
+	return v ? 64 - __builtin_clzll(v) : 0;
+}
+
+extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int ref_gnum,
+				  unsigned obj_bits, kssdhip_index **out)
+{
+	if (!out)
+		return kssdhip_fail(KSSDHIP_EINVAL, "index_open: null output handle");
+	*out = nullptr;
+	if (n && !records)
+		return kssdhip_fail(KSSDHIP_EINVAL, "index_open: null records");
+	if (n >= 0xffffffffull)
+		return kssdhip_fail(KSSDHIP_EINVAL, "index_open: %zu records exceed 2^32-1", n);
+	if (ref_gnum <= 0 || ref_gnum > (1 << GID_NBITS))
+		return kssdhip_fail(KSSDHIP_EINVAL, "index_open: ref_gnum %d out of range", ref_gnum);
+	if (obj_bits < 1 || obj_bits > 32)
+		return kssdhip_fail(KSSDHIP_EINVAL, "index_open: obj_bits %u out of range", obj_bits);
+	for (size_t probe = 0; probe < n && probe < 2; ++probe) {
+		const size_t at = probe ? n - 1 : 0;
+		if ((records[at].ctxgid & GID_MASK) >= (uint64_t)ref_gnum)
+			return kssdhip_fail(KSSDHIP_EINVAL, "index_open: gid beyond ref_gnum");
+	}
+
+	kssdhip_index *ix = new (std::nothrow) kssdhip_index();
+	if (!ix)
+		return kssdhip_fail(KSSDHIP_ENOMEM, "index_open: out of host memory");
+	ix->device = g_device;
+	ix->n = n;
+	ix->ref_gnum = ref_gnum;
+	ix->obj_bits = obj_bits;
+	ix->ctx_max = n ? (records[n - 1].ctxgid >> GID_NBITS) : 0;
+	const int bw = bit_width_u64(ix->ctx_max);
+	int k = n > 4 ? bit_width_u64(n) - 2 : 0; // ~2-4 records per bucket
+	if (k > 26)
+		k = 26;
+	if (k > bw)
+		k = bw;
+	ix->k = k;
+	ix->shift = bw - k;
+
+	int rc = 0;
+	void *d_raw = nullptr;
+	const size_t nb = ((size_t)1 << k) + 1;
+	do {
+		if (hipSetDevice(ix->device) != hipSuccess ||
+		    hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: stream creation failed");
+			break;
+		}
+		const size_t nalloc = n ? n : 1;
+		if (hipMalloc(&ix->d_ctxgid, nalloc * sizeof(uint64_t)) != hipSuccess ||
+		    hipMalloc(&ix->d_obj, nalloc * sizeof(uint32_t)) != hipSuccess ||
+		    hipMalloc(&ix->d_bucket, nb * sizeof(uint32_t)) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_ENOMEM, "index_open: device allocation of %zu records failed", n);
+			break;
+		}
+		if (n) {
+			if (hipMalloc(&d_raw, n * sizeof(kssd_ctxgidobj_t)) != hipSuccess) {
+				rc = kssdhip_fail(KSSDHIP_ENOMEM, "index_open: staging allocation failed");
+				break;
+			}
+			if (hipMemcpyAsync(d_raw, records, n * sizeof(kssd_ctxgidobj_t), hipMemcpyHostToDevice,
+					   ix->stream) != hipSuccess) {
+				rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: upload failed");
+				break;
+			}
+			hipLaunchKernelGGL(k_unpack_records, dim3(grid_for(n)), dim3(256), 0, ix->stream,
+					   (const uint32_t *)d_raw, (uint64_t)n, ix->d_ctxgid, ix->d_obj);
+		}
+		hipLaunchKernelGGL(k_buckets, dim3(grid_for(nb)), dim3(256), 0, ix->stream,
+				   ix->d_ctxgid, (uint64_t)n, k, ix->shift, ix->d_bucket);
+		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ix->stream) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: bucket build failed");
+			break;
+		}
+	} while (0);
+	if (d_raw)
+		(void)hipFree(d_raw);
+	if (rc) {
+		kssdhip_index_close(ix);
+		return rc;
+	}
+	*out = ix;
+	return 0;
+}
+
+extern "C" int kssdhip_index_close(kssdhip_index *ix)
+{
+	if (!ix)
+		return 0;
+	(void)hipSetDevice(ix->device);
+	if (ix->stream)
+		(void)hipStreamSynchronize(ix->stream);
+	if (ix->d_ctxgid)
+		(void)hipFree(ix->d_ctxgid);
+	if (ix->d_obj)
+		(void)hipFree(ix->d_obj);
+	if (ix->d_bucket)
+		(void)hipFree(ix->d_bucket);
+	DevBuf *bufs[] = {&ix->keys, &ix->skidx, &ix->ctx, &ix->obj, &ix->uctx, &ix->uobj, &ix->idx};
+	for (DevBuf *b : bufs)
+		b->release();
+	if (ix->stream)
+		(void)hipStreamDestroy(ix->stream);
+	delete ix;
+	return 0;
+}
+
+extern "C" size_t kssdhip_index_device_bytes(const kssdhip_index *ix)
+{
+	if (!ix)
+		return 0;
+	size_t b = (size_t)ix->n * 12 + ((((size_t)1) << ix->k) + 1) * 4;
+	b += ix->keys.bytes + ix->skidx.bytes + ix->ctx.bytes + ix->obj.bytes + ix->uctx.bytes +
+	     ix->uobj.bytes + ix->idx.bytes;
+	return b;
+}
+
+int kssdhip_index_info(const kssdhip_index *ix, IndexView *v)
+{
+	if (!ix || !v)
+		return kssdhip_fail(KSSDHIP_EINVAL, "null index");
+	v->ctxgid = ix->d_ctxgid;
+	v->obj = ix->d_obj;
+	v->bucket = ix->d_bucket;
+	v->n = ix->n;
+	v->ref_gnum = ix->ref_gnum;
+	v->obj_bits = ix->obj_bits;
+	v->shift = ix->shift;
+	v->k = ix->k;
+	v->ctx_max = ix->ctx_max;
+	v->device = ix->device;
+	return 0;
+}
+
+// ---------------------------------------------------------------- counting
+extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys,
+				       const uint64_t *d_skidx, int nq, uint64_t total,
+				       int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx,
+				       kssd_obj_section_t *d_obj, kssd_ctx_mut2_t *d_uctx,
+				       kssd_obj_section_t *d_uobj, void *stream)
+{
+	if (!ix)
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: null index");
+	if (nq < 0 || (nq > 0 && (!d_skidx || !d_ctx || !d_obj)) || (total && !d_keys))
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: bad arguments");
+	const bool unique = (flags & KSSDHIP_UNIQUE) && !(flags & KSSDHIP_LOWER);
+	if (unique && (!d_uctx || !d_uobj))
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: UNIQUE needs unique outputs");
+	if (flags & ~(KSSDHIP_LOWER | KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT))
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: unknown flags 0x%x", flags);
+	hipStream_t s = (hipStream_t)stream;
+	const uint64_t cells = (uint64_t)nq * (uint64_t)ix->ref_gnum;
+	if (cells) {
+		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
+				   (unsigned long long *)d_ctx, cells, 0ull);
+		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
+				   (unsigned long long *)d_obj, cells, 0ull);
+		if (unique) {
+			hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
+					   (unsigned long long *)d_uctx, cells, 0ull);
+			hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
+					   (unsigned long long *)d_uobj, cells, 0ull);
+		}
+	}
+	if (total && nq && ix->n) {
+		JoinArgs a;
+		a.ctxgid = ix->d_ctxgid;
+		a.robj = ix->d_obj;
+		a.bucket = ix->d_bucket;
+		a.n = ix->n;
+		a.shift = ix->shift;
+		a.ctx_max = ix->ctx_max;
+		a.ref_gnum = ix->ref_gnum;
+		a.keys = d_keys;
+		a.skidx = d_skidx;
+		a.nq = nq;
+		a.total = total;
+		a.qry_gid_offset = qry_gid_offset;
+		a.obj_bits = ix->obj_bits;
+		a.flags = flags;
+		a.ctx = (unsigned long long *)d_ctx;
+		a.objc = (unsigned long long *)d_obj;
+		a.uctx = (unsigned long long *)d_uctx;
+		a.uobj = (unsigned long long *)d_uobj;
+		hipLaunchKernelGGL(k_join, dim3(grid_for(total)), dim3(256), 0, s, a);
+	}
+	HIP_TRY(hipGetLastError());
+	return 0;
+}
+
+extern "C" int kssdhip_count_block(kssdhip_index *ix, const uint64_t *section_sk,
+				   const uint64_t *section_skidx, int nq, int qry_gid_offset,
+				   uint32_t flags, kssd_ctx_mut2_t *ctx, kssd_obj_section_t *obj,
+				   kssd_ctx_mut2_t *uctx, kssd_obj_section_t *uobj)
+{
+	if (!ix)
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: null index");
+	if (nq < 0 || (nq > 0 && (!section_skidx || !ctx || !obj)))
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: bad arguments");
+	if (nq == 0)
+		return 0;
+	const bool unique = (flags & KSSDHIP_UNIQUE) && !(flags & KSSDHIP_LOWER);
+	if (unique && (!uctx || !uobj))
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: UNIQUE needs unique outputs");
+	const uint64_t base = section_skidx[0];
+	std::vector<uint64_t> rel((size_t)nq + 1);
+	for (int i = 0; i <= nq; ++i) {
+		if (section_skidx[i] < base || (i && section_skidx[i] < section_skidx[i - 1]))
+			return kssdhip_fail(KSSDHIP_EINVAL, "count_block: section_skidx not ascending");
+		rel[i] = section_skidx[i] - base;
+	}
+	const uint64_t total = rel[nq];
+	if (total && !section_sk)
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: null keys");
+	const size_t cells = (size_t)nq * (size_t)ix->ref_gnum;
+	HIP_TRY(hipSetDevice(ix->device));
+	int rc;
+	if ((rc = ix->keys.reserve((total ? total : 1) * 8)) || (rc = ix->skidx.reserve(rel.size() * 8)) ||
+	    (rc = ix->ctx.reserve(cells * 8)) || (rc = ix->obj.reserve(cells * 8)))
+		return rc;
+	if (unique && ((rc = ix->uctx.reserve(cells * 8)) || (rc = ix->uobj.reserve(cells * 8))))
+		return rc;
+	hipStream_t s = ix->stream;
+	if (total)
+		HIP_TRY(hipMemcpyAsync(ix->keys.ptr, section_sk, total * 8, hipMemcpyHostToDevice, s));
+	HIP_TRY(hipMemcpyAsync(ix->skidx.ptr, rel.data(), rel.size() * 8, hipMemcpyHostToDevice, s));
+	rc = kssdhip_count_block_dev(ix, (const uint64_t *)ix->keys.ptr, (const uint64_t *)ix->skidx.ptr,
+				     nq, total, qry_gid_offset, flags, (kssd_ctx_mut2_t *)ix->ctx.ptr,
+				     (kssd_obj_section_t *)ix->obj.ptr,
+				     unique ? (kssd_ctx_mut2_t *)ix->uctx.ptr : nullptr,
+				     unique ? (kssd_obj_section_t *)ix->uobj.ptr : nullptr, s);
+	if (rc)
+		return rc;
+	HIP_TRY(hipMemcpyAsync(ctx, ix->ctx.ptr, cells * 8, hipMemcpyDeviceToHost, s));
+	HIP_TRY(hipMemcpyAsync(obj, ix->obj.ptr, cells * 8, hipMemcpyDeviceToHost, s));
+	if (unique) {
+		HIP_TRY(hipMemcpyAsync(uctx, ix->uctx.ptr, cells * 8, hipMemcpyDeviceToHost, s));
+		HIP_TRY(hipMemcpyAsync(uobj, ix->uobj.ptr, cells * 8, hipMemcpyDeviceToHost, s));
+	}
+	HIP_TRY(hipStreamSynchronize(s));
+	return 0;
+}
+
+extern "C" int kssdhip_find_first(kssdhip_index *ix, const uint64_t *a, size_t n, uint64_t *idx)
+{
+	if (!ix || (n && (!a || !idx)))
+		return kssdhip_fail(KSSDHIP_EINVAL, "find_first: bad arguments");
+	if (!n)
+		return 0;
+	HIP_TRY(hipSetDevice(ix->device));
+	int rc;
+	if ((rc = ix->keys.reserve(n * 8)) || (rc = ix->idx.reserve(n * 8)))
+		return rc;
+	hipStream_t s = ix->stream;
+	HIP_TRY(hipMemcpyAsync(ix->keys.ptr, a, n * 8, hipMemcpyHostToDevice, s));
+	if (ix->n)
+		hipLaunchKernelGGL(k_find_first, dim3(grid_for(n)), dim3(256), 0, s, ix->d_ctxgid,
+				   ix->d_bucket, ix->shift, ix->ctx_max, (const uint64_t *)ix->keys.ptr,
+				   (uint64_t)n, ix->obj_bits, (uint64_t *)ix->idx.ptr);
+	else
+		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(256), 0, s,
+				   (unsigned long long *)ix->idx.ptr, (uint64_t)n, ~0ull);
+	HIP_TRY(hipGetLastError());
+	HIP_TRY(hipMemcpyAsync(idx, ix->idx.ptr, n * 8, hipMemcpyDeviceToHost, s));
+	HIP_TRY(hipStreamSynchronize(s));
+	return 0;
+}
+
+// ---------------------------------------------------------------- plain kernel
+extern "C" int kssdhip_pair_features_dev(const uint64_t *d_a, size_t n, const uint64_t *d_b, size_t m,
+					 unsigned obj_bits, int ignore_b_conflict,
+					 kssd_ani_features_t *d_out, void *stream)
+{
+	if (!d_out || (n && !d_a) || (m && !d_b))
+		return kssdhip_fail(KSSDHIP_EINVAL, "pair_features: bad arguments");
+	if (obj_bits < 1 || obj_bits > 32)
+		return kssdhip_fail(KSSDHIP_EINVAL, "pair_features: obj_bits %u out of range", obj_bits);
+	hipStream_t s = (hipStream_t)stream;
+	hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)d_out, (uint64_t)5, 0u);
+	if (n && m) {
+		if (n > 0xffffffffull * 256ull)
+			return kssdhip_fail(KSSDHIP_EINVAL, "pair_features: sketch too large");
+		hipLaunchKernelGGL(k_pair, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_a,
+				   (uint64_t)n, d_b, (uint64_t)m, obj_bits, ignore_b_conflict, d_out);
+	}
+	HIP_TRY(hipGetLastError());
+	return 0;
+}
+
+namespace {
+struct PairScratch {
+	int device = -1;
+	hipStream_t stream = nullptr;
+	DevBuf a, b, out;
+};
+thread_local PairScratch g_pair;
+} // namespace
+
+extern "C" int kssdhip_pair_features(const uint64_t *a, size_t n, const uint64_t *b, size_t m,
+				     unsigned obj_bits, int ignore_b_conflict,
+				     kssd_ani_features_t *out)
+{
+	if (!out || (n && !a) || (m && !b))
+		return kssdhip_fail(KSSDHIP_EINVAL, "pair_features: bad arguments");
+	HIP_TRY(hipSetDevice(g_device));
+	PairScratch &ps = g_pair;
+	if (ps.device != g_device) {
+		ps.a.release();
+		ps.b.release();
+		ps.out.release();
+		if (ps.stream)
+			(void)hipStreamDestroy(ps.stream);
+		HIP_TRY(hipStreamCreateWithFlags(&ps.stream, hipStreamNonBlocking));
+		ps.device = g_device;
+	}
+	int rc;
+	if ((rc = ps.a.reserve((n ? n : 1) * 8)) || (rc = ps.b.reserve((m ? m : 1) * 8)) ||
+	    (rc = ps.out.reserve(sizeof(kssd_ani_features_t))))
+		return rc;
+	if (n)
+		HIP_TRY(hipMemcpyAsync(ps.a.ptr, a, n * 8, hipMemcpyHostToDevice, ps.stream));
+	if (m)
+		HIP_TRY(hipMemcpyAsync(ps.b.ptr, b, m * 8, hipMemcpyHostToDevice, ps.stream));
+	rc = kssdhip_pair_features_dev((const uint64_t *)ps.a.ptr, n, (const uint64_t *)ps.b.ptr, m,
+				       obj_bits, ignore_b_conflict, (kssd_ani_features_t *)ps.out.ptr,
+				       ps.stream);
+	if (rc)
+		return rc;
+	HIP_TRY(hipMemcpyAsync(out, ps.out.ptr, sizeof *out, hipMemcpyDeviceToHost, ps.stream));
+	HIP_TRY(hipStreamSynchronize(ps.stream));
+	return 0;
+}

@@ -1,0 +1,106 @@
+/*
+ * CPU oracle for the KSSD3A sketch-comparison path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C + OpenMP restatement of the
+ * reference algorithm, written from the reference sources cited beside each
+ * function.  Only the tests, the smoke check and the CPU-baseline leg of the
+ * benchmark may load it, and only as the checker or as the timed CPU
+ * baseline; the product path never calls it.
+ *
+ * Every function here is pinned by at least one check in tests/ that does not
+ * retype its formula: the reference binary's own outputs (golden TSVs built
+ * by oracle/_ref from /root/reference), the brute-force closed form of
+ * tests/test_unique_context.py:99-104 (min over object pairs of per-base
+ * mismatches), exhaustive enumeration for dna_popcount, and the identity /
+ * empty special cases.  See the test files named in each comment.
+ */
+#ifndef KSSD_ORACLE_H
+#define KSSD_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 12-byte packed index record, sketch_rearrange.h:19 */
+typedef struct __attribute__((packed)) {
+	uint64_t ctxgid; /* ctx << 20 | gid */
+	uint32_t obj;
+} orc_rec_t;
+
+/* command_ani.h:114-124 */
+typedef struct { uint32_t num_ctx, num_mut2_ctx; } orc_ctx_t;
+typedef struct { uint32_t diff_obj, diff_obj_section; } orc_obj_t;
+
+/* model_ani.h:215-222 (field order kept) */
+typedef struct {
+	uint32_t XnY_ctx, X_ctx, N_diff_obj_section, N_mut2_ctx, N_diff_obj;
+} orc_features_t;
+
+#define ORC_LOWER 1u
+#define ORC_UNIQUE 2u
+#define ORC_IGNORECONFLICT 4u
+
+/* dna_popcount.h:45-49 */
+int orc_dna_popcount(uint32_t x);
+
+/* command_ani.c:3656-3678 (F has 2^k + 1 entries) */
+int orc_fenceposts_build(const orc_rec_t *b, size_t n, int k, size_t *F);
+/* command_ani.c:3703-3738; idx[i] = leftmost record with equal ctx or SIZE_MAX */
+void orc_find_first(const uint64_t *a, size_t na, const orc_rec_t *b, size_t nb,
+		    const size_t *F, int k, unsigned obj_bits, size_t *idx);
+
+/*
+ * Index-join counting of one query block against a sorted index.
+ * Restates count_ctx_obj_frm_comb_sketch_section_with_unique
+ * (command_ani.h:463-575, counting part) and
+ * count_ctx_obj_frm_comb_sketch_section_lower (command_ani.h:635-721).
+ * section_skidx holds absolute offsets (nq+1 values); keys start at
+ * section_skidx[0].  Outputs are dense [nq x ref_gnum] and are ACCUMULATED
+ * into (the caller zeroes them), exactly as the reference does.
+ * uctx/uobj are only touched with ORC_UNIQUE (never with ORC_LOWER).
+ */
+void orc_count_block(const orc_rec_t *index, size_t n_index, int ref_gnum,
+		     unsigned obj_bits, const uint64_t *section_sk,
+		     const uint64_t *section_skidx, int nq, int qry_gid_offset,
+		     unsigned flags, orc_ctx_t *ctx, orc_obj_t *obj,
+		     orc_ctx_t *uctx, orc_obj_t *uobj, int nthreads);
+
+/* get_ani_features_ctx_min_over_conflicts_both_filtered, command_ani.h:283-346 */
+void orc_pair_features(const uint64_t *a, size_t n, const uint64_t *b, size_t m,
+		       unsigned obj_bits, int ignore_b_conflict, orc_features_t *out);
+
+/* Many pairs: out[p] for (qa[p], rb[p]) using sketch offsets; OpenMP. */
+void orc_pair_features_many(const uint64_t *qkeys, const uint64_t *qidx,
+			    const uint64_t *rkeys, const uint64_t *ridx,
+			    const int32_t *pq, const int32_t *pr, size_t npairs,
+			    unsigned obj_bits, int ignore_b_conflict,
+			    orc_features_t *out, int nthreads);
+
+/*
+ * Index build: ctxobj64_2ctxgidobj (sketch_rearrange.c:65-78) followed by a
+ * sort on (ctxgid, obj) (kssdlib_sort.c:145-148).  out has skidx[ng] records.
+ */
+void orc_build_index(const uint64_t *keys, const uint64_t *skidx, int ngenomes,
+		     unsigned obj_bits, orc_rec_t *out);
+
+/* count_ctx_runs_sorted_ctxobj64_local, command_ani.c:337-349 */
+uint32_t orc_ctx_runs(const uint64_t *a, size_t n, unsigned obj_bits, int ignoreconflict);
+/* ref_ctx_counts_from_ctxgidobj, command_ani.c:351-370 */
+void orc_ref_ctx_counts_from_index(const orc_rec_t *index, size_t n, int ref_gnum,
+				   const uint64_t *ref_skidx, int ignoreconflict,
+				   uint32_t *counts);
+
+/* FP64 formulas */
+double orc_naive_dist(const orc_features_t *f);                     /* command_ani.h:202-224 */
+double orc_p_dist(const orc_features_t *f, unsigned obj_bits);       /* command_ani.h:226-232 */
+double orc_lm3ways_dist(const orc_features_t *f, int drfold);        /* model_ani.h:285-305 */
+double orc_af_anib(const orc_features_t *f, int drfold);             /* model_ani.h:356-367 */
+double orc_mashD(int k, uint32_t ref_ctx, uint32_t qry_ctx, uint32_t xny); /* command_matrix.c:26-29 */
+double orc_aafD(int k, uint32_t ref_ctx, uint32_t qry_ctx, uint32_t xny);  /* command_matrix.c:30-33 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

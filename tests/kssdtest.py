@@ -1,0 +1,310 @@
+"""Shared helpers for the tests: sketch-directory I/O, random fixtures and
+ctypes bindings of the product library (libkssdhip.so) and of the CPU oracle
+(oracle/liboracle.so, used only as the checker).
+
+Sketch directory layout (reference: global_basic.h:138-149,
+command_sketch.c:29-39, tests/test_unique_context.py:25-41):
+  lcofiles.stat               struct '<I??2x6i' + infile_num * 256-byte names
+  comblco                     u64 keys, each sample sorted ascending
+  comblco.index               u64 prefix offsets (infile_num + 1)
+  sortedcomb_ctxgid64obj32    packed '<QI' records sorted by (ctxgid, obj)
+"""
+import ctypes
+import os
+import struct
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "kssd3a-public_amd"
+GOLDEN = ROOT / "tests" / "golden"
+STAT = struct.Struct("<I??2x6i")
+GID_NBITS = 20
+REC_DTYPE = np.dtype([("ctxgid", "<u8"), ("obj", "<u4")], align=False)
+assert REC_DTYPE.itemsize == 12
+
+
+# --------------------------------------------------------------- sketch dirs
+class SketchDir:
+    def __init__(self, hash_id, koc, conflict, coden_len, klen, hclen, holen,
+                 drfold, names, keys, index, records=None):
+        self.hash_id, self.koc, self.conflict = hash_id, koc, conflict
+        self.coden_len, self.klen, self.hclen, self.holen = coden_len, klen, hclen, holen
+        self.drfold = drfold
+        self.names = names
+        self.keys = keys
+        self.index = index
+        self.records = records
+
+    @property
+    def n(self):
+        return len(self.names)
+
+    @property
+    def obj_bits(self):
+        # const_comask_init, sketch_rearrange.c:7-39
+        ctx = 4 * self.coden_len if self.coden_len > 0 else 4 * self.hclen
+        return 2 * self.klen - ctx
+
+    def sample(self, i):
+        return self.keys[self.index[i]:self.index[i + 1]]
+
+
+def read_sketch_dir(path):
+    path = Path(path)
+    raw = (path / "lcofiles.stat").read_bytes()
+    fields = STAT.unpack_from(raw, 0)
+    n = fields[-1]
+    names = [raw[STAT.size + 256 * i: STAT.size + 256 * (i + 1)].split(b"\0", 1)[0].decode()
+             for i in range(n)]
+    keys = np.fromfile(path / "comblco", dtype="<u8")
+    index = np.fromfile(path / "comblco.index", dtype="<u8")
+    rec = None
+    if (path / "sortedcomb_ctxgid64obj32").exists():
+        rec = np.fromfile(path / "sortedcomb_ctxgid64obj32", dtype=REC_DTYPE)
+    return SketchDir(*fields[:-1], names, keys, index, rec)
+
+
+def write_sketch_dir(path, samples, names=None, conflict=True, coden_len=11, klen=32,
+                     hclen=11, holen=0, drfold=8, hash_id=0, indexed=False):
+    """samples: list of sorted u64 arrays (keys already in ctx<<obj|obj form)."""
+    path = Path(path)
+    path.mkdir(parents=True, exist_ok=True)
+    names = names or [f"{path.name}_{i}" for i in range(len(samples))]
+    keys = np.concatenate([np.asarray(s, dtype="<u8") for s in samples]) if samples else np.zeros(0, "<u8")
+    index = np.zeros(len(samples) + 1, dtype="<u8")
+    index[1:] = np.cumsum([len(s) for s in samples])
+    head = STAT.pack(hash_id, False, conflict, coden_len, klen, hclen, holen, drfold, len(samples))
+    (path / "lcofiles.stat").write_bytes(head + b"".join(n.encode().ljust(256, b"\0") for n in names))
+    keys.tofile(path / "comblco")
+    index.tofile(path / "comblco.index")
+    obj_bits = 2 * klen - (4 * coden_len if coden_len > 0 else 4 * hclen)
+    if indexed:
+        build_index_np(keys, index, obj_bits).tofile(path / "sortedcomb_ctxgid64obj32")
+    return read_sketch_dir(path)
+
+
+def build_index_np(keys, index, obj_bits):
+    """Inverted index (sketch_rearrange.c:65-78 + kssdlib_sort.c:145-148),
+    written independently with numpy for the tests."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    n = len(index) - 1
+    gid = np.repeat(np.arange(n, dtype=np.uint64), np.diff(index).astype(np.int64))
+    rec = np.empty(len(keys), dtype=REC_DTYPE)
+    rec["ctxgid"] = ((keys >> np.uint64(obj_bits)) << np.uint64(GID_NBITS)) | gid
+    rec["obj"] = (keys & np.uint64((1 << obj_bits) - 1)).astype(np.uint32)
+    order = np.lexsort((rec["obj"], rec["ctxgid"]))
+    return rec[order]
+
+
+# --------------------------------------------------------------- random fixtures
+def random_samples(rng, n_samples, n_ctx, mean_entries, obj_bits=20, conflict_p=0.1,
+                   obj_noise=0.3, shared_pool=None, ragged=True):
+    """Samples drawing contexts from a small shared alphabet so that contexts
+    collide across samples; some contexts keep several objects (conflicts)."""
+    pool = shared_pool if shared_pool is not None else rng.integers(1, 1 << 40, size=n_ctx, dtype=np.uint64)
+    base_obj = rng.integers(0, 1 << obj_bits, size=len(pool), dtype=np.uint64)
+    out = []
+    for _ in range(n_samples):
+        k = int(rng.integers(0, 2 * mean_entries + 1)) if ragged else mean_entries
+        k = min(k, len(pool))
+        ci = rng.choice(len(pool), size=k, replace=False)
+        keys = []
+        for c in ci:
+            nobj = 1 + (rng.random() < conflict_p) * int(rng.integers(1, 3))
+            for _ in range(nobj):
+                o = int(base_obj[c])
+                if rng.random() < obj_noise:
+                    # flip 1-3 random 2-bit lanes
+                    for lane in rng.choice(obj_bits // 2, size=int(rng.integers(1, 4)), replace=False):
+                        o ^= int(rng.integers(1, 4)) << (2 * int(lane))
+                keys.append((int(pool[c]) << obj_bits) | o)
+        out.append(np.unique(np.array(keys, dtype=np.uint64)))
+    return out
+
+
+def brute_counts(q, r, obj_bits, ignore_b_conflict=False):
+    """Closed form of tests/test_unique_context.py:99-104: per shared context,
+    the minimum over object pairs of the number of differing 2-bit lanes."""
+    def group(a):
+        g = {}
+        for k in a.tolist():
+            g.setdefault(k >> obj_bits, []).append(k & ((1 << obj_bits) - 1))
+        return g
+    gq, gr = group(q), group(r)
+    xny = dobj = dsec = mut2 = 0
+    for c in gq.keys() & gr.keys():
+        if ignore_b_conflict and len(gr[c]) > 1:
+            continue
+        d = min(sum(bool(((a ^ b) >> bit) & 3) for bit in range(0, 32, 2)) for a in gq[c] for b in gr[c])
+        xny += 1
+        if d:
+            dobj += 1
+            dsec += d
+            mut2 += d > 1
+    return xny, dobj, dsec, mut2
+
+
+# --------------------------------------------------------------- ctypes bindings
+class Features(ctypes.Structure):
+    _fields_ = [("XnY_ctx", ctypes.c_uint32), ("X_ctx", ctypes.c_uint32),
+                ("N_diff_obj_section", ctypes.c_uint32), ("N_mut2_ctx", ctypes.c_uint32),
+                ("N_diff_obj", ctypes.c_uint32)]
+
+    def tup(self):
+        return (self.XnY_ctx, self.N_diff_obj, self.N_diff_obj_section, self.N_mut2_ctx)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def load_oracle():
+    so = ROOT / "oracle" / "liboracle.so"
+    if not so.exists():
+        raise RuntimeError(f"{so} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(so))
+    V, S, U, I, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_int, ctypes.c_double
+    lib.orc_dna_popcount.argtypes = [ctypes.c_uint32]
+    lib.orc_dna_popcount.restype = I
+    lib.orc_fenceposts_build.argtypes = [V, S, I, V]
+    lib.orc_find_first.argtypes = [V, S, V, S, V, I, U, V]
+    lib.orc_find_first.restype = None
+    lib.orc_count_block.argtypes = [V, S, I, U, V, V, I, I, U, V, V, V, V, I]
+    lib.orc_count_block.restype = None
+    lib.orc_pair_features.argtypes = [V, S, V, S, U, I, ctypes.POINTER(Features)]
+    lib.orc_pair_features.restype = None
+    lib.orc_pair_features_many.argtypes = [V, V, V, V, V, V, S, U, I, V, I]
+    lib.orc_pair_features_many.restype = None
+    lib.orc_build_index.argtypes = [V, V, I, U, V]
+    lib.orc_build_index.restype = None
+    lib.orc_ctx_runs.argtypes = [V, S, U, I]
+    lib.orc_ctx_runs.restype = ctypes.c_uint32
+    lib.orc_ref_ctx_counts_from_index.argtypes = [V, S, I, V, I, V]
+    lib.orc_ref_ctx_counts_from_index.restype = None
+    for f in ("orc_naive_dist",):
+        getattr(lib, f).argtypes = [ctypes.POINTER(Features)]
+        getattr(lib, f).restype = D
+    lib.orc_p_dist.argtypes = [ctypes.POINTER(Features), U]
+    lib.orc_p_dist.restype = D
+    lib.orc_lm3ways_dist.argtypes = [ctypes.POINTER(Features), I]
+    lib.orc_lm3ways_dist.restype = D
+    lib.orc_af_anib.argtypes = [ctypes.POINTER(Features), I]
+    lib.orc_af_anib.restype = D
+    for f in ("orc_mashD", "orc_aafD"):
+        getattr(lib, f).argtypes = [I, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        getattr(lib, f).restype = D
+    return lib
+
+
+class Oracle:
+    """Thin numpy-facing wrapper over oracle/liboracle.so."""
+
+    def __init__(self):
+        self.lib = load_oracle()
+
+    def count_block(self, records, ref_gnum, obj_bits, keys, skidx_abs, qry_gid_offset=0,
+                    flags=0, threads=os.cpu_count() or 1):
+        nq = len(skidx_abs) - 1
+        ctx = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
+        obj = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
+        uctx = np.zeros_like(ctx)
+        uobj = np.zeros_like(obj)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        skidx_abs = np.ascontiguousarray(skidx_abs, dtype=np.uint64)
+        records = np.ascontiguousarray(records)
+        self.lib.orc_count_block(_p(records), len(records), ref_gnum, obj_bits, _p(keys),
+                                 _p(skidx_abs), nq, qry_gid_offset, flags, _p(ctx), _p(obj),
+                                 _p(uctx), _p(uobj), int(threads))
+        return ctx, obj, uctx, uobj
+
+    def pair(self, a, b, obj_bits, ignore_b_conflict=False):
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        f = Features()
+        self.lib.orc_pair_features(_p(a), len(a), _p(b), len(b), obj_bits, int(ignore_b_conflict),
+                                   ctypes.byref(f))
+        return f.tup()
+
+    def find_first(self, records, keys, obj_bits, k=17):
+        F = np.zeros((1 << k) + 1, dtype=np.uint64)
+        records = np.ascontiguousarray(records)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        self.lib.orc_fenceposts_build(_p(records), len(records), k, _p(F))
+        idx = np.zeros(len(keys), dtype=np.uint64)
+        self.lib.orc_find_first(_p(keys), len(keys), _p(records), len(records), _p(F), k,
+                                obj_bits, _p(idx))
+        return idx
+
+
+def load_kssdhip():
+    so = PKG / "libkssdhip.so"
+    if not so.exists():
+        raise RuntimeError(f"{so} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(so))
+    V, S, U, I = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_int
+    lib.kssdhip_last_error.restype = ctypes.c_char_p
+    lib.kssdhip_init.argtypes = [I]
+    lib.kssdhip_index_open.argtypes = [V, S, I, U, ctypes.POINTER(V)]
+    lib.kssdhip_index_close.argtypes = [V]
+    lib.kssdhip_index_device_bytes.argtypes = [V]
+    lib.kssdhip_index_device_bytes.restype = S
+    lib.kssdhip_count_block.argtypes = [V, V, V, I, I, ctypes.c_uint32, V, V, V, V]
+    lib.kssdhip_count_block_dev.argtypes = [V, V, V, I, ctypes.c_uint64, I, ctypes.c_uint32,
+                                            V, V, V, V, V]
+    lib.kssdhip_find_first.argtypes = [V, V, S, V]
+    lib.kssdhip_pair_features.argtypes = [V, S, V, S, U, I, ctypes.POINTER(Features)]
+    lib.kssdhip_pair_features_dev.argtypes = [V, S, V, S, U, I, V, V]
+    return lib
+
+
+class KssdHip:
+    """numpy-facing wrapper over the product C ABI (include/kssdhip.h)."""
+
+    def __init__(self, device=0):
+        self.lib = load_kssdhip()
+        self._check(self.lib.kssdhip_init(device))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"kssdhip error {rc}: {self.lib.kssdhip_last_error().decode()}")
+
+    def open_index(self, records, ref_gnum, obj_bits):
+        h = ctypes.c_void_p()
+        records = np.ascontiguousarray(records)
+        self._check(self.lib.kssdhip_index_open(_p(records), len(records), ref_gnum, obj_bits,
+                                                ctypes.byref(h)))
+        return h
+
+    def close_index(self, h):
+        self._check(self.lib.kssdhip_index_close(h))
+
+    def count_block(self, h, ref_gnum, keys, skidx_abs, qry_gid_offset=0, flags=0):
+        nq = len(skidx_abs) - 1
+        ctx = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
+        obj = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
+        uctx = np.zeros_like(ctx)
+        uobj = np.zeros_like(obj)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        skidx_abs = np.ascontiguousarray(skidx_abs, dtype=np.uint64)
+        self._check(self.lib.kssdhip_count_block(h, _p(keys), _p(skidx_abs), nq, qry_gid_offset,
+                                                 flags, _p(ctx), _p(obj), _p(uctx), _p(uobj)))
+        return ctx, obj, uctx, uobj
+
+    def find_first(self, h, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        idx = np.zeros(len(keys), dtype=np.uint64)
+        self._check(self.lib.kssdhip_find_first(h, _p(keys), len(keys), _p(idx)))
+        return idx
+
+    def pair(self, a, b, obj_bits, ignore_b_conflict=False):
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        f = Features()
+        self._check(self.lib.kssdhip_pair_features(_p(a), len(a), _p(b), len(b), obj_bits,
+                                                   int(ignore_b_conflict), ctypes.byref(f)))
+        return f.tup()
+
+
+FLAGS = {"LOWER": 1, "UNIQUE": 2, "IGNORECONFLICT": 4}

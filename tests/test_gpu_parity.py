@@ -1,0 +1,154 @@
+"""GPU parity: the HIP path (through the C ABI, libkssdhip.so) against the CPU
+oracle on the same seeded inputs.  Counts must be bit-exact."""
+import numpy as np
+import pytest
+
+import kssdtest as kt
+
+pytestmark = pytest.mark.gpu
+OBJ_BITS = 20
+
+
+def _block(samples):
+    keys = np.concatenate(samples) if samples else np.zeros(0, np.uint64)
+    idx = np.zeros(len(samples) + 1, np.uint64)
+    idx[1:] = np.cumsum([len(s) for s in samples])
+    return keys, idx
+
+
+def _fixture(seed, n_ref=9, n_qry=7, pool_n=300, mean=60, conflict_p=0.2, obj_bits=OBJ_BITS):
+    rng = np.random.default_rng(seed)
+    pool = rng.integers(1, 1 << (64 - obj_bits - 1), size=pool_n, dtype=np.uint64)
+    refs = kt.random_samples(rng, n_ref, pool_n, mean, obj_bits=obj_bits, shared_pool=pool,
+                             conflict_p=conflict_p)
+    qrys = kt.random_samples(rng, n_qry, pool_n, mean, obj_bits=obj_bits, shared_pool=pool,
+                             conflict_p=conflict_p)
+    return refs, qrys
+
+
+@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("flags", [0, 2, 4, 6])
+def test_count_block_matches_oracle(hip, oracle, seed, flags):
+    refs, qrys = _fixture(seed)
+    rkeys, ridx = _block(refs)
+    rec = kt.build_index_np(rkeys, ridx, OBJ_BITS)
+    qkeys, qidx = _block(qrys)
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        got = hip.count_block(h, len(refs), qkeys, qidx, flags=flags)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, len(refs), OBJ_BITS, qkeys, qidx, flags=flags)
+    names = ("ctx", "obj", "unique_ctx", "unique_obj")
+    for name, g, w in zip(names, got, want):
+        assert np.array_equal(g, w), (seed, flags, name)
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("ignore", [0, 4])
+def test_lower_triangle_blocks(hip, oracle, seed, ignore):
+    refs, _ = _fixture(1000 + seed, n_ref=23)
+    keys, idx = _block(refs)
+    rec = kt.build_index_np(keys, idx, OBJ_BITS)
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        # blocks of 5 with absolute offsets, like command_ani.c:1694-1701
+        for off in range(0, len(refs), 5):
+            end = min(off + 5, len(refs))
+            got = hip.count_block(h, len(refs), keys[idx[off]:idx[end]], idx[off:end + 1],
+                                  qry_gid_offset=off, flags=1 | ignore)
+            want = oracle.count_block(rec, len(refs), OBJ_BITS, keys[idx[off]:idx[end]],
+                                      idx[off:end + 1], qry_gid_offset=off, flags=1 | ignore)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), off
+    finally:
+        hip.close_index(h)
+
+
+@pytest.mark.parametrize("obj_bits", [20, 24, 28])
+def test_object_widths(hip, oracle, obj_bits):
+    # NUM_CODENS 11/10/9 with klen 32 give 20/24/28 object bits
+    # (tests/test_unique_context.py:257-266 in the reference)
+    refs, qrys = _fixture(77, obj_bits=obj_bits)
+    rkeys, ridx = _block(refs)
+    rec = kt.build_index_np(rkeys, ridx, obj_bits)
+    qkeys, qidx = _block(qrys)
+    h = hip.open_index(rec, len(refs), obj_bits)
+    try:
+        got = hip.count_block(h, len(refs), qkeys, qidx, flags=2)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, len(refs), obj_bits, qkeys, qidx, flags=2)
+    assert all(np.array_equal(g, w) for g, w in zip(got, want))
+    for qi, q in enumerate(qrys[:3]):
+        for g, r in enumerate(refs[:3]):
+            assert hip.pair(q, r, obj_bits) == oracle.pair(q, r, obj_bits)
+
+
+def test_empty_and_ragged(hip, oracle):
+    refs, qrys = _fixture(5, n_ref=6, n_qry=6)
+    refs[2] = np.zeros(0, np.uint64)          # empty reference
+    qrys[0] = np.zeros(0, np.uint64)          # empty first query
+    qrys[3] = np.zeros(0, np.uint64)          # empty middle query
+    qrys[5] = qrys[5][:1]                     # single-entry query
+    rkeys, ridx = _block(refs)
+    rec = kt.build_index_np(rkeys, ridx, OBJ_BITS)
+    qkeys, qidx = _block(qrys)
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        got = hip.count_block(h, len(refs), qkeys, qidx, flags=2)
+        assert hip.count_block(h, len(refs), qkeys[:0], qidx[:1])[0].shape == (0, len(refs), 2)
+        # all-empty block
+        got0 = hip.count_block(h, len(refs), np.zeros(0, np.uint64), np.zeros(3, np.uint64))
+        assert not got0[0].any() and not got0[1].any()
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, len(refs), OBJ_BITS, qkeys, qidx, flags=2)
+    assert all(np.array_equal(g, w) for g, w in zip(got, want))
+    assert hip.pair(np.zeros(0, np.uint64), refs[0], OBJ_BITS) == (0, 0, 0, 0)
+    assert hip.pair(refs[0], np.zeros(0, np.uint64), OBJ_BITS) == (0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("ignore", [False, True])
+def test_pair_matches_oracle(hip, oracle, seed, ignore):
+    refs, qrys = _fixture(300 + seed, n_ref=3, n_qry=3, mean=2000, pool_n=5000, conflict_p=0.3)
+    for a in qrys:
+        for b in refs:
+            assert hip.pair(a, b, OBJ_BITS, ignore) == oracle.pair(a, b, OBJ_BITS, ignore)
+    assert hip.pair(refs[0], refs[0], OBJ_BITS)[1:] == (0, 0, 0)
+
+
+def test_find_first_matches_oracle(hip, oracle):
+    refs, qrys = _fixture(9, n_ref=15, mean=500, pool_n=4000)
+    rkeys, ridx = _block(refs)
+    rec = kt.build_index_np(rkeys, ridx, OBJ_BITS)
+    q = np.unique(np.concatenate(qrys))
+    big = np.uint64(((1 << 43) - 5) << OBJ_BITS)
+    q = np.unique(np.concatenate([q, [big, np.uint64(0)]]))
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        got = hip.find_first(h, q)
+    finally:
+        hip.close_index(h)
+    assert np.array_equal(got, oracle.find_first(rec, q, OBJ_BITS))
+
+
+def test_larger_random_block(hip, oracle):
+    # a few thousand queries keys against a ~100k-record index
+    rng = np.random.default_rng(4242)
+    pool = rng.integers(1, 1 << 43, size=60000, dtype=np.uint64)
+    refs = kt.random_samples(rng, 40, 60000, 2500, shared_pool=pool, conflict_p=0.05)
+    qrys = kt.random_samples(rng, 64, 60000, 2500, shared_pool=pool, conflict_p=0.05)
+    rkeys, ridx = _block(refs)
+    rec = kt.build_index_np(rkeys, ridx, OBJ_BITS)
+    qkeys, qidx = _block(qrys)
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        got = hip.count_block(h, len(refs), qkeys, qidx, flags=2)
+        again = hip.count_block(h, len(refs), qkeys, qidx, flags=2)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, len(refs), OBJ_BITS, qkeys, qidx, flags=2)
+    assert all(np.array_equal(g, w) for g, w in zip(got, want))
+    # run-to-run determinism
+    assert all(np.array_equal(g, w) for g, w in zip(got, again))
