@@ -92,6 +92,24 @@ __global__ void k_unpack_records(const uint32_t *raw, uint64_t n, uint64_t *ctxg
 	}
 }
 
+// Flags records whose gid is out of range or that break the (ctxgid, obj)
+// order, so no later kernel can index a counter row out of bounds.
+__global__ void k_validate(const uint64_t *ctxgid, const uint32_t *obj, uint64_t n, uint32_t ref_gnum,
+			   uint32_t *bad)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (uint64_t)gridDim.x * blockDim.x) {
+		const uint64_t c = ctxgid[i];
+		bool ok = (c & GID_MASK) < ref_gnum;
+		if (i) {
+			const uint64_t p = ctxgid[i - 1];
+			ok = ok && (p < c || (p == c && obj[i - 1] <= obj[i]));
+		}
+		if (!ok)
+			atomicOr(bad, 1u);
+	}
+}
+
 // bucket[t] = first record with (ctx >> shift) >= t, bucket[2^k] = N.
 __global__ void k_buckets(const uint64_t *ctxgid, uint64_t n, int k, int shift, uint32_t *bucket)
 {
@@ -437,6 +455,21 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 			}
 			hipLaunchKernelGGL(k_unpack_records, dim3(grid_for(n)), dim3(256), 0, ix->stream,
 					   (const uint32_t *)d_raw, (uint64_t)n, ix->d_ctxgid, ix->d_obj);
+			uint32_t *d_bad = (uint32_t *)d_raw; // staging no longer needed after unpack
+			uint32_t bad = 0;
+			hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ix->stream, d_bad, (uint64_t)1, 0u);
+			hipLaunchKernelGGL(k_validate, dim3(grid_for(n)), dim3(256), 0, ix->stream,
+					   ix->d_ctxgid, ix->d_obj, (uint64_t)n, (uint32_t)ref_gnum, d_bad);
+			if (hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, ix->stream) != hipSuccess ||
+			    hipStreamSynchronize(ix->stream) != hipSuccess) {
+				rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: validation failed to run");
+				break;
+			}
+			if (bad) {
+				rc = kssdhip_fail(KSSDHIP_EINVAL,
+						  "index_open: records not sorted by (ctxgid, obj) or gid >= ref_gnum");
+				break;
+			}
 		}
 		hipLaunchKernelGGL(k_buckets, dim3(grid_for(nb)), dim3(256), 0, ix->stream,
 				   ix->d_ctxgid, (uint64_t)n, k, ix->shift, ix->d_bucket);

@@ -1,0 +1,97 @@
+/*
+ * Host-side (CPU) components around the HIP path: the sketcher that turns
+ * sequences into context-object keys, a seeded genome simulator used to
+ * make synthetic workloads, and the inverted-index builder.
+ *
+ * Key layout (default coden pattern, NUM_CODENS = 11, klen = 32):
+ *   key = ctx44 << 20 | obj20, ctx/obj extracted from the canonical 32-mer
+ *   with the mask 0xf3cf3cf3cf3cf3cf (sketch_rearrange.h:109-122,
+ *   command_sketch.h:66-72).
+ */
+#ifndef KSSD_HOST_H
+#define KSSD_HOST_H
+#include <stddef.h>
+#include <stdint.h>
+
+#include "kssdhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSSD_CODEN_CTXMASK 0xf3cf3cf3cf3cf3cfULL
+#define KSSD_KLEN 32
+#define KSSD_OBJ_BITS 20
+
+/* Growable u64 vector owned by the host library. */
+typedef struct {
+	uint64_t *a;
+	size_t n, cap;
+} kssd_u64vec_t;
+void kssd_u64vec_free(kssd_u64vec_t *v);
+
+/*
+ * Append the kept keys of one contiguous sequence of 2-bit base codes
+ * (0..3 = A,C,G,T; any value >= 4 is a non-ACGT base that restarts the
+ * window) to out.  Restates sketch_read_into_vec, command_sketch.c:3913-3949:
+ * canonical strand by the smaller ctx-masked k-mer (ties take the reverse
+ * strand), kept iff (u32)mix64(ctx-masked canonical k-mer) <= UINT32_MAX >> drfold.
+ */
+void kssd_sketch_codes(const uint8_t *codes, size_t len, int drfold, kssd_u64vec_t *out);
+
+/* Sort ascending, drop duplicates and (unless keep_conflict) drop every
+ * context that carries more than one object (kssdlib_sort.c:388-407). */
+void kssd_finalize_keys(kssd_u64vec_t *v, int keep_conflict);
+
+/* Map ASCII to base codes with the reference's Basemap (global_basic.c:131-142). */
+void kssd_ascii_to_codes(const char *s, size_t len, uint8_t *codes);
+
+/* Sketch a FASTA/plain-sequence file as ONE sample (all records pooled,
+ * the window restarting at each record).  Returns 0 or -1 (errno set). */
+int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd_u64vec_t *out);
+
+/* GET_SKETCHING_ID(hclen, holen, iolen, drfold, FILTER), command_sketch.h:49-52
+ * and command_sketch.c:771, for the coden pattern (hclen = holen = iolen = 0). */
+uint32_t kssd_sketching_id(int hclen, int holen, int iolen, int drfold);
+
+/* ---- synthetic genomes -------------------------------------------------- */
+typedef struct {
+	uint64_t anc_seed;  /* seed of the ancestor genome (uniform ACGT) */
+	uint64_t anc_len;   /* ancestor length in bases */
+	uint64_t mut_seed;  /* seed of the mutation process */
+	double sub_rate;    /* i.i.d. substitution rate (new base uniform among the other 3) */
+	double indel_frac;  /* bases touched by indels / length (events of 1..50 bp, 50/50 ins/del) */
+	uint64_t anc2_seed; /* nonzero: first derive the ancestor from anc_seed with ... */
+	double anc2_rate;   /* ... this substitution rate (clade structure) */
+} kssd_simspec_t;
+
+/* Generate genome `spec` into codes (malloc'd, *len set). */
+int kssd_sim_genome(const kssd_simspec_t *spec, uint8_t **codes, size_t *len);
+
+/*
+ * Simulate and sketch n genomes in parallel (nthreads OpenMP threads).
+ * keys: concatenated finalized sketches (malloc'd, caller frees with free());
+ * index: n+1 prefix offsets (caller-allocated).
+ */
+int kssd_simsketch(const kssd_simspec_t *specs, int n, int drfold, int keep_conflict,
+		   int nthreads, uint64_t **keys, uint64_t *index);
+
+/* ---- inverted index ------------------------------------------------------ */
+/*
+ * ctxobj64_2ctxgidobj (sketch_rearrange.c:65-78) + sort by (ctxgid, obj)
+ * (kssdlib_sort.c:145-148) with a parallel radix sort.  out has index[n]
+ * records.  Returns 0, or -1 if the input breaks the format's limits
+ * (n > 2^20 samples, more than UINT32_MAX entries; command_sketch.c:922-928).
+ */
+int kssd_build_index(const uint64_t *keys, const uint64_t *index, int n, unsigned obj_bits,
+		     int nthreads, kssd_ctxgidobj_t *out);
+
+/* Context totals (AF denominators): command_ani.c:337-401, :1533-1548. */
+uint32_t kssd_ctx_runs(const uint64_t *a, size_t n, unsigned obj_bits, int ignoreconflict);
+void kssd_sample_ctx_counts(const uint64_t *keys, const uint64_t *index, int n, unsigned obj_bits,
+			    int conflict, int ignoreconflict, uint32_t *counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,405 @@
+/*
+ * Sketcher (sequence -> sorted context-object keys) and seeded genome
+ * simulator.  See kssd_host.h for the contract.
+ */
+#define _GNU_SOURCE
+#include <ctype.h>
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <zlib.h>
+
+#include "kssd_host.h"
+
+void kssd_u64vec_free(kssd_u64vec_t *v)
+{
+	free(v->a);
+	v->a = NULL;
+	v->n = v->cap = 0;
+}
+
+static int vec_reserve(kssd_u64vec_t *v, size_t want)
+{
+	if (want <= v->cap)
+		return 0;
+	size_t cap = v->cap ? v->cap : 4096;
+	while (cap < want)
+		cap <<= 1;
+	uint64_t *a = realloc(v->a, cap * sizeof *a);
+	if (!a)
+		return -1;
+	v->a = a;
+	v->cap = cap;
+	return 0;
+}
+
+/* SplitMix64 finalizer, the sketch selection hash (kssdlib_hash.h:18-23). */
+static inline uint64_t mix64(uint64_t x)
+{
+	x += 0x9e3779b97f4a7c15ULL;
+	x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+	x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+	return x ^ (x >> 31);
+}
+
+/* Extract the 44 context bits and 20 object bits of a 32-mer under the
+ * coden mask: from the least significant end the mask alternates 4 context
+ * bits and 2 object bits ten times, then 4 context bits (0xf3cf...cf). */
+static inline uint64_t coden_key(uint64_t kmer)
+{
+	uint64_t ctx = 0, obj = 0;
+	for (int i = 0; i < 10; ++i) {
+		ctx |= (kmer & 0xFULL) << (4 * i);
+		kmer >>= 4;
+		obj |= (kmer & 0x3ULL) << (2 * i);
+		kmer >>= 2;
+	}
+	ctx |= (kmer & 0xFULL) << 40;
+	return (ctx << KSSD_OBJ_BITS) | obj;
+}
+
+void kssd_sketch_codes(const uint8_t *codes, size_t len, int drfold, kssd_u64vec_t *out)
+{
+	const uint32_t filter = drfold >= 32 ? 0u : (uint32_t)(UINT32_MAX >> drfold);
+	const unsigned len_mv = 2 * KSSD_KLEN - 2;
+	uint64_t fwd = 0, rev = 0;
+	int base = 0;
+	if (len < KSSD_KLEN)
+		return;
+	for (size_t pos = 0; pos < len; ++pos) {
+		const uint8_t b = codes[pos];
+		if (b > 3) {
+			base = 0;
+			fwd = rev = 0;
+			continue;
+		}
+		fwd = (fwd << 2) | b;
+		rev = (rev >> 2) | ((uint64_t)(b ^ 3u) << len_mv);
+		if (++base < KSSD_KLEN)
+			continue;
+		const uint64_t tf = fwd & KSSD_CODEN_CTXMASK, tr = rev & KSSD_CODEN_CTXMASK;
+		const int use_fwd = tf < tr;
+		if ((uint32_t)mix64(use_fwd ? tf : tr) > filter)
+			continue;
+		if (out->n == out->cap && vec_reserve(out, out->n + 1))
+			abort();
+		out->a[out->n++] = coden_key(use_fwd ? fwd : rev);
+	}
+}
+
+/* LSD radix sort of u64 (8 passes of 8 bits, skipping constant bytes). */
+static void radix_sort_u64(uint64_t *a, size_t n)
+{
+	if (n < 64) {
+		for (size_t i = 1; i < n; ++i) {
+			uint64_t v = a[i];
+			size_t j = i;
+			while (j && a[j - 1] > v) {
+				a[j] = a[j - 1];
+				--j;
+			}
+			a[j] = v;
+		}
+		return;
+	}
+	uint64_t *tmp = malloc(n * sizeof *tmp);
+	if (!tmp)
+		abort();
+	uint64_t *src = a, *dst = tmp;
+	for (int shift = 0; shift < 64; shift += 8) {
+		size_t cnt[256] = {0};
+		for (size_t i = 0; i < n; ++i)
+			cnt[(src[i] >> shift) & 0xFF]++;
+		if (cnt[(src[0] >> shift) & 0xFF] == n)
+			continue;
+		size_t sum = 0;
+		for (int b = 0; b < 256; ++b) {
+			size_t c = cnt[b];
+			cnt[b] = sum;
+			sum += c;
+		}
+		for (size_t i = 0; i < n; ++i)
+			dst[cnt[(src[i] >> shift) & 0xFF]++] = src[i];
+		uint64_t *t = src;
+		src = dst;
+		dst = t;
+	}
+	if (src != a)
+		memcpy(a, src, n * sizeof *a);
+	free(tmp);
+}
+
+void kssd_finalize_keys(kssd_u64vec_t *v, int keep_conflict)
+{
+	if (!v->n)
+		return;
+	radix_sort_u64(v->a, v->n);
+	size_t w = 1;
+	for (size_t i = 1; i < v->n; ++i)
+		if (v->a[i] != v->a[w - 1])
+			v->a[w++] = v->a[i];
+	v->n = w;
+	if (keep_conflict)
+		return;
+	/* remove_ctx_with_conflict_obj_noabund, kssdlib_sort.c:388-407 */
+	size_t i = 0;
+	w = 0;
+	while (i < v->n) {
+		size_t c = 1;
+		while (i + c < v->n && ((v->a[i] ^ v->a[i + c]) >> KSSD_OBJ_BITS) == 0)
+			++c;
+		if (c == 1)
+			v->a[w++] = v->a[i];
+		i += c;
+	}
+	v->n = w;
+}
+
+void kssd_ascii_to_codes(const char *s, size_t len, uint8_t *codes)
+{
+	for (size_t i = 0; i < len; ++i) {
+		switch (s[i]) {
+		case 'A': case 'a': codes[i] = 0; break;
+		case 'C': case 'c': codes[i] = 1; break;
+		case 'G': case 'g': codes[i] = 2; break;
+		case 'T': case 't': codes[i] = 3; break;
+		default: codes[i] = 4; break;
+		}
+	}
+}
+
+int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd_u64vec_t *out)
+{
+	gzFile fp = gzopen(path, "rb");
+	if (!fp)
+		return -1;
+	size_t cap = 1 << 20, n = 0;
+	uint8_t *codes = malloc(cap);
+	char *line = malloc(1 << 16);
+	if (!codes || !line) {
+		gzclose(fp);
+		free(codes);
+		free(line);
+		errno = ENOMEM;
+		return -1;
+	}
+	int in_fastq_qual = 0, fastq = -1, line_no = 0;
+	while (gzgets(fp, line, 1 << 16)) {
+		size_t l = strlen(line);
+		while (l && (line[l - 1] == '\n' || line[l - 1] == '\r'))
+			line[--l] = '\0';
+		if (fastq < 0 && l)
+			fastq = line[0] == '@';
+		if (fastq == 1) {
+			/* 4-line FASTQ records: header, sequence, '+', quality */
+			const int k = line_no++ % 4;
+			if (k == 0 || k == 2 || k == 3) {
+				if (k == 0 && n) {
+					kssd_sketch_codes(codes, n, drfold, out);
+					n = 0;
+				}
+				continue;
+			}
+		} else if (l && (line[0] == '>' || line[0] == ';')) {
+			if (n)
+				kssd_sketch_codes(codes, n, drfold, out);
+			n = 0;
+			continue;
+		}
+		(void)in_fastq_qual;
+		if (n + l > cap) {
+			while (n + l > cap)
+				cap <<= 1;
+			uint8_t *c2 = realloc(codes, cap);
+			if (!c2)
+				abort();
+			codes = c2;
+		}
+		kssd_ascii_to_codes(line, l, codes + n);
+		n += l;
+	}
+	if (n)
+		kssd_sketch_codes(codes, n, drfold, out);
+	gzclose(fp);
+	free(codes);
+	free(line);
+	kssd_finalize_keys(out, keep_conflict);
+	return 0;
+}
+
+uint32_t kssd_sketching_id(int hclen, int holen, int iolen, int drfold)
+{
+	const uint32_t filter = drfold >= 32 ? 0u : (uint32_t)(UINT32_MAX >> drfold);
+	return (uint32_t)mix64((uint64_t)(uint32_t)hclen) ^ (uint32_t)mix64((uint64_t)(uint32_t)holen) ^
+	       (uint32_t)mix64((uint64_t)(uint32_t)iolen) ^ (uint32_t)mix64((uint64_t)(uint32_t)drfold) ^
+	       (uint32_t)mix64((uint64_t)filter) ^ (uint32_t)mix64(31415926ULL);
+}
+
+/* ---------------------------------------------------------------- simulator */
+typedef struct {
+	uint64_t s[4];
+} rng_t;
+
+static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+static void rng_seed(rng_t *r, uint64_t seed)
+{
+	for (int i = 0; i < 4; ++i) {
+		seed += 0x9e3779b97f4a7c15ULL;
+		uint64_t z = seed;
+		z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+		z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+		r->s[i] = z ^ (z >> 31);
+	}
+}
+
+/* xoshiro256** */
+static inline uint64_t rng_next(rng_t *r)
+{
+	const uint64_t res = rotl(r->s[1] * 5, 7) * 9;
+	const uint64_t t = r->s[1] << 17;
+	r->s[2] ^= r->s[0];
+	r->s[3] ^= r->s[1];
+	r->s[1] ^= r->s[2];
+	r->s[0] ^= r->s[3];
+	r->s[2] ^= t;
+	r->s[3] = rotl(r->s[3], 45);
+	return res;
+}
+
+static inline double rng_unif(rng_t *r) { return (rng_next(r) >> 11) * 0x1.0p-53; }
+
+static void random_bases(rng_t *r, uint8_t *out, size_t n)
+{
+	size_t i = 0;
+	while (i < n) {
+		uint64_t x = rng_next(r);
+		for (int k = 0; k < 32 && i < n; ++k, x >>= 2)
+			out[i++] = (uint8_t)(x & 3);
+	}
+}
+
+/* i.i.d. substitutions: geometric skipping keeps this O(mutations). */
+static void substitute(rng_t *r, uint8_t *g, size_t n, double rate)
+{
+	if (rate <= 0)
+		return;
+	if (rate >= 1)
+		rate = 0.999999;
+	const double lq = log1p(-rate);
+	size_t pos = 0;
+	for (;;) {
+		const double u = rng_unif(r);
+		const double skip = floor(log1p(-u) / lq);
+		if (skip >= (double)(n - pos))
+			break;
+		pos += (size_t)skip;
+		g[pos] = (uint8_t)((g[pos] + 1 + (rng_next(r) % 3)) & 3);
+		if (++pos >= n)
+			break;
+	}
+}
+
+int kssd_sim_genome(const kssd_simspec_t *spec, uint8_t **codes, size_t *len)
+{
+	rng_t r;
+	size_t n = (size_t)spec->anc_len;
+	uint8_t *g = malloc(n ? n : 1);
+	if (!g)
+		return -1;
+	rng_seed(&r, spec->anc_seed);
+	random_bases(&r, g, n);
+	if (spec->anc2_seed) {
+		rng_seed(&r, spec->anc2_seed);
+		substitute(&r, g, n, spec->anc2_rate);
+	}
+	rng_seed(&r, spec->mut_seed);
+	substitute(&r, g, n, spec->sub_rate);
+	if (spec->indel_frac > 0 && n > 100) {
+		/* indel events of U{1..50} bp until ~indel_frac*n bases are touched */
+		const size_t budget = (size_t)(spec->indel_frac * (double)n);
+		const size_t cap = n + budget + 64;
+		uint8_t *o = malloc(cap);
+		if (!o) {
+			free(g);
+			return -1;
+		}
+		/* pick event positions as a sorted uniform sample */
+		size_t nev = budget / 25 + 1, touched = 0, ip = 0, op = 0;
+		double gap_mean = (double)n / (double)(nev + 1);
+		while (ip < n) {
+			size_t gap = (size_t)(-log1p(-rng_unif(&r)) * gap_mean);
+			if (gap > n - ip)
+				gap = n - ip;
+			memcpy(o + op, g + ip, gap);
+			op += gap;
+			ip += gap;
+			if (ip >= n || touched >= budget)
+				break;
+			const size_t el = 1 + rng_next(&r) % 50;
+			if (rng_next(&r) & 1) { /* insertion */
+				if (op + el > cap)
+					break;
+				random_bases(&r, o + op, el);
+				op += el;
+			} else {                 /* deletion */
+				ip += el < n - ip ? el : n - ip;
+			}
+			touched += el;
+		}
+		if (ip < n && op + (n - ip) <= cap) {
+			memcpy(o + op, g + ip, n - ip);
+			op += n - ip;
+		}
+		free(g);
+		g = o;
+		n = op;
+	}
+	*codes = g;
+	*len = n;
+	return 0;
+}
+
+int kssd_simsketch(const kssd_simspec_t *specs, int n, int drfold, int keep_conflict,
+		   int nthreads, uint64_t **keys_out, uint64_t *index)
+{
+	kssd_u64vec_t *v = calloc((size_t)(n > 0 ? n : 1), sizeof *v);
+	if (!v)
+		return -1;
+	int fail = 0;
+	if (nthreads <= 0)
+		nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+	for (int i = 0; i < n; ++i) {
+		uint8_t *g = NULL;
+		size_t len = 0;
+		if (kssd_sim_genome(&specs[i], &g, &len)) {
+			fail = 1;
+			continue;
+		}
+		kssd_sketch_codes(g, len, drfold, &v[i]);
+		free(g);
+		kssd_finalize_keys(&v[i], keep_conflict);
+	}
+	index[0] = 0;
+	for (int i = 0; i < n; ++i)
+		index[i + 1] = index[i] + v[i].n;
+	uint64_t *keys = malloc((index[n] ? index[n] : 1) * sizeof *keys);
+	if (!keys || fail) {
+		for (int i = 0; i < n; ++i)
+			kssd_u64vec_free(&v[i]);
+		free(v);
+		free(keys);
+		return -1;
+	}
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (int i = 0; i < n; ++i) {
+		if (v[i].n)
+			memcpy(keys + index[i], v[i].a, v[i].n * sizeof *keys);
+		kssd_u64vec_free(&v[i]);
+	}
+	free(v);
+	*keys_out = keys;
+	return 0;
+}

@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the reference checker
+binary (oracle/_ref/kssd3a_ref, compiled from /root/reference by
+`make -C oracle ref`).  Run in the container that has /root/reference:
+
+    python tests/golden/make_golden.py
+
+Every fixture is data: input sketch directories (or small FASTA files) and
+the reference's outputs on them.  Inputs are generated here from fixed seeds.
+
+  counts/case_XX/      hand-built sketch dirs (test_unique_context.py style,
+                       with object conflicts) + the reference's
+                       --unique-ctx-report (every count column of every pair
+                       with a shared context) and its detail output
+  genomes/             8 references + 8 queries simulated from 2 ancestors,
+                       sketched by the reference, plus its detail, matrix,
+                       triangle and self-matrix outputs (direct and indexed)
+  sketch/              small FASTA inputs and the reference's sketches of them
+"""
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT / "tests"))
+import kssdtest as kt  # noqa: E402
+
+BIN = ROOT / "oracle" / "_ref" / "kssd3a_ref"
+
+
+def run(args, env=None, cwd=None):
+    e = dict(os.environ, OMP_NUM_THREADS="2")
+    e.pop("KSSD3A_FORCE_REF_INDEX", None)
+    e.update(env or {})
+    p = subprocess.run([str(BIN), *map(str, args)], capture_output=True, text=True, env=e,
+                       cwd=cwd, timeout=600)
+    if p.returncode != 0:
+        raise RuntimeError(f"{args}: {p.stderr}")
+    return p.stdout
+
+
+# ------------------------------------------------------------------ counts
+def count_fixture(rng, n_ref, n_qry, ref_entries, qry_entries, ctx_range, obj_range):
+    def sample(n):
+        pairs = {(int(rng.integers(1, ctx_range)), int(rng.integers(0, obj_range))) for _ in range(n)}
+        return sorted(pairs)
+    refs = [sample(ref_entries) for _ in range(n_ref)]
+    qrys = [sample(qry_entries) for _ in range(n_qry)]
+    return refs, qrys
+
+
+def to_keys(sample, obj_bits=20):
+    return np.array([(c << obj_bits) | o for c, o in sample], dtype=np.uint64)
+
+
+def make_counts():
+    out = HERE / "counts"
+    shutil.rmtree(out, ignore_errors=True)
+    cases = []
+    # the reference's randomized oracle shape (test_unique_context.py:268-278)
+    for seed in range(8):
+        cases.append((seed, 4, 2, 25, 35, 40, 8))
+    # larger, more conflicts and more sharing
+    for seed in range(8, 12):
+        cases.append((seed, 12, 6, 200, 240, 400, 64))
+    # ragged: one empty reference, one empty query
+    cases.append((12, 5, 4, 60, 60, 120, 16))
+    for seed, nr, nq, re_, qe, cr, orr in cases:
+        rng = np.random.default_rng(seed)
+        refs, qrys = count_fixture(rng, nr, nq, re_, qe, cr, orr)
+        if seed == 12:
+            refs[1] = []
+            qrys[2] = []
+        d = out / f"case_{seed:02d}"
+        kt.write_sketch_dir(d / "ref", [to_keys(s) for s in refs],
+                            names=[f"ref_{i}" for i in range(nr)], indexed=True)
+        kt.write_sketch_dir(d / "qry", [to_keys(s) for s in qrys],
+                            names=[f"qry_{i}" for i in range(nq)])
+        base = ["ani", "-r", d / "ref", "--qraw", d / "qry", "-p2", "-n0", "-f0", "-t1"]
+        report = d / "unique_report.tsv"
+        report.unlink(missing_ok=True)
+        (d / "detail_qraw.tsv").write_text(run(base, env={"KSSD3A_FORCE_REF_INDEX": "1"}))
+        run([*base, "--unique-ctx-report", report])
+        (d / "detail_qraw_unique.tsv").write_text(run([*base, "--unique-ctx"]))
+        (d / "detail_qraw_ignoreconflict.tsv").write_text(
+            run([*base, "--ignoreconflict"], env={"KSSD3A_FORCE_REF_INDEX": "1"}))
+
+
+# ------------------------------------------------------------------ genomes
+def mutate(rng, seq, rate):
+    s = seq.copy()
+    hit = np.flatnonzero(rng.random(len(s)) < rate)
+    s[hit] = (s[hit] + rng.integers(1, 4, size=len(hit))) % 4
+    return s
+
+
+def write_fasta(path, name, seq):
+    letters = np.frombuffer(b"ACGT", dtype=np.uint8)[seq]
+    body = letters.tobytes()
+    with open(path, "wb") as fh:
+        fh.write(f">{name}\n".encode())
+        for i in range(0, len(body), 80):
+            fh.write(body[i:i + 80] + b"\n")
+
+
+def make_genomes():
+    out = HERE / "genomes"
+    shutil.rmtree(out, ignore_errors=True)
+    tmp = out / "_fasta"
+    tmp.mkdir(parents=True)
+    rng = np.random.default_rng(20261203)
+    refs, qrys = [], []
+    for a in range(2):
+        anc = rng.integers(0, 4, size=120_000, dtype=np.uint8)
+        for d in range(4):
+            r = mutate(rng, anc, rng.uniform(0.005, 0.05))
+            p = tmp / f"ref{a}{d}.fa"
+            write_fasta(p, f"ref{a}{d}", r)
+            refs.append(p)
+            q = mutate(rng, anc, rng.uniform(0.005, 0.05))
+            p = tmp / f"qry{a}{d}.fa"
+            write_fasta(p, f"qry{a}{d}", q)
+            qrys.append(p)
+    # names in lcofiles.stat are the paths as given: use relative names
+    run(["sketch", "-o", "ref", *[p.relative_to(out) for p in refs]], cwd=out)
+    run(["sketch", "-o", "qry", *[p.relative_to(out) for p in qrys]], cwd=out)
+    run(["sketch", "-o", "all", *[p.relative_to(out) for p in refs + qrys]], cwd=out)
+    R, Q, S = out / "ref", out / "qry", out / "all"
+    w = lambda name, text: (out / name).write_text(text)  # noqa: E731
+    # hash path (no index yet; ref_n <= 100)
+    w("detail_best_hash.tsv", run(["ani", "-r", R, "-q", Q]))
+    w("detail_all_s3_hash.tsv", run(["ani", "-r", R, "-q", Q, "-s3", "-n0", "-f0", "-t1"]))
+    w("matrix_ani_rq.tsv", run(["ani", "-r", R, "-q", Q, "--format", "matrix", "--values", "ani", "-s3"]))
+    w("self_matrix_direct_s3.tsv", run(["ani", "-q", S, "--format", "matrix", "--values", "ani", "-s3"]))
+    w("self_matrix_direct_s9.tsv", run(["ani", "-q", S, "--format", "matrix", "-s9"]))
+    w("self_matrix_indexed_s3.tsv", run(["ani", "-q", S, "--format", "matrix", "--values", "ani", "-s3"],
+                                        env={"KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD": "0"}))
+    w("self_triangle_indexed_s4.tsv", run(["ani", "-q", S, "--format", "triangle", "-s4"],
+                                          env={"KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD": "0"}))
+    w("matrix_triangle_default.tsv", run(["matrix", S]))
+    w("matrix_full_default.tsv", run(["matrix", "--format", "full", S]))
+    w("matrix_rq_default.tsv", run(["matrix", "-r", R, "-q", Q]))
+    # index path: build the inverted index, then the same detail requests
+    run(["sketch", "-i", R])
+    w("detail_best_index.tsv", run(["ani", "-r", R, "-q", Q]))
+    w("detail_all_s3_index.tsv", run(["ani", "-r", R, "-q", Q, "-s3", "-n0", "-f0", "-t1"]))
+    w("detail_all_s9_index.tsv", run(["ani", "-r", R, "-q", Q, "-s9", "-n0", "-f0", "-t1"]))
+    w("detail_qraw_index.tsv", run(["ani", "-r", R, "--qraw", Q, "-n0", "-f0", "-t1"]))
+    shutil.rmtree(tmp)
+
+
+def main():
+    if not BIN.exists():
+        raise SystemExit(f"{BIN} missing: run `make -C oracle ref` first")
+    make_counts()
+    make_genomes()
+    print("golden fixtures written under", HERE)
+
+
+if __name__ == "__main__":
+    main()

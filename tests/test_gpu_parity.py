@@ -152,3 +152,13 @@ def test_larger_random_block(hip, oracle):
     assert all(np.array_equal(g, w) for g, w in zip(got, want))
     # run-to-run determinism
     assert all(np.array_equal(g, w) for g, w in zip(got, again))
+
+
+def test_index_open_rejects_bad_index(hip):
+    refs, _ = _fixture(3, n_ref=4)
+    keys, idx = _block(refs)
+    rec = kt.build_index_np(keys, idx, OBJ_BITS)
+    with pytest.raises(RuntimeError, match="not sorted"):
+        hip.open_index(rec[::-1].copy(), 4, OBJ_BITS)
+    with pytest.raises(RuntimeError, match="gid"):
+        hip.open_index(rec, 2, OBJ_BITS)
