@@ -51,7 +51,8 @@ void kssd_ascii_to_codes(const char *s, size_t len, uint8_t *codes);
 int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd_u64vec_t *out);
 
 /* GET_SKETCHING_ID(hclen, holen, iolen, drfold, FILTER), command_sketch.h:49-52
- * and command_sketch.c:771, for the coden pattern (hclen = holen = iolen = 0). */
+ * and command_sketch.c:771.  The coden pattern passes (NUM_CODENS, 0, 0, drfold)
+ * (command_sketch_wrapper.c:924). */
 uint32_t kssd_sketching_id(int hclen, int holen, int iolen, int drfold);
 
 /* ---- synthetic genomes -------------------------------------------------- */

@@ -154,11 +154,37 @@ def make_genomes():
     shutil.rmtree(tmp)
 
 
+def make_sketch():
+    """Small FASTA inputs (multi-record, lowercase, non-ACGT runs) and the
+    reference's sketches of them, at the default -f8 and at -f4."""
+    out = HERE / "sketch"
+    shutil.rmtree(out, ignore_errors=True)
+    out.mkdir(parents=True)
+    rng = np.random.default_rng(77)
+    files = []
+    for i in range(3):
+        seq = rng.integers(0, 4, size=40_000, dtype=np.uint8)
+        letters = bytearray(np.frombuffer(b"ACGT", dtype=np.uint8)[seq].tobytes())
+        if i == 1:  # lowercase stretch and an N run
+            letters[1000:3000] = letters[1000:3000].lower()
+            letters[5000:5040] = b"N" * 40
+        text = b">rec_a\n" + bytes(letters[:25_000]) + b"\n"
+        if i != 0:  # a second record restarts the k-mer window
+            text += b">rec_b desc\n" + bytes(letters[25_000:]) + b"\nRYKM\n"
+        p = out / f"seq{i}.fa"
+        p.write_bytes(text)
+        files.append(p.name)
+    run(["sketch", "-o", "f8", *files], cwd=out)
+    run(["sketch", "-f", "4", "-o", "f4", *files], cwd=out)
+    run(["sketch", "--conflict", "-o", "f8c", *files], cwd=out)
+
+
 def main():
     if not BIN.exists():
         raise SystemExit(f"{BIN} missing: run `make -C oracle ref` first")
     make_counts()
     make_genomes()
+    make_sketch()
     print("golden fixtures written under", HERE)
 
 

@@ -308,3 +308,84 @@ class KssdHip:
 
 
 FLAGS = {"LOWER": 1, "UNIQUE": 2, "IGNORECONFLICT": 4}
+
+
+class U64Vec(ctypes.Structure):
+    _fields_ = [("a", ctypes.POINTER(ctypes.c_uint64)), ("n", ctypes.c_size_t),
+                ("cap", ctypes.c_size_t)]
+
+
+class SimSpec(ctypes.Structure):
+    _fields_ = [("anc_seed", ctypes.c_uint64), ("anc_len", ctypes.c_uint64),
+                ("mut_seed", ctypes.c_uint64), ("sub_rate", ctypes.c_double),
+                ("indel_frac", ctypes.c_double), ("anc2_seed", ctypes.c_uint64),
+                ("anc2_rate", ctypes.c_double)]
+
+
+def load_kssdhost():
+    so = PKG / "libkssdhost.so"
+    if not so.exists():
+        raise RuntimeError(f"{so} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(so))
+    V, S, I, U = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint
+    lib.kssd_u64vec_free.argtypes = [ctypes.POINTER(U64Vec)]
+    lib.kssd_u64vec_free.restype = None
+    lib.kssd_sketch_fasta_file.argtypes = [ctypes.c_char_p, I, I, ctypes.POINTER(U64Vec)]
+    lib.kssd_sketching_id.argtypes = [I, I, I, I]
+    lib.kssd_sketching_id.restype = ctypes.c_uint32
+    lib.kssd_simsketch.argtypes = [V, I, I, I, I, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint64)), V]
+    lib.kssd_build_index.argtypes = [V, V, I, U, I, V]
+    lib.kssd_sample_ctx_counts.argtypes = [V, V, I, U, I, I, V]
+    lib.kssd_sample_ctx_counts.restype = None
+    lib.kssd_sketch_codes.argtypes = [V, S, I, ctypes.POINTER(U64Vec)]
+    lib.kssd_sketch_codes.restype = None
+    lib.kssd_finalize_keys.argtypes = [ctypes.POINTER(U64Vec), I]
+    lib.kssd_finalize_keys.restype = None
+    return lib
+
+
+class Host:
+    """numpy-facing wrapper over libkssdhost.so (sketcher, simulator, index)."""
+    _libc = ctypes.CDLL(None)
+
+    def __init__(self):
+        self.lib = load_kssdhost()
+
+    def sketch_file(self, path, drfold=8, keep_conflict=False):
+        v = U64Vec()
+        if self.lib.kssd_sketch_fasta_file(str(path).encode(), drfold, int(keep_conflict),
+                                           ctypes.byref(v)) != 0:
+            raise OSError(f"cannot sketch {path}")
+        out = np.ctypeslib.as_array(v.a, shape=(v.n,)).copy() if v.n else np.zeros(0, np.uint64)
+        self.lib.kssd_u64vec_free(ctypes.byref(v))
+        return out
+
+    def simsketch(self, specs, drfold=8, keep_conflict=False, threads=os.cpu_count() or 1):
+        arr = (SimSpec * len(specs))(*specs)
+        keys = ctypes.POINTER(ctypes.c_uint64)()
+        index = np.zeros(len(specs) + 1, np.uint64)
+        if self.lib.kssd_simsketch(arr, len(specs), drfold, int(keep_conflict), int(threads),
+                                   ctypes.byref(keys), _p(index)) != 0:
+            raise RuntimeError("simsketch failed")
+        n = int(index[-1])
+        out = np.ctypeslib.as_array(keys, shape=(n,)).copy() if n else np.zeros(0, np.uint64)
+        self._libc.free.argtypes = [ctypes.c_void_p]
+        self._libc.free(ctypes.cast(keys, ctypes.c_void_p))
+        return out, index
+
+    def build_index(self, keys, index, obj_bits=20, threads=os.cpu_count() or 1):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.empty(len(keys), dtype=REC_DTYPE)
+        if self.lib.kssd_build_index(_p(keys), _p(index), len(index) - 1, obj_bits, int(threads),
+                                     _p(out)) != 0:
+            raise ValueError("index limits exceeded")
+        return out
+
+    def ctx_counts(self, keys, index, obj_bits=20, conflict=False, ignoreconflict=False):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.zeros(len(index) - 1, np.uint32)
+        self.lib.kssd_sample_ctx_counts(_p(keys), _p(index), len(index) - 1, obj_bits,
+                                        int(conflict), int(ignoreconflict), _p(out))
+        return out

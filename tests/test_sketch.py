@@ -1,0 +1,67 @@
+"""CPU tests of the host components: the sketcher against the reference's own
+sketches of the same FASTA files (tests/golden/sketch, -f8, -f4, --conflict),
+the sketching id, the inverted-index builder and the simulator."""
+import numpy as np
+import pytest
+
+import kssdtest as kt
+
+SK = kt.GOLDEN / "sketch"
+
+
+@pytest.fixture(scope="module")
+def host():
+    return kt.Host()
+
+
+@pytest.mark.parametrize("sub,drfold,conflict", [("f8", 8, False), ("f4", 4, False), ("f8c", 8, True)])
+def test_sketcher_matches_reference(host, sub, drfold, conflict):
+    ref = kt.read_sketch_dir(SK / sub)
+    assert ref.drfold == drfold and bool(ref.conflict) == conflict
+    for i, name in enumerate(ref.names):
+        got = host.sketch_file(SK / name, drfold=drfold, keep_conflict=conflict)
+        assert np.array_equal(got, ref.sample(i)), (sub, name)
+    # coden mode hashes NUM_CODENS in the first slot (command_sketch_wrapper.c:924)
+    assert host.lib.kssd_sketching_id(11, 0, 0, drfold) == ref.hash_id
+
+
+def test_index_builder_matches_reference_file(host):
+    g = kt.read_sketch_dir(kt.GOLDEN / "genomes" / "ref")
+    out = host.build_index(g.keys, g.index, g.obj_bits, threads=3)
+    assert out.tobytes() == g.records.tobytes()
+    # and on conflict-rich fixtures against the independent numpy build
+    for case in sorted((kt.GOLDEN / "counts").glob("case_*")):
+        r = kt.read_sketch_dir(case / "ref")
+        assert host.build_index(r.keys, r.index, r.obj_bits).tobytes() == r.records.tobytes()
+
+
+def test_ctx_counts(host, oracle):
+    c = kt.read_sketch_dir(kt.GOLDEN / "counts" / "case_09" / "ref")
+    for ign in (0, 1):
+        got = host.ctx_counts(c.keys, c.index, c.obj_bits, conflict=True, ignoreconflict=bool(ign))
+        want = [oracle.lib.orc_ctx_runs(kt._p(c.sample(i)), len(c.sample(i)), c.obj_bits, ign)
+                for i in range(c.n)]
+        assert list(got) == want
+        wi = np.zeros(c.n, np.uint32)
+        oracle.lib.orc_ref_ctx_counts_from_index(kt._p(c.records), len(c.records), c.n,
+                                                 kt._p(c.index), ign, kt._p(wi))
+        assert list(got) == list(wi)
+
+
+def test_simulator_is_deterministic_and_mutation_scaled(host):
+    specs = [kt.SimSpec(11, 300_000, 0, 0.0, 0.0, 0, 0.0),
+             kt.SimSpec(11, 300_000, 5, 0.01, 0.0, 0, 0.0),
+             kt.SimSpec(11, 300_000, 6, 0.05, 0.01, 0, 0.0)]
+    a, ia = host.simsketch(specs, threads=1)
+    b, ib = host.simsketch(specs, threads=3)
+    assert np.array_equal(a, b) and np.array_equal(ia, ib)
+    # expected ~L/256 entries at -f8 and sorted unique keys per sample
+    n0 = int(ia[1] - ia[0])
+    assert 900 < n0 < 1500
+    for i in range(3):
+        s = a[ia[i]:ia[i + 1]]
+        assert np.all(np.diff(s.astype(np.uint64)) > 0)
+    # shared contexts drop as divergence grows
+    shared = [len(np.intersect1d(a[ia[0]:ia[1]] >> np.uint64(20), a[ia[i]:ia[i + 1]] >> np.uint64(20)))
+              for i in (1, 2)]
+    assert n0 > shared[0] > shared[1] > 0
