@@ -61,6 +61,9 @@ typedef struct kssdhip_index kssdhip_index;
 #define KSSDHIP_LOWER 0x1u          /* only gid < qry_gid (command_ani.h:685-689) */
 #define KSSDHIP_UNIQUE 0x2u         /* also fill unique-owner counters (command_ani.h:564-574) */
 #define KSSDHIP_IGNORECONFLICT 0x4u /* skip reference runs with >1 object (command_ani.h:532) */
+#define KSSDHIP_ACCUMULATE 0x8u     /* _dev only: add into the outputs instead of
+                                       overwriting them (the reference's own
+                                       accumulate-into-zeroed contract) */
 
 /* error codes */
 #define KSSDHIP_EINVAL (-1)
@@ -141,13 +144,52 @@ int kssdhip_pair_features(const uint64_t *a, size_t n, const uint64_t *b,
 			  kssd_ani_features_t *out);
 
 /*
- * Device-resident form of the plain kernel (one launch per call):
- * d_a/d_b device arrays, d_out a device kssd_ani_features_t that is
- * overwritten.  Asynchronous on `stream`.
+ * Device-resident form of the plain kernel: exactly one kernel launch per
+ * call.  d_a/d_b are device arrays, d_out a device kssd_ani_features_t that
+ * is overwritten.  d_workspace is KSSDHIP_PAIR_WORKSPACE_BYTES of device
+ * memory prepared once with kssdhip_pair_workspace_init_dev; every call
+ * leaves it ready for the next one.  Calls sharing a workspace must be
+ * ordered (same stream).  Asynchronous on `stream`.
  */
+#define KSSDHIP_PAIR_WORKSPACE_BYTES 16400u
+int kssdhip_pair_workspace_init_dev(void *d_workspace, void *stream);
 int kssdhip_pair_features_dev(const uint64_t *d_a, size_t n, const uint64_t *d_b,
 			      size_t m, unsigned obj_bits, int ignore_b_conflict,
-			      kssd_ani_features_t *d_out, void *stream);
+			      kssd_ani_features_t *d_out, void *d_workspace, void *stream);
+
+/*
+ * FP64 epilogue on the device: one value per (query, reference) cell of a
+ * dense count block, as the matrix outputs compute it.
+ * Replaces, for the metrics that need no calibration model:
+ *   ani_matrix_pair_value        command_ani.c:1458-1496
+ *   ani_indexed_self_pair_metric command_ani.c:1627-1655
+ *   the matrix branch of ani_block_print_internal command_ani.c:3479-3515
+ * via make_selected_output_row (command_ani.c:1349-1379).
+ * metric is abs(-s): 3 CtxMoE, 4 Naive, 5 MashD, 6 AafD, 7 MashD_if_far,
+ * 8 AafD_if_far, 9 p_dist.  values_ani != 0 writes 1 - distance (s < 0).
+ * float_af != 0 rounds the align fractions to float first, as the
+ * mem_eff block printer does (command_ani.c:3489-3490).
+ * Cells with qry_ctx == 0 or ref_ctx == 0 get the exception value
+ * (e, or 1 - e for ANI).  lower != 0 evaluates only rgid < qry_gid_offset+q
+ * and writes nothing elsewhere.
+ */
+typedef struct {
+	int metric;
+	int values_ani;
+	int unassembled;  /* --qraw without --unified-metric: metrics 1..4 use Naive */
+	int float_af;
+	int lower;
+	int qry_gid_offset;
+	int drfold;       /* ani_model_drfold */
+	unsigned obj_bits; /* Bitslen.obj */
+	unsigned ctx_bits; /* Bitslen.ctx; Mash/AAF use k = ctx_bits / 2 */
+	double afcut;
+	double exception;  /* -e */
+} kssdhip_epilogue_opts_t;
+
+int kssdhip_epilogue_dev(const kssd_ctx_mut2_t *d_ctx, const kssd_obj_section_t *d_obj,
+			 int nq, int nr, const uint32_t *d_qry_ctx, const uint32_t *d_ref_ctx,
+			 const kssdhip_epilogue_opts_t *opts, double *d_out, void *stream);
 
 #ifdef __cplusplus
 }

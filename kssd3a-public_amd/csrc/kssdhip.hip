@@ -267,71 +267,125 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 	return v;
 }
 
-// Plain kernel: merge-join of one pair (command_ani.h:303-346).  One thread
-// per query key; the first key of each context group searches b.
+// Plain kernel: merge-join of one pair (command_ani.h:303-346) in ONE launch.
+// Threads stride over a's keys; the first key of each context group
+// binary-searches b.  Each block writes its partial sums to the workspace;
+// the last block to finish adds them up, writes *out and re-arms the
+// workspace counter, so no separate zeroing launch is needed.
 __global__ __launch_bounds__(256) void k_pair(const uint64_t *__restrict__ A, uint64_t n,
 					      const uint64_t *__restrict__ B, uint64_t m,
 					      unsigned obj_bits, int ignore_b,
-					      kssd_ani_features_t *out)
+					      kssd_ani_features_t *out, PairWorkspace *ws)
 {
 	const uint64_t objmask = obj_bits >= 64 ? ~0ull : ((1ull << obj_bits) - 1ull);
 	uint32_t xny = 0, dobj = 0, dsec = 0, mut2 = 0;
-	const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-	if (i < n) {
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (uint64_t)gridDim.x * blockDim.x) {
 		const uint64_t ca = A[i] >> obj_bits;
-		if (i == 0 || (A[i - 1] >> obj_bits) != ca) {
-			uint64_t ae = i + 1;
-			while (ae < n && (A[ae] >> obj_bits) == ca)
-				++ae;
-			uint64_t lo = 0, hi = m;
-			while (lo < hi) {
-				const uint64_t mid = lo + ((hi - lo) >> 1);
-				if ((B[mid] >> obj_bits) < ca)
-					lo = mid + 1;
-				else
-					hi = mid;
-			}
-			if (lo < m && (B[lo] >> obj_bits) == ca) {
-				uint64_t be = lo + 1;
-				while (be < m && (B[be] >> obj_bits) == ca)
-					++be;
-				if (!(ignore_b && be - lo > 1)) {
-					int mn = MIN_INIT;
-					for (uint64_t x = i; x < ae && mn; ++x) {
-						const uint32_t oa = (uint32_t)(A[x] & objmask);
-						for (uint64_t y = lo; y < be; ++y) {
-							const uint32_t d = oa ^ (uint32_t)(B[y] & objmask);
-							if (d == 0) {
-								mn = 0;
-								break;
-							}
-							const int c = dna_popcount(d);
-							mn = c < mn ? c : mn;
-						}
-					}
-					xny = 1;
-					if (mn > 0) {
-						dobj = 1;
-						dsec = (uint32_t)mn;
-						mut2 = mn > 1;
-					}
+		if (i && (A[i - 1] >> obj_bits) == ca)
+			continue;
+		uint64_t ae = i + 1;
+		while (ae < n && (A[ae] >> obj_bits) == ca)
+			++ae;
+		uint64_t lo = 0, hi = m;
+		while (lo < hi) {
+			const uint64_t mid = lo + ((hi - lo) >> 1);
+			if ((B[mid] >> obj_bits) < ca)
+				lo = mid + 1;
+			else
+				hi = mid;
+		}
+		if (lo >= m || (B[lo] >> obj_bits) != ca)
+			continue;
+		uint64_t be = lo + 1;
+		while (be < m && (B[be] >> obj_bits) == ca)
+			++be;
+		if (ignore_b && be - lo > 1)
+			continue;
+		int mn = MIN_INIT;
+		for (uint64_t x = i; x < ae && mn; ++x) {
+			const uint32_t oa = (uint32_t)(A[x] & objmask);
+			for (uint64_t y = lo; y < be; ++y) {
+				const uint32_t d = oa ^ (uint32_t)(B[y] & objmask);
+				if (d == 0) {
+					mn = 0;
+					break;
 				}
+				const int c = dna_popcount(d);
+				mn = c < mn ? c : mn;
 			}
+		}
+		xny += 1;
+		if (mn > 0) {
+			dobj += 1;
+			dsec += (uint32_t)mn;
+			mut2 += mn > 1;
 		}
 	}
 	xny = wave_sum(xny);
 	dobj = wave_sum(dobj);
 	dsec = wave_sum(dsec);
 	mut2 = wave_sum(mut2);
-	if ((threadIdx.x & 63) == 0 && xny) {
-		atomicAdd(&out->XnY_ctx, xny);
-		if (dobj) {
-			atomicAdd(&out->N_diff_obj, dobj);
-			atomicAdd(&out->N_diff_obj_section, dsec);
+	__shared__ uint4 part[4];
+	__shared__ bool last;
+	const int wave = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0)
+		part[wave] = make_uint4(xny, dobj, dsec, mut2);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint4 t = part[0];
+		for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+			t.x += part[w].x;
+			t.y += part[w].y;
+			t.z += part[w].z;
+			t.w += part[w].w;
 		}
-		if (mut2)
-			atomicAdd(&out->N_mut2_ctx, mut2);
+		ws->partial[blockIdx.x] = t;
+		__threadfence();
+		last = atomicAdd(&ws->done, 1u) == gridDim.x - 1;
 	}
+	__syncthreads();
+	if (!last)
+		return;
+	// last block: reduce the per-block partials
+	__threadfence();
+	uint4 t = make_uint4(0, 0, 0, 0);
+	for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+		const uint4 p = ws->partial[b];
+		t.x += p.x;
+		t.y += p.y;
+		t.z += p.z;
+		t.w += p.w;
+	}
+	t.x = wave_sum(t.x);
+	t.y = wave_sum(t.y);
+	t.z = wave_sum(t.z);
+	t.w = wave_sum(t.w);
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0)
+		part[wave] = t;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint4 r = part[0];
+		for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+			r.x += part[w].x;
+			r.y += part[w].y;
+			r.z += part[w].z;
+			r.w += part[w].w;
+		}
+		out->XnY_ctx = r.x;
+		out->X_ctx = 0;
+		out->N_diff_obj = r.y;
+		out->N_diff_obj_section = r.z;
+		out->N_mut2_ctx = r.w;
+		ws->done = 0;
+	}
+}
+
+__global__ void k_pair_ws_init(PairWorkspace *ws)
+{
+	if (threadIdx.x == 0)
+		ws->done = 0;
 }
 
 static unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 1u << 20)
@@ -551,11 +605,11 @@ extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys
 	const bool unique = (flags & KSSDHIP_UNIQUE) && !(flags & KSSDHIP_LOWER);
 	if (unique && (!d_uctx || !d_uobj))
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: UNIQUE needs unique outputs");
-	if (flags & ~(KSSDHIP_LOWER | KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT))
+	if (flags & ~(KSSDHIP_LOWER | KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT | KSSDHIP_ACCUMULATE))
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: unknown flags 0x%x", flags);
 	hipStream_t s = (hipStream_t)stream;
 	const uint64_t cells = (uint64_t)nq * (uint64_t)ix->ref_gnum;
-	if (cells) {
+	if (cells && !(flags & KSSDHIP_ACCUMULATE)) {
 		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
 				   (unsigned long long *)d_ctx, cells, 0ull);
 		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
@@ -672,22 +726,34 @@ extern "C" int kssdhip_find_first(kssdhip_index *ix, const uint64_t *a, size_t n
 }
 
 // ---------------------------------------------------------------- plain kernel
+static_assert(sizeof(PairWorkspace) <= KSSDHIP_PAIR_WORKSPACE_BYTES, "pair workspace size");
+
+extern "C" int kssdhip_pair_workspace_init_dev(void *d_workspace, void *stream)
+{
+	if (!d_workspace)
+		return kssdhip_fail(KSSDHIP_EINVAL, "pair workspace: null");
+	hipLaunchKernelGGL(k_pair_ws_init, dim3(1), dim3(64), 0, (hipStream_t)stream,
+			   (PairWorkspace *)d_workspace);
+	HIP_TRY(hipGetLastError());
+	return 0;
+}
+
 extern "C" int kssdhip_pair_features_dev(const uint64_t *d_a, size_t n, const uint64_t *d_b, size_t m,
 					 unsigned obj_bits, int ignore_b_conflict,
-					 kssd_ani_features_t *d_out, void *stream)
+					 kssd_ani_features_t *d_out, void *d_workspace, void *stream)
 {
-	if (!d_out || (n && !d_a) || (m && !d_b))
+	if (!d_out || !d_workspace || (n && !d_a) || (m && !d_b))
 		return kssdhip_fail(KSSDHIP_EINVAL, "pair_features: bad arguments");
 	if (obj_bits < 1 || obj_bits > 32)
 		return kssdhip_fail(KSSDHIP_EINVAL, "pair_features: obj_bits %u out of range", obj_bits);
-	hipStream_t s = (hipStream_t)stream;
-	hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)d_out, (uint64_t)5, 0u);
-	if (n && m) {
-		if (n > 0xffffffffull * 256ull)
-			return kssdhip_fail(KSSDHIP_EINVAL, "pair_features: sketch too large");
-		hipLaunchKernelGGL(k_pair, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_a,
-				   (uint64_t)n, d_b, (uint64_t)m, obj_bits, ignore_b_conflict, d_out);
-	}
+	uint64_t g = (n + 255) / 256;
+	if (g < 1)
+		g = 1;
+	if (g > PAIR_MAX_BLOCKS)
+		g = PAIR_MAX_BLOCKS;
+	hipLaunchKernelGGL(k_pair, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, d_a,
+			   (uint64_t)(m ? n : 0), d_b, (uint64_t)m, obj_bits, ignore_b_conflict, d_out,
+			   (PairWorkspace *)d_workspace);
 	HIP_TRY(hipGetLastError());
 	return 0;
 }
@@ -696,7 +762,7 @@ namespace {
 struct PairScratch {
 	int device = -1;
 	hipStream_t stream = nullptr;
-	DevBuf a, b, out;
+	DevBuf a, b, out, ws;
 };
 thread_local PairScratch g_pair;
 } // namespace
@@ -713,6 +779,7 @@ extern "C" int kssdhip_pair_features(const uint64_t *a, size_t n, const uint64_t
 		ps.a.release();
 		ps.b.release();
 		ps.out.release();
+		ps.ws.release();
 		if (ps.stream)
 			(void)hipStreamDestroy(ps.stream);
 		HIP_TRY(hipStreamCreateWithFlags(&ps.stream, hipStreamNonBlocking));
@@ -722,13 +789,18 @@ extern "C" int kssdhip_pair_features(const uint64_t *a, size_t n, const uint64_t
 	if ((rc = ps.a.reserve((n ? n : 1) * 8)) || (rc = ps.b.reserve((m ? m : 1) * 8)) ||
 	    (rc = ps.out.reserve(sizeof(kssd_ani_features_t))))
 		return rc;
+	if (!ps.ws.ptr) {
+		if ((rc = ps.ws.reserve(KSSDHIP_PAIR_WORKSPACE_BYTES)) ||
+		    (rc = kssdhip_pair_workspace_init_dev(ps.ws.ptr, ps.stream)))
+			return rc;
+	}
 	if (n)
 		HIP_TRY(hipMemcpyAsync(ps.a.ptr, a, n * 8, hipMemcpyHostToDevice, ps.stream));
 	if (m)
 		HIP_TRY(hipMemcpyAsync(ps.b.ptr, b, m * 8, hipMemcpyHostToDevice, ps.stream));
 	rc = kssdhip_pair_features_dev((const uint64_t *)ps.a.ptr, n, (const uint64_t *)ps.b.ptr, m,
 				       obj_bits, ignore_b_conflict, (kssd_ani_features_t *)ps.out.ptr,
-				       ps.stream);
+				       ps.ws.ptr, ps.stream);
 	if (rc)
 		return rc;
 	HIP_TRY(hipMemcpyAsync(out, ps.out.ptr, sizeof *out, hipMemcpyDeviceToHost, ps.stream));

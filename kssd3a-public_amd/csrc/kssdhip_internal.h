@@ -30,6 +30,14 @@ struct JoinArgs {
 	unsigned long long *uctx;
 	unsigned long long *uobj;
 };
+// Workspace of the one-launch plain pair kernel: per-block partial sums and
+// the count of finished blocks (left at zero by every launch).
+constexpr unsigned PAIR_MAX_BLOCKS = 1024;
+struct PairWorkspace {
+	unsigned done;
+	unsigned pad[3];
+	uint4 partial[PAIR_MAX_BLOCKS];
+};
 } // namespace kssd
 
 // grow-only device buffer

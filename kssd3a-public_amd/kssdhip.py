@@ -1,0 +1,287 @@
+"""Python bindings (ctypes) of the package's native libraries.
+
+  libkssdhip.so   HIP kernels for gfx950 behind the C ABI of include/kssdhip.h
+  libkssdhost.so  host components: sketcher, genome simulator, index builder
+  libamdhip64.so  the HIP runtime, for device buffers, streams and events
+
+The bindings fail loudly when a library is missing; there is no fallback.
+"""
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+REC_DTYPE = np.dtype([("ctxgid", "<u8"), ("obj", "<u4")], align=False)
+V, S, U, I, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_int, ctypes.c_double
+U32, U64 = ctypes.c_uint32, ctypes.c_uint64
+
+LOWER, UNIQUE, IGNORECONFLICT, ACCUMULATE = 1, 2, 4, 8
+PAIR_WORKSPACE_BYTES = 16400
+
+
+def _p(a):
+    return a.ctypes.data_as(V) if a is not None else None
+
+
+class Features(ctypes.Structure):
+    """kssd_ani_features_t == ani_features_t (model_ani.h:215-222)."""
+    _fields_ = [("XnY_ctx", U32), ("X_ctx", U32), ("N_diff_obj_section", U32),
+                ("N_mut2_ctx", U32), ("N_diff_obj", U32)]
+
+    def tup(self):
+        return (self.XnY_ctx, self.N_diff_obj, self.N_diff_obj_section, self.N_mut2_ctx)
+
+
+class EpilogueOpts(ctypes.Structure):
+    _fields_ = [("metric", I), ("values_ani", I), ("unassembled", I), ("float_af", I),
+                ("lower", I), ("qry_gid_offset", I), ("drfold", I), ("obj_bits", U),
+                ("ctx_bits", U), ("afcut", D), ("exception", D)]
+
+
+class U64Vec(ctypes.Structure):
+    _fields_ = [("a", ctypes.POINTER(U64)), ("n", S), ("cap", S)]
+
+
+class SimSpec(ctypes.Structure):
+    _fields_ = [("anc_seed", U64), ("anc_len", U64), ("mut_seed", U64), ("sub_rate", D),
+                ("indel_frac", D), ("anc2_seed", U64), ("anc2_rate", D)]
+
+
+# ------------------------------------------------------------------ HIP runtime
+class HipError(RuntimeError):
+    pass
+
+
+class Hip:
+    """Minimal HIP runtime access: device memory, streams and events."""
+    H2D, D2H, D2D = 1, 2, 3
+
+    def __init__(self, device=0):
+        self.lib = ctypes.CDLL("libamdhip64.so")
+        L = self.lib
+        L.hipGetErrorString.restype = ctypes.c_char_p
+        L.hipMalloc.argtypes = [ctypes.POINTER(V), S]
+        L.hipFree.argtypes = [V]
+        L.hipMemcpy.argtypes = [V, V, S, I]
+        L.hipMemcpyAsync.argtypes = [V, V, S, I, V]
+        L.hipStreamCreate.argtypes = [ctypes.POINTER(V)]
+        L.hipStreamSynchronize.argtypes = [V]
+        L.hipStreamDestroy.argtypes = [V]
+        L.hipEventCreate.argtypes = [ctypes.POINTER(V)]
+        L.hipEventRecord.argtypes = [V, V]
+        L.hipEventSynchronize.argtypes = [V]
+        L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), V, V]
+        L.hipEventDestroy.argtypes = [V]
+        L.hipSetDevice.argtypes = [I]
+        self.check(L.hipSetDevice(device))
+        self.device = device
+
+    def check(self, rc):
+        if rc != 0:
+            raise HipError(f"HIP error {rc}: {self.lib.hipGetErrorString(rc).decode()}")
+
+    def malloc(self, nbytes):
+        p = V()
+        self.check(self.lib.hipMalloc(ctypes.byref(p), max(int(nbytes), 1)))
+        return p
+
+    def free(self, p):
+        if p:
+            self.check(self.lib.hipFree(p))
+
+    def upload(self, arr, dptr=None):
+        arr = np.ascontiguousarray(arr)
+        if dptr is None:
+            dptr = self.malloc(arr.nbytes)
+        if arr.nbytes:
+            self.check(self.lib.hipMemcpy(dptr, _p(arr), arr.nbytes, self.H2D))
+        return dptr
+
+    def download(self, dptr, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            self.check(self.lib.hipMemcpy(_p(out), dptr, out.nbytes, self.D2H))
+        return out
+
+    def stream(self):
+        s = V()
+        self.check(self.lib.hipStreamCreate(ctypes.byref(s)))
+        return s
+
+    def sync(self, stream):
+        self.check(self.lib.hipStreamSynchronize(stream))
+
+    def event(self):
+        e = V()
+        self.check(self.lib.hipEventCreate(ctypes.byref(e)))
+        return e
+
+    def record(self, ev, stream):
+        self.check(self.lib.hipEventRecord(ev, stream))
+
+    def elapsed_ms(self, a, b):
+        self.check(self.lib.hipEventSynchronize(b))
+        ms = ctypes.c_float()
+        self.check(self.lib.hipEventElapsedTime(ctypes.byref(ms), a, b))
+        return ms.value
+
+
+# ------------------------------------------------------------------ libkssdhip
+def load_kssdhip():
+    so = PKG / "libkssdhip.so"
+    if not so.exists():
+        raise RuntimeError(f"{so} is not built: run `make -C kssd3a-public_amd`")
+    lib = ctypes.CDLL(str(so))
+    lib.kssdhip_last_error.restype = ctypes.c_char_p
+    lib.kssdhip_init.argtypes = [I]
+    lib.kssdhip_index_open.argtypes = [V, S, I, U, ctypes.POINTER(V)]
+    lib.kssdhip_index_close.argtypes = [V]
+    lib.kssdhip_index_device_bytes.argtypes = [V]
+    lib.kssdhip_index_device_bytes.restype = S
+    lib.kssdhip_count_block.argtypes = [V, V, V, I, I, U32, V, V, V, V]
+    lib.kssdhip_count_block_dev.argtypes = [V, V, V, I, U64, I, U32, V, V, V, V, V]
+    lib.kssdhip_find_first.argtypes = [V, V, S, V]
+    lib.kssdhip_pair_features.argtypes = [V, S, V, S, U, I, ctypes.POINTER(Features)]
+    lib.kssdhip_pair_workspace_init_dev.argtypes = [V, V]
+    lib.kssdhip_pair_features_dev.argtypes = [V, S, V, S, U, I, V, V, V]
+    lib.kssdhip_epilogue_dev.argtypes = [V, V, I, I, V, V, ctypes.POINTER(EpilogueOpts), V, V]
+    return lib
+
+
+class KssdHip:
+    """numpy-facing wrapper of the C ABI (include/kssdhip.h)."""
+
+    def __init__(self, device=0):
+        self.lib = load_kssdhip()
+        self.check(self.lib.kssdhip_init(device))
+
+    def check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"kssdhip error {rc}: {self.lib.kssdhip_last_error().decode()}")
+
+    def open_index(self, records, ref_gnum, obj_bits):
+        h = V()
+        records = np.ascontiguousarray(records)
+        self.check(self.lib.kssdhip_index_open(_p(records), len(records), ref_gnum, obj_bits,
+                                               ctypes.byref(h)))
+        return h
+
+    def close_index(self, h):
+        self.check(self.lib.kssdhip_index_close(h))
+
+    def count_block(self, h, ref_gnum, keys, skidx_abs, qry_gid_offset=0, flags=0):
+        nq = len(skidx_abs) - 1
+        ctx = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
+        obj = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
+        uctx = np.zeros_like(ctx)
+        uobj = np.zeros_like(obj)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        skidx_abs = np.ascontiguousarray(skidx_abs, dtype=np.uint64)
+        self.check(self.lib.kssdhip_count_block(h, _p(keys), _p(skidx_abs), nq, qry_gid_offset,
+                                                flags, _p(ctx), _p(obj), _p(uctx), _p(uobj)))
+        return ctx, obj, uctx, uobj
+
+    def count_block_dev(self, h, d_keys, d_skidx, nq, total, qry_gid_offset, flags,
+                        d_ctx, d_obj, d_uctx=None, d_uobj=None, stream=None):
+        self.check(self.lib.kssdhip_count_block_dev(h, d_keys, d_skidx, nq, total, qry_gid_offset,
+                                                    flags, d_ctx, d_obj, d_uctx, d_uobj, stream))
+
+    def find_first(self, h, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        idx = np.zeros(len(keys), dtype=np.uint64)
+        self.check(self.lib.kssdhip_find_first(h, _p(keys), len(keys), _p(idx)))
+        return idx
+
+    def pair(self, a, b, obj_bits, ignore_b_conflict=False):
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        f = Features()
+        self.check(self.lib.kssdhip_pair_features(_p(a), len(a), _p(b), len(b), obj_bits,
+                                                  int(ignore_b_conflict), ctypes.byref(f)))
+        return f.tup()
+
+    def pair_dev(self, d_a, n, d_b, m, obj_bits, ignore, d_out, d_ws, stream=None):
+        self.check(self.lib.kssdhip_pair_features_dev(d_a, n, d_b, m, obj_bits, int(ignore),
+                                                      d_out, d_ws, stream))
+
+    def epilogue_dev(self, d_ctx, d_obj, nq, nr, d_qctx, d_rctx, opts, d_out, stream=None):
+        self.check(self.lib.kssdhip_epilogue_dev(d_ctx, d_obj, nq, nr, d_qctx, d_rctx,
+                                                 ctypes.byref(opts), d_out, stream))
+
+
+# ------------------------------------------------------------------ libkssdhost
+def load_kssdhost():
+    so = PKG / "libkssdhost.so"
+    if not so.exists():
+        raise RuntimeError(f"{so} is not built: run `make -C kssd3a-public_amd`")
+    lib = ctypes.CDLL(str(so))
+    lib.kssd_u64vec_free.argtypes = [ctypes.POINTER(U64Vec)]
+    lib.kssd_u64vec_free.restype = None
+    lib.kssd_sketch_fasta_file.argtypes = [ctypes.c_char_p, I, I, ctypes.POINTER(U64Vec)]
+    lib.kssd_sketching_id.argtypes = [I, I, I, I]
+    lib.kssd_sketching_id.restype = U32
+    lib.kssd_simsketch.argtypes = [V, I, I, I, I, ctypes.POINTER(ctypes.POINTER(U64)), V]
+    lib.kssd_build_index.argtypes = [V, V, I, U, I, V]
+    lib.kssd_sample_ctx_counts.argtypes = [V, V, I, U, I, I, V]
+    lib.kssd_sample_ctx_counts.restype = None
+    lib.kssd_sketch_codes.argtypes = [V, S, I, ctypes.POINTER(U64Vec)]
+    lib.kssd_sketch_codes.restype = None
+    lib.kssd_finalize_keys.argtypes = [ctypes.POINTER(U64Vec), I]
+    lib.kssd_finalize_keys.restype = None
+    return lib
+
+
+def default_threads():
+    n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+
+
+class Host:
+    """numpy-facing wrapper of libkssdhost.so."""
+    _libc = ctypes.CDLL(None)
+
+    def __init__(self):
+        self.lib = load_kssdhost()
+        self._libc.free.argtypes = [V]
+
+    def sketch_file(self, path, drfold=8, keep_conflict=False):
+        v = U64Vec()
+        if self.lib.kssd_sketch_fasta_file(str(path).encode(), drfold, int(keep_conflict),
+                                           ctypes.byref(v)) != 0:
+            raise OSError(f"cannot sketch {path}")
+        out = np.ctypeslib.as_array(v.a, shape=(v.n,)).copy() if v.n else np.zeros(0, np.uint64)
+        self.lib.kssd_u64vec_free(ctypes.byref(v))
+        return out
+
+    def simsketch(self, specs, drfold=8, keep_conflict=False, threads=None):
+        arr = (SimSpec * len(specs))(*specs)
+        keys = ctypes.POINTER(U64)()
+        index = np.zeros(len(specs) + 1, np.uint64)
+        if self.lib.kssd_simsketch(arr, len(specs), drfold, int(keep_conflict),
+                                   int(threads or default_threads()), ctypes.byref(keys),
+                                   _p(index)) != 0:
+            raise RuntimeError("simsketch failed")
+        n = int(index[-1])
+        out = np.ctypeslib.as_array(keys, shape=(n,)).copy() if n else np.zeros(0, np.uint64)
+        self._libc.free(ctypes.cast(keys, V))
+        return out, index
+
+    def build_index(self, keys, index, obj_bits=20, threads=None):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.empty(len(keys), dtype=REC_DTYPE)
+        if self.lib.kssd_build_index(_p(keys), _p(index), len(index) - 1, obj_bits,
+                                     int(threads or default_threads()), _p(out)) != 0:
+            raise ValueError("index limits exceeded")
+        return out
+
+    def ctx_counts(self, keys, index, obj_bits=20, conflict=False, ignoreconflict=False):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.zeros(len(index) - 1, np.uint32)
+        self.lib.kssd_sample_ctx_counts(_p(keys), _p(index), len(index) - 1, obj_bits,
+                                        int(conflict), int(ignoreconflict), _p(out))
+        return out

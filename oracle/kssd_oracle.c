@@ -468,3 +468,38 @@ double orc_aafD(int k, uint32_t X, uint32_t Y, uint32_t XnY)
 	const uint32_t mn = X < Y ? X : Y;
 	return -log((double)XnY / (double)mn) / (double)(uint32_t)k;
 }
+
+void orc_matrix_values(const orc_ctx_t *ctx, const orc_obj_t *obj, int nq, int nr,
+		       const uint32_t *qctx, const uint32_t *rctx, int metric, int values_ani,
+		       double afcut, double e, int drfold, unsigned obj_bits, unsigned ctx_bits,
+		       double *out, int nthreads)
+{
+	/* make_selected_output_row for metrics 3..9 in the matrix outputs
+	 * (command_ani.c:1321-1379, :1458-1496): AF gate, exception cells. */
+	const uint32_t k = ctx_bits / 2;
+	if (nthreads <= 0)
+		nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (long c = 0; c < (long)nq * nr; ++c) {
+		const int q = (int)(c / nr), r = (int)(c % nr);
+		if (qctx[q] == 0 || rctx[r] == 0) {
+			out[c] = values_ani ? 1.0 - e : e;
+			continue;
+		}
+		const orc_features_t f = {ctx[c].num_ctx, 0, obj[c].diff_obj_section, ctx[c].num_mut2_ctx,
+					  obj[c].diff_obj};
+		const double afq = (double)f.XnY_ctx / qctx[q], afr = (double)f.XnY_ctx / rctx[r];
+		const int pass = (afq > afr ? afq : afr) >= afcut;
+		double d;
+		switch (metric) {
+		case 4: d = pass ? orc_naive_dist(&f) : e; break;
+		case 5: d = orc_mashD((int)k, rctx[r], qctx[q], f.XnY_ctx); break;
+		case 6: d = orc_aafD((int)k, rctx[r], qctx[q], f.XnY_ctx); break;
+		case 7: d = pass ? orc_lm3ways_dist(&f, drfold) : orc_mashD((int)k, rctx[r], qctx[q], f.XnY_ctx); break;
+		case 8: d = pass ? orc_lm3ways_dist(&f, drfold) : orc_aafD((int)k, rctx[r], qctx[q], f.XnY_ctx); break;
+		case 9: d = pass ? orc_p_dist(&f, obj_bits) : e; break;
+		default: d = pass ? orc_lm3ways_dist(&f, drfold) : e; break;
+		}
+		out[c] = values_ani ? 1.0 - d : d;
+	}
+}

@@ -100,6 +100,12 @@ double orc_af_anib(const orc_features_t *f, int drfold);             /* model_an
 double orc_mashD(int k, uint32_t ref_ctx, uint32_t qry_ctx, uint32_t xny); /* command_matrix.c:26-29 */
 double orc_aafD(int k, uint32_t ref_ctx, uint32_t qry_ctx, uint32_t xny);  /* command_matrix.c:30-33 */
 
+/* Dense matrix values for metrics 3..9 (command_ani.c:1321-1379, :1458-1496). */
+void orc_matrix_values(const orc_ctx_t *ctx, const orc_obj_t *obj, int nq, int nr,
+		       const uint32_t *qctx, const uint32_t *rctx, int metric, int values_ani,
+		       double afcut, double e, int drfold, unsigned obj_bits, unsigned ctx_bits,
+		       double *out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

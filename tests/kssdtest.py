@@ -14,10 +14,15 @@ import os
 import struct
 from pathlib import Path
 
+import sys
+
 import numpy as np
 
 ROOT = Path(__file__).resolve().parents[1]
 PKG = ROOT / "kssd3a-public_amd"
+sys.path.insert(0, str(PKG))
+from kssdhip import (Features, Host, Hip, KssdHip, EpilogueOpts, SimSpec,  # noqa: E402,F401
+                     load_kssdhip, load_kssdhost)
 GOLDEN = ROOT / "tests" / "golden"
 STAT = struct.Struct("<I??2x6i")
 GID_NBITS = 20
@@ -147,15 +152,6 @@ def brute_counts(q, r, obj_bits, ignore_b_conflict=False):
 
 
 # --------------------------------------------------------------- ctypes bindings
-class Features(ctypes.Structure):
-    _fields_ = [("XnY_ctx", ctypes.c_uint32), ("X_ctx", ctypes.c_uint32),
-                ("N_diff_obj_section", ctypes.c_uint32), ("N_mut2_ctx", ctypes.c_uint32),
-                ("N_diff_obj", ctypes.c_uint32)]
-
-    def tup(self):
-        return (self.XnY_ctx, self.N_diff_obj, self.N_diff_obj_section, self.N_mut2_ctx)
-
-
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
@@ -192,6 +188,8 @@ def load_oracle():
     lib.orc_lm3ways_dist.restype = D
     lib.orc_af_anib.argtypes = [ctypes.POINTER(Features), I]
     lib.orc_af_anib.restype = D
+    lib.orc_matrix_values.argtypes = [V, V, I, I, V, V, I, I, D, D, I, U, U, V, I]
+    lib.orc_matrix_values.restype = None
     for f in ("orc_mashD", "orc_aafD"):
         getattr(lib, f).argtypes = [I, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         getattr(lib, f).restype = D
@@ -238,154 +236,4 @@ class Oracle:
         return idx
 
 
-def load_kssdhip():
-    so = PKG / "libkssdhip.so"
-    if not so.exists():
-        raise RuntimeError(f"{so} not built (run __graft_entry__.build())")
-    lib = ctypes.CDLL(str(so))
-    V, S, U, I = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_int
-    lib.kssdhip_last_error.restype = ctypes.c_char_p
-    lib.kssdhip_init.argtypes = [I]
-    lib.kssdhip_index_open.argtypes = [V, S, I, U, ctypes.POINTER(V)]
-    lib.kssdhip_index_close.argtypes = [V]
-    lib.kssdhip_index_device_bytes.argtypes = [V]
-    lib.kssdhip_index_device_bytes.restype = S
-    lib.kssdhip_count_block.argtypes = [V, V, V, I, I, ctypes.c_uint32, V, V, V, V]
-    lib.kssdhip_count_block_dev.argtypes = [V, V, V, I, ctypes.c_uint64, I, ctypes.c_uint32,
-                                            V, V, V, V, V]
-    lib.kssdhip_find_first.argtypes = [V, V, S, V]
-    lib.kssdhip_pair_features.argtypes = [V, S, V, S, U, I, ctypes.POINTER(Features)]
-    lib.kssdhip_pair_features_dev.argtypes = [V, S, V, S, U, I, V, V]
-    return lib
-
-
-class KssdHip:
-    """numpy-facing wrapper over the product C ABI (include/kssdhip.h)."""
-
-    def __init__(self, device=0):
-        self.lib = load_kssdhip()
-        self._check(self.lib.kssdhip_init(device))
-
-    def _check(self, rc):
-        if rc != 0:
-            raise RuntimeError(f"kssdhip error {rc}: {self.lib.kssdhip_last_error().decode()}")
-
-    def open_index(self, records, ref_gnum, obj_bits):
-        h = ctypes.c_void_p()
-        records = np.ascontiguousarray(records)
-        self._check(self.lib.kssdhip_index_open(_p(records), len(records), ref_gnum, obj_bits,
-                                                ctypes.byref(h)))
-        return h
-
-    def close_index(self, h):
-        self._check(self.lib.kssdhip_index_close(h))
-
-    def count_block(self, h, ref_gnum, keys, skidx_abs, qry_gid_offset=0, flags=0):
-        nq = len(skidx_abs) - 1
-        ctx = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
-        obj = np.zeros((nq, ref_gnum, 2), dtype=np.uint32)
-        uctx = np.zeros_like(ctx)
-        uobj = np.zeros_like(obj)
-        keys = np.ascontiguousarray(keys, dtype=np.uint64)
-        skidx_abs = np.ascontiguousarray(skidx_abs, dtype=np.uint64)
-        self._check(self.lib.kssdhip_count_block(h, _p(keys), _p(skidx_abs), nq, qry_gid_offset,
-                                                 flags, _p(ctx), _p(obj), _p(uctx), _p(uobj)))
-        return ctx, obj, uctx, uobj
-
-    def find_first(self, h, keys):
-        keys = np.ascontiguousarray(keys, dtype=np.uint64)
-        idx = np.zeros(len(keys), dtype=np.uint64)
-        self._check(self.lib.kssdhip_find_first(h, _p(keys), len(keys), _p(idx)))
-        return idx
-
-    def pair(self, a, b, obj_bits, ignore_b_conflict=False):
-        a = np.ascontiguousarray(a, dtype=np.uint64)
-        b = np.ascontiguousarray(b, dtype=np.uint64)
-        f = Features()
-        self._check(self.lib.kssdhip_pair_features(_p(a), len(a), _p(b), len(b), obj_bits,
-                                                   int(ignore_b_conflict), ctypes.byref(f)))
-        return f.tup()
-
-
 FLAGS = {"LOWER": 1, "UNIQUE": 2, "IGNORECONFLICT": 4}
-
-
-class U64Vec(ctypes.Structure):
-    _fields_ = [("a", ctypes.POINTER(ctypes.c_uint64)), ("n", ctypes.c_size_t),
-                ("cap", ctypes.c_size_t)]
-
-
-class SimSpec(ctypes.Structure):
-    _fields_ = [("anc_seed", ctypes.c_uint64), ("anc_len", ctypes.c_uint64),
-                ("mut_seed", ctypes.c_uint64), ("sub_rate", ctypes.c_double),
-                ("indel_frac", ctypes.c_double), ("anc2_seed", ctypes.c_uint64),
-                ("anc2_rate", ctypes.c_double)]
-
-
-def load_kssdhost():
-    so = PKG / "libkssdhost.so"
-    if not so.exists():
-        raise RuntimeError(f"{so} not built (run __graft_entry__.build())")
-    lib = ctypes.CDLL(str(so))
-    V, S, I, U = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint
-    lib.kssd_u64vec_free.argtypes = [ctypes.POINTER(U64Vec)]
-    lib.kssd_u64vec_free.restype = None
-    lib.kssd_sketch_fasta_file.argtypes = [ctypes.c_char_p, I, I, ctypes.POINTER(U64Vec)]
-    lib.kssd_sketching_id.argtypes = [I, I, I, I]
-    lib.kssd_sketching_id.restype = ctypes.c_uint32
-    lib.kssd_simsketch.argtypes = [V, I, I, I, I, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint64)), V]
-    lib.kssd_build_index.argtypes = [V, V, I, U, I, V]
-    lib.kssd_sample_ctx_counts.argtypes = [V, V, I, U, I, I, V]
-    lib.kssd_sample_ctx_counts.restype = None
-    lib.kssd_sketch_codes.argtypes = [V, S, I, ctypes.POINTER(U64Vec)]
-    lib.kssd_sketch_codes.restype = None
-    lib.kssd_finalize_keys.argtypes = [ctypes.POINTER(U64Vec), I]
-    lib.kssd_finalize_keys.restype = None
-    return lib
-
-
-class Host:
-    """numpy-facing wrapper over libkssdhost.so (sketcher, simulator, index)."""
-    _libc = ctypes.CDLL(None)
-
-    def __init__(self):
-        self.lib = load_kssdhost()
-
-    def sketch_file(self, path, drfold=8, keep_conflict=False):
-        v = U64Vec()
-        if self.lib.kssd_sketch_fasta_file(str(path).encode(), drfold, int(keep_conflict),
-                                           ctypes.byref(v)) != 0:
-            raise OSError(f"cannot sketch {path}")
-        out = np.ctypeslib.as_array(v.a, shape=(v.n,)).copy() if v.n else np.zeros(0, np.uint64)
-        self.lib.kssd_u64vec_free(ctypes.byref(v))
-        return out
-
-    def simsketch(self, specs, drfold=8, keep_conflict=False, threads=os.cpu_count() or 1):
-        arr = (SimSpec * len(specs))(*specs)
-        keys = ctypes.POINTER(ctypes.c_uint64)()
-        index = np.zeros(len(specs) + 1, np.uint64)
-        if self.lib.kssd_simsketch(arr, len(specs), drfold, int(keep_conflict), int(threads),
-                                   ctypes.byref(keys), _p(index)) != 0:
-            raise RuntimeError("simsketch failed")
-        n = int(index[-1])
-        out = np.ctypeslib.as_array(keys, shape=(n,)).copy() if n else np.zeros(0, np.uint64)
-        self._libc.free.argtypes = [ctypes.c_void_p]
-        self._libc.free(ctypes.cast(keys, ctypes.c_void_p))
-        return out, index
-
-    def build_index(self, keys, index, obj_bits=20, threads=os.cpu_count() or 1):
-        keys = np.ascontiguousarray(keys, dtype=np.uint64)
-        index = np.ascontiguousarray(index, dtype=np.uint64)
-        out = np.empty(len(keys), dtype=REC_DTYPE)
-        if self.lib.kssd_build_index(_p(keys), _p(index), len(index) - 1, obj_bits, int(threads),
-                                     _p(out)) != 0:
-            raise ValueError("index limits exceeded")
-        return out
-
-    def ctx_counts(self, keys, index, obj_bits=20, conflict=False, ignoreconflict=False):
-        keys = np.ascontiguousarray(keys, dtype=np.uint64)
-        index = np.ascontiguousarray(index, dtype=np.uint64)
-        out = np.zeros(len(index) - 1, np.uint32)
-        self.lib.kssd_sample_ctx_counts(_p(keys), _p(index), len(index) - 1, obj_bits,
-                                        int(conflict), int(ignoreconflict), _p(out))
-        return out
