@@ -94,6 +94,28 @@ int kssdhip_index_close(kssdhip_index *index);
 size_t kssdhip_index_device_bytes(const kssdhip_index *index);
 
 /*
+ * Kernel shape of the counting path for a handle.  No reference counterpart
+ * (the reference's block loop has no device tiling); the project's
+ * autotuner sets it.
+ *   mode 0 auto: partitioned merge-join when the block carries at least
+ *          auto_min_keys query keys per index partition, lookup otherwise;
+ *        1 lookup: one thread per query key, bucket-table lookup into HBM;
+ *        2 partitioned: one workgroup per index partition of about
+ *          chunk_records records staged in LDS; queries are split at the
+ *          partition boundaries and merged against the staged records.
+ * chunk_records (partitions are rebuilt when it changes) must leave the
+ * staged partition within the 160 KB LDS: 64 <= chunk_records <= 12288.
+ */
+typedef struct {
+	int mode;
+	int chunk_records;
+	int block_threads; /* 64..1024, multiple of 64 */
+	double auto_min_keys;
+} kssdhip_tiling_t;
+int kssdhip_index_set_tiling(kssdhip_index *index, const kssdhip_tiling_t *tiling);
+int kssdhip_index_get_tiling(const kssdhip_index *index, kssdhip_tiling_t *tiling);
+
+/*
  * Count one block of queries against the index.
  * Replaces the counting part of
  *   count_ctx_obj_frm_comb_sketch_section_with_unique (command_ani.h:463-575)

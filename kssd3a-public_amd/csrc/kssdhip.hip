@@ -59,13 +59,7 @@ extern "C" int kssdhip_init(int device)
 // ---------------------------------------------------------------- device helpers
 namespace kssd {
 
-__device__ __forceinline__ int dna_popcount(uint32_t x)
-{
-	// dna_popcount.h:45-49: one bit per differing 2-bit base lane
-	x |= x >> 1;
-	x &= 0x55555555u;
-	return __popc(x);
-}
+__device__ __forceinline__ int dna_popcount(uint32_t x) { return dna_popcount_dev(x); }
 
 __global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v)
 {
@@ -403,21 +397,6 @@ static unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 1u << 
 using namespace kssd;
 
 // ---------------------------------------------------------------- index handle
-struct kssdhip_index {
-	int device = 0;
-	hipStream_t stream = nullptr;
-	uint64_t n = 0;
-	int ref_gnum = 0;
-	unsigned obj_bits = 0;
-	uint64_t *d_ctxgid = nullptr;
-	uint32_t *d_obj = nullptr;
-	uint32_t *d_bucket = nullptr;
-	int k = 0, shift = 0;
-	uint64_t ctx_max = 0;
-	// grow-only scratch for the host-pointer entry points
-	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx;
-};
-
 int DevBuf::reserve(size_t want)
 {
 	if (want <= bytes)
@@ -531,6 +510,11 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 			rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: bucket build failed");
 			break;
 		}
+		ix->mode = 0;
+		ix->block = 256;
+		ix->auto_min_keys = 512.0;
+		if ((rc = kssd_build_partitions(ix, 4096)))
+			break;
 	} while (0);
 	if (d_raw)
 		(void)hipFree(d_raw);
@@ -555,7 +539,8 @@ extern "C" int kssdhip_index_close(kssdhip_index *ix)
 		(void)hipFree(ix->d_obj);
 	if (ix->d_bucket)
 		(void)hipFree(ix->d_bucket);
-	DevBuf *bufs[] = {&ix->keys, &ix->skidx, &ix->ctx, &ix->obj, &ix->uctx, &ix->uobj, &ix->idx};
+	kssd_free_partitions(ix);
+	DevBuf *bufs[] = {&ix->keys, &ix->skidx, &ix->ctx, &ix->obj, &ix->uctx, &ix->uobj, &ix->idx, &ix->split};
 	for (DevBuf *b : bufs)
 		b->release();
 	if (ix->stream)
@@ -570,7 +555,7 @@ extern "C" size_t kssdhip_index_device_bytes(const kssdhip_index *ix)
 		return 0;
 	size_t b = (size_t)ix->n * 12 + ((((size_t)1) << ix->k) + 1) * 4;
 	b += ix->keys.bytes + ix->skidx.bytes + ix->ctx.bytes + ix->obj.bytes + ix->uctx.bytes +
-	     ix->uobj.bytes + ix->idx.bytes;
+	     ix->uobj.bytes + ix->idx.bytes + ix->split.bytes + ((size_t)ix->P + 1) * 12;
 	return b;
 }
 
@@ -641,7 +626,12 @@ extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys
 		a.objc = (unsigned long long *)d_obj;
 		a.uctx = (unsigned long long *)d_uctx;
 		a.uobj = (unsigned long long *)d_uobj;
-		hipLaunchKernelGGL(k_join, dim3(grid_for(total)), dim3(256), 0, s, a);
+		bool used = false;
+		int rc = kssd_launch_pjoin(ix, a, s, &used);
+		if (rc)
+			return rc;
+		if (!used)
+			hipLaunchKernelGGL(k_join, dim3(grid_for(total)), dim3(256), 0, s, a);
 	}
 	HIP_TRY(hipGetLastError());
 	return 0;

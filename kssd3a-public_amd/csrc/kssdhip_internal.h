@@ -10,6 +10,14 @@ constexpr int GID_NBITS = 20;                         // sketch_rearrange.h:7
 constexpr uint64_t GID_MASK = (1ull << GID_NBITS) - 1ull;
 constexpr int MIN_INIT = 12;                          // NUM_CODENS + 1, command_ani.h:535
 
+// dna_popcount.h:45-49: one bit per differing 2-bit base lane
+__device__ __forceinline__ int dna_popcount_dev(uint32_t x)
+{
+	x |= x >> 1;
+	x &= 0x55555555u;
+	return __popc(x);
+}
+
 struct JoinArgs {
 	const uint64_t *ctxgid;
 	const uint32_t *robj;
@@ -47,6 +55,40 @@ struct DevBuf {
 	int reserve(size_t want);
 	void release();
 };
+
+// Resident index handle (opaque in the C ABI).
+struct kssdhip_index {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	uint64_t n = 0;
+	int ref_gnum = 0;
+	unsigned obj_bits = 0;
+	uint64_t *d_ctxgid = nullptr;
+	uint32_t *d_obj = nullptr;
+	uint32_t *d_bucket = nullptr;
+	int k = 0, shift = 0;
+	uint64_t ctx_max = 0;
+	// partitions of the sorted index for the partitioned merge-join:
+	// partition p = records [pstart[p], pstart[p+1]), pctx[p] = its first
+	// context, pctx[P] = ~0; boundaries never split a context's records.
+	uint32_t P = 0;
+	uint32_t max_part = 0;
+	uint32_t *d_pstart = nullptr;
+	uint64_t *d_pctx = nullptr;
+	// tiling (kssdhip_tiling_t)
+	int mode = 0;
+	int chunk = 0;
+	int block = 256;
+	double auto_min_keys = 0;
+	// grow-only scratch: host-pointer entry points and the split table
+	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx, split;
+};
+
+// partitioned merge-join (kssdhip_pjoin.hip)
+int kssd_build_partitions(kssdhip_index *ix, int chunk);
+void kssd_free_partitions(kssdhip_index *ix);
+// true when the partitioned path was launched for this call
+int kssd_launch_pjoin(kssdhip_index *ix, const kssd::JoinArgs &a, hipStream_t s, bool *used);
 
 // read-only view of an index handle for other translation units
 struct IndexView {

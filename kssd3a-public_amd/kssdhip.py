@@ -41,6 +41,11 @@ class EpilogueOpts(ctypes.Structure):
                 ("ctx_bits", U), ("afcut", D), ("exception", D)]
 
 
+class Tiling(ctypes.Structure):
+    """kssdhip_tiling_t: mode 0 auto, 1 lookup, 2 partitioned merge-join."""
+    _fields_ = [("mode", I), ("chunk_records", I), ("block_threads", I), ("auto_min_keys", D)]
+
+
 class U64Vec(ctypes.Structure):
     _fields_ = [("a", ctypes.POINTER(U64)), ("n", S), ("cap", S)]
 
@@ -147,6 +152,8 @@ def load_kssdhip():
     lib.kssdhip_pair_features.argtypes = [V, S, V, S, U, I, ctypes.POINTER(Features)]
     lib.kssdhip_pair_workspace_init_dev.argtypes = [V, V]
     lib.kssdhip_pair_features_dev.argtypes = [V, S, V, S, U, I, V, V, V]
+    lib.kssdhip_index_set_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
+    lib.kssdhip_index_get_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
     lib.kssdhip_epilogue_dev.argtypes = [V, V, I, I, V, V, ctypes.POINTER(EpilogueOpts), V, V]
     return lib
 
@@ -168,6 +175,23 @@ class KssdHip:
         self.check(self.lib.kssdhip_index_open(_p(records), len(records), ref_gnum, obj_bits,
                                                ctypes.byref(h)))
         return h
+
+    def set_tiling(self, h, mode=None, chunk_records=None, block_threads=None, auto_min_keys=None):
+        t = self.get_tiling(h)
+        if mode is not None:
+            t.mode = mode
+        if chunk_records is not None:
+            t.chunk_records = chunk_records
+        if block_threads is not None:
+            t.block_threads = block_threads
+        if auto_min_keys is not None:
+            t.auto_min_keys = auto_min_keys
+        self.check(self.lib.kssdhip_index_set_tiling(h, ctypes.byref(t)))
+
+    def get_tiling(self, h):
+        t = Tiling()
+        self.check(self.lib.kssdhip_index_get_tiling(h, ctypes.byref(t)))
+        return t
 
     def close_index(self, h):
         self.check(self.lib.kssdhip_index_close(h))

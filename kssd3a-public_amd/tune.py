@@ -1,0 +1,102 @@
+"""Autotuner for the counting path's kernel shape (kssdhip_tiling_t).
+
+For a resident index and a device-resident query block it times each
+candidate tiling with HIP events on the caller's stream -- the lookup
+kernel and the partitioned merge-join over a grid of LDS chunk sizes and
+workgroup sizes -- and keeps the fastest.  Results are cached in a JSON
+file keyed by the device name and the shape of the problem (index records,
+query keys, queries, references, flags), so a serving process or a later
+run picks the shape up without re-measuring.
+"""
+import ctypes
+import json
+import os
+from pathlib import Path
+
+import kssdhip as kh
+
+DEFAULT_CACHE = Path(os.environ.get("KSSDHIP_TUNE_CACHE",
+                                    Path.home() / ".cache" / "kssdhip" / "tuning.json"))
+CHUNKS = (1024, 2048, 4096, 8192, 12288)
+BLOCKS = (128, 256, 512)
+
+
+def _bucket(x):
+    """Shape keys are rounded to a power of two so nearby sizes share an entry."""
+    x = int(x)
+    return 0 if x <= 0 else 1 << (x.bit_length() - 1)
+
+
+def device_name(rt):
+    try:
+        rt.lib.hipDeviceGetName.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        buf = ctypes.create_string_buffer(256)
+        rt.check(rt.lib.hipDeviceGetName(buf, 256, rt.device))
+        return buf.value.decode() or "hip"
+    except Exception:
+        return "hip"
+
+
+def shape_key(dev, n_index, total_keys, nq, nref, flags):
+    return f"{dev}|N{_bucket(n_index)}|K{_bucket(total_keys)}|Q{_bucket(nq)}|R{_bucket(nref)}|F{flags}"
+
+
+def load_cache(path=DEFAULT_CACHE):
+    try:
+        return json.loads(Path(path).read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(cache, path=DEFAULT_CACHE):
+    try:
+        Path(path).parent.mkdir(parents=True, exist_ok=True)
+        tmp = Path(str(path) + ".tmp")
+        tmp.write_text(json.dumps(cache, indent=1, sort_keys=True))
+        tmp.replace(path)
+    except OSError:
+        pass
+
+
+def candidates():
+    yield {"mode": 1, "chunk_records": 4096, "block_threads": 256}
+    for c in CHUNKS:
+        for b in BLOCKS:
+            yield {"mode": 2, "chunk_records": c, "block_threads": b}
+
+
+def tune(hip, rt, h, d_keys, d_skidx, nq, total, nref, n_index, d_ctx, d_obj, stream,
+         flags=0, reps=3, cache_path=DEFAULT_CACHE, force=False, log=None):
+    """Pick and apply the fastest tiling for this block shape.  Returns
+    (tiling dict, {candidate: ms}).  The outputs d_ctx/d_obj are scratch."""
+    dev = device_name(rt)
+    key = shape_key(dev, n_index, total, nq, nref, flags)
+    cache = load_cache(cache_path)
+    if not force and key in cache:
+        best = cache[key]["best"]
+        hip.set_tiling(h, mode=best["mode"], chunk_records=best["chunk_records"],
+                       block_threads=best["block_threads"])
+        return best, cache[key].get("times", {})
+    e0, e1 = rt.event(), rt.event()
+    times = {}
+    best, best_ms = None, float("inf")
+    for cand in candidates():
+        hip.set_tiling(h, **cand)
+        hip.count_block_dev(h, d_keys, d_skidx, nq, total, 0, flags, d_ctx, d_obj, None, None, stream)
+        rt.sync(stream)
+        rt.record(e0, stream)
+        for _ in range(reps):
+            hip.count_block_dev(h, d_keys, d_skidx, nq, total, 0, flags, d_ctx, d_obj, None, None,
+                                stream)
+        rt.record(e1, stream)
+        ms = rt.elapsed_ms(e0, e1) / reps
+        name = f"m{cand['mode']}_c{cand['chunk_records']}_b{cand['block_threads']}"
+        times[name] = ms
+        if log:
+            log(f"tune {name}: {ms:.3f} ms")
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    hip.set_tiling(h, **best)
+    cache[key] = {"best": best, "ms": best_ms, "times": times}
+    save_cache(cache, cache_path)
+    return best, times

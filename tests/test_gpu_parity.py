@@ -162,3 +162,54 @@ def test_index_open_rejects_bad_index(hip):
         hip.open_index(rec[::-1].copy(), 4, OBJ_BITS)
     with pytest.raises(RuntimeError, match="gid"):
         hip.open_index(rec, 2, OBJ_BITS)
+
+
+TILINGS = [(1, 4096, 256), (2, 64, 256), (2, 128, 64), (2, 4096, 256), (2, 12288, 1024)]
+
+
+@pytest.mark.parametrize("mode,chunk,block", TILINGS)
+@pytest.mark.parametrize("flags", [0, 2, 4, 6])
+def test_tilings_match_oracle(hip, oracle, mode, chunk, block, flags):
+    """Lookup and partitioned paths, small partitions (many boundaries, runs
+    longer than a partition) and the largest LDS stage."""
+    rng = np.random.default_rng(chunk + flags + mode)
+    pool = rng.integers(1, 1 << 43, size=3000, dtype=np.uint64)
+    refs = kt.random_samples(rng, 60, 3000, 300, shared_pool=pool, conflict_p=0.15)
+    # one context carried by every reference with two objects: a run of 120
+    # records, longer than the 64/128-record partitions plus their slack
+    hot = np.uint64(int(pool[7]) << 20)
+    refs = [np.unique(np.concatenate([r, [hot | np.uint64(5), hot | np.uint64(9)]])) for r in refs]
+    qrys = kt.random_samples(rng, 33, 3000, 400, shared_pool=pool, conflict_p=0.15)
+    qrys[4] = np.zeros(0, np.uint64)
+    rkeys, ridx = _block(refs)
+    rec = kt.build_index_np(rkeys, ridx, OBJ_BITS)
+    qkeys, qidx = _block(qrys)
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=mode, chunk_records=chunk, block_threads=block)
+        got = hip.count_block(h, len(refs), qkeys, qidx, flags=flags)
+        low = hip.count_block(h, len(refs), rkeys[ridx[10]:], ridx[10:], qry_gid_offset=10,
+                              flags=1 | (flags & 4))
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, len(refs), OBJ_BITS, qkeys, qidx, flags=flags)
+    for name, g, w in zip(("ctx", "obj", "uctx", "uobj"), got, want):
+        assert np.array_equal(g, w), (mode, chunk, flags, name)
+    wl = oracle.count_block(rec, len(refs), OBJ_BITS, rkeys[ridx[10]:], ridx[10:], qry_gid_offset=10,
+                            flags=1 | (flags & 4))
+    assert np.array_equal(low[0], wl[0]) and np.array_equal(low[1], wl[1])
+
+
+def test_tiling_validation(hip):
+    refs, _ = _fixture(3, n_ref=4)
+    keys, idx = _block(refs)
+    h = hip.open_index(kt.build_index_np(keys, idx, OBJ_BITS), 4, OBJ_BITS)
+    try:
+        t = hip.get_tiling(h)
+        assert (t.mode, t.chunk_records, t.block_threads) == (0, 4096, 256)
+        for bad in (dict(mode=3), dict(chunk_records=10), dict(chunk_records=20000),
+                    dict(block_threads=100)):
+            with pytest.raises(RuntimeError):
+                hip.set_tiling(h, **bad)
+    finally:
+        hip.close_index(h)
