@@ -110,6 +110,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all)")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--latency-queries", type=int, default=64)
+    ap.add_argument("--copy-probe-mib", type=int, default=256,
+                    help="size of a device-to-device copy run once before the timed region; its "
+                         "HBM counters calibrate the join kernel's (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path shards "
@@ -155,6 +159,12 @@ def main():
     tiling, tune_ms = tune.tune(hip, rt, h, d_qkeys, d_qidx, nqry, total, nref, len(records), d_ctx, d_obj,
                                 stream, flags=0, force=args.retune, log=log)
     log(f"autotuned tiling {tiling} in {time.time() - t:.1f}s")
+    if args.copy_probe_mib:
+        nb = args.copy_probe_mib << 20
+        a_, b_ = rt.malloc(nb), rt.malloc(nb)
+        rt.check(rt.lib.hipMemcpy(b_, a_, nb, rt.D2D))
+        rt.free(a_)
+        rt.free(b_)
     for _ in range(args.warmup):
         step()
     rt.sync(stream)
@@ -185,7 +195,7 @@ def main():
     # auto mode sends such a small block to the lookup kernel
     hip.set_tiling(h, mode=0)
     lat = []
-    sample_q = np.random.default_rng(1).choice(nqry, size=min(64, nqry), replace=False)
+    sample_q = np.random.default_rng(1).choice(nqry, size=min(args.latency_queries, nqry), replace=False)
     ea, eb = rt.event(), rt.event()
     for q in sample_q:
         nk = int(qidx[q + 1] - qidx[q])
