@@ -1,0 +1,848 @@
+/*
+ * Host epilogue: rows, ranking, matrices and TSV formatting.  See
+ * kssd_report.h for the reference functions each part follows.
+ */
+#define _GNU_SOURCE
+#include "kssd_report.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "kssd_models.h"
+
+/* ------------------------------------------------------------ buffers */
+int kssd_buf_printf(kssd_buf_t *b, const char *fmt, ...)
+{
+	for (;;) {
+		va_list ap;
+		va_start(ap, fmt);
+		const size_t room = b->m - b->l;
+		const int n = vsnprintf(b->s ? b->s + b->l : NULL, room, fmt, ap);
+		va_end(ap);
+		if (n < 0)
+			return -1;
+		if ((size_t)n < room) {
+			b->l += (size_t)n;
+			return 0;
+		}
+		size_t m = b->m ? b->m : 256;
+		while (m - b->l <= (size_t)n)
+			m <<= 1;
+		char *s = realloc(b->s, m);
+		if (!s)
+			return -1;
+		b->s = s;
+		b->m = m;
+	}
+}
+
+void kssd_buf_free(kssd_buf_t *b)
+{
+	free(b->s);
+	b->s = NULL;
+	b->l = b->m = 0;
+}
+
+/* ------------------------------------------------------------ calibrated model */
+extern const unsigned char kssd_refaf_blob[];
+extern const unsigned char kssd_refaf_blob_end[];
+
+typedef struct {
+	unsigned n_features, n_trees, n_nodes;
+	double baseline;
+	const uint32_t *offsets;
+	const double *threshold, *value;
+	const uint16_t *left, *right;
+	const int8_t *feature;
+	const uint8_t *is_leaf, *missing_left;
+} refaf_model_t;
+
+static refaf_model_t g_model;
+static int g_model_ok = -1;
+
+static int refaf_load(void)
+{
+	if (g_model_ok >= 0)
+		return g_model_ok;
+	const unsigned char *p = kssd_refaf_blob;
+	const size_t size = (size_t)(kssd_refaf_blob_end - kssd_refaf_blob);
+	g_model_ok = 0;
+	if (size < 32 || memcmp(p, "KSSDHGB1", 8) != 0)
+		return 0;
+	uint32_t hdr[4];
+	memcpy(hdr, p + 8, sizeof hdr);
+	refaf_model_t m;
+	m.n_features = hdr[0];
+	m.n_trees = hdr[1];
+	m.n_nodes = hdr[2];
+	memcpy(&m.baseline, p + 24, 8);
+	size_t off = 32;
+	const size_t need = off + 4 * ((size_t)m.n_trees + 1) + (size_t)m.n_nodes * (8 + 8 + 2 + 2 + 1 + 1 + 1);
+	if (m.n_features != 11 || size < need)
+		return 0;
+	/* the blob is 8-byte aligned and laid out so every array is naturally aligned */
+	m.offsets = (const uint32_t *)(p + off);
+	off += 4 * ((size_t)m.n_trees + 1);
+	off = (off + 7) & ~(size_t)7;
+	m.threshold = (const double *)(p + off);
+	off += 8 * (size_t)m.n_nodes;
+	m.value = (const double *)(p + off);
+	off += 8 * (size_t)m.n_nodes;
+	m.left = (const uint16_t *)(p + off);
+	off += 2 * (size_t)m.n_nodes;
+	m.right = (const uint16_t *)(p + off);
+	off += 2 * (size_t)m.n_nodes;
+	m.feature = (const int8_t *)(p + off);
+	off += m.n_nodes;
+	m.is_leaf = (const uint8_t *)(p + off);
+	off += m.n_nodes;
+	m.missing_left = (const uint8_t *)(p + off);
+	g_model = m;
+	g_model_ok = 1;
+	return 1;
+}
+
+static inline double clamp01(double x)
+{
+	return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+}
+
+double kssd_refaf_predict_ani(double raw_ani, double ref_af, unsigned xny, unsigned n_diff_obj,
+			      unsigned n_diff_obj_section, unsigned n_mut2_ctx, int drfold)
+{
+	/* model_refaf_hgb.h:7593-7628 */
+	if (xny == 0)
+		return raw_ani;
+	if (!refaf_load())
+		return raw_ani;
+	raw_ani = clamp01(raw_ani);
+	if (ref_af < 1e-12)
+		ref_af = 1e-12;
+	if (ref_af > 1.0)
+		ref_af = 1.0;
+	const double xny_d = (double)xny;
+	const double ref_aaf_ani = clamp01(1.0 + log(ref_af) / 11.0);
+	double x[11];
+	x[0] = raw_ani;
+	x[1] = ref_af;
+	x[2] = log1p(xny_d * kssd_fold_scale(drfold));
+	x[3] = (double)n_diff_obj / xny_d;
+	x[4] = (double)n_diff_obj_section / xny_d;
+	x[5] = (double)n_mut2_ctx / xny_d;
+	x[6] = ref_aaf_ani;
+	x[7] = raw_ani - ref_aaf_ani;
+	x[8] = ref_af < 0.2 ? 1.0 : 0.0;
+	x[9] = (ref_af >= 0.2 && ref_af < 0.5) ? 1.0 : 0.0;
+	x[10] = ref_af >= 0.5 ? 1.0 : 0.0;
+	/* model_refaf_hgb.h:7575-7591 */
+	const refaf_model_t *m = &g_model;
+	double pred = m->baseline;
+	for (unsigned t = 0; t < m->n_trees; ++t) {
+		const unsigned begin = m->offsets[t];
+		unsigned node = begin;
+		while (!m->is_leaf[node]) {
+			const double v = x[m->feature[node]];
+			const int go_left = isnan(v) ? m->missing_left[node] : (v <= m->threshold[node]);
+			node = begin + (go_left ? m->left[node] : m->right[node]);
+		}
+		pred += m->value[node];
+	}
+	return clamp01(pred);
+}
+
+/* ------------------------------------------------------------ rows */
+enum { CONF_LOW = 0, CONF_MEDIUM = 1, CONF_HIGH = 2 };
+
+static const char *const metric_names[9] = {"BestDist", "RecalDist", "CtxMoE", "Naive", "MashD",
+					    "AafD", "MashD_if_far", "AafD_if_far", "p_dist"};
+
+const char kssd_detail_header[] =
+	"Qry\tRef\tANI\tDistance\tConfidence\tSelected_metric\tXnY_ctx\tQry_align_fraction\t"
+	"blastn_Qry_align_fraction\tRef_align_fraction\tblastn_Ref_align_fraction\tN_diff_obj\t"
+	"N_diff_obj_section\tN_mut2_ctx\tRef_annotation";
+
+static inline bool af_pass(const kssd_report_opts_t *o, double afq, double afr)
+{
+	/* ani_report_af_pass, command_ani.h:70-78 */
+	return (afq > afr ? afq : afr) >= (double)o->afcut;
+}
+
+static inline uint32_t af_needed_ctx(const kssd_report_opts_t *o, uint32_t qc, uint32_t rc)
+{
+	/* ani_report_af_needed_ctx, command_ani.h:80-90 */
+	const uint32_t denom = qc < rc ? qc : rc;
+	const double need = (double)o->afcut * (double)denom;
+	if (need <= 0.0)
+		return 0;
+	if (need >= (double)UINT32_MAX)
+		return UINT32_MAX;
+	return (uint32_t)ceil(need);
+}
+
+static inline unsigned char confidence_from(double raw, double cal, double ref_af)
+{
+	/* ani_confidence_from_values, command_ani.c:100-108 */
+	const double drop = raw - cal;
+	if (ref_af < 0.5 || (raw >= 0.95 && drop > 0.02))
+		return CONF_LOW;
+	if (fabs(drop) > 0.01)
+		return CONF_MEDIUM;
+	return CONF_HIGH;
+}
+
+static inline double bounded_ani(double v)
+{
+	if (!isfinite(v))
+		return NAN;
+	return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+}
+
+static inline double ctx_k(const kssd_report_opts_t *o)
+{
+	return o->ctx_bits > 0 ? (double)o->ctx_bits / 2.0 : 22.0;
+}
+
+static double mash_ani(const kssd_report_opts_t *o, double overlap, double q, double r)
+{
+	/* mash_ani_from_counts, command_ani.c:164-175 */
+	if (overlap <= 0.0 || q <= 0.0 || r <= 0.0)
+		return 0.0;
+	const double denom = q + r - overlap;
+	if (denom <= 0.0)
+		return 0.0;
+	const double j = overlap / denom;
+	if (j <= 0.0)
+		return 0.0;
+	return bounded_ani(1.0 + log(2.0 * j / (1.0 + j)) / ctx_k(o));
+}
+
+static double aaf_ani(const kssd_report_opts_t *o, double overlap, double q, double r)
+{
+	/* aaf_ani_from_counts, command_ani.c:177-188 */
+	if (overlap <= 0.0 || q <= 0.0 || r <= 0.0)
+		return 0.0;
+	const double mn = q < r ? q : r;
+	if (mn <= 0.0)
+		return 0.0;
+	const double c = overlap / mn;
+	if (c <= 0.0)
+		return 0.0;
+	return bounded_ani(1.0 + log(c) / ctx_k(o));
+}
+
+static bool meta_like_assembly(const kssd_infile_meta_t *s)
+{
+	/* infile_meta_complete_like_assembly, command_ani.c:118-128 */
+	if (!s || s->meta_fmt_version != 1 || s->infile_fmt != 1 || s->total_length_bp == 0 ||
+	    s->median_length_bp == 0)
+		return false;
+	return s->total_length_bp >= 3500000ULL && s->record_count < 5 && s->median_length_bp > 100000U &&
+	       (double)s->total_length_bp / (double)s->median_length_bp < 5.0;
+}
+
+static bool meta_has_assembly(const kssd_infile_meta_t *s)
+{
+	return s && s->meta_fmt_version == 1 && s->infile_fmt == 1 && s->total_length_bp > 0 &&
+	       s->record_count > 0;
+}
+
+static inline bool best_guard_enabled(const kssd_report_opts_t *o)
+{
+	return o->fmt == 0 && !o->unassembled && !o->v;
+}
+
+static void fill_calibration(kssd_row_t *r, const kssd_report_opts_t *o, const kssd_infile_meta_t *qa)
+{
+	/* fill_row_calibration, command_ani.c:1129-1184 */
+	if (o->unassembled) {
+		r->calibrated_ani = r->ani;
+		r->best_ani = r->ani;
+		r->confidence = CONF_HIGH;
+		r->best_guarded = 0;
+		return;
+	}
+	if (r->XnY_ctx > 0 && r->N_diff_obj == 0 && r->N_diff_obj_section == 0 && r->N_mut2_ctx == 0 &&
+	    r->af_qry == 1.0 && r->af_ref == 1.0) {
+		r->calibrated_ani = 1.0;
+		r->best_ani = 1.0;
+		r->confidence = CONF_HIGH;
+		r->best_guarded = 0;
+		return;
+	}
+	r->calibrated_ani = kssd_refaf_predict_ani(r->ani, r->af_ref, (unsigned)r->XnY_ctx,
+						   (unsigned)r->N_diff_obj, (unsigned)r->N_diff_obj_section,
+						   (unsigned)r->N_mut2_ctx, o->drfold);
+	r->confidence = confidence_from(r->ani, r->calibrated_ani, r->af_ref);
+	r->best_ani = r->calibrated_ani;
+	r->best_guarded = 0;
+	const double min_af = r->af_qry < r->af_ref ? r->af_qry : r->af_ref;
+	if (!best_guard_enabled(o) || !meta_has_assembly(qa) || !meta_like_assembly(qa) || r->ani < 0.958 ||
+	    r->calibrated_ani < 0.962 || min_af < 0.25 || min_af >= 0.50 || r->XnY_ctx <= 0 ||
+	    r->af_qry <= 0.0 || r->af_ref <= 0.0)
+		return;
+	const double qc = (double)r->XnY_ctx / r->af_qry;
+	const double rc = (double)r->XnY_ctx / r->af_ref;
+	double exact = (double)r->XnY_ctx - (double)r->N_diff_obj;
+	if (exact < 0.0)
+		exact = 0.0;
+	const double mean = (mash_ani(o, (double)r->XnY_ctx, qc, rc) + aaf_ani(o, (double)r->XnY_ctx, qc, rc) +
+			     mash_ani(o, exact, qc, rc) + aaf_ani(o, exact, qc, rc)) / 4.0;
+	if (r->ani - mean < 0.015)
+		return;
+	r->best_ani = aaf_ani(o, exact, qc, rc);
+	r->best_guarded = 1;
+}
+
+static inline int request_id(const kssd_report_opts_t *o)
+{
+	int s = abs(o->s);
+	return (s < 1 || s > 9) ? 1 : s;
+}
+
+static inline bool value_available(double v) { return v > 0.0 && v <= 1.0; }
+
+static int effective_id(const kssd_row_t *r, const kssd_report_opts_t *o)
+{
+	/* selected_metric_effective_id, command_ani.c:1260-1270 */
+	int s = request_id(o);
+	if (o->unassembled && !o->unified_metric && s <= 4)
+		return 4;
+	if (s == 1 && !(!o->raw_output && !o->unassembled && value_available(r->best_ani)))
+		s = 2;
+	if (s == 2 && !(!o->raw_output && !o->unassembled && value_available(r->calibrated_ani)))
+		s = 3;
+	return s;
+}
+
+static double selected_distance(const kssd_row_t *r, const kssd_report_opts_t *o)
+{
+	/* selected_metric_distance_from_row, command_ani.c:1272-1296 */
+	switch (effective_id(r, o)) {
+	case 1: return 1.0 - r->best_ani;
+	case 2: return 1.0 - r->calibrated_ani;
+	case 3: return r->moe_dist;
+	case 4: return r->naive_dist;
+	case 5: return r->mash_dist;
+	case 6: return r->aaf_dist;
+	case 7: return r->af_pass ? r->moe_dist : r->mash_dist;
+	case 8: return r->af_pass ? r->moe_dist : r->aaf_dist;
+	case 9: return r->p_dist;
+	default: return r->metric;
+	}
+}
+
+void kssd_finalize_row(kssd_row_t *r, const kssd_report_opts_t *o)
+{
+	r->metric = selected_distance(r, o);
+	r->selected_ani = 1.0 - r->metric;
+}
+
+double kssd_blastn_af(uint32_t xny, uint32_t dobj, uint32_t dsec, uint32_t mut2, uint32_t x_ctx, int drfold)
+{
+	return kssd_af_anib(xny, dobj, dsec, mut2, x_ctx, drfold);
+}
+
+double kssd_naive_distance(uint32_t xny, uint32_t dobj, uint32_t dsec)
+{
+	return kssd_naive_dist(xny, dobj, dsec);
+}
+
+kssd_row_t kssd_make_row(uint32_t rn, uint32_t xny, uint32_t dobj, uint32_t dsec, uint32_t mut2,
+			 const kssd_report_opts_t *o, uint32_t qry_ctx, uint32_t ref_ctx,
+			 double af_qry, double blastn_af_qry, double af_ref, double blastn_af_ref,
+			 const kssd_infile_meta_t *qry_asm, const kssd_infile_meta_t *ref_asm)
+{
+	/* make_selected_output_row, command_ani.c:1349-1379, with
+	 * fill_row_base_distances :1321-1341 */
+	(void)ref_asm;
+	kssd_row_t r;
+	memset(&r, 0, sizeof r);
+	r.rn = rn;
+	r.af_qry = af_qry;
+	r.blastn_af_qry = blastn_af_qry;
+	r.af_ref = af_ref;
+	r.blastn_af_ref = blastn_af_ref;
+	r.XnY_ctx = (int)xny;
+	r.N_diff_obj = (int)dobj;
+	r.N_diff_obj_section = (int)dsec;
+	r.N_mut2_ctx = (int)mut2;
+	r.af_pass = af_pass(o, af_qry, af_ref);
+	if (r.af_pass) {
+		r.moe_dist = kssd_moe_dist(xny, dobj, dsec, mut2, o->drfold);
+		r.naive_dist = kssd_naive_dist(xny, dobj, dsec);
+		r.p_dist = kssd_p_dist(xny, dsec, o->obj_bits);
+	} else {
+		r.moe_dist = o->e;
+		r.naive_dist = o->e;
+		r.p_dist = o->e;
+	}
+	r.mash_dist = kssd_mashD(o->ctx_bits / 2, ref_ctx, qry_ctx, xny);
+	r.aaf_dist = kssd_aafD(o->ctx_bits / 2, ref_ctx, qry_ctx, xny);
+	r.ani = 1.0 - (o->v ? r.naive_dist : r.moe_dist);
+	if (!o->raw_output && r.af_pass)
+		fill_calibration(&r, o, qry_asm);
+	kssd_finalize_row(&r, o);
+	return r;
+}
+
+static const char *confidence_label(unsigned char c)
+{
+	return c == CONF_HIGH ? "high" : (c == CONF_MEDIUM ? "medium" : "low");
+}
+
+static const char *row_confidence(const kssd_row_t *r, const kssd_report_opts_t *o)
+{
+	/* selected_metric_confidence_label, command_ani.c:1304-1319 */
+	if (o->raw_output)
+		return "raw";
+	if (o->unassembled && !o->unified_metric && request_id(o) <= 4)
+		return "unassembled";
+	switch (effective_id(r, o)) {
+	case 1: return r->best_guarded ? "guarded_low_confidence" : confidence_label(r->confidence);
+	case 2: return confidence_label(r->confidence);
+	default: return "raw";
+	}
+}
+
+static bool passes_close_filter(const kssd_row_t *r, const kssd_report_opts_t *o)
+{
+	/* ani_row_passes_close_filter, command_ani.c:1232-1246 */
+	if (!o->max_pdist_set && o->max_diff_obj_section < 0)
+		return true;
+	if (o->max_diff_obj_section >= 0 && r->N_diff_obj_section > o->max_diff_obj_section)
+		return false;
+	if (o->max_pdist_set && (!isfinite(r->p_dist) || r->p_dist > o->max_pdist))
+		return false;
+	return true;
+}
+
+static bool close_filter_impossible(const kssd_report_opts_t *o, uint32_t dsec, uint32_t max_shared)
+{
+	/* ani_close_filter_impossible, command_ani.c:1209-1230, evaluated on the
+	 * final counts */
+	if (!o->max_pdist_set && o->max_diff_obj_section < 0)
+		return false;
+	double limit = INFINITY;
+	if (o->max_diff_obj_section >= 0)
+		limit = (double)o->max_diff_obj_section;
+	if (o->max_pdist_set) {
+		const double pos = o->obj_bits > 0 ? (double)o->obj_bits / 2.0 : 0.0;
+		if (pos <= 0.0)
+			return dsec > 0;
+		const double pl = o->max_pdist * (double)max_shared * pos;
+		if (pl < limit)
+			limit = pl;
+	}
+	return (double)dsec > floor(limit + 1e-12);
+}
+
+static inline double bounded_af(double v)
+{
+	if (!isfinite(v))
+		return v;
+	return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+}
+
+void kssd_append_detail_row(kssd_buf_t *b, const kssd_report_opts_t *o, const char *qry_name,
+			    const char *ref_name, const kssd_row_t *r, const char *annotation)
+{
+	/* append_unified_detail_row, command_ani.c:1381-1402 */
+	if (!passes_close_filter(r, o))
+		return;
+	const char *metric = o->unique_ctx ? (r->unique_ctx_selected ? "NaiveUnique" : "NaiveFull")
+					   : metric_names[effective_id(r, o) - 1];
+	kssd_buf_printf(b, "%s\t%s\t%lf\t%lf\t%s\t%s\t%d\t%f\t%f\t%f\t%f\t%d\t%d\t%d\t%s\n", qry_name, ref_name,
+			r->selected_ani, r->metric, row_confidence(r, o), metric, r->XnY_ctx, r->af_qry,
+			bounded_af(r->blastn_af_qry), r->af_ref, bounded_af(r->blastn_af_ref), r->N_diff_obj,
+			r->N_diff_obj_section, r->N_mut2_ctx,
+			(annotation && annotation[0]) ? annotation : "NA");
+}
+
+/* ------------------------------------------------------------ sorting */
+/* Stable top-down merge sort taking from the left run while cmp <= 0: the
+ * element order glibc's qsort produces (msort, glibc 2.35), which the
+ * reference's outputs depend on for ties. */
+typedef int (*cmp_fn)(const void *, const void *);
+
+static void msort_rec(char *b, size_t n, size_t sz, cmp_fn cmp, char *tmp)
+{
+	if (n <= 1)
+		return;
+	const size_t n1 = n / 2, n2 = n - n1;
+	char *b1 = b, *b2 = b + n1 * sz;
+	msort_rec(b1, n1, sz, cmp, tmp);
+	msort_rec(b2, n2, sz, cmp, tmp);
+	char *t = tmp;
+	size_t i = n1, j = n2;
+	while (i > 0 && j > 0) {
+		if (cmp(b1, b2) <= 0) {
+			memcpy(t, b1, sz);
+			b1 += sz;
+			--i;
+		} else {
+			memcpy(t, b2, sz);
+			b2 += sz;
+			--j;
+		}
+		t += sz;
+	}
+	if (i > 0)
+		memcpy(t, b1, i * sz);
+	memcpy(b, tmp, (n - j) * sz);
+}
+
+static void stable_sort(void *base, size_t n, size_t sz, cmp_fn cmp)
+{
+	if (n <= 1)
+		return;
+	char *tmp = malloc(n * sz);
+	if (!tmp)
+		abort();
+	msort_rec(base, n, sz, cmp, tmp);
+	free(tmp);
+}
+
+typedef struct {
+	uint32_t id;
+	float ani;
+} idani_t;
+
+static int cmp_idani_desc(const void *a, const void *b)
+{
+	/* compare_idani_desc, command_ani.h:361-366 */
+	const idani_t *x = a, *y = b;
+	return (x->ani < y->ani) - (x->ani > y->ani);
+}
+
+static void heap_sift_down(idani_t *h, int n, int i)
+{
+	for (;;) {
+		const int l = i * 2 + 1, r = l + 1;
+		int s = i;
+		if (l < n && h[l].ani < h[s].ani)
+			s = l;
+		if (r < n && h[r].ani < h[s].ani)
+			s = r;
+		if (s == i)
+			break;
+		const idani_t t = h[i];
+		h[i] = h[s];
+		h[s] = t;
+		i = s;
+	}
+}
+
+static int select_topN_sorted(idani_t *arr, int m, int topN)
+{
+	/* idani_select_topN_sorted and partial_topN_inplace_idani,
+	 * command_ani.h:369-433 */
+	if (m <= 0 || topN <= 0)
+		return 0;
+	if (topN > m)
+		topN = m;
+	if (m <= 128 || topN >= m / 2) {
+		stable_sort(arr, (size_t)m, sizeof *arr, cmp_idani_desc);
+		return topN;
+	}
+	if (m > 1) {
+		for (int i = (topN >> 1) - 1; i >= 0; --i)
+			heap_sift_down(arr, topN, i);
+		for (int i = topN; i < m; ++i) {
+			if (arr[i].ani <= arr[0].ani)
+				continue;
+			const idani_t t = arr[i];
+			arr[i] = arr[0];
+			arr[0] = t;
+			heap_sift_down(arr, topN, 0);
+		}
+		stable_sort(arr, (size_t)topN, sizeof *arr, cmp_idani_desc);
+	}
+	return topN;
+}
+
+typedef struct {
+	double ani;
+	uint32_t id;
+} unique_rank_t;
+
+static int cmp_unique_rank(const void *a, const void *b)
+{
+	/* compare_unique_ctx_rank, command_ani.h:456-461 */
+	const unique_rank_t *x = a, *y = b;
+	if (x->ani != y->ani)
+		return x->ani < y->ani ? 1 : -1;
+	return (x->id > y->id) - (x->id < y->id);
+}
+
+static int cmp_row_desc(const void *pa, const void *pb)
+{
+	/* cmp_ani_desc, command_ani.c:1087-1095 */
+	const kssd_row_t *a = pa, *b = pb;
+	if (a->selected_ani < b->selected_ani)
+		return 1;
+	if (a->selected_ani > b->selected_ani)
+		return -1;
+	if (a->af_qry < b->af_qry)
+		return 1;
+	if (a->af_qry > b->af_qry)
+		return -1;
+	return (a->rn > b->rn) - (a->rn < b->rn);
+}
+
+/* ------------------------------------------------------------ emitters */
+static inline const char *anno_at(char (*anno)[KSSD_PATHLEN], uint32_t i)
+{
+	return anno ? anno[i] : NULL;
+}
+
+static inline double naive_of(kssd_ctx_mut2_t c, kssd_obj_section_t ob)
+{
+	return kssd_naive_dist(c.num_ctx, ob.diff_obj, ob.diff_obj_section);
+}
+
+static inline bool use_unique(uint32_t matched, uint32_t full)
+{
+	/* ani_use_unique_context, command_ani.h:437-440 */
+	return full > 0 && (uint64_t)matched * 20 >= full;
+}
+
+void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
+			    const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
+			    char (*ref_anno)[KSSD_PATHLEN])
+{
+	const int R = c->nref;
+	const uint32_t ctxcut = (uint32_t)o->ctxcut;
+	const int ntop = o->ntop > 0 ? o->ntop : R;
+	if (R <= 0)
+		return;
+	/* ranking, command_ani.h:577-621 */
+	idani_t *ids = malloc((size_t)R * sizeof *ids);
+	unique_rank_t *ranks = o->unique_ctx ? malloc((size_t)R * sizeof *ranks) : NULL;
+	if (!ids || (o->unique_ctx && !ranks))
+		abort();
+	int npass = 0;
+	for (int j = 0; j < R; ++j) {
+		const uint32_t shared = c->ctx[j].num_ctx;
+		if (shared < ctxcut)
+			continue;
+		if (c->ref_ctx[j] == 0 || c->qry_ctx == 0 ||
+		    !af_pass(o, (double)shared / (double)c->qry_ctx, (double)shared / (double)c->ref_ctx[j]))
+			continue;
+		const float dist = (float)c->obj[j].diff_obj / c->ctx[j].num_ctx / 7;
+		double ani = (float)(1 - dist);
+		if (ranks) {
+			const bool u = use_unique(c->uctx[j].num_ctx, c->ref_ctx[j]);
+			ani = 1.0 - naive_of(u ? c->uctx[j] : c->ctx[j], u ? c->uobj[j] : c->obj[j]);
+		}
+		if (ani < o->anicut)
+			continue;
+		if (ranks)
+			ranks[npass] = (unique_rank_t){ani, (uint32_t)j};
+		ids[npass].id = (uint32_t)j;
+		ids[npass].ani = (float)ani;
+		npass++;
+	}
+	int out_n;
+	if (ranks) {
+		stable_sort(ranks, (size_t)npass, sizeof *ranks, cmp_unique_rank);
+		out_n = npass < ntop ? npass : ntop;
+		for (int r = 0; r < out_n; ++r)
+			ids[r].id = ranks[r].id;
+	} else {
+		out_n = select_topN_sorted(ids, npass, ntop);
+	}
+	/* printing, ani_block_print_internal command_ani.c:3470-3524 */
+	const int qsz = (int)c->qry_ctx;
+	for (int n = 0; n < out_n; ++n) {
+		const uint32_t j = ids[n].id;
+		const int rsz = (int)c->ref_ctx[j];
+		if (qsz == 0 || rsz == 0)
+			continue;
+		const uint32_t xny = c->ctx[j].num_ctx, mut2 = c->ctx[j].num_mut2_ctx;
+		const uint32_t dobj = c->obj[j].diff_obj, dsec = c->obj[j].diff_obj_section;
+		const float afq = (float)xny / qsz, afr = (float)xny / rsz;
+		const float bq = (float)kssd_af_anib(xny, dobj, dsec, mut2, (uint32_t)qsz, o->drfold);
+		const float br = (float)kssd_af_anib(xny, dobj, dsec, mut2, (uint32_t)rsz, o->drfold);
+		kssd_row_t row = kssd_make_row(j, xny, dobj, dsec, mut2, o, (uint32_t)qsz, (uint32_t)rsz, afq, bq,
+					       afr, br, c->qry_meta, c->ref_meta ? &c->ref_meta[j] : NULL);
+		if (o->unique_ctx) {
+			row.unique_ctx_selected = use_unique(c->uctx[j].num_ctx, c->ref_ctx[j]);
+			row.naive_dist = naive_of(row.unique_ctx_selected ? c->uctx[j] : c->ctx[j],
+						  row.unique_ctx_selected ? c->uobj[j] : c->obj[j]);
+			kssd_finalize_row(&row, o);
+		}
+		if (row.selected_ani >= o->anicut)
+			kssd_append_detail_row(out, o, qry_name, ref_names[j], &row, anno_at(ref_anno, j));
+	}
+	free(ranks);
+	free(ids);
+}
+
+void kssd_emit_index_matrix_row(kssd_buf_t *out, const kssd_report_opts_t *o,
+				const kssd_query_counts_t *c, const char *qry_name)
+{
+	/* matrix branch of ani_block_print_internal, command_ani.c:3465-3551 */
+	kssd_buf_printf(out, "%s", qry_name);
+	const int qsz = (int)c->qry_ctx;
+	for (int j = 0; j < c->nref; ++j) {
+		const int rsz = (int)c->ref_ctx[j];
+		if (qsz == 0 || rsz == 0) {
+			kssd_buf_printf(out, "\t%lf", kssd_matrix_exception(o));
+			continue;
+		}
+		const uint32_t xny = c->ctx[j].num_ctx, mut2 = c->ctx[j].num_mut2_ctx;
+		const uint32_t dobj = c->obj[j].diff_obj, dsec = c->obj[j].diff_obj_section;
+		const float afq = (float)xny / qsz, afr = (float)xny / rsz;
+		const float bq = (float)kssd_af_anib(xny, dobj, dsec, mut2, (uint32_t)qsz, o->drfold);
+		const float br = (float)kssd_af_anib(xny, dobj, dsec, mut2, (uint32_t)rsz, o->drfold);
+		kssd_row_t row = kssd_make_row((uint32_t)j, xny, dobj, dsec, mut2, o, (uint32_t)qsz, (uint32_t)rsz,
+					       afq, bq, afr, br, c->qry_meta,
+					       c->ref_meta ? &c->ref_meta[j] : NULL);
+		if (o->unique_ctx) {
+			row.unique_ctx_selected = use_unique(c->uctx[j].num_ctx, c->ref_ctx[j]);
+			row.naive_dist = naive_of(row.unique_ctx_selected ? c->uctx[j] : c->ctx[j],
+						  row.unique_ctx_selected ? c->uobj[j] : c->obj[j]);
+			kssd_finalize_row(&row, o);
+		}
+		kssd_buf_printf(out, "\t%lf", o->s < 0 ? row.selected_ani : row.metric);
+	}
+	kssd_buf_printf(out, "\n");
+}
+
+static void emit_sorted(kssd_buf_t *out, const kssd_report_opts_t *o, kssd_row_t *rows, size_t n,
+			size_t limit, const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
+			char (*ref_anno)[KSSD_PATHLEN])
+{
+	stable_sort(rows, n, sizeof *rows, cmp_row_desc);
+	if (limit < n)
+		n = limit;
+	for (size_t i = 0; i < n; ++i)
+		kssd_append_detail_row(out, o, qry_name, ref_names[rows[i].rn], &rows[i],
+				       anno_at(ref_anno, rows[i].rn));
+}
+
+void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
+			     const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
+			     char (*ref_anno)[KSSD_PATHLEN])
+{
+	/* compute_streamed_ref_row_one_qry, command_ani.c:2110-2201; order and
+	 * top N as stream_ref_sketches_one_qraw_lookup :2634-2650 */
+	kssd_row_t *rows = malloc((size_t)(c->nref > 0 ? c->nref : 1) * sizeof *rows);
+	if (!rows)
+		abort();
+	size_t n = 0;
+	const uint32_t qc = c->qry_ctx;
+	for (int j = 0; j < c->nref; ++j) {
+		const uint32_t rc = c->ref_ctx[j];
+		if (qc == 0 || rc == 0)
+			continue;
+		const uint32_t need = af_needed_ctx(o, qc, rc);
+		const uint32_t xny = c->ctx[j].num_ctx, mut2 = c->ctx[j].num_mut2_ctx;
+		const uint32_t dobj = c->obj[j].diff_obj, dsec = c->obj[j].diff_obj_section;
+		if (xny < need || close_filter_impossible(o, dsec, xny))
+			continue;
+		if (xny < (uint32_t)o->ctxcut)
+			continue;
+		const double afq = (double)xny / (double)qc, afr = (double)xny / (double)rc;
+		if (!af_pass(o, afq, afr))
+			continue;
+		const double bq = kssd_af_anib(xny, dobj, dsec, mut2, qc, o->drfold);
+		const double br = kssd_af_anib(xny, dobj, dsec, mut2, rc, o->drfold);
+		kssd_row_t row = kssd_make_row((uint32_t)j, xny, dobj, dsec, mut2, o, qc, rc, afq, bq, afr, br,
+					       c->qry_meta, c->ref_meta ? &c->ref_meta[j] : NULL);
+		if (!passes_close_filter(&row, o) || row.selected_ani < o->anicut)
+			continue;
+		rows[n++] = row;
+	}
+	emit_sorted(out, o, rows, n, o->ntop > 0 ? (size_t)o->ntop : n, qry_name, ref_names, ref_anno);
+	free(rows);
+}
+
+void kssd_emit_hash_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
+			   const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
+			   char (*ref_anno)[KSSD_PATHLEN])
+{
+	/* comb_manysmall_..._filter_and_sort_survivors, command_ani.c:3888-4000:
+	 * no ctxcut, ANI must exceed anicut, every survivor printed */
+	kssd_row_t *rows = malloc((size_t)(c->nref > 0 ? c->nref : 1) * sizeof *rows);
+	if (!rows)
+		abort();
+	size_t n = 0;
+	const uint32_t qc = c->qry_ctx;
+	for (int j = 0; j < c->nref; ++j) {
+		const uint32_t rc = c->ref_ctx[j];
+		if (rc == 0 || qc == 0)
+			continue;
+		const uint32_t need = af_needed_ctx(o, qc, rc);
+		const uint32_t xny = c->ctx[j].num_ctx, mut2 = c->ctx[j].num_mut2_ctx;
+		const uint32_t dobj = c->obj[j].diff_obj, dsec = c->obj[j].diff_obj_section;
+		if (xny < need || close_filter_impossible(o, dsec, xny))
+			continue;
+		const double afq = (double)xny / (double)qc, afr = (double)xny / (double)rc;
+		if (!af_pass(o, afq, afr))
+			continue;
+		const double bq = kssd_af_anib(xny, dobj, dsec, mut2, qc, o->drfold);
+		const double br = kssd_af_anib(xny, dobj, dsec, mut2, rc, o->drfold);
+		kssd_row_t row = kssd_make_row((uint32_t)j, xny, dobj, dsec, mut2, o, qc, rc, afq, bq, afr, br,
+					       c->qry_meta, c->ref_meta ? &c->ref_meta[j] : NULL);
+		if (!passes_close_filter(&row, o) || row.selected_ani <= o->anicut)
+			continue;
+		rows[n++] = row;
+	}
+	emit_sorted(out, o, rows, n, n, qry_name, ref_names, ref_anno);
+	free(rows);
+}
+
+/* ------------------------------------------------------------ matrices */
+double kssd_matrix_exception(const kssd_report_opts_t *o)
+{
+	/* ani_matrix_exception_value, command_ani.c:1448-1451 */
+	return o->s < 0 ? 1.0 - o->e : o->e;
+}
+
+double kssd_matrix_diagonal(const kssd_report_opts_t *o)
+{
+	return o->s < 0 ? 1.0 : 0.0;
+}
+
+double kssd_matrix_value(const kssd_report_opts_t *o, uint32_t xny, uint32_t dobj, uint32_t dsec,
+			 uint32_t mut2, uint32_t qry_ctx, uint32_t ref_ctx,
+			 const kssd_infile_meta_t *qry_meta, const kssd_infile_meta_t *ref_meta)
+{
+	/* ani_matrix_pair_value :1458-1496 / ani_indexed_self_pair_metric :1627-1655 */
+	if (qry_ctx == 0 || ref_ctx == 0)
+		return kssd_matrix_exception(o);
+	const double afq = (double)xny / (double)qry_ctx, afr = (double)xny / (double)ref_ctx;
+	const double bq = kssd_af_anib(xny, dobj, dsec, mut2, qry_ctx, o->drfold);
+	const double br = kssd_af_anib(xny, dobj, dsec, mut2, ref_ctx, o->drfold);
+	kssd_row_t row = kssd_make_row(0, xny, dobj, dsec, mut2, o, qry_ctx, ref_ctx, afq, bq, afr, br,
+				       qry_meta, ref_meta);
+	return o->s < 0 ? row.selected_ani : row.metric;
+}
+
+int32_t kssd_scale_fixed6(double v)
+{
+	/* ani_scale_fixed6, command_ani.c:1593-1604 */
+	if (!isfinite(v))
+		v = 1.0;
+	if (v > 2147.0)
+		v = 2147.0;
+	if (v < -2147.0)
+		v = -2147.0;
+	return (int32_t)(v >= 0.0 ? v * 1000000.0 + 0.5 : v * 1000000.0 - 0.5);
+}
+
+void kssd_put_fixed6(kssd_buf_t *b, int32_t scaled)
+{
+	/* ani_kput_scaled6, command_ani.c:1606-1625 */
+	uint32_t mag;
+	if (scaled < 0) {
+		kssd_buf_printf(b, "-");
+		mag = (uint32_t)(-(int64_t)scaled);
+	} else {
+		mag = (uint32_t)scaled;
+	}
+	kssd_buf_printf(b, "%u.%06u", mag / 1000000U, mag % 1000000U);
+}

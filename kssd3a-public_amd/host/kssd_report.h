@@ -1,0 +1,155 @@
+/*
+ * Host epilogue of the comparison path: from per-pair counts to the rows and
+ * matrices the `ani` command prints.  Counts come from the HIP kernels (or
+ * any source with the same layout); everything here is FP64 host code.
+ *
+ * Row semantics follow make_selected_output_row (command_ani.c:1349-1379)
+ * and its callees; the per-path filters and orders follow
+ *   index path   count_ctx_obj_frm_comb_sketch_section_with_unique ranking
+ *                (command_ani.h:577-621) + ani_block_print_internal
+ *                (command_ani.c:3437-3553)
+ *   stream path  compute_streamed_ref_row_one_qry (command_ani.c:2110-2201)
+ *                + cmp_ani_desc order (command_ani.c:1087-1095)
+ *   hash path    comb_manysmall_..._filter_and_sort_survivors
+ *                (command_ani.c:3888-4000)
+ *   matrices     comb_sortedsketch64_self_matrix, _indexed_self_full,
+ *                _indexed_self_triangle, comb_sortedsketch64Xcomb_sortedsketch64
+ */
+#ifndef KSSD_REPORT_H
+#define KSSD_REPORT_H
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "kssdhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSSD_PATHLEN 256
+
+/* lcofiles.infilemeta record, global_basic.h:182-193 */
+typedef struct {
+	uint64_t total_length_bp;
+	uint32_t record_count;
+	uint32_t median_length_bp;
+	float asm_level;
+	float length_cv;
+	uint8_t meta_fmt_version;
+	int8_t infile_fmt;
+	int8_t create_type;
+	uint8_t infile_flags;
+} kssd_infile_meta_t;
+
+/* The options of ani_opt_t (command_ani.h:18-68) that shape the output. */
+typedef struct {
+	int fmt;             /* 0 detail, 1 matrix, 2 triangle */
+	int s;               /* selected metric, sign = matrix values as ANI */
+	bool v;              /* naive model */
+	bool unassembled;    /* --qraw */
+	bool unified_metric;
+	bool ignoreconflict;
+	bool raw_output;
+	bool max_pdist_set;
+	double max_pdist;
+	int max_diff_obj_section; /* -1 = off */
+	int e;               /* exception value */
+	int ntop;            /* <= 0: all */
+	int ctxcut;
+	float afcut;
+	float anicut;
+	bool unique_ctx;
+	bool d;              /* diagonal in triangle output */
+	int drfold;          /* ani_model_drfold */
+	unsigned obj_bits;   /* Bitslen.obj */
+	unsigned ctx_bits;   /* Bitslen.ctx */
+} kssd_report_opts_t;
+
+/* ani_row_t (command_ani.c:1060-1085), the fields that reach the output */
+typedef struct {
+	uint32_t rn;
+	double af_qry, blastn_af_qry, af_ref, blastn_af_ref;
+	double ani, calibrated_ani, best_ani;
+	double moe_dist, naive_dist, p_dist, mash_dist, aaf_dist;
+	double metric, selected_ani;
+	unsigned char confidence;
+	unsigned char best_guarded;
+	bool af_pass;
+	bool unique_ctx_selected;
+	int XnY_ctx, N_diff_obj, N_diff_obj_section, N_mut2_ctx;
+} kssd_row_t;
+
+/* calibrated ANI model (refaf_hgb_predict_ani, model_refaf_hgb.h:7593-7628) */
+double kssd_refaf_predict_ani(double raw_ani, double ref_af, unsigned xny, unsigned n_diff_obj,
+			      unsigned n_diff_obj_section, unsigned n_mut2_ctx, int drfold);
+
+/* make_selected_output_row */
+kssd_row_t kssd_make_row(uint32_t rn, uint32_t xny, uint32_t dobj, uint32_t dsec, uint32_t mut2,
+			 const kssd_report_opts_t *o, uint32_t qry_ctx, uint32_t ref_ctx,
+			 double af_qry, double blastn_af_qry, double af_ref, double blastn_af_ref,
+			 const kssd_infile_meta_t *qry_asm, const kssd_infile_meta_t *ref_asm);
+/* selected metric after a counter swap (unique-context rows, command_ani.c:3503-3510) */
+void kssd_finalize_row(kssd_row_t *r, const kssd_report_opts_t *o);
+/* lm3ways_af_ANIb_from_features with X_ctx */
+double kssd_blastn_af(uint32_t xny, uint32_t dobj, uint32_t dsec, uint32_t mut2, uint32_t x_ctx,
+		      int drfold);
+double kssd_naive_distance(uint32_t xny, uint32_t dobj, uint32_t dsec);
+
+/* Append the detail row (append_unified_detail_row, command_ani.c:1381-1432)
+ * to a growable buffer; returns 0 or -1 (allocation). */
+typedef struct {
+	char *s;
+	size_t l, m;
+} kssd_buf_t;
+int kssd_buf_printf(kssd_buf_t *b, const char *fmt, ...);
+void kssd_buf_free(kssd_buf_t *b);
+void kssd_append_detail_row(kssd_buf_t *b, const kssd_report_opts_t *o, const char *qry_name,
+			    const char *ref_name, const kssd_row_t *r, const char *annotation);
+extern const char kssd_detail_header[];
+
+/* Rows with qry/ref context totals and counts for one query against all refs. */
+typedef struct {
+	const kssd_ctx_mut2_t *ctx;  /* [nref] */
+	const kssd_obj_section_t *obj;
+	const kssd_ctx_mut2_t *uctx; /* optional (unique-context) */
+	const kssd_obj_section_t *uobj;
+	uint32_t qry_ctx;
+	const uint32_t *ref_ctx;     /* [nref] */
+	int nref;
+	const kssd_infile_meta_t *qry_meta;  /* may be NULL */
+	const kssd_infile_meta_t *ref_meta;  /* [nref] or NULL */
+} kssd_query_counts_t;
+
+/* Index path (mem_eff) detail rows of one query, ranked and filtered as the
+ * reference does; appends the printed rows to out. */
+void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
+			    const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
+			    char (*ref_anno)[KSSD_PATHLEN]);
+/* Index path (mem_eff) matrix row: "name\t%lf..." with float-cast AFs. */
+void kssd_emit_index_matrix_row(kssd_buf_t *out, const kssd_report_opts_t *o,
+				const kssd_query_counts_t *c, const char *qry_name);
+/* Stream path (one query or few queries vs many references). */
+void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
+			     const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
+			     char (*ref_anno)[KSSD_PATHLEN]);
+/* Hash path (many small references, no index). */
+void kssd_emit_hash_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
+			   const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
+			   char (*ref_anno)[KSSD_PATHLEN]);
+
+/* Matrix cell (ani_matrix_pair_value / ani_indexed_self_pair_metric). */
+double kssd_matrix_value(const kssd_report_opts_t *o, uint32_t xny, uint32_t dobj, uint32_t dsec,
+			 uint32_t mut2, uint32_t qry_ctx, uint32_t ref_ctx,
+			 const kssd_infile_meta_t *qry_meta, const kssd_infile_meta_t *ref_meta);
+double kssd_matrix_exception(const kssd_report_opts_t *o);
+double kssd_matrix_diagonal(const kssd_report_opts_t *o);
+/* ani_scale_fixed6 / ani_kput_scaled6 (command_ani.c:1593-1625) */
+int32_t kssd_scale_fixed6(double v);
+void kssd_put_fixed6(kssd_buf_t *b, int32_t scaled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -55,6 +55,36 @@ class SimSpec(ctypes.Structure):
                 ("indel_frac", D), ("anc2_seed", U64), ("anc2_rate", D)]
 
 
+class ReportOpts(ctypes.Structure):
+    """kssd_report_opts_t (the output-shaping fields of the reference's ani_opt_t)."""
+    _fields_ = [("fmt", I), ("s", I), ("v", ctypes.c_bool), ("unassembled", ctypes.c_bool),
+                ("unified_metric", ctypes.c_bool), ("ignoreconflict", ctypes.c_bool),
+                ("raw_output", ctypes.c_bool), ("max_pdist_set", ctypes.c_bool), ("max_pdist", D),
+                ("max_diff_obj_section", I), ("e", I), ("ntop", I), ("ctxcut", I),
+                ("afcut", ctypes.c_float), ("anicut", ctypes.c_float), ("unique_ctx", ctypes.c_bool),
+                ("d", ctypes.c_bool), ("drfold", I), ("obj_bits", U), ("ctx_bits", U)]
+
+    @classmethod
+    def defaults(cls, **kw):
+        """The reference's ani defaults (command_ani_wrapper.c:174-221)."""
+        o = cls(fmt=0, s=1, v=False, unassembled=False, unified_metric=False, ignoreconflict=False,
+                raw_output=False, max_pdist_set=False, max_pdist=0.0, max_diff_obj_section=-1, e=1,
+                ntop=0, ctxcut=3, afcut=0.5, anicut=0.95, unique_ctx=False, d=False, drfold=8,
+                obj_bits=20, ctx_bits=44)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class QueryCounts(ctypes.Structure):
+    _fields_ = [("ctx", V), ("obj", V), ("uctx", V), ("uobj", V), ("qry_ctx", U32), ("ref_ctx", V),
+                ("nref", I), ("qry_meta", V), ("ref_meta", V)]
+
+
+class Buf(ctypes.Structure):
+    _fields_ = [("s", ctypes.c_char_p), ("l", S), ("m", S)]
+
+
 # ------------------------------------------------------------------ HIP runtime
 class HipError(RuntimeError):
     pass
@@ -255,6 +285,19 @@ def load_kssdhost():
     lib.kssd_sketch_codes.restype = None
     lib.kssd_finalize_keys.argtypes = [ctypes.POINTER(U64Vec), I]
     lib.kssd_finalize_keys.restype = None
+    for f in ("kssd_emit_index_detail", "kssd_emit_stream_detail", "kssd_emit_hash_detail"):
+        getattr(lib, f).argtypes = [ctypes.POINTER(Buf), ctypes.POINTER(ReportOpts),
+                                    ctypes.POINTER(QueryCounts), ctypes.c_char_p, V, V]
+        getattr(lib, f).restype = None
+    lib.kssd_emit_index_matrix_row.argtypes = [ctypes.POINTER(Buf), ctypes.POINTER(ReportOpts),
+                                               ctypes.POINTER(QueryCounts), ctypes.c_char_p]
+    lib.kssd_emit_index_matrix_row.restype = None
+    lib.kssd_buf_free.argtypes = [ctypes.POINTER(Buf)]
+    lib.kssd_buf_free.restype = None
+    lib.kssd_matrix_value.argtypes = [ctypes.POINTER(ReportOpts), U32, U32, U32, U32, U32, U32, V, V]
+    lib.kssd_matrix_value.restype = D
+    lib.kssd_refaf_predict_ani.argtypes = [D, D, U, U, U, U, I]
+    lib.kssd_refaf_predict_ani.restype = D
     return lib
 
 
@@ -301,6 +344,34 @@ class Host:
                                      int(threads or default_threads()), _p(out)) != 0:
             raise ValueError("index limits exceeded")
         return out
+
+    def emit(self, path, opts, ctx, obj, qry_ctx, ref_ctx, qry_name, ref_names, uctx=None, uobj=None,
+             ref_anno=None):
+        """Detail rows of one query on one of the reference's output paths
+        ('index', 'stream', 'hash') or its index-path matrix row ('matrix')."""
+        ctx = np.ascontiguousarray(ctx, dtype=np.uint32)
+        obj = np.ascontiguousarray(obj, dtype=np.uint32)
+        ref_ctx = np.ascontiguousarray(ref_ctx, dtype=np.uint32)
+        uctx = None if uctx is None else np.ascontiguousarray(uctx, dtype=np.uint32)
+        uobj = None if uobj is None else np.ascontiguousarray(uobj, dtype=np.uint32)
+        qc = QueryCounts(_p(ctx), _p(obj), _p(uctx), _p(uobj), int(qry_ctx), _p(ref_ctx),
+                         len(ref_ctx), None, None)
+        names = ctypes.create_string_buffer(b"".join(n.encode().ljust(256, b"\0") for n in ref_names),
+                                            256 * len(ref_names))
+        anno = None
+        if ref_anno is not None:
+            anno = ctypes.create_string_buffer(b"".join(n.encode().ljust(256, b"\0") for n in ref_anno),
+                                               256 * len(ref_anno))
+        b = Buf()
+        if path == "matrix":
+            self.lib.kssd_emit_index_matrix_row(ctypes.byref(b), ctypes.byref(opts), ctypes.byref(qc),
+                                                qry_name.encode())
+        else:
+            fn = getattr(self.lib, f"kssd_emit_{path}_detail")
+            fn(ctypes.byref(b), ctypes.byref(opts), ctypes.byref(qc), qry_name.encode(), names, anno)
+        text = ctypes.string_at(b.s, b.l).decode() if b.l else ""
+        self.lib.kssd_buf_free(ctypes.byref(b))
+        return text
 
     def ctx_counts(self, keys, index, obj_bits=20, conflict=False, ignoreconflict=False):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)

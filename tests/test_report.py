@@ -1,0 +1,91 @@
+"""CPU tests of the product's host epilogue (libkssdhost.so: rows, calibrated
+model, ranking, formatting) fed with counts from the oracle: the printed
+text must be byte-identical to the reference binary's outputs under
+tests/golden/ for each output path (index / stream / hash / matrices)."""
+import numpy as np
+import pytest
+
+import kssdtest as kt
+
+GEN = kt.GOLDEN / "genomes"
+COUNT_CASES = sorted((kt.GOLDEN / "counts").glob("case_*"))
+
+
+@pytest.fixture(scope="module")
+def host():
+    return kt.Host()
+
+
+def counts(oracle, ref, qry, flags=0, records=None):
+    rec = records if records is not None else (ref.records if ref.records is not None else
+                                               kt.build_index_np(ref.keys, ref.index, ref.obj_bits))
+    return oracle.count_block(rec, ref.n, ref.obj_bits, qry.keys, qry.index, flags=flags, threads=2)
+
+
+def ctx_totals(oracle, sk, ignoreconflict=False):
+    if not sk.conflict:
+        return np.diff(sk.index).astype(np.uint32)
+    return np.array([oracle.lib.orc_ctx_runs(kt._p(sk.sample(i)), len(sk.sample(i)), sk.obj_bits,
+                                             int(ignoreconflict)) for i in range(sk.n)], np.uint32)
+
+
+def render(host, path, opts, ref, qry, c, qctx, rctx, header=True):
+    ctx, obj, uctx, uobj = c
+    out = [kt.DETAIL_HEADER + "\n"] if header else []
+    for q in range(qry.n):
+        out.append(host.emit(path, opts, ctx[q], obj[q], qctx[q], rctx, qry.names[q], ref.names,
+                             uctx=uctx[q] if opts.unique_ctx else None,
+                             uobj=uobj[q] if opts.unique_ctx else None))
+    return "".join(out)
+
+
+@pytest.mark.parametrize("golden,kw", [
+    ("detail_all_s3_index.tsv", dict(s=3, anicut=0.0, afcut=0.0, ctxcut=1)),
+    ("detail_all_s9_index.tsv", dict(s=9, anicut=0.0, afcut=0.0, ctxcut=1)),
+    ("detail_best_index.tsv", dict()),
+    ("detail_qraw_index.tsv", dict(unassembled=True, anicut=0.0, afcut=0.0, ctxcut=1)),
+])
+def test_index_path_detail(host, oracle, golden, kw):
+    ref, qry = kt.read_sketch_dir(GEN / "ref"), kt.read_sketch_dir(GEN / "qry")
+    c = counts(oracle, ref, qry)
+    opts = kt.ReportOpts.defaults(**kw)
+    got = render(host, "index", opts, ref, qry, c, ctx_totals(oracle, qry), ctx_totals(oracle, ref))
+    assert got == (GEN / golden).read_text()
+
+
+@pytest.mark.parametrize("golden,kw", [
+    ("detail_best_hash.tsv", dict()),
+    ("detail_all_s3_hash.tsv", dict(s=3, anicut=0.0, afcut=0.0, ctxcut=1)),
+])
+def test_hash_path_detail(host, oracle, golden, kw):
+    ref, qry = kt.read_sketch_dir(GEN / "ref"), kt.read_sketch_dir(GEN / "qry")
+    c = counts(oracle, ref, qry)
+    opts = kt.ReportOpts.defaults(**kw)
+    got = render(host, "hash", opts, ref, qry, c, ctx_totals(oracle, qry), ctx_totals(oracle, ref))
+    assert got == (GEN / golden).read_text()
+
+
+@pytest.mark.parametrize("case", COUNT_CASES, ids=lambda p: p.name)
+def test_qraw_index_paths_on_conflict_fixtures(host, oracle, case):
+    ref, qry = kt.read_sketch_dir(case / "ref"), kt.read_sketch_dir(case / "qry")
+    base = dict(unassembled=True, anicut=0.0, afcut=0.0, ctxcut=1)
+    c = counts(oracle, ref, qry, flags=kt.FLAGS["UNIQUE"])
+    qctx = ctx_totals(oracle, qry)
+    rctx = ctx_totals(oracle, ref)
+    assert render(host, "index", kt.ReportOpts.defaults(**base), ref, qry, c, qctx, rctx) == \
+        (case / "detail_qraw.tsv").read_text()
+    assert render(host, "index", kt.ReportOpts.defaults(unique_ctx=True, **base), ref, qry, c, qctx, rctx) == \
+        (case / "detail_qraw_unique.tsv").read_text()
+    ci = counts(oracle, ref, qry, flags=kt.FLAGS["IGNORECONFLICT"])
+    assert render(host, "index", kt.ReportOpts.defaults(ignoreconflict=True, **base), ref, qry, ci, qctx,
+                  ctx_totals(oracle, ref, True)) == (case / "detail_qraw_ignoreconflict.tsv").read_text()
+
+
+def test_calibrated_model_special_cases(host):
+    lib = host.lib
+    # no shared context: the raw ANI comes back unchanged
+    assert lib.kssd_refaf_predict_ani(0.93, 0.7, 0, 0, 0, 0, 8) == 0.93
+    # output is a clamped probability-like ANI
+    for raw in (0.0, 0.5, 0.9, 0.99, 1.0):
+        v = lib.kssd_refaf_predict_ani(raw, 0.6, 500, 100, 120, 10, 8)
+        assert 0.0 <= v <= 1.0
