@@ -1,0 +1,1048 @@
+/*
+ * kssd3a-hip: command-line front end of the comparison path.
+ *
+ *   kssd3a-hip ani [options]   the reference's `kssd3a ani` for sketch
+ *                              directories: options of command_ani_wrapper.c:77-135,
+ *                              path choice of run_sketch_ani (:1082-1230),
+ *                              every count computed by libkssdhip on the GPU
+ *   kssd3a-hip sketch -i DIR   inverted index of a sketch directory
+ *                              (gen_inverted_index4comblco, command_sketch.c:916-938)
+ *
+ * Output bytes follow the reference path that the same inputs and options
+ * select (index, single/multi-query stream, hash, direct or indexed matrix).
+ * The reference's auto-sketching inputs (raw FASTA/FASTQ, --reflist,
+ * --qrylist, --pair) and the coverage/report side outputs are not offered.
+ */
+#define _GNU_SOURCE
+#include <err.h>
+#include <errno.h>
+#include <getopt.h>
+#include <limits.h>
+#include <omp.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include "kssd_host.h"
+#include "kssd_report.h"
+#include "kssd_sketchdir.h"
+#include "kssdhip.h"
+
+#define PROG "kssd3a-hip"
+#define EX_USAGE 64
+
+/* ------------------------------------------------------------ small helpers */
+static const char *g_cmd = "";
+static int g_timing;
+static double g_t0;
+
+static double now_s(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void stage(const char *what)
+{
+	if (g_timing)
+		fprintf(stderr, "[%s %s] %8.3f s  %s\n", PROG, g_cmd, now_s() - g_t0, what);
+}
+
+static void __attribute__((noreturn, format(printf, 1, 2))) usage_error(const char *fmt, ...)
+{
+	va_list ap;
+	fprintf(stderr, "%s %s: ", PROG, g_cmd);
+	va_start(ap, fmt);
+	vfprintf(stderr, fmt, ap);
+	va_end(ap);
+	fprintf(stderr, "\nTry `%s %s --help' for more information.\n", PROG, g_cmd);
+	exit(EX_USAGE);
+}
+
+static long parse_int(const char *name, const char *arg, long lo, long hi)
+{
+	char *end = NULL;
+	errno = 0;
+	const long v = strtol(arg, &end, 10);
+	if (errno || end == arg || *end || v < lo || v > hi)
+		usage_error("%s requires an integer in range %ld..%ld", name, lo, hi);
+	return v;
+}
+
+static double parse_double(const char *name, const char *arg, double lo, double hi)
+{
+	char *end = NULL;
+	errno = 0;
+	const double v = strtod(arg, &end);
+	if (errno || end == arg || *end || !(v >= lo && v <= hi))
+		usage_error("%s requires a number in range %g..%g", name, lo, hi);
+	return v;
+}
+
+static void copy_path(char *dst, size_t cap, const char *name, const char *arg)
+{
+	if (strlen(arg) >= cap)
+		usage_error("%s path is too long", name);
+	strcpy(dst, arg);
+}
+
+static long env_int(const char *name, long dflt, long lo, const char *what)
+{
+	const char *env = getenv(name);
+	if (!env || !env[0])
+		return dflt;
+	char *end = NULL;
+	errno = 0;
+	const long v = strtol(env, &end, 10);
+	if (errno || end == env || *end || v < lo || v > INT_MAX)
+		errx(EXIT_FAILURE, "invalid %s='%s' (use %s)", name, env, what);
+	return v;
+}
+
+static bool env_flag(const char *name)
+{
+	const char *env = getenv(name);
+	return env && env[0] && strcmp(env, "0") != 0;
+}
+
+static bool same_file(const char *a, const char *b)
+{
+	struct stat sa, sb;
+	return a && b && stat(a, &sa) == 0 && stat(b, &sb) == 0 && sa.st_dev == sb.st_dev &&
+	       sa.st_ino == sb.st_ino;
+}
+
+static void *xmalloc(size_t n)
+{
+	void *p = malloc(n ? n : 1);
+	if (!p)
+		err(EXIT_FAILURE, "out of memory (%zu bytes)", n);
+	return p;
+}
+
+static void *xcalloc(size_t n, size_t sz)
+{
+	void *p = calloc(n ? n : 1, sz ? sz : 1);
+	if (!p)
+		err(EXIT_FAILURE, "out of memory (%zu x %zu bytes)", n, sz);
+	return p;
+}
+
+static void open_dir(const char *dir, unsigned parts, kssd_sketchdir_t *s)
+{
+	char msg[512];
+	if (kssd_sketchdir_open(dir, parts, s, msg, sizeof msg) != 0)
+		errx(EXIT_FAILURE, "%s", msg);
+}
+
+static void hip_check(int rc, const char *what)
+{
+	if (rc != 0)
+		errx(EXIT_FAILURE, "%s: %s", what, kssdhip_last_error());
+}
+
+/* ------------------------------------------------------------ options */
+typedef struct {
+	kssd_report_opts_t r;
+	char refdir[PATH_MAX], qrydir[PATH_MAX], outf[PATH_MAX];
+	int p;
+	int device;
+	int values; /* 0 unset, 1 distance, -1 ani */
+	bool afcut_set;
+} ani_args_t;
+
+enum {
+	OPT_IGNORECONFLICT = 256,
+	OPT_METRIC,
+	OPT_MAX_PDIST,
+	OPT_MAX_SECTION,
+	OPT_UNIFIED,
+	OPT_FORMAT,
+	OPT_VALUES,
+	OPT_RAW,
+	OPT_UNIQUE,
+	OPT_DEVICE,
+	OPT_HELP,
+	OPT_NOT_OFFERED,
+};
+
+static const char ani_help[] =
+	"Usage: " PROG " ani [-r REF_SKETCH] (-q QRY_SKETCH | --qraw RAW_QRY_SKETCH) [options]\n"
+	"       " PROG " ani SELF_SKETCH -m1|-m2 [options]\n"
+	"ANI between sketch directories; counting runs on the GPU.\n\n"
+	"  -r, --ref=DIR            reference sketch directory\n"
+	"  -q, --query=DIR          query sketch directory\n"
+	"  -x, --qraw=DIR           raw-read query sketch directory (naive ANI mode)\n"
+	"  -v, --naive              naive distance/ANI\n"
+	"      --ignoreconflict     ignore reference contexts with conflicting objects\n"
+	"  -f, --afcut=F            min max(Qry,Ref) align fraction [0.5; 0.2 with --qraw]\n"
+	"  -n, --anicut=F           min ANI [0.95]\n"
+	"  -t, --ctxcut=N           min shared contexts [3]\n"
+	"  -s, --slmetrics=+-1..9   Best(1) Recalibrated(2) CtxMoE(3) Naive(4) MashD(5) AafD(6)\n"
+	"                           MashD_if_far(7) AafD_if_far(8) p_dist(9) [1]\n"
+	"      --metric=NAME        best, recalibrated, ctx-moe, ctx-naive, mash, aaf,\n"
+	"                           mash-if-far, aaf-if-far, p_dist\n"
+	"      --max-pdist=F        close-only detail filter on p_dist\n"
+	"      --max-diff-section=N close-only detail filter on object-section differences\n"
+	"      --unified-metric     honour -s in --qraw mode\n"
+	"  -d, --diagonal           print the diagonal in triangle output\n"
+	"  -e, --exception=N        missing/low-overlap matrix distance [1]\n"
+	"  -m, --outfmt=0|1|2       detail, matrix or triangle [0]\n"
+	"      --format=NAME        detail, matrix or triangle\n"
+	"      --values=distance|ani  matrix/triangle values\n"
+	"  -o, --outfile=FILE       output file [stdout]\n"
+	"      --raw-output         skip calibrated/best ANI\n"
+	"      --unique-ctx         adaptive reference-unique contexts (indexed -r, --qraw)\n"
+	"  -N, --top=N              at most N references per query [all]\n"
+	"  -p, --threads=N          host threads for indexing, ranking and formatting [all]\n"
+	"      --device=N           HIP device ordinal [0]\n";
+
+static void parse_ani(int argc, char **argv, ani_args_t *a)
+{
+	static const struct option longopts[] = {
+		{"ref", required_argument, NULL, 'r'},
+		{"query", required_argument, NULL, 'q'},
+		{"qraw", required_argument, NULL, 'x'},
+		{"naive", no_argument, NULL, 'v'},
+		{"ignoreconflict", no_argument, NULL, OPT_IGNORECONFLICT},
+		{"afcut", required_argument, NULL, 'f'},
+		{"anicut", required_argument, NULL, 'n'},
+		{"metric", required_argument, NULL, OPT_METRIC},
+		{"ctxcut", required_argument, NULL, 't'},
+		{"slmetrics", required_argument, NULL, 's'},
+		{"max-pdist", required_argument, NULL, OPT_MAX_PDIST},
+		{"max-diff-section", required_argument, NULL, OPT_MAX_SECTION},
+		{"unified-metric", no_argument, NULL, OPT_UNIFIED},
+		{"diagonal", no_argument, NULL, 'd'},
+		{"exception", required_argument, NULL, 'e'},
+		{"outfmt", required_argument, NULL, 'm'},
+		{"format", required_argument, NULL, OPT_FORMAT},
+		{"values", required_argument, NULL, OPT_VALUES},
+		{"outfile", required_argument, NULL, 'o'},
+		{"raw-output", no_argument, NULL, OPT_RAW},
+		{"unique-ctx", no_argument, NULL, OPT_UNIQUE},
+		{"top", required_argument, NULL, 'N'},
+		{"threads", required_argument, NULL, 'p'},
+		{"device", required_argument, NULL, OPT_DEVICE},
+		{"help", no_argument, NULL, OPT_HELP},
+		/* reference options this front end does not offer */
+		{"reflist", required_argument, NULL, OPT_NOT_OFFERED},
+		{"qrylist", required_argument, NULL, OPT_NOT_OFFERED},
+		{"unique-ctx-report", required_argument, NULL, OPT_NOT_OFFERED},
+		{"estimate-coverage", no_argument, NULL, OPT_NOT_OFFERED},
+		{"glist", required_argument, NULL, OPT_NOT_OFFERED},
+		{"control", required_argument, NULL, OPT_NOT_OFFERED},
+		{"pair", no_argument, NULL, OPT_NOT_OFFERED},
+		{"pipecmd", required_argument, NULL, OPT_NOT_OFFERED},
+		{NULL, 0, NULL, 0},
+	};
+	memset(a, 0, sizeof *a);
+	kssd_report_opts_t *o = &a->r;
+	/* ani_opt defaults, command_ani_wrapper.c:174-221 */
+	o->fmt = 0;
+	o->s = 1;
+	o->max_diff_obj_section = -1;
+	o->e = 1;
+	o->ntop = -1;
+	o->ctxcut = 3;
+	o->afcut = 0.5f;
+	o->anicut = 0.95f;
+	o->drfold = 8;
+	a->p = omp_get_max_threads();
+	optind = 1;
+	int c;
+	while ((c = getopt_long(argc, argv, "r:q:x:vf:n:t:s:de:m:o:N:p:", longopts, NULL)) != -1) {
+		switch (c) {
+		case 'r':
+			copy_path(a->refdir, sizeof a->refdir, "-r/--ref", optarg);
+			break;
+		case 'q':
+			copy_path(a->qrydir, sizeof a->qrydir, "-q/--query", optarg);
+			break;
+		case 'x':
+			copy_path(a->qrydir, sizeof a->qrydir, "--qraw", optarg);
+			o->unassembled = true;
+			o->v = true;
+			break;
+		case 'v':
+			o->v = true;
+			break;
+		case OPT_IGNORECONFLICT:
+			o->ignoreconflict = true;
+			break;
+		case 'f':
+			o->afcut = (float)parse_double("-f/--afcut (fraction, e.g. 0.8, not 80)", optarg, 0.0, 1.0);
+			a->afcut_set = true;
+			break;
+		case 'n':
+			o->anicut = (float)parse_double("-n/--anicut (fraction, e.g. 0.95, not 95)", optarg, 0.0, 1.0);
+			break;
+		case OPT_METRIC: {
+			static const char *const names[9] = {"best", "recalibrated", "ctx-moe", "ctx-naive", "mash",
+							     "aaf", "mash-if-far", "aaf-if-far", "p_dist"};
+			int m = 0;
+			for (int i = 0; i < 9; ++i)
+				if (strcmp(optarg, names[i]) == 0)
+					m = i + 1;
+			if (strcmp(optarg, "naive") == 0)
+				m = 4;
+			if (strcmp(optarg, "pdist") == 0)
+				m = 9;
+			if (!m)
+				usage_error("unknown --metric '%s'; use best, recalibrated, ctx-moe, ctx-naive, p_dist, "
+					    "mash, aaf, mash-if-far, or aaf-if-far", optarg);
+			o->s = o->s < 0 ? -m : m;
+			break;
+		}
+		case 't':
+			o->ctxcut = (int)parse_int("-t/--ctxcut", optarg, 0, INT_MAX);
+			break;
+		case 's':
+			o->s = (int)parse_int("-s/--slmetrics", optarg, -9, 9);
+			if (o->s == 0)
+				usage_error("-s/--slmetrics accepts -9..-1 or 1..9, but not 0");
+			break;
+		case OPT_MAX_PDIST:
+			o->max_pdist = parse_double("--max-pdist", optarg, 0.0, 1.0);
+			o->max_pdist_set = true;
+			break;
+		case OPT_MAX_SECTION:
+			o->max_diff_obj_section = (int)parse_int("--max-diff-section", optarg, 0, INT_MAX);
+			break;
+		case OPT_UNIFIED:
+			o->unified_metric = true;
+			break;
+		case 'd':
+			o->d = true;
+			break;
+		case 'e':
+			o->e = (int)parse_int("-e/--exception", optarg, 1, INT_MAX);
+			break;
+		case 'm':
+			o->fmt = (int)parse_int("-m/--outfmt", optarg, 0, 2);
+			break;
+		case OPT_FORMAT:
+			if (strcmp(optarg, "detail") == 0)
+				o->fmt = 0;
+			else if (strcmp(optarg, "matrix") == 0)
+				o->fmt = 1;
+			else if (strcmp(optarg, "triangle") == 0)
+				o->fmt = 2;
+			else
+				usage_error("--format must be detail, matrix, or triangle");
+			break;
+		case OPT_VALUES:
+			if (strcmp(optarg, "distance") == 0)
+				a->values = 1;
+			else if (strcmp(optarg, "ani") == 0)
+				a->values = -1;
+			else
+				usage_error("--values must be distance or ani");
+			break;
+		case 'o':
+			copy_path(a->outf, sizeof a->outf, "-o/--outfile", optarg);
+			break;
+		case OPT_RAW:
+			o->raw_output = true;
+			break;
+		case OPT_UNIQUE:
+			o->unique_ctx = true;
+			break;
+		case 'N':
+			o->ntop = (int)parse_int("-N/--top", optarg, 1, INT_MAX);
+			break;
+		case 'p':
+			a->p = (int)parse_int("-p/--threads", optarg, 1, 65536);
+			break;
+		case OPT_DEVICE:
+			a->device = (int)parse_int("--device", optarg, 0, 1024);
+			break;
+		case OPT_HELP:
+			fputs(ani_help, stdout);
+			exit(0);
+		case OPT_NOT_OFFERED:
+			usage_error("%s is not offered by this front end (sketch directories only)", argv[optind - 1]);
+		default:
+			fprintf(stderr, "Try `%s %s --help' for more information.\n", PROG, g_cmd);
+			exit(EX_USAGE);
+		}
+	}
+	/* end-of-arguments checks, command_ani_wrapper.c:538-620 */
+	if (a->values)
+		o->s = abs(o->s) * a->values;
+	const int npos = argc - optind;
+	if (o->unique_ctx) {
+		if (!o->unassembled || o->fmt != 0 || !a->refdir[0] || !a->qrydir[0] || npos)
+			usage_error("unique-context options require explicit -r/--qraw inputs and detail output");
+		if (o->unified_metric || o->raw_output || abs(o->s) > 4 || o->max_pdist_set ||
+		    o->max_diff_obj_section >= 0)
+			usage_error("--unique-ctx uses calibrated read-mode Naive ANI; incompatible metric, raw-output, "
+				    "unified-metric or close-pair filter");
+	}
+	if ((o->max_pdist_set || o->max_diff_obj_section >= 0) && o->fmt != 0)
+		usage_error("--max-pdist/--max-diff-section are supported only with detail output (-m0).");
+	if (!a->refdir[0] && !a->qrydir[0] && npos >= 2)
+		usage_error("positional reference/query inputs (auto-sketch) are not offered by this front end");
+	else if (!a->refdir[0] && !a->qrydir[0] && npos == 1 && o->fmt != 0)
+		copy_path(a->qrydir, sizeof a->qrydir, "sketch", argv[optind]);
+	else if (!a->qrydir[0])
+		usage_error("missing query: use -q, --qraw, or positional reference and query inputs");
+	else if (!a->refdir[0] && o->fmt == 0)
+		usage_error("-r/--ref is required for detail output; use -m1 or -m2 for one-sketch self matrices");
+	else if (npos)
+		usage_error("unexpected argument '%s'", argv[optind]);
+}
+
+/* ------------------------------------------------------------ output */
+static FILE *open_out(const ani_args_t *a)
+{
+	if (!a->outf[0])
+		return stdout;
+	FILE *fp = fopen(a->outf, "w");
+	if (!fp)
+		err(errno, "%s", a->outf);
+	return fp;
+}
+
+static void close_out(FILE *fp)
+{
+	if (fflush(fp) != 0 || ferror(fp))
+		err(EXIT_FAILURE, "Cannot write ANI output");
+	if (fp != stdout && fclose(fp) != 0)
+		err(EXIT_FAILURE, "Cannot finish ANI output");
+}
+
+static void put_buf(FILE *fp, kssd_buf_t *b)
+{
+	if (b->l && fwrite(b->s, 1, b->l, fp) != b->l)
+		err(EXIT_FAILURE, "Cannot write ANI output");
+	b->l = 0;
+}
+
+static void put_header_names(FILE *fp, const kssd_sketchdir_t *s)
+{
+	for (int i = 0; i < s->n; ++i)
+		fprintf(fp, "\t%s", s->names[i]);
+	fputc('\n', fp);
+}
+
+/* ------------------------------------------------------------ GPU counting */
+/* Cells (query x reference) per counting block: dense counters are 16 B per
+ * cell (32 B with the unique-owner copies); the block keeps them near 512 MiB. */
+static int block_queries(int nq, int nref, bool unique)
+{
+	const size_t budget = (size_t)512 << 20;
+	const size_t per_q = (size_t)(nref ? nref : 1) * (unique ? 32u : 16u);
+	size_t b = budget / per_q;
+	if (b < 1)
+		b = 1;
+	return b < (size_t)nq ? (int)b : nq;
+}
+
+static kssdhip_index *gpu_index(const kssd_ctxgidobj_t *rec, size_t n, int gnum, unsigned obj_bits)
+{
+	kssdhip_index *ix = NULL;
+	hip_check(kssdhip_index_open(rec, n, gnum, obj_bits, &ix), "cannot open the index on the GPU");
+	return ix;
+}
+
+static kssd_ctxgidobj_t *build_records(const kssd_sketchdir_t *s, int threads)
+{
+	const size_t n = s->index[s->n];
+	kssd_ctxgidobj_t *rec = xmalloc(n * sizeof *rec);
+	if (kssd_build_index(s->keys, s->index, s->n, s->obj_bits, threads, rec) != 0)
+		errx(EXIT_FAILURE, "cannot index %d samples with %zu entries (limits: 2^20 samples, 2^32 entries)",
+		     s->n, n);
+	return rec;
+}
+
+/* gen_inverted_index4comblco (command_sketch.c:916-938): write the index
+ * and its unique-marker bits next to the sketch. */
+static void write_index_for(const char *dir, int threads)
+{
+	kssd_sketchdir_t s;
+	open_dir(dir, KSSD_SK_KEYS, &s);
+	kssd_ctxgidobj_t *rec = build_records(&s, threads);
+	if (kssd_write_sorted_index(dir, rec, s.index[s.n]) != 0)
+		err(EXIT_FAILURE, "%s/%s", dir, KSSD_SORTED_INDEX_FILE);
+	free(rec);
+	kssd_sketchdir_close(&s);
+}
+
+/* Keys of `s` without the contexts that carry more than one object (the
+ * reference side of --ignoreconflict, command_ani.c:2905-2909). */
+static uint64_t *drop_conflict_runs(const kssd_sketchdir_t *s, uint64_t *index_out)
+{
+	uint64_t *out = xmalloc(s->index[s->n] * sizeof *out);
+	size_t w = 0;
+	index_out[0] = 0;
+	for (int g = 0; g < s->n; ++g) {
+		const uint64_t *a = s->keys + s->index[g];
+		const size_t len = (size_t)(s->index[g + 1] - s->index[g]);
+		for (size_t i = 0; i < len;) {
+			const uint64_t c = a[i] >> s->obj_bits;
+			const size_t b = i;
+			do {
+				++i;
+			} while (i < len && (a[i] >> s->obj_bits) == c);
+			if (i - b == 1)
+				out[w++] = a[b];
+		}
+		index_out[g + 1] = w;
+	}
+	return out;
+}
+
+/*
+ * Counts of every (query, reference) pair into ctx/obj [Q][R].  The side
+ * with fewer entries becomes the device index and the other side streams
+ * through it in blocks; the four counts are symmetric in the two sketches,
+ * and the reference-side --ignoreconflict rule is applied either by the
+ * kernel flag (reference indexed) or by dropping those contexts from the
+ * streamed reference keys.
+ */
+static void rect_counts(const kssd_sketchdir_t *ref, const kssd_sketchdir_t *qry, bool ignoreconflict,
+			int threads, kssd_ctx_mut2_t *ctx, kssd_obj_section_t *obj)
+{
+	const int R = ref->n, Q = qry->n;
+	const bool ignore = ignoreconflict && ref->stat.conflict;
+	if (R == 0 || Q == 0)
+		return;
+	const bool index_ref = ref->index[R] <= qry->index[Q];
+	if (index_ref) {
+		kssd_ctxgidobj_t *rec = build_records(ref, threads);
+		kssdhip_index *ix = gpu_index(rec, ref->index[R], R, ref->obj_bits);
+		free(rec);
+		const int blk = block_queries(Q, R, false);
+		for (int q0 = 0; q0 < Q; q0 += blk) {
+			const int nb = Q - q0 < blk ? Q - q0 : blk;
+			hip_check(kssdhip_count_block(ix, qry->keys + qry->index[q0], qry->index + q0, nb, q0,
+						      ignore ? KSSDHIP_IGNORECONFLICT : 0, ctx + (size_t)q0 * R,
+						      obj + (size_t)q0 * R, NULL, NULL),
+				  "GPU counting failed");
+		}
+		kssdhip_index_close(ix);
+		return;
+	}
+	kssd_ctxgidobj_t *rec = build_records(qry, threads);
+	kssdhip_index *ix = gpu_index(rec, qry->index[Q], Q, qry->obj_bits);
+	free(rec);
+	uint64_t *rindex = NULL, *rkeys = ref->keys;
+	if (ignore) {
+		rindex = xmalloc(((size_t)R + 1) * sizeof *rindex);
+		rkeys = drop_conflict_runs(ref, rindex);
+	}
+	const uint64_t *ridx = ignore ? rindex : ref->index;
+	const int blk = block_queries(R, Q, false);
+	kssd_ctx_mut2_t *tc = xmalloc((size_t)blk * Q * sizeof *tc);
+	kssd_obj_section_t *to = xmalloc((size_t)blk * Q * sizeof *to);
+	for (int r0 = 0; r0 < R; r0 += blk) {
+		const int nb = R - r0 < blk ? R - r0 : blk;
+		hip_check(kssdhip_count_block(ix, rkeys + ridx[r0], ridx + r0, nb, r0, 0, tc, to, NULL, NULL),
+			  "GPU counting failed");
+		for (int i = 0; i < nb; ++i)
+			for (int q = 0; q < Q; ++q) {
+				ctx[(size_t)q * R + r0 + i] = tc[(size_t)i * Q + q];
+				obj[(size_t)q * R + r0 + i] = to[(size_t)i * Q + q];
+			}
+	}
+	free(tc);
+	free(to);
+	if (ignore) {
+		free(rkeys);
+		free(rindex);
+	}
+	kssdhip_index_close(ix);
+}
+
+/* ------------------------------------------------------------ context totals */
+static uint32_t *entry_or_run_totals(const kssd_sketchdir_t *s, bool runs, bool ignoreconflict)
+{
+	uint32_t *t = xmalloc((size_t)(s->n ? s->n : 1) * sizeof *t);
+	kssd_sample_ctx_counts(s->keys, s->index, s->n, s->obj_bits, runs, ignoreconflict, t);
+	return t;
+}
+
+/* ------------------------------------------------------------ detail paths */
+typedef enum { PATH_INDEX, PATH_STREAM, PATH_MULTISTREAM, PATH_HASH } detail_path_t;
+
+static void emit_rows(FILE *fp, const ani_args_t *a, detail_path_t path, const kssd_sketchdir_t *ref,
+		      const kssd_sketchdir_t *qry, int q0, int nq, const kssd_ctx_mut2_t *ctx,
+		      const kssd_obj_section_t *obj, const kssd_ctx_mut2_t *uctx, const kssd_obj_section_t *uobj,
+		      const uint32_t *qry_ctx, const uint32_t *ref_ctx)
+{
+	const int R = ref->n;
+	kssd_buf_t *bufs = xcalloc((size_t)nq, sizeof *bufs);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
+	for (int i = 0; i < nq; ++i) {
+		const int q = q0 + i;
+		kssd_query_counts_t c = {
+			.ctx = ctx + (size_t)i * R,
+			.obj = obj + (size_t)i * R,
+			.uctx = uctx ? uctx + (size_t)i * R : NULL,
+			.uobj = uobj ? uobj + (size_t)i * R : NULL,
+			.qry_ctx = qry_ctx[q],
+			.ref_ctx = ref_ctx,
+			.nref = R,
+			.qry_meta = qry->meta ? &qry->meta[q] : NULL,
+			.ref_meta = ref->meta,
+		};
+		switch (path) {
+		case PATH_INDEX:
+			if (a->r.fmt)
+				kssd_emit_index_matrix_row(&bufs[i], &a->r, &c, qry->names[q]);
+			else
+				kssd_emit_index_detail(&bufs[i], &a->r, &c, qry->names[q], ref->names, ref->anno);
+			break;
+		case PATH_STREAM:
+		case PATH_MULTISTREAM:
+			kssd_emit_stream_detail(&bufs[i], &a->r, &c, qry->names[q], ref->names, ref->anno,
+						path == PATH_STREAM);
+			break;
+		case PATH_HASH:
+			kssd_emit_hash_detail(&bufs[i], &a->r, &c, qry->names[q], ref->names, ref->anno);
+			break;
+		}
+	}
+	for (int i = 0; i < nq; ++i) {
+		put_buf(fp, &bufs[i]);
+		kssd_buf_free(&bufs[i]);
+	}
+	free(bufs);
+}
+
+/* mem_eff_sorted_ctxgidobj_arrXcomb_sortedsketch64, command_ani.c:494-692 */
+static int run_index_path(const ani_args_t *a)
+{
+	kssd_sketchdir_t ref, qry;
+	open_dir(a->refdir, KSSD_SK_RECORDS | KSSD_SK_ANNO | KSSD_SK_META, &ref);
+	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META, &qry);
+	if (!ref.records)
+		errx(EXIT_FAILURE, "%s/%s: reference index missing", a->refdir, KSSD_SORTED_INDEX_FILE);
+	if (qry.stat.hash_id != ref.stat.hash_id)
+		errx(EXIT_FAILURE, "hash_id mismatch between query and reference sketches");
+	stage("sketches read");
+	const int R = ref.n, Q = qry.n;
+	const size_t nrec = ref.index[R];
+	uint32_t *ref_ctx = xmalloc((size_t)(R ? R : 1) * sizeof *ref_ctx);
+	if (ref.stat.conflict)
+		kssd_index_ctx_counts(ref.records, nrec, ref.index, R, a->r.ignoreconflict, ref_ctx);
+	else
+		for (int g = 0; g < R; ++g)
+			ref_ctx[g] = (uint32_t)(ref.index[g + 1] - ref.index[g]);
+	uint32_t *qry_ctx = entry_or_run_totals(&qry, qry.stat.conflict, false);
+	kssdhip_index *ix = gpu_index(ref.records, nrec, R, ref.obj_bits);
+	stage("index on the GPU");
+
+	FILE *fp = open_out(a);
+	if (a->r.fmt)
+		put_header_names(fp, &ref);
+	else
+		fprintf(fp, "%s\n", kssd_detail_header);
+	const bool unique = a->r.unique_ctx;
+	const uint32_t flags = (unique ? KSSDHIP_UNIQUE : 0) | (a->r.ignoreconflict ? KSSDHIP_IGNORECONFLICT : 0);
+	const int blk = block_queries(Q, R, unique);
+	const size_t cells = (size_t)blk * (size_t)(R ? R : 1);
+	kssd_ctx_mut2_t *ctx = xmalloc(cells * sizeof *ctx);
+	kssd_obj_section_t *obj = xmalloc(cells * sizeof *obj);
+	kssd_ctx_mut2_t *uctx = unique ? xmalloc(cells * sizeof *uctx) : NULL;
+	kssd_obj_section_t *uobj = unique ? xmalloc(cells * sizeof *uobj) : NULL;
+	for (int q0 = 0; q0 < Q; q0 += blk) {
+		const int nb = Q - q0 < blk ? Q - q0 : blk;
+		if (R > 0)
+			hip_check(kssdhip_count_block(ix, qry.keys + qry.index[q0], qry.index + q0, nb, q0, flags, ctx,
+						      obj, uctx, uobj),
+				  "GPU counting failed");
+		stage("block counted");
+		emit_rows(fp, a, PATH_INDEX, &ref, &qry, q0, nb, ctx, obj, uctx, uobj, qry_ctx, ref_ctx);
+		stage("block ranked and formatted");
+	}
+	close_out(fp);
+	kssdhip_index_close(ix);
+	free(ctx);
+	free(obj);
+	free(uctx);
+	free(uobj);
+	free(ref_ctx);
+	free(qry_ctx);
+	kssd_sketchdir_close(&ref);
+	kssd_sketchdir_close(&qry);
+	return 0;
+}
+
+/*
+ * Streamed paths (stream_ref_sketches_one_qraw_lookup, :2490-2666;
+ * stream_ref_sketches_multi_qraw_sortedindex, :2792-3015) and the hash path
+ * (comb_manysmall_..._filter_and_sort_survivors, :3888-4000): all pairs are
+ * counted on the GPU, then ranked per query as the matching path does.
+ */
+static int run_rect_detail(const ani_args_t *a, detail_path_t path)
+{
+	kssd_sketchdir_t ref, qry;
+	open_dir(a->refdir, KSSD_SK_KEYS | KSSD_SK_ANNO | KSSD_SK_META, &ref);
+	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META, &qry);
+	if (qry.stat.hash_id != ref.stat.hash_id)
+		errx(EXIT_FAILURE, "hash_id mismatch between query and reference sketches");
+	stage("sketches read");
+	const int R = ref.n, Q = qry.n;
+	uint32_t *qry_ctx, *ref_ctx;
+	if (path == PATH_HASH) {
+		qry_ctx = entry_or_run_totals(&qry, qry.stat.conflict, false);
+		ref_ctx = entry_or_run_totals(&ref, ref.stat.conflict, a->r.ignoreconflict);
+	} else {
+		/* the streamed paths count context runs whatever the conflict flag */
+		qry_ctx = entry_or_run_totals(&qry, true, false);
+		ref_ctx = entry_or_run_totals(&ref, true, a->r.ignoreconflict);
+	}
+	kssd_ctx_mut2_t *ctx = xcalloc((size_t)Q * R, sizeof *ctx);
+	kssd_obj_section_t *obj = xcalloc((size_t)Q * R, sizeof *obj);
+	/* the streamed paths skip multi-object reference contexts under
+	 * --ignoreconflict whatever the conflict flag; without conflicts there
+	 * are none, so the flag-gated rule of rect_counts is the same */
+	rect_counts(&ref, &qry, a->r.ignoreconflict, a->p, ctx, obj);
+	stage("pairs counted");
+	FILE *fp = open_out(a);
+	fprintf(fp, "%s\n", kssd_detail_header);
+	emit_rows(fp, a, path, &ref, &qry, 0, Q, ctx, obj, NULL, NULL, qry_ctx, ref_ctx);
+	stage("ranked and formatted");
+	close_out(fp);
+	free(ctx);
+	free(obj);
+	free(qry_ctx);
+	free(ref_ctx);
+	kssd_sketchdir_close(&ref);
+	kssd_sketchdir_close(&qry);
+	return 0;
+}
+
+/* ------------------------------------------------------------ matrices */
+/*
+ * Self matrices: comb_sortedsketch64_self_matrix (:3314-3365, "%lf" cells)
+ * and comb_sortedsketch64_indexed_self_full / _triangle (:1657-1842; the
+ * full matrix prints fixed-6 cells).  Pair (q, r) with q > r is evaluated
+ * with q as the query; the upper half mirrors it.
+ */
+static int run_self_matrix(const ani_args_t *a, bool fixed6, bool index_from_file)
+{
+	kssd_sketchdir_t s;
+	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META | (index_from_file ? KSSD_SK_RECORDS : 0), &s);
+	stage("sketch read");
+	const int n = s.n;
+	const kssd_report_opts_t *o = &a->r;
+	uint32_t *qc = entry_or_run_totals(&s, s.stat.conflict, false);
+	uint32_t *rc = o->ignoreconflict ? entry_or_run_totals(&s, s.stat.conflict, true) : qc;
+	kssd_ctxgidobj_t *built = s.records ? NULL : build_records(&s, a->p);
+	const kssd_ctxgidobj_t *rec = s.records ? s.records : built;
+	kssdhip_index *ix = n ? gpu_index(rec, s.index[n], n, s.obj_bits) : NULL;
+	free(built);
+	stage("index on the GPU");
+	const size_t lower_n = n > 1 ? (size_t)n * (size_t)(n - 1) / 2 : 0;
+	double *lower = xmalloc(lower_n * sizeof *lower);
+	const uint32_t flags = KSSDHIP_LOWER | (o->ignoreconflict ? KSSDHIP_IGNORECONFLICT : 0);
+	const int blk = block_queries(n, n, false);
+	kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * (n ? n : 1) * sizeof *ctx);
+	kssd_obj_section_t *obj = xmalloc((size_t)blk * (n ? n : 1) * sizeof *obj);
+	for (int q0 = 0; q0 < n; q0 += blk) {
+		const int nb = n - q0 < blk ? n - q0 : blk;
+		hip_check(kssdhip_count_block(ix, s.keys + s.index[q0], s.index + q0, nb, q0, flags, ctx, obj, NULL,
+					      NULL),
+			  "GPU counting failed");
+#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
+		for (int i = 0; i < nb; ++i) {
+			const int q = q0 + i;
+			for (int r = 0; r < q; ++r) {
+				const kssd_ctx_mut2_t *c = &ctx[(size_t)i * n + r];
+				const kssd_obj_section_t *b = &obj[(size_t)i * n + r];
+				lower[(size_t)q * (q - 1) / 2 + r] =
+					kssd_matrix_value(o, c->num_ctx, b->diff_obj, b->diff_obj_section, c->num_mut2_ctx,
+							  qc[q], rc[r], s.meta ? &s.meta[q] : NULL, s.meta ? &s.meta[r] : NULL);
+			}
+		}
+	}
+	stage("pairs counted and evaluated");
+	FILE *fp = open_out(a);
+	kssd_buf_t b = {0};
+	const double diag = kssd_matrix_diagonal(o);
+	if (o->fmt == 1) {
+		put_header_names(fp, &s);
+		for (int q = 0; q < n; ++q) {
+			kssd_buf_printf(&b, "%s", s.names[q]);
+			for (int r = 0; r < n; ++r) {
+				const double v = q == r ? diag
+						 : q > r ? lower[(size_t)q * (q - 1) / 2 + r]
+							 : lower[(size_t)r * (r - 1) / 2 + q];
+				if (fixed6) {
+					kssd_buf_printf(&b, "\t");
+					kssd_put_fixed6(&b, kssd_scale_fixed6(v));
+				} else {
+					kssd_buf_printf(&b, "\t%lf", v);
+				}
+			}
+			kssd_buf_printf(&b, "\n");
+			put_buf(fp, &b);
+		}
+	} else {
+		for (int q = 0; q < n; ++q) {
+			kssd_buf_printf(&b, "%s", s.names[q]);
+			for (int r = 0; r < q; ++r)
+				kssd_buf_printf(&b, "\t%lf", lower[(size_t)q * (q - 1) / 2 + r]);
+			if (o->d)
+				kssd_buf_printf(&b, "\t%lf", diag);
+			kssd_buf_printf(&b, "\n");
+			put_buf(fp, &b);
+		}
+	}
+	kssd_buf_free(&b);
+	stage("formatted");
+	close_out(fp);
+	if (ix)
+		kssdhip_index_close(ix);
+	free(ctx);
+	free(obj);
+	free(lower);
+	if (rc != qc)
+		free(rc);
+	free(qc);
+	kssd_sketchdir_close(&s);
+	return 0;
+}
+
+/* comb_sortedsketch64Xcomb_sortedsketch64, matrix branch (:3210-3245) */
+static int run_rect_matrix(const ani_args_t *a)
+{
+	kssd_sketchdir_t ref, qry;
+	open_dir(a->refdir, KSSD_SK_KEYS | KSSD_SK_META, &ref);
+	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META, &qry);
+	stage("sketches read");
+	const int R = ref.n, Q = qry.n;
+	const kssd_report_opts_t *o = &a->r;
+	uint32_t *qc = entry_or_run_totals(&qry, qry.stat.conflict, false);
+	uint32_t *rc = entry_or_run_totals(&ref, ref.stat.conflict, o->ignoreconflict);
+	kssd_ctx_mut2_t *ctx = xcalloc((size_t)Q * R, sizeof *ctx);
+	kssd_obj_section_t *obj = xcalloc((size_t)Q * R, sizeof *obj);
+	rect_counts(&ref, &qry, o->ignoreconflict, a->p, ctx, obj);
+	stage("pairs counted");
+	FILE *fp = open_out(a);
+	put_header_names(fp, &ref);
+	kssd_buf_t *bufs = xcalloc((size_t)Q, sizeof *bufs);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
+	for (int q = 0; q < Q; ++q) {
+		kssd_buf_printf(&bufs[q], "%s", qry.names[q]);
+		for (int r = 0; r < R; ++r) {
+			const kssd_ctx_mut2_t *c = &ctx[(size_t)q * R + r];
+			const kssd_obj_section_t *b = &obj[(size_t)q * R + r];
+			kssd_buf_printf(&bufs[q], "\t%lf",
+					kssd_matrix_value(o, c->num_ctx, b->diff_obj, b->diff_obj_section, c->num_mut2_ctx,
+							  qc[q], rc[r], qry.meta ? &qry.meta[q] : NULL,
+							  ref.meta ? &ref.meta[r] : NULL));
+		}
+		kssd_buf_printf(&bufs[q], "\n");
+	}
+	for (int q = 0; q < Q; ++q) {
+		put_buf(fp, &bufs[q]);
+		kssd_buf_free(&bufs[q]);
+	}
+	free(bufs);
+	stage("formatted");
+	close_out(fp);
+	free(ctx);
+	free(obj);
+	free(qc);
+	free(rc);
+	kssd_sketchdir_close(&ref);
+	kssd_sketchdir_close(&qry);
+	return 0;
+}
+
+/* ------------------------------------------------------------ dispatch */
+static bool matrix_should_use_index(int ref_n, int qry_n, int direct_threshold)
+{
+	/* ani_matrix_should_use_index, command_ani_wrapper.c:699-706 */
+	if (ref_n < 1 || qry_n < 1)
+		return false;
+	if (direct_threshold == 0)
+		return true;
+	return ref_n > direct_threshold || qry_n > direct_threshold;
+}
+
+static int matrix_direct_threshold(void)
+{
+	const char *name = "KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD";
+	if (!getenv(name) || !getenv(name)[0])
+		name = "KSSD3A_ANI_M1_DIRECT_THRESHOLD";
+	return (int)env_int(name, 1000, 0, "non-negative integer");
+}
+
+static void ensure_index(const char *dir, int threads)
+{
+	if (kssd_file_in_dir(dir, KSSD_SORTED_INDEX_FILE))
+		return;
+	write_index_for(dir, threads);
+	if (!kssd_file_in_dir(dir, KSSD_SORTED_INDEX_FILE))
+		errx(EXIT_FAILURE, "failed to create reference index '%s/%s'", dir, KSSD_SORTED_INDEX_FILE);
+}
+
+static void __attribute__((noreturn)) force_missing(const char *dir)
+{
+	errx(EXIT_FAILURE, "KSSD3A_FORCE_REF_INDEX requires reference index '%s/%s'", dir, KSSD_SORTED_INDEX_FILE);
+}
+
+/* run_sketch_ani, command_ani_wrapper.c:1082-1230 */
+static int run_ani(ani_args_t *a)
+{
+	kssd_report_opts_t *o = &a->r;
+	if (o->unique_ctx && !kssd_file_in_dir(a->refdir, KSSD_SORTED_INDEX_FILE))
+		errx(EXIT_FAILURE, "unique-context options require an existing unfiltered reference inverted index; "
+				   "run sketch -i on the reference sketch");
+	if (o->unassembled && !a->afcut_set)
+		o->afcut = 0.2f;
+	kssd_sketchdir_t q;
+	open_dir(a->qrydir, 0, &q);
+	const int qn = q.n;
+	o->drfold = q.stat.drfold;
+	o->ctx_bits = q.ctx_bits;
+	o->obj_bits = q.obj_bits;
+	kssd_sketchdir_close(&q);
+	const bool force = env_flag("KSSD3A_FORCE_REF_INDEX") || o->unique_ctx;
+	const int direct_n = o->fmt ? matrix_direct_threshold() : 0;
+
+	if (!a->refdir[0] || (o->fmt && same_file(a->refdir, a->qrydir))) {
+		if (o->fmt == 0)
+			errx(EXIT_FAILURE, "one-sketch ANI detail output is not supported; use -m1 full matrix or -m2 triangle");
+		const bool indexed = force || matrix_should_use_index(qn, qn, direct_n);
+		const bool have = kssd_file_in_dir(a->qrydir, KSSD_SORTED_INDEX_FILE);
+		if (o->fmt == 1 && indexed) {
+			if (force && !have)
+				force_missing(a->qrydir);
+			ensure_index(a->qrydir, a->p);
+			return run_self_matrix(a, true, true);
+		}
+		if (o->fmt == 2 && indexed) {
+			if (force && !have)
+				force_missing(a->qrydir);
+			return run_self_matrix(a, false, have);
+		}
+		return run_self_matrix(a, false, false);
+	}
+	if (o->fmt == 2)
+		errx(EXIT_FAILURE, "ANI triangle output requires one sketch or identical -r/-q sketches");
+	kssd_sketchdir_t r;
+	open_dir(a->refdir, 0, &r);
+	const int rn = r.n;
+	kssd_sketchdir_close(&r);
+	const bool small_query_stream = !force && o->fmt == 0 && rn > 0 && qn < rn;
+	if (small_query_stream && qn == 1)
+		return run_rect_detail(a, PATH_STREAM);
+	if (o->fmt == 0 && qn > 1) {
+		bool use_query_index =
+			small_query_stream && qn <= env_int("KSSD3A_QRAW_MULTI_THRESHOLD", 20, 1, "positive integer");
+		const char *mode = getenv("KSSD3A_QRAW_MULTI");
+		if (o->unassembled && !force && mode && mode[0]) {
+			if (strcmp(mode, "sortedindex") == 0)
+				use_query_index = true;
+			else if (strcmp(mode, "legacy") == 0 || strcmp(mode, "refindex") == 0)
+				use_query_index = false;
+			else
+				errx(EXIT_FAILURE, "invalid KSSD3A_QRAW_MULTI='%s' (use sortedindex, refindex, or legacy)",
+				     mode);
+		}
+		if (use_query_index)
+			return run_rect_detail(a, PATH_MULTISTREAM);
+	}
+	const bool ref_index = kssd_file_in_dir(a->refdir, KSSD_SORTED_INDEX_FILE);
+	if (o->fmt == 1) {
+		if (force) {
+			if (!ref_index)
+				force_missing(a->refdir);
+			return run_index_path(a);
+		}
+		if (matrix_should_use_index(rn, qn, direct_n)) {
+			ensure_index(a->refdir, a->p);
+			return run_index_path(a);
+		}
+	} else if (ref_index) {
+		return run_index_path(a);
+	}
+	const long auto_n = env_int("KSSD3A_AUTO_REF_INDEX_THRESHOLD", 100, 0, "non-negative integer; 0 disables");
+	if (!o->v && !o->unassembled && o->fmt == 0 && auto_n > 0 && rn > auto_n) {
+		write_index_for(a->refdir, a->p);
+		return run_index_path(a);
+	}
+	if (force)
+		force_missing(a->refdir);
+	if (o->fmt)
+		return run_rect_matrix(a);
+	return run_rect_detail(a, PATH_HASH);
+}
+
+static int cmd_ani(int argc, char **argv)
+{
+	ani_args_t a;
+	parse_ani(argc, argv, &a);
+	if (kssd_report_init() != 0)
+		errx(EXIT_FAILURE, "calibrated ANI model tables are damaged");
+	hip_check(kssdhip_init(a.device), "no usable GPU");
+	stage("GPU ready");
+	return run_ani(&a);
+}
+
+/* ------------------------------------------------------------ sketch -i */
+static int cmd_sketch(int argc, char **argv)
+{
+	static const struct option longopts[] = {
+		{"idx", required_argument, NULL, 'i'},
+		{"threads", required_argument, NULL, 'p'},
+		{"help", no_argument, NULL, OPT_HELP},
+		{NULL, 0, NULL, 0},
+	};
+	const char *dir = NULL;
+	int threads = omp_get_max_threads();
+	optind = 1;
+	int c;
+	while ((c = getopt_long(argc, argv, "i:p:", longopts, NULL)) != -1) {
+		switch (c) {
+		case 'i':
+			dir = optarg;
+			break;
+		case 'p':
+			threads = (int)parse_int("-p/--threads", optarg, 1, 65536);
+			break;
+		case OPT_HELP:
+			printf("Usage: %s sketch -i SKETCH_DIR [-p THREADS]\n"
+			       "Build SKETCH_DIR/%s and its .unique_bits.\n",
+			       PROG, KSSD_SORTED_INDEX_FILE);
+			return 0;
+		default:
+			fprintf(stderr, "Try `%s %s --help' for more information.\n", PROG, g_cmd);
+			exit(EX_USAGE);
+		}
+	}
+	if (!dir)
+		usage_error("only `sketch -i SKETCH_DIR` (inverted index) is offered; sketch sequences with kssd3a");
+	if (optind != argc)
+		usage_error("unexpected argument '%s'", argv[optind]);
+	write_index_for(dir, threads);
+	stage("index written");
+	return 0;
+}
+
+int main(int argc, char **argv)
+{
+	g_t0 = now_s();
+	g_timing = env_flag("KSSDHIP_TIMING");
+	if (argc < 2 || strcmp(argv[1], "--help") == 0 || strcmp(argv[1], "-h") == 0) {
+		printf("Usage: %s ani|sketch [options]\n", PROG);
+		return argc < 2 ? EX_USAGE : 0;
+	}
+	g_cmd = argv[1];
+	if (strcmp(g_cmd, "ani") == 0)
+		return cmd_ani(argc - 1, argv + 1);
+	if (strcmp(g_cmd, "sketch") == 0)
+		return cmd_sketch(argc - 1, argv + 1);
+	fprintf(stderr, "%s: unknown command '%s' (ani, sketch)\n", PROG, g_cmd);
+	return EX_USAGE;
+}

@@ -91,6 +91,10 @@ int kssd_build_index(const uint64_t *keys, const uint64_t *index, int n, unsigne
 uint32_t kssd_ctx_runs(const uint64_t *a, size_t n, unsigned obj_bits, int ignoreconflict);
 void kssd_sample_ctx_counts(const uint64_t *keys, const uint64_t *index, int n, unsigned obj_bits,
 			    int conflict, int ignoreconflict, uint32_t *counts);
+/* The same totals for a conflict-keeping reference, read off its inverted
+ * index (ref_ctx_counts_from_ctxgidobj, command_ani.c:351-370). */
+void kssd_index_ctx_counts(const kssd_ctxgidobj_t *rec, size_t n, const uint64_t *index, int ref_gnum,
+			   int ignoreconflict, uint32_t *counts);
 
 #ifdef __cplusplus
 }

@@ -121,3 +121,23 @@ void kssd_sample_ctx_counts(const uint64_t *keys, const uint64_t *index, int n, 
 				     : (uint32_t)len;
 	}
 }
+
+void kssd_index_ctx_counts(const kssd_ctxgidobj_t *rec, size_t n, const uint64_t *index, int ref_gnum,
+			   int ignoreconflict, uint32_t *counts)
+{
+	/* ref_ctx_counts_from_ctxgidobj, command_ani.c:351-370: entries per gid,
+	 * minus the extra objects of each multi-object run (the whole run under
+	 * ignoreconflict) */
+	for (int g = 0; g < ref_gnum; ++g)
+		counts[g] = (uint32_t)(index[g + 1] - index[g]);
+	for (size_t i = 0; i < n;) {
+		const uint64_t cg = rec[i].ctxgid;
+		size_t len = 0;
+		do {
+			++i;
+			++len;
+		} while (i < n && rec[i].ctxgid == cg);
+		if (len > 1)
+			counts[cg & ((1u << 20) - 1)] -= (uint32_t)(ignoreconflict ? len : len - 1);
+	}
+}

@@ -6,6 +6,7 @@
 #include "kssd_report.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdlib.h>
 #include <string.h>
@@ -60,17 +61,16 @@ typedef struct {
 } refaf_model_t;
 
 static refaf_model_t g_model;
-static int g_model_ok = -1;
+static int g_model_ok;
+static pthread_once_t g_model_once = PTHREAD_ONCE_INIT;
 
-static int refaf_load(void)
+static void refaf_parse(void)
 {
-	if (g_model_ok >= 0)
-		return g_model_ok;
 	const unsigned char *p = kssd_refaf_blob;
 	const size_t size = (size_t)(kssd_refaf_blob_end - kssd_refaf_blob);
 	g_model_ok = 0;
 	if (size < 32 || memcmp(p, "KSSDHGB1", 8) != 0)
-		return 0;
+		return;
 	uint32_t hdr[4];
 	memcpy(hdr, p + 8, sizeof hdr);
 	refaf_model_t m;
@@ -81,7 +81,7 @@ static int refaf_load(void)
 	size_t off = 32;
 	const size_t need = off + 4 * ((size_t)m.n_trees + 1) + (size_t)m.n_nodes * (8 + 8 + 2 + 2 + 1 + 1 + 1);
 	if (m.n_features != 11 || size < need)
-		return 0;
+		return;
 	/* the blob is 8-byte aligned and laid out so every array is naturally aligned */
 	m.offsets = (const uint32_t *)(p + off);
 	off += 4 * ((size_t)m.n_trees + 1);
@@ -101,7 +101,17 @@ static int refaf_load(void)
 	m.missing_left = (const uint8_t *)(p + off);
 	g_model = m;
 	g_model_ok = 1;
-	return 1;
+}
+
+static int refaf_load(void)
+{
+	pthread_once(&g_model_once, refaf_parse);
+	return g_model_ok;
+}
+
+int kssd_report_init(void)
+{
+	return refaf_load() ? 0 : -1;
 }
 
 static inline double clamp01(double x)
@@ -725,10 +735,11 @@ static void emit_sorted(kssd_buf_t *out, const kssd_report_opts_t *o, kssd_row_t
 
 void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
 			     const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
-			     char (*ref_anno)[KSSD_PATHLEN])
+			     char (*ref_anno)[KSSD_PATHLEN], int single)
 {
-	/* compute_streamed_ref_row_one_qry, command_ani.c:2110-2201; order and
-	 * top N as stream_ref_sketches_one_qraw_lookup :2634-2650 */
+	/* one query: compute_streamed_ref_row_one_qry, command_ani.c:2110-2201;
+	 * several: the per-reference loop of command_ani.c:2892-2968 (no AF or
+	 * close-filter early exit).  Order and top N as :2634-2650, :2983-2998 */
 	kssd_row_t *rows = malloc((size_t)(c->nref > 0 ? c->nref : 1) * sizeof *rows);
 	if (!rows)
 		abort();
@@ -741,7 +752,7 @@ void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const
 		const uint32_t need = af_needed_ctx(o, qc, rc);
 		const uint32_t xny = c->ctx[j].num_ctx, mut2 = c->ctx[j].num_mut2_ctx;
 		const uint32_t dobj = c->obj[j].diff_obj, dsec = c->obj[j].diff_obj_section;
-		if (xny < need || close_filter_impossible(o, dsec, xny))
+		if (single && (xny < need || close_filter_impossible(o, dsec, xny)))
 			continue;
 		if (xny < (uint32_t)o->ctxcut)
 			continue;
@@ -752,7 +763,7 @@ void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const
 		const double br = kssd_af_anib(xny, dobj, dsec, mut2, rc, o->drfold);
 		kssd_row_t row = kssd_make_row((uint32_t)j, xny, dobj, dsec, mut2, o, qc, rc, afq, bq, afr, br,
 					       c->qry_meta, c->ref_meta ? &c->ref_meta[j] : NULL);
-		if (!passes_close_filter(&row, o) || row.selected_ani < o->anicut)
+		if ((single && !passes_close_filter(&row, o)) || row.selected_ani < o->anicut)
 			continue;
 		rows[n++] = row;
 	}

@@ -81,6 +81,8 @@ typedef struct {
 	int XnY_ctx, N_diff_obj, N_diff_obj_section, N_mut2_ctx;
 } kssd_row_t;
 
+/* Load the calibrated model's tables (idempotent, thread-safe); 0 or -1. */
+int kssd_report_init(void);
 /* calibrated ANI model (refaf_hgb_predict_ani, model_refaf_hgb.h:7593-7628) */
 double kssd_refaf_predict_ani(double raw_ani, double ref_af, unsigned xny, unsigned n_diff_obj,
 			      unsigned n_diff_obj_section, unsigned n_mut2_ctx, int drfold);
@@ -130,10 +132,12 @@ void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const 
 /* Index path (mem_eff) matrix row: "name\t%lf..." with float-cast AFs. */
 void kssd_emit_index_matrix_row(kssd_buf_t *out, const kssd_report_opts_t *o,
 				const kssd_query_counts_t *c, const char *qry_name);
-/* Stream path (one query or few queries vs many references). */
+/* Stream paths: one query (single != 0, stream_ref_sketches_one_qraw_lookup,
+ * with its AF and close-filter early exits) or a few queries
+ * (stream_ref_sketches_multi_qraw_sortedindex) against many references. */
 void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
 			     const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
-			     char (*ref_anno)[KSSD_PATHLEN]);
+			     char (*ref_anno)[KSSD_PATHLEN], int single);
 /* Hash path (many small references, no index). */
 void kssd_emit_hash_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
 			   const char *qry_name, char (*ref_names)[KSSD_PATHLEN],

@@ -1,0 +1,258 @@
+/*
+ * Sketch-directory I/O: lcofiles.stat, comblco, comblco.index,
+ * sortedcomb_ctxgid64obj32 (+ .unique_bits), lcofiles.anno,
+ * lcofiles.infilemeta (global_basic.h:138-193, command_sketch.c:29-39,
+ * :850-938, :989-1005).
+ */
+#define _GNU_SOURCE
+#include "kssd_sketchdir.h"
+
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+
+static char *path_join(const char *dir, const char *name)
+{
+	char *p = NULL;
+	if (asprintf(&p, "%s/%s", dir, name) < 0)
+		return NULL;
+	return p;
+}
+
+bool kssd_file_in_dir(const char *dir, const char *name)
+{
+	char *p = path_join(dir, name);
+	struct stat st;
+	const bool ok = p && stat(p, &st) == 0 && S_ISREG(st.st_mode);
+	free(p);
+	return ok;
+}
+
+void *kssd_read_file(const char *path, size_t *size)
+{
+	FILE *fp = fopen(path, "rb");
+	if (!fp)
+		return NULL;
+	if (fseeko(fp, 0, SEEK_END) != 0) {
+		fclose(fp);
+		return NULL;
+	}
+	const off_t n = ftello(fp);
+	rewind(fp);
+	void *buf = malloc(n > 0 ? (size_t)n : 1);
+	if (!buf || (n > 0 && fread(buf, 1, (size_t)n, fp) != (size_t)n)) {
+		free(buf);
+		fclose(fp);
+		return NULL;
+	}
+	fclose(fp);
+	*size = (size_t)n;
+	return buf;
+}
+
+static void *read_in_dir(const char *dir, const char *name, size_t *size)
+{
+	char *p = path_join(dir, name);
+	void *b = p ? kssd_read_file(p, size) : NULL;
+	free(p);
+	return b;
+}
+
+int kssd_sketchdir_open(const char *dir, unsigned parts, kssd_sketchdir_t *s, char *err, size_t errlen)
+{
+	memset(s, 0, sizeof *s);
+	size_t n = 0;
+	void *stat = read_in_dir(dir, KSSD_STAT_FILE, &n);
+	if (!stat || n < sizeof(kssd_stat_t)) {
+		snprintf(err, errlen, "%s/%s: cannot read sketch statistics", dir, KSSD_STAT_FILE);
+		free(stat);
+		return -1;
+	}
+	memcpy(&s->stat, stat, sizeof s->stat);
+	const int num = s->stat.infile_num;
+	if (num < 0 || n < sizeof(kssd_stat_t) + (size_t)num * KSSD_PATHLEN) {
+		snprintf(err, errlen, "%s/%s: truncated (%zu bytes for %d samples)", dir, KSSD_STAT_FILE, n, num);
+		free(stat);
+		return -1;
+	}
+	s->n = num;
+	s->names = malloc((size_t)(num ? num : 1) * KSSD_PATHLEN);
+	if (!s->names) {
+		free(stat);
+		return -1;
+	}
+	memcpy(s->names, (char *)stat + sizeof(kssd_stat_t), (size_t)num * KSSD_PATHLEN);
+	for (int i = 0; i < num; ++i)
+		s->names[i][KSSD_PATHLEN - 1] = '\0';
+	free(stat);
+	/* Bitslen, const_comask_init (sketch_rearrange.c:7-39) */
+	s->ctx_bits = s->stat.coden_len > 0 ? 4u * (unsigned)s->stat.coden_len : 4u * (unsigned)s->stat.hclen;
+	s->obj_bits = 2u * (unsigned)s->stat.klen - s->ctx_bits;
+	s->index = read_in_dir(dir, KSSD_INDEX_FILE, &n);
+	if (!s->index || n != ((size_t)num + 1) * 8) {
+		snprintf(err, errlen, "%s/%s: expected %d offsets", dir, KSSD_INDEX_FILE, num + 1);
+		kssd_sketchdir_close(s);
+		return -1;
+	}
+	if (parts & KSSD_SK_KEYS) {
+		s->keys = read_in_dir(dir, KSSD_COMB_FILE, &n);
+		if (!s->keys || n != s->index[num] * 8) {
+			snprintf(err, errlen, "%s/%s: size does not match %s", dir, KSSD_COMB_FILE, KSSD_INDEX_FILE);
+			kssd_sketchdir_close(s);
+			return -1;
+		}
+	}
+	if ((parts & KSSD_SK_RECORDS) && kssd_file_in_dir(dir, KSSD_SORTED_INDEX_FILE)) {
+		s->records = read_in_dir(dir, KSSD_SORTED_INDEX_FILE, &n);
+		if (!s->records || n != s->index[num] * sizeof(kssd_ctxgidobj_t)) {
+			snprintf(err, errlen, "%s/%s: size does not match the sketch", dir, KSSD_SORTED_INDEX_FILE);
+			kssd_sketchdir_close(s);
+			return -1;
+		}
+	}
+	if ((parts & KSSD_SK_ANNO) && kssd_file_in_dir(dir, KSSD_ANNO_FILE)) {
+		s->anno = read_in_dir(dir, KSSD_ANNO_FILE, &n);
+		if (!s->anno || n != (size_t)num * KSSD_PATHLEN) {
+			snprintf(err, errlen, "%s/%s has %zu bytes, expected %zu", dir, KSSD_ANNO_FILE, n,
+				 (size_t)num * KSSD_PATHLEN);
+			kssd_sketchdir_close(s);
+			return -1;
+		}
+	}
+	if ((parts & KSSD_SK_META) && kssd_file_in_dir(dir, KSSD_META_FILE)) {
+		s->meta = read_in_dir(dir, KSSD_META_FILE, &n);
+		if (!s->meta || n != (size_t)num * sizeof(kssd_infile_meta_t)) {
+			snprintf(err, errlen, "%s/%s has %zu bytes, expected %zu", dir, KSSD_META_FILE, n,
+				 (size_t)num * sizeof(kssd_infile_meta_t));
+			kssd_sketchdir_close(s);
+			return -1;
+		}
+	}
+	return 0;
+}
+
+void kssd_sketchdir_close(kssd_sketchdir_t *s)
+{
+	free(s->names);
+	free(s->index);
+	free(s->keys);
+	free(s->records);
+	free(s->anno);
+	free(s->meta);
+	memset(s, 0, sizeof *s);
+}
+
+static int write_file(const char *dir, const char *name, const void *data, size_t n)
+{
+	char *p = path_join(dir, name);
+	if (!p)
+		return -1;
+	FILE *fp = fopen(p, "wb");
+	free(p);
+	if (!fp)
+		return -1;
+	const int ok = (n == 0 || fwrite(data, 1, n, fp) == n) && fclose(fp) == 0;
+	return ok ? 0 : -1;
+}
+
+int kssd_sketchdir_write(const char *dir, const kssd_stat_t *stat, char (*names)[KSSD_PATHLEN],
+			 const uint64_t *keys, const uint64_t *index)
+{
+	if (mkdir(dir, 0777) != 0 && errno != EEXIST)
+		return -1;
+	const size_t nh = sizeof *stat + (size_t)stat->infile_num * KSSD_PATHLEN;
+	char *head = calloc(1, nh);
+	if (!head)
+		return -1;
+	memcpy(head, stat, sizeof *stat);
+	memcpy(head + sizeof *stat, names, (size_t)stat->infile_num * KSSD_PATHLEN);
+	int rc = write_file(dir, KSSD_STAT_FILE, head, nh);
+	free(head);
+	if (rc)
+		return rc;
+	rc = write_file(dir, KSSD_COMB_FILE, keys, index[stat->infile_num] * 8);
+	if (rc)
+		return rc;
+	return write_file(dir, KSSD_INDEX_FILE, index, ((size_t)stat->infile_num + 1) * 8);
+}
+
+static int cmp_u32(const void *a, const void *b)
+{
+	const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+	return (x > y) - (x < y);
+}
+
+/* occurrences of v in the sorted array s[0..n) (at most 2 are needed) */
+static int count_sorted(const uint32_t *s, size_t n, uint32_t v)
+{
+	size_t lo = 0, hi = n;
+	while (lo < hi) {
+		const size_t mid = lo + (hi - lo) / 2;
+		if (s[mid] < v)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	int c = 0;
+	while (lo < n && s[lo] == v && c < 2) {
+		++lo;
+		++c;
+	}
+	return c;
+}
+
+int kssd_write_sorted_index(const char *dir, const kssd_ctxgidobj_t *rec, size_t n)
+{
+	int rc = write_file(dir, KSSD_SORTED_INDEX_FILE, rec, n * sizeof *rec);
+	if (rc)
+		return rc;
+	/* write_unique_marker_bits_from_sorted, command_sketch.c:855-914: one
+	 * bit per record (LSB first), set iff its (ctx, obj) occurs once in its
+	 * context group.  A group is ordered by gid, not by object, so large
+	 * groups are checked against a sorted copy of their objects. */
+	uint8_t *bits = calloc((n + 7) / 8 ? (n + 7) / 8 : 1, 1);
+	uint32_t *tmp = NULL;
+	size_t tmp_cap = 0;
+	if (!bits)
+		return -1;
+	for (size_t i = 0; i < n;) {
+		const uint64_t ctx = rec[i].ctxgid >> 20;
+		size_t e = i;
+		while (e < n && (rec[e].ctxgid >> 20) == ctx)
+			++e;
+		const size_t g = e - i;
+		if (g <= 16) {
+			for (size_t a = i; a < e; ++a) {
+				size_t same = 0;
+				for (size_t b = i; b < e && same < 2; ++b)
+					same += rec[b].obj == rec[a].obj;
+				if (same == 1)
+					bits[a >> 3] |= (uint8_t)(1u << (a & 7));
+			}
+		} else {
+			if (g > tmp_cap) {
+				uint32_t *t = realloc(tmp, g * sizeof *t);
+				if (!t) {
+					free(tmp);
+					free(bits);
+					return -1;
+				}
+				tmp = t;
+				tmp_cap = g;
+			}
+			for (size_t a = 0; a < g; ++a)
+				tmp[a] = rec[i + a].obj;
+			qsort(tmp, g, sizeof *tmp, cmp_u32);
+			for (size_t a = i; a < e; ++a)
+				if (count_sorted(tmp, g, rec[a].obj) == 1)
+					bits[a >> 3] |= (uint8_t)(1u << (a & 7));
+		}
+		i = e;
+	}
+	free(tmp);
+	rc = write_file(dir, KSSD_SORTED_INDEX_FILE ".unique_bits", bits, (n + 7) / 8);
+	free(bits);
+	return rc;
+}

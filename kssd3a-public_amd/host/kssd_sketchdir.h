@@ -1,0 +1,58 @@
+/* Sketch-directory I/O (formats of the reference's sketch directories). */
+#ifndef KSSD_SKETCHDIR_H
+#define KSSD_SKETCHDIR_H
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "kssd_report.h"
+#include "kssdhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSSD_STAT_FILE "lcofiles.stat"
+#define KSSD_COMB_FILE "comblco"
+#define KSSD_INDEX_FILE "comblco.index"
+#define KSSD_SORTED_INDEX_FILE "sortedcomb_ctxgid64obj32"
+#define KSSD_ANNO_FILE "lcofiles.anno"
+#define KSSD_META_FILE "lcofiles.infilemeta"
+
+/* dim_sketch_stat_t, global_basic.h:138-149 (32 bytes) */
+typedef struct {
+	uint32_t hash_id;
+	bool koc;
+	bool conflict;
+	int32_t coden_len, klen, hclen, holen, drfold, infile_num;
+} kssd_stat_t;
+
+enum { KSSD_SK_KEYS = 1, KSSD_SK_RECORDS = 2, KSSD_SK_ANNO = 4, KSSD_SK_META = 8 };
+
+typedef struct {
+	kssd_stat_t stat;
+	int n;
+	unsigned ctx_bits, obj_bits;
+	char (*names)[KSSD_PATHLEN];
+	uint64_t *index;             /* n + 1 */
+	uint64_t *keys;              /* index[n] (KSSD_SK_KEYS) */
+	kssd_ctxgidobj_t *records;   /* sorted index when present (KSSD_SK_RECORDS) */
+	char (*anno)[KSSD_PATHLEN];  /* optional */
+	kssd_infile_meta_t *meta;    /* optional */
+} kssd_sketchdir_t;
+
+bool kssd_file_in_dir(const char *dir, const char *name);
+void *kssd_read_file(const char *path, size_t *size);
+/* 0 on success; on failure err holds a message. */
+int kssd_sketchdir_open(const char *dir, unsigned parts, kssd_sketchdir_t *s, char *err, size_t errlen);
+void kssd_sketchdir_close(kssd_sketchdir_t *s);
+/* lcofiles.stat + comblco + comblco.index */
+int kssd_sketchdir_write(const char *dir, const kssd_stat_t *stat, char (*names)[KSSD_PATHLEN],
+			 const uint64_t *keys, const uint64_t *index);
+/* sortedcomb_ctxgid64obj32 + .unique_bits */
+int kssd_write_sorted_index(const char *dir, const kssd_ctxgidobj_t *rec, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -285,10 +285,13 @@ def load_kssdhost():
     lib.kssd_sketch_codes.restype = None
     lib.kssd_finalize_keys.argtypes = [ctypes.POINTER(U64Vec), I]
     lib.kssd_finalize_keys.restype = None
-    for f in ("kssd_emit_index_detail", "kssd_emit_stream_detail", "kssd_emit_hash_detail"):
+    for f in ("kssd_emit_index_detail", "kssd_emit_hash_detail"):
         getattr(lib, f).argtypes = [ctypes.POINTER(Buf), ctypes.POINTER(ReportOpts),
                                     ctypes.POINTER(QueryCounts), ctypes.c_char_p, V, V]
         getattr(lib, f).restype = None
+    lib.kssd_emit_stream_detail.argtypes = [ctypes.POINTER(Buf), ctypes.POINTER(ReportOpts),
+                                            ctypes.POINTER(QueryCounts), ctypes.c_char_p, V, V, I]
+    lib.kssd_emit_stream_detail.restype = None
     lib.kssd_emit_index_matrix_row.argtypes = [ctypes.POINTER(Buf), ctypes.POINTER(ReportOpts),
                                                ctypes.POINTER(QueryCounts), ctypes.c_char_p]
     lib.kssd_emit_index_matrix_row.restype = None
@@ -366,6 +369,9 @@ class Host:
         if path == "matrix":
             self.lib.kssd_emit_index_matrix_row(ctypes.byref(b), ctypes.byref(opts), ctypes.byref(qc),
                                                 qry_name.encode())
+        elif path in ("stream", "multistream"):
+            self.lib.kssd_emit_stream_detail(ctypes.byref(b), ctypes.byref(opts), ctypes.byref(qc),
+                                             qry_name.encode(), names, anno, int(path == "stream"))
         else:
             fn = getattr(self.lib, f"kssd_emit_{path}_detail")
             fn(ctypes.byref(b), ctypes.byref(opts), ctypes.byref(qc), qry_name.encode(), names, anno)

@@ -154,6 +154,48 @@ def make_genomes():
     shutil.rmtree(tmp)
 
 
+def subset_dir(src, dst, keep):
+    """Sketch dir holding samples `keep` of src (same header fields)."""
+    src, dst = Path(src), Path(dst)
+    shutil.rmtree(dst, ignore_errors=True)
+    dst.mkdir(parents=True)
+    stat = (src / "lcofiles.stat").read_bytes()
+    head = bytearray(stat[:32])
+    head[28:32] = np.uint32(len(keep)).tobytes()
+    names = [stat[32 + 256 * i:32 + 256 * (i + 1)] for i in keep]
+    (dst / "lcofiles.stat").write_bytes(bytes(head) + b"".join(names))
+    keys = np.fromfile(src / "comblco", dtype="<u8")
+    index = np.fromfile(src / "comblco.index", dtype="<u8")
+    parts = [keys[index[i]:index[i + 1]] for i in keep]
+    np.concatenate(parts).astype("<u8").tofile(dst / "comblco")
+    np.concatenate([[0], np.cumsum([len(x) for x in parts])]).astype("<u8").tofile(dst / "comblco.index")
+    for name, width in (("lcofiles.infilemeta", 32), ("lcofiles.anno", 256)):
+        if (src / name).exists():
+            raw = (src / name).read_bytes()
+            (dst / name).write_bytes(b"".join(raw[width * i:width * (i + 1)] for i in keep))
+
+
+def make_stream():
+    """Few-query requests against the 8-reference set: one query takes the
+    single-query streaming path, three take the multi-query path
+    (command_ani_wrapper.c:1168-1191)."""
+    out = HERE / "genomes"
+    R = out / "ref"
+    subset_dir(out / "qry", out / "qry1", [5])
+    subset_dir(out / "qry", out / "qry3", [0, 3, 6])
+    w = lambda name, text: (out / name).write_text(text)  # noqa: E731
+    for tag in ("qry1", "qry3"):
+        Q = out / tag
+        w(f"stream_{tag}_best.tsv", run(["ani", "-r", R, "-q", Q]))
+        w(f"stream_{tag}_all_s3.tsv", run(["ani", "-r", R, "-q", Q, "-s3", "-n0", "-f0", "-t1"]))
+        w(f"stream_{tag}_all_s9_top2.tsv", run(["ani", "-r", R, "-q", Q, "-s9", "-n0", "-f0", "-t1", "-N2"]))
+        w(f"stream_{tag}_qraw.tsv", run(["ani", "-r", R, "--qraw", Q, "-n0", "-t1"]))
+        w(f"stream_{tag}_maxpdist.tsv", run(["ani", "-r", R, "-q", Q, "-s4", "-n0", "-f0", "-t1",
+                                             "--max-pdist", "0.06"]))
+        w(f"stream_{tag}_maxsection.tsv", run(["ani", "-r", R, "-q", Q, "-s4", "-n0", "-f0", "-t1",
+                                               "--max-diff-section", "70"]))
+
+
 def make_sketch():
     """Small FASTA inputs (multi-record, lowercase, non-ACGT runs) and the
     reference's sketches of them, at the default -f8 and at -f4."""
@@ -184,6 +226,7 @@ def main():
         raise SystemExit(f"{BIN} missing: run `make -C oracle ref` first")
     make_counts()
     make_genomes()
+    make_stream()
     make_sketch()
     print("golden fixtures written under", HERE)
 

@@ -1,0 +1,194 @@
+"""The kssd3a-hip front end against the reference binary's outputs.
+
+CPU tests: argument errors (exit status 64 like the reference's argp
+errors), and `sketch -i`, whose index and unique-marker bits must be
+byte-identical to the reference's files.  GPU tests: every golden TSV under
+tests/golden/ regenerated by `kssd3a-hip ani` with the same arguments and
+environment that produced it from the reference (make_golden.py), compared
+byte for byte.  Each case runs on a private copy of its sketch directories
+because some paths write an index next to the sketch, as the reference does.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+import kssdtest as kt
+
+BIN = kt.PKG / "kssd3a-hip"
+GEN = kt.GOLDEN / "genomes"
+COUNT_CASES = sorted((kt.GOLDEN / "counts").glob("case_*"))
+INDEX_FILES = ("sortedcomb_ctxgid64obj32", "sortedcomb_ctxgid64obj32.unique_bits")
+
+
+def run_cli(args, env=None, timeout=120):
+    if not BIN.exists():
+        raise RuntimeError(f"{BIN} missing: build with `make -C kssd3a-public_amd`")
+    e = dict(os.environ)
+    for k in ("KSSD3A_FORCE_REF_INDEX", "KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD"):
+        e.pop(k, None)
+    e.update(env or {})
+    return subprocess.run([str(BIN), *map(str, args)], capture_output=True, text=True, env=e,
+                          timeout=timeout)
+
+
+def copy_dir(src, dst, with_index=True):
+    shutil.copytree(src, dst)
+    if not with_index:
+        for f in INDEX_FILES:
+            (dst / f).unlink(missing_ok=True)
+    return dst
+
+
+# ------------------------------------------------------------------ CPU
+@pytest.mark.parametrize("args,msg", [
+    (["ani", "-r", "x"], "missing query"),
+    (["ani", "-q", "x"], "-r/--ref is required for detail output"),
+    (["ani", "-r", "x", "-q", "y", "-s", "0"], "not 0"),
+    (["ani", "-r", "x", "-q", "y", "-s", "12"], "-s/--slmetrics"),
+    (["ani", "-r", "x", "-q", "y", "-f", "80"], "fraction"),
+    (["ani", "-r", "x", "-q", "y", "-m1", "--max-pdist", "0.1"], "detail output"),
+    (["ani", "-r", "x", "-q", "y", "--unique-ctx"], "unique-context options"),
+    (["ani", "-r", "x", "--qraw", "y", "--unique-ctx", "-s9"], "--unique-ctx uses"),
+    (["ani", "--reflist", "a", "--qrylist", "b"], "not offered"),
+    (["ani", "-r", "x", "-q", "y", "--metric", "bogus"], "unknown --metric"),
+    (["ani", "-r", "x", "-q", "y", "--format", "wide"], "--format must be"),
+    (["sketch", "-o", "out", "a.fa"], None),
+])
+def test_usage_errors_exit_64(args, msg):
+    p = run_cli(args)
+    assert p.returncode == 64, p.stderr
+    if msg:
+        assert msg in p.stderr
+
+
+def test_sketch_index_matches_reference(tmp_path):
+    d = copy_dir(GEN / "ref", tmp_path / "ref", with_index=False)
+    p = run_cli(["sketch", "-i", d, "-p", "2"])
+    assert p.returncode == 0, p.stderr
+    for f in INDEX_FILES:
+        assert (d / f).read_bytes() == (GEN / "ref" / f).read_bytes(), f
+
+
+def test_sketch_index_conflict_sketch(tmp_path):
+    """Conflict-keeping sketch (object conflicts inside contexts): the index
+    and unique bits against the numpy restatement of the format."""
+    import numpy as np
+    case = COUNT_CASES[-1]
+    d = copy_dir(case / "ref", tmp_path / "ref", with_index=False)
+    p = run_cli(["sketch", "-i", d])
+    assert p.returncode == 0, p.stderr
+    sk = kt.read_sketch_dir(d)
+    want = kt.build_index_np(sk.keys, sk.index, sk.obj_bits)
+    assert (d / INDEX_FILES[0]).read_bytes() == want.tobytes()
+    ctx = want["ctxgid"] >> np.uint64(20)
+    bits = np.zeros(len(want), bool)
+    for i in range(len(want)):
+        grp = want["obj"][ctx == ctx[i]]
+        bits[i] = int((grp == want["obj"][i]).sum()) == 1
+    assert (d / INDEX_FILES[1]).read_bytes() == np.packbits(bits, bitorder="little").tobytes()
+
+
+# ------------------------------------------------------------------ GPU
+def genome_dirs(tmp_path, ref_index):
+    return {
+        "R": copy_dir(GEN / "ref", tmp_path / "ref", with_index=ref_index),
+        "Q": copy_dir(GEN / "qry", tmp_path / "qry"),
+        "S": copy_dir(GEN / "all", tmp_path / "all", with_index=False),
+        "Q1": copy_dir(GEN / "qry1", tmp_path / "qry1"),
+        "Q3": copy_dir(GEN / "qry3", tmp_path / "qry3"),
+    }
+
+
+T0 = ["-n0", "-f0", "-t1"]
+GENOME_CASES = [
+    # (golden, reference index present, args, env)
+    ("detail_best_hash.tsv", False, ["-r", "R", "-q", "Q"], {}),
+    ("detail_all_s3_hash.tsv", False, ["-r", "R", "-q", "Q", "-s3", *T0], {}),
+    ("matrix_ani_rq.tsv", False, ["-r", "R", "-q", "Q", "--format", "matrix", "--values", "ani", "-s3"], {}),
+    ("self_matrix_direct_s3.tsv", False, ["-q", "S", "--format", "matrix", "--values", "ani", "-s3"], {}),
+    ("self_matrix_direct_s9.tsv", False, ["-q", "S", "--format", "matrix", "-s9"], {}),
+    ("self_matrix_direct_s9.tsv", False, ["-r", "S", "-q", "S", "--format", "matrix", "-s9"], {}),
+    ("self_matrix_indexed_s3.tsv", False, ["-q", "S", "--format", "matrix", "--values", "ani", "-s3"],
+     {"KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD": "0"}),
+    ("self_triangle_indexed_s4.tsv", False, ["-q", "S", "--format", "triangle", "-s4"],
+     {"KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD": "0"}),
+    ("detail_best_index.tsv", True, ["-r", "R", "-q", "Q"], {}),
+    ("detail_all_s3_index.tsv", True, ["-r", "R", "-q", "Q", "-s3", *T0], {}),
+    ("detail_all_s9_index.tsv", True, ["-r", "R", "-q", "Q", "-s9", *T0], {}),
+    ("detail_qraw_index.tsv", True, ["-r", "R", "--qraw", "Q", *T0], {}),
+]
+for tag in ("1", "3"):
+    GENOME_CASES += [
+        (f"stream_qry{tag}_best.tsv", True, ["-r", "R", "-q", f"Q{tag}"], {}),
+        (f"stream_qry{tag}_all_s3.tsv", False, ["-r", "R", "-q", f"Q{tag}", "-s3", *T0], {}),
+        (f"stream_qry{tag}_all_s9_top2.tsv", True, ["-r", "R", "-q", f"Q{tag}", "-s9", *T0, "-N2"], {}),
+        (f"stream_qry{tag}_qraw.tsv", True, ["-r", "R", "--qraw", f"Q{tag}", "-n0", "-t1"], {}),
+        (f"stream_qry{tag}_maxpdist.tsv", True,
+         ["-r", "R", "-q", f"Q{tag}", "-s4", *T0, "--max-pdist", "0.06"], {}),
+        (f"stream_qry{tag}_maxsection.tsv", True,
+         ["-r", "R", "-q", f"Q{tag}", "-s4", *T0, "--max-diff-section", "70"], {}),
+    ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("golden,ref_index,args,env", GENOME_CASES,
+                         ids=[f"{c[0]}:{' '.join(c[2])}" for c in GENOME_CASES])
+def test_cli_genomes_match_reference(tmp_path, golden, ref_index, args, env):
+    dirs = genome_dirs(tmp_path, ref_index)
+    argv = [str(dirs.get(a, a)) for a in args]
+    out = tmp_path / "out.tsv"
+    p = run_cli(["ani", *argv, "-o", out], env=env)
+    assert p.returncode == 0, p.stderr
+    assert out.read_text() == (GEN / golden).read_text()
+
+
+@pytest.mark.gpu
+def test_cli_stdout_and_auto_index(tmp_path):
+    """Without -o the rows go to stdout; the self full matrix above the
+    direct threshold writes the index next to the sketch first."""
+    dirs = genome_dirs(tmp_path, False)
+    p = run_cli(["ani", "-q", dirs["S"], "--format", "matrix", "--values", "ani", "-s3"],
+                env={"KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD": "0"})
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == (GEN / "self_matrix_indexed_s3.tsv").read_text()
+    assert (dirs["S"] / INDEX_FILES[0]).read_bytes() == (GEN / "all" / INDEX_FILES[0]).read_bytes()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", COUNT_CASES, ids=lambda p: p.name)
+def test_cli_conflict_fixtures_match_reference(tmp_path, case):
+    ref = copy_dir(case / "ref", tmp_path / "ref")
+    qry = copy_dir(case / "qry", tmp_path / "qry")
+    base = ["ani", "-r", ref, "--qraw", qry, "-p2", "-n0", "-f0", "-t1"]
+    force = {"KSSD3A_FORCE_REF_INDEX": "1"}
+    for extra, env, golden in (([], force, "detail_qraw.tsv"),
+                               (["--unique-ctx"], {}, "detail_qraw_unique.tsv"),
+                               (["--ignoreconflict"], force, "detail_qraw_ignoreconflict.tsv")):
+        p = run_cli([*base, *extra], env=env)
+        assert p.returncode == 0, p.stderr
+        assert p.stdout == (case / golden).read_text(), golden
+
+
+def test_sketch_index_unique_bits_large_groups(tmp_path):
+    """Context groups larger than the small-group scan (many samples share
+    one context, some with the same object)."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    samples = []
+    for i in range(60):
+        ctxs = np.array([7, 9, 11 + (i % 3)], np.uint64)
+        objs = np.array([rng.integers(0, 40), i, rng.integers(0, 4)], np.uint64)
+        samples.append(np.sort((ctxs << np.uint64(20)) | objs))
+    kt.write_sketch_dir(tmp_path / "big", samples)
+    p = run_cli(["sketch", "-i", tmp_path / "big"])
+    assert p.returncode == 0, p.stderr
+    sk = kt.read_sketch_dir(tmp_path / "big")
+    want = kt.build_index_np(sk.keys, sk.index, sk.obj_bits)
+    assert (tmp_path / "big" / INDEX_FILES[0]).read_bytes() == want.tobytes()
+    ctx = want["ctxgid"] >> np.uint64(20)
+    bits = np.array([int((want["obj"][ctx == ctx[i]] == want["obj"][i]).sum()) == 1
+                     for i in range(len(want))])
+    assert bits.any() and not bits.all()
+    assert (tmp_path / "big" / INDEX_FILES[1]).read_bytes() == np.packbits(bits, bitorder="little").tobytes()

@@ -89,3 +89,33 @@ def test_calibrated_model_special_cases(host):
     for raw in (0.0, 0.5, 0.9, 0.99, 1.0):
         v = lib.kssd_refaf_predict_ani(raw, 0.6, 500, 100, 120, 10, 8)
         assert 0.0 <= v <= 1.0
+
+
+def runs_totals(oracle, sk, ignoreconflict=False):
+    """Stream paths count distinct context runs whatever the conflict flag
+    (command_ani.c:2117-2123, :2905-2916)."""
+    return np.array([oracle.lib.orc_ctx_runs(kt._p(sk.sample(i)), len(sk.sample(i)), sk.obj_bits,
+                                             int(ignoreconflict)) for i in range(sk.n)], np.uint32)
+
+
+STREAM_CASES = [
+    ("best", dict()),
+    ("all_s3", dict(s=3, anicut=0.0, afcut=0.0, ctxcut=1)),
+    ("all_s9_top2", dict(s=9, anicut=0.0, afcut=0.0, ctxcut=1, ntop=2)),
+    ("qraw", dict(unassembled=True, anicut=0.0, ctxcut=1)),
+    ("maxpdist", dict(s=4, anicut=0.0, afcut=0.0, ctxcut=1, max_pdist_set=True, max_pdist=0.06)),
+    ("maxsection", dict(s=4, anicut=0.0, afcut=0.0, ctxcut=1, max_diff_obj_section=70)),
+]
+
+
+@pytest.mark.parametrize("tag", ["qry1", "qry3"])
+@pytest.mark.parametrize("name,kw", STREAM_CASES, ids=[c[0] for c in STREAM_CASES])
+def test_stream_paths_detail(host, oracle, tag, name, kw):
+    ref, qry = kt.read_sketch_dir(GEN / "ref"), kt.read_sketch_dir(GEN / tag)
+    if kw.get("unassembled"):
+        kw = dict(kw, afcut=0.2)  # --qraw without -f (command_ani_wrapper.c:1087-1088)
+    c = counts(oracle, ref, qry)
+    opts = kt.ReportOpts.defaults(**kw)
+    path = "stream" if qry.n == 1 else "multistream"
+    got = render(host, path, opts, ref, qry, c, runs_totals(oracle, qry), runs_totals(oracle, ref))
+    assert got == (GEN / f"stream_{tag}_{name}.tsv").read_text()
