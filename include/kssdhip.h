@@ -55,6 +55,18 @@ typedef struct {
 	uint32_t N_diff_obj;
 } kssd_ani_features_t;
 
+/* byte compatibility with the reference's types (sketch_rearrange.h:19,
+ * command_ani.h:114-124, model_ani.h:215-222) */
+#ifdef __cplusplus
+#define KSSDHIP_STATIC_ASSERT static_assert
+#else
+#define KSSDHIP_STATIC_ASSERT _Static_assert
+#endif
+KSSDHIP_STATIC_ASSERT(sizeof(kssd_ctxgidobj_t) == 12, "ctxgidobj_t is 12 packed bytes");
+KSSDHIP_STATIC_ASSERT(sizeof(kssd_ctx_mut2_t) == 8, "ctx_mut2_t is 2 x u32");
+KSSDHIP_STATIC_ASSERT(sizeof(kssd_obj_section_t) == 8, "obj_section_t is 2 x u32");
+KSSDHIP_STATIC_ASSERT(sizeof(kssd_ani_features_t) == 20, "ani_features_t is 5 x u32");
+
 typedef struct kssdhip_index kssdhip_index;
 
 /* count flags */

@@ -196,6 +196,19 @@ def make_stream():
                                                "--max-diff-section", "70"]))
 
 
+def make_metrics():
+    """Every selectable metric on the index path (detail) and the direct
+    rectangular matrix path, so each distance formula is pinned."""
+    out = HERE / "genomes"
+    R, Q = out / "ref", out / "qry"
+    w = lambda name, text: (out / name).write_text(text)  # noqa: E731
+    for s in (2, 5, 6, 7, 8):
+        w(f"detail_all_s{s}_index.tsv", run(["ani", "-r", R, "-q", Q, f"-s{s}", "-n0", "-f0", "-t1"]))
+    for s in (5, 6, 7, 8, -9):
+        w(f"matrix_rq_s{s}.tsv", run(["ani", "-r", R, "-q", Q, "--format", "matrix", f"-s{s}"],
+                                    env={"KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD": "1000"}))
+
+
 def make_sketch():
     """Small FASTA inputs (multi-record, lowercase, non-ACGT runs) and the
     reference's sketches of them, at the default -f8 and at -f4."""
@@ -227,6 +240,7 @@ def main():
     make_counts()
     make_genomes()
     make_stream()
+    make_metrics()
     make_sketch()
     print("golden fixtures written under", HERE)
 

@@ -119,6 +119,10 @@ GENOME_CASES = [
     ("detail_all_s9_index.tsv", True, ["-r", "R", "-q", "Q", "-s9", *T0], {}),
     ("detail_qraw_index.tsv", True, ["-r", "R", "--qraw", "Q", *T0], {}),
 ]
+GENOME_CASES += [(f"detail_all_s{s}_index.tsv", True, ["-r", "R", "-q", "Q", f"-s{s}", *T0], {})
+                 for s in (2, 5, 6, 7, 8)]
+GENOME_CASES += [(f"matrix_rq_s{s}.tsv", True, ["-r", "R", "-q", "Q", "--format", "matrix", f"-s{s}"], {})
+                 for s in (5, 6, 7, 8, -9)]
 for tag in ("1", "3"):
     GENOME_CASES += [
         (f"stream_qry{tag}_best.tsv", True, ["-r", "R", "-q", f"Q{tag}"], {}),

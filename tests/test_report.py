@@ -40,6 +40,7 @@ def render(host, path, opts, ref, qry, c, qctx, rctx, header=True):
 
 
 @pytest.mark.parametrize("golden,kw", [
+    *[(f"detail_all_s{s}_index.tsv", dict(s=s, anicut=0.0, afcut=0.0, ctxcut=1)) for s in (2, 5, 6, 7, 8)],
     ("detail_all_s3_index.tsv", dict(s=3, anicut=0.0, afcut=0.0, ctxcut=1)),
     ("detail_all_s9_index.tsv", dict(s=9, anicut=0.0, afcut=0.0, ctxcut=1)),
     ("detail_best_index.tsv", dict()),
@@ -119,3 +120,20 @@ def test_stream_paths_detail(host, oracle, tag, name, kw):
     path = "stream" if qry.n == 1 else "multistream"
     got = render(host, path, opts, ref, qry, c, runs_totals(oracle, qry), runs_totals(oracle, ref))
     assert got == (GEN / f"stream_{tag}_{name}.tsv").read_text()
+
+
+@pytest.mark.parametrize("s", [5, 6, 7, 8, -9])
+def test_direct_rect_matrix_cells(host, oracle, s):
+    """Direct rectangular matrix (comb_sortedsketch64Xcomb_sortedsketch64,
+    command_ani.c:3220-3245): every cell through the host's matrix value."""
+    ref, qry = kt.read_sketch_dir(GEN / "ref"), kt.read_sketch_dir(GEN / "qry")
+    ctx, obj, _, _ = counts(oracle, ref, qry)
+    qctx, rctx = ctx_totals(oracle, qry), ctx_totals(oracle, ref)
+    opts = kt.ReportOpts.defaults(fmt=1, s=s)
+    lines = ["".join("\t" + n for n in ref.names) + "\n"]
+    for q in range(qry.n):
+        cells = [host.lib.kssd_matrix_value(opts, int(ctx[q, r, 0]), int(obj[q, r, 0]),
+                                            int(obj[q, r, 1]), int(ctx[q, r, 1]),
+                                            int(qctx[q]), int(rctx[r]), None, None) for r in range(ref.n)]
+        lines.append(qry.names[q] + "".join("\t%f" % v for v in cells) + "\n")
+    assert "".join(lines) == (GEN / f"matrix_rq_s{s}.tsv").read_text()
