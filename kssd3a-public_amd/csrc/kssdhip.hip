@@ -147,20 +147,6 @@ __device__ __forceinline__ uint64_t lookup_ctx(const uint64_t *__restrict__ ctxg
 	return ~0ull;
 }
 
-// Largest q in [0, nq) with skidx[q] <= e (skidx[nq] > e holds).
-__device__ __forceinline__ int owner_query(const uint64_t *__restrict__ skidx, int nq, uint64_t e)
-{
-	int lo = 0, hi = nq;
-	while (hi - lo > 1) {
-		const int mid = (lo + hi) >> 1;
-		if (skidx[mid] <= e)
-			lo = mid;
-		else
-			hi = mid;
-	}
-	return lo;
-}
-
 // Index-join counting, one thread per query key; the first key of each
 // query context group does the work for the group (command_ani.h:506-575,
 // :668-716).
@@ -469,7 +455,9 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 			rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: stream creation failed");
 			break;
 		}
-		const size_t nalloc = n ? n : 1;
+		// padded so 16-byte vector loads of aligned 4-record groups that
+		// straddle the end stay inside the allocation
+		const size_t nalloc = n + INDEX_PAD;
 		if (hipMalloc(&ix->d_ctxgid, nalloc * sizeof(uint64_t)) != hipSuccess ||
 		    hipMalloc(&ix->d_obj, nalloc * sizeof(uint32_t)) != hipSuccess ||
 		    hipMalloc(&ix->d_bucket, nb * sizeof(uint32_t)) != hipSuccess) {
@@ -553,7 +541,7 @@ extern "C" size_t kssdhip_index_device_bytes(const kssdhip_index *ix)
 {
 	if (!ix)
 		return 0;
-	size_t b = (size_t)ix->n * 12 + ((((size_t)1) << ix->k) + 1) * 4;
+	size_t b = ((size_t)ix->n + INDEX_PAD) * 12 + ((((size_t)1) << ix->k) + 1) * 4;
 	b += ix->keys.bytes + ix->skidx.bytes + ix->ctx.bytes + ix->obj.bytes + ix->uctx.bytes +
 	     ix->uobj.bytes + ix->idx.bytes + ix->split.bytes + ((size_t)ix->P + 1) * 12;
 	return b;

@@ -9,6 +9,7 @@ namespace kssd {
 constexpr int GID_NBITS = 20;                         // sketch_rearrange.h:7
 constexpr uint64_t GID_MASK = (1ull << GID_NBITS) - 1ull;
 constexpr int MIN_INIT = 12;                          // NUM_CODENS + 1, command_ani.h:535
+constexpr size_t INDEX_PAD = 8;                       // device index arrays hold n + 8 records
 
 // dna_popcount.h:45-49: one bit per differing 2-bit base lane
 __device__ __forceinline__ int dna_popcount_dev(uint32_t x)
@@ -38,6 +39,21 @@ struct JoinArgs {
 	unsigned long long *uctx;
 	unsigned long long *uobj;
 };
+// Largest q in [0, nq) with skidx[q] <= e (skidx[nq] > e must hold): the
+// query that owns key e of a block.
+__device__ __forceinline__ int owner_query(const uint64_t *__restrict__ skidx, int nq, uint64_t e)
+{
+	int lo = 0, hi = nq;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (skidx[mid] <= e)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
 // Workspace of the one-launch plain pair kernel: per-block partial sums and
 // the count of finished blocks (left at zero by every launch).
 constexpr unsigned PAIR_MAX_BLOCKS = 1024;

@@ -3,14 +3,17 @@
 // The resident index is cut into P partitions of about `chunk` records whose
 // boundaries never split the records of one context.  For a block of queries
 // a split table S[q][p] (first key of query q whose context is >= the first
-// context of partition p) is computed once; then one workgroup per partition
-// stages the partition's records in LDS with coalesced loads and, one thread
-// per query, merges that query's keys of the partition against the staged
-// records: binary search in LDS, walk of the context's per-genome runs,
-// minimum per-base mismatch over the object cross product, and per-thread
-// accumulation of the pair counts, flushed with one 64-bit atomic per
-// (partition, query, genome) cell.  Consecutive partitions go to the same XCD
-// so the split-table and query-key lines they share stay in that XCD's L2.
+// context of partition p) is computed first (k_split).  Then one workgroup
+// per (partition, slice of up to QB queries) stages the partition's records
+// in LDS with 16-byte loads, builds a bucket table over the partition's
+// context range in LDS, prefix-sums the slice's key counts in the partition,
+// and deals the keys to its threads in equal contiguous runs.  Each key is
+// matched against the staged records: bucket probe plus a short scan, walk
+// of the context's per-genome runs, minimum per-base mismatch over the object
+// cross product; hits are summed per (query, genome) cell in registers and
+// flushed with one 64-bit atomic per cell and counter.  Consecutive
+// partitions go to the same XCD so the query-key lines they share stay in
+// that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +27,7 @@ using namespace kssd;
 namespace {
 
 constexpr uint32_t LDS_SLACK = 64; // partitions may exceed `chunk` by a short context run
+constexpr uint32_t STAGE_ALIGN = 4; // records are staged in aligned groups of 4 (16-byte loads)
 constexpr int NUM_XCD = 8;
 
 __global__ void k_part_cand(const uint64_t *__restrict__ ctxgid, uint64_t n, uint64_t chunk,
@@ -57,99 +61,239 @@ __global__ void k_part_ctx(const uint64_t *__restrict__ ctxgid, const uint32_t *
 		pctx[p] = p < P ? (ctxgid[pstart[p]] >> GID_NBITS) : ~0ull;
 }
 
-// S[q*(P+1)+p] = first key index (relative to the block) of query q whose
-// context is >= pctx[p]; pctx[P] = ~0 gives the end of q.
-__global__ __launch_bounds__(256) void k_split(const uint64_t *__restrict__ keys,
-					       const uint64_t *__restrict__ skidx, int nq,
-					       unsigned obj_bits, const uint64_t *__restrict__ pctx,
-					       uint32_t P, uint32_t *__restrict__ S)
+constexpr uint32_t SPLIT_RUN = 2; // partition boundaries per k_split thread
+
+struct SplitArgs {
+	const uint64_t *keys;
+	const uint64_t *skidx; // relative to keys, nq + 1
+	int nq;
+	unsigned obj_bits;
+	const uint64_t *pctx;  // P + 1, pctx[P] = ~0
+	uint32_t P;
+	uint32_t *S; // [nq][P + 1]
+};
+
+// S[q][p] = first key of query q whose context is >= pctx[p] (the end of q
+// when none).  One thread per (query, run of SPLIT_RUN boundaries): a binary
+// search for the run's first boundary, then a galloping search forward from
+// the previous boundary for each next one (a few keys apart), so the run's
+// entries cost a handful of cached loads each and are written contiguously.
+__global__ __launch_bounds__(256) void k_split(SplitArgs a)
 {
-	const uint64_t width = (uint64_t)P + 1;
-	const uint64_t total = (uint64_t)nq * width;
-	for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
-	     t += (uint64_t)gridDim.x * blockDim.x) {
-		const uint64_t q = t / width, p = t - q * width;
-		uint64_t lo = skidx[q], hi = skidx[q + 1];
-		const uint64_t c = pctx[p];
-		while (lo < hi) {
-			const uint64_t mid = lo + ((hi - lo) >> 1);
-			if ((keys[mid] >> obj_bits) < c)
-				lo = mid + 1;
-			else
-				hi = mid;
+	const uint32_t width = a.P + 1, runs = (width + SPLIT_RUN - 1) / SPLIT_RUN;
+	const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+	if (t >= (uint64_t)a.nq * runs)
+		return;
+	const int q = (int)(t / runs);
+	const uint32_t p0 = (uint32_t)(t - (uint64_t)q * runs) * SPLIT_RUN;
+	const uint32_t p1 = p0 + SPLIT_RUN < width ? p0 + SPLIT_RUN : width;
+	const uint64_t qb = a.skidx[q], qe = a.skidx[q + 1];
+	const unsigned ob = a.obj_bits;
+	uint32_t *row = a.S + (uint64_t)q * width;
+	uint64_t c = a.pctx[p0];
+	uint64_t lo = qb, hi = qe;
+	while (lo < hi) {
+		const uint64_t mid = lo + ((hi - lo) >> 1);
+		if ((a.keys[mid] >> ob) < c)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	uint64_t i = lo;
+	row[p0] = (uint32_t)i;
+	for (uint32_t p = p0 + 1; p < p1; ++p) {
+		c = a.pctx[p];
+		if (i < qe && (a.keys[i] >> ob) < c) {
+			// keys[l] < c; gallop until keys[h] >= c or h reaches qe
+			uint64_t l = i, step = 1, h = i + 1;
+			while (h < qe && (a.keys[h] >> ob) < c) {
+				l = h;
+				step <<= 1;
+				h = l + step;
+			}
+			if (h > qe)
+				h = qe;
+			uint64_t x = l + 1, y = h;
+			while (x < y) {
+				const uint64_t mid = x + ((y - x) >> 1);
+				if ((a.keys[mid] >> ob) < c)
+					x = mid + 1;
+				else
+					y = mid;
+			}
+			i = x;
 		}
-		S[t] = (uint32_t)lo;
+		row[p] = (uint32_t)i;
 	}
 }
 
 struct PJArgs {
 	JoinArgs j;
 	const uint32_t *pstart;
-	const uint32_t *S;
+	const uint32_t *S; // [nq][P + 1]
 	uint32_t P;
 	uint32_t cap; // records that fit the LDS stage
 };
 
-// Per-thread accumulation of consecutive hits on one (query, genome) cell.
+constexpr uint32_t QB = 1024; // queries per workgroup; grid.y covers the block's queries
+
+// Context buckets of a staged partition: bucket b holds the records whose
+// context c has (c - c0) >> shift == b; bt[b] = first record of bucket b.
+constexpr int PART_BUCKET_BITS = 10;
+constexpr uint32_t PART_BUCKETS = 1u << PART_BUCKET_BITS;
+struct PartTable {
+	const uint16_t *bt;
+	uint64_t c0, clast;
+	int shift;
+};
+
+// In-place exclusive scan of v[0, n) in LDS; returns the total.  Every
+// thread of the block calls it.  wsum: LDS scratch of blockDim/64 + 1 words.
+__device__ uint32_t block_exclusive_scan(uint32_t *v, uint32_t n, uint32_t *wsum)
+{
+	const uint32_t B = blockDim.x, t = threadIdx.x, nw = B / warpSize;
+	const uint32_t per = (n + B - 1) / B;
+	const uint32_t b0 = t * per < n ? t * per : n, b1 = b0 + per < n ? b0 + per : n;
+	uint32_t s = 0;
+	for (uint32_t k = b0; k < b1; ++k)
+		s += v[k];
+	const uint32_t lane = t & (warpSize - 1), w = t / warpSize;
+	uint32_t x = s; // inclusive scan over the wave
+	for (int d = 1; d < warpSize; d <<= 1) {
+		const uint32_t y = __shfl_up(x, d);
+		if (lane >= (uint32_t)d)
+			x += y;
+	}
+	if (lane == warpSize - 1)
+		wsum[w] = x;
+	__syncthreads();
+	if (t == 0) {
+		uint32_t run = 0;
+		for (uint32_t k = 0; k < nw; ++k) {
+			const uint32_t c = wsum[k];
+			wsum[k] = run;
+			run += c;
+		}
+		wsum[nw] = run;
+	}
+	__syncthreads();
+	uint32_t off = wsum[w] + x - s;
+	for (uint32_t k = b0; k < b1; ++k) {
+		const uint32_t c = v[k];
+		v[k] = off;
+		off += c;
+	}
+	const uint32_t total = wsum[nw];
+	__syncthreads();
+	return total;
+}
+
+// Per-thread accumulation of consecutive hits on one counter cell, flushed
+// with one 64-bit atomic per counter when the cell changes.
 struct CellAcc {
-	uint32_t gid = ~0u;
+	uint64_t cell = ~0ull;
 	unsigned long long c = 0, o = 0;
-	__device__ __forceinline__ void add(uint32_t g, int m, unsigned long long *ctx,
-					    unsigned long long *obj, uint64_t row)
+	__device__ __forceinline__ void add(uint64_t at, int m, unsigned long long *ctx, unsigned long long *obj)
 	{
-		if (g != gid) {
-			flush(ctx, obj, row);
-			gid = g;
+		if (at != cell) {
+			flush(ctx, obj);
+			cell = at;
 		}
 		c += 1ull | ((unsigned long long)(m > 1) << 32);
 		if (m)
 			o += 1ull | ((unsigned long long)m << 32);
 	}
-	__device__ __forceinline__ void flush(unsigned long long *ctx, unsigned long long *obj, uint64_t row)
+	__device__ __forceinline__ void flush(unsigned long long *ctx, unsigned long long *obj)
 	{
-		if (gid != ~0u) {
-			atomicAdd(&ctx[row + gid], c);
+		if (cell != ~0ull) {
+			atomicAdd(&ctx[cell], c);
 			if (o)
-				atomicAdd(&obj[row + gid], o);
+				atomicAdd(&obj[cell], o);
 		}
-		gid = ~0u;
+		cell = ~0ull;
 		c = o = 0;
 	}
 };
 
-// Merge one partition against all queries of the block.  REC/OBJ point at the
-// partition's records (LDS stage or global memory).
-template <typename RecPtr, typename ObjPtr>
-__device__ __forceinline__ void merge_partition(const PJArgs &a, uint32_t p, RecPtr rec, ObjPtr robj,
-						uint32_t cnt)
+// Count one partition against the workgroup's queries.  pre[0, nqb] =
+// exclusive prefix of the queries' key counts in this partition, start[k] =
+// first key of query qb0 + k in it.  The K keys, in flattened (query, key)
+// order, are dealt to the threads in equal contiguous runs, so every lane
+// does the same number of lookups and a lane's keys mostly belong to one
+// query; the first key of each context group does the work for the group
+// (command_ani.h:506-575, :668-716), and hits are summed per (query, genome)
+// cell in registers before one atomic per cell and counter.
+template <bool TABLE, typename RecPtr, typename ObjPtr>
+__device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr robj, uint32_t cnt,
+					   const PartTable &tab, const uint32_t *pre, const uint32_t *start,
+					   int qb0, int nqb, uint32_t K)
 {
 	const JoinArgs &j = a.j;
-	const uint64_t objmask = j.obj_bits >= 64 ? ~0ull : ((1ull << j.obj_bits) - 1ull);
+	const unsigned ob = j.obj_bits;
+	const uint64_t objmask = ob >= 64 ? ~0ull : ((1ull << ob) - 1ull);
 	const bool lower = (j.flags & KSSDHIP_LOWER) != 0;
 	const bool unique = !lower && (j.flags & KSSDHIP_UNIQUE) != 0;
 	const bool ignore = (j.flags & KSSDHIP_IGNORECONFLICT) != 0;
-	const uint64_t width = (uint64_t)a.P + 1;
-	for (int q = threadIdx.x; q < j.nq; q += blockDim.x) {
-		const uint64_t srow = (uint64_t)q * width + p;
-		uint32_t i = a.S[srow];
-		const uint32_t hi = a.S[srow + 1];
-		if (i >= hi)
-			continue;
-		const int qgid = j.qry_gid_offset + q;
-		const uint64_t row = (uint64_t)q * (uint64_t)j.ref_gnum;
-		CellAcc acc, uacc;
-		uint64_t key = j.keys[i];
-		while (i < hi) {
-			const uint64_t qctx = key >> j.obj_bits;
-			uint32_t s_end = i + 1;
-			uint64_t next = 0;
-			while (s_end < hi) {
-				next = j.keys[s_end];
-				if ((next >> j.obj_bits) != qctx)
-					break;
-				++s_end;
+	const uint32_t per = (K + blockDim.x - 1) / blockDim.x;
+	const uint32_t f_begin = threadIdx.x * per;
+	const uint32_t f_end = f_begin + per < K ? f_begin + per : K;
+	if (f_begin >= K)
+		return;
+	int qi; // largest qi with pre[qi] <= f_begin: the query owning key f_begin
+	{
+		int lo = 0, hi = nqb;
+		while (hi - lo > 1) {
+			const int mid = (lo + hi) >> 1;
+			if (pre[mid] <= f_begin)
+				lo = mid;
+			else
+				hi = mid;
+		}
+		qi = lo;
+	}
+	CellAcc acc, uacc;
+	uint64_t prev_key = 0, next_key = 0;
+	uint32_t prev_idx = ~0u, next_idx = ~0u;
+	for (uint32_t f = f_begin; f < f_end; ++f) {
+		while (pre[qi + 1] <= f)
+			++qi;
+		const uint32_t kidx = start[qi] + (f - pre[qi]);
+		const uint32_t qend = start[qi] + (pre[qi + 1] - pre[qi]);
+		const uint64_t key = kidx == next_idx ? next_key : j.keys[kidx];
+		const uint64_t qctx = key >> ob;
+		if (f != pre[qi]) {
+			const uint64_t prev = kidx - 1 == prev_idx ? prev_key : j.keys[kidx - 1];
+			if ((prev >> ob) == qctx) {
+				prev_key = key;
+				prev_idx = kidx;
+				continue; // not the first key of its context group
 			}
-			// lower_bound of the context among the partition's records
+		}
+		prev_key = key;
+		prev_idx = kidx;
+		uint32_t s_end = kidx + 1;
+		if (s_end < qend) {
+			next_key = j.keys[s_end];
+			next_idx = s_end;
+			if ((next_key >> ob) == qctx) {
+				++s_end;
+				while (s_end < qend && (j.keys[s_end] >> ob) == qctx)
+					++s_end;
+			}
+		}
+		// first record of the partition whose context is >= qctx
+		uint32_t d;
+		if (TABLE) {
+			if (qctx > tab.clast || qctx < tab.c0) {
+				d = cnt;
+			} else {
+				const uint32_t b = (uint32_t)((qctx - tab.c0) >> tab.shift);
+				d = tab.bt[b];
+				const uint32_t e = tab.bt[b + 1];
+				while (d < e && (rec[d] >> GID_NBITS) < qctx)
+					++d;
+			}
+		} else {
 			const uint64_t target = qctx << GID_NBITS;
 			uint32_t lo = 0, h2 = cnt;
 			while (lo < h2) {
@@ -159,65 +303,73 @@ __device__ __forceinline__ void merge_partition(const PJArgs &a, uint32_t p, Rec
 				else
 					h2 = mid;
 			}
-			uint32_t d = lo;
-			int owners = 0, sole_diff = 0;
-			uint32_t sole_gid = 0;
-			bool sole_allowed = false;
-			while (d < cnt) {
-				const uint64_t cg = rec[d];
-				if ((cg >> GID_NBITS) != qctx)
+			d = lo;
+		}
+		const int q = qb0 + qi;
+		const int qgid = j.qry_gid_offset + q;
+		const uint64_t row = (uint64_t)q * (uint64_t)j.ref_gnum;
+		int owners = 0, sole_diff = 0;
+		uint32_t sole_gid = 0;
+		bool sole_allowed = false;
+		while (d < cnt) {
+			const uint64_t cg = rec[d];
+			if ((cg >> GID_NBITS) != qctx)
+				break;
+			const uint32_t gid = (uint32_t)(cg & GID_MASK);
+			const uint32_t rb = d;
+			do {
+				++d;
+			} while (d < cnt && rec[d] == cg);
+			const uint32_t re = d;
+			if (lower && (int)gid >= qgid) {
+				if ((int)gid > qgid)
 					break;
-				const uint32_t gid = (uint32_t)(cg & GID_MASK);
-				const uint32_t rb = d;
-				do {
-					++d;
-				} while (d < cnt && rec[d] == cg);
-				const uint32_t re = d;
-				if (lower && (int)gid >= qgid) {
-					if ((int)gid > qgid)
+				continue;
+			}
+			if (unique) {
+				++owners;
+				sole_gid = gid;
+			}
+			if (ignore && re - rb > 1)
+				continue;
+			int m = MIN_INIT;
+			for (uint32_t x = kidx; x < s_end && m; ++x) {
+				const uint32_t oq = (uint32_t)((x == kidx ? key : j.keys[x]) & objmask);
+				for (uint32_t r = rb; r < re; ++r) {
+					const uint32_t diff = oq ^ robj[r];
+					if (diff == 0) {
+						m = 0;
 						break;
-					continue;
-				}
-				if (unique) {
-					++owners;
-					sole_gid = gid;
-				}
-				if (ignore && re - rb > 1)
-					continue;
-				int m = MIN_INIT;
-				for (uint32_t x = i; x < s_end && m; ++x) {
-					const uint32_t oq = (uint32_t)((x == i ? key : j.keys[x]) & objmask);
-					for (uint32_t r = rb; r < re; ++r) {
-						const uint32_t diff = oq ^ robj[r];
-						if (diff == 0) {
-							m = 0;
-							break;
-						}
-						const int c = dna_popcount_dev(diff);
-						m = c < m ? c : m;
 					}
-				}
-				acc.add(gid, m, j.ctx, j.objc, row);
-				if (unique) {
-					sole_diff = m;
-					sole_allowed = true;
+					const int c = dna_popcount_dev(diff);
+					m = c < m ? c : m;
 				}
 			}
-			if (unique && owners == 1 && sole_allowed)
-				uacc.add(sole_gid, sole_diff, j.uctx, j.uobj, row);
-			i = s_end;
-			key = next;
+			acc.add(row + gid, m, j.ctx, j.objc);
+			if (unique) {
+				sole_diff = m;
+				sole_allowed = true;
+			}
 		}
-		acc.flush(j.ctx, j.objc, row);
-		if (unique)
-			uacc.flush(j.uctx, j.uobj, row);
+		if (unique && owners == 1 && sole_allowed)
+			uacc.add(row + sole_gid, sole_diff, j.uctx, j.uobj);
 	}
+	acc.flush(j.ctx, j.objc);
+	if (unique)
+		uacc.flush(j.uctx, j.uobj);
 }
 
 __global__ void k_pjoin(PJArgs a)
 {
-	extern __shared__ uint64_t lds_rec[]; // [cap] ctxgid, then [cap] obj (u32)
+	// LDS: [cap] ctxgid | [cap] obj (u32) | [PART_BUCKETS + 2] u16 table |
+	//      [qcap + 1] u32 prefix | [qcap] u32 starts, qcap = min(nq, QB)
+	extern __shared__ uint64_t lds_rec[];
+	__shared__ uint32_t wsum[1024 / 64 + 1];
+	const uint32_t qcap = (uint32_t)a.j.nq < QB ? (uint32_t)a.j.nq : QB;
 	uint32_t *lds_obj = (uint32_t *)(lds_rec + a.cap);
+	uint16_t *lds_bt = (uint16_t *)(lds_obj + a.cap);
+	uint32_t *lds_pre = (uint32_t *)(lds_bt + PART_BUCKETS + 2);
+	uint32_t *lds_start = lds_pre + qcap + 1;
 	// XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a
 	// contiguous run of partitions.
 	const uint32_t G = gridDim.x, b = blockIdx.x;
@@ -225,18 +377,66 @@ __global__ void k_pjoin(PJArgs a)
 	const uint32_t p = x * base + (x < rem ? x : rem) + i;
 	if (p >= a.P)
 		return;
-	const uint32_t rb = a.pstart[p], re = a.pstart[p + 1], cnt = re - rb;
-	if (cnt <= a.cap) {
-		for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
-			lds_rec[k] = a.j.ctxgid[rb + k];
-			lds_obj[k] = a.j.robj[rb + k];
-		}
-		__syncthreads();
-		merge_partition(a, p, (const uint64_t *)lds_rec, (const uint32_t *)lds_obj, cnt);
-	} else {
-		// a context run longer than the stage: merge straight from HBM
-		merge_partition(a, p, a.j.ctxgid + rb, a.j.robj + rb, cnt);
+	const int qb0 = (int)(blockIdx.y * QB);
+	const int nqb = a.j.nq - qb0 < (int)QB ? a.j.nq - qb0 : (int)QB;
+	// this partition's key range of each of the workgroup's queries
+	const uint64_t width = (uint64_t)a.P + 1;
+	for (int k = threadIdx.x; k < nqb; k += blockDim.x) {
+		const uint64_t r = (uint64_t)(qb0 + k) * width + p;
+		const uint32_t s0 = a.S[r], s1 = a.S[r + 1];
+		lds_start[k] = s0;
+		lds_pre[k] = s1 - s0;
 	}
+	const uint32_t rb = a.pstart[p], re = a.pstart[p + 1], cnt = re - rb;
+	const uint32_t rb_al = rb & ~(STAGE_ALIGN - 1), re_al = (re + STAGE_ALIGN - 1) & ~(STAGE_ALIGN - 1);
+	const bool staged = re_al - rb_al <= a.cap;
+	if (staged) {
+		// stage [rb_al, re_al) with 16-byte loads; the index arrays are
+		// padded past n, so the last group stays in bounds
+		const uint32_t groups = (re_al - rb_al) / STAGE_ALIGN;
+		const uint4 *g_rec = (const uint4 *)(a.j.ctxgid + rb_al);
+		const uint4 *g_obj = (const uint4 *)(a.j.robj + rb_al);
+		uint4 *l_rec = (uint4 *)lds_rec;
+		uint4 *l_obj = (uint4 *)lds_obj;
+		for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
+			const uint4 r0 = g_rec[2 * g], r1 = g_rec[2 * g + 1], o = g_obj[g];
+			l_rec[2 * g] = r0;
+			l_rec[2 * g + 1] = r1;
+			l_obj[g] = o;
+		}
+	}
+	__syncthreads();
+	const uint32_t K = block_exclusive_scan(lds_pre, (uint32_t)nqb, wsum);
+	if (threadIdx.x == 0)
+		lds_pre[nqb] = K;
+	if (!staged) {
+		__syncthreads();
+		// a context run longer than the stage: merge straight from HBM
+		merge_keys<false>(a, a.j.ctxgid + rb, a.j.robj + rb, cnt, PartTable{}, lds_pre, lds_start, qb0,
+				  nqb, K);
+		return;
+	}
+	const uint32_t off = rb - rb_al;
+	const uint64_t *srec = lds_rec + off;
+	const uint32_t *sobj = lds_obj + off;
+	// bucket table over [c0, clast]: at most PART_BUCKETS buckets
+	PartTable tab;
+	tab.bt = lds_bt;
+	tab.c0 = srec[0] >> GID_NBITS;
+	tab.clast = srec[cnt - 1] >> GID_NBITS;
+	const uint64_t span = tab.clast - tab.c0; // buckets = (span >> shift) + 1
+	const int bits = span ? 64 - __clzll((long long)span) : 0;
+	tab.shift = bits > PART_BUCKET_BITS ? bits - PART_BUCKET_BITS : 0;
+	for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+		const int bk = (int)(((srec[k] >> GID_NBITS) - tab.c0) >> tab.shift);
+		const int bp = k ? (int)(((srec[k - 1] >> GID_NBITS) - tab.c0) >> tab.shift) : -1;
+		for (int bb = bp + 1; bb <= bk; ++bb)
+			lds_bt[bb] = (uint16_t)k;
+	}
+	if (threadIdx.x == 0)
+		lds_bt[(span >> tab.shift) + 1] = (uint16_t)cnt;
+	__syncthreads();
+	merge_keys<true>(a, srec, sobj, cnt, tab, lds_pre, lds_start, qb0, nqb, K);
 }
 
 unsigned grid_cap(uint64_t n, unsigned block)
@@ -328,21 +528,33 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, bool
 			  (ix->mode == 0 && (double)ja.total >= ix->auto_min_keys * (double)ix->P);
 	if (!want)
 		return 0;
-	const uint64_t width = (uint64_t)ix->P + 1;
-	const uint64_t sbytes = (uint64_t)ja.nq * width * 4;
+	const uint64_t sbytes = ((uint64_t)ix->P + 1) * (uint64_t)ja.nq * 4;
+	if (ix->block > 1024)
+		return 0;
 	int rc = ix->split.reserve(sbytes);
 	if (rc)
 		return rc;
 	uint32_t *S = (uint32_t *)ix->split.ptr;
-	hipLaunchKernelGGL(k_split, dim3(grid_cap((uint64_t)ja.nq * width, 256)), dim3(256), 0, s, ja.keys,
-			   ja.skidx, ja.nq, ja.obj_bits, (const uint64_t *)ix->d_pctx, ix->P, S);
+	SplitArgs sa;
+	sa.keys = ja.keys;
+	sa.skidx = ja.skidx;
+	sa.nq = ja.nq;
+	sa.obj_bits = ja.obj_bits;
+	sa.pctx = ix->d_pctx;
+	sa.P = ix->P;
+	sa.S = S;
+	const uint64_t runs = ((uint64_t)ix->P + SPLIT_RUN) / SPLIT_RUN;
+	const uint64_t nthreads = (uint64_t)ja.nq * runs;
+	hipLaunchKernelGGL(k_split, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, sa);
 	PJArgs a;
 	a.j = ja;
 	a.pstart = ix->d_pstart;
 	a.S = S;
 	a.P = ix->P;
-	a.cap = (uint32_t)ix->chunk + LDS_SLACK;
-	const size_t lds = (size_t)a.cap * 12;
+	a.cap = ((uint32_t)ix->chunk + LDS_SLACK + STAGE_ALIGN - 1) & ~(STAGE_ALIGN - 1);
+	const uint32_t qslices = (uint32_t)((ja.nq + QB - 1) / QB);
+	const uint32_t qb = (uint32_t)ja.nq < QB ? (uint32_t)ja.nq : QB;
+	const size_t lds = (size_t)a.cap * 12 + (PART_BUCKETS + 2) * 2 + ((size_t)2 * qb + 1) * 4;
 	static thread_local size_t lds_set = 0;
 	if (lds > 65536 && lds_set < lds) {
 		hipError_t e = hipFuncSetAttribute((const void *)k_pjoin,
@@ -351,7 +563,7 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, bool
 			return kssdhip_fail(KSSDHIP_EHIP, "pjoin: LDS %zu bytes: %s", lds, hipGetErrorString(e));
 		lds_set = lds;
 	}
-	hipLaunchKernelGGL(k_pjoin, dim3(ix->P), dim3(ix->block), lds, s, a);
+	hipLaunchKernelGGL(k_pjoin, dim3(ix->P, qslices), dim3(ix->block), lds, s, a);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess)
 		return kssdhip_fail(KSSDHIP_EHIP, "pjoin launch: %s", hipGetErrorString(e));
