@@ -82,6 +82,7 @@ typedef struct kssdhip_index kssdhip_index;
 #define KSSDHIP_ENOMEM (-2)
 #define KSSDHIP_EHIP (-3)
 #define KSSDHIP_ENODEV (-4)
+#define KSSDHIP_EAGAIN (-5) /* serving: queue full / not done yet */
 
 /* Message for the last failure on this thread ("" if none). */
 const char *kssdhip_last_error(void);
@@ -224,6 +225,53 @@ typedef struct {
 int kssdhip_epilogue_dev(const kssd_ctx_mut2_t *d_ctx, const kssd_obj_section_t *d_obj,
 			 int nq, int nr, const uint32_t *d_qry_ctx, const uint32_t *d_ref_ctx,
 			 const kssdhip_epilogue_opts_t *opts, double *d_out, void *stream);
+
+/*
+ * Serving loop (no reference counterpart: the reference answers one request
+ * per process, stream_ref_sketches_one_qraw_lookup, command_ani.c:2490-2666).
+ * A server keeps the index resident and answers query sketches submitted
+ * from any host thread with continuous batching: whenever one of `slots`
+ * streams is free, the queued requests at the head of the queue are admitted
+ * into one launch while their keys and dense counter rows fit that slot's
+ * share of `budget_bytes` of device memory (and pinned host memory of the
+ * same size); requests arriving meanwhile go to the next free slot, so
+ * batches overlap on the device.  Results are exactly those of
+ * kssdhip_count_block for the query alone.
+ */
+typedef struct kssdhip_server kssdhip_server;
+typedef struct {
+	size_t budget_bytes; /* device bytes for in-flight query state, split over the slots */
+	int slots;           /* concurrent batches (HIP streams), 1..16 */
+	int max_batch;       /* queries per launch */
+	int max_queued;      /* host queue bound; submit returns KSSDHIP_EAGAIN beyond it */
+	uint32_t flags;      /* KSSDHIP_UNIQUE, KSSDHIP_IGNORECONFLICT */
+} kssdhip_serve_opts_t;
+typedef struct {
+	double queue_ms;    /* submit -> launch */
+	double device_ms;   /* launch -> rows on the host */
+	double total_ms;    /* submit -> rows on the host */
+	int batch_queries;  /* queries in the launch that carried the request */
+} kssdhip_serve_stats_t;
+typedef struct {
+	int64_t batches, queries, rejected;
+	int max_batch_seen;
+	size_t slot_bytes;
+} kssdhip_serve_info_t;
+
+int kssdhip_serve_open(kssdhip_index *index, const kssdhip_serve_opts_t *opts, kssdhip_server **out);
+/* Queue one query sketch (n sorted keys, copied); returns a ticket >= 0 or
+ * a negative error (KSSDHIP_EAGAIN when max_queued requests are waiting). */
+int64_t kssdhip_serve_submit(kssdhip_server *server, const uint64_t *keys, size_t n);
+/* Wait for a ticket (timeout_ms < 0: no limit) and copy its ref_gnum-cell
+ * counter rows (unique rows when the server counts them; NULL skips a row).
+ * Each ticket is waited for once. */
+int kssdhip_serve_wait(kssdhip_server *server, int64_t ticket, kssd_ctx_mut2_t *ctx,
+		       kssd_obj_section_t *obj, kssd_ctx_mut2_t *unique_ctx,
+		       kssd_obj_section_t *unique_obj, int timeout_ms, kssdhip_serve_stats_t *stats);
+int kssdhip_serve_info(kssdhip_server *server, kssdhip_serve_info_t *info);
+/* Stop, drain the launched batches, fail the queued ones, free everything.
+ * No call on the server may follow or run concurrently. */
+int kssdhip_serve_close(kssdhip_server *server);
 
 #ifdef __cplusplus
 }

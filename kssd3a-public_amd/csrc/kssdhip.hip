@@ -565,11 +565,9 @@ int kssdhip_index_info(const kssdhip_index *ix, IndexView *v)
 }
 
 // ---------------------------------------------------------------- counting
-extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys,
-				       const uint64_t *d_skidx, int nq, uint64_t total,
-				       int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx,
-				       kssd_obj_section_t *d_obj, kssd_ctx_mut2_t *d_uctx,
-				       kssd_obj_section_t *d_uobj, void *stream)
+int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
+		   int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
+		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split)
 {
 	if (!ix)
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: null index");
@@ -580,7 +578,6 @@ extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: UNIQUE needs unique outputs");
 	if (flags & ~(KSSDHIP_LOWER | KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT | KSSDHIP_ACCUMULATE))
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: unknown flags 0x%x", flags);
-	hipStream_t s = (hipStream_t)stream;
 	const uint64_t cells = (uint64_t)nq * (uint64_t)ix->ref_gnum;
 	if (cells && !(flags & KSSDHIP_ACCUMULATE)) {
 		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
@@ -615,7 +612,7 @@ extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys
 		a.uctx = (unsigned long long *)d_uctx;
 		a.uobj = (unsigned long long *)d_uobj;
 		bool used = false;
-		int rc = kssd_launch_pjoin(ix, a, s, &used);
+		int rc = kssd_launch_pjoin(ix, a, s, split, &used);
 		if (rc)
 			return rc;
 		if (!used)
@@ -623,6 +620,18 @@ extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys
 	}
 	HIP_TRY(hipGetLastError());
 	return 0;
+}
+
+extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys,
+				       const uint64_t *d_skidx, int nq, uint64_t total,
+				       int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx,
+				       kssd_obj_section_t *d_obj, kssd_ctx_mut2_t *d_uctx,
+				       kssd_obj_section_t *d_uobj, void *stream)
+{
+	if (!ix)
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: null index");
+	return kssd_count_dev(ix, d_keys, d_skidx, nq, total, qry_gid_offset, flags, d_ctx, d_obj, d_uctx, d_uobj,
+			      (hipStream_t)stream, &ix->split);
 }
 
 extern "C" int kssdhip_count_block(kssdhip_index *ix, const uint64_t *section_sk,

@@ -103,8 +103,13 @@ struct kssdhip_index {
 // partitioned merge-join (kssdhip_pjoin.hip)
 int kssd_build_partitions(kssdhip_index *ix, int chunk);
 void kssd_free_partitions(kssdhip_index *ix);
-// true when the partitioned path was launched for this call
-int kssd_launch_pjoin(kssdhip_index *ix, const kssd::JoinArgs &a, hipStream_t s, bool *used);
+// true when the partitioned path was launched for this call; `split` holds
+// the call's split table (callers on different streams pass their own)
+int kssd_launch_pjoin(kssdhip_index *ix, const kssd::JoinArgs &a, hipStream_t s, DevBuf *split, bool *used);
+// kssdhip_count_block_dev with an explicit split-table workspace
+int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
+		   int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
+		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split);
 
 // read-only view of an index handle for other translation units
 struct IndexView {

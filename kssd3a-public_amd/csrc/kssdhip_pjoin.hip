@@ -519,7 +519,7 @@ int kssd_build_partitions(kssdhip_index *ix, int chunk)
 	return 0;
 }
 
-int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, bool *used)
+int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevBuf *split, bool *used)
 {
 	*used = false;
 	if (ix->P == 0 || ja.total >= 0xffffffffull)
@@ -531,10 +531,10 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, bool
 	const uint64_t sbytes = ((uint64_t)ix->P + 1) * (uint64_t)ja.nq * 4;
 	if (ix->block > 1024)
 		return 0;
-	int rc = ix->split.reserve(sbytes);
+	int rc = split->reserve(sbytes);
 	if (rc)
 		return rc;
-	uint32_t *S = (uint32_t *)ix->split.ptr;
+	uint32_t *S = (uint32_t *)split->ptr;
 	SplitArgs sa;
 	sa.keys = ja.keys;
 	sa.skidx = ja.skidx;

@@ -164,6 +164,24 @@ class Hip:
         return ms.value
 
 
+# ------------------------------------------------------------------ serving
+class ServeOpts(ctypes.Structure):
+    _fields_ = [("budget_bytes", S), ("slots", I), ("max_batch", I), ("max_queued", I), ("flags", U32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("batch_queries", I)]
+
+
+class ServeInfo(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_int64), ("queries", ctypes.c_int64), ("rejected", ctypes.c_int64),
+                ("max_batch_seen", I), ("slot_bytes", S)]
+
+
+EAGAIN = -5
+
+
 # ------------------------------------------------------------------ libkssdhip
 def load_kssdhip():
     so = PKG / "libkssdhip.so"
@@ -185,6 +203,12 @@ def load_kssdhip():
     lib.kssdhip_index_set_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
     lib.kssdhip_index_get_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
     lib.kssdhip_epilogue_dev.argtypes = [V, V, I, I, V, V, ctypes.POINTER(EpilogueOpts), V, V]
+    lib.kssdhip_serve_open.argtypes = [V, ctypes.POINTER(ServeOpts), ctypes.POINTER(V)]
+    lib.kssdhip_serve_submit.argtypes = [V, V, S]
+    lib.kssdhip_serve_submit.restype = ctypes.c_int64
+    lib.kssdhip_serve_wait.argtypes = [V, ctypes.c_int64, V, V, V, V, I, ctypes.POINTER(ServeStats)]
+    lib.kssdhip_serve_info.argtypes = [V, ctypes.POINTER(ServeInfo)]
+    lib.kssdhip_serve_close.argtypes = [V]
     return lib
 
 
@@ -260,6 +284,36 @@ class KssdHip:
     def pair_dev(self, d_a, n, d_b, m, obj_bits, ignore, d_out, d_ws, stream=None):
         self.check(self.lib.kssdhip_pair_features_dev(d_a, n, d_b, m, obj_bits, int(ignore),
                                                       d_out, d_ws, stream))
+
+    # ---- serving loop (kssdhip_serve_*)
+    def serve_open(self, h, budget_bytes, slots=2, max_batch=64, max_queued=4096, flags=0):
+        srv = V()
+        opts = ServeOpts(budget_bytes, slots, max_batch, max_queued, flags)
+        self.check(self.lib.kssdhip_serve_open(h, ctypes.byref(opts), ctypes.byref(srv)))
+        return srv
+
+    def serve_submit(self, srv, keys):
+        """Ticket (>= 0) or the negative error code (EAGAIN: queue full)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        return int(self.lib.kssdhip_serve_submit(srv, _p(keys), len(keys)))
+
+    def serve_wait(self, srv, ticket, ref_gnum, unique=False, timeout_ms=-1):
+        ctx = np.zeros((ref_gnum, 2), np.uint32)
+        obj = np.zeros((ref_gnum, 2), np.uint32)
+        uctx = np.zeros_like(ctx) if unique else None
+        uobj = np.zeros_like(obj) if unique else None
+        st = ServeStats()
+        self.check(self.lib.kssdhip_serve_wait(srv, ticket, _p(ctx), _p(obj), _p(uctx), _p(uobj), timeout_ms,
+                                               ctypes.byref(st)))
+        return ctx, obj, uctx, uobj, st
+
+    def serve_info(self, srv):
+        info = ServeInfo()
+        self.check(self.lib.kssdhip_serve_info(srv, ctypes.byref(info)))
+        return info
+
+    def serve_close(self, srv):
+        self.check(self.lib.kssdhip_serve_close(srv))
 
     def epilogue_dev(self, d_ctx, d_obj, nq, nr, d_qctx, d_rctx, opts, d_out, stream=None):
         self.check(self.lib.kssdhip_epilogue_dev(d_ctx, d_obj, nq, nr, d_qctx, d_rctx,

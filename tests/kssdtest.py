@@ -24,6 +24,7 @@ sys.path.insert(0, str(PKG))
 from kssdhip import (Features, Host, Hip, KssdHip, EpilogueOpts, SimSpec,  # noqa: E402,F401
                      load_kssdhip, load_kssdhost)
 GOLDEN = ROOT / "tests" / "golden"
+KSSDHIP_EINVAL, KSSDHIP_EAGAIN = -1, -5
 STAT = struct.Struct("<I??2x6i")
 GID_NBITS = 20
 REC_DTYPE = np.dtype([("ctxgid", "<u8"), ("obj", "<u4")], align=False)
