@@ -98,6 +98,159 @@ def cpu_baseline(records, nref, obj_bits, qkeys, qidx, qctx, rctx, budget_s, thr
     return nq, dt, ctx, obj, out
 
 
+def readset_like(rng, keys, coverage, obj_bits=20):
+    """Key set shaped like a read-set sketch (`--asone --conflict`) of a genome
+    read at `coverage`x with 1% per-base read errors: the genome's keys plus
+    sequencing-error entries, about 0.27 per key per unit of coverage
+    ((1 - 0.99^32) of the kept 32-mers hold an error).  10 of the 32 bases are
+    object bases, so 10/32 of the error entries keep a true context with one
+    object lane changed (a conflict); the rest land in new contexts."""
+    n = len(keys)
+    ne = int(0.27 * coverage * n)
+    n_obj = int(ne * 10 / 32)
+    src = rng.integers(0, max(n, 1), size=n_obj)
+    lane = rng.integers(0, obj_bits // 2, size=n_obj).astype(np.uint64)
+    flip = rng.integers(1, 4, size=n_obj).astype(np.uint64)
+    e1 = keys[src] ^ (flip << (np.uint64(2) * lane)) if n else np.zeros(0, np.uint64)
+    m = ne - n_obj
+    e2 = ((rng.integers(1, 1 << 44, size=m, dtype=np.uint64) << np.uint64(obj_bits))
+          | rng.integers(0, 1 << obj_bits, size=m, dtype=np.uint64))
+    return np.unique(np.concatenate([keys, e1, e2]))
+
+
+def serve_bench(args, host, hip, threads, plain_s_per_pair):
+    """Serving loop under Poisson load against a large resident index.
+
+    Index: args.serve_refs genomes = clades of 20, each clade a random
+    ancestor (lognormal length as above) whose members carry U(1%, 15%)
+    substitutions.  Requests: 70% assemblies (a new member of a random clade)
+    and 30% read sets (a reference genome plus 0.5% substitutions, read at
+    U(5, 30)x, see readset_like), arriving as a Poisson stream at
+    args.serve_rate per second.  Latency = arrival -> counter rows on the
+    host, measured per request.  The plain kernel's per-query figure is its
+    measured per-pair time times the index size."""
+    import threading
+    rng = np.random.default_rng(args.seed + 100)
+    nref = args.serve_refs - args.serve_refs % 20
+    nclade = nref // 20
+    clen = np.clip(rng.lognormal(np.log(4e6), 0.5, size=nclade), 1e6, 12e6).astype(np.uint64)
+    cseed = [int(args.seed * 9_000_011 + c + 1) for c in range(nclade)]
+    specs = [kh.SimSpec(cseed[c], int(clen[c]), int(args.seed * 5_000_003 + 20 * c + m + 1),
+                        float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0)
+             for c in range(nclade) for m in range(20)]
+    t = time.time()
+    rk, ri = host.simsketch(specs, threads=threads)
+    rec = host.build_index(rk, ri, 20, threads=threads)
+    del rk
+    log(f"serving index: {nref} genomes ({nclade} clades of 20), {len(rec)} records, built in "
+        f"{time.time() - t:.1f}s")
+    h = hip.open_index(rec, nref, 20)
+    nq = args.serve_queries
+    is_asm = rng.random(nq) < 0.7
+    qspecs, cov = [], np.zeros(nq)
+    for i in range(nq):
+        if is_asm[i]:
+            c = int(rng.integers(0, nclade))
+            qspecs.append(kh.SimSpec(cseed[c], int(clen[c]), int(args.seed * 3_000_017 + i + 1),
+                                     float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0))
+        else:
+            r = specs[int(rng.integers(0, nref))]
+            # the reference genome itself, then 0.5% sample substitutions
+            qspecs.append(kh.SimSpec(r.anc_seed, r.anc_len, int(args.seed * 3_000_017 + i + 1), 0.005, 0.0,
+                                     r.mut_seed, r.sub_rate))
+            cov[i] = rng.uniform(5, 30)
+    qk, qi = host.simsketch(qspecs, threads=threads)
+    queries = []
+    for i in range(nq):
+        k = qk[qi[i]:qi[i + 1]]
+        queries.append(k if is_asm[i] else readset_like(rng, k, cov[i]))
+    del qk
+    sizes = np.array([len(q) for q in queries])
+    log(f"serving requests: {nq} ({int(is_asm.sum())} assemblies, median {int(np.median(sizes[is_asm]))} keys; "
+        f"{int((~is_asm).sum())} read sets, median {int(np.median(sizes[~is_asm])) if (~is_asm).any() else 0} keys)")
+    budget = 2 << 30
+    srv = hip.serve_open(h, budget_bytes=budget, slots=4, max_batch=32, max_queued=1 << 20)
+    # warm the server (first launches load kernels)
+    for q in queries[:8]:
+        hip.serve_wait(srv, hip.serve_submit(srv, q), nref)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=nq))
+    tickets = [None] * nq
+    late = np.zeros(nq)
+    t0 = time.perf_counter() + 0.05
+
+    def producer():
+        for i in range(nq):
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            late[i] = max(0.0, time.perf_counter() - (t0 + arrivals[i]))
+            tickets[i] = hip.serve_submit(srv, queries[i])
+
+    th = threading.Thread(target=producer)
+    th.start()
+    lat = np.zeros(nq)
+    bsz = np.zeros(nq, np.int64)
+    for i in range(nq):
+        while tickets[i] is None:
+            time.sleep(0.0002)
+        if tickets[i] < 0:
+            raise RuntimeError(f"serve_submit failed: {tickets[i]}")
+        _, _, _, _, st = hip.serve_wait(srv, tickets[i], nref)
+        lat[i] = st.total_ms + late[i] * 1e3
+        bsz[i] = st.batch_queries
+    t_end = time.perf_counter()
+    th.join()
+    info = hip.serve_info(srv)
+    wall = t_end - t0
+    # saturation: every request submitted at once, so admission fills batches
+    tb = time.perf_counter()
+    burst_t = [hip.serve_submit(srv, q) for q in queries]
+    blat, bbsz = np.zeros(nq), np.zeros(nq, np.int64)
+    for i, tk in enumerate(burst_t):
+        _, _, _, _, st = hip.serve_wait(srv, tk, nref)
+        blat[i], bbsz[i] = st.total_ms, st.batch_queries
+    burst_wall = time.perf_counter() - tb
+    hip.serve_close(srv)
+    # CPU: the reference's forced-index per-query path (fenceposts over the
+    # whole index rebuilt per request, then the lookups), on 2 requests
+    sys.path.insert(0, str(ROOT / "tests"))
+    import kssdtest as kt
+    orc = kt.Oracle()
+    cpu_ms = []
+    for i in range(2):
+        q = queries[i]
+        ctx = np.zeros((1, nref, 2), np.uint32)
+        obj = np.zeros((1, nref, 2), np.uint32)
+        t = time.perf_counter()
+        orc.lib.orc_count_block(kt._p(rec), len(rec), nref, 20, kt._p(q),
+                                kt._p(np.array([0, len(q)], np.uint64)), 1, 0, 0, kt._p(ctx), kt._p(obj),
+                                None, None, threads)
+        cpu_ms.append((time.perf_counter() - t) * 1e3)
+    hip.close_index(h)
+    res = {
+        "index_genomes": nref, "index_records": int(len(rec)), "requests": nq,
+        "arrival_rate_per_s": args.serve_rate, "mix": "70% assemblies / 30% read sets",
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "p99_assembly": float(np.percentile(lat[is_asm], 99)),
+                       "p99_readset": float(np.percentile(lat[~is_asm], 99)) if (~is_asm).any() else None,
+                       "max": float(lat.max()), "what": "arrival -> counter rows on the host"},
+        "sustained_pairs_per_s": nq * nref / wall,
+        "batches": int(info.batches), "mean_batch": float(bsz.mean()), "max_batch_seen": int(info.max_batch_seen),
+        "saturated": {"what": "all requests submitted at once", "sustained_pairs_per_s": nq * nref / burst_wall,
+                      "queries_per_s": nq / burst_wall, "mean_batch": float(bbsz.mean()),
+                      "latency_ms_p50": float(np.percentile(blat, 50)),
+                      "latency_ms_p99": float(np.percentile(blat, 99))},
+        "budget_bytes": budget, "slots": 4,
+        "plain_kernel_per_query_ms": plain_s_per_pair * nref * 1e3,
+        "cpu_forced_index_per_query_ms": float(np.mean(cpu_ms)), "cpu_threads": threads,
+    }
+    log(f"serving: p50 {res['latency_ms']['p50']:.2f} ms, p99 {res['latency_ms']['p99']:.2f} ms, "
+        f"{res['sustained_pairs_per_s']:.3e} pairs/s over {wall:.1f}s, mean batch {bsz.mean():.2f}; "
+        f"plain per query {res['plain_kernel_per_query_ms']:.0f} ms, CPU per query {np.mean(cpu_ms):.0f} ms; "
+        f"saturated {res['saturated']['sustained_pairs_per_s']:.3e} pairs/s, mean batch {bbsz.mean():.1f}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +264,10 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all)")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--latency-queries", type=int, default=64)
+    ap.add_argument("--serve-refs", type=int, default=20000,
+                    help="reference genomes of the serving index (1000 clades of 20); 0 skips serving")
+    ap.add_argument("--serve-queries", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--copy-probe-mib", type=int, default=256,
                     help="size of a device-to-device copy run once before the timed region; its "
                          "HBM counters calibrate the join kernel's (0 = skip)")
@@ -186,9 +343,33 @@ def main():
         f"join {np.mean(join_ms):.3f} ms, {pairs_per_s:.3e} pairs/s")
 
     # ---------------- results back + parity on the CPU-baseline sample
-    ctx = rt.download(d_ctx, np.uint32, 2 * cells).reshape(nqry, nref, 2)
-    obj = rt.download(d_obj, np.uint32, 2 * cells).reshape(nqry, nref, 2)
-    ani = rt.download(d_out, np.float64, cells).reshape(nqry, nref)
+    ctx = rt.download(d_ctx, np.uint32, 2 * cells).reshape(nqry, nref, 2).copy()
+    obj = rt.download(d_obj, np.uint32, 2 * cells).reshape(nqry, nref, 2).copy()
+    ani = rt.download(d_out, np.float64, cells).reshape(nqry, nref).copy()
+
+    # ---------------- the same step captured in a HIP graph (counter zeroing by
+    # fill kernels inside the counting call, no memset nodes)
+    def graph_step():
+        hip.count_block_dev(h, d_qkeys, d_qidx, nqry, total, 0, 0, d_ctx, d_obj, None, None, stream)
+        hip.epilogue_dev(d_ctx, d_obj, nqry, nref, d_qctx, d_rctx, opts, d_out, stream)
+
+    graph_step()
+    rt.sync(stream)
+    g, gx = rt.capture(stream, graph_step)
+    for _ in range(args.warmup):
+        rt.graph_launch(gx, stream)
+    rt.sync(stream)
+    rt.record(e0, stream)
+    for _ in range(args.steps):
+        rt.graph_launch(gx, stream)
+    rt.record(e1, stream)
+    graph_ms = rt.elapsed_ms(e0, e1) / args.steps
+    graph_ok = bool(np.array_equal(rt.download(d_ctx, np.uint32, 2 * cells).reshape(nqry, nref, 2), ctx)
+                    and np.array_equal(rt.download(d_obj, np.uint32, 2 * cells).reshape(nqry, nref, 2), obj)
+                    and np.array_equal(rt.download(d_out, np.float64, cells).reshape(nqry, nref), ani,
+                                       equal_nan=True))
+    rt.graph_destroy(g, gx)
+    log(f"graph: {graph_ms:.3f} ms/step, agrees with eager: {graph_ok}")
     p_touched = int((ctx[:, :, 0] > 0).sum())
 
     # ---------------- per-query latency of the batched path (one query per launch);
@@ -258,6 +439,13 @@ def main():
     log(f"cpu oracle: {nq_cpu} queries x {nref} refs in {cpu_s:.2f}s on {threads} threads "
         f"= {cpu_pairs_per_s:.3e} pairs/s; counts bit-exact {counts_exact}, max rel ANI err {max_rel:.2e}")
 
+    # ---------------- serving loop on a large resident index
+    serving = serve_bench(args, host, hip, threads, plain_ms / npl / 1e3) if args.serve_refs else None
+
+    eager_pairs_per_s = pairs_per_s
+    if graph_ok:
+        ms_per_step = graph_ms
+        pairs_per_s = cells / (graph_ms / 1e3)
     rec = {
         "metric": "ANI pairs/sec on 1 MI355X, batched vs plain kernel; p50/p99 query latency",
         "value": pairs_per_s,
@@ -277,6 +465,10 @@ def main():
             "pairs": cells, "ref_entries": int(ridx[-1]), "query_entries": total,
             "index_records": len(records), "output": "dense counts + f64 CtxMoE ANI per pair",
         },
+        "step": {"timed": "HIP graph of one step (fill kernels + k_split + k_pjoin + FP64 epilogue)"
+                          if graph_ok else "eager launches",
+                 "graph_ms": graph_ms, "graph_agrees_with_eager": graph_ok,
+                 "eager_ms": total_ms / args.steps, "eager_pairs_per_s": eager_pairs_per_s},
         "plain_kernel": {"pairs_per_s": plain_pairs_per_s, "sample_pairs": npl,
                          "us_per_pair": plain_ms / npl * 1e3, "agrees_with_batched": plain_ok},
         "batched_over_plain": pairs_per_s / plain_pairs_per_s,
@@ -296,6 +488,7 @@ def main():
                          "sample": f"first {nq_cpu} queries x {nref} refs: oracle block count "
                                    "(fenceposts per block, guided OpenMP) + FP64 CtxMoE epilogue"},
         "gpu_over_cpu": pairs_per_s / cpu_pairs_per_s,
+        "serving": serving,
     }
     hip.close_index(h)
     print(json.dumps(rec), flush=True)

@@ -111,8 +111,37 @@ class Hip:
         L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), V, V]
         L.hipEventDestroy.argtypes = [V]
         L.hipSetDevice.argtypes = [I]
+        L.hipMemsetAsync.argtypes = [V, I, S, V]
+        L.hipStreamBeginCapture.argtypes = [V, I]
+        L.hipStreamEndCapture.argtypes = [V, ctypes.POINTER(V)]
+        L.hipGraphInstantiate.argtypes = [ctypes.POINTER(V), V, V, V, S]
+        L.hipGraphLaunch.argtypes = [V, V]
+        L.hipGraphExecDestroy.argtypes = [V]
+        L.hipGraphDestroy.argtypes = [V]
         self.check(L.hipSetDevice(device))
         self.device = device
+
+    # ---- graphs: capture the launches a callable makes on `stream`
+    def capture(self, stream, fn):
+        """Capture fn()'s work on stream into an instantiated graph; returns
+        (graph, exec).  fn must only enqueue device work (no allocation)."""
+        self.check(self.lib.hipStreamBeginCapture(stream, 0))  # hipStreamCaptureModeGlobal
+        try:
+            fn()
+        finally:
+            g = V()
+            rc = self.lib.hipStreamEndCapture(stream, ctypes.byref(g))
+        self.check(rc)
+        ex = V()
+        self.check(self.lib.hipGraphInstantiate(ctypes.byref(ex), g, None, None, 0))
+        return g, ex
+
+    def graph_launch(self, ex, stream):
+        self.check(self.lib.hipGraphLaunch(ex, stream))
+
+    def graph_destroy(self, g, ex):
+        self.check(self.lib.hipGraphExecDestroy(ex))
+        self.check(self.lib.hipGraphDestroy(g))
 
     def check(self, rc):
         if rc != 0:
