@@ -98,6 +98,23 @@ def cpu_baseline(records, nref, obj_bits, qkeys, qidx, qctx, rctx, budget_s, thr
     return nq, dt, ctx, obj, out
 
 
+def measured_traffic(tiling, nref, nqry):
+    """HBM bytes per step of k_split + k_pjoin from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload, when it was measured
+    with the tiling this run uses; (bytes or None, summary file name)."""
+    import re
+    f = ROOT / "profiles" / f"pjoin_split_c2_{nref}x{nqry}_traffic.md"
+    if not f.exists():
+        return None, None
+    text = f.read_text()
+    m = re.search(r"traffic_bytes_per_step=(\d+)", text)
+    mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+)", text)
+    if m and mt and tuple(map(int, mt.groups())) == (tiling["mode"], tiling["chunk_records"],
+                                                        tiling["block_threads"]):
+        return int(m.group(1)), f"profiles/{f.name}"
+    return None, f"profiles/{f.name} (measured with another tiling)"
+
+
 def readset_like(rng, keys, coverage, obj_bits=20):
     """Key set shaped like a read-set sketch (`--asone --conflict`) of a genome
     read at `coverage`x with 1% per-base read errors: the genome's keys plus
@@ -427,6 +444,7 @@ def main():
     algo_bytes = 8 * total + 12 * M + 16 * p_touched
     join_s = float(np.mean(join_ms)) / 1e3
     achieved = algo_bytes / join_s / 1e9
+    traffic, traffic_src = measured_traffic(tiling, nref, nqry)
 
     # ---------------- CPU baseline (oracle) + parity on its sample
     nq_cpu, cpu_s, cctx, cobj, cout = cpu_baseline(records, nref, obj_bits, qkeys, qidx, qctx, rctx,
@@ -479,7 +497,8 @@ def main():
         "parity": {"counts_bit_exact": counts_exact, "queries_checked": nq_cpu,
                    "max_rel_ani_err": max_rel, "ani_tolerance": 1e-12},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "traffic_GBs": traffic / join_s / 1e9 if traffic else None,
                      "kernel": "k_split+k_pjoin" if tiling["mode"] == 2 else "k_join", "algorithmic_bytes": algo_bytes,
                      "bytes_model": "8*N_Q + 12*M + 16*P_touched",
                      "N_Q": total, "M": M, "P_touched": p_touched,
