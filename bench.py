@@ -135,6 +135,140 @@ def readset_like(rng, keys, coverage, obj_bits=20):
     return np.unique(np.concatenate([keys, e1, e2]))
 
 
+def clade_specs(rng, nref, seed):
+    """nref genomes = clades of 20: a random ancestor per clade (lognormal
+    length, median 4 Mbp, sigma 0.5, clipped to 1-12 Mbp) whose members carry
+    U(1%, 15%) substitutions.  Returns (specs, clade seeds, clade lengths)."""
+    nclade = nref // 20
+    clen = np.clip(rng.lognormal(np.log(4e6), 0.5, size=nclade), 1e6, 12e6).astype(np.uint64)
+    cseed = [int(seed * 9_000_011 + c + 1) for c in range(nclade)]
+    specs = [kh.SimSpec(cseed[c], int(clen[c]), int(seed * 5_000_003 + 20 * c + m + 1),
+                        float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0)
+             for c in range(nclade) for m in range(20)]
+    return specs, cseed, clen
+
+
+def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
+    """Large resident index: args.index_refs genomes (clades of 20, as above),
+    built into the inverted index; two query sets counted in one batched
+    launch each, with inputs resident in HBM:
+      assemblies  args.index_queries new members of random clades (one
+                  reference-style block), dense counts + FP64 CtxMoE ANI;
+      read sets   args.index_readsets samples (a reference + 0.5%
+                  substitutions, read at U(10, 30)x, see readset_like),
+                  counted with the unique-owner counters (--qraw --unique-ctx).
+    Beside each: the oracle (C/OpenMP restatement of the reference's block
+    loop) on a bounded prefix of the same queries, and the plain kernel's
+    measured per-pair time."""
+    rng = np.random.default_rng(args.seed + 200)
+    nref = args.index_refs - args.index_refs % 20
+    specs, cseed, clen = clade_specs(rng, nref, args.seed + 200)
+    t = time.time()
+    rk, ri = host.simsketch(specs, threads=threads)
+    rec = host.build_index(rk, ri, 20, threads=threads)
+    rctx = host.ctx_counts(rk, ri, 20)
+    del rk
+    log(f"index bench: {nref} genomes, {len(rec)} records, built in {time.time() - t:.1f}s")
+    nclade = nref // 20
+    nq = args.index_queries
+    qspecs = []
+    for i in range(nq):
+        c = int(rng.integers(0, nclade))
+        qspecs.append(kh.SimSpec(cseed[c], int(clen[c]), int(args.seed * 4_000_037 + i + 1),
+                                 float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0))
+    qk, qi = host.simsketch(qspecs, threads=threads)
+    nrs = args.index_readsets
+    rs = []
+    for i in range(nrs):
+        r = specs[int(rng.integers(0, nref))]
+        rs.append(kh.SimSpec(r.anc_seed, r.anc_len, int(args.seed * 6_000_011 + i + 1), 0.005, 0.0,
+                             r.mut_seed, r.sub_rate))
+    sk, si = host.simsketch(rs, threads=threads)
+    reads = [readset_like(rng, sk[si[i]:si[i + 1]], rng.uniform(10, 30)) for i in range(nrs)]
+    del sk
+    rkeys = np.concatenate(reads)
+    ridx = np.zeros(nrs + 1, np.uint64)
+    ridx[1:] = np.cumsum([len(x) for x in reads])
+    h = hip.open_index(rec, nref, 20)
+    stream = rt.stream()
+    out = {"index_genomes": nref, "index_records": int(len(rec))}
+    orc = None
+    for name, keys, idx, flags in (("assemblies", qk, qi, 0), ("read_sets", rkeys, ridx, kh.UNIQUE)):
+        n = len(idx) - 1
+        cells = n * nref
+        unique = bool(flags & kh.UNIQUE)
+        d_k, d_i = rt.upload(keys), rt.upload(idx)
+        qctx = host.ctx_counts(keys, idx, 20, conflict=unique)
+        d_qc, d_rc = rt.upload(qctx), rt.upload(rctx)
+        d_c, d_o, d_out = rt.malloc(8 * cells), rt.malloc(8 * cells), rt.malloc(8 * cells)
+        d_uc = rt.malloc(8 * cells) if unique else None
+        d_uo = rt.malloc(8 * cells) if unique else None
+        # --qraw reads: naive metric (s4); assemblies: CtxMoE (s3)
+        opts = kh.EpilogueOpts(4 if unique else 3, 1, int(unique), 0, 0, 0, 8, 20, 44, 0.2 if unique else 0.5, 1.0)
+        total = int(idx[-1])
+
+        def step():
+            hip.count_block_dev(h, d_k, d_i, n, total, 0, flags, d_c, d_o, d_uc, d_uo, stream)
+            hip.epilogue_dev(d_c, d_o, n, nref, d_qc, d_rc, opts, d_out, stream)
+
+        if unique:
+            hip.set_tiling(h, mode=0)  # auto: lookup or partitioned join by block size
+            tiling = {"mode": 0}
+        else:
+            tiling, _ = tune.tune(hip, rt, h, d_k, d_i, n, total, nref, len(rec), d_c, d_o, stream,
+                                  flags=0, force=args.retune, log=log)
+        step()
+        rt.sync(stream)
+        e0, e1 = rt.event(), rt.event()
+        reps = 3
+        rt.record(e0, stream)
+        for _ in range(reps):
+            step()
+        rt.record(e1, stream)
+        ms = rt.elapsed_ms(e0, e1) / reps
+        pairs_per_s = cells / (ms / 1e3)
+        # oracle on a bounded prefix (fenceposts rebuilt per block, as the reference)
+        if orc is None:
+            sys.path.insert(0, str(ROOT / "tests"))
+            import kssdtest as kt
+            orc = kt.Oracle()
+        ncpu = 1
+        while True:
+            c_ctx = np.zeros((ncpu, nref, 2), np.uint32)
+            c_obj = np.zeros((ncpu, nref, 2), np.uint32)
+            c_uc = np.zeros_like(c_ctx)
+            c_uo = np.zeros_like(c_obj)
+            sub = np.ascontiguousarray(keys[: int(idx[ncpu])])
+            t = time.perf_counter()
+            orc.lib.orc_count_block(kt._p(rec), len(rec), nref, 20, kt._p(sub), kt._p(idx[: ncpu + 1]), ncpu,
+                                    0, flags, kt._p(c_ctx), kt._p(c_obj), kt._p(c_uc), kt._p(c_uo), threads)
+            cdt = time.perf_counter() - t
+            if cdt > args.cpu_budget / 6 or ncpu >= n:
+                break
+            ncpu = min(n, ncpu * 4)
+        g_ctx = rt.download(d_c, np.uint32, 2 * ncpu * nref).reshape(ncpu, nref, 2)
+        g_obj = rt.download(d_o, np.uint32, 2 * ncpu * nref).reshape(ncpu, nref, 2)
+        exact = bool(np.array_equal(g_ctx, c_ctx) and np.array_equal(g_obj, c_obj))
+        if unique:
+            exact = exact and bool(np.array_equal(rt.download(d_uc, np.uint32, 2 * ncpu * nref).reshape(ncpu, nref, 2),
+                                                  c_uc))
+        cpu_pps = ncpu * nref / cdt
+        plain_pps = 1.0 / plain_s_per_pair
+        out[name] = {"queries": n, "query_entries": total, "pairs": cells, "ms": ms, "pairs_per_s": pairs_per_s,
+                     "over_plain": pairs_per_s / plain_pps, "cpu_pairs_per_s": cpu_pps, "cpu_queries": ncpu,
+                     "over_cpu": pairs_per_s / cpu_pps, "counts_bit_exact_on_cpu_sample": exact,
+                     "flags": "unique-ctx" if unique else "none", "tiling": tiling}
+        log(f"index bench {name}: {n} x {nref} in {ms:.2f} ms = {pairs_per_s:.3e} pairs/s "
+            f"({pairs_per_s / plain_pps:.0f}x plain, {pairs_per_s / cpu_pps:.0f}x CPU on {ncpu} queries), "
+            f"bit-exact {exact}")
+        for d in (d_k, d_i, d_qc, d_rc, d_c, d_o, d_out, d_uc, d_uo):
+            if d:
+                rt.free(d)
+    hip.close_index(h)
+    out["cpu_threads"] = threads
+    return out
+
+
 def serve_bench(args, host, hip, threads, plain_s_per_pair):
     """Serving loop under Poisson load against a large resident index.
 
@@ -150,11 +284,7 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
     rng = np.random.default_rng(args.seed + 100)
     nref = args.serve_refs - args.serve_refs % 20
     nclade = nref // 20
-    clen = np.clip(rng.lognormal(np.log(4e6), 0.5, size=nclade), 1e6, 12e6).astype(np.uint64)
-    cseed = [int(args.seed * 9_000_011 + c + 1) for c in range(nclade)]
-    specs = [kh.SimSpec(cseed[c], int(clen[c]), int(args.seed * 5_000_003 + 20 * c + m + 1),
-                        float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0)
-             for c in range(nclade) for m in range(20)]
+    specs, cseed, clen = clade_specs(rng, nref, args.seed)
     t = time.time()
     rk, ri = host.simsketch(specs, threads=threads)
     rec = host.build_index(rk, ri, 20, threads=threads)
@@ -281,6 +411,10 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all)")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--latency-queries", type=int, default=64)
+    ap.add_argument("--index-refs", type=int, default=10000,
+                    help="reference genomes of the large-index bench (500 clades of 20); 0 skips it")
+    ap.add_argument("--index-queries", type=int, default=4096)
+    ap.add_argument("--index-readsets", type=int, default=200)
     ap.add_argument("--serve-refs", type=int, default=20000,
                     help="reference genomes of the serving index (1000 clades of 20); 0 skips serving")
     ap.add_argument("--serve-queries", type=int, default=2000)
@@ -458,6 +592,7 @@ def main():
         f"= {cpu_pairs_per_s:.3e} pairs/s; counts bit-exact {counts_exact}, max rel ANI err {max_rel:.2e}")
 
     # ---------------- serving loop on a large resident index
+    big_index = index_bench(args, host, hip, rt, threads, plain_ms / npl / 1e3) if args.index_refs else None
     serving = serve_bench(args, host, hip, threads, plain_ms / npl / 1e3) if args.serve_refs else None
 
     eager_pairs_per_s = pairs_per_s
@@ -507,6 +642,7 @@ def main():
                          "sample": f"first {nq_cpu} queries x {nref} refs: oracle block count "
                                    "(fenceposts per block, guided OpenMP) + FP64 CtxMoE epilogue"},
         "gpu_over_cpu": pairs_per_s / cpu_pairs_per_s,
+        "index_10k": big_index,
         "serving": serving,
     }
     hip.close_index(h)
