@@ -5,6 +5,8 @@
  *                              directories: options of command_ani_wrapper.c:77-135,
  *                              path choice of run_sketch_ani (:1082-1230),
  *                              every count computed by libkssdhip on the GPU
+ *   kssd3a-hip matrix [opts]   the reference's `kssd3a matrix` distance matrices
+ *                              (full / triangle, one metric per run)
  *   kssd3a-hip sketch -i DIR   inverted index of a sketch directory
  *                              (gen_inverted_index4comblco, command_sketch.c:916-938)
  *
@@ -726,6 +728,61 @@ static int run_rect_detail(const ani_args_t *a, detail_path_t path)
  * full matrix prints fixed-6 cells).  Pair (q, r) with q > r is evaluated
  * with q as the query; the upper half mirrors it.
  */
+/* Value of pair (q, r), q > r, of a self comparison from its counts. */
+typedef double (*pair_value_fn)(const void *arg, int q, int r, const kssd_ctx_mut2_t *c, const kssd_obj_section_t *o);
+
+/* Lower triangle of a self comparison: the pairs (q, r), r < q, counted on
+ * the GPU in query blocks with KSSDHIP_LOWER (count_ctx_obj_frm_comb_sketch_
+ * section_lower, command_ani.h:635-721) and valued on the host threads;
+ * value (q, r) is at q (q - 1) / 2 + r. */
+static double *self_lower(const kssd_sketchdir_t *s, int threads, bool ignoreconflict, pair_value_fn fn,
+			  const void *arg)
+{
+	const int n = s->n;
+	kssd_ctxgidobj_t *built = s->records ? NULL : build_records(s, threads);
+	const kssd_ctxgidobj_t *rec = s->records ? s->records : built;
+	kssdhip_index *ix = n ? gpu_index(rec, s->index[n], n, s->obj_bits) : NULL;
+	free(built);
+	stage("index on the GPU");
+	const size_t lower_n = n > 1 ? (size_t)n * (size_t)(n - 1) / 2 : 0;
+	double *lower = xmalloc(lower_n * sizeof *lower);
+	const uint32_t flags = KSSDHIP_LOWER | (ignoreconflict ? KSSDHIP_IGNORECONFLICT : 0);
+	const int blk = block_queries(n, n, false);
+	kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * (n ? n : 1) * sizeof *ctx);
+	kssd_obj_section_t *obj = xmalloc((size_t)blk * (n ? n : 1) * sizeof *obj);
+	for (int q0 = 0; q0 < n; q0 += blk) {
+		const int nb = n - q0 < blk ? n - q0 : blk;
+		hip_check(kssdhip_count_block(ix, s->keys + s->index[q0], s->index + q0, nb, q0, flags, ctx, obj, NULL,
+					      NULL),
+			  "GPU counting failed");
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+		for (int i = 0; i < nb; ++i) {
+			const int q = q0 + i;
+			for (int r = 0; r < q; ++r)
+				lower[(size_t)q * (q - 1) / 2 + r] =
+					fn(arg, q, r, &ctx[(size_t)i * n + r], &obj[(size_t)i * n + r]);
+		}
+	}
+	if (ix)
+		kssdhip_index_close(ix);
+	free(ctx);
+	free(obj);
+	return lower;
+}
+
+struct ani_cell {
+	const kssd_report_opts_t *o;
+	const uint32_t *qc, *rc;
+	const kssd_infile_meta_t *meta;
+};
+
+static double ani_cell_value(const void *arg, int q, int r, const kssd_ctx_mut2_t *c, const kssd_obj_section_t *b)
+{
+	const struct ani_cell *a = arg;
+	return kssd_matrix_value(a->o, c->num_ctx, b->diff_obj, b->diff_obj_section, c->num_mut2_ctx, a->qc[q], a->rc[r],
+				 a->meta ? &a->meta[q] : NULL, a->meta ? &a->meta[r] : NULL);
+}
+
 static int run_self_matrix(const ani_args_t *a, bool fixed6, bool index_from_file)
 {
 	kssd_sketchdir_t s;
@@ -735,34 +792,8 @@ static int run_self_matrix(const ani_args_t *a, bool fixed6, bool index_from_fil
 	const kssd_report_opts_t *o = &a->r;
 	uint32_t *qc = entry_or_run_totals(&s, s.stat.conflict, false);
 	uint32_t *rc = o->ignoreconflict ? entry_or_run_totals(&s, s.stat.conflict, true) : qc;
-	kssd_ctxgidobj_t *built = s.records ? NULL : build_records(&s, a->p);
-	const kssd_ctxgidobj_t *rec = s.records ? s.records : built;
-	kssdhip_index *ix = n ? gpu_index(rec, s.index[n], n, s.obj_bits) : NULL;
-	free(built);
-	stage("index on the GPU");
-	const size_t lower_n = n > 1 ? (size_t)n * (size_t)(n - 1) / 2 : 0;
-	double *lower = xmalloc(lower_n * sizeof *lower);
-	const uint32_t flags = KSSDHIP_LOWER | (o->ignoreconflict ? KSSDHIP_IGNORECONFLICT : 0);
-	const int blk = block_queries(n, n, false);
-	kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * (n ? n : 1) * sizeof *ctx);
-	kssd_obj_section_t *obj = xmalloc((size_t)blk * (n ? n : 1) * sizeof *obj);
-	for (int q0 = 0; q0 < n; q0 += blk) {
-		const int nb = n - q0 < blk ? n - q0 : blk;
-		hip_check(kssdhip_count_block(ix, s.keys + s.index[q0], s.index + q0, nb, q0, flags, ctx, obj, NULL,
-					      NULL),
-			  "GPU counting failed");
-#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
-		for (int i = 0; i < nb; ++i) {
-			const int q = q0 + i;
-			for (int r = 0; r < q; ++r) {
-				const kssd_ctx_mut2_t *c = &ctx[(size_t)i * n + r];
-				const kssd_obj_section_t *b = &obj[(size_t)i * n + r];
-				lower[(size_t)q * (q - 1) / 2 + r] =
-					kssd_matrix_value(o, c->num_ctx, b->diff_obj, b->diff_obj_section, c->num_mut2_ctx,
-							  qc[q], rc[r], s.meta ? &s.meta[q] : NULL, s.meta ? &s.meta[r] : NULL);
-			}
-		}
-	}
+	const struct ani_cell cell = {o, qc, rc, s.meta};
+	double *lower = self_lower(&s, a->p, o->ignoreconflict, ani_cell_value, &cell);
 	stage("pairs counted and evaluated");
 	FILE *fp = open_out(a);
 	kssd_buf_t b = {0};
@@ -799,10 +830,6 @@ static int run_self_matrix(const ani_args_t *a, bool fixed6, bool index_from_fil
 	kssd_buf_free(&b);
 	stage("formatted");
 	close_out(fp);
-	if (ix)
-		kssdhip_index_close(ix);
-	free(ctx);
-	free(obj);
 	free(lower);
 	if (rc != qc)
 		free(rc);
@@ -990,6 +1017,274 @@ static int cmd_ani(int argc, char **argv)
 	return run_ani(&a);
 }
 
+/* ------------------------------------------------------------ matrix */
+/*
+ * The reference's `kssd3a matrix` for its distance-matrix reports
+ * (command_matrix_wrapper.c:109-135 defaults, :399-470 checks; compute_triangle
+ * and compute_matrix, command_matrix.c:1293-1348): one sketch gives the
+ * indexed lower-triangle path (matrix_write_indexed_self_matrix, :439-619)
+ * printed as a triangle or a full matrix; -r/-q give the dense report
+ * (matrix_compute_dense, :258-315).  One metric per run; the sparse, cluster
+ * and dedup-plan reports are not offered.
+ */
+typedef struct {
+	char refdir[PATH_MAX], qrydir[PATH_MAX], outf[PATH_MAX];
+	int metric;
+	int format; /* 0 auto, 1 full, 2 triangle */
+	double e, diag;
+	bool d;
+	int p, device;
+} matrix_args_t;
+
+struct matrix_cell {
+	int metric;
+	double e;
+	const uint32_t *tot;
+	int drfold;
+	unsigned obj_bits, ctx_bits;
+};
+
+static double matrix_cell_value(const void *arg, int q, int r, const kssd_ctx_mut2_t *c, const kssd_obj_section_t *b)
+{
+	const struct matrix_cell *m = arg;
+	return kssd_matrix_context_value(m->metric, m->e, c->num_ctx, b->diff_obj, b->diff_obj_section, c->num_mut2_ctx,
+					 m->tot[q], m->tot[r], m->drfold, m->obj_bits, m->ctx_bits);
+}
+
+static void parse_matrix(int argc, char **argv, matrix_args_t *a)
+{
+	enum { M_FORMAT = 300, M_DIAGV, M_DEVICE, M_HELP, M_NOT_OFFERED };
+	static const struct option longopts[] = {
+		{"ref", required_argument, NULL, 'r'},
+		{"query", required_argument, NULL, 'q'},
+		{"metric", required_argument, NULL, 'm'},
+		{"format", required_argument, NULL, M_FORMAT},
+		{"exception", required_argument, NULL, 'e'},
+		{"diagonal", no_argument, NULL, 'd'},
+		{"diagonal-value", required_argument, NULL, M_DIAGV},
+		{"outfile", required_argument, NULL, 'o'},
+		{"threads", required_argument, NULL, 'p'},
+		{"device", required_argument, NULL, M_DEVICE},
+		{"help", no_argument, NULL, M_HELP},
+		{"sparse", no_argument, NULL, M_NOT_OFFERED},
+		{"cut", required_argument, NULL, M_NOT_OFFERED},
+		{"max-afcut", required_argument, NULL, M_NOT_OFFERED},
+		{"ctxcut", required_argument, NULL, M_NOT_OFFERED},
+		{"query-sketch-list", required_argument, NULL, M_NOT_OFFERED},
+		{"matrix-format", required_argument, NULL, M_NOT_OFFERED},
+		{"glist", required_argument, NULL, M_NOT_OFFERED},
+		{"ani", no_argument, NULL, M_NOT_OFFERED},
+		{NULL, 0, NULL, 0},
+	};
+	memset(a, 0, sizeof *a);
+	a->metric = KSSD_MM_NAIVE;
+	a->e = 1.0;
+	a->p = omp_get_max_threads();
+	optind = 1;
+	int c;
+	while ((c = getopt_long(argc, argv, "r:q:m:e:do:p:g:c:", longopts, NULL)) != -1) {
+		switch (c) {
+		case 'r':
+			copy_path(a->refdir, sizeof a->refdir, "-r/--ref", optarg);
+			break;
+		case 'q':
+			copy_path(a->qrydir, sizeof a->qrydir, "-q/--query", optarg);
+			break;
+		case 'm':
+			if (strpbrk(optarg, "&|")) {
+				a->metric = -2; /* an expression: rejected below for these formats */
+			} else {
+				a->metric = kssd_matrix_metric(optarg);
+				if (a->metric < 0)
+					usage_error("-m/--metric must be one metric such as ctx-naive, p_dist, mash, or aaf, "
+						    "or an expression like 'ctx-naive&aaf' or 'ctx-moe|mash'");
+			}
+			break;
+		case M_FORMAT:
+			if (strcmp(optarg, "full") == 0)
+				a->format = 1;
+			else if (strcmp(optarg, "triangle") == 0)
+				a->format = 2;
+			else if (!strcmp(optarg, "edges") || !strcmp(optarg, "sparse") || !strcmp(optarg, "clusters") ||
+				 !strcmp(optarg, "dedup-plan"))
+				usage_error("--format %s is not offered by this front end (full and triangle are)", optarg);
+			else
+				usage_error("--format must be full, triangle, edges, sparse, clusters, or dedup-plan");
+			break;
+		case 'e':
+			a->e = parse_double("-e/--exception", optarg, 0.0, 1e300);
+			break;
+		case 'd':
+			a->d = true;
+			break;
+		case M_DIAGV:
+			a->diag = parse_double("--diagonal-value", optarg, 0.0, 1e300);
+			a->d = true;
+			break;
+		case 'o':
+			copy_path(a->outf, sizeof a->outf, "-o/--outfile", optarg);
+			break;
+		case 'p':
+			a->p = (int)parse_int("-p/--threads", optarg, 1, 65536);
+			break;
+		case M_DEVICE:
+			a->device = (int)parse_int("--device", optarg, 0, 1024);
+			break;
+		case M_HELP:
+			printf("Usage: %s matrix [-m METRIC] [--format full|triangle] [-e E] [-d | --diagonal-value V] "
+			       "[-o FILE] (SKETCH | -q SKETCH | -r REF -q QRY)\n"
+			       "METRIC: ctx-naive (default), ctx-moe, p_dist, mash, aaf.\n",
+			       PROG);
+			exit(0);
+		case 'g':
+		case 'c':
+		case M_NOT_OFFERED:
+			usage_error("%s is not offered by this front end (distance matrices only)", argv[optind - 1]);
+		default:
+			fprintf(stderr, "Try `%s %s --help' for more information.\n", PROG, g_cmd);
+			exit(EX_USAGE);
+		}
+	}
+	const int npos = argc - optind;
+	if (!a->qrydir[0] && npos == 1)
+		copy_path(a->qrydir, sizeof a->qrydir, "sketch", argv[optind]);
+	else if (npos > 0)
+		usage_error("matrix accepts at most one positional sketch argument; use -r/-q for rectangular reports");
+	if (!a->qrydir[0])
+		usage_error("missing sketch input; provide one sketch argument, -q/--query, or --query-sketch-list");
+	if (a->refdir[0] && a->format == 2)
+		usage_error("--format triangle is only valid for one sketch");
+	if (a->metric == -2)
+		usage_error("combined --metric expressions are only valid for --sparse/--format edges, clusters, or "
+			    "dedup-plan");
+}
+
+static int matrix_self(const matrix_args_t *a)
+{
+	kssd_sketchdir_t s;
+	const bool have = kssd_file_in_dir(a->qrydir, KSSD_SORTED_INDEX_FILE);
+	open_dir(a->qrydir, KSSD_SK_KEYS | (have ? KSSD_SK_RECORDS : 0), &s);
+	stage("sketch read");
+	const int n = s.n;
+	uint32_t *tot = entry_or_run_totals(&s, s.stat.conflict, false);
+	const struct matrix_cell cell = {a->metric, a->e, tot, s.stat.drfold, s.obj_bits, s.ctx_bits};
+	double *lower = self_lower(&s, a->p, false, matrix_cell_value, &cell);
+	stage("pairs counted and evaluated");
+	FILE *fp = a->outf[0] ? fopen(a->outf, "w") : stdout;
+	if (!fp)
+		err(errno, "%s", a->outf);
+	kssd_buf_t b = {0};
+	if (a->format == 1) {
+		put_header_names(fp, &s);
+		const uint32_t dg = kssd_matrix_scale_fixed6(a->diag);
+		for (int q = 0; q < n; ++q) {
+			kssd_buf_printf(&b, "%s", s.names[q]);
+			for (int r = 0; r < n; ++r) {
+				const uint32_t v = q == r ? dg
+						 : kssd_matrix_scale_fixed6(q > r ? lower[(size_t)q * (q - 1) / 2 + r]
+										  : lower[(size_t)r * (r - 1) / 2 + q]);
+				kssd_buf_printf(&b, "\t");
+				kssd_matrix_put_scaled6(&b, v);
+			}
+			kssd_buf_printf(&b, "\n");
+			put_buf(fp, &b);
+		}
+	} else {
+		for (int q = 0; q < n; ++q) {
+			kssd_buf_printf(&b, "%s", s.names[q]);
+			for (int r = 0; r < q; ++r) {
+				kssd_buf_printf(&b, "\t");
+				kssd_matrix_put_fixed6(&b, lower[(size_t)q * (q - 1) / 2 + r]);
+			}
+			if (a->d) {
+				kssd_buf_printf(&b, "\t");
+				kssd_matrix_put_fixed6(&b, a->diag);
+			}
+			kssd_buf_printf(&b, "\n");
+			put_buf(fp, &b);
+		}
+	}
+	kssd_buf_free(&b);
+	close_out(fp);
+	stage("formatted");
+	free(lower);
+	free(tot);
+	kssd_sketchdir_close(&s);
+	return 0;
+}
+
+static int matrix_rect(const matrix_args_t *a)
+{
+	kssd_sketchdir_t ref, qry;
+	open_dir(a->refdir, KSSD_SK_KEYS, &ref);
+	open_dir(a->qrydir, KSSD_SK_KEYS, &qry);
+	/* pairwise_check_compatible, pairwise_graph.c:262-273 */
+	if (ref.stat.hash_id != qry.stat.hash_id)
+		errx(EXIT_FAILURE, "pairwise_check_compatible(): ref hash_id %u != qry %u", ref.stat.hash_id,
+		     qry.stat.hash_id);
+	stage("sketches read");
+	const int R = ref.n, Q = qry.n;
+	const bool same = strcmp(a->refdir, a->qrydir) == 0;
+	uint32_t *qc = entry_or_run_totals(&qry, qry.stat.conflict, false);
+	uint32_t *rc = entry_or_run_totals(&ref, ref.stat.conflict, false);
+	kssd_ctx_mut2_t *ctx = xcalloc((size_t)Q * R, sizeof *ctx);
+	kssd_obj_section_t *obj = xcalloc((size_t)Q * R, sizeof *obj);
+	rect_counts(&ref, &qry, false, a->p, ctx, obj);
+	stage("pairs counted");
+	FILE *fp = a->outf[0] ? fopen(a->outf, "w") : stdout;
+	if (!fp)
+		err(errno, "%s", a->outf);
+	put_header_names(fp, &ref);
+	kssd_buf_t *bufs = xcalloc((size_t)Q, sizeof *bufs);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
+	for (int q = 0; q < Q; ++q) {
+		kssd_buf_printf(&bufs[q], "%s", qry.names[q]);
+		const size_t qlen = (size_t)(qry.index[q + 1] - qry.index[q]);
+		for (int r = 0; r < R; ++r) {
+			double v = a->diag;
+			if (!(same && q == r)) {
+				const kssd_ctx_mut2_t *c = &ctx[(size_t)q * R + r];
+				const kssd_obj_section_t *o = &obj[(size_t)q * R + r];
+				v = kssd_pairwise_value(a->metric, a->e, qlen, (size_t)(ref.index[r + 1] - ref.index[r]),
+							c->num_ctx, o->diff_obj, o->diff_obj_section, c->num_mut2_ctx, qc[q],
+							rc[r], ref.stat.drfold, ref.obj_bits, ref.ctx_bits);
+			}
+			kssd_buf_printf(&bufs[q], "\t%lf", v);
+		}
+		kssd_buf_printf(&bufs[q], "\n");
+	}
+	for (int q = 0; q < Q; ++q) {
+		put_buf(fp, &bufs[q]);
+		kssd_buf_free(&bufs[q]);
+	}
+	free(bufs);
+	close_out(fp);
+	stage("formatted");
+	free(ctx);
+	free(obj);
+	free(qc);
+	free(rc);
+	kssd_sketchdir_close(&ref);
+	kssd_sketchdir_close(&qry);
+	return 0;
+}
+
+static int cmd_matrix(int argc, char **argv)
+{
+	matrix_args_t a;
+	parse_matrix(argc, argv, &a);
+	hip_check(kssdhip_init(a.device), "no usable GPU");
+	stage("GPU ready");
+	if (a.refdir[0]) {
+		if (a.format == 0)
+			a.format = 1;
+		return matrix_rect(&a);
+	}
+	if (a.format == 0)
+		a.format = 2;
+	return matrix_self(&a);
+}
+
 /* ------------------------------------------------------------ sketch -i */
 static int cmd_sketch(int argc, char **argv)
 {
@@ -1035,14 +1330,16 @@ int main(int argc, char **argv)
 	g_t0 = now_s();
 	g_timing = env_flag("KSSDHIP_TIMING");
 	if (argc < 2 || strcmp(argv[1], "--help") == 0 || strcmp(argv[1], "-h") == 0) {
-		printf("Usage: %s ani|sketch [options]\n", PROG);
+		printf("Usage: %s ani|matrix|sketch [options]\n", PROG);
 		return argc < 2 ? EX_USAGE : 0;
 	}
 	g_cmd = argv[1];
 	if (strcmp(g_cmd, "ani") == 0)
 		return cmd_ani(argc - 1, argv + 1);
+	if (strcmp(g_cmd, "matrix") == 0)
+		return cmd_matrix(argc - 1, argv + 1);
 	if (strcmp(g_cmd, "sketch") == 0)
 		return cmd_sketch(argc - 1, argv + 1);
-	fprintf(stderr, "%s: unknown command '%s' (ani, sketch)\n", PROG, g_cmd);
+	fprintf(stderr, "%s: unknown command '%s' (ani, matrix, sketch)\n", PROG, g_cmd);
 	return EX_USAGE;
 }

@@ -857,3 +857,151 @@ void kssd_put_fixed6(kssd_buf_t *b, int32_t scaled)
 	}
 	kssd_buf_printf(b, "%u.%06u", mag / 1000000U, mag % 1000000U);
 }
+
+/* ------------------------------------------------------------ matrix command */
+int kssd_matrix_metric(const char *a)
+{
+	/* pairwise_metric_from_string, pairwise_graph.c:69-98 */
+	if (!a)
+		return -1;
+	if (!strcmp(a, "ctx-moe") || !strcmp(a, "moe"))
+		return KSSD_MM_MOE;
+	if (!strcmp(a, "ctx-naive") || !strcmp(a, "naive"))
+		return KSSD_MM_NAIVE;
+	if (!strcmp(a, "p_dist") || !strcmp(a, "pdist") || !strcmp(a, "p-distance") || !strcmp(a, "micro-obj") ||
+	    !strcmp(a, "micro_obj_rate"))
+		return KSSD_MM_PDIST;
+	if (!strcmp(a, "mash") || !strcmp(a, "mashd") || !strcmp(a, "0"))
+		return KSSD_MM_MASH;
+	if (!strcmp(a, "aaf") || !strcmp(a, "aafd") || !strcmp(a, "1"))
+		return KSSD_MM_AAF;
+	return -1;
+}
+
+double kssd_matrix_context_value(int metric, double e, uint32_t xny, uint32_t dobj, uint32_t dsec,
+				 uint32_t mut2, uint32_t qc, uint32_t rc, int drfold, unsigned obj_bits,
+				 unsigned ctx_bits)
+{
+	/* matrix_context_feature_value + matrix_context_metric_distance,
+	 * command_matrix.c:400-419, :621-661 (one metric) */
+	if (xny == 0 || qc == 0 || rc == 0)
+		return e;
+	const uint32_t k = ctx_bits / 2;
+	double d;
+	switch (metric) {
+	case KSSD_MM_NAIVE:
+		d = kssd_naive_dist(xny, dobj, dsec);
+		break;
+	case KSSD_MM_MOE:
+		d = kssd_moe_dist(xny, dobj, dsec, mut2, drfold);
+		break;
+	case KSSD_MM_PDIST:
+		d = kssd_p_dist(xny, dsec, obj_bits);
+		break;
+	case KSSD_MM_MASH:
+		d = kssd_mashD(k, rc, qc, xny);
+		break;
+	case KSSD_MM_AAF:
+		d = kssd_aafD(k, rc, qc, xny);
+		break;
+	default:
+		d = 1.0;
+	}
+	if (!isfinite(d))
+		return e;
+	return d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+}
+
+static double clean_distance(double d)
+{
+	/* pairwise_clean_distance, pairwise_graph.c:14-21 */
+	if (!isfinite(d))
+		return 1.0;
+	return d < 0.0 ? 0.0 : d;
+}
+
+double kssd_pairwise_value(int metric, double e, size_t qry_len, size_t ref_len, uint32_t xny, uint32_t dobj,
+			   uint32_t dsec, uint32_t mut2, uint32_t qc, uint32_t rc, int drfold, unsigned obj_bits,
+			   unsigned ctx_bits)
+{
+	/* pairwise_eval_arrays, pairwise_graph.c:301-360, read by
+	 * matrix_eval_value (command_matrix.c:82-85) */
+	if (qry_len == 0 || ref_len == 0 || xny == 0)
+		return e;
+	const double k = ctx_bits > 0 ? (double)(ctx_bits / 2) : 32.0;
+	switch (metric) {
+	case KSSD_MM_MASH: {
+		/* pairwise_mash_distance, pairwise_graph.c:23-36 */
+		const double denom = (double)qc + (double)rc - (double)xny;
+		if (denom <= 0.0)
+			return 1.0;
+		const double j = (double)xny / denom;
+		const double t = (2.0 * j) / (1.0 + j);
+		if (t <= 0.0)
+			return 1.0;
+		return clean_distance(-log(t) / k);
+	}
+	case KSSD_MM_AAF: {
+		/* pairwise_aaf_distance, pairwise_graph.c:38-50 */
+		const uint32_t mn = qc < rc ? qc : rc;
+		if (mn == 0)
+			return 1.0;
+		const double c = (double)xny / (double)mn;
+		if (c <= 0.0)
+			return 1.0;
+		return clean_distance(-log(c) / k);
+	}
+	case KSSD_MM_NAIVE:
+		return clean_distance(kssd_naive_dist(xny, dobj, dsec));
+	case KSSD_MM_PDIST:
+		return clean_distance(kssd_p_dist(xny, dsec, obj_bits));
+	default:
+		return clean_distance(kssd_moe_dist(xny, dobj, dsec, mut2, drfold));
+	}
+}
+
+static void put_frac6(kssd_buf_t *b, uint32_t frac)
+{
+	char f[7];
+	for (int i = 5; i >= 0; --i) {
+		f[i] = (char)('0' + frac % 10);
+		frac /= 10;
+	}
+	f[6] = '\0';
+	kssd_buf_printf(b, "%s", f);
+}
+
+void kssd_matrix_put_fixed6(kssd_buf_t *b, double v)
+{
+	/* matrix_kput_fixed6, command_matrix.c:103-124 */
+	if (!isfinite(v) || v > 900000000000.0 || v < -900000000000.0) {
+		kssd_buf_printf(b, "%lf", v);
+		return;
+	}
+	if (v < 0.0) {
+		kssd_buf_printf(b, "-");
+		v = -v;
+	}
+	const uint64_t scaled = (uint64_t)(v * 1000000.0 + 0.5);
+	kssd_buf_printf(b, "%ld.", (long)(scaled / 1000000ULL));
+	put_frac6(b, (uint32_t)(scaled % 1000000ULL));
+}
+
+uint32_t kssd_matrix_scale_fixed6(double v)
+{
+	/* matrix_scale_fixed6, command_matrix.c:126-135 */
+	if (!isfinite(v))
+		v = 1.0;
+	if (v < 0.0)
+		v = 0.0;
+	if (v > 4294.0)
+		v = 4294.0;
+	return (uint32_t)(v * 1000000.0 + 0.5);
+}
+
+void kssd_matrix_put_scaled6(kssd_buf_t *b, uint32_t scaled)
+{
+	/* matrix_kput_scaled6, command_matrix.c:137-149 */
+	kssd_buf_printf(b, "%u.", scaled / 1000000U);
+	put_frac6(b, scaled % 1000000U);
+}

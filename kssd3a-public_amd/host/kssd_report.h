@@ -153,6 +153,28 @@ double kssd_matrix_diagonal(const kssd_report_opts_t *o);
 int32_t kssd_scale_fixed6(double v);
 void kssd_put_fixed6(kssd_buf_t *b, int32_t scaled);
 
+/* ---- the `matrix` command's values and cells (command_matrix.c) ---- */
+enum { KSSD_MM_NAIVE = 0, KSSD_MM_MOE, KSSD_MM_PDIST, KSSD_MM_MASH, KSSD_MM_AAF };
+/* One metric name as pairwise_metric_from_string (pairwise_graph.c:69-98)
+ * accepts it; -1 when unknown. */
+int kssd_matrix_metric(const char *name);
+/* Indexed self matrices: matrix_context_feature_value for one metric
+ * (command_matrix.c:400-419, :621-661): e when nothing is shared or a
+ * total is 0 or the value is not finite, else clamped to [0, 1]. */
+double kssd_matrix_context_value(int metric, double e, uint32_t xny, uint32_t dobj, uint32_t dsec,
+				 uint32_t mut2, uint32_t qry_ctx, uint32_t ref_ctx, int drfold,
+				 unsigned obj_bits, unsigned ctx_bits);
+/* Dense reports: pairwise_eval_arrays' distance (pairwise_graph.c:301-360),
+ * or e when the pair is not valid (an empty sample or nothing shared). */
+double kssd_pairwise_value(int metric, double e, size_t qry_len, size_t ref_len, uint32_t xny,
+			   uint32_t dobj, uint32_t dsec, uint32_t mut2, uint32_t qry_ctx, uint32_t ref_ctx,
+			   int drfold, unsigned obj_bits, unsigned ctx_bits);
+/* matrix_kput_fixed6 / matrix_scale_fixed6 / matrix_kput_scaled6
+ * (command_matrix.c:103-149) */
+void kssd_matrix_put_fixed6(kssd_buf_t *b, double v);
+uint32_t kssd_matrix_scale_fixed6(double v);
+void kssd_matrix_put_scaled6(kssd_buf_t *b, uint32_t scaled);
+
 #ifdef __cplusplus
 }
 #endif

@@ -382,6 +382,15 @@ def load_kssdhost():
     lib.kssd_buf_free.restype = None
     lib.kssd_matrix_value.argtypes = [ctypes.POINTER(ReportOpts), U32, U32, U32, U32, U32, U32, V, V]
     lib.kssd_matrix_value.restype = D
+    lib.kssd_matrix_metric.argtypes = [ctypes.c_char_p]
+    lib.kssd_matrix_context_value.argtypes = [I, D, U32, U32, U32, U32, U32, U32, I, U, U]
+    lib.kssd_matrix_context_value.restype = D
+    lib.kssd_pairwise_value.argtypes = [I, D, S, S, U32, U32, U32, U32, U32, U32, I, U, U]
+    lib.kssd_pairwise_value.restype = D
+    lib.kssd_matrix_put_fixed6.argtypes = [ctypes.POINTER(Buf), D]
+    lib.kssd_matrix_scale_fixed6.argtypes = [D]
+    lib.kssd_matrix_scale_fixed6.restype = U32
+    lib.kssd_matrix_put_scaled6.argtypes = [ctypes.POINTER(Buf), U32]
     lib.kssd_refaf_predict_ani.argtypes = [D, D, U, U, U, U, I]
     lib.kssd_refaf_predict_ani.restype = D
     return lib

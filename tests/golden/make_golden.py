@@ -209,6 +209,23 @@ def make_metrics():
                                     env={"KSSD3A_ANI_MATRIX_DIRECT_THRESHOLD": "1000"}))
 
 
+def make_matrix_cmd():
+    """The `matrix` command on the genome sketches: one-sketch triangle and
+    full matrices for every single metric, the rectangular report, the
+    exception and diagonal options."""
+    out = HERE / "genomes"
+    R, Q, S = out / "ref", out / "qry", out / "all"
+    w = lambda name, text: (out / name).write_text(text)  # noqa: E731
+    for m in ("ctx-moe", "p_dist", "mash", "aaf"):
+        w(f"matrix_triangle_{m}.tsv", run(["matrix", "-m", m, S]))
+        w(f"matrix_full_{m}.tsv", run(["matrix", "--format", "full", "-m", m, S]))
+        w(f"matrix_rq_{m}.tsv", run(["matrix", "-r", R, "-q", Q, "-m", m]))
+    w("matrix_triangle_diag.tsv", run(["matrix", "--diagonal-value", "0.25", "-e", "2", S]))
+    w("matrix_full_diag.tsv", run(["matrix", "--format", "full", "--diagonal-value", "0.25", "-m", "aaf", S]))
+    w("matrix_rq_e2.tsv", run(["matrix", "-r", R, "-q", Q, "-e", "2", "-m", "p_dist"]))
+    w("matrix_self_rq.tsv", run(["matrix", "-r", S, "-q", S, "--diagonal-value", "0.5"]))
+
+
 def make_sketch():
     """Small FASTA inputs (multi-record, lowercase, non-ACGT runs) and the
     reference's sketches of them, at the default -f8 and at -f4."""
@@ -241,6 +258,7 @@ def main():
     make_genomes()
     make_stream()
     make_metrics()
+    make_matrix_cmd()
     make_sketch()
     print("golden fixtures written under", HERE)
 

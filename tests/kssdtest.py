@@ -239,7 +239,7 @@ class Oracle:
 
 FLAGS = {"LOWER": 1, "UNIQUE": 2, "IGNORECONFLICT": 4}
 
-from kssdhip import ReportOpts  # noqa: E402,F401
+from kssdhip import Buf, ReportOpts  # noqa: E402,F401
 
 DETAIL_HEADER = ("Qry\tRef\tANI\tDistance\tConfidence\tSelected_metric\tXnY_ctx\tQry_align_fraction\t"
                  "blastn_Qry_align_fraction\tRef_align_fraction\tblastn_Ref_align_fraction\tN_diff_obj\t"

@@ -196,3 +196,37 @@ def test_sketch_index_unique_bits_large_groups(tmp_path):
                      for i in range(len(want))])
     assert bits.any() and not bits.all()
     assert (tmp_path / "big" / INDEX_FILES[1]).read_bytes() == np.packbits(bits, bitorder="little").tobytes()
+
+
+MATRIX_CASES = [("matrix_triangle_default.tsv", ["S"]), ("matrix_full_default.tsv", ["--format", "full", "S"]),
+                ("matrix_rq_default.tsv", ["-r", "R", "-q", "Q"]),
+                ("matrix_triangle_diag.tsv", ["--diagonal-value", "0.25", "-e", "2", "S"]),
+                ("matrix_full_diag.tsv", ["--format", "full", "--diagonal-value", "0.25", "-m", "aaf", "S"]),
+                ("matrix_rq_e2.tsv", ["-r", "R", "-q", "Q", "-e", "2", "-m", "p_dist"]),
+                ("matrix_self_rq.tsv", ["-r", "S", "-q", "S", "--diagonal-value", "0.5"])]
+for _m in ("ctx-moe", "p_dist", "mash", "aaf"):
+    MATRIX_CASES += [(f"matrix_triangle_{_m}.tsv", ["-m", _m, "S"]),
+                     (f"matrix_full_{_m}.tsv", ["--format", "full", "-m", _m, "S"]),
+                     (f"matrix_rq_{_m}.tsv", ["-r", "R", "-q", "Q", "-m", _m])]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("golden,args", MATRIX_CASES, ids=[c[0] for c in MATRIX_CASES])
+def test_cli_matrix_command_matches_reference(tmp_path, golden, args):
+    dirs = genome_dirs(tmp_path, True)
+    # the reference ran these with the index built next to "all" (the indexed ani runs came first)
+    shutil.copy(GEN / "all" / INDEX_FILES[0], dirs["S"] / INDEX_FILES[0])
+    argv = [str(dirs.get(x, x)) for x in args]
+    p = run_cli(["matrix", *argv])
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == (GEN / golden).read_text()
+
+
+def test_matrix_usage_errors():
+    for args, msg in ((["matrix", "-r", "x", "--format", "triangle", "y"], "only valid for one sketch"),
+                      (["matrix", "-m", "aaf&mash", "x"], "combined --metric"),
+                      (["matrix", "-m", "bogus", "x"], "-m/--metric must be"),
+                      (["matrix", "--sparse", "x"], "not offered"),
+                      (["matrix"], "missing sketch input")):
+        p = run_cli(args)
+        assert p.returncode == 64 and msg in p.stderr, (args, p.stderr)

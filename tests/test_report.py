@@ -137,3 +137,77 @@ def test_direct_rect_matrix_cells(host, oracle, s):
                                             int(qctx[q]), int(rctx[r]), None, None) for r in range(ref.n)]
         lines.append(qry.names[q] + "".join("\t%f" % v for v in cells) + "\n")
     assert "".join(lines) == (GEN / f"matrix_rq_s{s}.tsv").read_text()
+
+
+def _fmt(host, fn, *args):
+    b = kt.Buf()
+    getattr(host.lib, fn)(kt.ctypes.byref(b), *args)
+    text = kt.ctypes.string_at(b.s, b.l).decode() if b.l else ""
+    host.lib.kssd_buf_free(kt.ctypes.byref(b))
+    return text
+
+
+MATRIX_CMD = [(f"matrix_triangle_{m}.tsv", "triangle", m, 1.0, None) for m in ("ctx-moe", "p_dist", "mash", "aaf")] + \
+    [(f"matrix_full_{m}.tsv", "full", m, 1.0, None) for m in ("ctx-moe", "p_dist", "mash", "aaf")] + [
+    ("matrix_triangle_default.tsv", "triangle", "ctx-naive", 1.0, None),
+    ("matrix_full_default.tsv", "full", "ctx-naive", 1.0, None),
+    ("matrix_triangle_diag.tsv", "triangle", "ctx-naive", 2.0, 0.25),
+    ("matrix_full_diag.tsv", "full", "aaf", 1.0, 0.25),
+]
+
+
+@pytest.mark.parametrize("golden,fmt,metric,e,diag", MATRIX_CMD, ids=[c[0] for c in MATRIX_CMD])
+def test_matrix_command_self(host, oracle, golden, fmt, metric, e, diag):
+    """`matrix` on one sketch: the indexed lower-triangle path
+    (matrix_write_indexed_self_matrix, command_matrix.c:439-619)."""
+    sk = kt.read_sketch_dir(GEN / "all")
+    rec = kt.build_index_np(sk.keys, sk.index, sk.obj_bits)
+    ctx, obj, _, _ = oracle.count_block(rec, sk.n, sk.obj_bits, sk.keys, sk.index, flags=kt.FLAGS["LOWER"],
+                                        threads=2)
+    tot = ctx_totals(oracle, sk)
+    mm = host.lib.kssd_matrix_metric(metric.encode())
+    n = sk.n
+
+    def val(q, r):
+        return host.lib.kssd_matrix_context_value(mm, e, int(ctx[q, r, 0]), int(obj[q, r, 0]), int(obj[q, r, 1]),
+                                                  int(ctx[q, r, 1]), int(tot[q]), int(tot[r]), 8, 20, 44)
+    dv = diag if diag is not None else 0.0
+    lines = []
+    if fmt == "full":
+        lines.append("".join("\t" + x for x in sk.names))
+        for q in range(n):
+            cells = []
+            for r in range(n):
+                v = dv if q == r else (val(q, r) if q > r else val(r, q))
+                cells.append("\t" + _fmt(host, "kssd_matrix_put_scaled6", host.lib.kssd_matrix_scale_fixed6(v)))
+            lines.append(sk.names[q] + "".join(cells))
+    else:
+        for q in range(n):
+            cells = ["\t" + _fmt(host, "kssd_matrix_put_fixed6", val(q, r)) for r in range(q)]
+            if diag is not None:
+                cells.append("\t" + _fmt(host, "kssd_matrix_put_fixed6", dv))
+            lines.append(sk.names[q] + "".join(cells))
+    assert "\n".join(lines) + "\n" == (GEN / golden).read_text()
+
+
+@pytest.mark.parametrize("golden,metric,e", [("matrix_rq_default.tsv", "ctx-naive", 1.0),
+                                             *[(f"matrix_rq_{m}.tsv", m, 1.0) for m in ("ctx-moe", "p_dist",
+                                                                                       "mash", "aaf")],
+                                             ("matrix_rq_e2.tsv", "p_dist", 2.0)])
+def test_matrix_command_rect(host, oracle, golden, metric, e):
+    """`matrix -r -q`: the dense report (matrix_compute_dense,
+    command_matrix.c:258-315, pairwise_eval_arrays per cell)."""
+    ref, qry = kt.read_sketch_dir(GEN / "ref"), kt.read_sketch_dir(GEN / "qry")
+    ctx, obj, _, _ = counts(oracle, ref, qry)
+    qt, rt_ = ctx_totals(oracle, qry), ctx_totals(oracle, ref)
+    mm = host.lib.kssd_matrix_metric(metric.encode())
+    lines = ["".join("\t" + x for x in ref.names)]
+    for q in range(qry.n):
+        cells = []
+        for r in range(ref.n):
+            v = host.lib.kssd_pairwise_value(mm, e, len(qry.sample(q)), len(ref.sample(r)), int(ctx[q, r, 0]),
+                                             int(obj[q, r, 0]), int(obj[q, r, 1]), int(ctx[q, r, 1]), int(qt[q]),
+                                             int(rt_[r]), 8, 20, 44)
+            cells.append("\t%f" % v)
+        lines.append(qry.names[q] + "".join(cells))
+    assert "\n".join(lines) + "\n" == (GEN / golden).read_text()
