@@ -7,6 +7,9 @@
  *                              every count computed by libkssdhip on the GPU
  *   kssd3a-hip matrix [opts]   the reference's `kssd3a matrix` distance matrices
  *                              (full / triangle, one metric per run)
+ *   kssd3a-hip sketch -o DIR F  sketch FASTA/FASTQ files, one sample each, with the
+ *                              coden pattern (command_sketch.c:3913-3949, :989-1005,
+ *                              :1134-1240 for lcofiles.infilemeta)
  *   kssd3a-hip sketch -i DIR   inverted index of a sketch directory
  *                              (gen_inverted_index4comblco, command_sketch.c:916-938)
  *
@@ -1285,44 +1288,193 @@ static int cmd_matrix(int argc, char **argv)
 	return matrix_self(&a);
 }
 
-/* ------------------------------------------------------------ sketch -i */
+/* ------------------------------------------------------------ sketch */
+/* Input paths of `sketch`: positional files, then the lines of -l lists
+ * (the order the reference keeps them in, global_basic.c:240-420). */
+static char **sketch_inputs(int npos, char **pos, const char *list, int *n_out)
+{
+	int cap = 16, n = 0;
+	char **v = xmalloc((size_t)cap * sizeof *v);
+	for (int i = 0; i < npos; ++i) {
+		if (n == cap)
+			v = realloc(v, (size_t)(cap *= 2) * sizeof *v);
+		v[n++] = strdup(pos[i]);
+	}
+	if (list) {
+		FILE *fp = fopen(list, "r");
+		if (!fp)
+			err(errno, "%s", list);
+		char *line = NULL;
+		size_t m = 0;
+		ssize_t l;
+		while ((l = getline(&line, &m, fp)) >= 0) {
+			while (l && (line[l - 1] == '\n' || line[l - 1] == '\r' || line[l - 1] == ' '))
+				line[--l] = '\0';
+			if (!l)
+				continue;
+			if (n == cap)
+				v = realloc(v, (size_t)(cap *= 2) * sizeof *v);
+			v[n++] = strdup(line);
+		}
+		free(line);
+		fclose(fp);
+	}
+	*n_out = n;
+	return v;
+}
+
+static int sketch_files(const char *outdir, char **files, int n, int drfold, bool conflict, bool meta, int threads)
+{
+	for (int i = 0; i < n; ++i) {
+		struct stat st;
+		if (stat(files[i], &st) != 0 || !S_ISREG(st.st_mode))
+			errx(EXIT_FAILURE, "%s: not a readable file (directories are not scanned here; list files with -l)",
+			     files[i]);
+		if (!kssd_infile_fmt_from_path(files[i]))
+			errx(EXIT_FAILURE, "%s: not a FASTA/FASTQ name (.fasta .fna .fas .fa .fq .fastq, optionally .gz)",
+			     files[i]);
+		if (strlen(files[i]) >= KSSD_PATHLEN)
+			errx(EXIT_FAILURE, "%s: path longer than %d bytes", files[i], KSSD_PATHLEN - 1);
+	}
+	kssd_u64vec_t *sk = xcalloc((size_t)n, sizeof *sk);
+	kssd_infile_meta_t *mt = xcalloc((size_t)n, sizeof *mt);
+	int bad = -1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+	for (int i = 0; i < n; ++i) {
+		if (kssd_sketch_fasta_file(files[i], drfold, conflict, &sk[i]) != 0 ||
+		    (meta && kssd_infile_meta_for_path(files[i], &mt[i]) != 0)) {
+#pragma omp critical
+			bad = i;
+		}
+	}
+	if (bad >= 0)
+		err(EXIT_FAILURE, "%s", files[bad]);
+	stage("inputs sketched");
+	uint64_t *index = xmalloc(((size_t)n + 1) * sizeof *index);
+	index[0] = 0;
+	for (int i = 0; i < n; ++i)
+		index[i + 1] = index[i] + sk[i].n;
+	uint64_t *keys = xmalloc((size_t)index[n] * sizeof *keys);
+	for (int i = 0; i < n; ++i) {
+		if (sk[i].n)
+			memcpy(keys + index[i], sk[i].a, sk[i].n * sizeof *keys);
+		kssd_u64vec_free(&sk[i]);
+	}
+	char (*names)[KSSD_PATHLEN] = xcalloc((size_t)n, KSSD_PATHLEN);
+	for (int i = 0; i < n; ++i)
+		strcpy(names[i], files[i]);
+	/* lcofiles.stat of the coden pattern (command_sketch.c:989-1005,
+	 * command_sketch_wrapper.c:920-930) */
+	kssd_stat_t st = {0};
+	st.hash_id = kssd_sketching_id(11, 0, 0, drfold);
+	st.koc = false;
+	st.conflict = conflict;
+	st.coden_len = 11;
+	st.klen = 32;
+	st.hclen = 0;
+	st.holen = 0;
+	st.drfold = drfold;
+	st.infile_num = n;
+	if (kssd_sketchdir_write(outdir, &st, names, keys, index) != 0 ||
+	    (meta && kssd_write_infile_meta(outdir, mt, n) != 0))
+		err(EXIT_FAILURE, "%s", outdir);
+	stage("sketch written");
+	free(sk);
+	free(mt);
+	free(index);
+	free(keys);
+	free(names);
+	return 0;
+}
+
 static int cmd_sketch(int argc, char **argv)
 {
+	enum { S_CONFLICT = 400, S_NOMETA, S_HELP, S_NOT_OFFERED };
 	static const struct option longopts[] = {
 		{"idx", required_argument, NULL, 'i'},
+		{"index", required_argument, NULL, 'i'},
+		{"outdir", required_argument, NULL, 'o'},
+		{"list", required_argument, NULL, 'l'},
+		{"DimRdcFold", required_argument, NULL, 'f'},
+		{"conflict", no_argument, NULL, S_CONFLICT},
+		{"nocomputemeta", no_argument, NULL, S_NOMETA},
+		{"use_coden_ctxobj", no_argument, NULL, 'T'},
 		{"threads", required_argument, NULL, 'p'},
-		{"help", no_argument, NULL, OPT_HELP},
+		{"help", no_argument, NULL, S_HELP},
+		{"abundance", no_argument, NULL, S_NOT_OFFERED},
+		{"asone", no_argument, NULL, S_NOT_OFFERED},
+		{"splitmfa", no_argument, NULL, S_NOT_OFFERED},
+		{"anno", no_argument, NULL, S_NOT_OFFERED},
+		{"readsQC", no_argument, NULL, S_NOT_OFFERED},
+		{"pipecmd", required_argument, NULL, S_NOT_OFFERED},
 		{NULL, 0, NULL, 0},
 	};
-	const char *dir = NULL;
-	int threads = omp_get_max_threads();
+	const char *dir = NULL, *outdir = NULL, *list = NULL;
+	int threads = omp_get_max_threads(), drfold = 8;
+	bool conflict = false, meta = true;
 	optind = 1;
 	int c;
-	while ((c = getopt_long(argc, argv, "i:p:", longopts, NULL)) != -1) {
+	while ((c = getopt_long(argc, argv, "i:o:l:f:Tp:aA", longopts, NULL)) != -1) {
 		switch (c) {
 		case 'i':
 			dir = optarg;
 			break;
+		case 'o':
+			outdir = optarg;
+			break;
+		case 'l':
+			list = optarg;
+			break;
+		case 'f':
+			drfold = (int)parse_int("-f/--DimRdcFold", optarg, 0, 32);
+			break;
+		case 'T':
+			break;
+		case S_CONFLICT:
+			conflict = true;
+			break;
+		case S_NOMETA:
+			meta = false;
+			break;
 		case 'p':
 			threads = (int)parse_int("-p/--threads", optarg, 1, 65536);
 			break;
-		case OPT_HELP:
-			printf("Usage: %s sketch -i SKETCH_DIR [-p THREADS]\n"
-			       "Build SKETCH_DIR/%s and its .unique_bits.\n",
-			       PROG, KSSD_SORTED_INDEX_FILE);
+		case S_HELP:
+			printf("Usage: %s sketch -o OUTDIR [-f DRFOLD] [--conflict] [--nocomputemeta] [-l LIST] FILE...\n"
+			       "       %s sketch -i SKETCH_DIR\n"
+			       "Sketch FASTA/FASTQ[.gz] files with the coden context-object pattern (one sample per\n"
+			       "file), or build SKETCH_DIR/%s and its .unique_bits.\n",
+			       PROG, PROG, KSSD_SORTED_INDEX_FILE);
 			return 0;
+		case 'a':
+		case 'A':
+		case S_NOT_OFFERED:
+			usage_error("%s is not offered by this front end (one sample per FASTA/FASTQ file)", argv[optind - 1]);
 		default:
 			fprintf(stderr, "Try `%s %s --help' for more information.\n", PROG, g_cmd);
 			exit(EX_USAGE);
 		}
 	}
-	if (!dir)
-		usage_error("only `sketch -i SKETCH_DIR` (inverted index) is offered; sketch sequences with kssd3a");
-	if (optind != argc)
-		usage_error("unexpected argument '%s'", argv[optind]);
-	write_index_for(dir, threads);
-	stage("index written");
-	return 0;
+	if (dir) {
+		if (outdir || optind != argc)
+			usage_error("-i builds the index of an existing sketch; it takes no inputs");
+		write_index_for(dir, threads);
+		stage("index written");
+		return 0;
+	}
+	if (!outdir)
+		usage_error("give -o OUTDIR and input files, or -i SKETCH_DIR");
+	int n = 0;
+	char **files = sketch_inputs(argc - optind, argv + optind, list, &n);
+	if (n == 0)
+		usage_error("no input files");
+	if (n >= (1 << 20))
+		errx(EXIT_FAILURE, "%d inputs exceed the %d genomes a sketch can index", n, 1 << 20);
+	const int rc = sketch_files(outdir, files, n, drfold, conflict, meta, threads);
+	for (int i = 0; i < n; ++i)
+		free(files[i]);
+	free(files);
+	return rc;
 }
 
 int main(int argc, char **argv)

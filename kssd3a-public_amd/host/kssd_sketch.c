@@ -6,6 +6,7 @@
 #include <ctype.h>
 #include <errno.h>
 #include <math.h>
+#include <stdbool.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -185,30 +186,43 @@ int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd
 		errno = ENOMEM;
 		return -1;
 	}
-	int in_fastq_qual = 0, fastq = -1, line_no = 0;
+	/* lines longer than the buffer arrive in pieces: `start` marks the
+	 * first piece of a line, `skip` the rest of a header or quality line */
+	int fastq = -1, line_no = 0;
+	bool start = true, skip = false;
 	while (gzgets(fp, line, 1 << 16)) {
 		size_t l = strlen(line);
+		const bool ends = l && line[l - 1] == '\n';
 		while (l && (line[l - 1] == '\n' || line[l - 1] == '\r'))
 			line[--l] = '\0';
-		if (fastq < 0 && l)
-			fastq = line[0] == '@';
-		if (fastq == 1) {
-			/* 4-line FASTQ records: header, sequence, '+', quality */
-			const int k = line_no++ % 4;
-			if (k == 0 || k == 2 || k == 3) {
-				if (k == 0 && n) {
-					kssd_sketch_codes(codes, n, drfold, out);
-					n = 0;
+		const bool first_piece = start;
+		start = ends;
+		if (!first_piece) {
+			if (skip)
+				continue;
+		} else {
+			skip = false;
+			if (fastq < 0 && l)
+				fastq = line[0] == '@';
+			if (fastq == 1) {
+				/* 4-line FASTQ records: header, sequence, '+', quality */
+				const int k = line_no++ % 4;
+				if (k != 1) {
+					if (k == 0 && n) {
+						kssd_sketch_codes(codes, n, drfold, out);
+						n = 0;
+					}
+					skip = true;
+					continue;
 				}
+			} else if (l && (line[0] == '>' || line[0] == ';')) {
+				if (n)
+					kssd_sketch_codes(codes, n, drfold, out);
+				n = 0;
+				skip = true;
 				continue;
 			}
-		} else if (l && (line[0] == '>' || line[0] == ';')) {
-			if (n)
-				kssd_sketch_codes(codes, n, drfold, out);
-			n = 0;
-			continue;
 		}
-		(void)in_fastq_qual;
 		if (n + l > cap) {
 			while (n + l > cap)
 				cap <<= 1;

@@ -8,10 +8,12 @@
 #include "kssd_sketchdir.h"
 
 #include <errno.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <zlib.h>
 
 static char *path_join(const char *dir, const char *name)
 {
@@ -255,4 +257,155 @@ int kssd_write_sorted_index(const char *dir, const kssd_ctxgidobj_t *rec, size_t
 	rc = write_file(dir, KSSD_SORTED_INDEX_FILE ".unique_bits", bits, (n + 7) / 8);
 	free(bits);
 	return rc;
+}
+
+/* ------------------------------------------------------------ input metadata */
+static bool has_suffix(const char *s, const char *suf)
+{
+	const size_t a = strlen(s), b = strlen(suf);
+	return a >= b && strcmp(s + a - b, suf) == 0;
+}
+
+static bool is_compressed(const char *path)
+{
+	return has_suffix(path, ".gz") || has_suffix(path, ".bz2");
+}
+
+int kssd_infile_fmt_from_path(const char *path)
+{
+	/* the name without a .gz/.bz2 suffix, then the format's extensions */
+	char buf[4096];
+	if (!path || strlen(path) >= sizeof buf)
+		return 0;
+	strcpy(buf, path);
+	if (has_suffix(buf, ".gz"))
+		buf[strlen(buf) - 3] = '\0';
+	else if (has_suffix(buf, ".bz2"))
+		buf[strlen(buf) - 4] = '\0';
+	static const char *const fq[] = {".fq", ".fastq"};
+	static const char *const fa[] = {".fasta", ".fna", ".fas", ".fa"};
+	for (size_t i = 0; i < sizeof fq / sizeof fq[0]; ++i)
+		if (has_suffix(buf, fq[i]))
+			return 2;
+	for (size_t i = 0; i < sizeof fa / sizeof fa[0]; ++i)
+		if (has_suffix(buf, fa[i]))
+			return 1;
+	return 0;
+}
+
+static int cmp_u64(const void *a, const void *b)
+{
+	const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+	return (x > y) - (x < y);
+}
+
+/* Sequence length of every FASTA record (all characters of its sequence
+ * lines; a trailing CR is not counted), empty records skipped
+ * (append_asm_lengths_for_path, command_sketch.c:1205-1227). */
+static int fasta_record_lengths(const char *path, uint64_t **lens, size_t *n)
+{
+	gzFile fp = gzopen(path, "rb");
+	if (!fp)
+		return -1;
+	size_t cap = 64;
+	uint64_t *v = malloc(cap * sizeof *v);
+	char *line = malloc(1 << 16);
+	if (!v || !line) {
+		gzclose(fp);
+		free(v);
+		free(line);
+		return -1;
+	}
+	size_t k = 0;
+	uint64_t cur = 0;
+	bool in_rec = false, at_line_start = true, in_header = false;
+	while (gzgets(fp, line, 1 << 16)) {
+		size_t l = strlen(line);
+		const bool ends_line = l && line[l - 1] == '\n';
+		if (ends_line)
+			--l;
+		if (ends_line && l && line[l - 1] == '\r')
+			--l;
+		if (in_header) { /* the rest of a header longer than the buffer */
+			in_header = !ends_line;
+			at_line_start = ends_line;
+			continue;
+		}
+		if (at_line_start && l && line[0] == '>') {
+			in_header = !ends_line;
+			if (in_rec && cur) {
+				if (k == cap) {
+					uint64_t *w = realloc(v, (cap *= 2) * sizeof *v);
+					if (!w)
+						abort();
+					v = w;
+				}
+				v[k++] = cur;
+			}
+			in_rec = true;
+			cur = 0;
+		} else if (in_rec) {
+			cur += l;
+		}
+		at_line_start = ends_line;
+	}
+	if (in_rec && cur) {
+		if (k == cap) {
+			uint64_t *w = realloc(v, (cap + 1) * sizeof *v);
+			if (!w)
+				abort();
+			v = w;
+		}
+		v[k++] = cur;
+	}
+	gzclose(fp);
+	free(line);
+	*lens = v;
+	*n = k;
+	return 0;
+}
+
+int kssd_infile_meta_for_path(const char *path, kssd_infile_meta_t *out)
+{
+	memset(out, 0, sizeof *out);
+	const int fmt = kssd_infile_fmt_from_path(path);
+	out->meta_fmt_version = 1;
+	out->infile_fmt = (int8_t)fmt;
+	out->create_type = 0;
+	out->infile_flags = (uint8_t)((strcmp(path, "-") == 0 ? 0x02 : 0) | (is_compressed(path) ? 0x04 : 0));
+	out->asm_level = fmt == 2 ? 0.0f : -1.0f;
+	if (fmt != 1)
+		return 0;
+	uint64_t *len = NULL;
+	size_t n = 0;
+	if (fasta_record_lengths(path, &len, &n) != 0)
+		return -1;
+	if (n) {
+		/* infile_meta_from_lengths, command_sketch.c:1134-1174 */
+		uint64_t total = 0;
+		double mean = 0.0, m2 = 0.0;
+		for (size_t i = 0; i < n; ++i) {
+			total += len[i];
+			const double x = (double)len[i];
+			const double d = x - mean;
+			mean += d / (double)(i + 1);
+			m2 += d * (x - mean);
+		}
+		qsort(len, n, sizeof *len, cmp_u64);
+		out->total_length_bp = total;
+		out->record_count = n > UINT32_MAX ? UINT32_MAX : (uint32_t)n;
+		const uint64_t med = (n & 1) ? len[n / 2] : (len[n / 2 - 1] + len[n / 2]) / 2;
+		out->median_length_bp = med > UINT32_MAX ? UINT32_MAX : (uint32_t)med;
+		if (mean > 0.0 && n > 1)
+			out->length_cv = (float)(sqrt(m2 / (double)n) / mean);
+		if (total > 0)
+			out->asm_level = (float)((double)len[n - 1] / (double)total);
+	}
+	free(len);
+	return 0;
+}
+
+int kssd_write_infile_meta(const char *dir, const kssd_infile_meta_t *meta, int n)
+{
+	return write_file(dir, KSSD_META_FILE, meta, (size_t)n * sizeof *meta);
 }

@@ -52,6 +52,17 @@ int kssd_sketchdir_write(const char *dir, const kssd_stat_t *stat, char (*names)
 /* sortedcomb_ctxgid64obj32 + .unique_bits */
 int kssd_write_sorted_index(const char *dir, const kssd_ctxgidobj_t *rec, size_t n);
 
+/* Per-input metadata of a sketched file (sketch_infile_meta_stat_for_path,
+ * command_sketch.c:1229-1240 with infile_meta_from_lengths :1134-1174):
+ * FASTA record lengths summarised, format and flags from the file name.
+ * 0, or -1 when a FASTA file cannot be read. */
+int kssd_infile_meta_for_path(const char *path, kssd_infile_meta_t *out);
+/* lcofiles.infilemeta */
+int kssd_write_infile_meta(const char *dir, const kssd_infile_meta_t *meta, int n);
+/* Input file kind by name (isOK_fmt_infile with fasta_fmt / fastq_fmt,
+ * global_basic.c:166-189, global_basic.h:214-250): 1 FASTA, 2 FASTQ, 0 other. */
+int kssd_infile_fmt_from_path(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

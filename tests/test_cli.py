@@ -54,13 +54,50 @@ def copy_dir(src, dst, with_index=True):
     (["ani", "--reflist", "a", "--qrylist", "b"], "not offered"),
     (["ani", "-r", "x", "-q", "y", "--metric", "bogus"], "unknown --metric"),
     (["ani", "-r", "x", "-q", "y", "--format", "wide"], "--format must be"),
-    (["sketch", "-o", "out", "a.fa"], None),
+    (["sketch", "--asone", "-o", "out", "a.fa"], "not offered"),
+    (["sketch", "-o", "out"], "no input files"),
+    (["sketch"], "give -o OUTDIR"),
 ])
 def test_usage_errors_exit_64(args, msg):
     p = run_cli(args)
     assert p.returncode == 64, p.stderr
     if msg:
         assert msg in p.stderr
+
+
+@pytest.mark.parametrize("sub,args", [("f8", []), ("f4", ["-f", "4"]), ("f8c", ["--conflict"])])
+def test_sketch_matches_reference(tmp_path, sub, args):
+    """`sketch -o` on the reference's FASTA inputs: comblco, comblco.index,
+    lcofiles.infilemeta and the stat header byte-identical, names equal."""
+    sk = kt.GOLDEN / "sketch"
+    out = tmp_path / sub
+    p = subprocess.run([str(BIN), "sketch", *args, "-o", str(out), "seq0.fa", "seq1.fa", "seq2.fa"],
+                       capture_output=True, text=True, cwd=sk, timeout=120)
+    assert p.returncode == 0, p.stderr
+    for f in ("comblco", "comblco.index", "lcofiles.infilemeta"):
+        assert (out / f).read_bytes() == (sk / sub / f).read_bytes(), f
+    assert (out / "lcofiles.stat").read_bytes()[:32] == (sk / sub / "lcofiles.stat").read_bytes()[:32]
+    assert kt.read_sketch_dir(out).names == kt.read_sketch_dir(sk / sub).names
+
+
+def test_sketch_long_lines_and_gzip(tmp_path):
+    """Lines longer than the reader's buffer (a one-line genome, a long
+    header) and gzip input sketch like the wrapped plain file."""
+    import gzip
+    import numpy as np
+    rng = np.random.default_rng(4)
+    seq = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, 300_000)].tobytes()
+    wrapped = b">a\n" + b"\n".join(seq[i:i + 80] for i in range(0, len(seq), 80)) + b"\n"
+    (tmp_path / "w.fa").write_bytes(wrapped)
+    (tmp_path / "one.fa").write_bytes(b">a " + b"x" * 100_000 + b"\n" + seq + b"\n")
+    (tmp_path / "z.fa.gz").write_bytes(gzip.compress(wrapped))
+    p = run_cli(["sketch", "-o", tmp_path / "out", tmp_path / "w.fa", tmp_path / "one.fa", tmp_path / "z.fa.gz"])
+    assert p.returncode == 0, p.stderr
+    d = kt.read_sketch_dir(tmp_path / "out")
+    assert len(d.sample(0)) > 500
+    assert np.array_equal(d.sample(0), d.sample(1)) and np.array_equal(d.sample(0), d.sample(2))
+    meta = (tmp_path / "out" / "lcofiles.infilemeta").read_bytes()
+    assert meta[0:32][:16] == meta[32:64][:16]  # same record lengths summary
 
 
 def test_sketch_index_matches_reference(tmp_path):
