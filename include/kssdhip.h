@@ -118,12 +118,18 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  *          partition boundaries and merged against the staged records.
  * chunk_records (partitions are rebuilt when it changes) must leave the
  * staged partition within the 160 KB LDS: 64 <= chunk_records <= 12288.
+ * group_partitions (1..64): consecutive partitions one workgroup walks;
+ * above 1 the workgroup sums its hits per (query, genome) cell in an LDS
+ * table of KSSDHIP_CELL_TABLE entries and adds each cell to the outputs
+ * once, which needs chunk_records <= 6144.
  */
+#define KSSDHIP_CELL_TABLE 2048
 typedef struct {
 	int mode;
 	int chunk_records;
 	int block_threads; /* 64..1024, multiple of 64 */
 	double auto_min_keys;
+	int group_partitions;
 } kssdhip_tiling_t;
 int kssdhip_index_set_tiling(kssdhip_index *index, const kssdhip_tiling_t *tiling);
 int kssdhip_index_get_tiling(const kssdhip_index *index, kssdhip_tiling_t *tiling);

@@ -96,6 +96,7 @@ struct kssdhip_index {
 	int chunk = 0;
 	int block = 256;
 	double auto_min_keys = 0;
+	int group = 1;
 	// grow-only scratch: host-pointer entry points and the split table
 	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx, split;
 };

@@ -132,7 +132,21 @@ struct PJArgs {
 	const uint32_t *pstart;
 	const uint32_t *S; // [nq][P + 1]
 	uint32_t P;
-	uint32_t cap; // records that fit the LDS stage
+	uint32_t cap;   // records that fit the LDS stage
+	uint32_t group; // consecutive partitions per workgroup
+	uint32_t ct;    // LDS cell-table entries (0: none)
+};
+
+// LDS table of (query, genome) cells: key = unique << 31 | query-in-slice
+// << 20 | genome; linear probing; a key that does not find a slot within
+// CT_PROBES goes straight to the global counters.
+constexpr uint32_t CT_EMPTY = 0xffffffffu;
+constexpr int CT_PROBES = 16;
+struct CellTable {
+	uint32_t *key;
+	unsigned long long *c, *o;
+	uint32_t n;    // power of two, 0 = no table
+	uint32_t bits; // log2(n)
 };
 
 constexpr uint32_t QB = 1024; // queries per workgroup; grid.y covers the block's queries
@@ -189,28 +203,56 @@ __device__ uint32_t block_exclusive_scan(uint32_t *v, uint32_t n, uint32_t *wsum
 }
 
 // Per-thread accumulation of consecutive hits on one counter cell, flushed
-// with one 64-bit atomic per counter when the cell changes.
+// when the cell changes: into the workgroup's LDS cell table when it has one,
+// else with one 64-bit atomic per counter.
 struct CellAcc {
-	uint64_t cell = ~0ull;
+	uint64_t cell = ~0ull; // global cell index
+	uint32_t key = CT_EMPTY; // LDS table key
 	unsigned long long c = 0, o = 0;
-	__device__ __forceinline__ void add(uint64_t at, int m, unsigned long long *ctx, unsigned long long *obj)
+	__device__ __forceinline__ void add(uint64_t at, uint32_t k, int m, const CellTable &t, unsigned long long *ctx,
+					    unsigned long long *obj)
 	{
 		if (at != cell) {
-			flush(ctx, obj);
+			flush(t, ctx, obj);
 			cell = at;
+			key = k;
 		}
 		c += 1ull | ((unsigned long long)(m > 1) << 32);
 		if (m)
 			o += 1ull | ((unsigned long long)m << 32);
 	}
-	__device__ __forceinline__ void flush(unsigned long long *ctx, unsigned long long *obj)
+	__device__ __forceinline__ void flush(const CellTable &t, unsigned long long *ctx, unsigned long long *obj)
 	{
-		if (cell != ~0ull) {
+		if (cell == ~0ull)
+			return;
+		bool done = false;
+		if (t.n && key != CT_EMPTY) {
+			// Fibonacci hashing: the product's top bits mix every key bit
+			uint32_t h = (key * 0x9E3779B1u) >> (32 - t.bits);
+			for (int i = 0; i < CT_PROBES; ++i) {
+				uint32_t k = t.key[h];
+				if (k == CT_EMPTY) {
+					k = atomicCAS(&t.key[h], CT_EMPTY, key);
+					if (k == CT_EMPTY)
+						k = key;
+				}
+				if (k == key) {
+					atomicAdd(&t.c[h], c);
+					if (o)
+						atomicAdd(&t.o[h], o);
+					done = true;
+					break;
+				}
+				h = (h + 1) & (t.n - 1);
+			}
+		}
+		if (!done) {
 			atomicAdd(&ctx[cell], c);
 			if (o)
 				atomicAdd(&obj[cell], o);
 		}
 		cell = ~0ull;
+		key = CT_EMPTY;
 		c = o = 0;
 	}
 };
@@ -226,7 +268,7 @@ struct CellAcc {
 template <bool TABLE, typename RecPtr, typename ObjPtr>
 __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr robj, uint32_t cnt,
 					   const PartTable &tab, const uint32_t *pre, const uint32_t *start,
-					   int qb0, int nqb, uint32_t K)
+					   int qb0, int nqb, uint32_t K, const CellTable &ct)
 {
 	const JoinArgs &j = a.j;
 	const unsigned ob = j.obj_bits;
@@ -345,24 +387,26 @@ __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr r
 					m = c < m ? c : m;
 				}
 			}
-			acc.add(row + gid, m, j.ctx, j.objc);
+			acc.add(row + gid, ((uint32_t)qi << GID_NBITS) | gid, m, ct, j.ctx, j.objc);
 			if (unique) {
 				sole_diff = m;
 				sole_allowed = true;
 			}
 		}
 		if (unique && owners == 1 && sole_allowed)
-			uacc.add(row + sole_gid, sole_diff, j.uctx, j.uobj);
+			uacc.add(row + sole_gid, 0x80000000u | ((uint32_t)qi << GID_NBITS) | sole_gid, sole_diff, ct, j.uctx,
+				 j.uobj);
 	}
-	acc.flush(j.ctx, j.objc);
+	acc.flush(ct, j.ctx, j.objc);
 	if (unique)
-		uacc.flush(j.uctx, j.uobj);
+		uacc.flush(ct, j.uctx, j.uobj);
 }
 
 __global__ void k_pjoin(PJArgs a)
 {
 	// LDS: [cap] ctxgid | [cap] obj (u32) | [PART_BUCKETS + 2] u16 table |
-	//      [qcap + 1] u32 prefix | [qcap] u32 starts, qcap = min(nq, QB)
+	//      [qcap + 1] u32 prefix | [qcap] u32 starts, qcap = min(nq, QB) |
+	//      cell table: [ct] u32 keys (8-byte aligned) | [ct] u64 | [ct] u64
 	extern __shared__ uint64_t lds_rec[];
 	__shared__ uint32_t wsum[1024 / 64 + 1];
 	const uint32_t qcap = (uint32_t)a.j.nq < QB ? (uint32_t)a.j.nq : QB;
@@ -370,73 +414,105 @@ __global__ void k_pjoin(PJArgs a)
 	uint16_t *lds_bt = (uint16_t *)(lds_obj + a.cap);
 	uint32_t *lds_pre = (uint32_t *)(lds_bt + PART_BUCKETS + 2);
 	uint32_t *lds_start = lds_pre + qcap + 1;
+	CellTable ct;
+	ct.n = a.ct;
+	ct.bits = a.ct ? (uint32_t)(__ffs(a.ct) - 1) : 0;
+	ct.key = (uint32_t *)(((uintptr_t)(lds_start + qcap) + 7) & ~(uintptr_t)7);
+	ct.c = (unsigned long long *)(ct.key + a.ct);
+	ct.o = ct.c + a.ct;
 	// XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a
-	// contiguous run of partitions.
+	// contiguous run of partition groups.
 	const uint32_t G = gridDim.x, b = blockIdx.x;
 	const uint32_t x = b % NUM_XCD, i = b / NUM_XCD, base = G / NUM_XCD, rem = G % NUM_XCD;
-	const uint32_t p = x * base + (x < rem ? x : rem) + i;
-	if (p >= a.P)
+	const uint32_t pg = x * base + (x < rem ? x : rem) + i;
+	const uint32_t p_begin = pg * a.group;
+	if (p_begin >= a.P)
 		return;
+	const uint32_t p_end = p_begin + a.group < a.P ? p_begin + a.group : a.P;
 	const int qb0 = (int)(blockIdx.y * QB);
 	const int nqb = a.j.nq - qb0 < (int)QB ? a.j.nq - qb0 : (int)QB;
-	// this partition's key range of each of the workgroup's queries
+	for (uint32_t k = threadIdx.x; k < ct.n; k += blockDim.x) {
+		ct.key[k] = CT_EMPTY;
+		ct.c[k] = 0;
+		ct.o[k] = 0;
+	}
 	const uint64_t width = (uint64_t)a.P + 1;
-	for (int k = threadIdx.x; k < nqb; k += blockDim.x) {
-		const uint64_t r = (uint64_t)(qb0 + k) * width + p;
-		const uint32_t s0 = a.S[r], s1 = a.S[r + 1];
-		lds_start[k] = s0;
-		lds_pre[k] = s1 - s0;
-	}
-	const uint32_t rb = a.pstart[p], re = a.pstart[p + 1], cnt = re - rb;
-	const uint32_t rb_al = rb & ~(STAGE_ALIGN - 1), re_al = (re + STAGE_ALIGN - 1) & ~(STAGE_ALIGN - 1);
-	const bool staged = re_al - rb_al <= a.cap;
-	if (staged) {
-		// stage [rb_al, re_al) with 16-byte loads; the index arrays are
-		// padded past n, so the last group stays in bounds
-		const uint32_t groups = (re_al - rb_al) / STAGE_ALIGN;
-		const uint4 *g_rec = (const uint4 *)(a.j.ctxgid + rb_al);
-		const uint4 *g_obj = (const uint4 *)(a.j.robj + rb_al);
-		uint4 *l_rec = (uint4 *)lds_rec;
-		uint4 *l_obj = (uint4 *)lds_obj;
-		for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
-			const uint4 r0 = g_rec[2 * g], r1 = g_rec[2 * g + 1], o = g_obj[g];
-			l_rec[2 * g] = r0;
-			l_rec[2 * g + 1] = r1;
-			l_obj[g] = o;
+	for (uint32_t p = p_begin; p < p_end; ++p) {
+		__syncthreads(); // the previous partition is done with the shared arrays
+		// this partition's key range of each of the workgroup's queries
+		for (int k = threadIdx.x; k < nqb; k += blockDim.x) {
+			const uint64_t r = (uint64_t)(qb0 + k) * width + p;
+			const uint32_t s0 = a.S[r], s1 = a.S[r + 1];
+			lds_start[k] = s0;
+			lds_pre[k] = s1 - s0;
 		}
-	}
-	__syncthreads();
-	const uint32_t K = block_exclusive_scan(lds_pre, (uint32_t)nqb, wsum);
-	if (threadIdx.x == 0)
-		lds_pre[nqb] = K;
-	if (!staged) {
+		const uint32_t rb = a.pstart[p], re = a.pstart[p + 1], cnt = re - rb;
+		const uint32_t rb_al = rb & ~(STAGE_ALIGN - 1), re_al = (re + STAGE_ALIGN - 1) & ~(STAGE_ALIGN - 1);
+		const bool staged = re_al - rb_al <= a.cap;
+		if (staged) {
+			// stage [rb_al, re_al) with 16-byte loads; the index arrays are
+			// padded past n, so the last group stays in bounds
+			const uint32_t groups = (re_al - rb_al) / STAGE_ALIGN;
+			const uint4 *g_rec = (const uint4 *)(a.j.ctxgid + rb_al);
+			const uint4 *g_obj = (const uint4 *)(a.j.robj + rb_al);
+			uint4 *l_rec = (uint4 *)lds_rec;
+			uint4 *l_obj = (uint4 *)lds_obj;
+			for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
+				const uint4 r0 = g_rec[2 * g], r1 = g_rec[2 * g + 1], o = g_obj[g];
+				l_rec[2 * g] = r0;
+				l_rec[2 * g + 1] = r1;
+				l_obj[g] = o;
+			}
+		}
 		__syncthreads();
-		// a context run longer than the stage: merge straight from HBM
-		merge_keys<false>(a, a.j.ctxgid + rb, a.j.robj + rb, cnt, PartTable{}, lds_pre, lds_start, qb0,
-				  nqb, K);
+		const uint32_t K = block_exclusive_scan(lds_pre, (uint32_t)nqb, wsum);
+		if (threadIdx.x == 0)
+			lds_pre[nqb] = K;
+		if (!staged) {
+			__syncthreads();
+			// a context run longer than the stage: merge straight from HBM
+			merge_keys<false>(a, a.j.ctxgid + rb, a.j.robj + rb, cnt, PartTable{}, lds_pre, lds_start, qb0,
+					  nqb, K, ct);
+			continue;
+		}
+		const uint32_t off = rb - rb_al;
+		const uint64_t *srec = lds_rec + off;
+		const uint32_t *sobj = lds_obj + off;
+		// bucket table over [c0, clast]: at most PART_BUCKETS buckets
+		PartTable tab;
+		tab.bt = lds_bt;
+		tab.c0 = srec[0] >> GID_NBITS;
+		tab.clast = srec[cnt - 1] >> GID_NBITS;
+		const uint64_t span = tab.clast - tab.c0; // buckets = (span >> shift) + 1
+		const int bits = span ? 64 - __clzll((long long)span) : 0;
+		tab.shift = bits > PART_BUCKET_BITS ? bits - PART_BUCKET_BITS : 0;
+		for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+			const int bk = (int)(((srec[k] >> GID_NBITS) - tab.c0) >> tab.shift);
+			const int bp = k ? (int)(((srec[k - 1] >> GID_NBITS) - tab.c0) >> tab.shift) : -1;
+			for (int bb = bp + 1; bb <= bk; ++bb)
+				lds_bt[bb] = (uint16_t)k;
+		}
+		if (threadIdx.x == 0)
+			lds_bt[(span >> tab.shift) + 1] = (uint16_t)cnt;
+		__syncthreads();
+		merge_keys<true>(a, srec, sobj, cnt, tab, lds_pre, lds_start, qb0, nqb, K, ct);
+	}
+	if (!ct.n)
 		return;
-	}
-	const uint32_t off = rb - rb_al;
-	const uint64_t *srec = lds_rec + off;
-	const uint32_t *sobj = lds_obj + off;
-	// bucket table over [c0, clast]: at most PART_BUCKETS buckets
-	PartTable tab;
-	tab.bt = lds_bt;
-	tab.c0 = srec[0] >> GID_NBITS;
-	tab.clast = srec[cnt - 1] >> GID_NBITS;
-	const uint64_t span = tab.clast - tab.c0; // buckets = (span >> shift) + 1
-	const int bits = span ? 64 - __clzll((long long)span) : 0;
-	tab.shift = bits > PART_BUCKET_BITS ? bits - PART_BUCKET_BITS : 0;
-	for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
-		const int bk = (int)(((srec[k] >> GID_NBITS) - tab.c0) >> tab.shift);
-		const int bp = k ? (int)(((srec[k - 1] >> GID_NBITS) - tab.c0) >> tab.shift) : -1;
-		for (int bb = bp + 1; bb <= bk; ++bb)
-			lds_bt[bb] = (uint16_t)k;
-	}
-	if (threadIdx.x == 0)
-		lds_bt[(span >> tab.shift) + 1] = (uint16_t)cnt;
 	__syncthreads();
-	merge_keys<true>(a, srec, sobj, cnt, tab, lds_pre, lds_start, qb0, nqb, K);
+	// each cell the workgroup touched, added to the outputs once
+	const JoinArgs &j = a.j;
+	for (uint32_t k = threadIdx.x; k < ct.n; k += blockDim.x) {
+		const uint32_t key = ct.key[k];
+		if (key == CT_EMPTY)
+			continue;
+		const bool u = (key >> 31) != 0;
+		const uint64_t cell =
+			(uint64_t)(qb0 + (int)((key >> GID_NBITS) & (QB - 1))) * (uint64_t)j.ref_gnum + (key & GID_MASK);
+		atomicAdd(&(u ? j.uctx : j.ctx)[cell], ct.c[k]);
+		if (ct.o[k])
+			atomicAdd(&(u ? j.uobj : j.objc)[cell], ct.o[k]);
+	}
 }
 
 unsigned grid_cap(uint64_t n, unsigned block)
@@ -552,9 +628,12 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevB
 	a.S = S;
 	a.P = ix->P;
 	a.cap = ((uint32_t)ix->chunk + LDS_SLACK + STAGE_ALIGN - 1) & ~(STAGE_ALIGN - 1);
+	a.group = (uint32_t)ix->group;
+	a.ct = ix->group > 1 ? KSSDHIP_CELL_TABLE : 0;
 	const uint32_t qslices = (uint32_t)((ja.nq + QB - 1) / QB);
 	const uint32_t qb = (uint32_t)ja.nq < QB ? (uint32_t)ja.nq : QB;
-	const size_t lds = (size_t)a.cap * 12 + (PART_BUCKETS + 2) * 2 + ((size_t)2 * qb + 1) * 4;
+	const size_t lds = (size_t)a.cap * 12 + (PART_BUCKETS + 2) * 2 + ((size_t)2 * qb + 1) * 4 + 8 +
+			   (size_t)a.ct * 20;
 	static thread_local size_t lds_set = 0;
 	if (lds > 65536 && lds_set < lds) {
 		hipError_t e = hipFuncSetAttribute((const void *)k_pjoin,
@@ -563,7 +642,8 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevB
 			return kssdhip_fail(KSSDHIP_EHIP, "pjoin: LDS %zu bytes: %s", lds, hipGetErrorString(e));
 		lds_set = lds;
 	}
-	hipLaunchKernelGGL(k_pjoin, dim3(ix->P, qslices), dim3(ix->block), lds, s, a);
+	const uint32_t ngroups = (ix->P + a.group - 1) / a.group;
+	hipLaunchKernelGGL(k_pjoin, dim3(ngroups, qslices), dim3(ix->block), lds, s, a);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess)
 		return kssdhip_fail(KSSDHIP_EHIP, "pjoin launch: %s", hipGetErrorString(e));
@@ -581,6 +661,11 @@ extern "C" int kssdhip_index_set_tiling(kssdhip_index *ix, const kssdhip_tiling_
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: block_threads %d", t->block_threads);
 	if (t->auto_min_keys < 0)
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: auto_min_keys < 0");
+	if (t->group_partitions < 1 || t->group_partitions > 64)
+		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: group_partitions %d out of range 1..64",
+				    t->group_partitions);
+	if (t->group_partitions > 1 && t->chunk_records > 6144)
+		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: group_partitions > 1 needs chunk_records <= 6144");
 	if (hipSetDevice(ix->device) != hipSuccess || hipStreamSynchronize(ix->stream) != hipSuccess)
 		return kssdhip_fail(KSSDHIP_EHIP, "set_tiling: device not ready");
 	if (t->chunk_records != ix->chunk) {
@@ -591,6 +676,7 @@ extern "C" int kssdhip_index_set_tiling(kssdhip_index *ix, const kssdhip_tiling_
 	ix->mode = t->mode;
 	ix->block = t->block_threads;
 	ix->auto_min_keys = t->auto_min_keys;
+	ix->group = t->group_partitions;
 	return 0;
 }
 
@@ -602,5 +688,6 @@ extern "C" int kssdhip_index_get_tiling(const kssdhip_index *ix, kssdhip_tiling_
 	t->chunk_records = ix->chunk;
 	t->block_threads = ix->block;
 	t->auto_min_keys = ix->auto_min_keys;
+	t->group_partitions = ix->group;
 	return 0;
 }

@@ -42,8 +42,11 @@ class EpilogueOpts(ctypes.Structure):
 
 
 class Tiling(ctypes.Structure):
-    """kssdhip_tiling_t: mode 0 auto, 1 lookup, 2 partitioned merge-join."""
-    _fields_ = [("mode", I), ("chunk_records", I), ("block_threads", I), ("auto_min_keys", D)]
+    """kssdhip_tiling_t: mode 0 auto, 1 lookup, 2 partitioned merge-join;
+    group_partitions > 1 lets one workgroup of the join walk that many
+    consecutive partitions, summing hits per cell in LDS first."""
+    _fields_ = [("mode", I), ("chunk_records", I), ("block_threads", I), ("auto_min_keys", D),
+                ("group_partitions", I)]
 
 
 class U64Vec(ctypes.Structure):
@@ -259,8 +262,11 @@ class KssdHip:
                                                ctypes.byref(h)))
         return h
 
-    def set_tiling(self, h, mode=None, chunk_records=None, block_threads=None, auto_min_keys=None):
+    def set_tiling(self, h, mode=None, chunk_records=None, block_threads=None, auto_min_keys=None,
+                   group_partitions=None):
         t = self.get_tiling(h)
+        if group_partitions is not None:
+            t.group_partitions = group_partitions
         if mode is not None:
             t.mode = mode
         if chunk_records is not None:

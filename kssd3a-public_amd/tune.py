@@ -2,8 +2,9 @@
 
 For a resident index and a device-resident query block it times each
 candidate tiling with HIP events on the caller's stream -- the lookup
-kernel and the partitioned merge-join over a grid of LDS chunk sizes and
-workgroup sizes -- and keeps the fastest.  Results are cached in a JSON
+kernel and the partitioned merge-join over a grid of LDS chunk sizes,
+workgroup sizes and partition groups (partitions walked by one workgroup,
+which sums its hits per cell in LDS) -- and keeps the fastest.  Results are cached in a JSON
 file keyed by the device name and the shape of the problem (index records,
 query keys, queries, references, flags), so a serving process or a later
 run picks the shape up without re-measuring.
@@ -19,6 +20,9 @@ DEFAULT_CACHE = Path(os.environ.get("KSSDHIP_TUNE_CACHE",
                                     Path.home() / ".cache" / "kssdhip" / "tuning.json"))
 CHUNKS = (1024, 2048, 4096, 8192, 12288)
 BLOCKS = (128, 256, 512)
+GROUP_CHUNKS = (1024, 2048, 4096)  # partition groups need chunk_records <= 6144
+GROUP_BLOCKS = (256, 512)
+GROUPS = (2, 4, 8)
 
 
 def _bucket(x):
@@ -59,10 +63,20 @@ def save_cache(cache, path=DEFAULT_CACHE):
 
 
 def candidates():
-    yield {"mode": 1, "chunk_records": 4096, "block_threads": 256}
+    yield {"mode": 1, "chunk_records": 4096, "block_threads": 256, "group_partitions": 1}
     for c in CHUNKS:
         for b in BLOCKS:
-            yield {"mode": 2, "chunk_records": c, "block_threads": b}
+            yield {"mode": 2, "chunk_records": c, "block_threads": b, "group_partitions": 1}
+    for c in GROUP_CHUNKS:
+        for b in GROUP_BLOCKS:
+            for g in GROUPS:
+                yield {"mode": 2, "chunk_records": c, "block_threads": b, "group_partitions": g}
+
+
+def candidate_name(c):
+    name = f"m{c['mode']}_c{c['chunk_records']}_b{c['block_threads']}"
+    g = c.get("group_partitions", 1)
+    return name if g == 1 else f"{name}_g{g}"
 
 
 def tune(hip, rt, h, d_keys, d_skidx, nq, total, nref, n_index, d_ctx, d_obj, stream,
@@ -73,9 +87,10 @@ def tune(hip, rt, h, d_keys, d_skidx, nq, total, nref, n_index, d_ctx, d_obj, st
     key = shape_key(dev, n_index, total, nq, nref, flags)
     cache = load_cache(cache_path)
     if not force and key in cache:
-        best = cache[key]["best"]
+        best = dict(cache[key]["best"])
+        best.setdefault("group_partitions", 1)
         hip.set_tiling(h, mode=best["mode"], chunk_records=best["chunk_records"],
-                       block_threads=best["block_threads"])
+                       block_threads=best["block_threads"], group_partitions=best["group_partitions"])
         return best, cache[key].get("times", {})
     e0, e1 = rt.event(), rt.event()
     times = {}
@@ -90,7 +105,7 @@ def tune(hip, rt, h, d_keys, d_skidx, nq, total, nref, n_index, d_ctx, d_obj, st
                                 stream)
         rt.record(e1, stream)
         ms = rt.elapsed_ms(e0, e1) / reps
-        name = f"m{cand['mode']}_c{cand['chunk_records']}_b{cand['block_threads']}"
+        name = candidate_name(cand)
         times[name] = ms
         if log:
             log(f"tune {name}: {ms:.3f} ms")

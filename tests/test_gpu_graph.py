@@ -19,8 +19,8 @@ def _block(samples):
     return keys, idx
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_graph_replays_match_eager(hip, oracle, mode):
+@pytest.mark.parametrize("mode,group", [(1, 1), (2, 1), (2, 4)])
+def test_graph_replays_match_eager(hip, oracle, mode, group):
     rt = kt.Hip(0)
     rng = np.random.default_rng(21)
     pool = rng.integers(1, 1 << 40, size=900, dtype=np.uint64)
@@ -34,7 +34,7 @@ def test_graph_replays_match_eager(hip, oracle, mode):
     s = rt.stream()
     bufs = []
     try:
-        hip.set_tiling(h, mode=mode, chunk_records=256, block_threads=256)
+        hip.set_tiling(h, mode=mode, chunk_records=256, block_threads=256, group_partitions=group)
         d_k, d_i = rt.upload(qk), rt.upload(qi)
         qctx = np.diff(qi).astype(np.uint32)
         rctx = np.diff(ri).astype(np.uint32)

@@ -164,14 +164,18 @@ def test_index_open_rejects_bad_index(hip):
         hip.open_index(rec, 2, OBJ_BITS)
 
 
-TILINGS = [(1, 4096, 256), (2, 64, 256), (2, 128, 64), (2, 4096, 256), (2, 12288, 1024)]
+# (mode, chunk_records, block_threads, group_partitions)
+TILINGS = [(1, 4096, 256, 1), (2, 64, 256, 1), (2, 128, 64, 1), (2, 4096, 256, 1), (2, 12288, 1024, 1),
+           (2, 64, 256, 4), (2, 1024, 256, 8), (2, 6144, 512, 2), (2, 128, 64, 64)]
 
 
-@pytest.mark.parametrize("mode,chunk,block", TILINGS)
+@pytest.mark.parametrize("mode,chunk,block,group", TILINGS)
 @pytest.mark.parametrize("flags", [0, 2, 4, 6])
-def test_tilings_match_oracle(hip, oracle, mode, chunk, block, flags):
+def test_tilings_match_oracle(hip, oracle, mode, chunk, block, group, flags):
     """Lookup and partitioned paths, small partitions (many boundaries, runs
-    longer than a partition) and the largest LDS stage."""
+    longer than a partition), the largest LDS stage, and partition groups
+    whose LDS cell table holds every touched cell or overflows to the global
+    counters (64 partitions of 128 records: about 4000 cells per group)."""
     rng = np.random.default_rng(chunk + flags + mode)
     pool = rng.integers(1, 1 << 43, size=3000, dtype=np.uint64)
     refs = kt.random_samples(rng, 60, 3000, 300, shared_pool=pool, conflict_p=0.15)
@@ -186,7 +190,7 @@ def test_tilings_match_oracle(hip, oracle, mode, chunk, block, flags):
     qkeys, qidx = _block(qrys)
     h = hip.open_index(rec, len(refs), OBJ_BITS)
     try:
-        hip.set_tiling(h, mode=mode, chunk_records=chunk, block_threads=block)
+        hip.set_tiling(h, mode=mode, chunk_records=chunk, block_threads=block, group_partitions=group)
         got = hip.count_block(h, len(refs), qkeys, qidx, flags=flags)
         low = hip.count_block(h, len(refs), rkeys[ridx[10]:], ridx[10:], qry_gid_offset=10,
                               flags=1 | (flags & 4))
@@ -194,7 +198,7 @@ def test_tilings_match_oracle(hip, oracle, mode, chunk, block, flags):
         hip.close_index(h)
     want = oracle.count_block(rec, len(refs), OBJ_BITS, qkeys, qidx, flags=flags)
     for name, g, w in zip(("ctx", "obj", "uctx", "uobj"), got, want):
-        assert np.array_equal(g, w), (mode, chunk, flags, name)
+        assert np.array_equal(g, w), (mode, chunk, group, flags, name)
     wl = oracle.count_block(rec, len(refs), OBJ_BITS, rkeys[ridx[10]:], ridx[10:], qry_gid_offset=10,
                             flags=1 | (flags & 4))
     assert np.array_equal(low[0], wl[0]) and np.array_equal(low[1], wl[1])
@@ -206,9 +210,10 @@ def test_tiling_validation(hip):
     h = hip.open_index(kt.build_index_np(keys, idx, OBJ_BITS), 4, OBJ_BITS)
     try:
         t = hip.get_tiling(h)
-        assert (t.mode, t.chunk_records, t.block_threads) == (0, 4096, 256)
+        assert (t.mode, t.chunk_records, t.block_threads, t.group_partitions) == (0, 4096, 256, 1)
         for bad in (dict(mode=3), dict(chunk_records=10), dict(chunk_records=20000),
-                    dict(block_threads=100)):
+                    dict(block_threads=100), dict(group_partitions=0), dict(group_partitions=65),
+                    dict(chunk_records=8192, group_partitions=2)):
             with pytest.raises(RuntimeError):
                 hip.set_tiling(h, **bad)
     finally:
