@@ -108,9 +108,9 @@ def measured_traffic(tiling, nref, nqry):
         return None, None
     text = f.read_text()
     m = re.search(r"traffic_bytes_per_step=(\d+)", text)
-    mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+)", text)
+    mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
     if m and mt and tuple(map(int, mt.groups())) == (tiling["mode"], tiling["chunk_records"],
-                                                        tiling["block_threads"]):
+                                                        tiling["block_threads"], tiling.get("group_partitions", 1)):
         return int(m.group(1)), f"profiles/{f.name}"
     return None, f"profiles/{f.name} (measured with another tiling)"
 
@@ -410,6 +410,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all)")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--tiling", default="",
+                    help="fixed join tiling MODE,CHUNK,BLOCK[,GROUP] instead of the autotuner (for profiling runs)")
     ap.add_argument("--latency-queries", type=int, default=64)
     ap.add_argument("--index-refs", type=int, default=10000,
                     help="reference genomes of the large-index bench (500 clades of 20); 0 skips it")
@@ -464,9 +466,17 @@ def main():
         hip.epilogue_dev(d_ctx, d_obj, nqry, nref, d_qctx, d_rctx, opts, d_out, stream)
 
     t = time.time()
-    tiling, tune_ms = tune.tune(hip, rt, h, d_qkeys, d_qidx, nqry, total, nref, len(records), d_ctx, d_obj,
-                                stream, flags=0, force=args.retune, log=log)
-    log(f"autotuned tiling {tiling} in {time.time() - t:.1f}s")
+    if args.tiling:
+        v = [int(x) for x in args.tiling.split(",")]
+        tiling = {"mode": v[0], "chunk_records": v[1], "block_threads": v[2],
+                  "group_partitions": v[3] if len(v) > 3 else 1}
+        hip.set_tiling(h, **tiling)
+        tune_ms = {}
+        log(f"fixed tiling {tiling}")
+    else:
+        tiling, tune_ms = tune.tune(hip, rt, h, d_qkeys, d_qidx, nqry, total, nref, len(records), d_ctx, d_obj,
+                                    stream, flags=0, force=args.retune, log=log)
+        log(f"autotuned tiling {tiling} in {time.time() - t:.1f}s")
     if args.copy_probe_mib:
         nb = args.copy_probe_mib << 20
         a_, b_ = rt.malloc(nb), rt.malloc(nb)

@@ -119,11 +119,11 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  * chunk_records (partitions are rebuilt when it changes) must leave the
  * staged partition within the 160 KB LDS: 64 <= chunk_records <= 12288.
  * group_partitions (1..64): consecutive partitions one workgroup walks;
- * above 1 the workgroup sums its hits per (query, genome) cell in an LDS
- * table of KSSDHIP_CELL_TABLE entries and adds each cell to the outputs
- * once, which needs chunk_records <= 6144.
+ * above 1 the workgroup keeps one LDS slot per query that sums the hits on
+ * the first genome flushed for it over all the group's partitions and adds
+ * it to the outputs once (hits on other genomes go to the outputs
+ * directly), which needs chunk_records <= 6144.
  */
-#define KSSDHIP_CELL_TABLE 2048
 typedef struct {
 	int mode;
 	int chunk_records;

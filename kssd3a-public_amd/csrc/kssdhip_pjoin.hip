@@ -81,7 +81,13 @@ struct SplitArgs {
 __global__ __launch_bounds__(256) void k_split(SplitArgs a)
 {
 	const uint32_t width = a.P + 1, runs = (width + SPLIT_RUN - 1) / SPLIT_RUN;
-	const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+	// XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a
+	// contiguous run of workgroups (so of queries), whose binary searches
+	// share the upper levels of their search paths in that XCD's L2.
+	const uint32_t G = gridDim.x, b = blockIdx.x;
+	const uint32_t xcd = b % NUM_XCD, nth = b / NUM_XCD, base = G / NUM_XCD, rem = G % NUM_XCD;
+	const uint32_t wg = xcd * base + (xcd < rem ? xcd : rem) + nth;
+	const uint64_t t = wg * (uint64_t)blockDim.x + threadIdx.x;
 	if (t >= (uint64_t)a.nq * runs)
 		return;
 	const int q = (int)(t / runs);
@@ -127,6 +133,10 @@ __global__ __launch_bounds__(256) void k_split(SplitArgs a)
 	}
 }
 
+// LDS (address space 3) pointers: accesses through them compile to ds_*
+// instructions rather than flat ones.
+#define LDS __attribute__((address_space(3)))
+
 struct PJArgs {
 	JoinArgs j;
 	const uint32_t *pstart;
@@ -134,36 +144,57 @@ struct PJArgs {
 	uint32_t P;
 	uint32_t cap;   // records that fit the LDS stage
 	uint32_t group; // consecutive partitions per workgroup
-	uint32_t ct;    // LDS cell-table entries (0: none)
+	uint32_t slots; // LDS query slots (0 unless group > 1)
+	int bbits;      // log2 of the context buckets of a staged partition
 };
 
-// LDS table of (query, genome) cells: key = unique << 31 | query-in-slice
-// << 20 | genome; linear probing; a key that does not find a slot within
-// CT_PROBES goes straight to the global counters.
-constexpr uint32_t CT_EMPTY = 0xffffffffu;
-constexpr int CT_PROBES = 16;
-struct CellTable {
-	uint32_t *key;
-	unsigned long long *c, *o;
-	uint32_t n;    // power of two, 0 = no table
-	uint32_t bits; // log2(n)
+// Per-query hit slots of a partition group in LDS: slot k belongs to query
+// qb0 + k and holds the first genome flushed for it with the summed
+// counters; hits on that query's other genomes go straight to the global
+// counters.  With UNIQ a second set of nq slots holds the unique copies.
+constexpr uint32_t SLOT_EMPTY = 0xffffffffu;
+struct QuerySlots {
+	LDS uint32_t *gid;
+	LDS unsigned long long *c, *o;
 };
+
+// Adds (c, o) to query qi's slot if it holds (or can claim) genome gid.
+__device__ __forceinline__ bool slot_add(const QuerySlots &t, uint32_t qi, uint32_t gid, unsigned long long c,
+					 unsigned long long o)
+{
+	uint32_t g = t.gid[qi];
+	if (g == SLOT_EMPTY) {
+		uint32_t expected = SLOT_EMPTY;
+		__hip_atomic_compare_exchange_strong(&t.gid[qi], &expected, gid, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+						     __HIP_MEMORY_SCOPE_WORKGROUP);
+		g = expected == SLOT_EMPTY ? gid : expected;
+	}
+	if (g != gid)
+		return false;
+	__hip_atomic_fetch_add(&t.c[qi], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+	if (o)
+		__hip_atomic_fetch_add(&t.o[qi], o, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+	return true;
+}
 
 constexpr uint32_t QB = 1024; // queries per workgroup; grid.y covers the block's queries
 
 // Context buckets of a staged partition: bucket b holds the records whose
 // context c has (c - c0) >> shift == b; bt[b] = first record of bucket b.
-constexpr int PART_BUCKET_BITS = 10;
-constexpr uint32_t PART_BUCKETS = 1u << PART_BUCKET_BITS;
+#ifndef MAX_BUCKET_BITS
+#define MAX_BUCKET_BITS 13
+#endif
+// dynamic LDS a k_pjoin workgroup may take (160 KB per CU, less the static part)
+constexpr uint32_t PJ_LDS_MAX = 160 * 1024 - 256;
 struct PartTable {
-	const uint16_t *bt;
+	const LDS uint16_t *bt;
 	uint64_t c0, clast;
 	int shift;
 };
 
 // In-place exclusive scan of v[0, n) in LDS; returns the total.  Every
 // thread of the block calls it.  wsum: LDS scratch of blockDim/64 + 1 words.
-__device__ uint32_t block_exclusive_scan(uint32_t *v, uint32_t n, uint32_t *wsum)
+__device__ uint32_t block_exclusive_scan(LDS uint32_t *v, uint32_t n, uint32_t *wsum)
 {
 	const uint32_t B = blockDim.x, t = threadIdx.x, nw = B / warpSize;
 	const uint32_t per = (n + B - 1) / B;
@@ -203,56 +234,36 @@ __device__ uint32_t block_exclusive_scan(uint32_t *v, uint32_t n, uint32_t *wsum
 }
 
 // Per-thread accumulation of consecutive hits on one counter cell, flushed
-// when the cell changes: into the workgroup's LDS cell table when it has one,
-// else with one 64-bit atomic per counter.
+// when the cell changes: into the workgroup's query slots when it has them
+// (CT), else with one 64-bit atomic per counter.
+template <bool CT>
 struct CellAcc {
 	uint64_t cell = ~0ull; // global cell index
-	uint32_t key = CT_EMPTY; // LDS table key
+	uint32_t qi = 0, gid = 0;
 	unsigned long long c = 0, o = 0;
-	__device__ __forceinline__ void add(uint64_t at, uint32_t k, int m, const CellTable &t, unsigned long long *ctx,
-					    unsigned long long *obj)
+	__device__ __forceinline__ void add(uint64_t at, uint32_t q, uint32_t g, int m, const QuerySlots &t,
+					    unsigned long long *ctx, unsigned long long *obj)
 	{
 		if (at != cell) {
 			flush(t, ctx, obj);
 			cell = at;
-			key = k;
+			qi = q;
+			gid = g;
 		}
 		c += 1ull | ((unsigned long long)(m > 1) << 32);
 		if (m)
 			o += 1ull | ((unsigned long long)m << 32);
 	}
-	__device__ __forceinline__ void flush(const CellTable &t, unsigned long long *ctx, unsigned long long *obj)
+	__device__ __forceinline__ void flush(const QuerySlots &t, unsigned long long *ctx, unsigned long long *obj)
 	{
 		if (cell == ~0ull)
 			return;
-		bool done = false;
-		if (t.n && key != CT_EMPTY) {
-			// Fibonacci hashing: the product's top bits mix every key bit
-			uint32_t h = (key * 0x9E3779B1u) >> (32 - t.bits);
-			for (int i = 0; i < CT_PROBES; ++i) {
-				uint32_t k = t.key[h];
-				if (k == CT_EMPTY) {
-					k = atomicCAS(&t.key[h], CT_EMPTY, key);
-					if (k == CT_EMPTY)
-						k = key;
-				}
-				if (k == key) {
-					atomicAdd(&t.c[h], c);
-					if (o)
-						atomicAdd(&t.o[h], o);
-					done = true;
-					break;
-				}
-				h = (h + 1) & (t.n - 1);
-			}
-		}
-		if (!done) {
+		if (!CT || !slot_add(t, qi, gid, c, o)) {
 			atomicAdd(&ctx[cell], c);
 			if (o)
 				atomicAdd(&obj[cell], o);
 		}
 		cell = ~0ull;
-		key = CT_EMPTY;
 		c = o = 0;
 	}
 };
@@ -264,17 +275,19 @@ struct CellAcc {
 // does the same number of lookups and a lane's keys mostly belong to one
 // query; the first key of each context group does the work for the group
 // (command_ani.h:506-575, :668-716), and hits are summed per (query, genome)
-// cell in registers before one atomic per cell and counter.
-template <bool TABLE, typename RecPtr, typename ObjPtr>
+// cell in registers before one atomic per cell and counter.  UNIQ: also the
+// unique-owner copies (command_ani.h:611-617); STAGED: rec/robj and the
+// bucket table are in LDS, else rec/robj are the partition in HBM and the
+// first record is found by binary search.
+template <bool UNIQ, bool CT, bool STAGED, typename RecPtr, typename ObjPtr>
 __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr robj, uint32_t cnt,
-					   const PartTable &tab, const uint32_t *pre, const uint32_t *start,
-					   int qb0, int nqb, uint32_t K, const CellTable &ct)
+					   const PartTable &tab, const LDS uint32_t *pre, const LDS uint32_t *start,
+					   int qb0, int nqb, uint32_t K, const QuerySlots &qs, const QuerySlots &us)
 {
 	const JoinArgs &j = a.j;
 	const unsigned ob = j.obj_bits;
 	const uint64_t objmask = ob >= 64 ? ~0ull : ((1ull << ob) - 1ull);
 	const bool lower = (j.flags & KSSDHIP_LOWER) != 0;
-	const bool unique = !lower && (j.flags & KSSDHIP_UNIQUE) != 0;
 	const bool ignore = (j.flags & KSSDHIP_IGNORECONFLICT) != 0;
 	const uint32_t per = (K + blockDim.x - 1) / blockDim.x;
 	const uint32_t f_begin = threadIdx.x * per;
@@ -293,17 +306,22 @@ __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr r
 		}
 		qi = lo;
 	}
-	CellAcc acc, uacc;
+	CellAcc<CT> acc, uacc;
 	uint64_t prev_key = 0, next_key = 0;
 	uint32_t prev_idx = ~0u, next_idx = ~0u;
+	uint32_t q_pre = pre[qi], q_next = pre[qi + 1], q_start = start[qi];
 	for (uint32_t f = f_begin; f < f_end; ++f) {
-		while (pre[qi + 1] <= f)
+		while (q_next <= f) {
 			++qi;
-		const uint32_t kidx = start[qi] + (f - pre[qi]);
-		const uint32_t qend = start[qi] + (pre[qi + 1] - pre[qi]);
+			q_pre = q_next;
+			q_next = pre[qi + 1];
+			q_start = start[qi];
+		}
+		const uint32_t kidx = q_start + (f - q_pre);
+		const uint32_t qend = q_start + (q_next - q_pre);
 		const uint64_t key = kidx == next_idx ? next_key : j.keys[kidx];
 		const uint64_t qctx = key >> ob;
-		if (f != pre[qi]) {
+		if (f != q_pre) {
 			const uint64_t prev = kidx - 1 == prev_idx ? prev_key : j.keys[kidx - 1];
 			if ((prev >> ob) == qctx) {
 				prev_key = key;
@@ -325,7 +343,7 @@ __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr r
 		}
 		// first record of the partition whose context is >= qctx
 		uint32_t d;
-		if (TABLE) {
+		if (STAGED) {
 			if (qctx > tab.clast || qctx < tab.c0) {
 				d = cnt;
 			} else {
@@ -368,7 +386,7 @@ __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr r
 					break;
 				continue;
 			}
-			if (unique) {
+			if (UNIQ) {
 				++owners;
 				sole_gid = gid;
 			}
@@ -387,39 +405,60 @@ __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr r
 					m = c < m ? c : m;
 				}
 			}
-			acc.add(row + gid, ((uint32_t)qi << GID_NBITS) | gid, m, ct, j.ctx, j.objc);
-			if (unique) {
+			acc.add(row + gid, (uint32_t)qi, gid, m, qs, j.ctx, j.objc);
+			if (UNIQ) {
 				sole_diff = m;
 				sole_allowed = true;
 			}
 		}
-		if (unique && owners == 1 && sole_allowed)
-			uacc.add(row + sole_gid, 0x80000000u | ((uint32_t)qi << GID_NBITS) | sole_gid, sole_diff, ct, j.uctx,
-				 j.uobj);
+		if (UNIQ && owners == 1 && sole_allowed)
+			uacc.add(row + sole_gid, (uint32_t)qi, sole_gid, sole_diff, us, j.uctx, j.uobj);
 	}
-	acc.flush(ct, j.ctx, j.objc);
-	if (unique)
-		uacc.flush(ct, j.uctx, j.uobj);
+	acc.flush(qs, j.ctx, j.objc);
+	if (UNIQ)
+		uacc.flush(us, j.uctx, j.uobj);
 }
 
+// Byte offsets of k_pjoin's LDS arrays for a stage of `cap` records (a
+// multiple of 4), 2^bbits buckets, qcap queries and `slots` query slots:
+// [cap] u64 ctxgid | [cap] u32 obj | [2^bbits + 2] u16 bucket table |
+// [qcap + 1] u32 prefix | [qcap] u32 starts | [slots] u32 genome (8-byte
+// aligned) | [slots] u64 ctx counters | [slots] u64 obj counters.
+struct PJLayout {
+	uint32_t bt, pre, start, sg, sc, so, bytes;
+	__host__ __device__ PJLayout(uint32_t cap, int bbits, uint32_t qcap, uint32_t slots)
+	{
+		bt = cap * 12;
+		pre = bt + ((1u << bbits) + 2) * 2;
+		start = pre + (qcap + 1) * 4;
+		sg = (start + qcap * 4 + 7) & ~7u;
+		sc = (sg + slots * 4 + 7) & ~7u;
+		so = sc + slots * 8;
+		bytes = so + slots * 8;
+	}
+};
+
+template <bool UNIQ, bool CT>
 __global__ void k_pjoin(PJArgs a)
 {
-	// LDS: [cap] ctxgid | [cap] obj (u32) | [PART_BUCKETS + 2] u16 table |
-	//      [qcap + 1] u32 prefix | [qcap] u32 starts, qcap = min(nq, QB) |
-	//      cell table: [ct] u32 keys (8-byte aligned) | [ct] u64 | [ct] u64
-	extern __shared__ uint64_t lds_rec[];
+	// LDS arrays as PJLayout places them, qcap = min(nq, QB)
+	extern __shared__ uint64_t lds_dyn[];
 	__shared__ uint32_t wsum[1024 / 64 + 1];
+	LDS char *const lds = (LDS char *)lds_dyn;
 	const uint32_t qcap = (uint32_t)a.j.nq < QB ? (uint32_t)a.j.nq : QB;
-	uint32_t *lds_obj = (uint32_t *)(lds_rec + a.cap);
-	uint16_t *lds_bt = (uint16_t *)(lds_obj + a.cap);
-	uint32_t *lds_pre = (uint32_t *)(lds_bt + PART_BUCKETS + 2);
-	uint32_t *lds_start = lds_pre + qcap + 1;
-	CellTable ct;
-	ct.n = a.ct;
-	ct.bits = a.ct ? (uint32_t)(__ffs(a.ct) - 1) : 0;
-	ct.key = (uint32_t *)(((uintptr_t)(lds_start + qcap) + 7) & ~(uintptr_t)7);
-	ct.c = (unsigned long long *)(ct.key + a.ct);
-	ct.o = ct.c + a.ct;
+	const PJLayout lay(a.cap, a.bbits, qcap, a.slots);
+	LDS uint64_t *const lds_rec = (LDS uint64_t *)lds;
+	LDS uint32_t *const lds_obj = (LDS uint32_t *)(lds + a.cap * 8);
+	LDS uint16_t *const lds_bt = (LDS uint16_t *)(lds + lay.bt);
+	LDS uint32_t *const lds_pre = (LDS uint32_t *)(lds + lay.pre);
+	LDS uint32_t *const lds_start = (LDS uint32_t *)(lds + lay.start);
+	QuerySlots qs, us; // us: the unique copies' slots, after the nq plain ones
+	qs.gid = (LDS uint32_t *)(lds + lay.sg);
+	qs.c = (LDS unsigned long long *)(lds + lay.sc);
+	qs.o = (LDS unsigned long long *)(lds + lay.so);
+	us.gid = qs.gid + qcap;
+	us.c = qs.c + qcap;
+	us.o = qs.o + qcap;
 	// XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a
 	// contiguous run of partition groups.
 	const uint32_t G = gridDim.x, b = blockIdx.x;
@@ -431,10 +470,12 @@ __global__ void k_pjoin(PJArgs a)
 	const uint32_t p_end = p_begin + a.group < a.P ? p_begin + a.group : a.P;
 	const int qb0 = (int)(blockIdx.y * QB);
 	const int nqb = a.j.nq - qb0 < (int)QB ? a.j.nq - qb0 : (int)QB;
-	for (uint32_t k = threadIdx.x; k < ct.n; k += blockDim.x) {
-		ct.key[k] = CT_EMPTY;
-		ct.c[k] = 0;
-		ct.o[k] = 0;
+	if (CT) {
+		for (uint32_t k = threadIdx.x; k < a.slots; k += blockDim.x) {
+			qs.gid[k] = SLOT_EMPTY;
+			qs.c[k] = 0;
+			qs.o[k] = 0;
+		}
 	}
 	const uint64_t width = (uint64_t)a.P + 1;
 	for (uint32_t p = p_begin; p < p_end; ++p) {
@@ -444,7 +485,7 @@ __global__ void k_pjoin(PJArgs a)
 			const uint64_t r = (uint64_t)(qb0 + k) * width + p;
 			const uint32_t s0 = a.S[r], s1 = a.S[r + 1];
 			lds_start[k] = s0;
-			lds_pre[k] = s1 - s0;
+			lds_pre[k] = s1 > s0 ? s1 - s0 : 0; // (unsorted keys must not underflow)
 		}
 		const uint32_t rb = a.pstart[p], re = a.pstart[p + 1], cnt = re - rb;
 		const uint32_t rb_al = rb & ~(STAGE_ALIGN - 1), re_al = (re + STAGE_ALIGN - 1) & ~(STAGE_ALIGN - 1);
@@ -455,8 +496,8 @@ __global__ void k_pjoin(PJArgs a)
 			const uint32_t groups = (re_al - rb_al) / STAGE_ALIGN;
 			const uint4 *g_rec = (const uint4 *)(a.j.ctxgid + rb_al);
 			const uint4 *g_obj = (const uint4 *)(a.j.robj + rb_al);
-			uint4 *l_rec = (uint4 *)lds_rec;
-			uint4 *l_obj = (uint4 *)lds_obj;
+			uint4 *l_rec = (uint4 *)lds_dyn; // generic pointers into the LDS stage
+			uint4 *l_obj = (uint4 *)(lds_dyn + a.cap);
 			for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
 				const uint4 r0 = g_rec[2 * g], r1 = g_rec[2 * g + 1], o = g_obj[g];
 				l_rec[2 * g] = r0;
@@ -471,21 +512,21 @@ __global__ void k_pjoin(PJArgs a)
 		if (!staged) {
 			__syncthreads();
 			// a context run longer than the stage: merge straight from HBM
-			merge_keys<false>(a, a.j.ctxgid + rb, a.j.robj + rb, cnt, PartTable{}, lds_pre, lds_start, qb0,
-					  nqb, K, ct);
+			merge_keys<UNIQ, CT, false>(a, a.j.ctxgid + rb, a.j.robj + rb, cnt, PartTable{}, lds_pre, lds_start,
+						    qb0, nqb, K, qs, us);
 			continue;
 		}
 		const uint32_t off = rb - rb_al;
-		const uint64_t *srec = lds_rec + off;
-		const uint32_t *sobj = lds_obj + off;
-		// bucket table over [c0, clast]: at most PART_BUCKETS buckets
+		const LDS uint64_t *srec = lds_rec + off;
+		const LDS uint32_t *sobj = lds_obj + off;
+		// bucket table over [c0, clast]: at most 2^bbits buckets
 		PartTable tab;
 		tab.bt = lds_bt;
 		tab.c0 = srec[0] >> GID_NBITS;
 		tab.clast = srec[cnt - 1] >> GID_NBITS;
 		const uint64_t span = tab.clast - tab.c0; // buckets = (span >> shift) + 1
 		const int bits = span ? 64 - __clzll((long long)span) : 0;
-		tab.shift = bits > PART_BUCKET_BITS ? bits - PART_BUCKET_BITS : 0;
+		tab.shift = bits > a.bbits ? bits - a.bbits : 0;
 		for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
 			const int bk = (int)(((srec[k] >> GID_NBITS) - tab.c0) >> tab.shift);
 			const int bp = k ? (int)(((srec[k - 1] >> GID_NBITS) - tab.c0) >> tab.shift) : -1;
@@ -495,23 +536,22 @@ __global__ void k_pjoin(PJArgs a)
 		if (threadIdx.x == 0)
 			lds_bt[(span >> tab.shift) + 1] = (uint16_t)cnt;
 		__syncthreads();
-		merge_keys<true>(a, srec, sobj, cnt, tab, lds_pre, lds_start, qb0, nqb, K, ct);
+		merge_keys<UNIQ, CT, true>(a, srec, sobj, cnt, tab, lds_pre, lds_start, qb0, nqb, K, qs, us);
 	}
-	if (!ct.n)
+	if (!CT)
 		return;
 	__syncthreads();
-	// each cell the workgroup touched, added to the outputs once
+	// each claimed slot added to the outputs once
 	const JoinArgs &j = a.j;
-	for (uint32_t k = threadIdx.x; k < ct.n; k += blockDim.x) {
-		const uint32_t key = ct.key[k];
-		if (key == CT_EMPTY)
+	for (uint32_t k = threadIdx.x; k < a.slots; k += blockDim.x) {
+		const uint32_t g = qs.gid[k];
+		if (g == SLOT_EMPTY)
 			continue;
-		const bool u = (key >> 31) != 0;
-		const uint64_t cell =
-			(uint64_t)(qb0 + (int)((key >> GID_NBITS) & (QB - 1))) * (uint64_t)j.ref_gnum + (key & GID_MASK);
-		atomicAdd(&(u ? j.uctx : j.ctx)[cell], ct.c[k]);
-		if (ct.o[k])
-			atomicAdd(&(u ? j.uobj : j.objc)[cell], ct.o[k]);
+		const bool u = k >= qcap;
+		const uint64_t cell = (uint64_t)(qb0 + (int)(u ? k - qcap : k)) * (uint64_t)j.ref_gnum + g;
+		atomicAdd(&(u ? j.uctx : j.ctx)[cell], qs.c[k]);
+		if (qs.o[k])
+			atomicAdd(&(u ? j.uobj : j.objc)[cell], qs.o[k]);
 	}
 }
 
@@ -629,21 +669,39 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevB
 	a.P = ix->P;
 	a.cap = ((uint32_t)ix->chunk + LDS_SLACK + STAGE_ALIGN - 1) & ~(STAGE_ALIGN - 1);
 	a.group = (uint32_t)ix->group;
-	a.ct = ix->group > 1 ? KSSDHIP_CELL_TABLE : 0;
 	const uint32_t qslices = (uint32_t)((ja.nq + QB - 1) / QB);
 	const uint32_t qb = (uint32_t)ja.nq < QB ? (uint32_t)ja.nq : QB;
-	const size_t lds = (size_t)a.cap * 12 + (PART_BUCKETS + 2) * 2 + ((size_t)2 * qb + 1) * 4 + 8 +
-			   (size_t)a.ct * 20;
-	static thread_local size_t lds_set = 0;
-	if (lds > 65536 && lds_set < lds) {
-		hipError_t e = hipFuncSetAttribute((const void *)k_pjoin,
-						   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	const bool uniq = (ja.flags & KSSDHIP_UNIQUE) && !(ja.flags & KSSDHIP_LOWER);
+	a.slots = ix->group > 1 ? qb * (uniq ? 2 : 1) : 0;
+	// about one staged record per bucket, as far as the LDS allows
+	a.bbits = 1;
+	while (a.bbits < MAX_BUCKET_BITS && (2u << a.bbits) <= a.cap &&
+	       PJLayout(a.cap, a.bbits + 1, qb, a.slots).bytes <= PJ_LDS_MAX)
+		++a.bbits;
+	const size_t lds = PJLayout(a.cap, a.bbits, qb, a.slots).bytes;
+	if (lds > PJ_LDS_MAX)
+		return kssdhip_fail(KSSDHIP_EINVAL, "pjoin: LDS %zu bytes over the limit", lds);
+	const bool ctab = a.slots != 0;
+	const void *fn = uniq ? (ctab ? (const void *)k_pjoin<true, true> : (const void *)k_pjoin<true, false>)
+			      : (ctab ? (const void *)k_pjoin<false, true> : (const void *)k_pjoin<false, false>);
+	static thread_local size_t lds_set[4] = {0, 0, 0, 0};
+	const int fi = (uniq ? 2 : 0) + (ctab ? 1 : 0);
+	if (lds > 65536 && lds_set[fi] < lds) {
+		hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 		if (e != hipSuccess)
 			return kssdhip_fail(KSSDHIP_EHIP, "pjoin: LDS %zu bytes: %s", lds, hipGetErrorString(e));
-		lds_set = lds;
+		lds_set[fi] = lds;
 	}
 	const uint32_t ngroups = (ix->P + a.group - 1) / a.group;
-	hipLaunchKernelGGL(k_pjoin, dim3(ngroups, qslices), dim3(ix->block), lds, s, a);
+	const dim3 grid(ngroups, qslices), block(ix->block);
+	if (uniq && ctab)
+		hipLaunchKernelGGL((k_pjoin<true, true>), grid, block, lds, s, a);
+	else if (uniq)
+		hipLaunchKernelGGL((k_pjoin<true, false>), grid, block, lds, s, a);
+	else if (ctab)
+		hipLaunchKernelGGL((k_pjoin<false, true>), grid, block, lds, s, a);
+	else
+		hipLaunchKernelGGL((k_pjoin<false, false>), grid, block, lds, s, a);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess)
 		return kssdhip_fail(KSSDHIP_EHIP, "pjoin launch: %s", hipGetErrorString(e));

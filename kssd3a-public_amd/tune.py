@@ -20,8 +20,8 @@ DEFAULT_CACHE = Path(os.environ.get("KSSDHIP_TUNE_CACHE",
                                     Path.home() / ".cache" / "kssdhip" / "tuning.json"))
 CHUNKS = (1024, 2048, 4096, 8192, 12288)
 BLOCKS = (128, 256, 512)
-GROUP_CHUNKS = (1024, 2048, 4096)  # partition groups need chunk_records <= 6144
-GROUP_BLOCKS = (256, 512)
+GROUP_CHUNKS = (2048, 3072, 4096, 6144)  # partition groups need chunk_records <= 6144
+GROUP_BLOCKS = (512, 1024)
 GROUPS = (2, 4, 8)
 
 
