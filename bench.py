@@ -148,6 +148,27 @@ def clade_specs(rng, nref, seed):
     return specs, cseed, clen
 
 
+def index_build_bench(host, hip, rk, ri, threads, reps=1):
+    """The inverted index of sketch (rk, ri) built by the host builder and by
+    kssdhip_build_index (host buffers in and out, so its time includes both
+    PCIe copies; best of `reps`); the GPU records must be the host's bytes."""
+    t = time.perf_counter()
+    rec = host.build_index(rk, ri, 20, threads=threads)
+    host_s = time.perf_counter() - t
+    hip.build_index(rk[: int(ri[1])], ri[:2])  # warm the kernels
+    gpu_s = float("inf")
+    for _ in range(reps):
+        t = time.perf_counter()
+        grec, gbits = hip.build_index(rk, ri)
+        gpu_s = min(gpu_s, time.perf_counter() - t)
+    build = {"records": int(len(rec)), "gpu_ms_host_buffers": gpu_s * 1e3,
+             "gpu_records_per_s": len(rec) / gpu_s, "host_ms": host_s * 1e3, "host_threads": threads,
+             "host_what": "kssd_build_index (bucketed parallel sort, records only, no unique bits)",
+             "gpu_what": "kssdhip_build_index: upload keys, records + LSD radix sort + unique bits, download both",
+             "records_identical": bool(grec.tobytes() == rec.tobytes()), "unique_bits_bytes": len(gbits)}
+    return rec, build
+
+
 def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
     """Large resident index: args.index_refs genomes (clades of 20, as above),
     built into the inverted index; two query sets counted in one batched
@@ -165,10 +186,14 @@ def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
     specs, cseed, clen = clade_specs(rng, nref, args.seed + 200)
     t = time.time()
     rk, ri = host.simsketch(specs, threads=threads)
-    rec = host.build_index(rk, ri, 20, threads=threads)
+    t_sk = time.time() - t
+    rec, build = index_build_bench(host, hip, rk, ri, threads)
+    host_build_s = build["host_ms"] / 1e3
+    gpu_build_s = build["gpu_ms_host_buffers"] / 1e3
     rctx = host.ctx_counts(rk, ri, 20)
     del rk
-    log(f"index bench: {nref} genomes, {len(rec)} records, built in {time.time() - t:.1f}s")
+    log(f"index bench: {nref} genomes, {len(rec)} records; sketched in {t_sk:.1f}s; index on the host "
+        f"{host_build_s * 1e3:.0f} ms, on the GPU {gpu_build_s * 1e3:.0f} ms (identical: {build['records_identical']})")
     nclade = nref // 20
     nq = args.index_queries
     qspecs = []
@@ -191,7 +216,7 @@ def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
     ridx[1:] = np.cumsum([len(x) for x in reads])
     h = hip.open_index(rec, nref, 20)
     stream = rt.stream()
-    out = {"index_genomes": nref, "index_records": int(len(rec))}
+    out = {"index_genomes": nref, "index_records": int(len(rec)), "index_build": build}
     orc = None
     for name, keys, idx, flags in (("assemblies", qk, qi, 0), ("read_sets", rkeys, ridx, kh.UNIQUE)):
         n = len(idx) - 1
@@ -410,6 +435,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all)")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--index-build-only", type=int, default=0, metavar="GENOMES",
+                    help="only time the index build of GENOMES clade genomes (host builder vs GPU) and exit")
     ap.add_argument("--tiling", default="",
                     help="fixed join tiling MODE,CHUNK,BLOCK[,GROUP] instead of the autotuner (for profiling runs)")
     ap.add_argument("--latency-queries", type=int, default=64)
@@ -431,6 +458,15 @@ def main():
         sys.exit(2)
     threads = args.threads or min(16, os.cpu_count() or 1)
     host = kh.Host()
+    if args.index_build_only:
+        # the index build alone: clade genomes as in the index bench
+        rng = np.random.default_rng(args.seed + 200)
+        nref = args.index_build_only - args.index_build_only % 20
+        specs, _, _ = clade_specs(rng, nref, args.seed + 200)
+        rk, ri = host.simsketch(specs, threads=threads)
+        _, build = index_build_bench(host, kh.KssdHip(0), rk, ri, threads, reps=3)
+        print(json.dumps({"index_build": build, "index_genomes": nref}), flush=True)
+        return
     rkeys, ridx, qkeys, qidx = make_workload(args.nref, args.nqry, args.seed, threads, host)
     obj_bits = 20
     t = time.time()

@@ -103,6 +103,24 @@ int kssdhip_init(int device);
 int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int ref_gnum,
 		       unsigned obj_bits, kssdhip_index **out);
 int kssdhip_index_close(kssdhip_index *index);
+
+/*
+ * Build the sorted context index of a sketch on the current device: the
+ * records {ctx << 20 | gid, obj} of every key sorted by (ctxgid, obj)
+ * (the bytes of sortedcomb_ctxgid64obj32) and, when unique_bits is not
+ * NULL, its unique-marker bitset ((n + 7) / 8 bytes, LSB first: bit i set
+ * iff record i's object occurs once among its context's records).
+ * Replaces: gen_inverted_index4comblco (command_sketch.c:916-938:
+ * ctxobj64_2ctxgidobj, sketch_rearrange.c:65-78, and ctxgidobj_sort_array,
+ * kssdlib_sort.c:215-219) and write_unique_marker_bits_from_sorted
+ * (command_sketch.c:855-914).
+ * keys/skidx: comblco and comblco.index (skidx[0] == 0, n = skidx[nsamples]
+ * < 2^32 - 1); each sample's keys ascending, as comblco stores them
+ * (KSSDHIP_EINVAL otherwise); nsamples < 2^20; obj_bits in 1..32 with every
+ * context (key >> obj_bits) below 2^44.  records: n entries.
+ */
+int kssdhip_build_index(const uint64_t *keys, const uint64_t *skidx, int nsamples, unsigned obj_bits,
+			kssd_ctxgidobj_t *records, uint8_t *unique_bits);
 /* Device bytes held by the handle (index arrays, bucket table, scratch). */
 size_t kssdhip_index_device_bytes(const kssdhip_index *index);
 

@@ -455,25 +455,48 @@ static kssdhip_index *gpu_index(const kssd_ctxgidobj_t *rec, size_t n, int gnum,
 	return ix;
 }
 
-static kssd_ctxgidobj_t *build_records(const kssd_sketchdir_t *s, int threads)
+/* The sorted index of a sketch without one (the in-memory index of the
+ * reference's indexed paths, comb_sortedsketch64_2sortedcomb_ctxgid64obj32,
+ * command_ani.c:733-746), built on the GPU; unique_bits: (n + 7) / 8 bytes
+ * or NULL. */
+static kssd_ctxgidobj_t *build_records_bits(const kssd_sketchdir_t *s, uint8_t *unique_bits)
 {
 	const size_t n = s->index[s->n];
-	kssd_ctxgidobj_t *rec = xmalloc(n * sizeof *rec);
-	if (kssd_build_index(s->keys, s->index, s->n, s->obj_bits, threads, rec) != 0)
-		errx(EXIT_FAILURE, "cannot index %d samples with %zu entries (limits: 2^20 samples, 2^32 entries)",
-		     s->n, n);
+	kssd_ctxgidobj_t *rec = xmalloc(n ? n * sizeof *rec : 1);
+	hip_check(kssdhip_build_index(s->keys, s->index, s->n, s->obj_bits, rec, unique_bits),
+		  "cannot build the index on the GPU");
 	return rec;
 }
 
+static kssd_ctxgidobj_t *build_records(const kssd_sketchdir_t *s, int threads)
+{
+	(void)threads;
+	return build_records_bits(s, NULL);
+}
+
 /* gen_inverted_index4comblco (command_sketch.c:916-938): write the index
- * and its unique-marker bits next to the sketch. */
-static void write_index_for(const char *dir, int threads)
+ * and its unique-marker bits next to the sketch; on the GPU, or with the
+ * host builder (host_index). */
+static void write_index_for(const char *dir, int threads, bool host_index, int device)
 {
 	kssd_sketchdir_t s;
 	open_dir(dir, KSSD_SK_KEYS, &s);
-	kssd_ctxgidobj_t *rec = build_records(&s, threads);
-	if (kssd_write_sorted_index(dir, rec, s.index[s.n]) != 0)
+	const size_t n = s.index[s.n];
+	kssd_ctxgidobj_t *rec;
+	uint8_t *bits = NULL;
+	if (host_index) {
+		rec = xmalloc(n ? n * sizeof *rec : 1);
+		if (kssd_build_index(s.keys, s.index, s.n, s.obj_bits, threads, rec) != 0)
+			errx(EXIT_FAILURE, "cannot index %d samples with %zu entries (limits: 2^20 samples, 2^32 entries)",
+			     s.n, n);
+	} else {
+		hip_check(kssdhip_init(device), "no usable GPU (--host-index builds the index on the host)");
+		bits = xmalloc((n + 7) / 8 + 1);
+		rec = build_records_bits(&s, bits);
+	}
+	if (kssd_write_index_files(dir, rec, n, bits) != 0)
 		err(EXIT_FAILURE, "%s/%s", dir, KSSD_SORTED_INDEX_FILE);
+	free(bits);
 	free(rec);
 	kssd_sketchdir_close(&s);
 }
@@ -907,11 +930,11 @@ static int matrix_direct_threshold(void)
 	return (int)env_int(name, 1000, 0, "non-negative integer");
 }
 
-static void ensure_index(const char *dir, int threads)
+static void ensure_index(const char *dir, int threads, int device)
 {
 	if (kssd_file_in_dir(dir, KSSD_SORTED_INDEX_FILE))
 		return;
-	write_index_for(dir, threads);
+	write_index_for(dir, threads, false, device);
 	if (!kssd_file_in_dir(dir, KSSD_SORTED_INDEX_FILE))
 		errx(EXIT_FAILURE, "failed to create reference index '%s/%s'", dir, KSSD_SORTED_INDEX_FILE);
 }
@@ -948,7 +971,7 @@ static int run_ani(ani_args_t *a)
 		if (o->fmt == 1 && indexed) {
 			if (force && !have)
 				force_missing(a->qrydir);
-			ensure_index(a->qrydir, a->p);
+			ensure_index(a->qrydir, a->p, a->device);
 			return run_self_matrix(a, true, true);
 		}
 		if (o->fmt == 2 && indexed) {
@@ -991,7 +1014,7 @@ static int run_ani(ani_args_t *a)
 			return run_index_path(a);
 		}
 		if (matrix_should_use_index(rn, qn, direct_n)) {
-			ensure_index(a->refdir, a->p);
+			ensure_index(a->refdir, a->p, a->device);
 			return run_index_path(a);
 		}
 	} else if (ref_index) {
@@ -999,7 +1022,7 @@ static int run_ani(ani_args_t *a)
 	}
 	const long auto_n = env_int("KSSD3A_AUTO_REF_INDEX_THRESHOLD", 100, 0, "non-negative integer; 0 disables");
 	if (!o->v && !o->unassembled && o->fmt == 0 && auto_n > 0 && rn > auto_n) {
-		write_index_for(a->refdir, a->p);
+		write_index_for(a->refdir, a->p, false, a->device);
 		return run_index_path(a);
 	}
 	if (force)
@@ -1389,7 +1412,7 @@ static int sketch_files(const char *outdir, char **files, int n, int drfold, boo
 
 static int cmd_sketch(int argc, char **argv)
 {
-	enum { S_CONFLICT = 400, S_NOMETA, S_HELP, S_NOT_OFFERED };
+	enum { S_CONFLICT = 400, S_NOMETA, S_HELP, S_NOT_OFFERED, S_HOST_INDEX, S_DEVICE };
 	static const struct option longopts[] = {
 		{"idx", required_argument, NULL, 'i'},
 		{"index", required_argument, NULL, 'i'},
@@ -1400,6 +1423,8 @@ static int cmd_sketch(int argc, char **argv)
 		{"nocomputemeta", no_argument, NULL, S_NOMETA},
 		{"use_coden_ctxobj", no_argument, NULL, 'T'},
 		{"threads", required_argument, NULL, 'p'},
+		{"host-index", no_argument, NULL, S_HOST_INDEX},
+		{"device", required_argument, NULL, S_DEVICE},
 		{"help", no_argument, NULL, S_HELP},
 		{"abundance", no_argument, NULL, S_NOT_OFFERED},
 		{"asone", no_argument, NULL, S_NOT_OFFERED},
@@ -1410,8 +1435,8 @@ static int cmd_sketch(int argc, char **argv)
 		{NULL, 0, NULL, 0},
 	};
 	const char *dir = NULL, *outdir = NULL, *list = NULL;
-	int threads = omp_get_max_threads(), drfold = 8;
-	bool conflict = false, meta = true;
+	int threads = omp_get_max_threads(), drfold = 8, device = 0;
+	bool conflict = false, meta = true, host_index = false;
 	optind = 1;
 	int c;
 	while ((c = getopt_long(argc, argv, "i:o:l:f:Tp:aA", longopts, NULL)) != -1) {
@@ -1439,11 +1464,18 @@ static int cmd_sketch(int argc, char **argv)
 		case 'p':
 			threads = (int)parse_int("-p/--threads", optarg, 1, 65536);
 			break;
+		case S_HOST_INDEX:
+			host_index = true;
+			break;
+		case S_DEVICE:
+			device = (int)parse_int("--device", optarg, 0, 1024);
+			break;
 		case S_HELP:
 			printf("Usage: %s sketch -o OUTDIR [-f DRFOLD] [--conflict] [--nocomputemeta] [-l LIST] FILE...\n"
-			       "       %s sketch -i SKETCH_DIR\n"
+			       "       %s sketch -i SKETCH_DIR [--device N | --host-index [-p THREADS]]\n"
 			       "Sketch FASTA/FASTQ[.gz] files with the coden context-object pattern (one sample per\n"
-			       "file), or build SKETCH_DIR/%s and its .unique_bits.\n",
+			       "file), or build SKETCH_DIR/%s and its .unique_bits (on the GPU;\n"
+			       "--host-index: on the host).\n",
 			       PROG, PROG, KSSD_SORTED_INDEX_FILE);
 			return 0;
 		case 'a':
@@ -1458,7 +1490,7 @@ static int cmd_sketch(int argc, char **argv)
 	if (dir) {
 		if (outdir || optind != argc)
 			usage_error("-i builds the index of an existing sketch; it takes no inputs");
-		write_index_for(dir, threads);
+		write_index_for(dir, threads, host_index, device);
 		stage("index written");
 		return 0;
 	}

@@ -205,6 +205,16 @@ static int count_sorted(const uint32_t *s, size_t n, uint32_t v)
 	return c;
 }
 
+int kssd_write_index_files(const char *dir, const kssd_ctxgidobj_t *rec, size_t n, const uint8_t *unique_bits)
+{
+	if (!unique_bits)
+		return kssd_write_sorted_index(dir, rec, n);
+	int rc = write_file(dir, KSSD_SORTED_INDEX_FILE, rec, n * sizeof *rec);
+	if (rc)
+		return rc;
+	return write_file(dir, KSSD_SORTED_INDEX_FILE ".unique_bits", unique_bits, (n + 7) / 8);
+}
+
 int kssd_write_sorted_index(const char *dir, const kssd_ctxgidobj_t *rec, size_t n)
 {
 	int rc = write_file(dir, KSSD_SORTED_INDEX_FILE, rec, n * sizeof *rec);

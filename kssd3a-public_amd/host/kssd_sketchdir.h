@@ -49,8 +49,10 @@ void kssd_sketchdir_close(kssd_sketchdir_t *s);
 /* lcofiles.stat + comblco + comblco.index */
 int kssd_sketchdir_write(const char *dir, const kssd_stat_t *stat, char (*names)[KSSD_PATHLEN],
 			 const uint64_t *keys, const uint64_t *index);
-/* sortedcomb_ctxgid64obj32 + .unique_bits */
+/* sortedcomb_ctxgid64obj32 + .unique_bits (computed here) */
 int kssd_write_sorted_index(const char *dir, const kssd_ctxgidobj_t *rec, size_t n);
+/* the same with the unique bits given ((n + 7) / 8 bytes; NULL: computed) */
+int kssd_write_index_files(const char *dir, const kssd_ctxgidobj_t *rec, size_t n, const uint8_t *unique_bits);
 
 /* Per-input metadata of a sketched file (sketch_infile_meta_stat_for_path,
  * command_sketch.c:1229-1240 with infile_meta_from_lengths :1134-1174):

@@ -224,6 +224,7 @@ def load_kssdhip():
     lib.kssdhip_init.argtypes = [I]
     lib.kssdhip_index_open.argtypes = [V, S, I, U, ctypes.POINTER(V)]
     lib.kssdhip_index_close.argtypes = [V]
+    lib.kssdhip_build_index.argtypes = [V, V, I, U, V, V]
     lib.kssdhip_index_device_bytes.argtypes = [V]
     lib.kssdhip_index_device_bytes.restype = S
     lib.kssdhip_count_block.argtypes = [V, V, V, I, I, U32, V, V, V, V]
@@ -261,6 +262,19 @@ class KssdHip:
         self.check(self.lib.kssdhip_index_open(_p(records), len(records), ref_gnum, obj_bits,
                                                ctypes.byref(h)))
         return h
+
+    def build_index(self, keys, skidx, obj_bits=20, unique=True):
+        """Sorted index records (structured array ctxgid/obj) and, with
+        `unique`, the unique-marker bitset bytes, built on the device."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        skidx = np.ascontiguousarray(skidx, dtype=np.uint64)
+        n = int(skidx[-1]) if len(skidx) else 0
+        rec = np.empty(n, dtype=REC_DTYPE)
+        bits = np.zeros(max((n + 7) // 8, 1), dtype=np.uint8) if unique else None
+        self.check(self.lib.kssdhip_build_index(_p(keys) if len(keys) else None, _p(skidx), len(skidx) - 1,
+                                                obj_bits, _p(rec) if n else _p(np.zeros(1, np.uint8)),
+                                                _p(bits) if unique else None))
+        return rec, (bits[:(n + 7) // 8].tobytes() if unique else None)
 
     def set_tiling(self, h, mode=None, chunk_records=None, block_threads=None, auto_min_keys=None,
                    group_partitions=None):

@@ -100,21 +100,27 @@ def test_sketch_long_lines_and_gzip(tmp_path):
     assert meta[0:32][:16] == meta[32:64][:16]  # same record lengths summary
 
 
-def test_sketch_index_matches_reference(tmp_path):
+# sketch -i builds the index on the GPU by default; --host-index on the host
+BUILDERS = [pytest.param([], marks=pytest.mark.gpu, id="gpu"), pytest.param(["--host-index"], id="host")]
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_sketch_index_matches_reference(tmp_path, builder):
     d = copy_dir(GEN / "ref", tmp_path / "ref", with_index=False)
-    p = run_cli(["sketch", "-i", d, "-p", "2"])
+    p = run_cli(["sketch", "-i", d, "-p", "2", *builder])
     assert p.returncode == 0, p.stderr
     for f in INDEX_FILES:
         assert (d / f).read_bytes() == (GEN / "ref" / f).read_bytes(), f
 
 
-def test_sketch_index_conflict_sketch(tmp_path):
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_sketch_index_conflict_sketch(tmp_path, builder):
     """Conflict-keeping sketch (object conflicts inside contexts): the index
     and unique bits against the numpy restatement of the format."""
     import numpy as np
     case = COUNT_CASES[-1]
     d = copy_dir(case / "ref", tmp_path / "ref", with_index=False)
-    p = run_cli(["sketch", "-i", d])
+    p = run_cli(["sketch", "-i", d, *builder])
     assert p.returncode == 0, p.stderr
     sk = kt.read_sketch_dir(d)
     want = kt.build_index_np(sk.keys, sk.index, sk.obj_bits)
@@ -212,7 +218,8 @@ def test_cli_conflict_fixtures_match_reference(tmp_path, case):
         assert p.stdout == (case / golden).read_text(), golden
 
 
-def test_sketch_index_unique_bits_large_groups(tmp_path):
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_sketch_index_unique_bits_large_groups(tmp_path, builder):
     """Context groups larger than the small-group scan (many samples share
     one context, some with the same object)."""
     import numpy as np
@@ -223,7 +230,7 @@ def test_sketch_index_unique_bits_large_groups(tmp_path):
         objs = np.array([rng.integers(0, 40), i, rng.integers(0, 4)], np.uint64)
         samples.append(np.sort((ctxs << np.uint64(20)) | objs))
     kt.write_sketch_dir(tmp_path / "big", samples)
-    p = run_cli(["sketch", "-i", tmp_path / "big"])
+    p = run_cli(["sketch", "-i", tmp_path / "big", *builder])
     assert p.returncode == 0, p.stderr
     sk = kt.read_sketch_dir(tmp_path / "big")
     want = kt.build_index_np(sk.keys, sk.index, sk.obj_bits)
