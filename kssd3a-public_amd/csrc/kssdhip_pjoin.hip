@@ -673,10 +673,17 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevB
 	const uint32_t qb = (uint32_t)ja.nq < QB ? (uint32_t)ja.nq : QB;
 	const bool uniq = (ja.flags & KSSDHIP_UNIQUE) && !(ja.flags & KSSDHIP_LOWER);
 	a.slots = ix->group > 1 ? qb * (uniq ? 2 : 1) : 0;
-	// about one staged record per bucket, as far as the LDS allows
+	// about one staged record per bucket, as far as the LDS allows without
+	// fitting fewer workgroups per CU than the smallest table does (a CU has
+	// 160 KB of LDS; the kernel's static LDS is under 256 bytes)
+	auto per_cu = [](size_t bytes) { return (uint32_t)((160u * 1024u) / (bytes + 256)); };
 	a.bbits = 1;
+	while (a.bbits < 10 && (2u << a.bbits) <= a.cap)
+		++a.bbits;
+	const uint32_t wgs = per_cu(PJLayout(a.cap, a.bbits, qb, a.slots).bytes);
 	while (a.bbits < MAX_BUCKET_BITS && (2u << a.bbits) <= a.cap &&
-	       PJLayout(a.cap, a.bbits + 1, qb, a.slots).bytes <= PJ_LDS_MAX)
+	       PJLayout(a.cap, a.bbits + 1, qb, a.slots).bytes <= PJ_LDS_MAX &&
+	       per_cu(PJLayout(a.cap, a.bbits + 1, qb, a.slots).bytes) >= wgs)
 		++a.bbits;
 	const size_t lds = PJLayout(a.cap, a.bbits, qb, a.slots).bytes;
 	if (lds > PJ_LDS_MAX)
