@@ -185,6 +185,20 @@ int kssdhip_count_block_dev(kssdhip_index *index, const uint64_t *d_keys,
 			    kssd_obj_section_t *d_unique_obj, void *stream);
 
 /*
+ * Votes of the sparse-edge nomination (matrix --format edges/clusters on an
+ * indexed sketch): for every query of the block and every genome gid below
+ * the query's own (qry_gid_offset + q), the number of the query's sampled
+ * keys (every sample_step-th, from its first) whose context the genome
+ * carries, counting only contexts with at most max_ctx_freq index records
+ * (0: all).  votes: nq * ref_gnum u32, row-major.
+ * Replaces: the vote loop of pairwise_indexed_self_scan
+ * (pairwise_graph.c:1200-1235).  Same key and offset conventions as
+ * kssdhip_count_block; sample_step >= 1.
+ */
+int kssdhip_count_votes(kssdhip_index *index, const uint64_t *section_sk, const uint64_t *section_skidx,
+			int nq, int qry_gid_offset, uint32_t max_ctx_freq, uint32_t sample_step, uint32_t *votes);
+
+/*
  * Leftmost index record whose context equals each query key's context, or
  * SIZE_MAX.  Replaces kssd_find_first_occurrences_fenceposts
  * (command_ani.c:3703-3738).  a: n sorted keys (host); idx: n results (host).

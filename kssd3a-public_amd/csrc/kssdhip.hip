@@ -61,6 +61,13 @@ namespace kssd {
 
 __device__ __forceinline__ int dna_popcount(uint32_t x) { return dna_popcount_dev(x); }
 
+// out[i] = low 32 bits of in[i] (the shared-context count of a counter cell)
+__global__ void k_low_words(const unsigned long long *__restrict__ in, uint64_t n, uint32_t *out)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		out[i] = (uint32_t)in[i];
+}
+
 __global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v)
 {
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -160,6 +167,8 @@ __global__ __launch_bounds__(256) void k_join(JoinArgs a)
 	     e += (uint64_t)gridDim.x * blockDim.x) {
 		const int q = owner_query(a.skidx, a.nq, e);
 		const uint64_t qb = a.skidx[q], qe = a.skidx[q + 1];
+		if (a.vote_step > 1 && (e - qb) % a.vote_step)
+			continue; // not a sampled key
 		const uint64_t qctx = a.keys[e] >> a.obj_bits;
 		if (e > qb && (a.keys[e - 1] >> a.obj_bits) == qctx)
 			continue;
@@ -171,6 +180,13 @@ __global__ __launch_bounds__(256) void k_join(JoinArgs a)
 		uint64_t d = lookup_ctx(a.ctxgid, a.bucket, a.shift, qctx);
 		if (d == ~0ull)
 			continue;
+		if (a.vote_max_freq) { // skip contexts carried by more index records
+			uint64_t x = d;
+			while (x < a.n && x - d <= a.vote_max_freq && (a.ctxgid[x] >> GID_NBITS) == qctx)
+				++x;
+			if (x - d > a.vote_max_freq)
+				continue;
+		}
 		const int qgid = a.qry_gid_offset + q;
 		int owners = 0, sole_diff = 0;
 		uint32_t sole_gid = 0;
@@ -567,7 +583,8 @@ int kssdhip_index_info(const kssdhip_index *ix, IndexView *v)
 // ---------------------------------------------------------------- counting
 int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
 		   int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
-		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split)
+		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split,
+		   uint32_t vote_max_freq, uint32_t vote_step)
 {
 	if (!ix)
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: null index");
@@ -607,6 +624,8 @@ int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_
 		a.qry_gid_offset = qry_gid_offset;
 		a.obj_bits = ix->obj_bits;
 		a.flags = flags;
+		a.vote_max_freq = vote_max_freq;
+		a.vote_step = vote_step ? vote_step : 1;
 		a.ctx = (unsigned long long *)d_ctx;
 		a.objc = (unsigned long long *)d_obj;
 		a.uctx = (unsigned long long *)d_uctx;
@@ -683,6 +702,51 @@ extern "C" int kssdhip_count_block(kssdhip_index *ix, const uint64_t *section_sk
 		HIP_TRY(hipMemcpyAsync(uctx, ix->uctx.ptr, cells * 8, hipMemcpyDeviceToHost, s));
 		HIP_TRY(hipMemcpyAsync(uobj, ix->uobj.ptr, cells * 8, hipMemcpyDeviceToHost, s));
 	}
+	HIP_TRY(hipStreamSynchronize(s));
+	return 0;
+}
+
+extern "C" int kssdhip_count_votes(kssdhip_index *ix, const uint64_t *section_sk, const uint64_t *section_skidx,
+				   int nq, int qry_gid_offset, uint32_t max_ctx_freq, uint32_t sample_step,
+				   uint32_t *votes)
+{
+	if (!ix)
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_votes: null index");
+	if (nq < 0 || (nq > 0 && (!section_skidx || !votes)) || sample_step < 1)
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_votes: bad arguments");
+	if (nq == 0)
+		return 0;
+	const uint64_t base = section_skidx[0];
+	std::vector<uint64_t> rel((size_t)nq + 1);
+	for (int i = 0; i <= nq; ++i) {
+		if (section_skidx[i] < base || (i && section_skidx[i] < section_skidx[i - 1]))
+			return kssdhip_fail(KSSDHIP_EINVAL, "count_votes: section_skidx not ascending");
+		rel[i] = section_skidx[i] - base;
+	}
+	const uint64_t total = rel[nq];
+	if (total && !section_sk)
+		return kssdhip_fail(KSSDHIP_EINVAL, "count_votes: null keys");
+	const size_t cells = (size_t)nq * (size_t)ix->ref_gnum;
+	HIP_TRY(hipSetDevice(ix->device));
+	int rc;
+	if ((rc = ix->keys.reserve((total ? total : 1) * 8)) || (rc = ix->skidx.reserve(rel.size() * 8)) ||
+	    (rc = ix->ctx.reserve(cells * 8)) || (rc = ix->obj.reserve(cells * 8)))
+		return rc;
+	hipStream_t s = ix->stream;
+	if (total)
+		HIP_TRY(hipMemcpyAsync(ix->keys.ptr, section_sk, total * 8, hipMemcpyHostToDevice, s));
+	HIP_TRY(hipMemcpyAsync(ix->skidx.ptr, rel.data(), rel.size() * 8, hipMemcpyHostToDevice, s));
+	rc = kssd_count_dev(ix, (const uint64_t *)ix->keys.ptr, (const uint64_t *)ix->skidx.ptr, nq, total,
+			    qry_gid_offset, KSSDHIP_LOWER, (kssd_ctx_mut2_t *)ix->ctx.ptr,
+			    (kssd_obj_section_t *)ix->obj.ptr, nullptr, nullptr, s, &ix->split, max_ctx_freq,
+			    sample_step);
+	if (rc)
+		return rc;
+	if (cells)
+		hipLaunchKernelGGL(k_low_words, dim3(grid_for(cells)), dim3(256), 0, s,
+				   (const unsigned long long *)ix->ctx.ptr, cells, (uint32_t *)ix->obj.ptr);
+	HIP_TRY(hipGetLastError());
+	HIP_TRY(hipMemcpyAsync(votes, ix->obj.ptr, cells * 4, hipMemcpyDeviceToHost, s));
 	HIP_TRY(hipStreamSynchronize(s));
 	return 0;
 }

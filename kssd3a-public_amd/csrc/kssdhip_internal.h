@@ -34,6 +34,11 @@ struct JoinArgs {
 	int qry_gid_offset;
 	unsigned obj_bits;
 	unsigned flags;
+	// vote counting (the sparse-edge nomination of pairwise_indexed_self_scan,
+	// pairwise_graph.c:1200-1235): only every vote_step-th key of a query,
+	// and only contexts with at most vote_max_freq index records (0: all)
+	uint32_t vote_max_freq;
+	uint32_t vote_step;
 	unsigned long long *ctx;
 	unsigned long long *objc;
 	unsigned long long *uctx;
@@ -110,7 +115,8 @@ int kssd_launch_pjoin(kssdhip_index *ix, const kssd::JoinArgs &a, hipStream_t s,
 // kssdhip_count_block_dev with an explicit split-table workspace
 int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
 		   int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
-		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split);
+		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split,
+		   uint32_t vote_max_freq = 0, uint32_t vote_step = 1);
 
 // read-only view of an index handle for other translation units
 struct IndexView {

@@ -310,15 +310,21 @@ __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr r
 	uint64_t prev_key = 0, next_key = 0;
 	uint32_t prev_idx = ~0u, next_idx = ~0u;
 	uint32_t q_pre = pre[qi], q_next = pre[qi + 1], q_start = start[qi];
+	const uint32_t step = j.vote_step;
+	uint32_t q_first = step > 1 ? (uint32_t)j.skidx[qb0 + qi] : 0; // the query's first key
 	for (uint32_t f = f_begin; f < f_end; ++f) {
 		while (q_next <= f) {
 			++qi;
 			q_pre = q_next;
 			q_next = pre[qi + 1];
 			q_start = start[qi];
+			if (step > 1)
+				q_first = (uint32_t)j.skidx[qb0 + qi];
 		}
 		const uint32_t kidx = q_start + (f - q_pre);
 		const uint32_t qend = q_start + (q_next - q_pre);
+		if (step > 1 && (kidx - q_first) % step)
+			continue; // not a sampled key
 		const uint64_t key = kidx == next_idx ? next_key : j.keys[kidx];
 		const uint64_t qctx = key >> ob;
 		if (f != q_pre) {
@@ -364,6 +370,13 @@ __device__ __forceinline__ void merge_keys(const PJArgs &a, RecPtr rec, ObjPtr r
 					h2 = mid;
 			}
 			d = lo;
+		}
+		if (j.vote_max_freq) { // skip contexts carried by more index records
+			uint32_t x = d;
+			while (x < cnt && x - d <= j.vote_max_freq && (rec[x] >> GID_NBITS) == qctx)
+				++x;
+			if (x - d > j.vote_max_freq)
+				continue;
 		}
 		const int q = qb0 + qi;
 		const int qgid = j.qry_gid_offset + q;

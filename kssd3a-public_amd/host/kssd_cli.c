@@ -1050,15 +1050,20 @@ static int cmd_ani(int argc, char **argv)
  * and compute_matrix, command_matrix.c:1293-1348): one sketch gives the
  * indexed lower-triangle path (matrix_write_indexed_self_matrix, :439-619)
  * printed as a triangle or a full matrix; -r/-q give the dense report
- * (matrix_compute_dense, :258-315).  One metric per run; the sparse, cluster
- * and dedup-plan reports are not offered.
+ * (matrix_compute_dense, :258-315).  --format edges (--sparse) gives the
+ * sparse edge report (matrix_compute_edges, :992-1014, and the
+ * vote-nominated indexed scan, :1140-1180 with pairwise_graph.c:1148-1316),
+ * with metric expressions.  The cluster and dedup-plan reports are not
+ * offered.
  */
 typedef struct {
 	char refdir[PATH_MAX], qrydir[PATH_MAX], outf[PATH_MAX];
 	int metric;
-	int format; /* 0 auto, 1 full, 2 triangle */
-	double e, diag;
-	bool d;
+	kssd_metric_expr_t expr;
+	int format; /* 0 auto, 1 full, 2 triangle, 3 edges */
+	double e, diag, cut, max_afcut;
+	bool d, cut_set;
+	uint32_t ctxcut, max_ctx_freq, min_votes, sample_step;
 	int p, device;
 } matrix_args_t;
 
@@ -1079,7 +1084,8 @@ static double matrix_cell_value(const void *arg, int q, int r, const kssd_ctx_mu
 
 static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 {
-	enum { M_FORMAT = 300, M_DIAGV, M_DEVICE, M_HELP, M_NOT_OFFERED };
+	enum { M_FORMAT = 300, M_DIAGV, M_DEVICE, M_HELP, M_NOT_OFFERED, M_SPARSE, M_CUT, M_MAXAF, M_CTXCUT,
+	       M_MAXFREQ, M_MINVOTES, M_STEP };
 	static const struct option longopts[] = {
 		{"ref", required_argument, NULL, 'r'},
 		{"query", required_argument, NULL, 'q'},
@@ -1092,10 +1098,18 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		{"threads", required_argument, NULL, 'p'},
 		{"device", required_argument, NULL, M_DEVICE},
 		{"help", no_argument, NULL, M_HELP},
-		{"sparse", no_argument, NULL, M_NOT_OFFERED},
-		{"cut", required_argument, NULL, M_NOT_OFFERED},
-		{"max-afcut", required_argument, NULL, M_NOT_OFFERED},
-		{"ctxcut", required_argument, NULL, M_NOT_OFFERED},
+		{"sparse", no_argument, NULL, M_SPARSE},
+		{"cut", required_argument, NULL, M_CUT},
+		{"max-afcut", required_argument, NULL, M_MAXAF},
+		{"ctxcut", required_argument, NULL, M_CTXCUT},
+		{"index-max-ctx-freq", required_argument, NULL, M_MAXFREQ},
+		{"index-min-votes", required_argument, NULL, M_MINVOTES},
+		{"index-sample-step", required_argument, NULL, M_STEP},
+		{"edge-out", required_argument, NULL, M_NOT_OFFERED},
+		{"keep-out", required_argument, NULL, M_NOT_OFFERED},
+		{"remove-out", required_argument, NULL, M_NOT_OFFERED},
+		{"keep-matrix-out", required_argument, NULL, M_NOT_OFFERED},
+		{"dedup-strategy", required_argument, NULL, M_NOT_OFFERED},
 		{"query-sketch-list", required_argument, NULL, M_NOT_OFFERED},
 		{"matrix-format", required_argument, NULL, M_NOT_OFFERED},
 		{"glist", required_argument, NULL, M_NOT_OFFERED},
@@ -1104,8 +1118,13 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 	};
 	memset(a, 0, sizeof *a);
 	a->metric = KSSD_MM_NAIVE;
+	a->expr = kssd_metric_expr_single(KSSD_MM_NAIVE);
 	a->e = 1.0;
 	a->p = omp_get_max_threads();
+	/* matrix_opt defaults, command_matrix_wrapper.c:119-130 */
+	a->max_ctx_freq = 256;
+	a->min_votes = 1;
+	a->sample_step = 1;
 	optind = 1;
 	int c;
 	while ((c = getopt_long(argc, argv, "r:q:m:e:do:p:g:c:", longopts, NULL)) != -1) {
@@ -1117,23 +1136,22 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 			copy_path(a->qrydir, sizeof a->qrydir, "-q/--query", optarg);
 			break;
 		case 'm':
-			if (strpbrk(optarg, "&|")) {
-				a->metric = -2; /* an expression: rejected below for these formats */
-			} else {
-				a->metric = kssd_matrix_metric(optarg);
-				if (a->metric < 0)
-					usage_error("-m/--metric must be one metric such as ctx-naive, p_dist, mash, or aaf, "
-						    "or an expression like 'ctx-naive&aaf' or 'ctx-moe|mash'");
-			}
+			if (kssd_metric_expr_parse(optarg, &a->expr) != 0)
+				usage_error("-m/--metric must be one metric such as ctx-naive, p_dist, mash, or aaf, "
+					    "or an expression like 'ctx-naive&aaf' or 'ctx-moe|mash'");
+			a->metric = a->expr.count > 1 ? -2 : a->expr.metrics[0];
 			break;
 		case M_FORMAT:
 			if (strcmp(optarg, "full") == 0)
 				a->format = 1;
 			else if (strcmp(optarg, "triangle") == 0)
 				a->format = 2;
-			else if (!strcmp(optarg, "edges") || !strcmp(optarg, "sparse") || !strcmp(optarg, "clusters") ||
-				 !strcmp(optarg, "dedup-plan"))
-				usage_error("--format %s is not offered by this front end (full and triangle are)", optarg);
+			else if (!strcmp(optarg, "edges") || !strcmp(optarg, "edge") || !strcmp(optarg, "sparse"))
+				a->format = 3;
+			else if (!strcmp(optarg, "clusters") || !strcmp(optarg, "cluster") || !strcmp(optarg, "dedup-plan") ||
+				 !strcmp(optarg, "dedup"))
+				usage_error("--format %s is not offered by this front end (full, triangle and edges are)",
+					    optarg);
 			else
 				usage_error("--format must be full, triangle, edges, sparse, clusters, or dedup-plan");
 			break;
@@ -1156,11 +1174,37 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		case M_DEVICE:
 			a->device = (int)parse_int("--device", optarg, 0, 1024);
 			break;
+		case M_SPARSE:
+			a->format = 3;
+			break;
+		case M_CUT:
+			a->cut = parse_double("--cut", optarg, 0.0, 1e300);
+			a->cut_set = true;
+			break;
+		case M_MAXAF:
+			a->max_afcut = parse_double("--max-afcut", optarg, 0.0, 1.0);
+			break;
+		case M_CTXCUT:
+			a->ctxcut = (uint32_t)parse_int("--ctxcut", optarg, 0, INT32_MAX);
+			break;
+		case M_MAXFREQ:
+			a->max_ctx_freq = (uint32_t)parse_int("--index-max-ctx-freq", optarg, 0, INT32_MAX);
+			break;
+		case M_MINVOTES:
+			a->min_votes = (uint32_t)parse_int("--index-min-votes", optarg, 1, INT32_MAX);
+			break;
+		case M_STEP:
+			a->sample_step = (uint32_t)parse_int("--index-sample-step", optarg, 1, INT32_MAX);
+			break;
 		case M_HELP:
 			printf("Usage: %s matrix [-m METRIC] [--format full|triangle] [-e E] [-d | --diagonal-value V] "
 			       "[-o FILE] (SKETCH | -q SKETCH | -r REF -q QRY)\n"
-			       "METRIC: ctx-naive (default), ctx-moe, p_dist, mash, aaf.\n",
-			       PROG);
+			       "       %s matrix --format edges --cut D [-m METRIC|EXPR] [--ctxcut N] [--max-afcut F]\n"
+			       "             [--index-max-ctx-freq N] [--index-min-votes N] [--index-sample-step N]\n"
+			       "             [-o FILE] (SKETCH | -r REF -q QRY)\n"
+			       "METRIC: ctx-naive (default), ctx-moe, p_dist, mash, aaf; EXPR: metrics joined by & or |\n"
+			       "(edges only).\n",
+			       PROG, PROG);
 			exit(0);
 		case 'g':
 		case 'c':
@@ -1180,9 +1224,11 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		usage_error("missing sketch input; provide one sketch argument, -q/--query, or --query-sketch-list");
 	if (a->refdir[0] && a->format == 2)
 		usage_error("--format triangle is only valid for one sketch");
-	if (a->metric == -2)
+	if (a->metric == -2 && a->format != 3)
 		usage_error("combined --metric expressions are only valid for --sparse/--format edges, clusters, or "
 			    "dedup-plan");
+	if (a->format == 3 && !a->cut_set)
+		usage_error("--sparse/--format edges, clusters, or dedup-plan requires --cut");
 }
 
 static int matrix_self(const matrix_args_t *a)
@@ -1295,12 +1341,255 @@ static int matrix_rect(const matrix_args_t *a)
 	return 0;
 }
 
+/* ---- sparse edge reports ---- */
+typedef struct {
+	int q, r;
+	uint64_t order; /* indexed scan: the context where r got its first vote from q */
+	kssd_pair_eval_t ev;
+} edge_t;
+
+typedef struct {
+	edge_t *v;
+	size_t n, cap;
+} edge_list_t;
+
+static void edge_push(edge_list_t *l, const edge_t *e)
+{
+	if (l->n == l->cap) {
+		l->cap = l->cap ? 2 * l->cap : 16;
+		l->v = realloc(l->v, l->cap * sizeof *l->v);
+		if (!l->v)
+			err(EXIT_FAILURE, "edges");
+	}
+	l->v[l->n++] = *e;
+}
+
+static int edge_by_order(const void *pa, const void *pb)
+{
+	const edge_t *a = pa, *b = pb;
+	if (a->order != b->order)
+		return a->order < b->order ? -1 : 1;
+	return (a->r > b->r) - (a->r < b->r);
+}
+
+static int edge_by_ref(const void *pa, const void *pb)
+{
+	const edge_t *a = pa, *b = pb;
+	if (a->r != b->r)
+		return (a->r > b->r) - (a->r < b->r);
+	return (a->q > b->q) - (a->q < b->q);
+}
+
+/* Number of index records of context c (records sorted by ctxgid). */
+static size_t ctx_group_size(const kssd_ctxgidobj_t *rec, size_t n, uint64_t c)
+{
+	size_t lo = 0, hi = n;
+	while (lo < hi) {
+		const size_t mid = lo + (hi - lo) / 2;
+		if ((rec[mid].ctxgid >> 20) < c)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	size_t e = lo;
+	while (e < n && (rec[e].ctxgid >> 20) == c)
+		++e;
+	return e - lo;
+}
+
+/* The context where genome r gets its first vote from query q in the
+ * indexed scan (pairwise_graph.c:1200-1235 walks q's sampled keys in
+ * order and, for each, the context's records in gid order): the first of
+ * q's every step-th key whose context r carries and whose index group has
+ * at most max_freq records (0: any). */
+static uint64_t first_vote_ctx(const kssd_sketchdir_t *s, const kssd_ctxgidobj_t *rec, size_t nrec, int q, int r,
+			       uint32_t max_freq, uint32_t step)
+{
+	const unsigned ob = s->obj_bits;
+	const uint64_t *a = s->keys + s->index[q], *b = s->keys + s->index[r];
+	const size_t na = (size_t)(s->index[q + 1] - s->index[q]), nb = (size_t)(s->index[r + 1] - s->index[r]);
+	size_t j = 0;
+	for (size_t i = 0; i < na; i += step) {
+		const uint64_t c = a[i] >> ob;
+		while (j < nb && (b[j] >> ob) < c)
+			++j;
+		if (j < nb && (b[j] >> ob) == c && (!max_freq || ctx_group_size(rec, nrec, c) <= max_freq))
+			return c;
+	}
+	return UINT64_MAX;
+}
+
+static void put_edges(FILE *fp, const kssd_sketchdir_t *qs, const kssd_sketchdir_t *rs, const edge_t *v, size_t n,
+		      const char *label)
+{
+	kssd_buf_t b = {0};
+	for (size_t i = 0; i < n; ++i) {
+		kssd_put_edge(&b, qs->names[v[i].q], rs->names[v[i].r], &v[i].ev, label);
+		if (b.l > (1u << 20))
+			put_buf(fp, &b);
+	}
+	put_buf(fp, &b);
+	kssd_buf_free(&b);
+}
+
+/* One sketch: the vote-nominated indexed scan when the sketch is indexed
+ * and keeps no conflicts (matrix_can_use_sorted_index, command_matrix.c:
+ * 1045-1053; pairwise_indexed_self_scan, pairwise_graph.c:1148-1316),
+ * else every pair (matrix_compute_edges, command_matrix.c:992-1014).  The
+ * GPU counts the shared-context votes and the pair counts; rows come out in
+ * query order, each row's edges in first-vote order (the reference's order
+ * with one thread; with more its rows interleave). */
+static int matrix_edges_self(const matrix_args_t *a)
+{
+	kssd_sketchdir_t s;
+	const bool have = kssd_file_in_dir(a->qrydir, KSSD_SORTED_INDEX_FILE);
+	open_dir(a->qrydir, KSSD_SK_KEYS | (have ? KSSD_SK_RECORDS : 0), &s);
+	stage("sketch read");
+	const int n = s.n;
+	const bool indexed = have && !s.stat.conflict;
+	uint32_t *tot = entry_or_run_totals(&s, s.stat.conflict, false);
+	kssd_ctxgidobj_t *built = s.records ? NULL : build_records(&s, a->p);
+	const kssd_ctxgidobj_t *rec = s.records ? s.records : built;
+	const size_t nrec = s.index[n];
+	kssdhip_index *ix = n ? gpu_index(rec, nrec, n, s.obj_bits) : NULL;
+	edge_list_t *rows = xcalloc((size_t)(n ? n : 1), sizeof *rows);
+	const int blk = block_queries(n, n, false);
+	kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * (n ? n : 1) * sizeof *ctx);
+	kssd_obj_section_t *obj = xmalloc((size_t)blk * (n ? n : 1) * sizeof *obj);
+	uint32_t *votes = indexed ? xmalloc((size_t)blk * (n ? n : 1) * sizeof *votes) : NULL;
+	for (int q0 = 0; q0 < n; q0 += blk) {
+		const int nb = n - q0 < blk ? n - q0 : blk;
+		hip_check(kssdhip_count_block(ix, s.keys + s.index[q0], s.index + q0, nb, q0, KSSDHIP_LOWER, ctx, obj,
+					      NULL, NULL),
+			  "GPU counting failed");
+		if (indexed)
+			hip_check(kssdhip_count_votes(ix, s.keys + s.index[q0], s.index + q0, nb, q0, a->max_ctx_freq,
+						      a->sample_step, votes),
+				  "GPU vote counting failed");
+#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
+		for (int i = 0; i < nb; ++i) {
+			const int q = q0 + i;
+			const size_t qlen = (size_t)(s.index[q + 1] - s.index[q]);
+			for (int r = 0; r < q; ++r) {
+				const size_t c = (size_t)i * n + r;
+				if (indexed && (votes[c] == 0 || votes[c] < a->min_votes))
+					continue;
+				edge_t e = {q, r, 0, {0}};
+				e.ev = kssd_pair_eval(&a->expr, qlen, (size_t)(s.index[r + 1] - s.index[r]), ctx[c].num_ctx,
+						      obj[c].diff_obj, obj[c].diff_obj_section, ctx[c].num_mut2_ctx, tot[q],
+						      tot[r], s.stat.drfold, s.obj_bits, s.ctx_bits);
+				if (!kssd_pair_passes_edge(&e.ev, a->cut, a->ctxcut, a->max_afcut))
+					continue;
+				if (indexed)
+					e.order = first_vote_ctx(&s, rec, nrec, q, r, a->max_ctx_freq, a->sample_step);
+				edge_push(&rows[q], &e);
+			}
+		}
+	}
+	stage("pairs counted and evaluated");
+	if (ix)
+		kssdhip_index_close(ix);
+	FILE *fp = a->outf[0] ? fopen(a->outf, "w") : stdout;
+	if (!fp)
+		err(errno, "%s", a->outf);
+	kssd_buf_t b = {0};
+	kssd_put_edge_header(&b);
+	put_buf(fp, &b);
+	kssd_buf_free(&b);
+	if (indexed) {
+		for (int q = 0; q < n; ++q) {
+			qsort(rows[q].v, rows[q].n, sizeof *rows[q].v, edge_by_order);
+			put_edges(fp, &s, &s, rows[q].v, rows[q].n, a->expr.label);
+		}
+	} else {
+		edge_list_t all = {0};
+		for (int q = 0; q < n; ++q)
+			for (size_t k = 0; k < rows[q].n; ++k)
+				edge_push(&all, &rows[q].v[k]);
+		if (all.n)
+			qsort(all.v, all.n, sizeof *all.v, edge_by_ref);
+		put_edges(fp, &s, &s, all.v, all.n, a->expr.label);
+		free(all.v);
+	}
+	close_out(fp);
+	stage("formatted");
+	for (int q = 0; q < n; ++q)
+		free(rows[q].v);
+	free(rows);
+	free(votes);
+	free(ctx);
+	free(obj);
+	free(built);
+	free(tot);
+	kssd_sketchdir_close(&s);
+	return 0;
+}
+
+/* -r/-q: every (ref, query) pair, refs outer (matrix_compute_edges,
+ * command_matrix.c:992-1014); with the same sketch on both sides only
+ * queries after the ref. */
+static int matrix_edges_rect(const matrix_args_t *a)
+{
+	kssd_sketchdir_t ref, qry;
+	open_dir(a->refdir, KSSD_SK_KEYS, &ref);
+	open_dir(a->qrydir, KSSD_SK_KEYS, &qry);
+	if (ref.stat.hash_id != qry.stat.hash_id)
+		errx(EXIT_FAILURE, "pairwise_check_compatible(): ref hash_id %u != qry %u", ref.stat.hash_id,
+		     qry.stat.hash_id);
+	stage("sketches read");
+	const int R = ref.n, Q = qry.n;
+	const bool same = strcmp(a->refdir, a->qrydir) == 0;
+	uint32_t *qc = entry_or_run_totals(&qry, qry.stat.conflict, false);
+	uint32_t *rc = entry_or_run_totals(&ref, ref.stat.conflict, false);
+	kssd_ctx_mut2_t *ctx = xcalloc((size_t)Q * R + 1, sizeof *ctx);
+	kssd_obj_section_t *obj = xcalloc((size_t)Q * R + 1, sizeof *obj);
+	rect_counts(&ref, &qry, false, a->p, ctx, obj);
+	stage("pairs counted");
+	edge_list_t *rows = xcalloc((size_t)(R ? R : 1), sizeof *rows);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
+	for (int r = 0; r < R; ++r) {
+		const size_t rlen = (size_t)(ref.index[r + 1] - ref.index[r]);
+		for (int q = same ? r + 1 : 0; q < Q; ++q) {
+			const size_t c = (size_t)q * R + r;
+			edge_t e = {q, r, 0, {0}};
+			e.ev = kssd_pair_eval(&a->expr, (size_t)(qry.index[q + 1] - qry.index[q]), rlen, ctx[c].num_ctx,
+					      obj[c].diff_obj, obj[c].diff_obj_section, ctx[c].num_mut2_ctx, qc[q], rc[r],
+					      ref.stat.drfold, ref.obj_bits, ref.ctx_bits);
+			if (kssd_pair_passes_edge(&e.ev, a->cut, a->ctxcut, a->max_afcut))
+				edge_push(&rows[r], &e);
+		}
+	}
+	FILE *fp = a->outf[0] ? fopen(a->outf, "w") : stdout;
+	if (!fp)
+		err(errno, "%s", a->outf);
+	kssd_buf_t b = {0};
+	kssd_put_edge_header(&b);
+	put_buf(fp, &b);
+	kssd_buf_free(&b);
+	for (int r = 0; r < R; ++r) {
+		put_edges(fp, &qry, &ref, rows[r].v, rows[r].n, a->expr.label);
+		free(rows[r].v);
+	}
+	close_out(fp);
+	stage("formatted");
+	free(rows);
+	free(ctx);
+	free(obj);
+	free(qc);
+	free(rc);
+	kssd_sketchdir_close(&ref);
+	kssd_sketchdir_close(&qry);
+	return 0;
+}
+
 static int cmd_matrix(int argc, char **argv)
 {
 	matrix_args_t a;
 	parse_matrix(argc, argv, &a);
 	hip_check(kssdhip_init(a.device), "no usable GPU");
 	stage("GPU ready");
+	if (a.format == 3)
+		return a.refdir[0] ? matrix_edges_rect(&a) : matrix_edges_self(&a);
 	if (a.refdir[0]) {
 		if (a.format == 0)
 			a.format = 1;

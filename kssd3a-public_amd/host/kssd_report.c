@@ -960,6 +960,179 @@ double kssd_pairwise_value(int metric, double e, size_t qry_len, size_t ref_len,
 	}
 }
 
+const char *kssd_metric_name(int metric)
+{
+	switch (metric) {
+	case KSSD_MM_MOE:
+		return "ctx-moe";
+	case KSSD_MM_NAIVE:
+		return "ctx-naive";
+	case KSSD_MM_PDIST:
+		return "p_dist";
+	case KSSD_MM_MASH:
+		return "mash";
+	case KSSD_MM_AAF:
+		return "aaf";
+	}
+	return "ctx-moe";
+}
+
+static int expr_label(kssd_metric_expr_t *e)
+{
+	const char *sep = e->op == KSSD_EXPR_AND ? "&" : e->op == KSSD_EXPR_OR ? "|" : "";
+	size_t used = 0;
+	e->label[0] = '\0';
+	for (int i = 0; i < e->count; ++i) {
+		const int w = snprintf(e->label + used, sizeof e->label - used, "%s%s", i ? sep : "",
+				       kssd_metric_name(e->metrics[i]));
+		if (w < 0 || (size_t)w >= sizeof e->label - used)
+			return -1;
+		used += (size_t)w;
+	}
+	return 0;
+}
+
+kssd_metric_expr_t kssd_metric_expr_single(int metric)
+{
+	kssd_metric_expr_t e;
+	memset(&e, 0, sizeof e);
+	e.metrics[0] = metric;
+	e.count = 1;
+	e.op = KSSD_EXPR_SINGLE;
+	(void)expr_label(&e);
+	return e;
+}
+
+int kssd_metric_expr_parse(const char *arg, kssd_metric_expr_t *out)
+{
+	/* whitespace dropped; one kind of operator; no leading, trailing or
+	 * doubled operator; every term a metric name */
+	if (!arg || !out || !arg[0])
+		return -1;
+	char compact[96];
+	size_t n = 0;
+	for (const char *p = arg; *p; ++p) {
+		if (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')
+			continue;
+		if (n + 1 >= sizeof compact)
+			return -1;
+		compact[n++] = *p;
+	}
+	compact[n] = '\0';
+	if (!n)
+		return -1;
+	const int has_and = strchr(compact, '&') != NULL, has_or = strchr(compact, '|') != NULL;
+	if (has_and && has_or)
+		return -1;
+	kssd_metric_expr_t e;
+	memset(&e, 0, sizeof e);
+	e.op = has_and ? KSSD_EXPR_AND : has_or ? KSSD_EXPR_OR : KSSD_EXPR_SINGLE;
+	if (e.op != KSSD_EXPR_SINGLE) {
+		const char op = has_and ? '&' : '|';
+		if (compact[0] == op || compact[n - 1] == op)
+			return -1;
+		for (size_t i = 1; i < n; ++i)
+			if (compact[i] == op && compact[i - 1] == op)
+				return -1;
+	}
+	const char delim[2] = {has_and ? '&' : has_or ? '|' : '\0', '\0'};
+	char *save = NULL;
+	char *tok = e.op == KSSD_EXPR_SINGLE ? compact : strtok_r(compact, delim, &save);
+	while (tok) {
+		if (e.count >= KSSD_EXPR_MAX)
+			return -1;
+		const int m = kssd_matrix_metric(tok);
+		if (m < 0)
+			return -1;
+		e.metrics[e.count++] = m;
+		tok = e.op == KSSD_EXPR_SINGLE ? NULL : strtok_r(NULL, delim, &save);
+	}
+	if (!e.count)
+		return -1;
+	if (e.count == 1)
+		e.op = KSSD_EXPR_SINGLE;
+	if (expr_label(&e))
+		return -1;
+	*out = e;
+	return 0;
+}
+
+static kssd_pair_eval_t pair_eval_one(int metric, size_t qry_len, size_t ref_len, uint32_t xny, uint32_t dobj,
+				      uint32_t dsec, uint32_t mut2, uint32_t qc, uint32_t rc, int drfold,
+				      unsigned obj_bits, unsigned ctx_bits)
+{
+	/* pairwise_eval_arrays, pairwise_graph.c:301-360 */
+	kssd_pair_eval_t ev;
+	memset(&ev, 0, sizeof ev);
+	ev.distance = 1.0;
+	if (qry_len == 0 || ref_len == 0)
+		return ev;
+	ev.xny = xny;
+	ev.dobj = dobj;
+	ev.dsec = dsec;
+	ev.mut2 = mut2;
+	ev.af_qry = qc > 0 ? (double)xny / (double)qc : 0.0;
+	ev.af_ref = rc > 0 ? (double)xny / (double)rc : 0.0;
+	ev.max_af = ev.af_qry > ev.af_ref ? ev.af_qry : ev.af_ref;
+	if (xny == 0)
+		return ev;
+	ev.distance = kssd_pairwise_value(metric, 1.0, qry_len, ref_len, xny, dobj, dsec, mut2, qc, rc, drfold,
+					  obj_bits, ctx_bits);
+	ev.similarity = 1.0 - ev.distance;
+	ev.valid = 1;
+	return ev;
+}
+
+kssd_pair_eval_t kssd_pair_eval(const kssd_metric_expr_t *e, size_t qry_len, size_t ref_len, uint32_t xny,
+				uint32_t dobj, uint32_t dsec, uint32_t mut2, uint32_t qc, uint32_t rc, int drfold,
+				unsigned obj_bits, unsigned ctx_bits)
+{
+	if (!e || e->count == 0)
+		return pair_eval_one(KSSD_MM_MOE, qry_len, ref_len, xny, dobj, dsec, mut2, qc, rc, drfold, obj_bits,
+				     ctx_bits);
+	if (e->count == 1 || e->op == KSSD_EXPR_SINGLE)
+		return pair_eval_one(e->metrics[0], qry_len, ref_len, xny, dobj, dsec, mut2, qc, rc, drfold, obj_bits,
+				     ctx_bits);
+	/* pairwise_eval_expr_merge, pairwise_graph.c:420-448 */
+	kssd_pair_eval_t merged;
+	memset(&merged, 0, sizeof merged);
+	merged.distance = 1.0;
+	for (int i = 0; i < e->count; ++i) {
+		const kssd_pair_eval_t ev = pair_eval_one(e->metrics[i], qry_len, ref_len, xny, dobj, dsec, mut2, qc,
+							  rc, drfold, obj_bits, ctx_bits);
+		if (e->op == KSSD_EXPR_AND) {
+			if (!ev.valid)
+				return merged;
+			if (!merged.valid || ev.distance > merged.distance)
+				merged = ev;
+		} else if (ev.valid && (!merged.valid || ev.distance < merged.distance)) {
+			merged = ev;
+		}
+	}
+	if (merged.valid)
+		merged.similarity = 1.0 - merged.distance;
+	return merged;
+}
+
+int kssd_pair_passes_edge(const kssd_pair_eval_t *ev, double cut, uint32_t ctxcut, double max_afcut)
+{
+	return ev && ev->valid && ev->distance < cut && ev->xny >= ctxcut && ev->max_af >= max_afcut;
+}
+
+void kssd_put_edge_header(kssd_buf_t *b)
+{
+	kssd_buf_printf(b, "Qry\tRef\tDistance\tSimilarity\tMetric\tXnY_ctx\tQry_align_fraction\tRef_align_fraction\t"
+			   "max_AF\tN_diff_obj\tN_diff_obj_section\tN_mut2_ctx\n");
+}
+
+void kssd_put_edge(kssd_buf_t *b, const char *qry_name, const char *ref_name, const kssd_pair_eval_t *ev,
+		   const char *label)
+{
+	kssd_buf_printf(b, "%s\t%s\t%.12g\t%.12g\t%s\t%u\t%.12g\t%.12g\t%.12g\t%u\t%u\t%u\n", qry_name, ref_name,
+			ev->distance, ev->similarity, label, ev->xny, ev->af_qry, ev->af_ref, ev->max_af, ev->dobj,
+			ev->dsec, ev->mut2);
+}
+
 static void put_frac6(kssd_buf_t *b, uint32_t frac)
 {
 	char f[7];

@@ -169,6 +169,42 @@ double kssd_matrix_context_value(int metric, double e, uint32_t xny, uint32_t do
 double kssd_pairwise_value(int metric, double e, size_t qry_len, size_t ref_len, uint32_t xny,
 			   uint32_t dobj, uint32_t dsec, uint32_t mut2, uint32_t qry_ctx, uint32_t ref_ctx,
 			   int drfold, unsigned obj_bits, unsigned ctx_bits);
+/* ---- sparse edge reports (matrix --format edges) ---- */
+/* pairwise_eval_t (pairwise_graph.h): one pair's evaluation */
+typedef struct {
+	double distance, similarity, af_qry, af_ref, max_af;
+	uint32_t xny, dobj, dsec, mut2;
+	int valid;
+} kssd_pair_eval_t;
+#define KSSD_EXPR_MAX 4
+enum { KSSD_EXPR_SINGLE = 0, KSSD_EXPR_AND, KSSD_EXPR_OR };
+/* pairwise_metric_expr_t: up to 4 metrics joined by & (all valid, the
+ * largest distance) or | (the smallest valid distance), with its label */
+typedef struct {
+	int metrics[KSSD_EXPR_MAX];
+	int count, op;
+	char label[96];
+} kssd_metric_expr_t;
+/* pairwise_metric_name (pairwise_graph.c:52-67) */
+const char *kssd_metric_name(int metric);
+/* pairwise_metric_expr_single / pairwise_metric_expr_from_string
+ * (pairwise_graph.c:153-234); the latter returns 0 on success, -1 when
+ * the text is not an expression. */
+kssd_metric_expr_t kssd_metric_expr_single(int metric);
+int kssd_metric_expr_parse(const char *text, kssd_metric_expr_t *out);
+/* pairwise_eval_expr_samples for lco sketches (pairwise_graph.c:301-360,
+ * :420-513) from the pair's counts: qry_len/ref_len the samples' entries,
+ * qry_ctx/ref_ctx their context totals. */
+kssd_pair_eval_t kssd_pair_eval(const kssd_metric_expr_t *e, size_t qry_len, size_t ref_len, uint32_t xny,
+				uint32_t dobj, uint32_t dsec, uint32_t mut2, uint32_t qry_ctx, uint32_t ref_ctx,
+				int drfold, unsigned obj_bits, unsigned ctx_bits);
+/* pairwise_eval_passes_edge (pairwise_graph.c:518-530) */
+int kssd_pair_passes_edge(const kssd_pair_eval_t *ev, double cut, uint32_t ctxcut, double max_afcut);
+/* matrix_print_edge_header / matrix_print_edge (command_matrix.c:87-101) */
+void kssd_put_edge_header(kssd_buf_t *b);
+void kssd_put_edge(kssd_buf_t *b, const char *qry_name, const char *ref_name, const kssd_pair_eval_t *ev,
+		   const char *label);
+
 /* matrix_kput_fixed6 / matrix_scale_fixed6 / matrix_kput_scaled6
  * (command_matrix.c:103-149) */
 void kssd_matrix_put_fixed6(kssd_buf_t *b, double v);

@@ -230,6 +230,7 @@ def load_kssdhip():
     lib.kssdhip_count_block.argtypes = [V, V, V, I, I, U32, V, V, V, V]
     lib.kssdhip_count_block_dev.argtypes = [V, V, V, I, U64, I, U32, V, V, V, V, V]
     lib.kssdhip_find_first.argtypes = [V, V, S, V]
+    lib.kssdhip_count_votes.argtypes = [V, V, V, I, I, U32, U32, V]
     lib.kssdhip_pair_features.argtypes = [V, S, V, S, U, I, ctypes.POINTER(Features)]
     lib.kssdhip_pair_workspace_init_dev.argtypes = [V, V]
     lib.kssdhip_pair_features_dev.argtypes = [V, S, V, S, U, I, V, V, V]
@@ -262,6 +263,15 @@ class KssdHip:
         self.check(self.lib.kssdhip_index_open(_p(records), len(records), ref_gnum, obj_bits,
                                                ctypes.byref(h)))
         return h
+
+    def count_votes(self, h, ref_gnum, keys, skidx_abs, qry_gid_offset=0, max_ctx_freq=0, sample_step=1):
+        nq = len(skidx_abs) - 1
+        votes = np.zeros((nq, ref_gnum), dtype=np.uint32)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        skidx_abs = np.ascontiguousarray(skidx_abs, dtype=np.uint64)
+        self.check(self.lib.kssdhip_count_votes(h, _p(keys), _p(skidx_abs), nq, qry_gid_offset, max_ctx_freq,
+                                                sample_step, _p(votes)))
+        return votes
 
     def build_index(self, keys, skidx, obj_bits=20, unique=True):
         """Sorted index records (structured array ctxgid/obj) and, with

@@ -226,6 +226,43 @@ def make_matrix_cmd():
     w("matrix_self_rq.tsv", run(["matrix", "-r", S, "-q", S, "--diagonal-value", "0.5"]))
 
 
+def make_matrix_edges():
+    """`matrix --format edges` (sparse edge reports) with one thread: the
+    vote-nominated indexed scan on the indexed genome sketch (metrics,
+    expressions, cuts and the index vote options), every pair on the same
+    sketch without its index and on a conflict sketch, and -r/-q."""
+    out = HERE / "genomes"
+    R, Q, S = out / "ref", out / "qry", out / "all"
+    w = lambda name, text: (out / name).write_text(text)  # noqa: E731
+    cases = {
+        "edges_default.tsv": ["--cut", "0.06"],
+        "edges_mash.tsv": ["--cut", "0.2", "-m", "mash"],
+        "edges_and.tsv": ["--cut", "0.06", "-m", "ctx-naive&aaf"],
+        "edges_or.tsv": ["--cut", "0.05", "-m", "ctx-moe|mash"],
+        "edges_cuts.tsv": ["--cut", "0.06", "--ctxcut", "150", "--max-afcut", "0.35"],
+        "edges_maxfreq.tsv": ["--cut", "0.06", "--index-max-ctx-freq", "3"],
+        "edges_votes.tsv": ["--cut", "0.07", "--index-min-votes", "150", "-m", "p_dist"],
+        "edges_step.tsv": ["--cut", "0.06", "--index-sample-step", "3", "--index-max-ctx-freq", "5"],
+    }
+    for name, args in cases.items():
+        w(name, run(["matrix", "--format", "edges", "-p", "1", *args, S]))
+    # the same sketch without its index: every pair
+    tmp = HERE / "_noindex"
+    shutil.rmtree(tmp, ignore_errors=True)
+    shutil.copytree(S, tmp)
+    for f in ("sortedcomb_ctxgid64obj32", "sortedcomb_ctxgid64obj32.unique_bits"):
+        (tmp / f).unlink(missing_ok=True)
+    w("edges_noindex.tsv", run(["matrix", "--sparse", "-p", "1", "--cut", "0.06", tmp]))
+    w("edges_noindex_aaf.tsv", run(["matrix", "--sparse", "-p", "1", "--cut", "0.3", "-m", "aaf|p_dist", tmp]))
+    shutil.rmtree(tmp)
+    w("edges_rq.tsv", run(["matrix", "--format", "edges", "-p", "1", "--cut", "0.06", "-r", R, "-q", Q]))
+    w("edges_rq_same.tsv", run(["matrix", "--format", "edges", "-p", "1", "--cut", "0.05", "-r", S, "-q", S]))
+    # a conflict sketch (indexed, but conflicts rule the indexed scan out)
+    c = HERE / "counts" / "case_12"
+    (c / "edges_conflict.tsv").write_text(run(["matrix", "--format", "edges", "-p", "1", "--cut", "0.9",
+                                               "-m", "p_dist", c / "ref"]))
+
+
 def make_sketch():
     """Small FASTA inputs (multi-record, lowercase, non-ACGT runs) and the
     reference's sketches of them, at the default -f8 and at -f4."""
@@ -254,11 +291,16 @@ def make_sketch():
 def main():
     if not BIN.exists():
         raise SystemExit(f"{BIN} missing: run `make -C oracle ref` first")
+    if len(sys.argv) > 1:  # named generators only, e.g. `make_golden.py matrix_edges`
+        for name in sys.argv[1:]:
+            globals()[f"make_{name}"]()
+        return
     make_counts()
     make_genomes()
     make_stream()
     make_metrics()
     make_matrix_cmd()
+    make_matrix_edges()
     make_sketch()
     print("golden fixtures written under", HERE)
 

@@ -266,11 +266,56 @@ def test_cli_matrix_command_matches_reference(tmp_path, golden, args):
     assert p.stdout == (GEN / golden).read_text()
 
 
+EDGE = ["--format", "edges"]
+EDGE_CASES = [  # (golden, args, the sketch keeps its index)
+    ("edges_default.tsv", [*EDGE, "--cut", "0.06", "S"], True),
+    ("edges_mash.tsv", [*EDGE, "--cut", "0.2", "-m", "mash", "S"], True),
+    ("edges_and.tsv", [*EDGE, "--cut", "0.06", "-m", "ctx-naive&aaf", "S"], True),
+    ("edges_or.tsv", [*EDGE, "--cut", "0.05", "-m", "ctx-moe|mash", "S"], True),
+    ("edges_cuts.tsv", [*EDGE, "--cut", "0.06", "--ctxcut", "150", "--max-afcut", "0.35", "S"], True),
+    ("edges_maxfreq.tsv", [*EDGE, "--cut", "0.06", "--index-max-ctx-freq", "3", "S"], True),
+    ("edges_votes.tsv", [*EDGE, "--cut", "0.07", "--index-min-votes", "150", "-m", "p_dist", "S"], True),
+    ("edges_step.tsv", [*EDGE, "--cut", "0.06", "--index-sample-step", "3", "--index-max-ctx-freq", "5", "S"], True),
+    ("edges_noindex.tsv", ["--sparse", "--cut", "0.06", "S"], False),
+    ("edges_noindex_aaf.tsv", ["--sparse", "--cut", "0.3", "-m", "aaf|p_dist", "S"], False),
+    ("edges_rq.tsv", [*EDGE, "--cut", "0.06", "-r", "R", "-q", "Q"], False),
+    ("edges_rq_same.tsv", [*EDGE, "--cut", "0.05", "-r", "S", "-q", "S"], True),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("golden,args,indexed", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
+def test_cli_matrix_edges_match_reference(tmp_path, golden, args, indexed):
+    """Sparse edge reports (the reference ran them with -p 1): the
+    vote-nominated indexed scan on the indexed sketch, every pair without
+    the index, and -r/-q."""
+    dirs = genome_dirs(tmp_path, True)
+    if indexed:
+        for f in INDEX_FILES:
+            shutil.copy(GEN / "all" / f, dirs["S"] / f)
+    p = run_cli(["matrix", "-p", "4", *[str(dirs.get(x, x)) for x in args]])
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == (GEN / golden).read_text()
+
+
+@pytest.mark.gpu
+def test_cli_matrix_edges_conflict_sketch(tmp_path):
+    """A conflict sketch with an index still takes the every-pair path."""
+    case = COUNT_CASES[-1]
+    d = copy_dir(case / "ref", tmp_path / "ref")
+    p = run_cli(["matrix", "--format", "edges", "--cut", "0.9", "-m", "p_dist", d])
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == (case / "edges_conflict.tsv").read_text()
+
+
 def test_matrix_usage_errors():
     for args, msg in ((["matrix", "-r", "x", "--format", "triangle", "y"], "only valid for one sketch"),
                       (["matrix", "-m", "aaf&mash", "x"], "combined --metric"),
+                      (["matrix", "-m", "aaf&mash|p_dist", "--sparse", "--cut", "1", "x"], "-m/--metric must be"),
                       (["matrix", "-m", "bogus", "x"], "-m/--metric must be"),
-                      (["matrix", "--sparse", "x"], "not offered"),
+                      (["matrix", "--sparse", "x"], "requires --cut"),
+                      (["matrix", "--format", "clusters", "--cut", "0.1", "x"], "not offered"),
+                      (["matrix", "--edge-out", "e.tsv", "x"], "not offered"),
                       (["matrix"], "missing sketch input")):
         p = run_cli(args)
         assert p.returncode == 64 and msg in p.stderr, (args, p.stderr)
