@@ -1053,14 +1053,15 @@ static int cmd_ani(int argc, char **argv)
  * (matrix_compute_dense, :258-315).  --format edges (--sparse) gives the
  * sparse edge report (matrix_compute_edges, :992-1014, and the
  * vote-nominated indexed scan, :1140-1180 with pairwise_graph.c:1148-1316),
- * with metric expressions.  The cluster and dedup-plan reports are not
- * offered.
+ * with metric expressions; --format clusters gives the connected components
+ * of those edges with their best representative (matrix_write_component_report,
+ * :1055-1118).  The dedup-plan report is not offered.
  */
 typedef struct {
-	char refdir[PATH_MAX], qrydir[PATH_MAX], outf[PATH_MAX];
+	char refdir[PATH_MAX], qrydir[PATH_MAX], outf[PATH_MAX], edge_outf[PATH_MAX];
 	int metric;
 	kssd_metric_expr_t expr;
-	int format; /* 0 auto, 1 full, 2 triangle, 3 edges */
+	int format; /* 0 auto, 1 full, 2 triangle, 3 edges, 4 clusters */
 	double e, diag, cut, max_afcut;
 	bool d, cut_set;
 	uint32_t ctxcut, max_ctx_freq, min_votes, sample_step;
@@ -1085,7 +1086,7 @@ static double matrix_cell_value(const void *arg, int q, int r, const kssd_ctx_mu
 static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 {
 	enum { M_FORMAT = 300, M_DIAGV, M_DEVICE, M_HELP, M_NOT_OFFERED, M_SPARSE, M_CUT, M_MAXAF, M_CTXCUT,
-	       M_MAXFREQ, M_MINVOTES, M_STEP };
+	       M_MAXFREQ, M_MINVOTES, M_STEP, M_EDGEOUT };
 	static const struct option longopts[] = {
 		{"ref", required_argument, NULL, 'r'},
 		{"query", required_argument, NULL, 'q'},
@@ -1105,7 +1106,7 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		{"index-max-ctx-freq", required_argument, NULL, M_MAXFREQ},
 		{"index-min-votes", required_argument, NULL, M_MINVOTES},
 		{"index-sample-step", required_argument, NULL, M_STEP},
-		{"edge-out", required_argument, NULL, M_NOT_OFFERED},
+		{"edge-out", required_argument, NULL, M_EDGEOUT},
 		{"keep-out", required_argument, NULL, M_NOT_OFFERED},
 		{"remove-out", required_argument, NULL, M_NOT_OFFERED},
 		{"keep-matrix-out", required_argument, NULL, M_NOT_OFFERED},
@@ -1148,9 +1149,11 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 				a->format = 2;
 			else if (!strcmp(optarg, "edges") || !strcmp(optarg, "edge") || !strcmp(optarg, "sparse"))
 				a->format = 3;
-			else if (!strcmp(optarg, "clusters") || !strcmp(optarg, "cluster") || !strcmp(optarg, "dedup-plan") ||
-				 !strcmp(optarg, "dedup"))
-				usage_error("--format %s is not offered by this front end (full, triangle and edges are)",
+			else if (!strcmp(optarg, "clusters") || !strcmp(optarg, "cluster") || !strcmp(optarg, "components"))
+				a->format = 4;
+			else if (!strcmp(optarg, "dedup-plan") || !strcmp(optarg, "dedup"))
+				usage_error("--format %s is not offered by this front end (full, triangle, edges and clusters "
+					    "are)",
 					    optarg);
 			else
 				usage_error("--format must be full, triangle, edges, sparse, clusters, or dedup-plan");
@@ -1196,12 +1199,15 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		case M_STEP:
 			a->sample_step = (uint32_t)parse_int("--index-sample-step", optarg, 1, INT32_MAX);
 			break;
+		case M_EDGEOUT:
+			copy_path(a->edge_outf, sizeof a->edge_outf, "--edge-out", optarg);
+			break;
 		case M_HELP:
 			printf("Usage: %s matrix [-m METRIC] [--format full|triangle] [-e E] [-d | --diagonal-value V] "
 			       "[-o FILE] (SKETCH | -q SKETCH | -r REF -q QRY)\n"
-			       "       %s matrix --format edges --cut D [-m METRIC|EXPR] [--ctxcut N] [--max-afcut F]\n"
-			       "             [--index-max-ctx-freq N] [--index-min-votes N] [--index-sample-step N]\n"
-			       "             [-o FILE] (SKETCH | -r REF -q QRY)\n"
+			       "       %s matrix --format edges|clusters --cut D [-m METRIC|EXPR] [--ctxcut N]\n"
+			       "             [--max-afcut F] [--index-max-ctx-freq N] [--index-min-votes N]\n"
+			       "             [--index-sample-step N] [--edge-out FILE] [-o FILE] (SKETCH | -r REF -q QRY)\n"
 			       "METRIC: ctx-naive (default), ctx-moe, p_dist, mash, aaf; EXPR: metrics joined by & or |\n"
 			       "(edges only).\n",
 			       PROG, PROG);
@@ -1224,11 +1230,13 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		usage_error("missing sketch input; provide one sketch argument, -q/--query, or --query-sketch-list");
 	if (a->refdir[0] && a->format == 2)
 		usage_error("--format triangle is only valid for one sketch");
-	if (a->metric == -2 && a->format != 3)
+	if (a->metric == -2 && a->format != 3 && a->format != 4)
 		usage_error("combined --metric expressions are only valid for --sparse/--format edges, clusters, or "
 			    "dedup-plan");
-	if (a->format == 3 && !a->cut_set)
+	if ((a->format == 3 || a->format == 4) && !a->cut_set)
 		usage_error("--sparse/--format edges, clusters, or dedup-plan requires --cut");
+	if (a->format == 4 && a->refdir[0])
+		usage_error("--format clusters/dedup-plan requires one sketch, not -r/-q");
 }
 
 static int matrix_self(const matrix_args_t *a)
@@ -1439,19 +1447,14 @@ static void put_edges(FILE *fp, const kssd_sketchdir_t *qs, const kssd_sketchdir
  * GPU counts the shared-context votes and the pair counts; rows come out in
  * query order, each row's edges in first-vote order (the reference's order
  * with one thread; with more its rows interleave). */
-static int matrix_edges_self(const matrix_args_t *a)
+static edge_list_t *self_edges(const matrix_args_t *a, const kssd_sketchdir_t *s, const uint32_t *tot,
+				bool indexed)
 {
-	kssd_sketchdir_t s;
-	const bool have = kssd_file_in_dir(a->qrydir, KSSD_SORTED_INDEX_FILE);
-	open_dir(a->qrydir, KSSD_SK_KEYS | (have ? KSSD_SK_RECORDS : 0), &s);
-	stage("sketch read");
-	const int n = s.n;
-	const bool indexed = have && !s.stat.conflict;
-	uint32_t *tot = entry_or_run_totals(&s, s.stat.conflict, false);
-	kssd_ctxgidobj_t *built = s.records ? NULL : build_records(&s, a->p);
-	const kssd_ctxgidobj_t *rec = s.records ? s.records : built;
-	const size_t nrec = s.index[n];
-	kssdhip_index *ix = n ? gpu_index(rec, nrec, n, s.obj_bits) : NULL;
+	const int n = s->n;
+	kssd_ctxgidobj_t *built = s->records ? NULL : build_records(s, a->p);
+	const kssd_ctxgidobj_t *rec = s->records ? s->records : built;
+	const size_t nrec = s->index[n];
+	kssdhip_index *ix = n ? gpu_index(rec, nrec, n, s->obj_bits) : NULL;
 	edge_list_t *rows = xcalloc((size_t)(n ? n : 1), sizeof *rows);
 	const int blk = block_queries(n, n, false);
 	kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * (n ? n : 1) * sizeof *ctx);
@@ -1459,29 +1462,29 @@ static int matrix_edges_self(const matrix_args_t *a)
 	uint32_t *votes = indexed ? xmalloc((size_t)blk * (n ? n : 1) * sizeof *votes) : NULL;
 	for (int q0 = 0; q0 < n; q0 += blk) {
 		const int nb = n - q0 < blk ? n - q0 : blk;
-		hip_check(kssdhip_count_block(ix, s.keys + s.index[q0], s.index + q0, nb, q0, KSSDHIP_LOWER, ctx, obj,
+		hip_check(kssdhip_count_block(ix, s->keys + s->index[q0], s->index + q0, nb, q0, KSSDHIP_LOWER, ctx, obj,
 					      NULL, NULL),
 			  "GPU counting failed");
 		if (indexed)
-			hip_check(kssdhip_count_votes(ix, s.keys + s.index[q0], s.index + q0, nb, q0, a->max_ctx_freq,
+			hip_check(kssdhip_count_votes(ix, s->keys + s->index[q0], s->index + q0, nb, q0, a->max_ctx_freq,
 						      a->sample_step, votes),
 				  "GPU vote counting failed");
 #pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
 		for (int i = 0; i < nb; ++i) {
 			const int q = q0 + i;
-			const size_t qlen = (size_t)(s.index[q + 1] - s.index[q]);
+			const size_t qlen = (size_t)(s->index[q + 1] - s->index[q]);
 			for (int r = 0; r < q; ++r) {
 				const size_t c = (size_t)i * n + r;
 				if (indexed && (votes[c] == 0 || votes[c] < a->min_votes))
 					continue;
 				edge_t e = {q, r, 0, {0}};
-				e.ev = kssd_pair_eval(&a->expr, qlen, (size_t)(s.index[r + 1] - s.index[r]), ctx[c].num_ctx,
+				e.ev = kssd_pair_eval(&a->expr, qlen, (size_t)(s->index[r + 1] - s->index[r]), ctx[c].num_ctx,
 						      obj[c].diff_obj, obj[c].diff_obj_section, ctx[c].num_mut2_ctx, tot[q],
-						      tot[r], s.stat.drfold, s.obj_bits, s.ctx_bits);
+						      tot[r], s->stat.drfold, s->obj_bits, s->ctx_bits);
 				if (!kssd_pair_passes_edge(&e.ev, a->cut, a->ctxcut, a->max_afcut))
 					continue;
 				if (indexed)
-					e.order = first_vote_ctx(&s, rec, nrec, q, r, a->max_ctx_freq, a->sample_step);
+					e.order = first_vote_ctx(s, rec, nrec, q, r, a->max_ctx_freq, a->sample_step);
 				edge_push(&rows[q], &e);
 			}
 		}
@@ -1489,37 +1492,164 @@ static int matrix_edges_self(const matrix_args_t *a)
 	stage("pairs counted and evaluated");
 	if (ix)
 		kssdhip_index_close(ix);
-	FILE *fp = a->outf[0] ? fopen(a->outf, "w") : stdout;
+	free(votes);
+	free(ctx);
+	free(obj);
+	free(built);
+	return rows;
+}
+
+/* Every accepted edge of `rows` in the order the reference reports them:
+ * the indexed scan's (query rows, first-vote order inside a row) or
+ * matrix_compute_edges' (ref-major). */
+static edge_list_t ordered_edges(edge_list_t *rows, int n, bool indexed)
+{
+	edge_list_t all = {0};
+	for (int q = 0; q < n; ++q) {
+		if (indexed)
+			qsort(rows[q].v, rows[q].n, sizeof *rows[q].v, edge_by_order);
+		for (size_t k = 0; k < rows[q].n; ++k)
+			edge_push(&all, &rows[q].v[k]);
+	}
+	if (!indexed && all.n)
+		qsort(all.v, all.n, sizeof *all.v, edge_by_ref);
+	return all;
+}
+
+static FILE *open_or_stdout(const char *path)
+{
+	FILE *fp = path[0] ? fopen(path, "w") : stdout;
 	if (!fp)
-		err(errno, "%s", a->outf);
+		err(errno, "%s", path);
+	return fp;
+}
+
+static void write_edge_file(const char *path, const kssd_sketchdir_t *s, const edge_list_t *all, const char *label)
+{
+	FILE *fp = open_or_stdout(path);
 	kssd_buf_t b = {0};
 	kssd_put_edge_header(&b);
 	put_buf(fp, &b);
 	kssd_buf_free(&b);
-	if (indexed) {
-		for (int q = 0; q < n; ++q) {
-			qsort(rows[q].v, rows[q].n, sizeof *rows[q].v, edge_by_order);
-			put_edges(fp, &s, &s, rows[q].v, rows[q].n, a->expr.label);
-		}
-	} else {
-		edge_list_t all = {0};
-		for (int q = 0; q < n; ++q)
-			for (size_t k = 0; k < rows[q].n; ++k)
-				edge_push(&all, &rows[q].v[k]);
-		if (all.n)
-			qsort(all.v, all.n, sizeof *all.v, edge_by_ref);
-		put_edges(fp, &s, &s, all.v, all.n, a->expr.label);
-		free(all.v);
+	put_edges(fp, s, s, all->v, all->n, label);
+	close_out(fp);
+}
+
+/* One sketch: the vote-nominated indexed scan when the sketch is indexed
+ * and keeps no conflicts (matrix_can_use_sorted_index, command_matrix.c:
+ * 1045-1053; pairwise_indexed_self_scan, pairwise_graph.c:1148-1316),
+ * else every pair (matrix_compute_edges, command_matrix.c:992-1014).  The
+ * GPU counts the shared-context votes and the pair counts; rows come out in
+ * query order, each row's edges in first-vote order (the reference's order
+ * with one thread; with more its rows interleave). */
+static int matrix_edges_self(const matrix_args_t *a)
+{
+	kssd_sketchdir_t s;
+	const bool have = kssd_file_in_dir(a->qrydir, KSSD_SORTED_INDEX_FILE);
+	open_dir(a->qrydir, KSSD_SK_KEYS | (have ? KSSD_SK_RECORDS : 0), &s);
+	stage("sketch read");
+	const bool indexed = have && !s.stat.conflict;
+	uint32_t *tot = entry_or_run_totals(&s, s.stat.conflict, false);
+	edge_list_t *rows = self_edges(a, &s, tot, indexed);
+	edge_list_t all = ordered_edges(rows, s.n, indexed);
+	write_edge_file(a->outf, &s, &all, a->expr.label);
+	stage("formatted");
+	for (int q = 0; q < s.n; ++q)
+		free(rows[q].v);
+	free(rows);
+	free(all.v);
+	free(tot);
+	kssd_sketchdir_close(&s);
+	return 0;
+}
+
+static int uf_find(int *parent, int x)
+{
+	while (parent[x] != x) {
+		parent[x] = parent[parent[x]];
+		x = parent[x];
 	}
+	return x;
+}
+
+/* Connected components of the accepted edges (pairwise_build_components,
+ * pairwise_graph.c:975-1030, or the indexed scan's unions), each with its
+ * best member by kssd_quality_compare (pairwise_component_result_finalize,
+ * :682-712); cluster ids number the components in order of their first
+ * member (matrix_write_component_report, command_matrix.c:1055-1118).
+ * --edge-out: the accepted edges, as the scan (indexed) or the pair loop
+ * (i < j, queried as i against j) reports them. */
+static int matrix_clusters(const matrix_args_t *a)
+{
+	kssd_sketchdir_t s;
+	const bool have = kssd_file_in_dir(a->qrydir, KSSD_SORTED_INDEX_FILE);
+	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META | (have ? KSSD_SK_RECORDS : 0), &s);
+	stage("sketch read");
+	const int n = s.n;
+	const bool indexed = have && !s.stat.conflict;
+	uint32_t *tot = entry_or_run_totals(&s, s.stat.conflict, false);
+	edge_list_t *rows = self_edges(a, &s, tot, indexed);
+	edge_list_t all = ordered_edges(rows, n, indexed);
+	int *parent = xmalloc((size_t)(n ? n : 1) * sizeof *parent);
+	int *size = xcalloc((size_t)(n ? n : 1), sizeof *size);
+	int *rep = xmalloc((size_t)(n ? n : 1) * sizeof *rep);
+	int *cid = xcalloc((size_t)(n ? n : 1), sizeof *cid);
+	for (int i = 0; i < n; ++i) {
+		parent[i] = i;
+		rep[i] = -1;
+	}
+	for (size_t k = 0; k < all.n; ++k) {
+		const int x = uf_find(parent, all.v[k].q), y = uf_find(parent, all.v[k].r);
+		if (x != y)
+			parent[x > y ? x : y] = x < y ? x : y;
+	}
+	for (int i = 0; i < n; ++i)
+		++size[uf_find(parent, i)];
+	for (int i = 0; i < n; ++i) {
+		const int root = uf_find(parent, i);
+		if (rep[root] < 0 || kssd_quality_compare(s.meta, s.index, i, rep[root]) > 0)
+			rep[root] = i;
+	}
+	int next = 1;
+	for (int i = 0; i < n; ++i) {
+		const int root = uf_find(parent, i);
+		if (!cid[root])
+			cid[root] = next++;
+	}
+	if (a->edge_outf[0]) {
+		if (!indexed) /* the pair loop queries the smaller index against the larger */
+			for (size_t k = 0; k < all.n; ++k) {
+				edge_t *e = &all.v[k];
+				const int i = e->r, j = e->q;
+				e->ev = kssd_pair_eval(&a->expr, (size_t)(s.index[i + 1] - s.index[i]),
+						       (size_t)(s.index[j + 1] - s.index[j]), e->ev.xny, e->ev.dobj, e->ev.dsec,
+						       e->ev.mut2, tot[i], tot[j], s.stat.drfold, s.obj_bits, s.ctx_bits);
+				e->q = i;
+				e->r = j;
+			}
+		write_edge_file(a->edge_outf, &s, &all, a->expr.label);
+	}
+	FILE *fp = open_or_stdout(a->outf);
+	kssd_buf_t b = {0};
+	kssd_buf_printf(&b, "sample\tcluster_id\tcomponent_size\trepresentative\n");
+	for (int i = 0; i < n; ++i) {
+		const int root = uf_find(parent, i);
+		kssd_buf_printf(&b, "%s\tcluster_%06d\t%d\t%s\n", s.names[i], cid[root], size[root], s.names[rep[root]]);
+		if (b.l > (1u << 20))
+			put_buf(fp, &b);
+	}
+	put_buf(fp, &b);
+	kssd_buf_free(&b);
 	close_out(fp);
 	stage("formatted");
 	for (int q = 0; q < n; ++q)
 		free(rows[q].v);
 	free(rows);
-	free(votes);
-	free(ctx);
-	free(obj);
-	free(built);
+	free(all.v);
+	free(parent);
+	free(size);
+	free(rep);
+	free(cid);
 	free(tot);
 	kssd_sketchdir_close(&s);
 	return 0;
@@ -1590,6 +1720,8 @@ static int cmd_matrix(int argc, char **argv)
 	stage("GPU ready");
 	if (a.format == 3)
 		return a.refdir[0] ? matrix_edges_rect(&a) : matrix_edges_self(&a);
+	if (a.format == 4)
+		return matrix_clusters(&a);
 	if (a.refdir[0]) {
 		if (a.format == 0)
 			a.format = 1;

@@ -1119,6 +1119,61 @@ int kssd_pair_passes_edge(const kssd_pair_eval_t *ev, double cut, uint32_t ctxcu
 	return ev && ev->valid && ev->distance < cut && ev->xny >= ctxcut && ev->max_af >= max_afcut;
 }
 
+static int meta_rankable(const kssd_infile_meta_t *meta, int i)
+{
+	/* pairwise_meta_rankable: a current-version record with a length */
+	return meta && meta[i].meta_fmt_version == 1u && meta[i].total_length_bp > 0;
+}
+
+static double finite_or(double v, double fallback)
+{
+	return isfinite(v) ? v : fallback;
+}
+
+static int positive_lower_cmp(uint32_t a, uint32_t b)
+{
+	/* fewer records is better, but 0 (unknown) loses */
+	if (a == b)
+		return 0;
+	if (a == 0)
+		return -1;
+	if (b == 0)
+		return 1;
+	return a < b ? 1 : -1;
+}
+
+int kssd_quality_compare(const kssd_infile_meta_t *meta, const uint64_t *index, int a, int b)
+{
+	if (a == b)
+		return 0;
+	const int ra = meta_rankable(meta, a), rb = meta_rankable(meta, b);
+	if (ra != rb)
+		return ra ? 1 : -1;
+	if (ra && rb) {
+		const kssd_infile_meta_t *ma = &meta[a], *mb = &meta[b];
+		const int fa = ma->infile_fmt == 1, fb = mb->infile_fmt == 1; /* FASTA first */
+		if (fa != fb)
+			return fa ? 1 : -1;
+		const double aa = finite_or((double)ma->asm_level, -1.0), ab = finite_or((double)mb->asm_level, -1.0);
+		if (aa != ab)
+			return aa > ab ? 1 : -1;
+		if (ma->total_length_bp != mb->total_length_bp)
+			return ma->total_length_bp > mb->total_length_bp ? 1 : -1;
+		const int rc = positive_lower_cmp(ma->record_count, mb->record_count);
+		if (rc)
+			return rc;
+		if (ma->median_length_bp != mb->median_length_bp)
+			return ma->median_length_bp > mb->median_length_bp ? 1 : -1;
+		const double ca = finite_or((double)ma->length_cv, INFINITY), cb = finite_or((double)mb->length_cv, INFINITY);
+		if (ca != cb)
+			return ca < cb ? 1 : -1;
+	}
+	const uint64_t ea = index[a + 1] - index[a], eb = index[b + 1] - index[b];
+	if (ea != eb)
+		return ea > eb ? 1 : -1;
+	return a < b ? 1 : -1;
+}
+
 void kssd_put_edge_header(kssd_buf_t *b)
 {
 	kssd_buf_printf(b, "Qry\tRef\tDistance\tSimilarity\tMetric\tXnY_ctx\tQry_align_fraction\tRef_align_fraction\t"

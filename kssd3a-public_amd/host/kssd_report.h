@@ -200,6 +200,10 @@ kssd_pair_eval_t kssd_pair_eval(const kssd_metric_expr_t *e, size_t qry_len, siz
 				int drfold, unsigned obj_bits, unsigned ctx_bits);
 /* pairwise_eval_passes_edge (pairwise_graph.c:518-530) */
 int kssd_pair_passes_edge(const kssd_pair_eval_t *ev, double cut, uint32_t ctxcut, double max_afcut);
+/* pairwise_quality_compare (pairwise_graph.c:532-603): > 0 when sample a
+ * is the better cluster representative, < 0 when b is; meta may be NULL
+ * (then only sketch entries and the index decide); index: comblco.index. */
+int kssd_quality_compare(const kssd_infile_meta_t *meta, const uint64_t *index, int a, int b);
 /* matrix_print_edge_header / matrix_print_edge (command_matrix.c:87-101) */
 void kssd_put_edge_header(kssd_buf_t *b);
 void kssd_put_edge(kssd_buf_t *b, const char *qry_name, const char *ref_name, const kssd_pair_eval_t *ev,

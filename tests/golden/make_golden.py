@@ -263,6 +263,29 @@ def make_matrix_edges():
                                                "-m", "p_dist", c / "ref"]))
 
 
+def make_matrix_clusters():
+    """`matrix --format clusters` (with one thread): components of the
+    indexed scan's edges and of every pair's, their representatives (by the
+    input metadata, or by sketch size without it) and --edge-out."""
+    out = HERE / "genomes"
+    S = out / "all"
+    tmp = HERE / "_noindex"
+    shutil.rmtree(tmp, ignore_errors=True)
+    shutil.copytree(S, tmp)
+    for f in ("sortedcomb_ctxgid64obj32", "sortedcomb_ctxgid64obj32.unique_bits", "lcofiles.infilemeta"):
+        (tmp / f).unlink(missing_ok=True)
+    for name, d, args in (("clusters_default", S, ["--cut", "0.05"]),
+                          ("clusters_wide", S, ["--cut", "0.9", "-m", "mash|aaf"]),
+                          ("clusters_cuts", S, ["--cut", "0.06", "--ctxcut", "200", "--index-max-ctx-freq", "4"]),
+                          ("clusters_noindex", tmp, ["--cut", "0.05"]),
+                          ("clusters_noindex_wide", tmp, ["--cut", "0.9", "-m", "ctx-moe&p_dist"])):
+        e = tmp.parent / "_edges.tsv"
+        (out / f"{name}.tsv").write_text(run(["matrix", "--format", "clusters", "-p", "1", "--edge-out", e,
+                                              *args, d]))
+        shutil.move(e, out / f"{name}_edges.tsv")
+    shutil.rmtree(tmp)
+
+
 def make_sketch():
     """Small FASTA inputs (multi-record, lowercase, non-ACGT runs) and the
     reference's sketches of them, at the default -f8 and at -f4."""
@@ -301,6 +324,7 @@ def main():
     make_metrics()
     make_matrix_cmd()
     make_matrix_edges()
+    make_matrix_clusters()
     make_sketch()
     print("golden fixtures written under", HERE)
 

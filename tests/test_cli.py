@@ -298,6 +298,34 @@ def test_cli_matrix_edges_match_reference(tmp_path, golden, args, indexed):
     assert p.stdout == (GEN / golden).read_text()
 
 
+CLUSTER_CASES = [  # (golden stem, args, the sketch keeps its index and metadata)
+    ("clusters_default", ["--cut", "0.05"], True),
+    ("clusters_wide", ["--cut", "0.9", "-m", "mash|aaf"], True),
+    ("clusters_cuts", ["--cut", "0.06", "--ctxcut", "200", "--index-max-ctx-freq", "4"], True),
+    ("clusters_noindex", ["--cut", "0.05"], False),
+    ("clusters_noindex_wide", ["--cut", "0.9", "-m", "ctx-moe&p_dist"], False),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stem,args,indexed", CLUSTER_CASES, ids=[c[0] for c in CLUSTER_CASES])
+def test_cli_matrix_clusters_match_reference(tmp_path, stem, args, indexed):
+    """Cluster reports and their --edge-out edges: components of the
+    indexed scan's edges (representatives by input metadata) and of every
+    pair's (no index, no metadata: by sketch size)."""
+    dirs = genome_dirs(tmp_path, True)
+    if indexed:
+        for f in INDEX_FILES:
+            shutil.copy(GEN / "all" / f, dirs["S"] / f)
+    else:
+        (dirs["S"] / "lcofiles.infilemeta").unlink(missing_ok=True)
+    edges = tmp_path / "edges.tsv"
+    p = run_cli(["matrix", "--format", "clusters", "--edge-out", edges, *args, dirs["S"]])
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == (GEN / f"{stem}.tsv").read_text()
+    assert edges.read_text() == (GEN / f"{stem}_edges.tsv").read_text()
+
+
 @pytest.mark.gpu
 def test_cli_matrix_edges_conflict_sketch(tmp_path):
     """A conflict sketch with an index still takes the every-pair path."""
@@ -314,8 +342,9 @@ def test_matrix_usage_errors():
                       (["matrix", "-m", "aaf&mash|p_dist", "--sparse", "--cut", "1", "x"], "-m/--metric must be"),
                       (["matrix", "-m", "bogus", "x"], "-m/--metric must be"),
                       (["matrix", "--sparse", "x"], "requires --cut"),
-                      (["matrix", "--format", "clusters", "--cut", "0.1", "x"], "not offered"),
-                      (["matrix", "--edge-out", "e.tsv", "x"], "not offered"),
+                      (["matrix", "--format", "dedup-plan", "--cut", "0.1", "x"], "not offered"),
+                      (["matrix", "--format", "clusters", "--cut", "0.1", "-r", "x", "-q", "y"], "requires one sketch"),
+                      (["matrix", "--keep-out", "k.txt", "x"], "not offered"),
                       (["matrix"], "missing sketch input")):
         p = run_cli(args)
         assert p.returncode == 64 and msg in p.stderr, (args, p.stderr)
