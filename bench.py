@@ -169,6 +169,46 @@ def index_build_bench(host, hip, rk, ri, threads, reps=1):
     return rec, build
 
 
+def edges_bench(args, host, threads):
+    """`matrix --format edges` end to end on a GENOMES-genome clade sketch
+    written to disk with its index: kssd3a-hip (votes and pair counts on the
+    GPU) against the reference's own binary (oracle/_ref/kssd3a_ref, when
+    present) with the same threads; the edge sets must agree (the
+    reference's row order depends on its thread schedule)."""
+    import subprocess
+    import tempfile
+    rng = np.random.default_rng(args.seed + 300)
+    nref = args.edges_genomes - args.edges_genomes % 20
+    specs, _, _ = clade_specs(rng, nref, args.seed + 300)
+    rk, ri = host.simsketch(specs, threads=threads)
+    hip = kh.KssdHip(0)
+    rec, bits = hip.build_index(rk, ri)
+    out = {"genomes": nref, "records": int(len(rec)), "cut": args.edges_cut, "threads": threads}
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        sk = Path(d) / "sketch"
+        host.write_sketch(sk, rk, ri, [f"genome{i:05d}" for i in range(nref)], records=rec, unique_bits=bits)
+        del rk, rec, bits
+        runs = {"gpu": [str(kh.PKG / "kssd3a-hip")]}
+        refbin = ROOT / "oracle" / "_ref" / "kssd3a_ref"
+        if refbin.exists():
+            runs["reference"] = [str(refbin)]
+        text = {}
+        for name, cmd in runs.items():
+            argv = [*cmd, "matrix", "--format", "edges", "--cut", str(args.edges_cut), "-p", str(threads), str(sk)]
+            subprocess.run(argv, capture_output=True, check=True)  # warm the page cache
+            t = time.perf_counter()
+            p = subprocess.run(argv, capture_output=True, text=True, check=True)
+            out[f"{name}_s"] = time.perf_counter() - t
+            text[name] = p.stdout
+        lines = text["gpu"].splitlines()
+        out["edges"] = len(lines) - 1
+        if "reference" in text:
+            ref_lines = text["reference"].splitlines()
+            out["same_edges"] = lines[0] == ref_lines[0] and sorted(lines[1:]) == sorted(ref_lines[1:])
+            out["reference_over_gpu"] = out["reference_s"] / out["gpu_s"]
+    return out
+
+
 def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
     """Large resident index: args.index_refs genomes (clades of 20, as above),
     built into the inverted index; two query sets counted in one batched
@@ -435,6 +475,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all)")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--edges-genomes", type=int, default=0, metavar="GENOMES",
+                    help="only time `matrix --format edges` on GENOMES clade genomes (GPU vs the reference "
+                         "binary) and exit")
+    ap.add_argument("--edges-cut", type=float, default=0.05)
     ap.add_argument("--index-build-only", type=int, default=0, metavar="GENOMES",
                     help="only time the index build of GENOMES clade genomes (host builder vs GPU) and exit")
     ap.add_argument("--tiling", default="",
@@ -458,6 +502,9 @@ def main():
         sys.exit(2)
     threads = args.threads or min(16, os.cpu_count() or 1)
     host = kh.Host()
+    if args.edges_genomes:
+        print(json.dumps({"matrix_edges": edges_bench(args, host, threads)}), flush=True)
+        return
     if args.index_build_only:
         # the index build alone: clade genomes as in the index bench
         rng = np.random.default_rng(args.seed + 200)

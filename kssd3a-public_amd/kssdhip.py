@@ -49,6 +49,12 @@ class Tiling(ctypes.Structure):
                 ("group_partitions", I)]
 
 
+class KssdStat(ctypes.Structure):
+    """kssd_stat_t (lcofiles.stat header, 32 bytes)."""
+    _fields_ = [("hash_id", U32), ("koc", ctypes.c_bool), ("conflict", ctypes.c_bool), ("coden_len", I),
+                ("klen", I), ("hclen", I), ("holen", I), ("drfold", I), ("infile_num", I)]
+
+
 class U64Vec(ctypes.Structure):
     _fields_ = [("a", ctypes.POINTER(U64)), ("n", S), ("cap", S)]
 
@@ -388,6 +394,8 @@ def load_kssdhost():
     lib.kssd_u64vec_free.argtypes = [ctypes.POINTER(U64Vec)]
     lib.kssd_u64vec_free.restype = None
     lib.kssd_sketch_fasta_file.argtypes = [ctypes.c_char_p, I, I, ctypes.POINTER(U64Vec)]
+    lib.kssd_sketchdir_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(KssdStat), V, V, V]
+    lib.kssd_write_index_files.argtypes = [ctypes.c_char_p, V, S, V]
     lib.kssd_sketching_id.argtypes = [I, I, I, I]
     lib.kssd_sketching_id.restype = U32
     lib.kssd_simsketch.argtypes = [V, I, I, I, I, ctypes.POINTER(ctypes.POINTER(U64)), V]
@@ -460,6 +468,26 @@ class Host:
         out = np.ctypeslib.as_array(keys, shape=(n,)).copy() if n else np.zeros(0, np.uint64)
         self._libc.free(ctypes.cast(keys, V))
         return out, index
+
+    def write_sketch(self, path, keys, index, names, drfold=8, conflict=False, records=None, unique_bits=None):
+        """A sketch directory (lcofiles.stat, comblco, comblco.index) of
+        coden-pattern keys; with `records`, also its sorted index files."""
+        Path(path).mkdir(parents=True, exist_ok=True)
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        st = KssdStat(int(self.lib.kssd_sketching_id(11, 0, 0, drfold)), False, conflict, 11, 32, 11, 0, drfold,
+                      len(index) - 1)
+        nm = (ctypes.c_char * 256 * max(len(names), 1))()
+        for i, n in enumerate(names):
+            nm[i].value = n.encode()[:255]
+        if self.lib.kssd_sketchdir_write(str(path).encode(), ctypes.byref(st), ctypes.cast(nm, V), _p(keys),
+                                         _p(index)) != 0:
+            raise OSError(f"cannot write the sketch {path}")
+        if records is not None:
+            ub = np.frombuffer(unique_bits, np.uint8) if unique_bits is not None else None
+            if self.lib.kssd_write_index_files(str(path).encode(), _p(records), len(records),
+                                               _p(ub) if ub is not None else None) != 0:
+                raise OSError(f"cannot write the index of {path}")
 
     def build_index(self, keys, index, obj_bits=20, threads=None):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
