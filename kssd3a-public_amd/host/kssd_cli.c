@@ -16,7 +16,7 @@
  * Output bytes follow the reference path that the same inputs and options
  * select (index, single/multi-query stream, hash, direct or indexed matrix).
  * The reference's auto-sketching inputs (raw FASTA/FASTQ, --reflist,
- * --qrylist, --pair) and the coverage/report side outputs are not offered.
+ * --qrylist, --pair) and the --unique-ctx-report side output are not offered.
  */
 #define _GNU_SOURCE
 #include <err.h>
@@ -170,6 +170,7 @@ enum {
 	OPT_VALUES,
 	OPT_RAW,
 	OPT_UNIQUE,
+	OPT_COVERAGE,
 	OPT_DEVICE,
 	OPT_HELP,
 	OPT_NOT_OFFERED,
@@ -202,6 +203,7 @@ static const char ani_help[] =
 	"  -o, --outfile=FILE       output file [stdout]\n"
 	"      --raw-output         skip calibrated/best ANI\n"
 	"      --unique-ctx         adaptive reference-unique contexts (indexed -r, --qraw)\n"
+	"      --estimate-coverage  coverage columns from the query's comblco.a (--qraw, detail)\n"
 	"  -N, --top=N              at most N references per query [all]\n"
 	"  -p, --threads=N          host threads for indexing, ranking and formatting [all]\n"
 	"      --device=N           HIP device ordinal [0]\n";
@@ -230,6 +232,7 @@ static void parse_ani(int argc, char **argv, ani_args_t *a)
 		{"outfile", required_argument, NULL, 'o'},
 		{"raw-output", no_argument, NULL, OPT_RAW},
 		{"unique-ctx", no_argument, NULL, OPT_UNIQUE},
+		{"estimate-coverage", no_argument, NULL, OPT_COVERAGE},
 		{"top", required_argument, NULL, 'N'},
 		{"threads", required_argument, NULL, 'p'},
 		{"device", required_argument, NULL, OPT_DEVICE},
@@ -238,7 +241,6 @@ static void parse_ani(int argc, char **argv, ani_args_t *a)
 		{"reflist", required_argument, NULL, OPT_NOT_OFFERED},
 		{"qrylist", required_argument, NULL, OPT_NOT_OFFERED},
 		{"unique-ctx-report", required_argument, NULL, OPT_NOT_OFFERED},
-		{"estimate-coverage", no_argument, NULL, OPT_NOT_OFFERED},
 		{"glist", required_argument, NULL, OPT_NOT_OFFERED},
 		{"control", required_argument, NULL, OPT_NOT_OFFERED},
 		{"pair", no_argument, NULL, OPT_NOT_OFFERED},
@@ -357,6 +359,9 @@ static void parse_ani(int argc, char **argv, ani_args_t *a)
 		case OPT_UNIQUE:
 			o->unique_ctx = true;
 			break;
+		case OPT_COVERAGE:
+			o->estimate_coverage = true;
+			break;
 		case 'N':
 			o->ntop = (int)parse_int("-N/--top", optarg, 1, INT_MAX);
 			break;
@@ -388,6 +393,10 @@ static void parse_ani(int argc, char **argv, ani_args_t *a)
 			usage_error("--unique-ctx uses calibrated read-mode Naive ANI; incompatible metric, raw-output, "
 				    "unified-metric or close-pair filter");
 	}
+	if (o->estimate_coverage && o->fmt != 0)
+		usage_error("--estimate-coverage is supported only with detail output (-m0).");
+	if (o->estimate_coverage && !o->unassembled)
+		usage_error("--estimate-coverage currently requires --qraw raw/unassembled query mode.");
 	if ((o->max_pdist_set || o->max_diff_obj_section >= 0) && o->fmt != 0)
 		usage_error("--max-pdist/--max-diff-section are supported only with detail output (-m0).");
 	if (!a->refdir[0] && !a->qrydir[0] && npos >= 2)
@@ -403,6 +412,15 @@ static void parse_ani(int argc, char **argv, ani_args_t *a)
 }
 
 /* ------------------------------------------------------------ output */
+/* print_ani_detail_header, command_ani.c:190-196 */
+static void put_detail_header(FILE *fp, const ani_args_t *a)
+{
+	if (a->r.estimate_coverage)
+		fprintf(fp, "%s\t%s\n", kssd_detail_header, kssd_coverage_header);
+	else
+		fprintf(fp, "%s\n", kssd_detail_header);
+}
+
 static FILE *open_out(const ani_args_t *a)
 {
 	if (!a->outf[0])
@@ -595,20 +613,100 @@ static uint32_t *entry_or_run_totals(const kssd_sketchdir_t *s, bool runs, bool 
 	return t;
 }
 
+/* ------------------------------------------------------------ coverage inputs */
+/* What --estimate-coverage reads beyond the sketches: the query's read
+ * counts and the reference's unique-marker bits. */
+typedef struct {
+	uint32_t *qcounts; /* NULL: the query has none */
+	uint8_t *bits;     /* NULL: markers come from the reported rows */
+} cov_inputs_t;
+
+/* The reference's .unique_bits when it and the sorted index are present and
+ * sized for the sketch (load_coverage_unique_bits, command_ani.c:2209-2250). */
+static uint8_t *load_unique_bits(const char *dir, const kssd_sketchdir_t *ref)
+{
+	if (!kssd_file_in_dir(dir, KSSD_UNIQUE_BITS_FILE))
+		return NULL;
+	if (!ref->records) {
+		warnx("ignoring %s/%s because %s is missing", dir, KSSD_UNIQUE_BITS_FILE, KSSD_SORTED_INDEX_FILE);
+		return NULL;
+	}
+	char *path = xmalloc(strlen(dir) + sizeof KSSD_UNIQUE_BITS_FILE + 2);
+	sprintf(path, "%s/%s", dir, KSSD_UNIQUE_BITS_FILE);
+	size_t n = 0;
+	uint8_t *bits = kssd_read_file(path, &n);
+	free(path);
+	const size_t want = (size_t)((ref->index[ref->n] + 7) >> 3);
+	if (bits && n != want) {
+		warnx("ignoring %s/%s: %zu bytes, expected %zu", dir, KSSD_UNIQUE_BITS_FILE, n, want);
+		free(bits);
+		bits = NULL;
+	}
+	return bits;
+}
+
+/* The query's comblco.a: read when present in the index path (a size
+ * mismatch warns and drops it, command_ani.c:538-554), and when the sketch
+ * says it keeps counts in the streamed paths (global_basic.c:1123-1124). */
+static uint32_t *load_query_counts(const char *dir, const kssd_sketchdir_t *qry, bool index_path)
+{
+	if (index_path ? !kssd_file_in_dir(dir, KSSD_ABUND_FILE) : !qry->stat.koc)
+		return NULL;
+	char *path = xmalloc(strlen(dir) + sizeof KSSD_ABUND_FILE + 2);
+	sprintf(path, "%s/%s", dir, KSSD_ABUND_FILE);
+	size_t n = 0;
+	uint32_t *c = kssd_read_file(path, &n);
+	const size_t want = (size_t)qry->index[qry->n] * sizeof *c;
+	if (!c && !index_path)
+		err(EXIT_FAILURE, "%s", path);
+	if (c && n != want) {
+		if (!index_path)
+			errx(EXIT_FAILURE, "%s: %zu bytes, expected %zu", path, n, want);
+		warnx("ignoring %s: %zu bytes, expected %zu", path, n, want);
+		free(c);
+		c = NULL;
+	}
+	free(path);
+	return c;
+}
+
+static void load_coverage(const ani_args_t *a, const kssd_sketchdir_t *ref, const kssd_sketchdir_t *qry,
+			  bool index_path, cov_inputs_t *cv)
+{
+	memset(cv, 0, sizeof *cv);
+	if (!a->r.estimate_coverage)
+		return;
+	cv->bits = load_unique_bits(a->refdir, ref);
+	cv->qcounts = load_query_counts(a->qrydir, qry, index_path);
+}
+
 /* ------------------------------------------------------------ detail paths */
 typedef enum { PATH_INDEX, PATH_STREAM, PATH_MULTISTREAM, PATH_HASH } detail_path_t;
 
 static void emit_rows(FILE *fp, const ani_args_t *a, detail_path_t path, const kssd_sketchdir_t *ref,
 		      const kssd_sketchdir_t *qry, int q0, int nq, const kssd_ctx_mut2_t *ctx,
 		      const kssd_obj_section_t *obj, const kssd_ctx_mut2_t *uctx, const kssd_obj_section_t *uobj,
-		      const uint32_t *qry_ctx, const uint32_t *ref_ctx)
+		      const uint32_t *qry_ctx, const uint32_t *ref_ctx, const cov_inputs_t *cv)
 {
 	const int R = ref->n;
 	kssd_buf_t *bufs = xcalloc((size_t)nq, sizeof *bufs);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
 	for (int i = 0; i < nq; ++i) {
 		const int q = q0 + i;
+		/* the hash path computes no coverage (its rows print NA:not_computed) */
+		const kssd_coverage_src_t src = {
+			.qkeys = qry->keys + qry->index[q],
+			.qcounts = cv->qcounts ? cv->qcounts + qry->index[q] : NULL,
+			.qn = (size_t)(qry->index[q + 1] - qry->index[q]),
+			.ref_keys = ref->keys,
+			.ref_index = ref->index,
+			.ref_records = cv->bits ? ref->records : NULL,
+			.ref_unique_bits = cv->bits,
+			.ref_nrec = (size_t)ref->index[R],
+			.obj_bits = ref->obj_bits,
+		};
 		kssd_query_counts_t c = {
+			.cov = a->r.estimate_coverage && path != PATH_HASH ? &src : NULL,
 			.ctx = ctx + (size_t)i * R,
 			.obj = obj + (size_t)i * R,
 			.uctx = uctx ? uctx + (size_t)i * R : NULL,
@@ -647,10 +745,13 @@ static void emit_rows(FILE *fp, const ani_args_t *a, detail_path_t path, const k
 static int run_index_path(const ani_args_t *a)
 {
 	kssd_sketchdir_t ref, qry;
-	open_dir(a->refdir, KSSD_SK_RECORDS | KSSD_SK_ANNO | KSSD_SK_META, &ref);
+	open_dir(a->refdir,
+		 KSSD_SK_RECORDS | KSSD_SK_ANNO | KSSD_SK_META | (a->r.estimate_coverage ? KSSD_SK_KEYS : 0), &ref);
 	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META, &qry);
 	if (!ref.records)
 		errx(EXIT_FAILURE, "%s/%s: reference index missing", a->refdir, KSSD_SORTED_INDEX_FILE);
+	cov_inputs_t cv;
+	load_coverage(a, &ref, &qry, true, &cv);
 	if (qry.stat.hash_id != ref.stat.hash_id)
 		errx(EXIT_FAILURE, "hash_id mismatch between query and reference sketches");
 	stage("sketches read");
@@ -670,7 +771,7 @@ static int run_index_path(const ani_args_t *a)
 	if (a->r.fmt)
 		put_header_names(fp, &ref);
 	else
-		fprintf(fp, "%s\n", kssd_detail_header);
+		put_detail_header(fp, a);
 	const bool unique = a->r.unique_ctx;
 	const uint32_t flags = (unique ? KSSDHIP_UNIQUE : 0) | (a->r.ignoreconflict ? KSSDHIP_IGNORECONFLICT : 0);
 	const int blk = block_queries(Q, R, unique);
@@ -686,11 +787,13 @@ static int run_index_path(const ani_args_t *a)
 						      obj, uctx, uobj),
 				  "GPU counting failed");
 		stage("block counted");
-		emit_rows(fp, a, PATH_INDEX, &ref, &qry, q0, nb, ctx, obj, uctx, uobj, qry_ctx, ref_ctx);
+		emit_rows(fp, a, PATH_INDEX, &ref, &qry, q0, nb, ctx, obj, uctx, uobj, qry_ctx, ref_ctx, &cv);
 		stage("block ranked and formatted");
 	}
 	close_out(fp);
 	kssdhip_index_close(ix);
+	free(cv.qcounts);
+	free(cv.bits);
 	free(ctx);
 	free(obj);
 	free(uctx);
@@ -711,10 +814,14 @@ static int run_index_path(const ani_args_t *a)
 static int run_rect_detail(const ani_args_t *a, detail_path_t path)
 {
 	kssd_sketchdir_t ref, qry;
-	open_dir(a->refdir, KSSD_SK_KEYS | KSSD_SK_ANNO | KSSD_SK_META, &ref);
+	const bool cov = a->r.estimate_coverage && path != PATH_HASH;
+	open_dir(a->refdir, KSSD_SK_KEYS | KSSD_SK_ANNO | KSSD_SK_META | (cov ? KSSD_SK_RECORDS : 0), &ref);
 	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META, &qry);
 	if (qry.stat.hash_id != ref.stat.hash_id)
 		errx(EXIT_FAILURE, "hash_id mismatch between query and reference sketches");
+	cov_inputs_t cv = {0};
+	if (cov)
+		load_coverage(a, &ref, &qry, false, &cv);
 	stage("sketches read");
 	const int R = ref.n, Q = qry.n;
 	uint32_t *qry_ctx, *ref_ctx;
@@ -734,10 +841,12 @@ static int run_rect_detail(const ani_args_t *a, detail_path_t path)
 	rect_counts(&ref, &qry, a->r.ignoreconflict, a->p, ctx, obj);
 	stage("pairs counted");
 	FILE *fp = open_out(a);
-	fprintf(fp, "%s\n", kssd_detail_header);
-	emit_rows(fp, a, path, &ref, &qry, 0, Q, ctx, obj, NULL, NULL, qry_ctx, ref_ctx);
+	put_detail_header(fp, a);
+	emit_rows(fp, a, path, &ref, &qry, 0, Q, ctx, obj, NULL, NULL, qry_ctx, ref_ctx, &cv);
 	stage("ranked and formatted");
 	close_out(fp);
+	free(cv.qcounts);
+	free(cv.bits);
 	free(ctx);
 	free(obj);
 	free(qry_ctx);
@@ -1767,73 +1876,188 @@ static char **sketch_inputs(int npos, char **pos, const char *list, int *n_out)
 	return v;
 }
 
-static int sketch_files(const char *outdir, char **files, int n, int drfold, bool conflict, bool meta, int threads)
+typedef struct {
+	int drfold, threads;
+	bool conflict, meta, abundance, asone;
+	const char *sample_name;
+	kssd_count_opts_t counts;
+} sketch_opts_t;
+
+/* One sample's keys: collected, counted when a count option asks for it,
+ * filtered by count, then cleared of conflicting contexts
+ * (finalize_filter_conflict_and_write, command_sketch.c:5194-5214; the
+ * per-file loop of sketch_many_files_in_parallel, :4455-4510). */
+typedef struct {
+	kssd_u64vec_t keys;
+	uint32_t *counts; /* kept with -A */
+	kssd_count_filter_t filter;
+} sample_keys_t;
+
+static void finish_sample(sample_keys_t *sm, const sketch_opts_t *o)
 {
-	for (int i = 0; i < n; ++i) {
-		struct stat st;
-		if (stat(files[i], &st) != 0 || !S_ISREG(st.st_mode))
-			errx(EXIT_FAILURE, "%s: not a readable file (directories are not scanned here; list files with -l)",
-			     files[i]);
-		if (!kssd_infile_fmt_from_path(files[i]))
-			errx(EXIT_FAILURE, "%s: not a FASTA/FASTQ name (.fasta .fna .fas .fa .fq .fastq, optionally .gz)",
-			     files[i]);
-		if (strlen(files[i]) >= KSSD_PATHLEN)
-			errx(EXIT_FAILURE, "%s: path longer than %d bytes", files[i], KSSD_PATHLEN - 1);
+	if (!kssd_count_opts_need_counts(&o->counts)) {
+		kssd_finalize_keys(&sm->keys, o->conflict);
+		return;
 	}
-	kssd_u64vec_t *sk = xcalloc((size_t)n, sizeof *sk);
-	kssd_infile_meta_t *mt = xcalloc((size_t)n, sizeof *mt);
+	if (kssd_count_keys(&sm->keys, &sm->counts) != 0)
+		errx(EXIT_FAILURE, "out of memory counting k-mers");
+	sm->filter = kssd_apply_count_filters(sm->keys.a, sm->counts, &sm->keys.n, &o->counts);
+	if (!o->conflict)
+		kssd_drop_conflicts(sm->keys.a, sm->counts, &sm->keys.n);
+	if (!o->abundance) {
+		free(sm->counts);
+		sm->counts = NULL;
+	}
+}
+
+static void check_input(const char *path)
+{
+	struct stat st;
+	if (stat(path, &st) != 0 || !S_ISREG(st.st_mode))
+		errx(EXIT_FAILURE, "%s: not a readable file (directories are not scanned here; list files with -l)", path);
+	if (!kssd_infile_fmt_from_path(path))
+		errx(EXIT_FAILURE, "%s: not a FASTA/FASTQ name (.fasta .fna .fas .fa .fq .fastq, optionally .gz)", path);
+	if (strlen(path) >= KSSD_PATHLEN)
+		errx(EXIT_FAILURE, "%s: path longer than %d bytes", path, KSSD_PATHLEN - 1);
+}
+
+/* --asone: every input pooled into one sample (sketch_few_files_with_
+ * intrafile_parallel, command_sketch.c:5435-5497): the format shared by all
+ * inputs (else unknown, flagged mixed), FASTA record lengths of all inputs
+ * in order, the first path (or --sample-name) as the name. */
+static void sketch_asone(char **files, int n, const sketch_opts_t *o, sample_keys_t *sm, kssd_infile_meta_t *mt)
+{
+	kssd_u64vec_t *part = xcalloc((size_t)n, sizeof *part);
 	int bad = -1;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
-	for (int i = 0; i < n; ++i) {
-		if (kssd_sketch_fasta_file(files[i], drfold, conflict, &sk[i]) != 0 ||
-		    (meta && kssd_infile_meta_for_path(files[i], &mt[i]) != 0)) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(o->threads)
+	for (int i = 0; i < n; ++i)
+		if (kssd_collect_fastx_file(files[i], o->drfold, &part[i]) != 0) {
 #pragma omp critical
 			bad = i;
 		}
-	}
 	if (bad >= 0)
 		err(EXIT_FAILURE, "%s", files[bad]);
-	stage("inputs sketched");
-	uint64_t *index = xmalloc(((size_t)n + 1) * sizeof *index);
-	index[0] = 0;
+	size_t total = 0;
 	for (int i = 0; i < n; ++i)
-		index[i + 1] = index[i] + sk[i].n;
-	uint64_t *keys = xmalloc((size_t)index[n] * sizeof *keys);
+		total += part[i].n;
+	sm->keys.a = xmalloc((total ? total : 1) * sizeof *sm->keys.a);
+	sm->keys.cap = total ? total : 1;
 	for (int i = 0; i < n; ++i) {
-		if (sk[i].n)
-			memcpy(keys + index[i], sk[i].a, sk[i].n * sizeof *keys);
-		kssd_u64vec_free(&sk[i]);
+		if (part[i].n)
+			memcpy(sm->keys.a + sm->keys.n, part[i].a, part[i].n * sizeof *part[i].a);
+		sm->keys.n += part[i].n;
+		kssd_u64vec_free(&part[i]);
 	}
-	char (*names)[KSSD_PATHLEN] = xcalloc((size_t)n, KSSD_PATHLEN);
+	free(part);
+	finish_sample(sm, o);
+	if (!o->meta)
+		return;
+	int fmt = kssd_infile_fmt_from_path(files[0]);
+	unsigned flags = 0x10u; /* KSSD_INFILE_FLAG_MIXED_FORMAT */
+	for (int i = 0; i < n; ++i) {
+		flags |= kssd_infile_flags_from_path(files[i]);
+		if (kssd_infile_fmt_from_path(files[i]) != fmt)
+			fmt = 0;
+	}
+	if (fmt)
+		flags &= ~0x10u;
+	uint64_t *len = NULL;
+	size_t nl = 0, cap = 0;
+	if (fmt == 1)
+		for (int i = 0; i < n; ++i)
+			if (kssd_fasta_record_lengths(files[i], &len, &nl, &cap) != 0)
+				err(EXIT_FAILURE, "%s", files[i]);
+	kssd_infile_meta_from_lengths(len, nl, fmt, 1 /* KSSD_CREATE_ASONE */, flags, mt);
+	free(len);
+}
+
+static int sketch_files(const char *outdir, char **files, int n, const sketch_opts_t *o)
+{
 	for (int i = 0; i < n; ++i)
+		check_input(files[i]);
+	const int ns = o->asone ? 1 : n;
+	sample_keys_t *sm = xcalloc((size_t)ns, sizeof *sm);
+	kssd_infile_meta_t *mt = xcalloc((size_t)ns, sizeof *mt);
+	const bool report = o->counts.percentile > 0.0 || o->counts.reads_qc;
+	if (o->asone) {
+		sketch_asone(files, n, o, &sm[0], &mt[0]);
+		if (report)
+			kssd_print_count_filter(stderr, &o->counts, "as-one sketch", &sm[0].filter, 0);
+	} else {
+		int bad = -1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(o->threads)
+		for (int i = 0; i < n; ++i) {
+			if (kssd_collect_fastx_file(files[i], o->drfold, &sm[i].keys) != 0 ||
+			    (o->meta && kssd_infile_meta_for_path(files[i], &mt[i]) != 0)) {
+#pragma omp critical
+				bad = i;
+				continue;
+			}
+			finish_sample(&sm[i], o);
+		}
+		if (bad >= 0)
+			err(EXIT_FAILURE, "%s", files[bad]);
+		if (report)
+			for (int i = 0; i < n; ++i)
+				kssd_print_count_filter(stderr, &o->counts, files[i], &sm[i].filter, 1);
+	}
+	stage("inputs sketched");
+	uint64_t *index = xmalloc(((size_t)ns + 1) * sizeof *index);
+	index[0] = 0;
+	for (int i = 0; i < ns; ++i)
+		index[i + 1] = index[i] + sm[i].keys.n;
+	uint64_t *keys = xmalloc((size_t)(index[ns] ? index[ns] : 1) * sizeof *keys);
+	uint32_t *ab = o->abundance ? xmalloc((size_t)(index[ns] ? index[ns] : 1) * sizeof *ab) : NULL;
+	kssd_qc_stat_t *qc = xcalloc((size_t)ns, sizeof *qc);
+	for (int i = 0; i < ns; ++i) {
+		if (sm[i].keys.n) {
+			memcpy(keys + index[i], sm[i].keys.a, sm[i].keys.n * sizeof *keys);
+			if (ab)
+				memcpy(ab + index[i], sm[i].counts, sm[i].keys.n * sizeof *ab);
+		}
+		qc[i] = sm[i].filter.qc;
+		kssd_u64vec_free(&sm[i].keys);
+		free(sm[i].counts);
+	}
+	char (*names)[KSSD_PATHLEN] = xcalloc((size_t)ns, KSSD_PATHLEN);
+	for (int i = 0; i < ns; ++i)
 		strcpy(names[i], files[i]);
+	if (o->sample_name)
+		snprintf(names[0], KSSD_PATHLEN, "%s", o->sample_name); /* name_asone_sample */
 	/* lcofiles.stat of the coden pattern (command_sketch.c:989-1005,
-	 * command_sketch_wrapper.c:920-930) */
+	 * command_sketch_wrapper.c:1186-1209) */
 	kssd_stat_t st = {0};
-	st.hash_id = kssd_sketching_id(11, 0, 0, drfold);
-	st.koc = false;
-	st.conflict = conflict;
+	st.hash_id = kssd_sketching_id(11, 0, 0, o->drfold);
+	st.koc = o->abundance;
+	st.conflict = o->conflict;
 	st.coden_len = 11;
 	st.klen = 32;
 	st.hclen = 0;
 	st.holen = 0;
-	st.drfold = drfold;
-	st.infile_num = n;
+	st.drfold = o->drfold;
+	st.infile_num = ns;
+	const size_t nk = (size_t)index[ns];
 	if (kssd_sketchdir_write(outdir, &st, names, keys, index) != 0 ||
-	    (meta && kssd_write_infile_meta(outdir, mt, n) != 0))
+	    (o->meta && kssd_write_infile_meta(outdir, mt, ns) != 0) ||
+	    (ab && kssd_write_dir_file(outdir, KSSD_ABUND_FILE, ab, nk * sizeof *ab) != 0) ||
+	    ((o->abundance || o->counts.reads_qc) &&
+	     kssd_write_dir_file(outdir, KSSD_QC_FILE, qc, (size_t)ns * sizeof *qc) != 0))
 		err(EXIT_FAILURE, "%s", outdir);
 	stage("sketch written");
-	free(sk);
+	free(sm);
 	free(mt);
 	free(index);
 	free(keys);
+	free(ab);
+	free(qc);
 	free(names);
 	return 0;
 }
 
 static int cmd_sketch(int argc, char **argv)
 {
-	enum { S_CONFLICT = 400, S_NOMETA, S_HELP, S_NOT_OFFERED, S_HOST_INDEX, S_DEVICE };
+	enum { S_CONFLICT = 400, S_NOMETA, S_HELP, S_NOT_OFFERED, S_HOST_INDEX, S_DEVICE, S_NPCT, S_NCAP,
+	       S_READSQC, S_SAMPLE_NAME };
 	static const struct option longopts[] = {
 		{"idx", required_argument, NULL, 'i'},
 		{"index", required_argument, NULL, 'i'},
@@ -1847,20 +2071,27 @@ static int cmd_sketch(int argc, char **argv)
 		{"host-index", no_argument, NULL, S_HOST_INDEX},
 		{"device", required_argument, NULL, S_DEVICE},
 		{"help", no_argument, NULL, S_HELP},
-		{"abundance", no_argument, NULL, S_NOT_OFFERED},
-		{"asone", no_argument, NULL, S_NOT_OFFERED},
+		{"abundance", no_argument, NULL, 'A'},
+		{"asone", no_argument, NULL, 'a'},
+		{"sample-name", required_argument, NULL, S_SAMPLE_NAME},
+		{"LstKmerOcrs", required_argument, NULL, 'n'},
+		{"npercentile", required_argument, NULL, S_NPCT},
+		{"ncap", required_argument, NULL, S_NCAP},
+		{"readsQC", no_argument, NULL, S_READSQC},
 		{"splitmfa", no_argument, NULL, S_NOT_OFFERED},
 		{"anno", no_argument, NULL, S_NOT_OFFERED},
-		{"readsQC", no_argument, NULL, S_NOT_OFFERED},
+		{"separate", no_argument, NULL, S_NOT_OFFERED},
+		{"position", no_argument, NULL, S_NOT_OFFERED},
 		{"pipecmd", required_argument, NULL, S_NOT_OFFERED},
 		{NULL, 0, NULL, 0},
 	};
 	const char *dir = NULL, *outdir = NULL, *list = NULL;
-	int threads = omp_get_max_threads(), drfold = 8, device = 0;
-	bool conflict = false, meta = true, host_index = false;
+	int threads = omp_get_max_threads(), device = 0;
+	bool host_index = false;
+	sketch_opts_t so = {.drfold = 8, .meta = true, .counts = {.min_count = 1}};
 	optind = 1;
 	int c;
-	while ((c = getopt_long(argc, argv, "i:o:l:f:Tp:aA", longopts, NULL)) != -1) {
+	while ((c = getopt_long(argc, argv, "i:o:l:f:Tp:aAn:", longopts, NULL)) != -1) {
 		switch (c) {
 		case 'i':
 			dir = optarg;
@@ -1872,15 +2103,42 @@ static int cmd_sketch(int argc, char **argv)
 			list = optarg;
 			break;
 		case 'f':
-			drfold = (int)parse_int("-f/--DimRdcFold", optarg, 0, 32);
+			so.drfold = (int)parse_int("-f/--DimRdcFold", optarg, 0, 24);
 			break;
 		case 'T':
 			break;
 		case S_CONFLICT:
-			conflict = true;
+			so.conflict = true;
 			break;
 		case S_NOMETA:
-			meta = false;
+			so.meta = false;
+			break;
+		case 'A':
+			so.abundance = true;
+			so.counts.record_qc = 1;
+			break;
+		case 'a':
+			so.asone = true;
+			break;
+		case S_SAMPLE_NAME:
+			for (const char *q = optarg; *q; ++q)
+				if ((unsigned char)*q < 0x20 || *q == 0x7f)
+					usage_error("--sample-name must not contain control characters");
+			so.sample_name = optarg;
+			break;
+		case 'n':
+			so.counts.min_count = (int)parse_int("-n/--LstKmerOcrs", optarg, 1, 65536);
+			break;
+		case S_NPCT:
+			so.counts.percentile = parse_double("--npercentile", optarg, 0.0, 1.0);
+			break;
+		case S_NCAP:
+			so.counts.cap = (int)parse_int("--ncap", optarg, 0, 65536);
+			break;
+		case S_READSQC:
+			so.counts.reads_qc = 1; /* implies a lower cutoff of at least 2 */
+			if (so.counts.min_count < 2)
+				so.counts.min_count = 2;
 			break;
 		case 'p':
 			threads = (int)parse_int("-p/--threads", optarg, 1, 65536);
@@ -1892,22 +2150,24 @@ static int cmd_sketch(int argc, char **argv)
 			device = (int)parse_int("--device", optarg, 0, 1024);
 			break;
 		case S_HELP:
-			printf("Usage: %s sketch -o OUTDIR [-f DRFOLD] [--conflict] [--nocomputemeta] [-l LIST] FILE...\n"
+			printf("Usage: %s sketch -o OUTDIR [-f DRFOLD] [--conflict] [--nocomputemeta] [-l LIST]\n"
+			       "              [-A] [--asone [--sample-name NAME]] [-n MIN] [--npercentile P]\n"
+			       "              [--ncap N] [--readsQC] FILE...\n"
 			       "       %s sketch -i SKETCH_DIR [--device N | --host-index [-p THREADS]]\n"
 			       "Sketch FASTA/FASTQ[.gz] files with the coden context-object pattern (one sample per\n"
-			       "file), or build SKETCH_DIR/%s and its .unique_bits (on the GPU;\n"
-			       "--host-index: on the host).\n",
+			       "file, or all pooled with --asone; -A keeps per-k-mer counts in comblco.a), or build\n"
+			       "SKETCH_DIR/%s and its .unique_bits (on the GPU; --host-index: on the host).\n",
 			       PROG, PROG, KSSD_SORTED_INDEX_FILE);
 			return 0;
-		case 'a':
-		case 'A':
 		case S_NOT_OFFERED:
-			usage_error("%s is not offered by this front end (one sample per FASTA/FASTQ file)", argv[optind - 1]);
+			usage_error("%s is not offered by this front end", argv[optind - 1]);
 		default:
 			fprintf(stderr, "Try `%s %s --help' for more information.\n", PROG, g_cmd);
 			exit(EX_USAGE);
 		}
 	}
+	if (so.sample_name && (!so.asone || dir))
+		usage_error("--sample-name requires raw-input --asone mode without --splitmfa or --separate");
 	if (dir) {
 		if (outdir || optind != argc)
 			usage_error("-i builds the index of an existing sketch; it takes no inputs");
@@ -1921,9 +2181,10 @@ static int cmd_sketch(int argc, char **argv)
 	char **files = sketch_inputs(argc - optind, argv + optind, list, &n);
 	if (n == 0)
 		usage_error("no input files");
-	if (n >= (1 << 20))
+	if (!so.asone && n >= (1 << 20))
 		errx(EXIT_FAILURE, "%d inputs exceed the %d genomes a sketch can index", n, 1 << 20);
-	const int rc = sketch_files(outdir, files, n, drfold, conflict, meta, threads);
+	so.threads = threads;
+	const int rc = sketch_files(outdir, files, n, &so);
 	for (int i = 0; i < n; ++i)
 		free(files[i]);
 	free(files);

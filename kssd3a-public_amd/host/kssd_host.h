@@ -12,6 +12,7 @@
 #define KSSD_HOST_H
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #include "kssdhip.h"
 
@@ -49,6 +50,51 @@ void kssd_ascii_to_codes(const char *s, size_t len, uint8_t *codes);
 /* Sketch a FASTA/plain-sequence file as ONE sample (all records pooled,
  * the window restarting at each record).  Returns 0 or -1 (errno set). */
 int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd_u64vec_t *out);
+
+/* The keys of every k-mer window of one FASTA/FASTQ[.gz] file appended to
+ * out as they occur (unsorted, repeats kept): the input of the counted
+ * read-set modes.  Returns 0 or -1 (errno set). */
+int kssd_collect_fastx_file(const char *path, int drfold, kssd_u64vec_t *out);
+/* Sort v ascending and merge repeats; *counts (malloc'd, NULL when v is
+ * empty) holds each key's number of occurrences (build_kv_from_vec,
+ * command_sketch.c:4156-4187).  0 or -1 (allocation). */
+int kssd_count_keys(kssd_u64vec_t *v, uint32_t **counts);
+/* Drop every context carried by more than one object from sorted keys,
+ * keeping counts (may be NULL) aligned (kssdlib_sort.c:358-386). */
+void kssd_drop_conflicts(uint64_t *keys, uint32_t *counts, size_t *n);
+
+/* ---- k-mer count filters and read QC (command_sketch.c:56-768) ----------- */
+#define KSSD_QC_RANGE_VALID 1u
+#define KSSD_QC_RANGE_APPLIED 2u
+/* lcofiles.qc record (dim_sketch_qc_stat_t, global_basic.h:156-162) */
+typedef struct {
+	uint32_t flags, lower, upper, mode;
+} kssd_qc_stat_t;
+typedef struct {
+	int min_count;     /* -n/--LstKmerOcrs (1 = off) */
+	double percentile; /* --npercentile (0 = off) */
+	int cap;           /* --ncap (0 = off) */
+	int reads_qc;      /* --readsQC */
+	int record_qc;     /* infer the QC range without applying it (-A) */
+} kssd_count_opts_t;
+typedef struct {
+	uint32_t raw_cutoff, cutoff, upper, mode;
+	int used_reads_qc;
+	kssd_qc_stat_t qc;
+} kssd_count_filter_t;
+/* Whether the options need per-key counts (sketch_needs_kmer_counts,
+ * command_sketch.c:51-54, with -A given as record_qc). */
+int kssd_count_opts_need_counts(const kssd_count_opts_t *o);
+/* apply_kmer_count_filters (command_sketch.c:717-767) on sorted keys with
+ * their counts: lower cutoff from -n and the weighted count percentile,
+ * the inferred read-QC range (recorded with -A, applied with --readsQC),
+ * the cap, then keys outside [cutoff, upper] dropped in place. */
+kssd_count_filter_t kssd_apply_count_filters(uint64_t *keys, uint32_t *counts, size_t *n,
+					     const kssd_count_opts_t *o);
+/* The stderr line print_kmer_count_filter_result (command_sketch.c:690-715)
+ * writes for --npercentile / --readsQC runs. */
+void kssd_print_count_filter(FILE *fp, const kssd_count_opts_t *o, const char *name,
+			     const kssd_count_filter_t *r, int leading_newline);
 
 /* GET_SKETCHING_ID(hclen, holen, iolen, drfold, FILTER), command_sketch.h:49-52
  * and command_sketch.c:771.  The coden pattern passes (NUM_CODENS, 0, 0, drfold)

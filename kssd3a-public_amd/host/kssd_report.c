@@ -172,6 +172,10 @@ const char kssd_detail_header[] =
 	"Qry\tRef\tANI\tDistance\tConfidence\tSelected_metric\tXnY_ctx\tQry_align_fraction\t"
 	"blastn_Qry_align_fraction\tRef_align_fraction\tblastn_Ref_align_fraction\tN_diff_obj\t"
 	"N_diff_obj_section\tN_mut2_ctx\tRef_annotation";
+const char kssd_coverage_header[] =
+	"Coverage_estimator\tCoverage_ref_markers\tCoverage_observed_markers\tCoverage_observed_fraction\t"
+	"Coverage_count_sum\tEstimated_depth\tPositive_mean_depth\tPositive_median_depth\tCount_CV\t"
+	"Estimated_abundance_mass\tEstimated_abundance_fraction\tAbundance_fraction_scope";
 
 static inline bool af_pass(const kssd_report_opts_t *o, double afq, double afr)
 {
@@ -455,19 +459,188 @@ static inline double bounded_af(double v)
 	return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
 }
 
+/* append_coverage_double, command_ani.c:1118-1126 */
+static void put_cov(kssd_buf_t *b, double v)
+{
+	if (!isfinite(v))
+		kssd_buf_printf(b, "\tNA");
+	else if (fabs(v) >= 1000.0)
+		kssd_buf_printf(b, "\t%.3f", v);
+	else
+		kssd_buf_printf(b, "\t%.6g", v);
+}
+
+static const char *coverage_label(unsigned char status) /* coverage_estimator_label, :1106-1116 */
+{
+	switch (status) {
+	case 0: return "NA:not_computed";
+	case 2: return "NA:query_missing_comblco.a";
+	case 3: return "NA:ref_read_error";
+	case 4: return "refdb_unique_ctxobj_bitset";
+	default: return "reported_unique_ctxobj_sum";
+	}
+}
+
 void kssd_append_detail_row(kssd_buf_t *b, const kssd_report_opts_t *o, const char *qry_name,
 			    const char *ref_name, const kssd_row_t *r, const char *annotation)
 {
-	/* append_unified_detail_row, command_ani.c:1381-1402 */
+	/* append_unified_detail_row, command_ani.c:1381-1432 */
 	if (!passes_close_filter(r, o))
 		return;
 	const char *metric = o->unique_ctx ? (r->unique_ctx_selected ? "NaiveUnique" : "NaiveFull")
 					   : metric_names[effective_id(r, o) - 1];
-	kssd_buf_printf(b, "%s\t%s\t%lf\t%lf\t%s\t%s\t%d\t%f\t%f\t%f\t%f\t%d\t%d\t%d\t%s\n", qry_name, ref_name,
+	kssd_buf_printf(b, "%s\t%s\t%lf\t%lf\t%s\t%s\t%d\t%f\t%f\t%f\t%f\t%d\t%d\t%d\t%s", qry_name, ref_name,
 			r->selected_ani, r->metric, row_confidence(r, o), metric, r->XnY_ctx, r->af_qry,
 			bounded_af(r->blastn_af_qry), r->af_ref, bounded_af(r->blastn_af_ref), r->N_diff_obj,
 			r->N_diff_obj_section, r->N_mut2_ctx,
 			(annotation && annotation[0]) ? annotation : "NA");
+	if (o->estimate_coverage) {
+		const unsigned char st = r->coverage_status;
+		kssd_buf_printf(b, "\t%s", coverage_label(st));
+		if (st == 1 || st == 4) {
+			kssd_buf_printf(b, "\t%u\t%u", r->cov_markers, r->cov_observed);
+			put_cov(b, r->cov_observed_fraction);
+			put_cov(b, r->cov_count_sum);
+			put_cov(b, r->est_depth);
+			put_cov(b, r->pos_mean_depth);
+			put_cov(b, r->pos_median_depth);
+			put_cov(b, r->count_cv);
+			put_cov(b, r->abundance_mass);
+			put_cov(b, r->abundance_fraction);
+			kssd_buf_printf(b, "\t%s", st == 4 ? "refdb" : "reported_rows");
+		} else {
+			for (int k = 0; k < 11; ++k)
+				kssd_buf_printf(b, "\tNA");
+		}
+	}
+	kssd_buf_printf(b, "\n");
+}
+
+/* ------------------------------------------------------------ coverage */
+static size_t lower_u64(const uint64_t *a, size_t n, uint64_t v)
+{
+	size_t lo = 0, hi = n;
+	while (lo < hi) {
+		const size_t mid = lo + (hi - lo) / 2;
+		if (a[mid] < v)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+static int cmp_u64_asc(const void *a, const void *b)
+{
+	const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+	return (x > y) - (x < y);
+}
+
+static int cmp_u32_asc(const void *a, const void *b)
+{
+	const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+	return (x > y) - (x < y);
+}
+
+/* the record (ctxgid, obj) is in the sorted index with its unique bit set
+ * (coverage_unique_bitset_get_key, command_ani.c:2261-2280) */
+static bool refdb_marker(const kssd_coverage_src_t *s, uint64_t ctxgid, uint32_t obj)
+{
+	size_t lo = 0, hi = s->ref_nrec;
+	while (lo < hi) {
+		const size_t mid = lo + (hi - lo) / 2;
+		const kssd_ctxgidobj_t *v = &s->ref_records[mid];
+		if (v->ctxgid < ctxgid || (v->ctxgid == ctxgid && v->obj < obj))
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo < s->ref_nrec && s->ref_records[lo].ctxgid == ctxgid && s->ref_records[lo].obj == obj &&
+	       (s->ref_unique_bits[lo >> 3] >> (lo & 7)) & 1u;
+}
+
+void kssd_populate_coverage(kssd_row_t *rows, size_t n, const kssd_coverage_src_t *s)
+{
+	if (!n)
+		return;
+	if (!s->qcounts) {
+		for (size_t i = 0; i < n; ++i)
+			rows[i].coverage_status = 2;
+		return;
+	}
+	const bool refdb = s->ref_records && s->ref_unique_bits;
+	uint64_t *pool = NULL;
+	size_t npool = 0;
+	if (!refdb) { /* keys of every reported row's reference, sorted */
+		for (size_t i = 0; i < n; ++i)
+			npool += (size_t)(s->ref_index[rows[i].rn + 1] - s->ref_index[rows[i].rn]);
+		pool = malloc((npool ? npool : 1) * sizeof *pool);
+		if (!pool)
+			abort();
+		size_t w = 0;
+		for (size_t i = 0; i < n; ++i) {
+			const uint64_t b = s->ref_index[rows[i].rn], e = s->ref_index[rows[i].rn + 1];
+			memcpy(pool + w, s->ref_keys + b, (size_t)(e - b) * sizeof *pool);
+			w += (size_t)(e - b);
+		}
+		qsort(pool, npool, sizeof *pool, cmp_u64_asc);
+	}
+	const uint64_t objmask = (s->obj_bits >= 64) ? UINT64_MAX : ((1ULL << s->obj_bits) - 1ULL);
+	double total = 0.0;
+	for (size_t i = 0; i < n; ++i) {
+		kssd_row_t *r = &rows[i];
+		const uint64_t b = s->ref_index[r->rn], e = s->ref_index[r->rn + 1];
+		uint32_t *pos = malloc((size_t)(e - b ? e - b : 1) * sizeof *pos);
+		if (!pos)
+			abort();
+		uint32_t markers = 0, seen = 0;
+		double sum = 0.0, sumsq = 0.0;
+		for (uint64_t k = b; k < e; ++k) {
+			const uint64_t key = s->ref_keys[k];
+			bool marker;
+			if (refdb) {
+				const uint64_t ctx = s->obj_bits >= 64 ? 0 : key >> s->obj_bits;
+				marker = refdb_marker(s, (ctx << 20) | r->rn, (uint32_t)(key & objmask));
+			} else {
+				const size_t lo = lower_u64(pool, npool, key);
+				marker = lo < npool && pool[lo] == key && (lo + 1 == npool || pool[lo + 1] != key);
+			}
+			if (!marker)
+				continue;
+			markers++;
+			const size_t qi = lower_u64(s->qkeys, s->qn, key);
+			if (qi < s->qn && s->qkeys[qi] == key) {
+				const uint32_t c = s->qcounts[qi];
+				pos[seen++] = c;
+				sum += (double)c;
+				sumsq += (double)c * (double)c;
+			}
+		}
+		r->coverage_status = refdb ? 4 : 1;
+		r->cov_markers = markers;
+		r->cov_observed = seen;
+		r->cov_observed_fraction = markers ? (double)seen / (double)markers : 0.0;
+		r->cov_count_sum = sum;
+		r->est_depth = markers ? sum / (double)markers : 0.0;
+		r->pos_mean_depth = seen ? sum / (double)seen : 0.0;
+		if (seen) {
+			qsort(pos, seen, sizeof *pos, cmp_u32_asc);
+			r->pos_median_depth = (seen & 1u) ? pos[seen / 2u]
+							  : ((double)pos[seen / 2u - 1u] + (double)pos[seen / 2u]) / 2.0;
+			const double mean = r->pos_mean_depth;
+			double var = sumsq / (double)seen - mean * mean;
+			if (var < 0.0 && var > -1e-9)
+				var = 0.0;
+			r->count_cv = mean > 0.0 && var > 0.0 ? sqrt(var) / mean : 0.0;
+		}
+		r->abundance_mass = sum;
+		total += sum;
+		free(pos);
+	}
+	if (total > 0.0)
+		for (size_t i = 0; i < n; ++i)
+			rows[i].abundance_fraction = rows[i].abundance_mass / total;
+	free(pool);
 }
 
 /* ------------------------------------------------------------ sorting */
@@ -663,7 +836,13 @@ void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const 
 	} else {
 		out_n = select_topN_sorted(ids, npass, ntop);
 	}
-	/* printing, ani_block_print_internal command_ani.c:3470-3524 */
+	/* printing, ani_block_print_internal command_ani.c:3470-3539; with
+	 * coverage the printed rows are held back until their coverage is known */
+	const bool cov = o->estimate_coverage && c->cov;
+	kssd_row_t *held = cov ? malloc((size_t)(out_n > 0 ? out_n : 1) * sizeof *held) : NULL;
+	size_t nheld = 0;
+	if (cov && !held)
+		abort();
 	const int qsz = (int)c->qry_ctx;
 	for (int n = 0; n < out_n; ++n) {
 		const uint32_t j = ids[n].id;
@@ -683,8 +862,19 @@ void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const 
 						  row.unique_ctx_selected ? c->uobj[j] : c->obj[j]);
 			kssd_finalize_row(&row, o);
 		}
-		if (row.selected_ani >= o->anicut)
+		if (row.selected_ani < o->anicut)
+			continue;
+		if (cov)
+			held[nheld++] = row;
+		else
 			kssd_append_detail_row(out, o, qry_name, ref_names[j], &row, anno_at(ref_anno, j));
+	}
+	if (cov) {
+		kssd_populate_coverage(held, nheld, c->cov);
+		for (size_t r = 0; r < nheld; ++r)
+			kssd_append_detail_row(out, o, qry_name, ref_names[held[r].rn], &held[r],
+					       anno_at(ref_anno, held[r].rn));
+		free(held);
 	}
 	free(ranks);
 	free(ids);
@@ -723,11 +913,13 @@ void kssd_emit_index_matrix_row(kssd_buf_t *out, const kssd_report_opts_t *o,
 
 static void emit_sorted(kssd_buf_t *out, const kssd_report_opts_t *o, kssd_row_t *rows, size_t n,
 			size_t limit, const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
-			char (*ref_anno)[KSSD_PATHLEN])
+			char (*ref_anno)[KSSD_PATHLEN], const kssd_coverage_src_t *cov)
 {
 	stable_sort(rows, n, sizeof *rows, cmp_row_desc);
 	if (limit < n)
 		n = limit;
+	if (o->estimate_coverage && cov) /* :2634-2650, :2983-2998 */
+		kssd_populate_coverage(rows, n, cov);
 	for (size_t i = 0; i < n; ++i)
 		kssd_append_detail_row(out, o, qry_name, ref_names[rows[i].rn], &rows[i],
 				       anno_at(ref_anno, rows[i].rn));
@@ -767,7 +959,7 @@ void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const
 			continue;
 		rows[n++] = row;
 	}
-	emit_sorted(out, o, rows, n, o->ntop > 0 ? (size_t)o->ntop : n, qry_name, ref_names, ref_anno);
+	emit_sorted(out, o, rows, n, o->ntop > 0 ? (size_t)o->ntop : n, qry_name, ref_names, ref_anno, c->cov);
 	free(rows);
 }
 
@@ -802,7 +994,7 @@ void kssd_emit_hash_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const k
 			continue;
 		rows[n++] = row;
 	}
-	emit_sorted(out, o, rows, n, n, qry_name, ref_names, ref_anno);
+	emit_sorted(out, o, rows, n, n, qry_name, ref_names, ref_anno, NULL); /* no coverage here */
 	free(rows);
 }
 

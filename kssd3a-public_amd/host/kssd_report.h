@@ -65,6 +65,7 @@ typedef struct {
 	int drfold;          /* ani_model_drfold */
 	unsigned obj_bits;   /* Bitslen.obj */
 	unsigned ctx_bits;   /* Bitslen.ctx */
+	bool estimate_coverage; /* append the coverage columns */
 } kssd_report_opts_t;
 
 /* ani_row_t (command_ani.c:1060-1085), the fields that reach the output */
@@ -79,6 +80,13 @@ typedef struct {
 	bool af_pass;
 	bool unique_ctx_selected;
 	int XnY_ctx, N_diff_obj, N_diff_obj_section, N_mut2_ctx;
+	/* --estimate-coverage: status 0 not computed, 1 markers unique among the
+	 * reported rows, 2 query without counts, 4 markers from the reference's
+	 * unique-marker bits */
+	unsigned char coverage_status;
+	uint32_t cov_markers, cov_observed;
+	double cov_observed_fraction, cov_count_sum, est_depth, pos_mean_depth, pos_median_depth, count_cv,
+		abundance_mass, abundance_fraction;
 } kssd_row_t;
 
 /* Load the calibrated model's tables (idempotent, thread-safe); 0 or -1. */
@@ -111,6 +119,29 @@ void kssd_append_detail_row(kssd_buf_t *b, const kssd_report_opts_t *o, const ch
 			    const char *ref_name, const kssd_row_t *r, const char *annotation);
 extern const char kssd_detail_header[];
 
+extern const char kssd_coverage_header[];
+
+/* What --estimate-coverage reads for one query (command_ani.c:2203-2455):
+ * the query's keys and their read counts (NULL: the query has no
+ * comblco.a), every reference sample's keys, and optionally the reference's
+ * sorted index with its unique-marker bits. */
+typedef struct {
+	const uint64_t *qkeys;
+	const uint32_t *qcounts;
+	size_t qn;
+	const uint64_t *ref_keys, *ref_index;
+	const kssd_ctxgidobj_t *ref_records; /* NULL: markers from the reported rows */
+	const uint8_t *ref_unique_bits;
+	size_t ref_nrec;
+	unsigned obj_bits;
+} kssd_coverage_src_t;
+/* populate_selected_coverage_for_query_arrays (command_ani.c:2318-2455):
+ * a reported row's markers are its reference's keys that are unique — by
+ * the unique-marker bits when given, else among the keys of all n reported
+ * rows — and its depth figures come from the query's counts of the markers
+ * it holds; abundance fractions are shares of the rows' summed counts. */
+void kssd_populate_coverage(kssd_row_t *rows, size_t n, const kssd_coverage_src_t *src);
+
 /* Rows with qry/ref context totals and counts for one query against all refs. */
 typedef struct {
 	const kssd_ctx_mut2_t *ctx;  /* [nref] */
@@ -122,6 +153,7 @@ typedef struct {
 	int nref;
 	const kssd_infile_meta_t *qry_meta;  /* may be NULL */
 	const kssd_infile_meta_t *ref_meta;  /* [nref] or NULL */
+	const kssd_coverage_src_t *cov;      /* --estimate-coverage inputs (NULL: not computed) */
 } kssd_query_counts_t;
 
 /* Index path (mem_eff) detail rows of one query, ranked and filtered as the

@@ -173,6 +173,58 @@ void kssd_ascii_to_codes(const char *s, size_t len, uint8_t *codes)
 
 int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd_u64vec_t *out)
 {
+	if (kssd_collect_fastx_file(path, drfold, out) != 0)
+		return -1;
+	kssd_finalize_keys(out, keep_conflict);
+	return 0;
+}
+
+int kssd_count_keys(kssd_u64vec_t *v, uint32_t **counts)
+{
+	*counts = NULL;
+	if (!v->n)
+		return 0;
+	radix_sort_u64(v->a, v->n);
+	uint32_t *c = malloc(v->n * sizeof *c);
+	if (!c)
+		return -1;
+	size_t w = 0;
+	c[0] = 1;
+	for (size_t i = 1; i < v->n; ++i) {
+		if (v->a[i] == v->a[w]) {
+			c[w]++;
+		} else {
+			v->a[++w] = v->a[i];
+			c[w] = 1;
+		}
+	}
+	v->n = w + 1;
+	*counts = c;
+	return 0;
+}
+
+void kssd_drop_conflicts(uint64_t *keys, uint32_t *counts, size_t *n)
+{
+	/* remove_ctx_with_conflict_obj, kssdlib_sort.c:358-386: a context kept
+	 * only when one object carries it; counts stay aligned */
+	size_t i = 0, w = 0;
+	while (i < *n) {
+		size_t c = 1;
+		while (i + c < *n && ((keys[i] ^ keys[i + c]) >> KSSD_OBJ_BITS) == 0)
+			++c;
+		if (c == 1) {
+			keys[w] = keys[i];
+			if (counts)
+				counts[w] = counts[i];
+			++w;
+		}
+		i += c;
+	}
+	*n = w;
+}
+
+int kssd_collect_fastx_file(const char *path, int drfold, kssd_u64vec_t *out)
+{
 	gzFile fp = gzopen(path, "rb");
 	if (!fp)
 		return -1;
@@ -239,7 +291,6 @@ int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd
 	gzclose(fp);
 	free(codes);
 	free(line);
-	kssd_finalize_keys(out, keep_conflict);
 	return 0;
 }
 

@@ -312,46 +312,44 @@ static int cmp_u64(const void *a, const void *b)
 /* Sequence length of every FASTA record (all characters of its sequence
  * lines; a trailing CR is not counted), empty records skipped
  * (append_asm_lengths_for_path, command_sketch.c:1205-1227). */
-static int fasta_record_lengths(const char *path, uint64_t **lens, size_t *n)
+int kssd_fasta_record_lengths(const char *path, uint64_t **lens, size_t *n, size_t *cap)
 {
 	gzFile fp = gzopen(path, "rb");
 	if (!fp)
 		return -1;
-	size_t cap = 64;
-	uint64_t *v = malloc(cap * sizeof *v);
 	char *line = malloc(1 << 16);
-	if (!v || !line) {
+	if (!line) {
 		gzclose(fp);
-		free(v);
-		free(line);
 		return -1;
 	}
-	size_t k = 0;
 	uint64_t cur = 0;
 	bool in_rec = false, at_line_start = true, in_header = false;
-	while (gzgets(fp, line, 1 << 16)) {
-		size_t l = strlen(line);
+	for (;;) {
+		const bool got = gzgets(fp, line, 1 << 16) != NULL;
+		size_t l = got ? strlen(line) : 0;
 		const bool ends_line = l && line[l - 1] == '\n';
+		const bool header = got && !in_header && at_line_start && l && line[0] == '>';
+		if ((!got || header) && in_rec && cur) {
+			if (*n == *cap) {
+				const size_t c2 = *cap ? *cap * 2 : 64;
+				uint64_t *w = realloc(*lens, c2 * sizeof *w);
+				if (!w)
+					abort();
+				*lens = w;
+				*cap = c2;
+			}
+			(*lens)[(*n)++] = cur;
+		}
+		if (!got)
+			break;
 		if (ends_line)
 			--l;
 		if (ends_line && l && line[l - 1] == '\r')
 			--l;
 		if (in_header) { /* the rest of a header longer than the buffer */
 			in_header = !ends_line;
-			at_line_start = ends_line;
-			continue;
-		}
-		if (at_line_start && l && line[0] == '>') {
+		} else if (header) {
 			in_header = !ends_line;
-			if (in_rec && cur) {
-				if (k == cap) {
-					uint64_t *w = realloc(v, (cap *= 2) * sizeof *v);
-					if (!w)
-						abort();
-					v = w;
-				}
-				v[k++] = cur;
-			}
 			in_rec = true;
 			cur = 0;
 		} else if (in_rec) {
@@ -359,60 +357,64 @@ static int fasta_record_lengths(const char *path, uint64_t **lens, size_t *n)
 		}
 		at_line_start = ends_line;
 	}
-	if (in_rec && cur) {
-		if (k == cap) {
-			uint64_t *w = realloc(v, (cap + 1) * sizeof *v);
-			if (!w)
-				abort();
-			v = w;
-		}
-		v[k++] = cur;
-	}
 	gzclose(fp);
 	free(line);
-	*lens = v;
-	*n = k;
 	return 0;
+}
+
+void kssd_infile_meta_from_lengths(uint64_t *len, size_t n, int fmt, int create_type, unsigned flags,
+				   kssd_infile_meta_t *out)
+{
+	memset(out, 0, sizeof *out);
+	out->meta_fmt_version = 1;
+	out->infile_fmt = (int8_t)fmt;
+	out->create_type = (int8_t)create_type;
+	out->infile_flags = (uint8_t)flags;
+	out->asm_level = fmt == 2 ? 0.0f : -1.0f;
+	if (!n)
+		return;
+	/* running mean and squared deviations in input order, then the median
+	 * of the sorted lengths */
+	uint64_t total = 0;
+	double mean = 0.0, m2 = 0.0;
+	for (size_t i = 0; i < n; ++i) {
+		total += len[i];
+		const double x = (double)len[i];
+		const double d = x - mean;
+		mean += d / (double)(i + 1);
+		m2 += d * (x - mean);
+	}
+	qsort(len, n, sizeof *len, cmp_u64);
+	out->total_length_bp = total;
+	out->record_count = n > UINT32_MAX ? UINT32_MAX : (uint32_t)n;
+	const uint64_t med = (n & 1) ? len[n / 2] : (len[n / 2 - 1] + len[n / 2]) / 2;
+	out->median_length_bp = med > UINT32_MAX ? UINT32_MAX : (uint32_t)med;
+	if (mean > 0.0 && n > 1)
+		out->length_cv = (float)(sqrt(m2 / (double)n) / mean);
+	if (fmt == 1 && total > 0)
+		out->asm_level = (float)((double)len[n - 1] / (double)total);
+}
+
+unsigned kssd_infile_flags_from_path(const char *path)
+{
+	return (strcmp(path, "-") == 0 ? 0x02u : 0u) | (is_compressed(path) ? 0x04u : 0u);
 }
 
 int kssd_infile_meta_for_path(const char *path, kssd_infile_meta_t *out)
 {
-	memset(out, 0, sizeof *out);
 	const int fmt = kssd_infile_fmt_from_path(path);
-	out->meta_fmt_version = 1;
-	out->infile_fmt = (int8_t)fmt;
-	out->create_type = 0;
-	out->infile_flags = (uint8_t)((strcmp(path, "-") == 0 ? 0x02 : 0) | (is_compressed(path) ? 0x04 : 0));
-	out->asm_level = fmt == 2 ? 0.0f : -1.0f;
-	if (fmt != 1)
-		return 0;
 	uint64_t *len = NULL;
-	size_t n = 0;
-	if (fasta_record_lengths(path, &len, &n) != 0)
+	size_t n = 0, cap = 0;
+	if (fmt == 1 && kssd_fasta_record_lengths(path, &len, &n, &cap) != 0)
 		return -1;
-	if (n) {
-		/* infile_meta_from_lengths, command_sketch.c:1134-1174 */
-		uint64_t total = 0;
-		double mean = 0.0, m2 = 0.0;
-		for (size_t i = 0; i < n; ++i) {
-			total += len[i];
-			const double x = (double)len[i];
-			const double d = x - mean;
-			mean += d / (double)(i + 1);
-			m2 += d * (x - mean);
-		}
-		qsort(len, n, sizeof *len, cmp_u64);
-		out->total_length_bp = total;
-		out->record_count = n > UINT32_MAX ? UINT32_MAX : (uint32_t)n;
-		const uint64_t med = (n & 1) ? len[n / 2] : (len[n / 2 - 1] + len[n / 2]) / 2;
-		out->median_length_bp = med > UINT32_MAX ? UINT32_MAX : (uint32_t)med;
-		if (mean > 0.0 && n > 1)
-			out->length_cv = (float)(sqrt(m2 / (double)n) / mean);
-		if (total > 0)
-			out->asm_level = (float)((double)len[n - 1] / (double)total);
-	}
+	kssd_infile_meta_from_lengths(len, n, fmt, 0, kssd_infile_flags_from_path(path), out);
 	free(len);
 	return 0;
+}
+
+int kssd_write_dir_file(const char *dir, const char *name, const void *data, size_t n)
+{
+	return write_file(dir, name, data, n);
 }
 
 int kssd_write_infile_meta(const char *dir, const kssd_infile_meta_t *meta, int n)

@@ -18,6 +18,9 @@ extern "C" {
 #define KSSD_SORTED_INDEX_FILE "sortedcomb_ctxgid64obj32"
 #define KSSD_ANNO_FILE "lcofiles.anno"
 #define KSSD_META_FILE "lcofiles.infilemeta"
+#define KSSD_ABUND_FILE "comblco.a"
+#define KSSD_QC_FILE "lcofiles.qc"
+#define KSSD_UNIQUE_BITS_FILE "sortedcomb_ctxgid64obj32.unique_bits"
 
 /* dim_sketch_stat_t, global_basic.h:138-149 (32 bytes) */
 typedef struct {
@@ -59,6 +62,17 @@ int kssd_write_index_files(const char *dir, const kssd_ctxgidobj_t *rec, size_t 
  * FASTA record lengths summarised, format and flags from the file name.
  * 0, or -1 when a FASTA file cannot be read. */
 int kssd_infile_meta_for_path(const char *path, kssd_infile_meta_t *out);
+/* The summary infile_meta_from_lengths (command_sketch.c:1134-1173) makes of
+ * record lengths (n may be 0; lens is sorted in place). */
+void kssd_infile_meta_from_lengths(uint64_t *lens, size_t n, int fmt, int create_type, unsigned flags,
+				   kssd_infile_meta_t *out);
+/* Lengths of the non-empty records of a FASTA file appended to *lens
+ * (grown with realloc; *n entries, *cap capacity); 0 or -1. */
+int kssd_fasta_record_lengths(const char *path, uint64_t **lens, size_t *n, size_t *cap);
+/* infile_flags_from_path without a pipe command (command_sketch.c:1106-1116) */
+unsigned kssd_infile_flags_from_path(const char *path);
+/* Write one whole file of a sketch directory; 0 or -1 (errno set). */
+int kssd_write_dir_file(const char *dir, const char *name, const void *data, size_t n);
 /* lcofiles.infilemeta */
 int kssd_write_infile_meta(const char *dir, const kssd_infile_meta_t *meta, int n);
 /* Input file kind by name (isOK_fmt_infile with fasta_fmt / fastq_fmt,

@@ -71,7 +71,8 @@ class ReportOpts(ctypes.Structure):
                 ("raw_output", ctypes.c_bool), ("max_pdist_set", ctypes.c_bool), ("max_pdist", D),
                 ("max_diff_obj_section", I), ("e", I), ("ntop", I), ("ctxcut", I),
                 ("afcut", ctypes.c_float), ("anicut", ctypes.c_float), ("unique_ctx", ctypes.c_bool),
-                ("d", ctypes.c_bool), ("drfold", I), ("obj_bits", U), ("ctx_bits", U)]
+                ("d", ctypes.c_bool), ("drfold", I), ("obj_bits", U), ("ctx_bits", U),
+                ("estimate_coverage", ctypes.c_bool)]
 
     @classmethod
     def defaults(cls, **kw):
@@ -79,7 +80,7 @@ class ReportOpts(ctypes.Structure):
         o = cls(fmt=0, s=1, v=False, unassembled=False, unified_metric=False, ignoreconflict=False,
                 raw_output=False, max_pdist_set=False, max_pdist=0.0, max_diff_obj_section=-1, e=1,
                 ntop=0, ctxcut=3, afcut=0.5, anicut=0.95, unique_ctx=False, d=False, drfold=8,
-                obj_bits=20, ctx_bits=44)
+                obj_bits=20, ctx_bits=44, estimate_coverage=False)
         for k, v in kw.items():
             setattr(o, k, v)
         return o
@@ -87,7 +88,7 @@ class ReportOpts(ctypes.Structure):
 
 class QueryCounts(ctypes.Structure):
     _fields_ = [("ctx", V), ("obj", V), ("uctx", V), ("uobj", V), ("qry_ctx", U32), ("ref_ctx", V),
-                ("nref", I), ("qry_meta", V), ("ref_meta", V)]
+                ("nref", I), ("qry_meta", V), ("ref_meta", V), ("cov", V)]
 
 
 class Buf(ctypes.Structure):

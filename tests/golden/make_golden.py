@@ -311,6 +311,92 @@ def make_sketch():
     run(["sketch", "--conflict", "-o", "f8c", *files], cwd=out)
 
 
+def write_reads(path, rng, genome, n_reads, read_len=100, err=0.01):
+    """Single-end reads of `genome` (both strands) with i.i.d. base errors,
+    as 4-line FASTQ with a constant quality line ('@' never starts one)."""
+    comp = np.array([3, 2, 1, 0], dtype=np.uint8)
+    letters = np.frombuffer(b"ACGT", dtype=np.uint8)
+    recs = []
+    for r in range(n_reads):
+        s = int(rng.integers(0, len(genome) - read_len))
+        read = genome[s:s + read_len].copy()
+        if rng.random() < 0.5:
+            read = comp[read[::-1]]
+        hit = rng.random(read_len) < err
+        read[hit] = (read[hit] + rng.integers(1, 4, size=int(hit.sum()), dtype=np.uint8)) % 4
+        recs.append(b"@r%d\n%s\n+\n%s\n" % (r, letters[read].tobytes(), b"I" * read_len))
+    data = b"".join(recs)
+    if str(path).endswith(".gz"):
+        import gzip
+        with gzip.GzipFile(path, "wb", mtime=0) as fh:
+            fh.write(data)
+    else:
+        Path(path).write_bytes(data)
+
+
+READSET_SKETCHES = {  # output dir: arguments of `sketch` (inputs under readsets/)
+    "rs_A": ["-A", "-f", "2", "a_1.fq.gz", "a_2.fq.gz", "b.fq.gz", "c.fq"],
+    "rs_plain": ["-f", "2", "a_1.fq.gz", "b.fq.gz"],
+    "rs_asone": ["--asone", "--conflict", "-A", "-f", "2", "a_1.fq.gz", "a_2.fq.gz"],
+    "rs_n2": ["-n", "2", "-f", "2", "a_1.fq.gz", "b.fq.gz", "c.fq"],
+    "rs_qc": ["--readsQC", "-A", "-f", "2", "a_1.fq.gz", "b.fq.gz"],
+    "rs_pct": ["--npercentile", "0.1", "--ncap", "3", "-A", "-f", "2", "a_1.fq.gz", "b.fq.gz"],
+    "rs_named": ["--asone", "-A", "--sample-name", "sampleA", "-f", "2", "a_1.fq.gz", "a_2.fq.gz", "ga.fa"],
+    "asm_asone": ["--asone", "-f", "2", "ga.fa", "gb.fa"],
+    "refs": ["-f", "2", "ga.fa", "gb.fa", "gc.fa"],
+}
+COVERAGE_CASES = {  # golden: (ani arguments, environment), sketches from READSET_SKETCHES
+    "cov_index": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5"],
+                  {"KSSD3A_FORCE_REF_INDEX": "1"}),
+    "cov_stream": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5"], {}),
+    "cov_single": (["-r", "refs", "--qraw", "rs_asone", "--estimate-coverage", "-n", "0.5"], {}),
+    "cov_noidx": (["-r", "refs_noidx", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5"], {}),
+    "cov_noab": (["-r", "refs", "--qraw", "rs_plain", "--estimate-coverage", "-n", "0.5"], {}),
+    "cov_top1": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5", "-N", "1"],
+                 {"KSSD3A_FORCE_REF_INDEX": "1"}),
+}
+
+
+def make_readsets():
+    """Read sets (FASTQ, gzip and plain) of two related 20 kbp genomes and
+    the reference's sketches of them in the counted modes (-A, --asone, -n,
+    --readsQC, --npercentile/--ncap, --sample-name), plus its
+    --estimate-coverage detail output against the genomes' sketch."""
+    out = HERE / "readsets"
+    shutil.rmtree(out, ignore_errors=True)
+    out.mkdir(parents=True)
+    rng = np.random.default_rng(4242)
+    ga = rng.integers(0, 4, size=20_000, dtype=np.uint8)
+    gb = mutate(rng, ga, 0.02)
+    gc = rng.integers(0, 4, size=15_000, dtype=np.uint8)
+    for name, g in (("ga", ga), ("gb", gb), ("gc", gc)):
+        write_fasta(out / f"{name}.fa", name, g)
+    write_reads(out / "a_1.fq.gz", rng, ga, 1000)
+    write_reads(out / "a_2.fq.gz", rng, ga, 1000)
+    write_reads(out / "b.fq.gz", rng, gb, 3000)
+    write_reads(out / "c.fq", rng, ga, 400)
+    for d, args in READSET_SKETCHES.items():
+        run(["sketch", "-o", d, *args], cwd=out)
+    shutil.copytree(out / "refs", out / "refs_noidx")
+    run(["sketch", "-i", "refs"], cwd=out)
+    for name, (args, env) in COVERAGE_CASES.items():
+        (out / f"{name}.tsv").write_text(run(["ani", *args], env=env, cwd=out))
+    # the sketch directories are kept as SHA-256 digests of their files (the
+    # tests rebuild them from the inputs above and compare)
+    import hashlib
+    import json
+    digests = {}
+    for d in [*READSET_SKETCHES, "refs_noidx"]:
+        for f in sorted((out / d).iterdir()):
+            if f.name == "lcofiles.stat":  # name slots may hold bytes past the terminator
+                sk = kt.read_sketch_dir(out / d)
+                digests[f"{d}/{f.name}"] = {"header": f.read_bytes()[:32].hex(), "names": sk.names}
+            else:
+                digests[f"{d}/{f.name}"] = hashlib.sha256(f.read_bytes()).hexdigest()
+        shutil.rmtree(out / d)
+    (out / "sketch_digests.json").write_text(json.dumps(digests, indent=1, sort_keys=True) + "\n")
+
+
 def main():
     if not BIN.exists():
         raise SystemExit(f"{BIN} missing: run `make -C oracle ref` first")
@@ -326,6 +412,7 @@ def main():
     make_matrix_edges()
     make_matrix_clusters()
     make_sketch()
+    make_readsets()
     print("golden fixtures written under", HERE)
 
 

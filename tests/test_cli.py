@@ -54,7 +54,7 @@ def copy_dir(src, dst, with_index=True):
     (["ani", "--reflist", "a", "--qrylist", "b"], "not offered"),
     (["ani", "-r", "x", "-q", "y", "--metric", "bogus"], "unknown --metric"),
     (["ani", "-r", "x", "-q", "y", "--format", "wide"], "--format must be"),
-    (["sketch", "--asone", "-o", "out", "a.fa"], "not offered"),
+    (["sketch", "--splitmfa", "-o", "out", "a.fa"], "not offered"),
     (["sketch", "-o", "out"], "no input files"),
     (["sketch"], "give -o OUTDIR"),
 ])
