@@ -25,6 +25,7 @@
 #include <limits.h>
 #include <omp.h>
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1163,16 +1164,20 @@ static int cmd_ani(int argc, char **argv)
  * sparse edge report (matrix_compute_edges, :992-1014, and the
  * vote-nominated indexed scan, :1140-1180 with pairwise_graph.c:1148-1316),
  * with metric expressions; --format clusters gives the connected components
- * of those edges with their best representative (matrix_write_component_report,
- * :1055-1118).  The dedup-plan report is not offered.
+ * of those edges with their best representative, and --format dedup-plan the
+ * keep/remove plan of a greedy or full-linkage grouping of them, with the
+ * kept samples' distance matrix (matrix_write_component_report, :1055-1118;
+ * pairwise_graph.c:792-973).
  */
 typedef struct {
 	char refdir[PATH_MAX], qrydir[PATH_MAX], outf[PATH_MAX], edge_outf[PATH_MAX];
+	char keep_outf[PATH_MAX], remove_outf[PATH_MAX], keep_matrix_outf[PATH_MAX], keep_idmap_outf[PATH_MAX];
 	int metric;
 	kssd_metric_expr_t expr;
-	int format; /* 0 auto, 1 full, 2 triangle, 3 edges, 4 clusters */
+	int format; /* 0 auto, 1 full, 2 triangle, 3 edges, 4 clusters, 5 dedup-plan */
+	int full_linkage, keep_phylip;
 	double e, diag, cut, max_afcut;
-	bool d, cut_set;
+	bool d, cut_set, max_afcut_set, strategy_set;
 	uint32_t ctxcut, max_ctx_freq, min_votes, sample_step;
 	int p, device;
 } matrix_args_t;
@@ -1195,7 +1200,8 @@ static double matrix_cell_value(const void *arg, int q, int r, const kssd_ctx_mu
 static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 {
 	enum { M_FORMAT = 300, M_DIAGV, M_DEVICE, M_HELP, M_NOT_OFFERED, M_SPARSE, M_CUT, M_MAXAF, M_CTXCUT,
-	       M_MAXFREQ, M_MINVOTES, M_STEP, M_EDGEOUT };
+	       M_MAXFREQ, M_MINVOTES, M_STEP, M_EDGEOUT, M_KEEPOUT, M_REMOVEOUT, M_KMOUT, M_KMFMT, M_KMIDMAP,
+	       M_STRATEGY };
 	static const struct option longopts[] = {
 		{"ref", required_argument, NULL, 'r'},
 		{"query", required_argument, NULL, 'q'},
@@ -1216,10 +1222,13 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		{"index-min-votes", required_argument, NULL, M_MINVOTES},
 		{"index-sample-step", required_argument, NULL, M_STEP},
 		{"edge-out", required_argument, NULL, M_EDGEOUT},
-		{"keep-out", required_argument, NULL, M_NOT_OFFERED},
-		{"remove-out", required_argument, NULL, M_NOT_OFFERED},
-		{"keep-matrix-out", required_argument, NULL, M_NOT_OFFERED},
-		{"dedup-strategy", required_argument, NULL, M_NOT_OFFERED},
+		{"control", required_argument, NULL, M_CUT},
+		{"keep-out", required_argument, NULL, M_KEEPOUT},
+		{"remove-out", required_argument, NULL, M_REMOVEOUT},
+		{"keep-matrix-out", required_argument, NULL, M_KMOUT},
+		{"keep-matrix-format", required_argument, NULL, M_KMFMT},
+		{"keep-matrix-idmap", required_argument, NULL, M_KMIDMAP},
+		{"dedup-strategy", required_argument, NULL, M_STRATEGY},
 		{"query-sketch-list", required_argument, NULL, M_NOT_OFFERED},
 		{"matrix-format", required_argument, NULL, M_NOT_OFFERED},
 		{"glist", required_argument, NULL, M_NOT_OFFERED},
@@ -1252,20 +1261,18 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 			a->metric = a->expr.count > 1 ? -2 : a->expr.metrics[0];
 			break;
 		case M_FORMAT:
-			if (strcmp(optarg, "full") == 0)
+			if (!strcmp(optarg, "full") || !strcmp(optarg, "matrix"))
 				a->format = 1;
-			else if (strcmp(optarg, "triangle") == 0)
+			else if (!strcmp(optarg, "triangle") || !strcmp(optarg, "tri"))
 				a->format = 2;
 			else if (!strcmp(optarg, "edges") || !strcmp(optarg, "edge") || !strcmp(optarg, "sparse"))
 				a->format = 3;
 			else if (!strcmp(optarg, "clusters") || !strcmp(optarg, "cluster") || !strcmp(optarg, "components"))
 				a->format = 4;
 			else if (!strcmp(optarg, "dedup-plan") || !strcmp(optarg, "dedup"))
-				usage_error("--format %s is not offered by this front end (full, triangle, edges and clusters "
-					    "are)",
-					    optarg);
+				a->format = 5;
 			else
-				usage_error("--format must be full, triangle, edges, sparse, clusters, or dedup-plan");
+				usage_error("--format must be one of full, triangle, edges/sparse, clusters, or dedup-plan");
 			break;
 		case 'e':
 			a->e = parse_double("-e/--exception", optarg, 0.0, 1e300);
@@ -1289,12 +1296,14 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		case M_SPARSE:
 			a->format = 3;
 			break;
+		case 'c':
 		case M_CUT:
 			a->cut = parse_double("--cut", optarg, 0.0, 1e300);
 			a->cut_set = true;
 			break;
 		case M_MAXAF:
 			a->max_afcut = parse_double("--max-afcut", optarg, 0.0, 1.0);
+			a->max_afcut_set = true;
 			break;
 		case M_CTXCUT:
 			a->ctxcut = (uint32_t)parse_int("--ctxcut", optarg, 0, INT32_MAX);
@@ -1311,18 +1320,52 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		case M_EDGEOUT:
 			copy_path(a->edge_outf, sizeof a->edge_outf, "--edge-out", optarg);
 			break;
+		case M_KEEPOUT:
+			copy_path(a->keep_outf, sizeof a->keep_outf, "--keep-out", optarg);
+			break;
+		case M_REMOVEOUT:
+			copy_path(a->remove_outf, sizeof a->remove_outf, "--remove-out", optarg);
+			break;
+		case M_KMOUT:
+			copy_path(a->keep_matrix_outf, sizeof a->keep_matrix_outf, "--keep-matrix-out", optarg);
+			break;
+		case M_KMIDMAP:
+			copy_path(a->keep_idmap_outf, sizeof a->keep_idmap_outf, "--keep-matrix-idmap", optarg);
+			break;
+		case M_KMFMT: /* parse_keep_matrix_format, command_matrix_wrapper.c:202-212 */
+			if (!strcmp(optarg, "tsv") || !strcmp(optarg, "table"))
+				a->keep_phylip = 0;
+			else if (!strcmp(optarg, "phylip") || !strcmp(optarg, "phy") || !strcmp(optarg, "fillip"))
+				a->keep_phylip = 1;
+			else
+				usage_error("--keep-matrix-format must be one of tsv or phylip");
+			break;
+		case M_STRATEGY: /* pairwise_dedup_strategy_from_string, pairwise_graph.c:118-141 */
+			a->strategy_set = true;
+			if (!strcmp(optarg, "greedy") || !strcmp(optarg, "rep") || !strcmp(optarg, "representative") ||
+			    !strcmp(optarg, "rep-centric") || !strcmp(optarg, "representative-centric"))
+				a->full_linkage = 0;
+			else if (!strcmp(optarg, "full") || !strcmp(optarg, "full-linkage") ||
+				 !strcmp(optarg, "full_linkage") || !strcmp(optarg, "complete") ||
+				 !strcmp(optarg, "complete-linkage") || !strcmp(optarg, "complete_linkage") ||
+				 !strcmp(optarg, "clique"))
+				a->full_linkage = 1;
+			else
+				usage_error("--dedup-strategy must be one of greedy, full-linkage, complete-linkage, or clique");
+			break;
 		case M_HELP:
 			printf("Usage: %s matrix [-m METRIC] [--format full|triangle] [-e E] [-d | --diagonal-value V] "
 			       "[-o FILE] (SKETCH | -q SKETCH | -r REF -q QRY)\n"
-			       "       %s matrix --format edges|clusters --cut D [-m METRIC|EXPR] [--ctxcut N]\n"
+			       "       %s matrix --format edges|clusters|dedup-plan --cut D [-m METRIC|EXPR] [--ctxcut N]\n"
 			       "             [--max-afcut F] [--index-max-ctx-freq N] [--index-min-votes N]\n"
 			       "             [--index-sample-step N] [--edge-out FILE] [-o FILE] (SKETCH | -r REF -q QRY)\n"
+			       "       dedup-plan: [--dedup-strategy greedy|full-linkage] [--keep-out F] [--remove-out F]\n"
+			       "             [--keep-matrix-out F [--keep-matrix-format tsv|phylip [--keep-matrix-idmap F]]]\n"
 			       "METRIC: ctx-naive (default), ctx-moe, p_dist, mash, aaf; EXPR: metrics joined by & or |\n"
-			       "(edges only).\n",
+			       "(edges, clusters and dedup plans).\n",
 			       PROG, PROG);
 			exit(0);
 		case 'g':
-		case 'c':
 		case M_NOT_OFFERED:
 			usage_error("%s is not offered by this front end (distance matrices only)", argv[optind - 1]);
 		default:
@@ -1339,13 +1382,62 @@ static void parse_matrix(int argc, char **argv, matrix_args_t *a)
 		usage_error("missing sketch input; provide one sketch argument, -q/--query, or --query-sketch-list");
 	if (a->refdir[0] && a->format == 2)
 		usage_error("--format triangle is only valid for one sketch");
-	if (a->metric == -2 && a->format != 3 && a->format != 4)
+	if (a->metric == -2 && a->format != 3 && a->format != 4 && a->format != 5)
 		usage_error("combined --metric expressions are only valid for --sparse/--format edges, clusters, or "
 			    "dedup-plan");
-	if ((a->format == 3 || a->format == 4) && !a->cut_set)
+	if (a->format >= 3 && !a->cut_set)
 		usage_error("--sparse/--format edges, clusters, or dedup-plan requires --cut");
-	if (a->format == 4 && a->refdir[0])
+	if ((a->format == 4 || a->format == 5) && a->refdir[0])
 		usage_error("--format clusters/dedup-plan requires one sketch, not -r/-q");
+	/* the rest of the end-of-arguments checks, command_matrix_wrapper.c:433-509 */
+	if ((a->keep_outf[0] || a->remove_outf[0] || a->keep_matrix_outf[0] || a->keep_idmap_outf[0]) && a->format != 5)
+		usage_error("--keep-out/--remove-out/--keep-matrix-out/--keep-matrix-idmap require --format dedup-plan");
+	if (a->strategy_set && a->format != 5)
+		usage_error("--dedup-strategy requires --format dedup-plan");
+	if (a->keep_phylip && !a->keep_matrix_outf[0])
+		usage_error("--keep-matrix-format requires --keep-matrix-out");
+	if (a->keep_idmap_outf[0] && !a->keep_matrix_outf[0])
+		usage_error("--keep-matrix-idmap requires --keep-matrix-out");
+	if (a->keep_idmap_outf[0] && !a->keep_phylip)
+		usage_error("--keep-matrix-idmap requires --keep-matrix-format phylip");
+	static const struct {
+		size_t a, b;
+		const char *msg;
+	} differ[] = {
+		{offsetof(matrix_args_t, keep_outf), offsetof(matrix_args_t, remove_outf),
+		 "--keep-out and --remove-out must be different files"},
+		{offsetof(matrix_args_t, keep_outf), offsetof(matrix_args_t, outf), "--keep-out must differ from --outfile"},
+		{offsetof(matrix_args_t, remove_outf), offsetof(matrix_args_t, outf),
+		 "--remove-out must differ from --outfile"},
+		{offsetof(matrix_args_t, keep_matrix_outf), offsetof(matrix_args_t, outf),
+		 "--keep-matrix-out must differ from --outfile"},
+		{offsetof(matrix_args_t, edge_outf), offsetof(matrix_args_t, outf), "--edge-out must differ from --outfile"},
+		{offsetof(matrix_args_t, keep_outf), offsetof(matrix_args_t, edge_outf),
+		 "--keep-out must differ from --edge-out"},
+		{offsetof(matrix_args_t, remove_outf), offsetof(matrix_args_t, edge_outf),
+		 "--remove-out must differ from --edge-out"},
+		{offsetof(matrix_args_t, keep_matrix_outf), offsetof(matrix_args_t, edge_outf),
+		 "--keep-matrix-out must differ from --edge-out"},
+		{offsetof(matrix_args_t, keep_matrix_outf), offsetof(matrix_args_t, keep_outf),
+		 "--keep-matrix-out must differ from --keep-out"},
+		{offsetof(matrix_args_t, keep_matrix_outf), offsetof(matrix_args_t, remove_outf),
+		 "--keep-matrix-out must differ from --remove-out"},
+		{offsetof(matrix_args_t, keep_idmap_outf), offsetof(matrix_args_t, outf),
+		 "--keep-matrix-idmap must differ from --outfile"},
+		{offsetof(matrix_args_t, keep_idmap_outf), offsetof(matrix_args_t, edge_outf),
+		 "--keep-matrix-idmap must differ from --edge-out"},
+		{offsetof(matrix_args_t, keep_idmap_outf), offsetof(matrix_args_t, keep_outf),
+		 "--keep-matrix-idmap must differ from --keep-out"},
+		{offsetof(matrix_args_t, keep_idmap_outf), offsetof(matrix_args_t, remove_outf),
+		 "--keep-matrix-idmap must differ from --remove-out"},
+		{offsetof(matrix_args_t, keep_idmap_outf), offsetof(matrix_args_t, keep_matrix_outf),
+		 "--keep-matrix-idmap must differ from --keep-matrix-out"},
+	};
+	for (size_t i = 0; i < sizeof differ / sizeof differ[0]; ++i) {
+		const char *x = (const char *)a + differ[i].a, *y = (const char *)a + differ[i].b;
+		if (x[0] && y[0] && strcmp(x, y) == 0)
+			usage_error("%s", differ[i].msg);
+	}
 }
 
 static int matrix_self(const matrix_args_t *a)
@@ -1681,15 +1773,270 @@ static int uf_find(int *parent, int x)
 	return x;
 }
 
-/* Connected components of the accepted edges (pairwise_build_components,
- * pairwise_graph.c:975-1030, or the indexed scan's unions), each with its
- * best member by kssd_quality_compare (pairwise_component_result_finalize,
- * :682-712); cluster ids number the components in order of their first
- * member (matrix_write_component_report, command_matrix.c:1055-1118).
- * --edge-out: the accepted edges, as the scan (indexed) or the pair loop
- * (i < j, queried as i against j) reports them. */
-static int matrix_clusters(const matrix_args_t *a)
+/* Samples best first (pairwise_quality_order, pairwise_graph.c:719-790): a
+ * top-down merge sort whose merge takes the left head when it is better by
+ * kssd_quality_compare, ties going to the smaller index. */
+static bool quality_better(const kssd_sketchdir_t *s, int a, int b)
 {
+	const int c = kssd_quality_compare(s->meta, s->index, a, b);
+	return c > 0 || (c == 0 && a < b);
+}
+
+static void quality_sort(int *idx, int *tmp, int lo, int hi, const kssd_sketchdir_t *s)
+{
+	if (hi - lo <= 1)
+		return;
+	const int mid = lo + (hi - lo) / 2;
+	quality_sort(idx, tmp, lo, mid, s);
+	quality_sort(idx, tmp, mid, hi, s);
+	int i = lo, j = mid, k = lo;
+	while (i < mid && j < hi)
+		tmp[k++] = quality_better(s, idx[i], idx[j]) ? idx[i++] : idx[j++];
+	while (i < mid)
+		tmp[k++] = idx[i++];
+	while (j < hi)
+		tmp[k++] = idx[j++];
+	memcpy(idx + lo, tmp + lo, (size_t)(hi - lo) * sizeof *idx);
+}
+
+static int cmp_u64_key(const void *a, const void *b)
+{
+	const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+	return (x > y) - (x < y);
+}
+
+/* Dedup groups.  Greedy (pairwise_greedy_dedup_from_edges, pairwise_graph.c:
+ * 792-860): in quality order each sample not yet removed keeps itself and
+ * removes every neighbour not yet removed.  Full linkage (pairwise_complete_
+ * linkage_dedup_from_edges, :895-973): each unassigned sample in quality order
+ * opens a group and takes, again in quality order, every unassigned sample
+ * linked to all members so far.  group[i] is the representative of i. */
+static void dedup_groups(int n, const edge_list_t *all, const int *order, bool full_linkage, int *group,
+			 int *gsize)
+{
+	for (int i = 0; i < n; ++i) {
+		group[i] = -1;
+		gsize[i] = 0;
+	}
+	if (!full_linkage) {
+		size_t *off = xcalloc((size_t)n + 1, sizeof *off);
+		for (size_t k = 0; k < all->n; ++k) {
+			off[all->v[k].q + 1]++;
+			off[all->v[k].r + 1]++;
+		}
+		for (int i = 0; i < n; ++i)
+			off[i + 1] += off[i];
+		int *adj = xmalloc((off[n] ? off[n] : 1) * sizeof *adj);
+		size_t *cur = xmalloc(((size_t)n + 1) * sizeof *cur);
+		memcpy(cur, off, ((size_t)n + 1) * sizeof *cur);
+		for (size_t k = 0; k < all->n; ++k) {
+			adj[cur[all->v[k].q]++] = all->v[k].r;
+			adj[cur[all->v[k].r]++] = all->v[k].q;
+		}
+		for (int oi = 0; oi < n; ++oi) {
+			const int rep = order[oi];
+			if (group[rep] >= 0)
+				continue;
+			group[rep] = rep;
+			gsize[rep] = 1;
+			for (size_t t = off[rep]; t < off[rep + 1]; ++t) {
+				const int x = adj[t];
+				if (group[x] >= 0)
+					continue;
+				group[x] = rep;
+				gsize[rep]++;
+			}
+		}
+		free(adj);
+		free(cur);
+		free(off);
+		return;
+	}
+	/* edges as sorted a * n + b keys, both directions */
+	uint64_t *key = xmalloc((2 * all->n + 1) * sizeof *key);
+	for (size_t k = 0; k < all->n; ++k) {
+		key[2 * k] = (uint64_t)all->v[k].q * (uint64_t)n + (uint64_t)all->v[k].r;
+		key[2 * k + 1] = (uint64_t)all->v[k].r * (uint64_t)n + (uint64_t)all->v[k].q;
+	}
+	qsort(key, 2 * all->n, sizeof *key, cmp_u64_key);
+	int *members = xmalloc((size_t)(n ? n : 1) * sizeof *members);
+	for (int oi = 0; oi < n; ++oi) {
+		const int rep = order[oi];
+		if (group[rep] >= 0)
+			continue;
+		group[rep] = rep;
+		gsize[rep] = 1;
+		members[0] = rep;
+		int nm = 1;
+		for (int ci = 0; ci < n; ++ci) {
+			const int x = order[ci];
+			if (group[x] >= 0)
+				continue;
+			bool linked = true;
+			for (int m = 0; m < nm && linked; ++m) {
+				const uint64_t want = (uint64_t)x * (uint64_t)n + (uint64_t)members[m];
+				linked = bsearch(&want, key, 2 * all->n, sizeof *key, cmp_u64_key) != NULL;
+			}
+			if (!linked)
+				continue;
+			group[x] = rep;
+			gsize[rep]++;
+			members[nm++] = x;
+		}
+	}
+	free(members);
+	free(key);
+}
+
+/* matrix_meta_valid (command_matrix.c:155-162) */
+static bool meta_valid(const kssd_sketchdir_t *s, int i)
+{
+	return s->meta && s->meta[i].meta_fmt_version == 1 && s->meta[i].total_length_bp > 0;
+}
+
+/* Why rep was kept over sample (matrix_rep_rule, command_matrix.c:182-231). */
+static const char *rep_rule(const kssd_sketchdir_t *s, int sample, int rep)
+{
+	if (sample == rep)
+		return "self";
+	const bool sv = meta_valid(s, sample), rv = meta_valid(s, rep);
+	if (sv != rv)
+		return rv ? "valid_metadata" : "sample_valid_metadata";
+	if (sv) {
+		const kssd_infile_meta_t *a = &s->meta[sample], *b = &s->meta[rep];
+		const bool af = a->infile_fmt == 1, bf = b->infile_fmt == 1;
+		if (af != bf)
+			return bf ? "fasta_input" : "sample_fasta_input";
+		if (a->asm_level != b->asm_level)
+			return b->asm_level > a->asm_level ? "higher_asm_level" : "sample_higher_asm_level";
+		if (a->total_length_bp != b->total_length_bp)
+			return b->total_length_bp > a->total_length_bp ? "longer_total_length" : "sample_longer_total_length";
+		if (a->record_count != b->record_count) {
+			if (a->record_count == 0)
+				return "sample_missing_record_count";
+			if (b->record_count == 0)
+				return "rep_missing_record_count";
+			return b->record_count < a->record_count ? "fewer_records" : "sample_fewer_records";
+		}
+		if (a->median_length_bp != b->median_length_bp)
+			return b->median_length_bp > a->median_length_bp ? "longer_median_length"
+									: "sample_longer_median_length";
+		if (a->length_cv != b->length_cv)
+			return b->length_cv < a->length_cv ? "lower_length_cv" : "sample_lower_length_cv";
+	}
+	const uint64_t se = s->index[sample + 1] - s->index[sample], re = s->index[rep + 1] - s->index[rep];
+	if (se != re)
+		return re > se ? "larger_sketch_entries" : "sample_larger_sketch_entries";
+	return rep < sample ? "earlier_sample_order" : "sample_earlier_sample_order";
+}
+
+/* matrix_print_meta_field (command_matrix.c:233-244) */
+static void put_meta_fields(kssd_buf_t *b, const kssd_sketchdir_t *s, int i)
+{
+	if (!meta_valid(s, i)) {
+		kssd_buf_printf(b, "\tNA\tNA\tNA\tNA\tNA\tNA");
+		return;
+	}
+	const kssd_infile_meta_t *m = &s->meta[i];
+	kssd_buf_printf(b, "\t%s\t%.6g\t%llu\t%u\t%u\t%.6g",
+			m->infile_fmt == 1 ? "FASTA" : (m->infile_fmt == 2 ? "FASTQ" : "NA"), (double)m->asm_level,
+			(unsigned long long)m->total_length_bp, m->record_count, m->median_length_bp,
+			(double)m->length_cv);
+}
+
+/* The kept samples' dense distance matrix (matrix_write_keep_matrix and its
+ * indexed form, command_matrix.c:785-960): rows and columns in sample order,
+ * each row counted on the GPU against every sample, cells the metric
+ * expression's value (the diagonal value on the diagonal); TSV with "%lf"
+ * cells or PHYLIP with S%09d ids and " %.12g" cells, and the id map. */
+static void write_keep_matrix(const matrix_args_t *a, const kssd_sketchdir_t *s, const kssd_ctxgidobj_t *rec,
+			      const uint32_t *tot, const int *kept, int nk)
+{
+	if (a->keep_idmap_outf[0]) {
+		FILE *fp = open_or_stdout(a->keep_idmap_outf);
+		fprintf(fp, "id\tsample\n");
+		for (int k = 0; k < nk; ++k)
+			fprintf(fp, "S%09d\t%s\n", k + 1, s->names[kept[k]]);
+		close_out(fp);
+	}
+	FILE *fp = open_or_stdout(a->keep_matrix_outf);
+	kssd_buf_t b = {0};
+	if (a->keep_phylip) {
+		kssd_buf_printf(&b, "%d\n", nk);
+	} else {
+		for (int k = 0; k < nk; ++k)
+			kssd_buf_printf(&b, "\t%s", s->names[kept[k]]);
+		kssd_buf_printf(&b, "\n");
+	}
+	put_buf(fp, &b);
+	const int n = s->n;
+	if (nk > 0) {
+		kssdhip_index *ix = gpu_index(rec, s->index[n], n, s->obj_bits);
+		const int blk = block_queries(nk, n, false);
+		kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * n * sizeof *ctx);
+		kssd_obj_section_t *obj = xmalloc((size_t)blk * n * sizeof *obj);
+		kssd_buf_t *rows = xcalloc((size_t)blk, sizeof *rows);
+		uint64_t *skidx = xmalloc(((size_t)blk + 1) * sizeof *skidx);
+		for (int k0 = 0; k0 < nk; k0 += blk) {
+			const int nb = nk - k0 < blk ? nk - k0 : blk;
+			/* the block's kept samples gathered into one contiguous run of keys */
+			skidx[0] = 0;
+			for (int i = 0; i < nb; ++i)
+				skidx[i + 1] = skidx[i] + (s->index[kept[k0 + i] + 1] - s->index[kept[k0 + i]]);
+			uint64_t *keys = xmalloc((size_t)(skidx[nb] ? skidx[nb] : 1) * sizeof *keys);
+			for (int i = 0; i < nb; ++i)
+				memcpy(keys + skidx[i], s->keys + s->index[kept[k0 + i]],
+				       (size_t)(skidx[i + 1] - skidx[i]) * sizeof *keys);
+			hip_check(kssdhip_count_block(ix, keys, skidx, nb, 0, 0, ctx, obj, NULL, NULL),
+				  "GPU counting failed");
+			free(keys);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
+			for (int i = 0; i < nb; ++i) {
+				const int q = kept[k0 + i];
+				if (a->keep_phylip)
+					kssd_buf_printf(&rows[i], "S%09d", k0 + i + 1); /* "%-10s" of a 10-character id */
+				else
+					kssd_buf_printf(&rows[i], "%s", s->names[q]);
+				for (int k = 0; k < nk; ++k) {
+					const int r = kept[k];
+					const size_t c = (size_t)i * n + r;
+					const double v = q == r ? a->diag
+							: kssd_matrix_context_expr_value(&a->expr, a->e, ctx[c].num_ctx, obj[c].diff_obj,
+											 obj[c].diff_obj_section, ctx[c].num_mut2_ctx,
+											 tot[q], tot[r], s->stat.drfold, s->obj_bits,
+											 s->ctx_bits);
+					kssd_buf_printf(&rows[i], a->keep_phylip ? " %.12g" : "\t%lf", v);
+				}
+				kssd_buf_printf(&rows[i], "\n");
+			}
+			for (int i = 0; i < nb; ++i) {
+				put_buf(fp, &rows[i]);
+				kssd_buf_free(&rows[i]);
+			}
+		}
+		free(skidx);
+		free(rows);
+		free(ctx);
+		free(obj);
+		kssdhip_index_close(ix);
+	}
+	kssd_buf_free(&b);
+	close_out(fp);
+}
+
+/* Clusters (format 4): the connected components of the accepted edges
+ * (pairwise_build_components, pairwise_graph.c:975-1030, or the indexed
+ * scan's unions), each with its best member by kssd_quality_compare
+ * (pairwise_component_result_finalize, :682-712).  Dedup plans (format 5):
+ * the greedy or full-linkage groups of the same edges, one keep/remove row
+ * per sample with the rule that chose its representative, --keep-out /
+ * --remove-out name lists and the kept samples' matrix.  Cluster ids number
+ * the groups in order of their first member (matrix_write_component_report,
+ * command_matrix.c:1055-1118).  --edge-out: the accepted edges, as the scan
+ * (indexed) or the pair loop (i < j, queried as i against j) reports them. */
+static int matrix_components(const matrix_args_t *a)
+{
+	const bool dedup = a->format == 5;
 	kssd_sketchdir_t s;
 	const bool have = kssd_file_in_dir(a->qrydir, KSSD_SORTED_INDEX_FILE);
 	open_dir(a->qrydir, KSSD_SK_KEYS | KSSD_SK_META | (have ? KSSD_SK_RECORDS : 0), &s);
@@ -1697,34 +2044,48 @@ static int matrix_clusters(const matrix_args_t *a)
 	const int n = s.n;
 	const bool indexed = have && !s.stat.conflict;
 	uint32_t *tot = entry_or_run_totals(&s, s.stat.conflict, false);
-	edge_list_t *rows = self_edges(a, &s, tot, indexed);
+	matrix_args_t eff = *a;
+	if (dedup && !a->max_afcut_set)
+		eff.max_afcut = 0.8; /* compute_triangle, command_matrix.c:1314-1316 */
+	edge_list_t *rows = self_edges(&eff, &s, tot, indexed);
 	edge_list_t all = ordered_edges(rows, n, indexed);
-	int *parent = xmalloc((size_t)(n ? n : 1) * sizeof *parent);
+	int *group = xmalloc((size_t)(n ? n : 1) * sizeof *group);
 	int *size = xcalloc((size_t)(n ? n : 1), sizeof *size);
 	int *rep = xmalloc((size_t)(n ? n : 1) * sizeof *rep);
 	int *cid = xcalloc((size_t)(n ? n : 1), sizeof *cid);
-	for (int i = 0; i < n; ++i) {
-		parent[i] = i;
-		rep[i] = -1;
-	}
-	for (size_t k = 0; k < all.n; ++k) {
-		const int x = uf_find(parent, all.v[k].q), y = uf_find(parent, all.v[k].r);
-		if (x != y)
-			parent[x > y ? x : y] = x < y ? x : y;
-	}
-	for (int i = 0; i < n; ++i)
-		++size[uf_find(parent, i)];
-	for (int i = 0; i < n; ++i) {
-		const int root = uf_find(parent, i);
-		if (rep[root] < 0 || kssd_quality_compare(s.meta, s.index, i, rep[root]) > 0)
-			rep[root] = i;
+	if (dedup) {
+		int *order = xmalloc((size_t)(n ? n : 1) * sizeof *order);
+		int *tmp = xmalloc((size_t)(n ? n : 1) * sizeof *tmp);
+		for (int i = 0; i < n; ++i)
+			order[i] = i;
+		quality_sort(order, tmp, 0, n, &s);
+		dedup_groups(n, &all, order, a->full_linkage, group, size);
+		for (int i = 0; i < n; ++i)
+			rep[i] = group[i];
+		free(order);
+		free(tmp);
+	} else {
+		for (int i = 0; i < n; ++i) {
+			group[i] = i;
+			rep[i] = -1;
+		}
+		for (size_t k = 0; k < all.n; ++k) {
+			const int x = uf_find(group, all.v[k].q), y = uf_find(group, all.v[k].r);
+			if (x != y)
+				group[x > y ? x : y] = x < y ? x : y;
+		}
+		for (int i = 0; i < n; ++i)
+			group[i] = uf_find(group, i);
+		for (int i = 0; i < n; ++i)
+			++size[group[i]];
+		for (int i = 0; i < n; ++i)
+			if (rep[group[i]] < 0 || kssd_quality_compare(s.meta, s.index, i, rep[group[i]]) > 0)
+				rep[group[i]] = i;
 	}
 	int next = 1;
-	for (int i = 0; i < n; ++i) {
-		const int root = uf_find(parent, i);
-		if (!cid[root])
-			cid[root] = next++;
-	}
+	for (int i = 0; i < n; ++i)
+		if (!cid[group[i]])
+			cid[group[i]] = next++;
 	if (a->edge_outf[0]) {
 		if (!indexed) /* the pair loop queries the smaller index against the larger */
 			for (size_t k = 0; k < all.n; ++k) {
@@ -1739,23 +2100,62 @@ static int matrix_clusters(const matrix_args_t *a)
 		write_edge_file(a->edge_outf, &s, &all, a->expr.label);
 	}
 	FILE *fp = open_or_stdout(a->outf);
+	FILE *keep_fp = dedup && a->keep_outf[0] ? open_or_stdout(a->keep_outf) : NULL;
+	FILE *remove_fp = dedup && a->remove_outf[0] ? open_or_stdout(a->remove_outf) : NULL;
 	kssd_buf_t b = {0};
-	kssd_buf_printf(&b, "sample\tcluster_id\tcomponent_size\trepresentative\n");
+	if (dedup)
+		kssd_buf_printf(&b, "sample\tcomponent_id\taction\trepresentative\treason\trep_rule\tcomponent_size\t"
+				    "sample_sketch_entries\trep_sketch_entries\tsample_fmt\tsample_asm_level\t"
+				    "sample_total_length_bp\tsample_record_count\tsample_median_length_bp\t"
+				    "sample_length_cv\trep_fmt\trep_asm_level\trep_total_length_bp\trep_record_count\t"
+				    "rep_median_length_bp\trep_length_cv\n");
+	else
+		kssd_buf_printf(&b, "sample\tcluster_id\tcomponent_size\trepresentative\n");
+	int *kept = xmalloc((size_t)(n ? n : 1) * sizeof *kept);
+	int nk = 0;
 	for (int i = 0; i < n; ++i) {
-		const int root = uf_find(parent, i);
-		kssd_buf_printf(&b, "%s\tcluster_%06d\t%d\t%s\n", s.names[i], cid[root], size[root], s.names[rep[root]]);
+		const int root = group[i], r = rep[root];
+		if (dedup) {
+			const bool keep = i == r;
+			const char *reason = size[root] == 1 ? "singleton" : (keep ? "best_representative" : "duplicate");
+			if (keep)
+				kept[nk++] = i;
+			if (keep && keep_fp)
+				fprintf(keep_fp, "%s\n", s.names[i]);
+			if (!keep && remove_fp)
+				fprintf(remove_fp, "%s\n", s.names[i]);
+			kssd_buf_printf(&b, "%s\tcluster_%06d\t%s\t%s\t%s\t%s\t%d\t%llu\t%llu", s.names[i], cid[root],
+					keep ? "keep" : "remove", s.names[r], reason, rep_rule(&s, i, r), size[root],
+					(unsigned long long)(s.index[i + 1] - s.index[i]),
+					(unsigned long long)(s.index[r + 1] - s.index[r]));
+			put_meta_fields(&b, &s, i);
+			put_meta_fields(&b, &s, r);
+			kssd_buf_printf(&b, "\n");
+		} else {
+			kssd_buf_printf(&b, "%s\tcluster_%06d\t%d\t%s\n", s.names[i], cid[root], size[root], s.names[r]);
+		}
 		if (b.l > (1u << 20))
 			put_buf(fp, &b);
 	}
 	put_buf(fp, &b);
 	kssd_buf_free(&b);
 	close_out(fp);
+	if (keep_fp)
+		close_out(keep_fp);
+	if (remove_fp)
+		close_out(remove_fp);
+	if (dedup && a->keep_matrix_outf[0]) {
+		kssd_ctxgidobj_t *built = s.records || !nk ? NULL : build_records(&s, a->p);
+		write_keep_matrix(a, &s, s.records ? s.records : built, tot, kept, nk);
+		free(built);
+	}
 	stage("formatted");
 	for (int q = 0; q < n; ++q)
 		free(rows[q].v);
 	free(rows);
 	free(all.v);
-	free(parent);
+	free(kept);
+	free(group);
 	free(size);
 	free(rep);
 	free(cid);
@@ -1829,8 +2229,8 @@ static int cmd_matrix(int argc, char **argv)
 	stage("GPU ready");
 	if (a.format == 3)
 		return a.refdir[0] ? matrix_edges_rect(&a) : matrix_edges_self(&a);
-	if (a.format == 4)
-		return matrix_clusters(&a);
+	if (a.format == 4 || a.format == 5)
+		return matrix_components(&a);
 	if (a.refdir[0]) {
 		if (a.format == 0)
 			a.format = 1;

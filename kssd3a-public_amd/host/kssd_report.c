@@ -1104,6 +1104,26 @@ double kssd_matrix_context_value(int metric, double e, uint32_t xny, uint32_t do
 	return d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
 }
 
+double kssd_matrix_context_expr_value(const kssd_metric_expr_t *x, double e, uint32_t xny, uint32_t dobj,
+				      uint32_t dsec, uint32_t mut2, uint32_t qc, uint32_t rc, int drfold,
+				      unsigned obj_bits, unsigned ctx_bits)
+{
+	/* matrix_context_feature_value over an expression, command_matrix.c:621-661:
+	 * each metric clamped to [0, 1], & keeps the largest, | the smallest */
+	if (xny == 0 || qc == 0 || rc == 0 || x->count == 0)
+		return e;
+	double sel = 0.0;
+	for (int i = 0; i < x->count; ++i) {
+		const double d = kssd_matrix_context_value(x->metrics[i], NAN, xny, dobj, dsec, mut2, qc, rc, drfold,
+							   obj_bits, ctx_bits);
+		if (isnan(d))
+			return e;
+		if (i == 0 || (x->op == KSSD_EXPR_AND ? d > sel : d < sel))
+			sel = d;
+	}
+	return sel;
+}
+
 static double clean_distance(double d)
 {
 	/* pairwise_clean_distance, pairwise_graph.c:14-21 */

@@ -217,6 +217,12 @@ typedef struct {
 	int count, op;
 	char label[96];
 } kssd_metric_expr_t;
+/* kssd_matrix_context_value over a metric expression (matrix_context_feature_value,
+ * command_matrix.c:621-661): e when nothing is shared, a total is 0 or a
+ * metric is not finite; else the largest (&) or smallest (|) clamped value. */
+double kssd_matrix_context_expr_value(const kssd_metric_expr_t *x, double e, uint32_t xny, uint32_t dobj,
+				      uint32_t dsec, uint32_t mut2, uint32_t qc, uint32_t rc, int drfold,
+				      unsigned obj_bits, unsigned ctx_bits);
 /* pairwise_metric_name (pairwise_graph.c:52-67) */
 const char *kssd_metric_name(int metric);
 /* pairwise_metric_expr_single / pairwise_metric_expr_from_string

@@ -286,6 +286,45 @@ def make_matrix_clusters():
     shutil.rmtree(tmp)
 
 
+DEDUP_CASES = {  # name: (sketch: "all" or "noindex", arguments), outputs beside the main report
+    "dedup_default": ("all", ["--cut", "0.05"]),
+    "dedup_full": ("all", ["--cut", "0.06", "--max-afcut", "0.1", "--dedup-strategy", "full-linkage",
+                           "--keep-matrix-format", "phylip", "--keep-matrix-idmap", "@idmap"]),
+    "dedup_greedy": ("all", ["--cut", "0.06", "--max-afcut", "0.1"]),
+    "dedup_wide": ("all", ["--cut", "0.9", "--max-afcut", "0.3", "-m", "mash|aaf"]),
+    "dedup_noindex": ("noindex", ["--cut", "0.055", "--max-afcut", "0", "-m", "ctx-moe&p_dist",
+                                  "--diagonal-value", "0.5"]),
+    "dedup_noindex_full": ("noindex", ["--cut", "0.9", "--dedup-strategy", "clique", "--max-afcut", "0"]),
+}
+DEDUP_OUTPUTS = {"--keep-out": "keep", "--remove-out": "remove", "--keep-matrix-out": "keep_matrix",
+                 "--edge-out": "edges"}
+
+
+def make_matrix_dedup():
+    """`matrix --format dedup-plan` (one thread): greedy and full-linkage
+    plans over the indexed scan's edges and every pair's, with --keep-out,
+    --remove-out, --edge-out and the kept samples' matrix (TSV, or PHYLIP
+    with its id map)."""
+    out = HERE / "genomes"
+    tmp = HERE / "_noindex"
+    shutil.rmtree(tmp, ignore_errors=True)
+    shutil.copytree(out / "all", tmp)
+    for f in ("sortedcomb_ctxgid64obj32", "sortedcomb_ctxgid64obj32.unique_bits", "lcofiles.infilemeta"):
+        (tmp / f).unlink(missing_ok=True)
+    for name, (which, args) in DEDUP_CASES.items():
+        d = out / "all" if which == "all" else tmp
+        extra = []
+        for opt, suffix in DEDUP_OUTPUTS.items():
+            extra += [opt, HERE / f"_{suffix}.out"]
+        args = [str(HERE / "_idmap.out") if a == "@idmap" else a for a in args]
+        (out / f"{name}.tsv").write_text(run(["matrix", "--format", "dedup-plan", "-p", "1", *extra, *args, d]))
+        for suffix in [*DEDUP_OUTPUTS.values(), "idmap"]:
+            f = HERE / f"_{suffix}.out"
+            if f.exists():
+                shutil.move(f, out / f"{name}_{suffix}.txt")
+    shutil.rmtree(tmp)
+
+
 def make_sketch():
     """Small FASTA inputs (multi-record, lowercase, non-ACGT runs) and the
     reference's sketches of them, at the default -f8 and at -f4."""
@@ -411,6 +450,7 @@ def main():
     make_matrix_cmd()
     make_matrix_edges()
     make_matrix_clusters()
+    make_matrix_dedup()
     make_sketch()
     make_readsets()
     print("golden fixtures written under", HERE)

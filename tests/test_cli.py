@@ -326,6 +326,48 @@ def test_cli_matrix_clusters_match_reference(tmp_path, stem, args, indexed):
     assert edges.read_text() == (GEN / f"{stem}_edges.tsv").read_text()
 
 
+DEDUP_CASES = {  # make_golden.py DEDUP_CASES: (sketch keeps its index and metadata, args)
+    "dedup_default": (True, ["--cut", "0.05"]),
+    "dedup_full": (True, ["--cut", "0.06", "--max-afcut", "0.1", "--dedup-strategy", "full-linkage",
+                          "--keep-matrix-format", "phylip", "--keep-matrix-idmap", "@idmap"]),
+    "dedup_greedy": (True, ["--cut", "0.06", "--max-afcut", "0.1"]),
+    "dedup_wide": (True, ["--cut", "0.9", "--max-afcut", "0.3", "-m", "mash|aaf"]),
+    "dedup_noindex": (False, ["--cut", "0.055", "--max-afcut", "0", "-m", "ctx-moe&p_dist",
+                              "--diagonal-value", "0.5"]),
+    "dedup_noindex_full": (False, ["--cut", "0.9", "--dedup-strategy", "clique", "--max-afcut", "0"]),
+}
+DEDUP_OUTPUTS = {"--keep-out": "keep", "--remove-out": "remove", "--keep-matrix-out": "keep_matrix",
+                 "--edge-out": "edges"}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stem", DEDUP_CASES)
+def test_cli_matrix_dedup_plan_matches_reference(tmp_path, stem):
+    """Dedup plans (greedy and full-linkage, over the indexed scan's edges
+    and every pair's) with every side output: keep and remove lists, the
+    kept samples' matrix (TSV; PHYLIP with its id map), --edge-out."""
+    indexed, args = DEDUP_CASES[stem]
+    dirs = genome_dirs(tmp_path, True)
+    if indexed:
+        for f in INDEX_FILES:
+            shutil.copy(GEN / "all" / f, dirs["S"] / f)
+    else:
+        (dirs["S"] / "lcofiles.infilemeta").unlink(missing_ok=True)
+    extra = []
+    for opt, suffix in DEDUP_OUTPUTS.items():
+        extra += [opt, tmp_path / f"{suffix}.out"]
+    args = [str(tmp_path / "idmap.out") if a == "@idmap" else a for a in args]
+    p = run_cli(["matrix", "--format", "dedup-plan", *extra, *args, dirs["S"]])
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == (GEN / f"{stem}.tsv").read_text()
+    for suffix in [*DEDUP_OUTPUTS.values(), "idmap"]:
+        want = GEN / f"{stem}_{suffix}.txt"
+        got = tmp_path / f"{suffix}.out"
+        assert got.exists() == want.exists(), suffix
+        if want.exists():
+            assert got.read_text() == want.read_text(), suffix
+
+
 @pytest.mark.gpu
 def test_cli_matrix_edges_conflict_sketch(tmp_path):
     """A conflict sketch with an index still takes the every-pair path."""
@@ -342,9 +384,20 @@ def test_matrix_usage_errors():
                       (["matrix", "-m", "aaf&mash|p_dist", "--sparse", "--cut", "1", "x"], "-m/--metric must be"),
                       (["matrix", "-m", "bogus", "x"], "-m/--metric must be"),
                       (["matrix", "--sparse", "x"], "requires --cut"),
-                      (["matrix", "--format", "dedup-plan", "--cut", "0.1", "x"], "not offered"),
+                      (["matrix", "--format", "dedup-plan", "-r", "x", "-q", "y", "--cut", "1"], "requires one sketch"),
                       (["matrix", "--format", "clusters", "--cut", "0.1", "-r", "x", "-q", "y"], "requires one sketch"),
-                      (["matrix", "--keep-out", "k.txt", "x"], "not offered"),
+                      (["matrix", "--keep-out", "k.txt", "x"], "require --format dedup-plan"),
+                      (["matrix", "--dedup-strategy", "greedy", "x"], "requires --format dedup-plan"),
+                      (["matrix", "--format", "dedup", "--cut", "1", "--dedup-strategy", "x", "y"], "--dedup-strategy must"),
+                      (["matrix", "--format", "dedup", "--cut", "1", "--keep-matrix-format", "phylip", "y"],
+                       "requires --keep-matrix-out"),
+                      (["matrix", "--format", "dedup", "--cut", "1", "--keep-matrix-out", "m", "--keep-matrix-idmap",
+                        "i", "y"], "requires --keep-matrix-format phylip"),
+                      (["matrix", "--format", "dedup", "--cut", "1", "--keep-out", "a", "--remove-out", "a", "y"],
+                       "must be different files"),
+                      (["matrix", "--format", "dedup", "--cut", "1", "--keep-matrix-out", "a", "-o", "a", "y"],
+                       "--keep-matrix-out must differ from --outfile"),
+                      (["matrix", "--glist", "g", "x"], "not offered"),
                       (["matrix"], "missing sketch input")):
         p = run_cli(args)
         assert p.returncode == 64 and msg in p.stderr, (args, p.stderr)
