@@ -115,6 +115,31 @@ def measured_traffic(tiling, nref, nqry):
     return None, f"profiles/{f.name} (measured with another tiling)"
 
 
+def measured_lds(tiling, nref, nqry):
+    """k_pjoin's LDS figures per dispatch from the committed rocprofv3 summary
+    of this workload when measured with this run's tiling: (dict or None,
+    summary file name)."""
+    import re
+    f = ROOT / "profiles" / f"pjoin_split_c2_{nref}x{nqry}_lds.md"
+    if not f.exists():
+        return None, None
+    text = f.read_text()
+    mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
+    if not mt or tuple(map(int, mt.groups())) != (tiling["mode"], tiling["chunk_records"], tiling["block_threads"],
+                                                  tiling.get("group_partitions", 1)):
+        return None, f"profiles/{f.name} (measured with another tiling)"
+    vals = {k: float(v) for k, v in re.findall(r"^(lds_\w+)=([0-9.eE+-]+)$", text, re.M)}
+    return vals, f"profiles/{f.name}"
+
+
+def matched_events(records, qkeys, obj_bits):
+    """H: (query key, index record) pairs sharing a context, the join's
+    unit of work (one object-mismatch minimum and one counter update each)."""
+    rctx = records["ctxgid"] >> np.uint64(20)
+    qctx = qkeys >> np.uint64(obj_bits)
+    return int((np.searchsorted(rctx, qctx, "right") - np.searchsorted(rctx, qctx, "left")).sum())
+
+
 def readset_like(rng, keys, coverage, obj_bits=20):
     """Key set shaped like a read-set sketch (`--asone --conflict`) of a genome
     read at `coverage`x with 1% per-base read errors: the genome's keys plus
@@ -672,6 +697,9 @@ def main():
     join_s = float(np.mean(join_ms)) / 1e3
     achieved = algo_bytes / join_s / 1e9
     traffic, traffic_src = measured_traffic(tiling, nref, nqry)
+    H = matched_events(records, qkeys, obj_bits)
+    lds, lds_src = measured_lds(tiling, nref, nqry)
+    lds_peak = 256 * 128 * 2.4e9 / 1e9  # GB/s: 256 CUs x 128 B/clk x 2.4 GHz
 
     # ---------------- CPU baseline (oracle) + parity on its sample
     nq_cpu, cpu_s, cctx, cobj, cout = cpu_baseline(records, nref, obj_bits, qkeys, qidx, qctx, rctx,
@@ -731,6 +759,16 @@ def main():
                      "bytes_model": "8*N_Q + 12*M + 16*P_touched",
                      "N_Q": total, "M": M, "P_touched": p_touched,
                      "avg_launch_ms": float(np.mean(join_ms))},
+        "lds": {"events": H, "events_per_s": H / join_s,
+                "events_what": "(query key, index record) pairs sharing a context, each one object-mismatch "
+                               "minimum and one counter update",
+                "bytes_per_launch": lds.get("lds_bytes_per_dispatch") if lds else None,
+                "achieved_GBs": lds["lds_bytes_per_dispatch"] / join_s / 1e9 if lds else None,
+                "peak_GBs": lds_peak,
+                "frac": lds["lds_bytes_per_dispatch"] / join_s / 1e9 / lds_peak if lds else None,
+                "lds_util_pct": lds.get("lds_util_pct") if lds else None,
+                "bank_conflict": lds.get("lds_bank_conflict") if lds else None,
+                "source": lds_src},
         "cpu_baseline": {"value": cpu_pairs_per_s, "unit": "pairs/s", "cores": threads, "kind": "port",
                          "sample": f"first {nq_cpu} queries x {nref} refs: oracle block count "
                                    "(fenceposts per block, guided OpenMP) + FP64 CtxMoE epilogue"},

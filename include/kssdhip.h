@@ -174,7 +174,8 @@ int kssdhip_count_block(kssdhip_index *index, const uint64_t *section_sk,
 /*
  * Device-resident form of kssdhip_count_block for batched and serving use:
  * d_keys/d_skidx/d_ctx/... are device pointers; d_skidx holds section_gnum+1
- * offsets RELATIVE to d_keys.  Asynchronous on `stream` (a hipStream_t).
+ * offsets RELATIVE to d_keys (d_skidx[0] = 0, d_skidx[section_gnum] =
+ * total_keys).  Asynchronous on `stream` (a hipStream_t).
  * No reference counterpart beyond kssdhip_count_block.
  */
 int kssdhip_count_block_dev(kssdhip_index *index, const uint64_t *d_keys,

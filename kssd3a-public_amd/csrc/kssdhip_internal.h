@@ -96,6 +96,9 @@ struct kssdhip_index {
 	uint32_t max_part = 0;
 	uint32_t *d_pstart = nullptr;
 	uint64_t *d_pctx = nullptr;
+	// plut[b] = first p with pctx[p] >= b << 32 (b = 0..4096): the window of
+	// partition starts holding a 44-bit context's top 12 bits
+	uint32_t *d_plut = nullptr;
 	// tiling (kssdhip_tiling_t)
 	int mode = 0;
 	int chunk = 0;

@@ -61,6 +61,26 @@ __global__ void k_part_ctx(const uint64_t *__restrict__ ctxgid, const uint32_t *
 		pctx[p] = p < P ? (ctxgid[pstart[p]] >> GID_NBITS) : ~0ull;
 }
 
+constexpr int PLUT_BITS = 12;    // partition lookup table over the top bits of a 44-bit context
+constexpr int PLUT_SHIFT = 44 - PLUT_BITS;
+constexpr uint32_t PLUT_N = (1u << PLUT_BITS) + 1;
+
+__global__ void k_plut(const uint64_t *__restrict__ pctx, uint32_t P, uint32_t *plut)
+{
+	for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < PLUT_N; b += gridDim.x * blockDim.x) {
+		const uint64_t c = (uint64_t)b << PLUT_SHIFT;
+		uint32_t lo = 0, hi = P; // pctx[P] = ~0 >= c
+		while (lo < hi) {
+			const uint32_t mid = lo + ((hi - lo) >> 1);
+			if (pctx[mid] < c)
+				lo = mid + 1;
+			else
+				hi = mid;
+		}
+		plut[b] = lo;
+	}
+}
+
 constexpr uint32_t SPLIT_RUN = 2; // partition boundaries per k_split thread
 
 struct SplitArgs {
@@ -130,6 +150,124 @@ __global__ __launch_bounds__(256) void k_split(SplitArgs a)
 			i = x;
 		}
 		row[p] = (uint32_t)i;
+	}
+}
+
+// The split table from the keys' side, for blocks with many keys per table
+// entry.  k_split_fill first sets every entry of row q to the end of query q
+// (what entries past its last key, and every entry of an empty query, hold).
+// Then, with f(i) = number of partition starts <= the context of key i, key
+// i of query q is the first key of q at or past partitions [f(i-1), f(i))
+// (the first key of q: [0, f(i))), so k_split_keys writes exactly those
+// entries.  Each wave takes a contiguous run of 256-key tiles, four
+// consecutive keys per lane (independent loads and searches in flight; f
+// through the partition lookup table and a short search of pctx, both
+// cache-resident).  Every entry holds a key index of its own query
+// whatever the key order, as k_pjoin's key ranges require.
+constexpr uint32_t SPLITK_WAVES = 4;
+constexpr uint32_t SPLITK_KPL = 4;   // keys per lane per tile
+constexpr uint32_t SPLITK_TILES = 8; // 256-key tiles per wave
+
+__global__ void k_split_fill(const uint64_t *__restrict__ skidx, int nq, uint32_t width, uint32_t *S)
+{
+	const uint64_t n = (uint64_t)nq * width;
+	for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n;
+	     e += (uint64_t)gridDim.x * blockDim.x)
+		S[e] = (uint32_t)skidx[e / width + 1];
+}
+
+struct SplitKArgs {
+	SplitArgs s;
+	const uint32_t *plut;
+	uint64_t total;
+};
+
+__device__ __forceinline__ uint32_t parts_at_or_below(const uint64_t *__restrict__ pctx,
+						      const uint32_t *__restrict__ plut, uint64_t c)
+{
+	if ((c >> PLUT_SHIFT) >= PLUT_N - 1) // past every 44-bit context: all P starts lie below
+		return plut[PLUT_N - 1];
+	const uint32_t b = (uint32_t)(c >> PLUT_SHIFT);
+	uint32_t lo = plut[b], hi = plut[b + 1]; // pctx[p] < c for p < lo, > c for p >= hi
+	while (lo < hi) {
+		const uint32_t mid = lo + ((hi - lo) >> 1);
+		if (pctx[mid] <= c)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+__global__ __launch_bounds__(SPLITK_WAVES * 64) void k_split_keys(SplitKArgs a)
+{
+	const uint32_t lane = threadIdx.x & 63;
+	const uint64_t w = (uint64_t)blockIdx.x * SPLITK_WAVES + (threadIdx.x >> 6);
+	const uint64_t tile = 64 * SPLITK_KPL, span = (uint64_t)SPLITK_TILES * tile;
+	const uint64_t start = w * span;
+	if (start >= a.total)
+		return;
+	const uint64_t end = start + span < a.total ? start + span : a.total;
+	const uint32_t width = a.s.P + 1;
+	const uint64_t *__restrict__ skidx = a.s.skidx;
+	const unsigned ob = a.s.obj_bits;
+	// the query of the run's first key by a 64-way search (each round the
+	// lanes probe 64 evenly spaced offsets), and f of the key before it when
+	// that key is in the same query (uniform across the wave)
+	int lo = 0, hi = a.s.nq; // skidx[lo] <= start < skidx[hi]
+	while (hi - lo > 1) {
+		const int step = (hi - lo + 63) / 64;
+		const int probe = lo + (int)lane * step;
+		const bool le = probe < hi && skidx[probe] <= start;
+		const uint64_t m = __ballot(le); // a prefix of the lanes
+		const int k = 63 - __clzll((long long)m);
+		lo = lo + k * step;
+		hi = lo + step < hi ? lo + step : hi;
+	}
+	int q = lo;
+	uint64_t qend = skidx[q + 1]; // first key past query q
+	uint32_t prevf = start > skidx[q] ? parts_at_or_below(a.s.pctx, a.plut, a.s.keys[start - 1] >> ob) : 0;
+	for (uint64_t base = start; base < end; base += tile) {
+		// each lane takes SPLITK_KPL consecutive keys: independent loads and
+		// searches, gaps between its own keys found locally
+		const uint64_t i0 = base + (uint64_t)lane * SPLITK_KPL;
+		uint32_t f[SPLITK_KPL];
+		uint64_t c[SPLITK_KPL];
+#pragma unroll
+		for (int j = 0; j < (int)SPLITK_KPL; ++j)
+			c[j] = i0 + j < end ? a.s.keys[i0 + j] >> ob : 0;
+#pragma unroll
+		for (int j = 0; j < (int)SPLITK_KPL; ++j)
+			f[j] = i0 + j < end ? parts_at_or_below(a.s.pctx, a.plut, c[j]) : 0;
+		const int nv = i0 >= end ? 0 : (int)(end - i0 < SPLITK_KPL ? end - i0 : SPLITK_KPL);
+		uint32_t pf = __shfl_up(nv ? f[nv - 1] : 0u, 1);
+		if (lane == 0)
+			pf = prevf;
+		int ql = q;
+		uint64_t qe = qend, qb = 0;
+		bool first = false;
+#pragma unroll
+		for (int j = 0; j < (int)SPLITK_KPL; ++j) {
+			if (j >= nv)
+				break;
+			const uint64_t i = i0 + j;
+			while (qe <= i) { // rarely: the key starts a later query
+				qb = qe;
+				++ql;
+				qe = skidx[ql + 1];
+				first = true;
+			}
+			const bool starts = first && i == qb;
+			uint32_t *row = a.s.S + (uint64_t)ql * width;
+			for (uint32_t p = starts || i == 0 ? 0 : pf; p < f[j]; ++p)
+				row[p] = (uint32_t)i;
+			pf = f[j];
+		}
+		// carry the last key's query and f into the next tile (a full tile
+		// unless this is the run's last)
+		prevf = __shfl(pf, 63);
+		q = __shfl(ql, 63);
+		qend = __shfl(qe, 63);
 	}
 }
 
@@ -586,8 +724,11 @@ void kssd_free_partitions(kssdhip_index *ix)
 		(void)hipFree(ix->d_pstart);
 	if (ix->d_pctx)
 		(void)hipFree(ix->d_pctx);
+	if (ix->d_plut)
+		(void)hipFree(ix->d_plut);
 	ix->d_pstart = nullptr;
 	ix->d_pctx = nullptr;
+	ix->d_plut = nullptr;
 	ix->P = 0;
 	ix->max_part = 0;
 }
@@ -627,7 +768,7 @@ int kssd_build_partitions(kssdhip_index *ix, int chunk)
 	for (uint32_t p = 0; p < P; ++p)
 		mx = std::max(mx, pstart[p + 1] - pstart[p]);
 	if (hipMalloc(&ix->d_pstart, (P + 1) * 4) != hipSuccess ||
-	    hipMalloc(&ix->d_pctx, (P + 1) * 8) != hipSuccess) {
+	    hipMalloc(&ix->d_pctx, (P + 1) * 8) != hipSuccess || hipMalloc(&ix->d_plut, PLUT_N * 4) != hipSuccess) {
 		kssd_free_partitions(ix);
 		return kssdhip_fail(KSSDHIP_ENOMEM, "partition table: device allocation failed");
 	}
@@ -635,6 +776,8 @@ int kssd_build_partitions(kssdhip_index *ix, int chunk)
 	if (e == hipSuccess) {
 		hipLaunchKernelGGL(k_part_ctx, dim3(grid_cap(P + 1, 256)), dim3(256), 0, ix->stream,
 				   ix->d_ctxgid, ix->d_pstart, P, ix->d_pctx);
+		hipLaunchKernelGGL(k_plut, dim3(grid_cap(PLUT_N, 256)), dim3(256), 0, ix->stream, ix->d_pctx, P,
+				   ix->d_plut);
 		e = hipGetLastError();
 	}
 	if (e == hipSuccess)
@@ -672,9 +815,24 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevB
 	sa.pctx = ix->d_pctx;
 	sa.P = ix->P;
 	sa.S = S;
-	const uint64_t runs = ((uint64_t)ix->P + SPLIT_RUN) / SPLIT_RUN;
-	const uint64_t nthreads = (uint64_t)ja.nq * runs;
-	hipLaunchKernelGGL(k_split, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, sa);
+	const uint64_t entries = ((uint64_t)ix->P + 1) * (uint64_t)ja.nq;
+	if (ja.total * 8 >= entries) {
+		// keys' side: every row set to its query's end, then a run of key tiles per wave
+		hipLaunchKernelGGL(k_split_fill, dim3(grid_cap(entries, 256)), dim3(256), 0, s, ja.skidx, ja.nq, ix->P + 1,
+				   S);
+		SplitKArgs ka;
+		ka.s = sa;
+		ka.plut = ix->d_plut;
+		ka.total = ja.total;
+		const uint64_t span = (uint64_t)SPLITK_TILES * 64 * SPLITK_KPL;
+		const uint64_t waves = (ja.total + span - 1) / span;
+		hipLaunchKernelGGL(k_split_keys, dim3((unsigned)((waves + SPLITK_WAVES - 1) / SPLITK_WAVES)),
+				   dim3(SPLITK_WAVES * 64), 0, s, ka);
+	} else {
+		const uint64_t runs = ((uint64_t)ix->P + SPLIT_RUN) / SPLIT_RUN;
+		const uint64_t nthreads = (uint64_t)ja.nq * runs;
+		hipLaunchKernelGGL(k_split, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, sa);
+	}
 	PJArgs a;
 	a.j = ja;
 	a.pstart = ix->d_pstart;

@@ -204,6 +204,40 @@ def test_tilings_match_oracle(hip, oracle, mode, chunk, block, group, flags):
     assert np.array_equal(low[0], wl[0]) and np.array_equal(low[1], wl[1])
 
 
+@pytest.mark.parametrize("dense", [False, True])
+@pytest.mark.parametrize("group", [1, 4])
+def test_split_table_both_sides(hip, oracle, dense, group):
+    """The split table from the partitions' side (few keys per table entry:
+    queries of 0-3 keys against 64-record partitions) and from the keys' side
+    (many keys per entry), with empty queries, single-key queries, keys below
+    the first and above the last indexed context, and tile-boundary query
+    ends (64-key tiles, 2048-key wave runs)."""
+    rng = np.random.default_rng(91 + dense + 10 * group)
+    pool = rng.integers(1 << 8, 1 << 42, size=4000, dtype=np.uint64)
+    refs = kt.random_samples(rng, 50, 4000, 400, shared_pool=pool, conflict_p=0.1)
+    if dense:
+        sizes = [0, 1, 63, 64, 65, 0, 2047, 2048, 2049, 700, 0, 1, 3000]
+    else:
+        sizes = [0, 1, 3, 0, 2, 1, 0, 3, 1, 2, 0, 0, 1]
+    qrys = []
+    for n in sizes:
+        ctx = np.unique(np.concatenate([rng.choice(pool, size=n), [np.uint64(1), np.uint64((1 << 44) - 1)]]))
+        ctx = rng.permutation(ctx)[:n]
+        qrys.append(np.sort((ctx << np.uint64(20)) | rng.integers(0, 1 << 20, size=len(ctx), dtype=np.uint64)))
+    rkeys, ridx = _block(refs)
+    rec = kt.build_index_np(rkeys, ridx, OBJ_BITS)
+    qkeys, qidx = _block(qrys)
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=2, chunk_records=64, block_threads=256, group_partitions=group)
+        got = hip.count_block(h, len(refs), qkeys, qidx, flags=2)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, len(refs), OBJ_BITS, qkeys, qidx, flags=2)
+    for name, g, w in zip(("ctx", "obj", "uctx", "uobj"), got, want):
+        assert np.array_equal(g, w), (dense, group, name)
+
+
 def test_tiling_validation(hip):
     refs, _ = _fixture(3, n_ref=4)
     keys, idx = _block(refs)
