@@ -266,6 +266,22 @@ int kssdhip_epilogue_dev(const kssd_ctx_mut2_t *d_ctx, const kssd_obj_section_t 
 			 const kssdhip_epilogue_opts_t *opts, double *d_out, void *stream);
 
 /*
+ * Cells of a dense value block (nq x nr doubles, e.g. kssdhip_epilogue_dev's
+ * ANI) with value >= min_value, as (query, ref, value) in no particular
+ * order; at most `cap` are stored and *d_count receives how many there are
+ * (it may exceed cap).  The call zeroes *d_count with a kernel, so it can be
+ * captured in a graph.  Replaces, for device-resident blocks, the anicut
+ * test of the reference's row selection (`selected_ani >= anicut`,
+ * command_ani.c:3517, :2197); ranking and output stay on the host.
+ */
+typedef struct {
+	uint32_t query, ref;
+	double value;
+} kssdhip_hit_t;
+int kssdhip_select_hits_dev(const double *d_values, int nq, int nr, double min_value, uint32_t cap,
+			    kssdhip_hit_t *d_hits, uint32_t *d_count, void *stream);
+
+/*
  * Serving loop (no reference counterpart: the reference answers one request
  * per process, stream_ref_sketches_one_qraw_lookup, command_ani.c:2490-2666).
  * A server keeps the index resident and answers query sketches submitted

@@ -128,6 +128,9 @@ class Hip:
         L.hipGraphLaunch.argtypes = [V, V]
         L.hipGraphExecDestroy.argtypes = [V]
         L.hipGraphDestroy.argtypes = [V]
+        L.hipHostMalloc.argtypes = [ctypes.POINTER(V), S, U32]
+        L.hipHostFree.argtypes = [V]
+        L.hipStreamWaitEvent.argtypes = [V, V, U32]
         self.check(L.hipSetDevice(device))
         self.device = device
 
@@ -165,6 +168,23 @@ class Hip:
     def free(self, p):
         if p:
             self.check(self.lib.hipFree(p))
+
+    def host_malloc(self, nbytes):
+        """Pinned host memory (hipHostMalloc), for asynchronous copies."""
+        p = V()
+        self.check(self.lib.hipHostMalloc(ctypes.byref(p), max(int(nbytes), 1), 0))
+        return p
+
+    def host_free(self, p):
+        if p:
+            self.check(self.lib.hipHostFree(p))
+
+    def copy_async(self, dst, src, nbytes, kind, stream):
+        if nbytes:
+            self.check(self.lib.hipMemcpyAsync(dst, src, int(nbytes), kind, stream))
+
+    def wait_event(self, stream, ev):
+        self.check(self.lib.hipStreamWaitEvent(stream, ev, 0))
 
     def upload(self, arr, dptr=None):
         arr = np.ascontiguousarray(arr)
@@ -204,6 +224,9 @@ class Hip:
 
 
 # ------------------------------------------------------------------ serving
+HIT_DTYPE = np.dtype([("query", "<u4"), ("ref", "<u4"), ("value", "<f8")])  # kssdhip_hit_t
+
+
 class ServeOpts(ctypes.Structure):
     _fields_ = [("budget_bytes", S), ("slots", I), ("max_batch", I), ("max_queued", I), ("flags", U32)]
 
@@ -244,6 +267,7 @@ def load_kssdhip():
     lib.kssdhip_index_set_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
     lib.kssdhip_index_get_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
     lib.kssdhip_epilogue_dev.argtypes = [V, V, I, I, V, V, ctypes.POINTER(EpilogueOpts), V, V]
+    lib.kssdhip_select_hits_dev.argtypes = [V, I, I, ctypes.c_double, U32, V, V, V]
     lib.kssdhip_serve_open.argtypes = [V, ctypes.POINTER(ServeOpts), ctypes.POINTER(V)]
     lib.kssdhip_serve_submit.argtypes = [V, V, S]
     lib.kssdhip_serve_submit.restype = ctypes.c_int64
@@ -384,6 +408,10 @@ class KssdHip:
     def epilogue_dev(self, d_ctx, d_obj, nq, nr, d_qctx, d_rctx, opts, d_out, stream=None):
         self.check(self.lib.kssdhip_epilogue_dev(d_ctx, d_obj, nq, nr, d_qctx, d_rctx,
                                                  ctypes.byref(opts), d_out, stream))
+
+    def select_hits_dev(self, d_values, nq, nr, min_value, cap, d_hits, d_count, stream=None):
+        self.check(self.lib.kssdhip_select_hits_dev(d_values, nq, nr, float(min_value), cap, d_hits, d_count,
+                                                    stream))
 
 
 # ------------------------------------------------------------------ libkssdhost

@@ -21,7 +21,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[1]
 PKG = ROOT / "kssd3a-public_amd"
 sys.path.insert(0, str(PKG))
-from kssdhip import (Features, Host, Hip, KssdHip, EpilogueOpts, SimSpec,  # noqa: E402,F401
+from kssdhip import (Features, Host, Hip, KssdHip, EpilogueOpts, SimSpec, HIT_DTYPE,  # noqa: E402,F401
                      load_kssdhip, load_kssdhost)
 GOLDEN = ROOT / "tests" / "golden"
 KSSDHIP_EINVAL, KSSDHIP_EAGAIN = -1, -5

@@ -132,3 +132,36 @@ def test_pair_dev_back_to_back_launches(hip, oracle, rt):
     finally:
         for p in (d_r, d_q, d_out, d_ws):
             rt.free(p)
+
+
+@pytest.mark.parametrize("cut", [0.0, 0.95, 2.0])
+def test_select_hits_matches_numpy(hip, rt, cut):
+    """kssdhip_select_hits_dev: every cell >= the cut, each once, with its
+    (query, ref) position; the count past the capacity, only `cap` stored;
+    a second call on the same counter starts again from zero."""
+    rng = np.random.default_rng(int(cut * 100) + 5)
+    nq, nr = 37, 1013
+    v = rng.random((nq, nr))
+    v[3, :] = 0.95  # ties at the cut count as hits
+    d_v = rt.upload(v)
+    want = np.argwhere(v >= cut)
+    cap = len(want) + 10
+    d_hits, d_n = rt.malloc(16 * cap), rt.malloc(4)
+    s = rt.stream()
+    for _ in range(2):
+        hip.select_hits_dev(d_v, nq, nr, cut, cap, d_hits, d_n, s)
+        rt.sync(s)
+        n = int(rt.download(d_n, np.uint32, 1)[0])
+        assert n == len(want)
+        got = rt.download(d_hits, np.uint8, 16 * n).view(kt.HIT_DTYPE)
+        order = np.lexsort((got["ref"], got["query"]))
+        assert np.array_equal(np.stack([got["query"][order], got["ref"][order]], 1), want)
+        assert np.array_equal(got["value"][order], v[want[:, 0], want[:, 1]])
+    small = max(len(want) // 3, 1)
+    hip.select_hits_dev(d_v, nq, nr, cut, small, d_hits, d_n, s)
+    rt.sync(s)
+    assert int(rt.download(d_n, np.uint32, 1)[0]) == len(want)
+    part = rt.download(d_hits, np.uint8, 16 * min(small, len(want))).view(kt.HIT_DTYPE)
+    assert all(v[q, r] >= cut for q, r in zip(part["query"], part["ref"]))
+    for p in (d_v, d_hits, d_n):
+        rt.free(p)
