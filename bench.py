@@ -234,6 +234,119 @@ def edges_bench(args, host, threads):
     return out
 
 
+def write_fastq_pair(path1, path2, rng, genome, coverage, read_len=150, insert=300, sd=30, err=0.01):
+    """Paired-end reads of `genome` (2-bit codes): fragments of N(insert, sd)
+    bp from either strand, read 1 the fragment's first read_len bases, read 2
+    the reverse complement of its last; i.i.d. per-base errors to one of the
+    other three bases; a constant quality line.  Plain 4-line FASTQ."""
+    L = len(genome)
+    n = int(coverage * L / (2 * read_len))
+    frag = np.clip(rng.normal(insert, sd, size=n).astype(np.int64), read_len, L - 1)
+    start = (rng.random(n) * (L - frag)).astype(np.int64)
+    cols = np.arange(read_len)
+    r1 = genome[start[:, None] + cols]
+    r2 = 3 - genome[(start + frag - 1)[:, None] - cols]
+    flip = rng.random(n) < 0.5  # fragments from the reverse strand: swap the mates
+    r1[flip], r2[flip] = r2[flip].copy(), r1[flip].copy()
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    for path, reads, tag in ((path1, r1, b"1"), (path2, r2, b"2")):
+        hit = rng.random(reads.shape) < err
+        reads[hit] = (reads[hit] + rng.integers(1, 4, size=int(hit.sum()), dtype=np.uint8)) % 4
+        head = np.array([b"@r%09d/%s\n" % (i, tag) for i in range(n)], dtype="S14").view(np.uint8).reshape(n, 14)
+        rec = np.empty((n, 14 + read_len + 3 + read_len + 1), np.uint8)
+        rec[:, :14] = head
+        rec[:, 14:14 + read_len] = letters[reads]
+        rec[:, 14 + read_len:17 + read_len] = np.frombuffer(b"\n+\n", np.uint8)
+        rec[:, 17 + read_len:17 + 2 * read_len] = ord("I")
+        rec[:, -1] = ord("\n")
+        Path(path).write_bytes(rec.tobytes())
+    return n
+
+
+def readset_bench(args, host, threads):
+    """The read-set pipeline on files, kssd3a-hip against the reference's own
+    binary (oracle/_ref/kssd3a_ref, built from its sources) with the same
+    threads, and their outputs compared byte for byte:
+      genomes  args.readsets_bench clade genomes (as the index bench) as FASTA:
+               `sketch -o`, then `sketch -i` (GPU index builder);
+      samples  args.readsets_samples read sets, each a reference genome plus 0.5%
+               substitutions read as 2x150 bp pairs (insert 300 +- 30) at
+               U(10, 30)x with 1% per-base errors, plain FASTQ pairs:
+               `sketch --asone --conflict -A` per sample;
+      ani      each sample against the genomes: `--qraw --unique-ctx
+               --estimate-coverage` (indexed path with the unique-owner
+               counters; coverage columns from comblco.a and .unique_bits)."""
+    import subprocess
+    import tempfile
+    rng = np.random.default_rng(args.seed + 400)
+    nref = args.readsets_bench - args.readsets_bench % 20
+    specs, _, _ = clade_specs(rng, nref, args.seed + 400)
+    refbin = ROOT / "oracle" / "_ref" / "kssd3a_ref"
+    runs = {"gpu": str(kh.PKG / "kssd3a-hip")}
+    if refbin.exists():
+        runs["reference"] = str(refbin)
+    out = {"genomes": nref, "samples": args.readsets_samples, "threads": threads}
+    letters = np.frombuffer(b"ACGT", np.uint8)
+
+    def timed(name, argv, cwd):
+        t = time.perf_counter()
+        p = subprocess.run(argv, capture_output=True, cwd=cwd)
+        if p.returncode:
+            raise RuntimeError(f"{name}: {' '.join(argv[1:4])}...: {p.stderr.decode()[-400:]}")
+        return time.perf_counter() - t, p.stdout
+
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        d = Path(d)
+        gnames = []
+        for i, sp in enumerate(specs):
+            g = host.sim_genome(sp)
+            gnames.append(f"g{i:05d}.fa")
+            (d / gnames[-1]).write_bytes(b">g%05d\n" % i + letters[g].tobytes() + b"\n")
+        samples, bases = [], 0
+        for k in range(args.readsets_samples):
+            sp = specs[int(rng.integers(0, nref))]
+            sample = kh.SimSpec(sp.anc_seed, sp.anc_len, int(args.seed * 3_000_017 + k + 1), 0.005, 0.0,
+                                sp.mut_seed, sp.sub_rate)
+            g = host.sim_genome(sample)
+            cov = float(rng.uniform(10, 30))
+            n = write_fastq_pair(d / f"s{k:03d}_1.fq", d / f"s{k:03d}_2.fq", rng, g, cov)
+            bases += 2 * 150 * n
+            samples.append((f"s{k:03d}_1.fq", f"s{k:03d}_2.fq"))
+        out["read_bases"] = bases
+        log(f"readsets bench: {nref} genomes and {len(samples)} read sets ({bases / 1e9:.2f} Gbp of reads) written")
+        res = {}
+        for name, exe in runs.items():
+            r = {}
+            p = str(threads)
+            r["sketch_genomes_s"], _ = timed(name, [exe, "sketch", "-p", p, "-o", f"G_{name}", *gnames], d)
+            r["index_s"], _ = timed(name, [exe, "sketch", "-i", f"G_{name}"], d)
+            r["sketch_reads_s"], r["ani_s"], tsv = 0.0, 0.0, []
+            for k, (a, b) in enumerate(samples):
+                dt, _ = timed(name, [exe, "sketch", "--asone", "--conflict", "-A", "-p", p, "-o",
+                                     f"R{k}_{name}", a, b], d)
+                r["sketch_reads_s"] += dt
+                dt, o = timed(name, [exe, "ani", "-r", f"G_{name}", "--qraw", f"R{k}_{name}", "--unique-ctx",
+                                     "--estimate-coverage", "-p", p], d)
+                r["ani_s"] += dt
+                tsv.append(o)
+            res[name] = (r, tsv)
+            out[name] = r
+            log(f"readsets bench {name}: " + ", ".join(f"{k} {v:.2f}" for k, v in r.items()))
+        if "reference" in res:
+            same = {}
+            for f in ("comblco", "comblco.index", "lcofiles.infilemeta", "sortedcomb_ctxgid64obj32",
+                      "sortedcomb_ctxgid64obj32.unique_bits"):
+                same[f"genomes/{f}"] = (d / "G_gpu" / f).read_bytes() == (d / "G_reference" / f).read_bytes()
+            for f in ("comblco", "comblco.index", "comblco.a", "lcofiles.qc", "lcofiles.infilemeta"):
+                same[f"reads/{f}"] = all((d / f"R{k}_gpu" / f).read_bytes() == (d / f"R{k}_reference" / f).read_bytes()
+                                         for k in range(len(samples)))
+            same["ani_tsv"] = res["gpu"][1] == res["reference"][1]
+            out["identical"] = same
+            out["reference_over_gpu"] = {k: res["reference"][0][k] / res["gpu"][0][k] for k in res["gpu"][0]}
+            log(f"readsets bench identical outputs: {all(same.values())} {same if not all(same.values()) else ''}")
+    return out
+
+
 def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
     """Large resident index: args.index_refs genomes (clades of 20, as above),
     built into the inverted index; two query sets counted in one batched
@@ -609,6 +722,10 @@ def main():
                     help="only time `matrix --format edges` on GENOMES clade genomes (GPU vs the reference "
                          "binary) and exit")
     ap.add_argument("--edges-cut", type=float, default=0.05)
+    ap.add_argument("--readsets-bench", type=int, default=0, metavar="GENOMES",
+                    help="only run the read-set pipeline on files (sketch, index, read-set sketches, "
+                         "--qraw --unique-ctx --estimate-coverage) against the reference binary and exit")
+    ap.add_argument("--readsets-samples", type=int, default=6)
     ap.add_argument("--index-build-only", type=int, default=0, metavar="GENOMES",
                     help="only time the index build of GENOMES clade genomes (host builder vs GPU) and exit")
     ap.add_argument("--tiling", default="",
@@ -636,6 +753,9 @@ def main():
     host = kh.Host()
     if args.edges_genomes:
         print(json.dumps({"matrix_edges": edges_bench(args, host, threads)}), flush=True)
+        return
+    if args.readsets_bench:
+        print(json.dumps({"readset_pipeline": readset_bench(args, host, threads)}), flush=True)
         return
     if args.index_build_only:
         # the index build alone: clade genomes as in the index bench

@@ -2329,14 +2329,8 @@ static void sketch_asone(char **files, int n, const sketch_opts_t *o, sample_key
 {
 	kssd_u64vec_t *part = xcalloc((size_t)n, sizeof *part);
 	int bad = -1;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(o->threads)
-	for (int i = 0; i < n; ++i)
-		if (kssd_collect_fastx_file(files[i], o->drfold, &part[i]) != 0) {
-#pragma omp critical
-			bad = i;
-		}
-	if (bad >= 0)
-		err(EXIT_FAILURE, "%s", files[bad]);
+	if (kssd_collect_files(files, n, o->drfold, o->threads, part, &bad) != 0)
+		err(EXIT_FAILURE, "%s", bad >= 0 ? files[bad] : "inputs");
 	size_t total = 0;
 	for (int i = 0; i < n; ++i)
 		total += part[i].n;
@@ -2385,16 +2379,20 @@ static int sketch_files(const char *outdir, char **files, int n, const sketch_op
 			kssd_print_count_filter(stderr, &o->counts, "as-one sketch", &sm[0].filter, 0);
 	} else {
 		int bad = -1;
+		kssd_u64vec_t *raw = xcalloc((size_t)n, sizeof *raw);
+		if (kssd_collect_files(files, n, o->drfold, o->threads, raw, &bad) != 0)
+			err(EXIT_FAILURE, "%s", bad >= 0 ? files[bad] : "inputs");
 #pragma omp parallel for schedule(dynamic, 1) num_threads(o->threads)
 		for (int i = 0; i < n; ++i) {
-			if (kssd_collect_fastx_file(files[i], o->drfold, &sm[i].keys) != 0 ||
-			    (o->meta && kssd_infile_meta_for_path(files[i], &mt[i]) != 0)) {
+			sm[i].keys = raw[i];
+			if (o->meta && kssd_infile_meta_for_path(files[i], &mt[i]) != 0) {
 #pragma omp critical
 				bad = i;
 				continue;
 			}
 			finish_sample(&sm[i], o);
 		}
+		free(raw);
 		if (bad >= 0)
 			err(EXIT_FAILURE, "%s", files[bad]);
 		if (report)

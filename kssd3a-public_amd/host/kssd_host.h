@@ -55,6 +55,12 @@ int kssd_sketch_fasta_file(const char *path, int drfold, int keep_conflict, kssd
  * out as they occur (unsorted, repeats kept): the input of the counted
  * read-set modes.  Returns 0 or -1 (errno set). */
 int kssd_collect_fastx_file(const char *path, int drfold, kssd_u64vec_t *out);
+/* kssd_collect_fastx_file for n files at once, out[i] for file i: plain
+ * files are mapped and cut into pieces of about 1 MB (FASTA records split
+ * with a 31-base overlap, FASTQ reads grouped) sketched by `threads`
+ * threads; gzip files go one per thread through the line reader.  The keys
+ * are those of the line reader.  0, or -1 with *bad the failing file. */
+int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_u64vec_t *out, int *bad);
 /* Sort v ascending and merge repeats; *counts (malloc'd, NULL when v is
  * empty) holds each key's number of occurrences (build_kv_from_vec,
  * command_sketch.c:4156-4187).  0 or -1 (allocation). */

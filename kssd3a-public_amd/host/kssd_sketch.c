@@ -10,6 +10,11 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <fcntl.h>
+#include <omp.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include "kssd_host.h"
@@ -468,4 +473,359 @@ int kssd_simsketch(const kssd_simspec_t *specs, int n, int drfold, int keep_conf
 	free(v);
 	*keys_out = keys;
 	return 0;
+}
+
+/* ------------------------------------------------------------ plain files, in parallel */
+/* Base code of an input byte (the Basemap of global_basic.c:131-142): 0..3
+ * for ACGT/acgt, 4 for anything else (a window reset), 5 for the line ends
+ * a record's lines are joined over. */
+static uint8_t g_code[256];
+static void init_codes(void)
+{
+	for (int c = 0; c < 256; ++c)
+		g_code[c] = 4;
+	g_code['A'] = g_code['a'] = 0;
+	g_code['C'] = g_code['c'] = 1;
+	g_code['G'] = g_code['g'] = 2;
+	g_code['T'] = g_code['t'] = 3;
+	g_code['\n'] = 5;
+}
+
+/* A run of sequence bytes: a FASTA record's lines (line ends joined; a CR
+ * before a line end, or at the end, dropped) or one FASTQ read line. */
+typedef struct {
+	const char *s;
+	size_t from, emit, to; /* sketch [from, to), keep k-mers ending at or past emit */
+	int file;
+} span_t;
+
+typedef struct {
+	span_t *v;
+	size_t n, cap;
+} spans_t;
+
+static void span_push(spans_t *x, span_t sp)
+{
+	if (x->n == x->cap) {
+		x->cap = x->cap ? 2 * x->cap : 1024;
+		x->v = realloc(x->v, x->cap * sizeof *x->v);
+		if (!x->v)
+			abort();
+	}
+	x->v[x->n++] = sp;
+}
+
+/* a CR that only precedes the line end (or the end) is dropped with it */
+static inline bool cr_at_line_end(const char *s, size_t i, size_t to)
+{
+	size_t j = i + 1;
+	while (j < to && s[j] == '\r')
+		++j;
+	return j == to || s[j] == '\n';
+}
+
+/* kssd_sketch_codes over raw bytes [from, to), line ends skipped */
+static void sketch_bytes(const char *s, size_t from, size_t emit, size_t to, int drfold, kssd_u64vec_t *out)
+{
+	const uint32_t filter = drfold >= 32 ? 0u : (uint32_t)(UINT32_MAX >> drfold);
+	const unsigned len_mv = 2 * KSSD_KLEN - 2;
+	uint64_t fwd = 0, rev = 0;
+	int base = 0;
+	for (size_t i = from; i < to; ++i) {
+		uint8_t b = g_code[(unsigned char)s[i]];
+		if (b == 5)
+			continue;
+		if (b == 4) {
+			if (s[i] == '\r' && cr_at_line_end(s, i, to))
+				continue;
+			base = 0;
+			fwd = rev = 0;
+			continue;
+		}
+		fwd = (fwd << 2) | b;
+		rev = (rev >> 2) | ((uint64_t)(b ^ 3u) << len_mv);
+		if (++base < KSSD_KLEN || i < emit)
+			continue;
+		const uint64_t tf = fwd & KSSD_CODEN_CTXMASK, tr = rev & KSSD_CODEN_CTXMASK;
+		const int use_fwd = tf < tr;
+		if ((uint32_t)mix64(use_fwd ? tf : tr) > filter)
+			continue;
+		if (out->n == out->cap && vec_reserve(out, out->n + 1))
+			abort();
+		out->a[out->n++] = coden_key(use_fwd ? fwd : rev);
+	}
+}
+
+/* start of a piece that begins sketching 31 positions before `at` */
+static size_t back_31(const char *s, size_t lo, size_t at)
+{
+	size_t i = at;
+	int pos = 0;
+	while (i > lo && pos < KSSD_KLEN - 1) {
+		--i;
+		const char c = s[i];
+		if (c == '\n' || (c == '\r' && cr_at_line_end(s, i, at)))
+			continue;
+		++pos;
+	}
+	return i;
+}
+
+#define PIECE_BYTES (1u << 20)
+
+/* Spans of one mapped plain file: the same reading as the line parser of
+ * kssd_collect_fastx_file (format from the first non-empty line, FASTQ in
+ * 4-line records, FASTA records split at '>' or ';' header lines, sequence
+ * before the first header kept), FASTA records cut into pieces of about
+ * PIECE_BYTES, FASTQ reads grouped about as much per piece. */
+static void file_spans(const char *s, size_t n, int file, spans_t *out)
+{
+	int fastq = -1;
+	long line_no = 0;
+	size_t rec = 0; /* FASTA: start of the current record's lines */
+	bool in_rec = false;
+	size_t i = 0;
+	while (i < n) {
+		const char *nl = memchr(s + i, '\n', n - i);
+		const size_t le = nl ? (size_t)(nl - s) : n; /* line [i, le) */
+		size_t l = le;
+		while (l > i && s[l - 1] == '\r')
+			--l;
+		const size_t next = nl ? le + 1 : n;
+		if (fastq < 0 && l > i)
+			fastq = s[i] == '@';
+		if (fastq == 1) {
+			if (line_no++ % 4 == 1 && l > i)
+				span_push(out, (span_t){s, i, i, l, file});
+		} else if (l > i && (s[i] == '>' || s[i] == ';')) {
+			if (in_rec && rec < i)
+				span_push(out, (span_t){s, rec, rec, i, file});
+			rec = next;
+			in_rec = true;
+		} else if (!in_rec) {
+			rec = i; /* sequence before any header */
+			in_rec = true;
+		}
+		i = next;
+	}
+	if (fastq != 1 && in_rec && rec < n)
+		span_push(out, (span_t){s, rec, rec, n, file});
+}
+
+/* First byte of the first non-empty line (a line of CRs counts as empty),
+ * n when there is none. */
+static size_t first_nonempty(const char *s, size_t n)
+{
+	size_t i = 0;
+	while (i < n) {
+		const char *nl = memchr(s + i, '\n', n - i);
+		const size_t le = nl ? (size_t)(nl - s) : n;
+		size_t l = le;
+		while (l > i && s[l - 1] == '\r')
+			--l;
+		if (l > i)
+			return i;
+		i = nl ? le + 1 : n;
+	}
+	return n;
+}
+
+/* The read lines of a large plain FASTQ file, found by `threads` threads:
+ * the file cut into chunks, each chunk's line ends counted, and each
+ * chunk's lines (those starting in it) parsed knowing their line number
+ * from the first non-empty line (4-line records, as file_spans reads them). */
+static void fastq_spans_parallel(const char *s, size_t n, size_t f0, int file, int threads, spans_t *out)
+{
+	const int nc = threads * 4;
+	size_t *b = malloc((size_t)(nc + 1) * sizeof *b), *nl = calloc((size_t)nc + 1, sizeof *nl);
+	spans_t *part = calloc((size_t)nc, sizeof *part);
+	if (!b || !nl || !part)
+		abort();
+	for (int k = 0; k <= nc; ++k)
+		b[k] = f0 + (n - f0) / (size_t)nc * (size_t)k;
+	b[nc] = n;
+#pragma omp parallel for schedule(static) num_threads(threads)
+	for (int k = 0; k < nc; ++k) {
+		size_t c = 0;
+		for (const char *p = s + b[k], *e = s + b[k + 1]; (p = memchr(p, '\n', (size_t)(e - p))) != NULL; ++p)
+			++c;
+		nl[k + 1] = c;
+	}
+	for (int k = 0; k < nc; ++k)
+		nl[k + 1] += nl[k]; /* line ends in [f0, b[k]) */
+#pragma omp parallel for schedule(static) num_threads(threads)
+	for (int k = 0; k < nc; ++k) {
+		size_t i = b[k], line = nl[k];
+		if (k > 0 && s[i - 1] != '\n') {
+			const char *q = memchr(s + i, '\n', n - i);
+			if (!q)
+				continue;
+			i = (size_t)(q - s) + 1;
+			++line;
+		}
+		while (i < b[k + 1]) {
+			const char *q = memchr(s + i, '\n', n - i);
+			const size_t le = q ? (size_t)(q - s) : n;
+			size_t l = le;
+			while (l > i && s[l - 1] == '\r')
+				--l;
+			if (line % 4 == 1 && l > i)
+				span_push(&part[k], (span_t){s, i, i, l, file});
+			++line;
+			i = q ? le + 1 : n;
+		}
+	}
+	for (int k = 0; k < nc; ++k) {
+		for (size_t j = 0; j < part[k].n; ++j)
+			span_push(out, part[k].v[j]);
+		free(part[k].v);
+	}
+	free(part);
+	free(b);
+	free(nl);
+}
+
+static void split_spans(const spans_t *in, spans_t *out)
+{
+	for (size_t k = 0; k < in->n; ++k) {
+		const span_t sp = in->v[k];
+		if (sp.to - sp.from <= PIECE_BYTES) {
+			span_push(out, sp);
+			continue;
+		}
+		for (size_t at = sp.from; at < sp.to; at += PIECE_BYTES) {
+			const size_t to = at + PIECE_BYTES < sp.to ? at + PIECE_BYTES : sp.to;
+			span_push(out, (span_t){sp.s, back_31(sp.s, sp.from, at), at, to, sp.file});
+		}
+	}
+}
+
+static bool gzip_magic(const char *path)
+{
+	FILE *fp = fopen(path, "rb");
+	if (!fp)
+		return false;
+	unsigned char m[2] = {0, 0};
+	const bool gz = fread(m, 1, 2, fp) == 2 && m[0] == 0x1f && m[1] == 0x8b;
+	fclose(fp);
+	return gz;
+}
+
+int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_u64vec_t *out, int *bad)
+{
+	static int inited;
+	if (!__atomic_load_n(&inited, __ATOMIC_ACQUIRE)) {
+#pragma omp critical(kssd_codes)
+		if (!inited) {
+			init_codes();
+			__atomic_store_n(&inited, 1, __ATOMIC_RELEASE);
+		}
+	}
+	*bad = -1;
+	char **map = calloc((size_t)(n ? n : 1), sizeof *map);
+	size_t *len = calloc((size_t)(n ? n : 1), sizeof *len);
+	bool *gz = calloc((size_t)(n ? n : 1), sizeof *gz);
+	if (!map || !len || !gz)
+		abort();
+	spans_t spans = {0}, pieces = {0};
+	int rc = 0;
+	for (int f = 0; f < n && !rc; ++f) {
+		gz[f] = gzip_magic(paths[f]);
+		if (gz[f])
+			continue;
+		const int fd = open(paths[f], O_RDONLY);
+		struct stat st;
+		if (fd < 0 || fstat(fd, &st) != 0) {
+			if (fd >= 0)
+				close(fd);
+			*bad = f;
+			rc = -1;
+			break;
+		}
+		len[f] = (size_t)st.st_size;
+		if (len[f]) {
+			void *m = mmap(NULL, len[f], PROT_READ, MAP_PRIVATE, fd, 0);
+			if (m == MAP_FAILED) {
+				close(fd);
+				*bad = f;
+				rc = -1;
+				break;
+			}
+			madvise(m, len[f], MADV_SEQUENTIAL | MADV_WILLNEED);
+			map[f] = m;
+			const size_t f0 = first_nonempty(map[f], len[f]);
+			if (threads > 1 && len[f] - f0 > (8u << 20) && map[f][f0] == '@')
+				fastq_spans_parallel(map[f], len[f], f0, f, threads, &spans);
+			else
+				file_spans(map[f], len[f], f, &spans);
+		}
+		close(fd);
+	}
+	if (!rc) {
+		split_spans(&spans, &pieces);
+		/* FASTQ reads grouped into pieces of about PIECE_BYTES */
+		size_t ng = 0;
+		size_t *gstart = malloc((pieces.n + 1) * sizeof *gstart);
+		if (!gstart)
+			abort();
+		for (size_t k = 0; k < pieces.n;) {
+			size_t e = k, bytes = 0;
+			while (e < pieces.n && pieces.v[e].file == pieces.v[k].file &&
+			       (e == k || bytes + (pieces.v[e].to - pieces.v[e].from) <= PIECE_BYTES)) {
+				bytes += pieces.v[e].to - pieces.v[e].from;
+				++e;
+			}
+			gstart[ng++] = k;
+			k = e;
+		}
+		gstart[ng] = pieces.n;
+		kssd_u64vec_t *part = calloc(ng ? ng : 1, sizeof *part);
+		int fail = -1;
+		if (!part)
+			abort();
+#pragma omp parallel num_threads(threads)
+		{
+#pragma omp for schedule(dynamic, 1) nowait
+			for (size_t g = 0; g < ng; ++g)
+				for (size_t k = gstart[g]; k < gstart[g + 1]; ++k) {
+					const span_t *sp = &pieces.v[k];
+					sketch_bytes(sp->s, sp->from, sp->emit, sp->to, drfold, &part[g]);
+				}
+			/* gzip files: one per thread through the line reader */
+#pragma omp for schedule(dynamic, 1)
+			for (int f = 0; f < n; ++f)
+				if (gz[f] && kssd_collect_fastx_file(paths[f], drfold, &out[f]) != 0) {
+#pragma omp critical(kssd_collect_fail)
+					fail = f;
+				}
+		}
+		if (fail >= 0) {
+			*bad = fail;
+			rc = -1;
+		}
+		for (size_t g = 0; g < ng && !rc; ++g) {
+			kssd_u64vec_t *o = &out[pieces.v[gstart[g]].file];
+			if (part[g].n) {
+				if (vec_reserve(o, o->n + part[g].n))
+					abort();
+				memcpy(o->a + o->n, part[g].a, part[g].n * sizeof *o->a);
+				o->n += part[g].n;
+			}
+		}
+		for (size_t g = 0; g < ng; ++g)
+			kssd_u64vec_free(&part[g]);
+		free(part);
+		free(gstart);
+	}
+	for (int f = 0; f < n; ++f)
+		if (map[f])
+			munmap(map[f], len[f]);
+	free(spans.v);
+	free(pieces.v);
+	free(map);
+	free(len);
+	free(gz);
+	if (rc)
+		errno = errno ? errno : EIO;
+	return rc;
 }

@@ -428,6 +428,8 @@ def load_kssdhost():
     lib.kssd_sketching_id.argtypes = [I, I, I, I]
     lib.kssd_sketching_id.restype = U32
     lib.kssd_simsketch.argtypes = [V, I, I, I, I, ctypes.POINTER(ctypes.POINTER(U64)), V]
+    lib.kssd_sim_genome.argtypes = [ctypes.POINTER(SimSpec), ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)),
+                                    ctypes.POINTER(S)]
     lib.kssd_build_index.argtypes = [V, V, I, U, I, V]
     lib.kssd_sample_ctx_counts.argtypes = [V, V, I, U, I, I, V]
     lib.kssd_sample_ctx_counts.restype = None
@@ -497,6 +499,16 @@ class Host:
         out = np.ctypeslib.as_array(keys, shape=(n,)).copy() if n else np.zeros(0, np.uint64)
         self._libc.free(ctypes.cast(keys, V))
         return out, index
+
+    def sim_genome(self, spec):
+        """The simulator's genome for `spec` as 2-bit base codes (uint8 0..3)."""
+        codes = ctypes.POINTER(ctypes.c_uint8)()
+        n = S()
+        if self.lib.kssd_sim_genome(ctypes.byref(spec), ctypes.byref(codes), ctypes.byref(n)) != 0:
+            raise RuntimeError("sim_genome failed")
+        out = np.ctypeslib.as_array(codes, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint8)
+        self._libc.free(ctypes.cast(codes, V))
+        return out
 
     def write_sketch(self, path, keys, index, names, drfold=8, conflict=False, records=None, unique_bits=None):
         """A sketch directory (lcofiles.stat, comblco, comblco.index) of

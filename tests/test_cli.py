@@ -100,6 +100,45 @@ def test_sketch_long_lines_and_gzip(tmp_path):
     assert meta[0:32][:16] == meta[32:64][:16]  # same record lengths summary
 
 
+def test_sketch_plain_pieces_match_line_reader(tmp_path):
+    """Plain files are mapped and sketched in pieces of about 1 MB by several
+    threads (FASTA records cut with a 31-base overlap, FASTQ reads grouped);
+    gzip files go through the line reader.  The same content both ways must
+    give the same keys and counts: a 3 Mbp two-record FASTA with 61-base
+    lines, CRLF line ends, lowercase and N runs around the piece boundaries,
+    sequence before the first header; and a 2.5 MB FASTQ."""
+    import gzip
+    import numpy as np
+    rng = np.random.default_rng(11)
+    letters = np.frombuffer(b"ACGTacgt", np.uint8)
+    seq = bytearray(letters[rng.integers(0, 8, 3_000_000)].tobytes())
+    for at in (1 << 20, 2 << 20, (1 << 20) + 45, 2_000_007):
+        seq[at - 3:at + 2] = b"NNNNN"
+    lines = [bytes(seq[i:i + 61]) for i in range(0, len(seq), 61)]
+    body = b"".join(ln + (b"\r\n" if k % 7 == 0 else b"\n") for k, ln in enumerate(lines))
+    fa = b"ACGTTGCAAGGCTTACGATCGATCGGATCCATGCA\n>r1 first\n" + body[:1_700_000] + b"\n>r2\n" + body[1_700_000:]
+    reads = []
+    for r in range(8000):
+        s0 = int(rng.integers(0, len(seq) - 150))
+        reads.append(b"@q%d\n%s\n+\n%s\n" % (r, bytes(seq[s0:s0 + 150]).upper(), b"I" * 150))
+    fq = b"".join(reads)
+    for name, data in (("g.fa", fa), ("r.fq", fq)):
+        (tmp_path / name).write_bytes(data)
+        (tmp_path / (name + ".gz")).write_bytes(gzip.compress(data, 1))
+    for args in (["-f", "4", "g.fa", "g.fa.gz"], ["-f", "2", "-A", "--conflict", "r.fq", "r.fq.gz"]):
+        out = tmp_path / ("o" + args[2])
+        p = subprocess.run([str(BIN), "sketch", "-p", "8", "-o", str(out), *args], capture_output=True, text=True,
+                           cwd=tmp_path, timeout=120)
+        assert p.returncode == 0, p.stderr
+        d = kt.read_sketch_dir(out)
+        assert len(d.sample(0)) > 1000
+        assert np.array_equal(d.sample(0), d.sample(1)), args
+        if "-A" in args:
+            a = np.fromfile(out / "comblco.a", dtype=np.uint32)
+            n0 = int(d.index[1])
+            assert np.array_equal(a[:n0], a[n0:])
+
+
 # sketch -i builds the index on the GPU by default; --host-index on the host
 BUILDERS = [pytest.param([], marks=pytest.mark.gpu, id="gpu"), pytest.param(["--host-index"], id="host")]
 

@@ -152,6 +152,7 @@ COVERAGE_CASES = {  # make_golden.py COVERAGE_CASES
     "cov_single": (["-r", "refs", "--qraw", "rs_asone", "--estimate-coverage", "-n", "0.5"], {}),
     "cov_noidx": (["-r", "refs_noidx", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5"], {}),
     "cov_noab": (["-r", "refs", "--qraw", "rs_plain", "--estimate-coverage", "-n", "0.5"], {}),
+    "cov_unique": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "--unique-ctx", "-n", "0.5"], {}),
     "cov_top1": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5", "-N", "1"],
                  {"KSSD3A_FORCE_REF_INDEX": "1"}),
 }
