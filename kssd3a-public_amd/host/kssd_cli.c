@@ -31,6 +31,7 @@
 #include <string.h>
 #include <sys/stat.h>
 #include <time.h>
+#include <unistd.h>
 
 #include "kssd_host.h"
 #include "kssd_report.h"
@@ -2598,12 +2599,16 @@ int main(int argc, char **argv)
 		return argc < 2 ? EX_USAGE : 0;
 	}
 	g_cmd = argv[1];
-	if (strcmp(g_cmd, "ani") == 0)
-		return cmd_ani(argc - 1, argv + 1);
-	if (strcmp(g_cmd, "matrix") == 0)
-		return cmd_matrix(argc - 1, argv + 1);
-	if (strcmp(g_cmd, "sketch") == 0)
-		return cmd_sketch(argc - 1, argv + 1);
-	fprintf(stderr, "%s: unknown command '%s' (ani, matrix, sketch)\n", PROG, g_cmd);
-	return EX_USAGE;
+	int rc;
+	if (strcmp(g_cmd, "ani") == 0) {
+		rc = cmd_ani(argc - 1, argv + 1);
+	} else if (strcmp(g_cmd, "matrix") == 0) {
+		rc = cmd_matrix(argc - 1, argv + 1);
+	} else if (strcmp(g_cmd, "sketch") == 0) {
+		rc = cmd_sketch(argc - 1, argv + 1);
+	} else {
+		fprintf(stderr, "%s: unknown command '%s' (ani, matrix, sketch)\n", PROG, g_cmd);
+		return EX_USAGE;
+	}
+	return rc;
 }

@@ -12,7 +12,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <fcntl.h>
+#include <stdbool.h>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 static char *path_join(const char *dir, const char *name)
@@ -34,23 +37,42 @@ bool kssd_file_in_dir(const char *dir, const char *name)
 
 void *kssd_read_file(const char *path, size_t *size)
 {
-	FILE *fp = fopen(path, "rb");
-	if (!fp)
+	const int fd = open(path, O_RDONLY);
+	struct stat st;
+	if (fd < 0)
 		return NULL;
-	if (fseeko(fp, 0, SEEK_END) != 0) {
-		fclose(fp);
+	if (fstat(fd, &st) != 0 || !S_ISREG(st.st_mode)) {
+		close(fd);
 		return NULL;
 	}
-	const off_t n = ftello(fp);
-	rewind(fp);
-	void *buf = malloc(n > 0 ? (size_t)n : 1);
-	if (!buf || (n > 0 && fread(buf, 1, (size_t)n, fp) != (size_t)n)) {
+	const size_t n = (size_t)st.st_size;
+	char *buf = malloc(n ? n : 1);
+	bool ok = buf != NULL;
+	/* large files in 8 MB pieces by several threads: copying from the page
+	 * cache is what reading the index and the keys costs */
+	const size_t piece = 8u << 20;
+	const size_t np = (n + piece - 1) / piece;
+#pragma omp parallel for schedule(dynamic, 1) if (np > 2) num_threads(np < 16 ? (int)np : 16)
+	for (size_t k = 0; k < np; ++k) {
+		if (!ok)
+			continue;
+		size_t at = k * piece;
+		const size_t end = at + piece < n ? at + piece : n;
+		while (at < end) {
+			const ssize_t r = pread(fd, buf + at, end - at, (off_t)at);
+			if (r <= 0) {
+				ok = false;
+				break;
+			}
+			at += (size_t)r;
+		}
+	}
+	close(fd);
+	if (!ok) {
 		free(buf);
-		fclose(fp);
 		return NULL;
 	}
-	fclose(fp);
-	*size = (size_t)n;
+	*size = n;
 	return buf;
 }
 
