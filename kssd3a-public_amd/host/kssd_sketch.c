@@ -497,6 +497,7 @@ typedef struct {
 	const char *s;
 	size_t from, emit, to; /* sketch [from, to), keep k-mers ending at or past emit */
 	int file;
+	int line; /* one line, no line ends inside (a FASTQ read) */
 } span_t;
 
 typedef struct {
@@ -522,6 +523,35 @@ static inline bool cr_at_line_end(const char *s, size_t i, size_t to)
 	while (j < to && s[j] == '\r')
 		++j;
 	return j == to || s[j] == '\n';
+}
+
+/* kssd_sketch_codes over one line of raw bytes (a FASTQ read): every
+ * non-ACGT byte resets the window */
+static void sketch_line(const char *s, size_t from, size_t to, int drfold, kssd_u64vec_t *out)
+{
+	const uint32_t filter = drfold >= 32 ? 0u : (uint32_t)(UINT32_MAX >> drfold);
+	const unsigned len_mv = 2 * KSSD_KLEN - 2;
+	uint64_t fwd = 0, rev = 0;
+	int base = 0;
+	for (size_t i = from; i < to; ++i) {
+		const uint64_t b = g_code[(unsigned char)s[i]];
+		if (__builtin_expect(b > 3, 0)) {
+			base = 0;
+			fwd = rev = 0;
+			continue;
+		}
+		fwd = (fwd << 2) | b;
+		rev = (rev >> 2) | ((b ^ 3u) << len_mv);
+		if (__builtin_expect(++base < KSSD_KLEN, 0))
+			continue;
+		const uint64_t tf = fwd & KSSD_CODEN_CTXMASK, tr = rev & KSSD_CODEN_CTXMASK;
+		const int use_fwd = tf < tr;
+		if (__builtin_expect((uint32_t)mix64(use_fwd ? tf : tr) > filter, 1))
+			continue;
+		if (out->n == out->cap && vec_reserve(out, out->n + 1))
+			abort();
+		out->a[out->n++] = coden_key(use_fwd ? fwd : rev);
+	}
 }
 
 /* kssd_sketch_codes over raw bytes [from, to), line ends skipped */
@@ -596,10 +626,10 @@ static void file_spans(const char *s, size_t n, int file, spans_t *out)
 			fastq = s[i] == '@';
 		if (fastq == 1) {
 			if (line_no++ % 4 == 1 && l > i)
-				span_push(out, (span_t){s, i, i, l, file});
+				span_push(out, (span_t){s, i, i, l, file, 1});
 		} else if (l > i && (s[i] == '>' || s[i] == ';')) {
 			if (in_rec && rec < i)
-				span_push(out, (span_t){s, rec, rec, i, file});
+				span_push(out, (span_t){s, rec, rec, i, file, 0});
 			rec = next;
 			in_rec = true;
 		} else if (!in_rec) {
@@ -609,7 +639,7 @@ static void file_spans(const char *s, size_t n, int file, spans_t *out)
 		i = next;
 	}
 	if (fastq != 1 && in_rec && rec < n)
-		span_push(out, (span_t){s, rec, rec, n, file});
+		span_push(out, (span_t){s, rec, rec, n, file, 0});
 }
 
 /* First byte of the first non-empty line (a line of CRs counts as empty),
@@ -670,7 +700,7 @@ static void fastq_spans_parallel(const char *s, size_t n, size_t f0, int file, i
 			while (l > i && s[l - 1] == '\r')
 				--l;
 			if (line % 4 == 1 && l > i)
-				span_push(&part[k], (span_t){s, i, i, l, file});
+				span_push(&part[k], (span_t){s, i, i, l, file, 1});
 			++line;
 			i = q ? le + 1 : n;
 		}
@@ -695,7 +725,7 @@ static void split_spans(const spans_t *in, spans_t *out)
 		}
 		for (size_t at = sp.from; at < sp.to; at += PIECE_BYTES) {
 			const size_t to = at + PIECE_BYTES < sp.to ? at + PIECE_BYTES : sp.to;
-			span_push(out, (span_t){sp.s, back_31(sp.s, sp.from, at), at, to, sp.file});
+			span_push(out, (span_t){sp.s, back_31(sp.s, sp.from, at), at, to, sp.file, sp.line});
 		}
 	}
 }
@@ -744,7 +774,7 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 		}
 		len[f] = (size_t)st.st_size;
 		if (len[f]) {
-			void *m = mmap(NULL, len[f], PROT_READ, MAP_PRIVATE, fd, 0);
+			void *m = mmap(NULL, len[f], PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
 			if (m == MAP_FAILED) {
 				close(fd);
 				*bad = f;
@@ -789,7 +819,10 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 			for (size_t g = 0; g < ng; ++g)
 				for (size_t k = gstart[g]; k < gstart[g + 1]; ++k) {
 					const span_t *sp = &pieces.v[k];
-					sketch_bytes(sp->s, sp->from, sp->emit, sp->to, drfold, &part[g]);
+					if (sp->line && sp->emit == sp->from)
+						sketch_line(sp->s, sp->from, sp->to, drfold, &part[g]);
+					else
+						sketch_bytes(sp->s, sp->from, sp->emit, sp->to, drfold, &part[g]);
 				}
 			/* gzip files: one per thread through the line reader */
 #pragma omp for schedule(dynamic, 1)
