@@ -19,6 +19,7 @@
  * --qrylist, --pair) and the --unique-ctx-report side output are not offered.
  */
 #define _GNU_SOURCE
+#include <dlfcn.h>
 #include <err.h>
 #include <errno.h>
 #include <getopt.h>
@@ -40,6 +41,61 @@
 
 #define PROG "kssd3a-hip"
 #define EX_USAGE 64
+
+/* ------------------------------------------------------------ the GPU library */
+/* libkssdhip.so (beside this binary) is opened on the first GPU call, so the
+ * host-only commands (sketch -o, sketch -i --host-index) start without
+ * loading the HIP runtime. */
+static struct {
+	const char *(*last_error)(void);
+	int (*init)(int);
+	int (*index_open)(const kssd_ctxgidobj_t *, size_t, int, unsigned, kssdhip_index **);
+	int (*index_close)(kssdhip_index *);
+	int (*build_index)(const uint64_t *, const uint64_t *, int, unsigned, kssd_ctxgidobj_t *, uint8_t *);
+	int (*count_block)(kssdhip_index *, const uint64_t *, const uint64_t *, int, int, uint32_t, kssd_ctx_mut2_t *,
+			   kssd_obj_section_t *, kssd_ctx_mut2_t *, kssd_obj_section_t *);
+	int (*count_votes)(kssdhip_index *, const uint64_t *, const uint64_t *, int, int, uint32_t, uint32_t,
+			   uint32_t *);
+} g_hip;
+
+static void *hip_sym(void *lib, const char *name)
+{
+	void *f = dlsym(lib, name);
+	if (!f)
+		errx(EXIT_FAILURE, "libkssdhip.so: missing %s", name);
+	return f;
+}
+
+static typeof(g_hip) *hip_api(void)
+{
+	if (g_hip.init)
+		return &g_hip;
+	char exe[PATH_MAX];
+	const ssize_t n = readlink("/proc/self/exe", exe, sizeof exe - 32);
+	if (n <= 0)
+		err(EXIT_FAILURE, "cannot locate %s", PROG);
+	exe[n] = '\0';
+	char *slash = strrchr(exe, '/');
+	strcpy(slash ? slash + 1 : exe, "libkssdhip.so");
+	void *lib = dlopen(exe, RTLD_NOW | RTLD_LOCAL);
+	if (!lib)
+		errx(EXIT_FAILURE, "GPU library: %s", dlerror());
+	g_hip.last_error = (const char *(*)(void))hip_sym(lib, "kssdhip_last_error");
+	g_hip.index_open = (typeof(g_hip.index_open))hip_sym(lib, "kssdhip_index_open");
+	g_hip.index_close = (typeof(g_hip.index_close))hip_sym(lib, "kssdhip_index_close");
+	g_hip.build_index = (typeof(g_hip.build_index))hip_sym(lib, "kssdhip_build_index");
+	g_hip.count_block = (typeof(g_hip.count_block))hip_sym(lib, "kssdhip_count_block");
+	g_hip.count_votes = (typeof(g_hip.count_votes))hip_sym(lib, "kssdhip_count_votes");
+	g_hip.init = (int (*)(int))hip_sym(lib, "kssdhip_init");
+	return &g_hip;
+}
+#define kssdhip_last_error() (hip_api()->last_error())
+#define kssdhip_init(...) (hip_api()->init(__VA_ARGS__))
+#define kssdhip_index_open(...) (hip_api()->index_open(__VA_ARGS__))
+#define kssdhip_index_close(...) (hip_api()->index_close(__VA_ARGS__))
+#define kssdhip_build_index(...) (hip_api()->build_index(__VA_ARGS__))
+#define kssdhip_count_block(...) (hip_api()->count_block(__VA_ARGS__))
+#define kssdhip_count_votes(...) (hip_api()->count_votes(__VA_ARGS__))
 
 /* ------------------------------------------------------------ small helpers */
 static const char *g_cmd = "";
@@ -2330,7 +2386,7 @@ static void sketch_asone(char **files, int n, const sketch_opts_t *o, sample_key
 {
 	kssd_u64vec_t *part = xcalloc((size_t)n, sizeof *part);
 	int bad = -1;
-	if (kssd_collect_files(files, n, o->drfold, o->threads, part, &bad) != 0)
+	if (kssd_collect_files(files, n, o->drfold, o->threads, part, &bad, NULL, NULL) != 0)
 		err(EXIT_FAILURE, "%s", bad >= 0 ? files[bad] : "inputs");
 	size_t total = 0;
 	for (int i = 0; i < n; ++i)
@@ -2381,19 +2437,31 @@ static int sketch_files(const char *outdir, char **files, int n, const sketch_op
 	} else {
 		int bad = -1;
 		kssd_u64vec_t *raw = xcalloc((size_t)n, sizeof *raw);
-		if (kssd_collect_files(files, n, o->drfold, o->threads, raw, &bad) != 0)
+		kssd_u64vec_t *lens = o->meta ? xcalloc((size_t)n, sizeof *lens) : NULL;
+		int *lens_done = xcalloc((size_t)n, sizeof *lens_done);
+		if (kssd_collect_files(files, n, o->drfold, o->threads, raw, &bad, lens, lens_done) != 0)
 			err(EXIT_FAILURE, "%s", bad >= 0 ? files[bad] : "inputs");
 #pragma omp parallel for schedule(dynamic, 1) num_threads(o->threads)
 		for (int i = 0; i < n; ++i) {
 			sm[i].keys = raw[i];
-			if (o->meta && kssd_infile_meta_for_path(files[i], &mt[i]) != 0) {
+			if (o->meta) {
+				/* record lengths from the sketching pass when it had them */
+				const int fmt = kssd_infile_fmt_from_path(files[i]);
+				if (lens_done[i])
+					kssd_infile_meta_from_lengths(lens[i].a, fmt == 1 ? lens[i].n : 0, fmt, 0,
+								      kssd_infile_flags_from_path(files[i]), &mt[i]);
+				else if (kssd_infile_meta_for_path(files[i], &mt[i]) != 0) {
 #pragma omp critical
-				bad = i;
-				continue;
+					bad = i;
+					continue;
+				}
+				kssd_u64vec_free(&lens[i]);
 			}
 			finish_sample(&sm[i], o);
 		}
 		free(raw);
+		free(lens);
+		free(lens_done);
 		if (bad >= 0)
 			err(EXIT_FAILURE, "%s", files[bad]);
 		if (report)

@@ -60,7 +60,11 @@ int kssd_collect_fastx_file(const char *path, int drfold, kssd_u64vec_t *out);
  * with a 31-base overlap, FASTQ reads grouped) sketched by `threads`
  * threads; gzip files go one per thread through the line reader.  The keys
  * are those of the line reader.  0, or -1 with *bad the failing file. */
-int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_u64vec_t *out, int *bad);
+int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_u64vec_t *out, int *bad,
+		       kssd_u64vec_t *fasta_lens, int *lens_done);
+/* (fasta_lens, when given: for the plain files read in one pass, the record
+ * lengths kssd_fasta_record_lengths would find, lens_done[i] = 1; empty for
+ * FASTQ.  Files left at lens_done[i] = 0 were not measured.) */
 /* Sort v ascending and merge repeats; *counts (malloc'd, NULL when v is
  * empty) holds each key's number of occurrences (build_kv_from_vec,
  * command_sketch.c:4156-4187).  0 or -1 (allocation). */

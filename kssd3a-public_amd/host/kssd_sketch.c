@@ -608,12 +608,23 @@ static size_t back_31(const char *s, size_t lo, size_t at)
  * 4-line records, FASTA records split at '>' or ';' header lines, sequence
  * before the first header kept), FASTA records cut into pieces of about
  * PIECE_BYTES, FASTQ reads grouped about as much per piece. */
-static void file_spans(const char *s, size_t n, int file, spans_t *out)
+static void lens_push(kssd_u64vec_t *v, uint64_t x)
+{
+	if (v->n == v->cap && vec_reserve(v, v->n + 1))
+		abort();
+	v->a[v->n++] = x;
+}
+
+static void file_spans(const char *s, size_t n, int file, spans_t *out, kssd_u64vec_t *lens)
 {
 	int fastq = -1;
 	long line_no = 0;
 	size_t rec = 0; /* FASTA: start of the current record's lines */
 	bool in_rec = false;
+	/* record lengths as kssd_fasta_record_lengths reads them: records open
+	 * at '>' lines, every other line adds its length less one trailing CR */
+	bool len_rec = false;
+	uint64_t cur = 0;
 	size_t i = 0;
 	while (i < n) {
 		const char *nl = memchr(s + i, '\n', n - i);
@@ -624,6 +635,16 @@ static void file_spans(const char *s, size_t n, int file, spans_t *out)
 		const size_t next = nl ? le + 1 : n;
 		if (fastq < 0 && l > i)
 			fastq = s[i] == '@';
+		if (lens) {
+			if (le > i && s[i] == '>') {
+				if (len_rec && cur)
+					lens_push(lens, cur);
+				len_rec = true;
+				cur = 0;
+			} else if (len_rec) { /* one CR before the line's LF is not counted */
+				cur += le - i - (nl && le > i && s[le - 1] == '\r');
+			}
+		}
 		if (fastq == 1) {
 			if (line_no++ % 4 == 1 && l > i)
 				span_push(out, (span_t){s, i, i, l, file, 1});
@@ -640,6 +661,8 @@ static void file_spans(const char *s, size_t n, int file, spans_t *out)
 	}
 	if (fastq != 1 && in_rec && rec < n)
 		span_push(out, (span_t){s, rec, rec, n, file, 0});
+	if (lens && len_rec && cur)
+		lens_push(lens, cur);
 }
 
 /* First byte of the first non-empty line (a line of CRs counts as empty),
@@ -741,7 +764,8 @@ static bool gzip_magic(const char *path)
 	return gz;
 }
 
-int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_u64vec_t *out, int *bad)
+int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_u64vec_t *out, int *bad,
+		       kssd_u64vec_t *fasta_lens, int *lens_done)
 {
 	static int inited;
 	if (!__atomic_load_n(&inited, __ATOMIC_ACQUIRE)) {
@@ -784,10 +808,13 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 			madvise(m, len[f], MADV_SEQUENTIAL | MADV_WILLNEED);
 			map[f] = m;
 			const size_t f0 = first_nonempty(map[f], len[f]);
-			if (threads > 1 && len[f] - f0 > (8u << 20) && map[f][f0] == '@')
+			if (threads > 1 && len[f] - f0 > (8u << 20) && map[f][f0] == '@') {
 				fastq_spans_parallel(map[f], len[f], f0, f, threads, &spans);
-			else
-				file_spans(map[f], len[f], f, &spans);
+			} else {
+				file_spans(map[f], len[f], f, &spans, fasta_lens ? &fasta_lens[f] : NULL);
+				if (lens_done)
+					lens_done[f] = fasta_lens != NULL;
+			}
 		}
 		close(fd);
 	}
