@@ -782,42 +782,65 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 	if (!map || !len || !gz)
 		abort();
 	spans_t spans = {0}, pieces = {0};
+	spans_t *fspans = calloc((size_t)(n ? n : 1), sizeof *fspans);
+	if (!fspans)
+		abort();
 	int rc = 0;
-	for (int f = 0; f < n && !rc; ++f) {
+	/* map the files and find their spans, several files at once (large
+	 * FASTQ files are scanned by all threads afterwards) */
+	bool *big_fq = calloc((size_t)(n ? n : 1), sizeof *big_fq);
+	size_t *fq0 = calloc((size_t)(n ? n : 1), sizeof *fq0);
+	if (!big_fq || !fq0)
+		abort();
+	int first_bad = n;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+	for (int f = 0; f < n; ++f) {
 		gz[f] = gzip_magic(paths[f]);
 		if (gz[f])
 			continue;
 		const int fd = open(paths[f], O_RDONLY);
 		struct stat st;
-		if (fd < 0 || fstat(fd, &st) != 0) {
-			if (fd >= 0)
-				close(fd);
-			*bad = f;
-			rc = -1;
-			break;
+		void *m = MAP_FAILED;
+		if (fd >= 0 && fstat(fd, &st) == 0) {
+			len[f] = (size_t)st.st_size;
+			m = len[f] ? mmap(NULL, len[f], PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0) : NULL;
 		}
-		len[f] = (size_t)st.st_size;
-		if (len[f]) {
-			void *m = mmap(NULL, len[f], PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
-			if (m == MAP_FAILED) {
-				close(fd);
-				*bad = f;
-				rc = -1;
-				break;
-			}
-			madvise(m, len[f], MADV_SEQUENTIAL | MADV_WILLNEED);
-			map[f] = m;
-			const size_t f0 = first_nonempty(map[f], len[f]);
-			if (threads > 1 && len[f] - f0 > (8u << 20) && map[f][f0] == '@') {
-				fastq_spans_parallel(map[f], len[f], f0, f, threads, &spans);
-			} else {
-				file_spans(map[f], len[f], f, &spans, fasta_lens ? &fasta_lens[f] : NULL);
-				if (lens_done)
-					lens_done[f] = fasta_lens != NULL;
-			}
+		if (fd >= 0)
+			close(fd);
+		if (fd < 0 || m == MAP_FAILED) {
+#pragma omp critical(kssd_collect_fail)
+			if (f < first_bad)
+				first_bad = f;
+			continue;
 		}
-		close(fd);
+		if (!len[f])
+			continue;
+		madvise(m, len[f], MADV_SEQUENTIAL | MADV_WILLNEED);
+		map[f] = m;
+		fq0[f] = first_nonempty(map[f], len[f]);
+		if (threads > 1 && len[f] - fq0[f] > (8u << 20) && map[f][fq0[f]] == '@') {
+			big_fq[f] = true;
+		} else {
+			file_spans(map[f], len[f], f, &fspans[f], fasta_lens ? &fasta_lens[f] : NULL);
+			if (lens_done)
+				lens_done[f] = fasta_lens != NULL;
+		}
 	}
+	if (first_bad < n) {
+		*bad = first_bad;
+		rc = -1;
+	}
+	for (int f = 0; f < n && !rc; ++f) {
+		if (big_fq[f])
+			fastq_spans_parallel(map[f], len[f], fq0[f], f, threads, &fspans[f]);
+		for (size_t k = 0; k < fspans[f].n; ++k)
+			span_push(&spans, fspans[f].v[k]);
+	}
+	for (int f = 0; f < n; ++f)
+		free(fspans[f].v);
+	free(fspans);
+	free(big_fq);
+	free(fq0);
 	if (!rc) {
 		split_spans(&spans, &pieces);
 		/* FASTQ reads grouped into pieces of about PIECE_BYTES */
