@@ -495,10 +495,13 @@ static void init_codes(void)
  * before a line end, or at the end, dropped) or one FASTQ read line. */
 typedef struct {
 	const char *s;
-	size_t from, emit, to; /* sketch [from, to), keep k-mers ending at or past emit */
+	size_t from, emit, to; /* sketch [from, to), keep k-mers ending at or past emit;
+				* for a FASTQ chunk, emit is its first line's number */
 	int file;
-	int line; /* one line, no line ends inside (a FASTQ read) */
+	int line; /* SPAN_LINE: one line, no line ends inside (a FASTQ read) */
 } span_t;
+
+enum { SPAN_BYTES = 0, SPAN_LINE = 1, SPAN_FASTQ_CHUNK = 2 };
 
 typedef struct {
 	span_t *v;
@@ -683,66 +686,72 @@ static size_t first_nonempty(const char *s, size_t n)
 	return n;
 }
 
-/* The read lines of a large plain FASTQ file, found by `threads` threads:
- * the file cut into chunks, each chunk's line ends counted, and each
- * chunk's lines (those starting in it) parsed knowing their line number
- * from the first non-empty line (4-line records, as file_spans reads them). */
-static void fastq_spans_parallel(const char *s, size_t n, size_t f0, int file, int threads, spans_t *out)
+/* A large plain FASTQ file cut into line-aligned chunks of about
+ * PIECE_BYTES for the sketching threads: the line ends of each raw cut
+ * counted in parallel, each cut moved forward past the end of the line it
+ * falls in, and each chunk (span kind SPAN_FASTQ_CHUNK) told the line number
+ * of its first line counted from the first non-empty line, so its worker
+ * knows which lines are sequence lines (4-line records, as file_spans reads
+ * them). */
+static void fastq_chunks(const char *s, size_t n, size_t f0, int file, int threads, spans_t *out)
 {
-	const int nc = threads * 4;
-	size_t *b = malloc((size_t)(nc + 1) * sizeof *b), *nl = calloc((size_t)nc + 1, sizeof *nl);
-	spans_t *part = calloc((size_t)nc, sizeof *part);
-	if (!b || !nl || !part)
+	const size_t nc = (n - f0 + PIECE_BYTES - 1) / PIECE_BYTES;
+	size_t *cut = malloc((nc + 1) * sizeof *cut), *nl = calloc(nc + 1, sizeof *nl);
+	if (!cut || !nl)
 		abort();
-	for (int k = 0; k <= nc; ++k)
-		b[k] = f0 + (n - f0) / (size_t)nc * (size_t)k;
-	b[nc] = n;
-#pragma omp parallel for schedule(static) num_threads(threads)
-	for (int k = 0; k < nc; ++k) {
+	for (size_t k = 0; k < nc; ++k)
+		cut[k] = f0 + k * PIECE_BYTES;
+	cut[nc] = n;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+	for (size_t k = 0; k < nc; ++k) {
 		size_t c = 0;
-		for (const char *p = s + b[k], *e = s + b[k + 1]; (p = memchr(p, '\n', (size_t)(e - p))) != NULL; ++p)
+		for (const char *p = s + cut[k], *e = s + cut[k + 1]; (p = memchr(p, '\n', (size_t)(e - p))) != NULL; ++p)
 			++c;
 		nl[k + 1] = c;
 	}
-	for (int k = 0; k < nc; ++k)
-		nl[k + 1] += nl[k]; /* line ends in [f0, b[k]) */
-#pragma omp parallel for schedule(static) num_threads(threads)
-	for (int k = 0; k < nc; ++k) {
-		size_t i = b[k], line = nl[k];
-		if (k > 0 && s[i - 1] != '\n') {
-			const char *q = memchr(s + i, '\n', n - i);
-			if (!q)
-				continue;
-			i = (size_t)(q - s) + 1;
-			++line;
+	for (size_t k = 0; k < nc; ++k)
+		nl[k + 1] += nl[k]; /* line ends in [f0, cut[k]) */
+	size_t prev = f0, prev_line = 0;
+	for (size_t k = 1; k <= nc; ++k) {
+		size_t at = cut[k], line = nl[k];
+		if (k < nc && s[at - 1] != '\n') {
+			const char *q = memchr(s + at, '\n', n - at);
+			at = q ? (size_t)(q - s) + 1 : n;
+			line += q != NULL;
 		}
-		while (i < b[k + 1]) {
-			const char *q = memchr(s + i, '\n', n - i);
-			const size_t le = q ? (size_t)(q - s) : n;
+		if (at > prev) {
+			span_push(out, (span_t){s, prev, prev_line, at, file, SPAN_FASTQ_CHUNK});
+			prev = at;
+			prev_line = line;
+		}
+	}
+	free(cut);
+	free(nl);
+}
+
+/* The sequence lines (line number 1 mod 4) of the whole lines in [from, to),
+ * `line` being the number of the first, trailing CRs dropped. */
+static void sketch_fastq_chunk(const char *s, size_t from, size_t to, size_t line, int drfold, kssd_u64vec_t *out)
+{
+	for (size_t i = from; i < to; ++line) {
+		const char *q = memchr(s + i, '\n', to - i);
+		const size_t le = q ? (size_t)(q - s) : to;
+		if (line % 4 == 1) {
 			size_t l = le;
 			while (l > i && s[l - 1] == '\r')
 				--l;
-			if (line % 4 == 1 && l > i)
-				span_push(&part[k], (span_t){s, i, i, l, file, 1});
-			++line;
-			i = q ? le + 1 : n;
+			if (l > i)
+				sketch_line(s, i, l, drfold, out);
 		}
+		i = q ? le + 1 : to;
 	}
-	for (int k = 0; k < nc; ++k) {
-		for (size_t j = 0; j < part[k].n; ++j)
-			span_push(out, part[k].v[j]);
-		free(part[k].v);
-	}
-	free(part);
-	free(b);
-	free(nl);
 }
 
 static void split_spans(const spans_t *in, spans_t *out)
 {
 	for (size_t k = 0; k < in->n; ++k) {
 		const span_t sp = in->v[k];
-		if (sp.to - sp.from <= PIECE_BYTES) {
+		if (sp.to - sp.from <= PIECE_BYTES || sp.line == SPAN_FASTQ_CHUNK) {
 			span_push(out, sp);
 			continue;
 		}
@@ -786,11 +795,11 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 	if (!fspans)
 		abort();
 	int rc = 0;
-	/* map the files and find their spans, several files at once (large
-	 * FASTQ files are scanned by all threads afterwards) */
-	bool *big_fq = calloc((size_t)(n ? n : 1), sizeof *big_fq);
+	/* map the files and find their spans, several files at once (FASTQ
+	 * files over 8 MB are cut into chunks by all threads afterwards) */
+	bool *fq_file = calloc((size_t)(n ? n : 1), sizeof *fq_file);
 	size_t *fq0 = calloc((size_t)(n ? n : 1), sizeof *fq0);
-	if (!big_fq || !fq0)
+	if (!fq_file || !fq0)
 		abort();
 	int first_bad = n;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
@@ -818,8 +827,11 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 		madvise(m, len[f], MADV_SEQUENTIAL | MADV_WILLNEED);
 		map[f] = m;
 		fq0[f] = first_nonempty(map[f], len[f]);
-		if (threads > 1 && len[f] - fq0[f] > (8u << 20) && map[f][fq0[f]] == '@') {
-			big_fq[f] = true;
+		if (fq0[f] < len[f] && map[f][fq0[f]] == '@') {
+			if (len[f] - fq0[f] <= (8u << 20))
+				fastq_chunks(map[f], len[f], fq0[f], f, 1, &fspans[f]);
+			else
+				fq_file[f] = true;
 		} else {
 			file_spans(map[f], len[f], f, &fspans[f], fasta_lens ? &fasta_lens[f] : NULL);
 			if (lens_done)
@@ -831,15 +843,15 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 		rc = -1;
 	}
 	for (int f = 0; f < n && !rc; ++f) {
-		if (big_fq[f])
-			fastq_spans_parallel(map[f], len[f], fq0[f], f, threads, &fspans[f]);
+		if (fq_file[f])
+			fastq_chunks(map[f], len[f], fq0[f], f, threads, &fspans[f]);
 		for (size_t k = 0; k < fspans[f].n; ++k)
 			span_push(&spans, fspans[f].v[k]);
 	}
 	for (int f = 0; f < n; ++f)
 		free(fspans[f].v);
 	free(fspans);
-	free(big_fq);
+	free(fq_file);
 	free(fq0);
 	if (!rc) {
 		split_spans(&spans, &pieces);
@@ -869,7 +881,9 @@ int kssd_collect_files(char *const *paths, int n, int drfold, int threads, kssd_
 			for (size_t g = 0; g < ng; ++g)
 				for (size_t k = gstart[g]; k < gstart[g + 1]; ++k) {
 					const span_t *sp = &pieces.v[k];
-					if (sp->line && sp->emit == sp->from)
+					if (sp->line == SPAN_FASTQ_CHUNK)
+						sketch_fastq_chunk(sp->s, sp->from, sp->to, sp->emit, drfold, &part[g]);
+					else if (sp->line && sp->emit == sp->from)
 						sketch_line(sp->s, sp->from, sp->to, drfold, &part[g]);
 					else
 						sketch_bytes(sp->s, sp->from, sp->emit, sp->to, drfold, &part[g]);

@@ -102,11 +102,12 @@ def test_sketch_long_lines_and_gzip(tmp_path):
 
 def test_sketch_plain_pieces_match_line_reader(tmp_path):
     """Plain files are mapped and sketched in pieces of about 1 MB by several
-    threads (FASTA records cut with a 31-base overlap, FASTQ reads grouped);
-    gzip files go through the line reader.  The same content both ways must
-    give the same keys and counts: a 3 Mbp two-record FASTA with 61-base
-    lines, CRLF line ends, lowercase and N runs around the piece boundaries,
-    sequence before the first header; and a 2.5 MB FASTQ."""
+    threads (FASTA records cut with a 31-base overlap, FASTQ files in
+    line-aligned chunks); gzip files go through the line reader.  The same
+    content both ways must give the same keys and counts: a 3 Mbp two-record
+    FASTA with 61-base lines, CRLF line ends, lowercase and N runs around the
+    piece boundaries, sequence before the first header; a 2.5 MB FASTQ; and
+    a 12 MB FASTQ with leading empty lines, CRLF records and a 1.5 Mbp read."""
     import gzip
     import numpy as np
     rng = np.random.default_rng(11)
@@ -122,10 +123,22 @@ def test_sketch_plain_pieces_match_line_reader(tmp_path):
         s0 = int(rng.integers(0, len(seq) - 150))
         reads.append(b"@q%d\n%s\n+\n%s\n" % (r, bytes(seq[s0:s0 + 150]).upper(), b"I" * 150))
     fq = b"".join(reads)
-    for name, data in (("g.fa", fa), ("r.fq", fq)):
+    # over 8 MB: cut into line-aligned chunks by all threads; leading empty
+    # lines, CRLF records, a 1.5 Mbp read spanning chunk cuts
+    big = [b"\n\r\n"]
+    for r in range(30000):
+        s0 = int(rng.integers(0, len(seq) - 150))
+        end = b"\r\n" if r % 5 == 0 else b"\n"
+        big.append(b"@b%d%s%s%s+%s%s%s" % (r, end, bytes(seq[s0:s0 + 150]).upper(), end, end, b"I" * 150, end))
+        if r == 3000:
+            big.append(b"@long\n%s\n+\n%s\n" % (bytes(seq[:1_500_000]).upper(), b"I" * 1_500_000))
+    fqb = b"".join(big)
+    assert len(fqb) > 8 << 20
+    for name, data in (("g.fa", fa), ("r.fq", fq), ("b.fq", fqb)):
         (tmp_path / name).write_bytes(data)
         (tmp_path / (name + ".gz")).write_bytes(gzip.compress(data, 1))
-    for args in (["-f", "4", "g.fa", "g.fa.gz"], ["-f", "2", "-A", "--conflict", "r.fq", "r.fq.gz"]):
+    for args in (["-f", "4", "g.fa", "g.fa.gz"], ["-f", "2", "-A", "--conflict", "r.fq", "r.fq.gz"],
+                 ["-f", "2", "-A", "--conflict", "b.fq", "b.fq.gz"]):
         out = tmp_path / ("o" + args[2])
         p = subprocess.run([str(BIN), "sketch", "-p", "8", "-o", str(out), *args], capture_output=True, text=True,
                            cwd=tmp_path, timeout=120)
