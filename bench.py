@@ -293,6 +293,10 @@ def readset_bench(args, host, threads):
         p = subprocess.run(argv, capture_output=True, cwd=cwd)
         if p.returncode:
             raise RuntimeError(f"{name}: {' '.join(argv[1:4])}...: {p.stderr.decode()[-400:]}")
+        if name == "gpu" and os.environ.get("KSSDHIP_TIMING"):  # the CLI's stage times
+            for ln in p.stderr.decode().splitlines():
+                if ln.startswith("[kssd3a-hip"):
+                    log(ln)
         return time.perf_counter() - t, p.stdout
 
     with tempfile.TemporaryDirectory(dir="/tmp") as d:

@@ -485,8 +485,7 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 				rc = kssdhip_fail(KSSDHIP_ENOMEM, "index_open: staging allocation failed");
 				break;
 			}
-			if (hipMemcpyAsync(d_raw, records, n * sizeof(kssd_ctxgidobj_t), hipMemcpyHostToDevice,
-					   ix->stream) != hipSuccess) {
+			if (copy_h2d(d_raw, records, n * sizeof(kssd_ctxgidobj_t), ix->stream) != hipSuccess) {
 				rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: upload failed");
 				break;
 			}
@@ -687,7 +686,7 @@ extern "C" int kssdhip_count_block(kssdhip_index *ix, const uint64_t *section_sk
 		return rc;
 	hipStream_t s = ix->stream;
 	if (total)
-		HIP_TRY(hipMemcpyAsync(ix->keys.ptr, section_sk, total * 8, hipMemcpyHostToDevice, s));
+		HIP_TRY(copy_h2d(ix->keys.ptr, section_sk, total * 8, s));
 	HIP_TRY(hipMemcpyAsync(ix->skidx.ptr, rel.data(), rel.size() * 8, hipMemcpyHostToDevice, s));
 	rc = kssdhip_count_block_dev(ix, (const uint64_t *)ix->keys.ptr, (const uint64_t *)ix->skidx.ptr,
 				     nq, total, qry_gid_offset, flags, (kssd_ctx_mut2_t *)ix->ctx.ptr,
@@ -696,11 +695,11 @@ extern "C" int kssdhip_count_block(kssdhip_index *ix, const uint64_t *section_sk
 				     unique ? (kssd_obj_section_t *)ix->uobj.ptr : nullptr, s);
 	if (rc)
 		return rc;
-	HIP_TRY(hipMemcpyAsync(ctx, ix->ctx.ptr, cells * 8, hipMemcpyDeviceToHost, s));
-	HIP_TRY(hipMemcpyAsync(obj, ix->obj.ptr, cells * 8, hipMemcpyDeviceToHost, s));
+	HIP_TRY(copy_d2h(ctx, ix->ctx.ptr, cells * 8, s));
+	HIP_TRY(copy_d2h(obj, ix->obj.ptr, cells * 8, s));
 	if (unique) {
-		HIP_TRY(hipMemcpyAsync(uctx, ix->uctx.ptr, cells * 8, hipMemcpyDeviceToHost, s));
-		HIP_TRY(hipMemcpyAsync(uobj, ix->uobj.ptr, cells * 8, hipMemcpyDeviceToHost, s));
+		HIP_TRY(copy_d2h(uctx, ix->uctx.ptr, cells * 8, s));
+		HIP_TRY(copy_d2h(uobj, ix->uobj.ptr, cells * 8, s));
 	}
 	HIP_TRY(hipStreamSynchronize(s));
 	return 0;
@@ -734,7 +733,7 @@ extern "C" int kssdhip_count_votes(kssdhip_index *ix, const uint64_t *section_sk
 		return rc;
 	hipStream_t s = ix->stream;
 	if (total)
-		HIP_TRY(hipMemcpyAsync(ix->keys.ptr, section_sk, total * 8, hipMemcpyHostToDevice, s));
+		HIP_TRY(copy_h2d(ix->keys.ptr, section_sk, total * 8, s));
 	HIP_TRY(hipMemcpyAsync(ix->skidx.ptr, rel.data(), rel.size() * 8, hipMemcpyHostToDevice, s));
 	rc = kssd_count_dev(ix, (const uint64_t *)ix->keys.ptr, (const uint64_t *)ix->skidx.ptr, nq, total,
 			    qry_gid_offset, KSSDHIP_LOWER, (kssd_ctx_mut2_t *)ix->ctx.ptr,
@@ -746,7 +745,7 @@ extern "C" int kssdhip_count_votes(kssdhip_index *ix, const uint64_t *section_sk
 		hipLaunchKernelGGL(k_low_words, dim3(grid_for(cells)), dim3(256), 0, s,
 				   (const unsigned long long *)ix->ctx.ptr, cells, (uint32_t *)ix->obj.ptr);
 	HIP_TRY(hipGetLastError());
-	HIP_TRY(hipMemcpyAsync(votes, ix->obj.ptr, cells * 4, hipMemcpyDeviceToHost, s));
+	HIP_TRY(copy_d2h(votes, ix->obj.ptr, cells * 4, s));
 	HIP_TRY(hipStreamSynchronize(s));
 	return 0;
 }

@@ -541,7 +541,7 @@ extern "C" int kssdhip_build_index(const uint64_t *keys, const uint64_t *skidx, 
 	    (rc = m.get(&sums, nsum)) || (rc = m.get(&bits, nwords)) || (rc = m.get(&flags, 4)) ||
 	    (rc = m.get(&large, large_cap)) || (rc = m.get(&over, large_cap)) || (rc = m.get(&d_out, n)))
 		return rc;
-	hipError_t e = hipMemcpyAsync(d_keys, keys, n * 8, hipMemcpyHostToDevice, s);
+	hipError_t e = copy_h2d(d_keys, keys, n * 8, s);
 	if (e == hipSuccess)
 		e = hipMemcpyAsync(d_skidx, skidx, ((size_t)nsamples + 1) * 8, hipMemcpyHostToDevice, s);
 	if (e == hipSuccess)
@@ -613,7 +613,7 @@ extern "C" int kssdhip_build_index(const uint64_t *keys, const uint64_t *skidx, 
 				return kssdhip_fail(KSSDHIP_EHIP, "build_index: %s", hipGetErrorString(e));
 		}
 	}
-	e = hipMemcpyAsync(records, d_out, n * sizeof(kssd_ctxgidobj_t), hipMemcpyDeviceToHost, s);
+	e = copy_d2h(records, d_out, n * sizeof(kssd_ctxgidobj_t), s);
 	if (e == hipSuccess && unique_bits) {
 		std::vector<uint32_t> hb;
 		e = hipStreamSynchronize(s);

@@ -19,6 +19,13 @@ __device__ __forceinline__ int dna_popcount_dev(uint32_t x)
 	return __popc(x);
 }
 
+// Copies between pageable host memory and the device, staged through pinned
+// buffers by several host threads when large (kssdhip_copy.hip). copy_h2d
+// returns once the source may be reused (the copy is queued on s); copy_d2h
+// returns with dst filled.
+hipError_t copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
+hipError_t copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 struct JoinArgs {
 	const uint64_t *ctxgid;
 	const uint32_t *robj;

@@ -25,6 +25,7 @@
 #include <getopt.h>
 #include <limits.h>
 #include <omp.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stddef.h>
 #include <stdio.h>
@@ -41,6 +42,29 @@
 
 #define PROG "kssd3a-hip"
 #define EX_USAGE 64
+
+/* Every error exit first waits for the thread that may be starting the HIP
+ * runtime (hip_start), so the process never exits while it runs. */
+static void hip_settle(void);
+static void stage(const char *what);
+static void __attribute__((noreturn, format(printf, 2, 3))) cli_errx(int code, const char *fmt, ...)
+{
+	hip_settle();
+	va_list ap;
+	va_start(ap, fmt);
+	verrx(code, fmt, ap);
+}
+static void __attribute__((noreturn, format(printf, 2, 3))) cli_err(int code, const char *fmt, ...)
+{
+	const int e = errno;
+	hip_settle();
+	errno = e;
+	va_list ap;
+	va_start(ap, fmt);
+	verr(code, fmt, ap);
+}
+#define errx(...) cli_errx(__VA_ARGS__)
+#define err(...) cli_err(__VA_ARGS__)
 
 /* ------------------------------------------------------------ the GPU library */
 /* libkssdhip.so (beside this binary) is opened on the first GPU call, so the
@@ -66,7 +90,7 @@ static void *hip_sym(void *lib, const char *name)
 	return f;
 }
 
-static typeof(g_hip) *hip_api(void)
+static typeof(g_hip) *hip_load(void)
 {
 	if (g_hip.init)
 		return &g_hip;
@@ -88,6 +112,49 @@ static typeof(g_hip) *hip_api(void)
 	g_hip.count_votes = (typeof(g_hip.count_votes))hip_sym(lib, "kssdhip_count_votes");
 	g_hip.init = (int (*)(int))hip_sym(lib, "kssdhip_init");
 	return &g_hip;
+}
+
+/* The GPU library loaded and the HIP runtime started on a thread of its own
+ * (about 25 ms) while the command reads its inputs; the first GPU call waits
+ * for it and selects the device on the calling thread (the library keeps the
+ * device and its error text per thread). */
+static struct {
+	pthread_t thread;
+	int device;
+	bool pending;
+} g_hip_start;
+
+static void *hip_start_main(void *arg)
+{
+	(void)arg;
+	(void)hip_load()->init(g_hip_start.device);
+	return NULL;
+}
+
+static void hip_start(int device)
+{
+	g_hip_start.device = device;
+	if (!g_hip.init && pthread_create(&g_hip_start.thread, NULL, hip_start_main, NULL) == 0)
+		g_hip_start.pending = true;
+}
+
+static void hip_settle(void)
+{
+	if (g_hip_start.pending && !pthread_equal(pthread_self(), g_hip_start.thread)) {
+		g_hip_start.pending = false;
+		pthread_join(g_hip_start.thread, NULL);
+	}
+}
+
+static typeof(g_hip) *hip_api(void)
+{
+	if (g_hip_start.pending) {
+		hip_settle();
+		if (hip_load()->init(g_hip_start.device) != 0)
+			errx(EXIT_FAILURE, "no usable GPU: %s", g_hip.last_error());
+		stage("GPU ready");
+	}
+	return hip_load();
 }
 #define kssdhip_last_error() (hip_api()->last_error())
 #define kssdhip_init(...) (hip_api()->init(__VA_ARGS__))
@@ -117,6 +184,7 @@ static void stage(const char *what)
 
 static void __attribute__((noreturn, format(printf, 1, 2))) usage_error(const char *fmt, ...)
 {
+	hip_settle();
 	va_list ap;
 	fprintf(stderr, "%s %s: ", PROG, g_cmd);
 	va_start(ap, fmt);
@@ -556,6 +624,8 @@ static kssd_ctxgidobj_t *build_records(const kssd_sketchdir_t *s, int threads)
 static void write_index_for(const char *dir, int threads, bool host_index, int device)
 {
 	kssd_sketchdir_t s;
+	if (!host_index)
+		hip_start(device);
 	open_dir(dir, KSSD_SK_KEYS, &s);
 	const size_t n = s.index[s.n];
 	kssd_ctxgidobj_t *rec;
@@ -1205,8 +1275,7 @@ static int cmd_ani(int argc, char **argv)
 	parse_ani(argc, argv, &a);
 	if (kssd_report_init() != 0)
 		errx(EXIT_FAILURE, "calibrated ANI model tables are damaged");
-	hip_check(kssdhip_init(a.device), "no usable GPU");
-	stage("GPU ready");
+	hip_start(a.device);
 	return run_ani(&a);
 }
 
@@ -2282,8 +2351,7 @@ static int cmd_matrix(int argc, char **argv)
 {
 	matrix_args_t a;
 	parse_matrix(argc, argv, &a);
-	hip_check(kssdhip_init(a.device), "no usable GPU");
-	stage("GPU ready");
+	hip_start(a.device);
 	if (a.format == 3)
 		return a.refdir[0] ? matrix_edges_rect(&a) : matrix_edges_self(&a);
 	if (a.format == 4 || a.format == 5)
@@ -2678,5 +2746,6 @@ int main(int argc, char **argv)
 		fprintf(stderr, "%s: unknown command '%s' (ani, matrix, sketch)\n", PROG, g_cmd);
 		return EX_USAGE;
 	}
+	hip_settle();
 	return rc;
 }

@@ -154,6 +154,40 @@ def test_larger_random_block(hip, oracle):
     assert all(np.array_equal(g, w) for g, w in zip(got, again))
 
 
+def test_staged_host_copies(hip, oracle):
+    """Host buffers over 8 MiB are copied through pinned chunks of 2 MiB by
+    four host threads, unevenly shared here (7 and 5 chunks, a 11 KiB tail):
+    a 12.3 MiB index upload, 8.8 MiB of query keys up and four 8.01 MiB
+    counter arrays down (unique counters on), and the index builder's
+    8.2 MiB of keys up and 12.3 MiB of records down."""
+    rng = np.random.default_rng(777)
+    pool = rng.integers(1, 1 << 43, size=200000, dtype=np.uint64)
+
+    def samples(n, mean):  # contexts drawn from the shared pool (repeats: conflicts), random objects
+        out = []
+        for _ in range(n):
+            m = int(rng.integers(mean // 2, mean * 3 // 2))
+            ctx = pool[rng.integers(0, len(pool), size=m)]
+            out.append(np.unique((ctx << np.uint64(OBJ_BITS)) | rng.integers(0, 1 << OBJ_BITS, size=m, dtype=np.uint64)))
+        return out
+
+    refs, qrys = samples(300, 3600), samples(3500, 330)
+    rkeys, ridx = _block(refs)
+    qkeys, qidx = _block(qrys)
+    rec = kt.build_index_np(rkeys, ridx, OBJ_BITS)
+    assert min(rec.nbytes, qkeys.nbytes, rkeys.nbytes, len(refs) * len(qrys) * 8) > 8 << 20
+    built, _ = hip.build_index(rkeys, ridx)
+    assert built.tobytes() == rec.tobytes()
+    h = hip.open_index(rec, len(refs), OBJ_BITS)
+    try:
+        got = hip.count_block(h, len(refs), qkeys, qidx, flags=6)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, len(refs), OBJ_BITS, qkeys, qidx, flags=6)
+    assert len(got) == len(want) == 4
+    assert all(np.array_equal(g, w) for g, w in zip(got, want))
+
+
 def test_index_open_rejects_bad_index(hip):
     refs, _ = _fixture(3, n_ref=4)
     keys, idx = _block(refs)
