@@ -297,6 +297,7 @@ def readset_bench(args, host, threads):
             for ln in p.stderr.decode().splitlines():
                 if ln.startswith("[kssd3a-hip"):
                     log(ln)
+            log(f"[{argv[1]} wall] {time.perf_counter() - t:.3f} s")
         return time.perf_counter() - t, p.stdout
 
     with tempfile.TemporaryDirectory(dir="/tmp") as d:

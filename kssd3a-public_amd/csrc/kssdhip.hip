@@ -15,6 +15,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -43,6 +45,25 @@ extern "C" const char *kssdhip_last_error(void) { return g_err; }
 			return kssdhip_fail(KSSDHIP_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
 	} while (0)
 
+// The first stream made on a device costs about 12 ms. kssdhip_init makes
+// one per device ahead of time (once per process), and the first index
+// opened on that device takes it over, so a caller that runs kssdhip_init
+// on a thread of its own while it reads its inputs gets the cost off its
+// critical path.
+static std::mutex g_spare_mu;
+static hipStream_t g_spare[64];
+static bool g_spare_made[64];
+
+static hipStream_t take_spare_stream(int device)
+{
+	std::lock_guard<std::mutex> lock(g_spare_mu);
+	if (device < 0 || device >= 64)
+		return nullptr;
+	hipStream_t st = g_spare[device];
+	g_spare[device] = nullptr;
+	return st;
+}
+
 extern "C" int kssdhip_init(int device)
 {
 	int n = 0;
@@ -53,6 +74,14 @@ extern "C" int kssdhip_init(int device)
 	HIP_TRY(hipSetDevice(device));
 	g_device = device;
 	g_err[0] = '\0';
+	if (device < 64) {
+		std::lock_guard<std::mutex> lock(g_spare_mu);
+		if (!g_spare_made[device]) {
+			g_spare_made[device] = true;
+			if (hipStreamCreateWithFlags(&g_spare[device], hipStreamNonBlocking) != hipSuccess)
+				g_spare[device] = nullptr; // index_open makes its own
+		}
+	}
 	return 0;
 }
 
@@ -425,9 +454,28 @@ static int bit_width_u64(uint64_t v)
 	return v ? 64 - __builtin_clzll(v) : 0;
 }
 
+// KSSDHIP_TIMING set: the phases of kssdhip_index_open on stderr, each
+// ended by a stream synchronisation of its own
+static void open_phase(hipStream_t s, const char *what)
+{
+	static const bool on = getenv("KSSDHIP_TIMING") != nullptr;
+	static double last = 0;
+	if (!on)
+		return;
+	if (s)
+		(void)hipStreamSynchronize(s);
+	timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	const double t = ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+	if (last > 0)
+		fprintf(stderr, "[kssd3a-hip index_open] %7.2f ms  %s\n", t - last, what);
+	last = t;
+}
+
 extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int ref_gnum,
 				  unsigned obj_bits, kssdhip_index **out)
 {
+	open_phase(nullptr, "start");
 	if (!out)
 		return kssdhip_fail(KSSDHIP_EINVAL, "index_open: null output handle");
 	*out = nullptr;
@@ -467,10 +515,12 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 	const size_t nb = ((size_t)1 << k) + 1;
 	do {
 		if (hipSetDevice(ix->device) != hipSuccess ||
-		    hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking) != hipSuccess) {
+		    (!(ix->stream = take_spare_stream(ix->device)) &&
+		     hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking) != hipSuccess)) {
 			rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: stream creation failed");
 			break;
 		}
+		open_phase(nullptr, "stream created");
 		// padded so 16-byte vector loads of aligned 4-record groups that
 		// straddle the end stay inside the allocation
 		const size_t nalloc = n + INDEX_PAD;
@@ -480,6 +530,7 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 			rc = kssdhip_fail(KSSDHIP_ENOMEM, "index_open: device allocation of %zu records failed", n);
 			break;
 		}
+		open_phase(nullptr, "index arrays allocated");
 		if (n) {
 			if (hipMalloc(&d_raw, n * sizeof(kssd_ctxgidobj_t)) != hipSuccess) {
 				rc = kssdhip_fail(KSSDHIP_ENOMEM, "index_open: staging allocation failed");
@@ -489,6 +540,7 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 				rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: upload failed");
 				break;
 			}
+			open_phase(ix->stream, "records uploaded");
 			hipLaunchKernelGGL(k_unpack_records, dim3(grid_for(n)), dim3(256), 0, ix->stream,
 					   (const uint32_t *)d_raw, (uint64_t)n, ix->d_ctxgid, ix->d_obj);
 			uint32_t *d_bad = (uint32_t *)d_raw; // staging no longer needed after unpack
@@ -507,6 +559,7 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 				break;
 			}
 		}
+		open_phase(nullptr, "unpacked and validated");
 		hipLaunchKernelGGL(k_buckets, dim3(grid_for(nb)), dim3(256), 0, ix->stream,
 				   ix->d_ctxgid, (uint64_t)n, k, ix->shift, ix->d_bucket);
 		if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ix->stream) != hipSuccess) {
@@ -516,8 +569,10 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 		ix->mode = 0;
 		ix->block = 256;
 		ix->auto_min_keys = 512.0;
+		open_phase(nullptr, "bucket table");
 		if ((rc = kssd_build_partitions(ix, 4096)))
 			break;
+		open_phase(nullptr, "partition table");
 	} while (0);
 	if (d_raw)
 		(void)hipFree(d_raw);

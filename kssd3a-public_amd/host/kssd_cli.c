@@ -920,6 +920,7 @@ static int run_index_path(const ani_args_t *a)
 	}
 	close_out(fp);
 	kssdhip_index_close(ix);
+	stage("index released");
 	free(cv.qcounts);
 	free(cv.bits);
 	free(ctx);

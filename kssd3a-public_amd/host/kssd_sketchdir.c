@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <fcntl.h>
+#include <omp.h>
 #include <stdbool.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -49,10 +50,14 @@ void *kssd_read_file(const char *path, size_t *size)
 	char *buf = malloc(n ? n : 1);
 	bool ok = buf != NULL;
 	/* large files in 8 MB pieces by several threads: copying from the page
-	 * cache is what reading the index and the keys costs */
+	 * cache is what reading the index and the keys costs (one core left to
+	 * whatever the caller runs beside, such as starting the GPU runtime) */
 	const size_t piece = 8u << 20;
 	const size_t np = (n + piece - 1) / piece;
-#pragma omp parallel for schedule(dynamic, 1) if (np > 2) num_threads(np < 16 ? (int)np : 16)
+	int nt = omp_get_num_procs() - 1;
+	nt = nt < 1 ? 1 : nt > 16 ? 16 : nt;
+	nt = np < (size_t)nt ? (int)np : nt;
+#pragma omp parallel for schedule(dynamic, 1) if (np > 2) num_threads(nt)
 	for (size_t k = 0; k < np; ++k) {
 		if (!ok)
 			continue;
