@@ -131,11 +131,21 @@ hipError_t staged_copy(void *dst, const void *src, size_t bytes, hipStream_t s, 
 		return e;
 	std::atomic<int> err{0};
 	std::vector<std::thread> th;
-	th.reserve(COPY_THREADS);
-	for (int w = 0; w < COPY_THREADS; ++w)
-		th.emplace_back(h2d ? h2d_worker : d2h_worker, dev, p, w, (char *)dst, (const char *)src, bytes, s, &err);
+	bool started = true;
+	try {
+		th.reserve(COPY_THREADS);
+		for (int w = 0; w < COPY_THREADS; ++w)
+			th.emplace_back(h2d ? h2d_worker : d2h_worker, dev, p, w, (char *)dst, (const char *)src, bytes, s,
+					&err);
+	} catch (...) { // no thread to be had: the workers started finish, then one plain copy of it all
+		started = false;
+	}
 	for (std::thread &t : th)
 		t.join();
+	if (!started) {
+		e = hipMemcpyAsync(dst, src, bytes, h2d ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, s);
+		return e == hipSuccess && !h2d ? hipStreamSynchronize(s) : e;
+	}
 	return (hipError_t)err.load();
 }
 
