@@ -62,7 +62,34 @@ def save_cache(cache, path=DEFAULT_CACHE):
         pass
 
 
-def candidates():
+def cu_count(rt):
+    """Compute units of the runtime's device (hipDeviceAttributeMultiprocessorCount = 63)."""
+    try:
+        v = ctypes.c_int(0)
+        rt.lib.hipDeviceGetAttribute.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
+        rt.check(rt.lib.hipDeviceGetAttribute(ctypes.byref(v), 63, rt.device))
+        return v.value
+    except Exception:
+        return 0
+
+
+def fitted_chunks(n_index, cus):
+    """(chunk, group) pairs whose partition groups come to one to four whole
+    rounds of workgroups over the compute units (one workgroup per CU at a
+    time), where the fixed chunk sizes leave the last round part-empty."""
+    out = []
+    if n_index <= 0 or cus <= 0:
+        return out
+    for g in GROUPS:
+        for rounds in (1, 2, 3, 4):
+            c = -(-n_index // (cus * g * rounds))
+            c = (c + 63) // 64 * 64
+            if GROUP_CHUNKS[0] <= c <= GROUP_CHUNKS[-1] and c not in GROUP_CHUNKS and (c, g) not in out:
+                out.append((c, g))
+    return out
+
+
+def candidates(n_index=0, cus=0):
     yield {"mode": 1, "chunk_records": 4096, "block_threads": 256, "group_partitions": 1}
     for c in CHUNKS:
         for b in BLOCKS:
@@ -71,6 +98,9 @@ def candidates():
         for b in GROUP_BLOCKS:
             for g in GROUPS:
                 yield {"mode": 2, "chunk_records": c, "block_threads": b, "group_partitions": g}
+    for c, g in fitted_chunks(n_index, cus):
+        for b in GROUP_BLOCKS:
+            yield {"mode": 2, "chunk_records": c, "block_threads": b, "group_partitions": g}
 
 
 def candidate_name(c):
@@ -95,7 +125,7 @@ def tune(hip, rt, h, d_keys, d_skidx, nq, total, nref, n_index, d_ctx, d_obj, st
     e0, e1 = rt.event(), rt.event()
     times = {}
     best, best_ms = None, float("inf")
-    for cand in candidates():
+    for cand in candidates(n_index, cu_count(rt)):
         hip.set_tiling(h, **cand)
         hip.count_block_dev(h, d_keys, d_skidx, nq, total, 0, flags, d_ctx, d_obj, None, None, stream)
         rt.sync(stream)
