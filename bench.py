@@ -103,16 +103,17 @@ def measured_traffic(tiling, nref, nqry):
     FETCH_SIZE/WRITE_SIZE summary of this workload, when it was measured
     with the tiling this run uses; (bytes or None, summary file name)."""
     import re
-    f = ROOT / "profiles" / f"pjoin_split_c2_{nref}x{nqry}_traffic.md"
-    if not f.exists():
+    files = sorted((ROOT / "profiles").glob(f"pjoin_split_c2_{nref}x{nqry}_traffic*.md"))
+    if not files:
         return None, None
-    text = f.read_text()
-    m = re.search(r"traffic_bytes_per_step=(\d+)", text)
-    mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
-    if m and mt and tuple(map(int, mt.groups())) == (tiling["mode"], tiling["chunk_records"],
-                                                        tiling["block_threads"], tiling.get("group_partitions", 1)):
-        return int(m.group(1)), f"profiles/{f.name}"
-    return None, f"profiles/{f.name} (measured with another tiling)"
+    want = (tiling["mode"], tiling["chunk_records"], tiling["block_threads"], tiling.get("group_partitions", 1))
+    for f in files:  # one summary per measured tiling
+        text = f.read_text()
+        m = re.search(r"traffic_bytes_per_step=(\d+)", text)
+        mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
+        if m and mt and tuple(map(int, mt.groups())) == want:
+            return int(m.group(1)), f"profiles/{f.name}"
+    return None, f"profiles/{files[0].name} (measured with other tilings)"
 
 
 def measured_lds(tiling, nref, nqry):
@@ -120,16 +121,17 @@ def measured_lds(tiling, nref, nqry):
     of this workload when measured with this run's tiling: (dict or None,
     summary file name)."""
     import re
-    f = ROOT / "profiles" / f"pjoin_split_c2_{nref}x{nqry}_lds.md"
-    if not f.exists():
+    files = sorted((ROOT / "profiles").glob(f"pjoin_split_c2_{nref}x{nqry}_lds*.md"))
+    if not files:
         return None, None
-    text = f.read_text()
-    mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
-    if not mt or tuple(map(int, mt.groups())) != (tiling["mode"], tiling["chunk_records"], tiling["block_threads"],
-                                                  tiling.get("group_partitions", 1)):
-        return None, f"profiles/{f.name} (measured with another tiling)"
-    vals = {k: float(v) for k, v in re.findall(r"^(lds_\w+)=([0-9.eE+-]+)$", text, re.M)}
-    return vals, f"profiles/{f.name}"
+    want = (tiling["mode"], tiling["chunk_records"], tiling["block_threads"], tiling.get("group_partitions", 1))
+    for f in files:  # one summary per measured tiling
+        text = f.read_text()
+        mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
+        if mt and tuple(map(int, mt.groups())) == want:
+            vals = {k: float(v) for k, v in re.findall(r"^(lds_\w+)=([0-9.eE+-]+)$", text, re.M)}
+            return vals, f"profiles/{f.name}"
+    return None, f"profiles/{files[0].name} (measured with other tilings)"
 
 
 def matched_events(records, qkeys, obj_bits):
