@@ -83,6 +83,8 @@ typedef struct kssdhip_index kssdhip_index;
 #define KSSDHIP_EHIP (-3)
 #define KSSDHIP_ENODEV (-4)
 #define KSSDHIP_EAGAIN (-5) /* serving: queue full / not done yet */
+#define KSSDHIP_ENOSPC (-6) /* an output did not fit its buffer; the call
+                               reports the size it needs */
 
 /* Message for the last failure on this thread ("" if none). */
 const char *kssdhip_last_error(void);
@@ -128,12 +130,21 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  * Kernel shape of the counting path for a handle.  No reference counterpart
  * (the reference's block loop has no device tiling); the project's
  * autotuner sets it.
- *   mode 0 auto: partitioned merge-join when the block carries at least
- *          auto_min_keys query keys per index partition, lookup otherwise;
- *        1 lookup: one thread per query key, bucket-table lookup into HBM;
+ *   mode 0 auto: per call, from the block's shape: the lookup kernel for
+ *          small blocks (fewer than auto_min_keys keys per index
+ *          partition), the partitioned merge-join when queries carry
+ *          many keys per index partition (dense blocks), the query-major
+ *          join otherwise;
+ *        1 lookup: one thread per query key, bucket-table lookup into HBM,
+ *          counters updated with global atomics;
  *        2 partitioned: one workgroup per index partition of about
  *          chunk_records records staged in LDS; queries are split at the
- *          partition boundaries and merged against the staged records.
+ *          partition boundaries and merged against the staged records;
+ *        3 query-major: persistent workgroups take whole queries from a
+ *          device work queue, look each context up through the index's
+ *          fingerprinted bucket table and sum the query's counters in an
+ *          LDS table, then write the query's row once (no counter atomics
+ *          in HBM).
  * chunk_records (partitions are rebuilt when it changes) must leave the
  * staged partition within the 160 KB LDS: 64 <= chunk_records <= 12288.
  * group_partitions (1..64): consecutive partitions one workgroup walks;
@@ -141,6 +152,11 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  * the first genome flushed for it over all the group's partitions and adds
  * it to the outputs once (hits on other genomes go to the outputs
  * directly), which needs chunk_records <= 6144.
+ * query_threads (mode 3; 0 = default 512): threads per query-major
+ * workgroup, 64..1024, multiple of 64.  query_slots (mode 3; 0 = default):
+ * LDS counter slots per query when ref_gnum exceeds what a direct
+ * per-genome LDS row holds, a power of two in 256..8192; a query that
+ * touches more genomes spills the rest to a per-workgroup row in HBM.
  */
 typedef struct {
 	int mode;
@@ -148,6 +164,8 @@ typedef struct {
 	int block_threads; /* 64..1024, multiple of 64 */
 	double auto_min_keys;
 	int group_partitions;
+	int query_threads;
+	int query_slots;
 } kssdhip_tiling_t;
 int kssdhip_index_set_tiling(kssdhip_index *index, const kssdhip_tiling_t *tiling);
 int kssdhip_index_get_tiling(const kssdhip_index *index, kssdhip_tiling_t *tiling);
@@ -280,6 +298,74 @@ typedef struct {
 } kssdhip_hit_t;
 int kssdhip_select_hits_dev(const double *d_values, int nq, int nr, double min_value, uint32_t cap,
 			    kssdhip_hit_t *d_hits, uint32_t *d_count, void *stream);
+
+/*
+ * Count a block of queries against the index AND apply the index path's
+ * pass gate on the device, returning only the (query, reference) pairs
+ * that pass it.  Replaces count_ctx_obj_frm_comb_sketch_section_with_unique
+ * with fmt == 0 (command_ani.h:463-621): the counting and the per-pair gate
+ * of its ranking loop (command_ani.h:585-606):
+ *   shared contexts >= ctxcut;
+ *   qry_ctx[q] > 0, ref_ctx[r] > 0 and max(shared/qry_ctx, shared/ref_ctx)
+ *     >= afcut (ani_report_af_pass, command_ani.h:70-78);
+ *   rank_ani >= anicut, where rank_ani is the float-rounded crude ANI
+ *     (float)(1 - (float)diff_obj / num_ctx / 7) or, with unique_rank, the
+ *     unique-context naive ANI (command_ani.h:597-604).
+ * The ordering and top-N cut of the passing pairs (command_ani.h:607-621)
+ * stay with the caller, who gets every passing pair with its counts.
+ * ctxcut must be >= 1 (with 0 every pair passes the first gate; use
+ * kssdhip_count_block).  With unique_rank the device keeps pairs whose
+ * unique-context ANI is within 1e-9 below anicut as well, so a caller that
+ * re-evaluates the rule on the host in FP64 sees every pair it would keep.
+ * value: the FP64 value of each kept pair as kssdhip_epilogue_dev computes
+ * it for a cell (its lower/qry_gid_offset fields are ignored).
+ * flags: KSSDHIP_UNIQUE (needed for unique_rank), KSSDHIP_IGNORECONFLICT.
+ */
+typedef struct {
+	uint32_t ctxcut;
+	float afcut;  /* float, as ani_opt_t holds it (command_ani.h:38) */
+	float anicut; /* command_ani.h:40 */
+	int unique_rank;
+	kssdhip_epilogue_opts_t value;
+} kssdhip_select_opts_t;
+
+typedef struct {
+	uint32_t query; /* block-relative query index */
+	uint32_t ref;   /* reference gid */
+	kssd_ctx_mut2_t ctx;
+	kssd_obj_section_t obj;
+	kssd_ctx_mut2_t unique_ctx;    /* zero without KSSDHIP_UNIQUE */
+	kssd_obj_section_t unique_obj;
+	double rank_ani; /* the gate's ranking value (see above) */
+	double value;    /* opts->value metric of the pair */
+} kssdhip_ani_row_t;
+KSSDHIP_STATIC_ASSERT(sizeof(kssdhip_ani_row_t) == 56, "kssdhip_ani_row_t is 56 bytes");
+
+/*
+ * Host-pointer form: section_sk / section_skidx as kssdhip_count_block
+ * (absolute offsets), qry_ctx: section_gnum context totals of the block's
+ * queries, ref_ctx: ref_gnum totals of the references.  rows receives the
+ * passing pairs ordered by (query, ref); *n_rows is set to their number.
+ * When it exceeds cap nothing beyond cap is stored and KSSDHIP_ENOSPC is
+ * returned, so the caller can grow rows and call again.
+ */
+int kssdhip_select_block(kssdhip_index *index, const uint64_t *section_sk, const uint64_t *section_skidx,
+			 int section_gnum, uint32_t flags, const uint32_t *qry_ctx, const uint32_t *ref_ctx,
+			 const kssdhip_select_opts_t *opts, kssdhip_ani_row_t *rows, size_t cap, size_t *n_rows);
+
+/*
+ * Device-resident form: relative offsets as kssdhip_count_block_dev.  The
+ * passing pairs of query q are d_rows[d_q_first[q] .. + d_q_count[q]) in no
+ * particular order; *d_row_count receives the total, which may exceed cap
+ * (rows past cap are not stored).  The call zeroes its counters with
+ * kernels and, once the handle's workspace has grown to the block's size,
+ * allocates nothing, so it can be captured in a HIP graph (capturing a call
+ * that would grow the workspace fails with KSSDHIP_EINVAL).
+ */
+int kssdhip_select_block_dev(kssdhip_index *index, const uint64_t *d_keys, const uint64_t *d_skidx, int section_gnum,
+			     uint64_t total_keys, uint32_t flags, const uint32_t *d_qry_ctx, const uint32_t *d_ref_ctx,
+			     const kssdhip_select_opts_t *opts, kssdhip_ani_row_t *d_rows, uint32_t cap,
+			     uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, void *stream);
 
 /*
  * Serving loop (no reference counterpart: the reference answers one request

@@ -441,6 +441,19 @@ int DevBuf::reserve(size_t want)
 	return 0;
 }
 
+int kssd_reserve_for_stream(DevBuf *b, size_t want, hipStream_t s, const char *what)
+{
+	if (want <= b->bytes)
+		return 0;
+	hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+	if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+		return kssdhip_fail(KSSDHIP_EINVAL,
+				    "%s: %zu bytes needed while the stream is being captured (the handle's workspace "
+				    "holds %zu); run the call once outside the capture first",
+				    what, want, b->bytes);
+	return b->reserve(want);
+}
+
 void DevBuf::release()
 {
 	if (ptr)
@@ -573,6 +586,9 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 		if ((rc = kssd_build_partitions(ix, 4096)))
 			break;
 		open_phase(nullptr, "partition table");
+		if ((rc = kssd_qjoin_build(ix)))
+			break;
+		open_phase(nullptr, "query-major tables");
 	} while (0);
 	if (d_raw)
 		(void)hipFree(d_raw);
@@ -598,7 +614,9 @@ extern "C" int kssdhip_index_close(kssdhip_index *ix)
 	if (ix->d_bucket)
 		(void)hipFree(ix->d_bucket);
 	kssd_free_partitions(ix);
-	DevBuf *bufs[] = {&ix->keys, &ix->skidx, &ix->ctx, &ix->obj, &ix->uctx, &ix->uobj, &ix->idx, &ix->split};
+	kssd_qjoin_free(ix);
+	DevBuf *bufs[] = {&ix->keys,  &ix->skidx, &ix->ctx,  &ix->obj,  &ix->uctx,  &ix->uobj,  &ix->idx,
+			  &ix->split, &ix->qws,   &ix->rows, &ix->qmeta, &ix->qctxb, &ix->rctxb};
 	for (DevBuf *b : bufs)
 		b->release();
 	if (ix->stream)
@@ -614,6 +632,8 @@ extern "C" size_t kssdhip_index_device_bytes(const kssdhip_index *ix)
 	size_t b = ((size_t)ix->n + INDEX_PAD) * 12 + ((((size_t)1) << ix->k) + 1) * 4;
 	b += ix->keys.bytes + ix->skidx.bytes + ix->ctx.bytes + ix->obj.bytes + ix->uctx.bytes +
 	     ix->uobj.bytes + ix->idx.bytes + ix->split.bytes + ((size_t)ix->P + 1) * 12;
+	b += ((size_t)ix->n + 1) * 16 + ((((size_t)1) << ix->qk) * 2 + 1) * 4 + ix->qws.bytes + ix->rows.bytes +
+	     ix->qmeta.bytes + ix->qctxb.bytes + ix->rctxb.bytes;
 	return b;
 }
 
@@ -637,7 +657,7 @@ int kssdhip_index_info(const kssdhip_index *ix, IndexView *v)
 // ---------------------------------------------------------------- counting
 int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
 		   int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
-		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split,
+		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split, DevBuf *qws,
 		   uint32_t vote_max_freq, uint32_t vote_step)
 {
 	if (!ix)
@@ -650,7 +670,13 @@ int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_
 	if (flags & ~(KSSDHIP_LOWER | KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT | KSSDHIP_ACCUMULATE))
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: unknown flags 0x%x", flags);
 	const uint64_t cells = (uint64_t)nq * (uint64_t)ix->ref_gnum;
-	if (cells && !(flags & KSSDHIP_ACCUMULATE)) {
+	// the query-major kernel writes every cell of its rows itself (or adds
+	// into them under ACCUMULATE), so it needs no zeroed outputs
+	const bool work = total && nq && ix->n;
+	const bool voting = vote_max_freq || vote_step > 1;
+	const bool qjoin = work && qws && !voting && ix->d_qrec &&
+			   (ix->mode == 3 || (ix->mode == 0 && kssd_qjoin_auto(ix, nq, total)));
+	if (cells && !(flags & KSSDHIP_ACCUMULATE) && !qjoin) {
 		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
 				   (unsigned long long *)d_ctx, cells, 0ull);
 		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,
@@ -662,7 +688,7 @@ int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_
 					   (unsigned long long *)d_uobj, cells, 0ull);
 		}
 	}
-	if (total && nq && ix->n) {
+	if (work) {
 		JoinArgs a;
 		a.ctxgid = ix->d_ctxgid;
 		a.robj = ix->d_obj;
@@ -684,8 +710,15 @@ int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_
 		a.objc = (unsigned long long *)d_obj;
 		a.uctx = (unsigned long long *)d_uctx;
 		a.uobj = (unsigned long long *)d_uobj;
+		if (qjoin) {
+			int rc = kssd_launch_qjoin_dense(ix, a, s, qws);
+			if (rc)
+				return rc;
+			HIP_TRY(hipGetLastError());
+			return 0;
+		}
 		bool used = false;
-		int rc = kssd_launch_pjoin(ix, a, s, split, &used);
+		int rc = ix->mode == 3 ? 0 : kssd_launch_pjoin(ix, a, s, split, &used);
 		if (rc)
 			return rc;
 		if (!used)
@@ -704,7 +737,7 @@ extern "C" int kssdhip_count_block_dev(kssdhip_index *ix, const uint64_t *d_keys
 	if (!ix)
 		return kssdhip_fail(KSSDHIP_EINVAL, "count_block: null index");
 	return kssd_count_dev(ix, d_keys, d_skidx, nq, total, qry_gid_offset, flags, d_ctx, d_obj, d_uctx, d_uobj,
-			      (hipStream_t)stream, &ix->split);
+			      (hipStream_t)stream, &ix->split, &ix->qws);
 }
 
 extern "C" int kssdhip_count_block(kssdhip_index *ix, const uint64_t *section_sk,
@@ -792,7 +825,7 @@ extern "C" int kssdhip_count_votes(kssdhip_index *ix, const uint64_t *section_sk
 	HIP_TRY(hipMemcpyAsync(ix->skidx.ptr, rel.data(), rel.size() * 8, hipMemcpyHostToDevice, s));
 	rc = kssd_count_dev(ix, (const uint64_t *)ix->keys.ptr, (const uint64_t *)ix->skidx.ptr, nq, total,
 			    qry_gid_offset, KSSDHIP_LOWER, (kssd_ctx_mut2_t *)ix->ctx.ptr,
-			    (kssd_obj_section_t *)ix->obj.ptr, nullptr, nullptr, s, &ix->split, max_ctx_freq,
+			    (kssd_obj_section_t *)ix->obj.ptr, nullptr, nullptr, s, &ix->split, nullptr, max_ctx_freq,
 			    sample_step);
 	if (rc)
 		return rc;

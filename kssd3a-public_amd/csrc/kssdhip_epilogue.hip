@@ -4,6 +4,9 @@
 #include "kssd_models.h"
 #include "kssdhip.h"
 #include "kssdhip_internal.h"
+#include "kssdhip_epi.h"
+
+using namespace kssd;
 
 namespace {
 
@@ -17,23 +20,11 @@ struct EpiArgs {
 	double *out;
 };
 
-// selected_metric_effective_id without calibration (command_ani.c:1260-1270)
-__device__ __forceinline__ int effective_metric(const kssdhip_epilogue_opts_t &o)
-{
-	int s = o.metric;
-	if (s < 1 || s > 9)
-		s = 1;
-	if (o.unassembled && s <= 4)
-		return 4;
-	return s;
-}
-
 __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
 {
 	const uint64_t cells = (uint64_t)a.nq * (uint64_t)a.nr;
 	const double exc = a.o.values_ani ? 1.0 - a.o.exception : a.o.exception;
-	const int s = effective_metric(a.o);
-	const uint32_t k = a.o.ctx_bits / 2;
+	const int s = epi_effective_metric(a.o);
 	for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < cells;
 	     c += (uint64_t)gridDim.x * blockDim.x) {
 		const int q = (int)(c / (uint64_t)a.nr);
@@ -46,39 +37,7 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
 			continue;
 		}
 		const uint2 cm = a.ctx[c], os = a.obj[c];
-		const uint32_t xny = cm.x, mut2 = cm.y, dobj = os.x, dsec = os.y;
-		double afq = (double)xny / (double)qc, afr = (double)xny / (double)rc;
-		if (a.o.float_af) {
-			afq = (double)((float)xny / (float)qc);
-			afr = (double)((float)xny / (float)rc);
-		}
-		// fill_row_base_distances (command_ani.c:1321-1341)
-		const bool af_pass = (afq > afr ? afq : afr) >= a.o.afcut;
-		double dist;
-		switch (s) {
-		case 4:
-			dist = af_pass ? kssd_naive_dist(xny, dobj, dsec) : a.o.exception;
-			break;
-		case 5:
-			dist = kssd_mashD(k, rc, qc, xny);
-			break;
-		case 6:
-			dist = kssd_aafD(k, rc, qc, xny);
-			break;
-		case 7:
-			dist = af_pass ? kssd_moe_dist(xny, dobj, dsec, mut2, a.o.drfold) : kssd_mashD(k, rc, qc, xny);
-			break;
-		case 8:
-			dist = af_pass ? kssd_moe_dist(xny, dobj, dsec, mut2, a.o.drfold) : kssd_aafD(k, rc, qc, xny);
-			break;
-		case 9:
-			dist = af_pass ? kssd_p_dist(xny, dsec, a.o.obj_bits) : a.o.exception;
-			break;
-		default: // 3 (and 1/2 where no calibration is available: CtxMoE)
-			dist = af_pass ? kssd_moe_dist(xny, dobj, dsec, mut2, a.o.drfold) : a.o.exception;
-			break;
-		}
-		a.out[c] = a.o.values_ani ? 1.0 - dist : dist;
+		a.out[c] = epi_cell_value(a.o, s, cm.x, cm.y, os.x, os.y, qc, rc);
 	}
 }
 
@@ -147,11 +106,9 @@ extern "C" int kssdhip_epilogue_dev(const kssd_ctx_mut2_t *d_ctx, const kssd_obj
 {
 	if (!opts || nq < 0 || nr < 0)
 		return kssdhip_fail(KSSDHIP_EINVAL, "epilogue: bad arguments");
-	if (opts->metric == 1 || opts->metric == 2 || opts->metric < 1 || opts->metric > 9) {
-		if (!(opts->unassembled && opts->metric >= 1 && opts->metric <= 4))
-			return kssdhip_fail(KSSDHIP_EINVAL,
-					    "epilogue: metric %d needs the calibration model (use 3..9)", opts->metric);
-	}
+	if (!epi_metric_supported(*opts))
+		return kssdhip_fail(KSSDHIP_EINVAL, "epilogue: metric %d needs the calibration model (use 3..9)",
+				    opts->metric);
 	const uint64_t cells = (uint64_t)nq * (uint64_t)nr;
 	if (!cells)
 		return 0;

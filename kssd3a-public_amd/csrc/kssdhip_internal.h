@@ -112,8 +112,21 @@ struct kssdhip_index {
 	int block = 256;
 	double auto_min_keys = 0;
 	int group = 1;
-	// grow-only scratch: host-pointer entry points and the split table
-	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx, split;
+	int qthreads = 0; // query-major workgroup size (0: default)
+	int qslots = 0;   // query-major LDS slots (0: default)
+	// query-major join (kssdhip_qjoin.hip): the records again as 16-byte
+	// {ctxgid, obj, 0} entries, the first record of each of 2^qk context
+	// buckets (bucket t = ctx >> qshift; qstart[2^qk] = n) and a 32-bit
+	// fingerprint mask of the contexts in each bucket
+	uint4 *d_qrec = nullptr;
+	uint32_t *d_qstart = nullptr;
+	uint32_t *d_qfp = nullptr;
+	int qk = 0, qshift = 0;
+	int cus = 0; // compute units of the device
+	int qj_per_cu[2] = {0, 0}; // query-major workgroups per CU (plain, unique counters) at the tiling
+	// grow-only scratch: host-pointer entry points, the split table, the
+	// query-major work counter and spill rows, select outputs
+	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx, split, qws, rows, qmeta, qctxb, rctxb;
 };
 
 // partitioned merge-join (kssdhip_pjoin.hip)
@@ -122,11 +135,28 @@ void kssd_free_partitions(kssdhip_index *ix);
 // true when the partitioned path was launched for this call; `split` holds
 // the call's split table (callers on different streams pass their own)
 int kssd_launch_pjoin(kssdhip_index *ix, const kssd::JoinArgs &a, hipStream_t s, DevBuf *split, bool *used);
-// kssdhip_count_block_dev with an explicit split-table workspace
+// kssdhip_count_block_dev with explicit workspaces (split table; query-major
+// work counter and spill rows, nullptr: that kernel is not used)
 int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
 		   int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
-		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split,
+		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split, DevBuf *qws,
 		   uint32_t vote_max_freq = 0, uint32_t vote_step = 1);
+
+// query-major join (kssdhip_qjoin.hip)
+int kssd_qjoin_build(kssdhip_index *ix);
+// occupancy of the query-major kernels at the handle's tiling (index open,
+// set_tiling), so that launches make no runtime queries
+int kssd_qjoin_prepare(kssdhip_index *ix);
+void kssd_qjoin_free(kssdhip_index *ix);
+// dense rows of a block by the query-major kernel; `ws` is the call's
+// workspace (work counter and spill rows)
+int kssd_launch_qjoin_dense(kssdhip_index *ix, const kssd::JoinArgs &a, hipStream_t s, DevBuf *ws);
+// true when a block of this shape should take the query-major kernel in
+// auto mode
+bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total);
+// fails with KSSDHIP_EINVAL when `want` bytes would grow `b` while `s` is
+// being captured into a graph, else reserves them
+int kssd_reserve_for_stream(DevBuf *b, size_t want, hipStream_t s, const char *what);
 
 // read-only view of an index handle for other translation units
 struct IndexView {

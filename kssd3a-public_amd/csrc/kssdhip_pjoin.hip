@@ -803,7 +803,7 @@ int kssd_launch_pjoin(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevB
 	const uint64_t sbytes = ((uint64_t)ix->P + 1) * (uint64_t)ja.nq * 4;
 	if (ix->block > 1024)
 		return 0;
-	int rc = split->reserve(sbytes);
+	int rc = kssd_reserve_for_stream(split, sbytes, s, "split table");
 	if (rc)
 		return rc;
 	uint32_t *S = (uint32_t *)split->ptr;
@@ -891,8 +891,13 @@ extern "C" int kssdhip_index_set_tiling(kssdhip_index *ix, const kssdhip_tiling_
 {
 	if (!ix || !t)
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: null argument");
-	if (t->mode < 0 || t->mode > 2)
+	if (t->mode < 0 || t->mode > 3)
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: mode %d", t->mode);
+	if (t->query_threads && (t->query_threads < 64 || t->query_threads > 1024 || t->query_threads % 64))
+		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: query_threads %d", t->query_threads);
+	if (t->query_slots && (t->query_slots < 256 || t->query_slots > 8192 || (t->query_slots & (t->query_slots - 1))))
+		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: query_slots %d (a power of two in 256..8192)",
+				    t->query_slots);
 	if (t->block_threads < 64 || t->block_threads > 1024 || t->block_threads % 64)
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: block_threads %d", t->block_threads);
 	if (t->auto_min_keys < 0)
@@ -913,7 +918,9 @@ extern "C" int kssdhip_index_set_tiling(kssdhip_index *ix, const kssdhip_tiling_
 	ix->block = t->block_threads;
 	ix->auto_min_keys = t->auto_min_keys;
 	ix->group = t->group_partitions;
-	return 0;
+	ix->qthreads = t->query_threads;
+	ix->qslots = t->query_slots;
+	return kssd_qjoin_prepare(ix);
 }
 
 extern "C" int kssdhip_index_get_tiling(const kssdhip_index *ix, kssdhip_tiling_t *t)
@@ -925,5 +932,7 @@ extern "C" int kssdhip_index_get_tiling(const kssdhip_index *ix, kssdhip_tiling_
 	t->block_threads = ix->block;
 	t->auto_min_keys = ix->auto_min_keys;
 	t->group_partitions = ix->group;
+	t->query_threads = ix->qthreads;
+	t->query_slots = ix->qslots;
 	return 0;
 }

@@ -1,0 +1,1063 @@
+// Query-major join: the counting path for blocks whose queries carry few
+// keys per index partition (indexes much larger than one query), and the
+// device-side pass gate of the index path.
+//
+// The reference counts one query per OpenMP iteration: every context group
+// of the query finds its first index record through the fenceposts, walks
+// the context's per-genome runs and adds into the query's own row of the
+// dense counters (command_ani.h:481-575).  Here a persistent workgroup takes
+// whole queries from a device work queue.  Its threads stride over the
+// query's keys; the first key of each context group looks the context up in
+// a bucket table whose entries carry a 32-bit fingerprint mask of the
+// contexts in the bucket (most query contexts are absent from the index and
+// stop there), scans the bucket's 16-byte {ctxgid, obj} records, walks the
+// runs and adds each pair's counters into an LDS table of the query (one
+// cell per genome when the reference set is small enough, else an
+// open-addressed table keyed by genome; a genome that finds no slot goes to
+// the workgroup's spill row in HBM).  When the query is done the workgroup
+// writes its dense row once, or applies the pass gate of the ranking loop
+// (command_ani.h:585-606) to the touched pairs and appends the passing ones
+// to a compact list.  No counter is updated with a global atomic, and no
+// dense block has to be zeroed first.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "kssdhip.h"
+#include "kssdhip_epi.h"
+#include "kssdhip_internal.h"
+
+using namespace kssd;
+
+#define LDS __attribute__((address_space(3)))
+
+namespace {
+
+constexpr uint32_t EMPTY = 0xffffffffu;
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr int MAX_PROBES = 32;
+constexpr uint32_t DIRECT_LDS_MAX = 64u * 1024u; // direct per-genome rows up to this many LDS bytes
+constexpr int DEFAULT_THREADS = 512;
+constexpr uint32_t DEFAULT_SLOTS = 2048;
+// workspace: [0, WS_HEAD) work counter and the order's histogram and
+// cursors (65 words), then the work order (nq words, WS_ALIGN-rounded),
+// then the spill rows
+constexpr size_t WS_HEAD = 512;
+constexpr uint32_t WS_HEAD_WORDS = 65;
+constexpr size_t WS_ALIGN = 256;
+#ifndef QJ_KB
+#define QJ_KB 2
+#endif
+constexpr int KB = QJ_KB; // keys per thread whose lookups are in flight together (1, 2 or 4)
+
+enum { QJ_DENSE = 0, QJ_ACCUM = 1, QJ_SELECT = 2 };
+
+// One bit of a bucket's fingerprint mask per context: the top 5 bits of a
+// multiplicative hash of the whole context (the bucket index uses only the
+// context's top bits).
+__host__ __device__ __forceinline__ uint32_t fp_bit(uint64_t c)
+{
+	return 1u << (uint32_t)((c * 0x9E3779B97F4A7C15ull) >> 59);
+}
+
+// 16-byte records {ctxgid low, ctxgid high, obj, 0}
+__global__ void k_pack16(const uint64_t *__restrict__ ctxgid, const uint32_t *__restrict__ obj, uint64_t n,
+			 uint4 *rec)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+		const uint64_t c = ctxgid[i];
+		rec[i] = make_uint4((uint32_t)c, (uint32_t)(c >> 32), obj[i], 0u);
+	}
+}
+
+// start[t] = first record whose context >> shift >= t (t = 0 .. nb; nb = 2^k,
+// start[nb] = n)
+__global__ void k_qbkt_start(const uint64_t *__restrict__ ctxgid, uint64_t n, int shift, uint64_t nb, uint32_t *start)
+{
+	for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t <= nb; t += (uint64_t)gridDim.x * blockDim.x) {
+		uint64_t lo = 0, hi = n;
+		if (t == nb) {
+			lo = n;
+		} else {
+			while (lo < hi) {
+				const uint64_t mid = lo + ((hi - lo) >> 1);
+				if ((ctxgid[mid] >> (GID_NBITS + shift)) < t)
+					lo = mid + 1;
+				else
+					hi = mid;
+			}
+		}
+		start[t] = (uint32_t)lo;
+	}
+}
+
+// fp[t] = OR of fp_bit over the contexts of bucket t's records
+__global__ void k_qbkt_fp(const uint64_t *__restrict__ ctxgid, uint64_t nb, const uint32_t *__restrict__ start,
+			  uint32_t *fp)
+{
+	for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < nb; t += (uint64_t)gridDim.x * blockDim.x) {
+		const uint32_t b = start[t], e = start[t + 1];
+		uint32_t m = 0;
+		for (uint32_t d = b; d < e; ++d)
+			m |= fp_bit(ctxgid[d] >> GID_NBITS);
+		fp[t] = m;
+	}
+}
+
+// Work order of a block: queries grouped by the bit length of their key
+// count, longest first (so the largest queries do not end the launch
+// alone).  head[1 .. 32] histogram, head[33 .. 64] cursors (zeroed before).
+__device__ __forceinline__ uint32_t size_class(const uint64_t *skidx, uint32_t q)
+{
+	const uint64_t k = skidx[q + 1] - skidx[q];
+	return k ? 63u - (uint32_t)__clzll((long long)k) : 0u; // 0 .. 31 for k < 2^32
+}
+
+__global__ void k_order_hist(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t *head)
+{
+	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x)
+		atomicAdd(&head[1 + (size_class(skidx, q) & 31u)], 1u);
+}
+
+__global__ void k_order_scan(uint32_t *head)
+{
+	if (blockIdx.x || threadIdx.x)
+		return;
+	uint32_t at = 0;
+	for (int b = 31; b >= 0; --b) {
+		head[33 + b] = at;
+		at += head[1 + b];
+	}
+}
+
+__global__ void k_order_scatter(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t *head, uint32_t *order)
+{
+	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x)
+		order[atomicAdd(&head[33 + (size_class(skidx, q) & 31u)], 1u)] = q;
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n)
+{
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+		p[i] = 0;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint64_t n)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		p[i] = 0;
+}
+
+struct QJArgs {
+	// index
+	const uint4 *rec;
+	const uint32_t *qstart; // 2^k + 1 bucket starts
+	const uint32_t *qfp;    // 2^k fingerprint masks
+	uint64_t n;
+	int qshift;
+	uint64_t ctx_max;
+	uint32_t R; // ref_gnum
+	unsigned obj_bits;
+	// block
+	const uint64_t *keys;
+	const uint64_t *skidx; // relative, nq + 1
+	uint32_t nq;
+	int qry_gid_offset;
+	uint32_t flags;
+	// LDS table: slots (a power of two, sbits = log2) or direct rows
+	uint32_t slots;
+	int sbits;
+	// work queue and spill rows (zero at rest: a workgroup zeroes what it spilled)
+	uint32_t *next;
+	const uint32_t *order; // queries in processing order
+	unsigned long long *spill; // gridDim.x rows of R cells x (2 | 4) u64
+	int mode;
+	// dense outputs (QJ_DENSE, QJ_ACCUM)
+	unsigned long long *ctx, *objc, *uctx, *uobj;
+	// select (QJ_SELECT)
+	const uint32_t *qry_ctx, *ref_ctx;
+	uint32_t ctxcut;
+	double afcut;
+	float anicut;
+	int unique_rank;
+	kssdhip_epilogue_opts_t vo;
+	int vmetric;
+	kssdhip_ani_row_t *rows;
+	uint32_t cap;
+	uint32_t *row_count;
+	uint32_t *q_first, *q_count;
+};
+
+// The query's counter table in LDS: c = {num_ctx, num_mut2_ctx}, o =
+// {diff_obj, diff_obj_section} as one u64 each (low word first, as the dense
+// cells), uc/uo the unique-owner copies; gid the key of each slot (hash
+// tables only).
+struct QTable {
+	LDS unsigned long long *c, *o, *uc, *uo;
+	LDS uint32_t *gid;
+	uint32_t mask;
+	int shift;
+};
+
+__device__ __forceinline__ uint32_t slot_find(const QTable &t, uint32_t gid, bool insert)
+{
+	uint32_t s = (gid * 2654435761u) >> t.shift;
+	for (int p = 0; p < MAX_PROBES; ++p) {
+		const uint32_t g = t.gid[s];
+		if (g == gid)
+			return s;
+		if (g == EMPTY) {
+			if (!insert)
+				return NONE;
+			uint32_t expected = EMPTY;
+			__hip_atomic_compare_exchange_strong(&t.gid[s], &expected, gid, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+							     __HIP_MEMORY_SCOPE_WORKGROUP);
+			if (expected == EMPTY || expected == gid)
+				return s;
+		}
+		s = (s + 1) & t.mask;
+	}
+	return NONE;
+}
+
+template <bool UNIQ, bool DIRECT>
+__device__ __forceinline__ void qj_add(const QTable &t, uint32_t gid, int m, bool ucopy,
+				       unsigned long long *spill_row, LDS uint32_t *spilled)
+{
+	const unsigned long long dc = 1ull | ((unsigned long long)(m > 1) << 32);
+	const unsigned long long dob = 1ull | ((unsigned long long)m << 32);
+	const uint32_t s = DIRECT ? gid : slot_find(t, gid, true);
+	if (s != NONE) {
+		LDS unsigned long long *c = ucopy ? t.uc : t.c;
+		LDS unsigned long long *o = ucopy ? t.uo : t.o;
+		__hip_atomic_fetch_add(&c[s], dc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		if (m)
+			__hip_atomic_fetch_add(&o[s], dob, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		return;
+	}
+	// no slot left for this genome: the workgroup's spill row (read back
+	// and zeroed when the query's output is written)
+	*spilled = 1u;
+	unsigned long long *cell = spill_row + (uint64_t)gid * (UNIQ ? 4u : 2u) + (ucopy ? 2u : 0u);
+	atomicAdd(cell, dc);
+	if (m)
+		atomicAdd(cell + 1, dob);
+}
+
+__device__ __forceinline__ uint64_t rec_ctxgid(const uint4 &r)
+{
+	return ((uint64_t)r.y << 32) | r.x;
+}
+
+// Four consecutive records held in registers, loaded together (one 64-byte
+// span), so that a bucket scan or a run walk waits for memory about once
+// per four records.  Only records below n are ever returned.  Accesses move
+// forward only; a window that starts empty has base = (first index) - 4, so
+// that i - base >= 4 (mod 2^32) reloads it on the first access.
+struct RecWin {
+	uint32_t base;
+	uint4 r[4];
+};
+
+#ifndef QJ_WIN
+#define QJ_WIN 1
+#endif
+__device__ __forceinline__ uint4 rw_get(RecWin &w, const QJArgs &a, uint32_t i)
+{
+	if (QJ_WIN == 1)
+		return a.rec[i];
+	if (i - w.base >= 4u) {
+		w.base = i;
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			w.r[k] = (uint64_t)i + k < a.n ? a.rec[i + k] : make_uint4(0, 0, 0, 0);
+	}
+	const uint32_t k = i - w.base;
+	return k == 0 ? w.r[0] : k == 1 ? w.r[1] : k == 2 ? w.r[2] : w.r[3];
+}
+
+// One context group of the query (keys [i, s_end) share context c, whose
+// bucket holds records [st, en)): find the context's first record and walk
+// its per-genome runs (command_ani.h:521-575; LOWER: :668-716).
+template <bool UNIQ, bool DIRECT>
+__device__ __forceinline__ void qj_group(const QJArgs &a, const QTable &t, uint64_t i, uint64_t key, uint64_t c,
+					 uint32_t st, uint32_t en, uint64_t ke, int qgid, unsigned long long *spill_row,
+					 LDS uint32_t *spilled)
+{
+	const unsigned ob = a.obj_bits;
+	const uint64_t objmask = ob >= 64 ? ~0ull : ((1ull << ob) - 1ull);
+	const bool lower = (a.flags & KSSDHIP_LOWER) != 0;
+	const bool ignore = (a.flags & KSSDHIP_IGNORECONFLICT) != 0;
+	const uint64_t target = c << GID_NBITS;
+	RecWin w;
+	w.base = st - 4u;
+	uint32_t d = st;
+	bool found = false;
+	uint4 r = make_uint4(0, 0, 0, 0);
+	for (; d < en; ++d) {
+		r = rw_get(w, a, d);
+		const uint64_t cg = rec_ctxgid(r);
+		if (cg >= target) {
+			found = (cg >> GID_NBITS) == c;
+			break;
+		}
+	}
+	if (!found)
+		return;
+	uint64_t s_end = i + 1;
+	while (s_end < ke && (a.keys[s_end] >> ob) == c)
+		++s_end;
+	int owners = 0, sole_diff = 0;
+	uint32_t sole_gid = 0;
+	bool sole_allowed = false;
+	for (;;) { // r = record d, the first of a genome run of context c
+		const uint64_t cg = rec_ctxgid(r);
+		const uint32_t gid = (uint32_t)(cg & GID_MASK);
+		const uint32_t rb = d;
+		uint4 nx = r;
+		bool more;
+		do {
+			++d;
+			more = d < a.n;
+			if (more)
+				nx = rw_get(w, a, d);
+		} while (more && rec_ctxgid(nx) == cg);
+		const uint32_t re = d;
+		more = more && (rec_ctxgid(nx) >> GID_NBITS) == c;
+		bool count = true;
+		if (lower && (int)gid >= qgid) {
+			if ((int)gid > qgid)
+				break;
+			count = false;
+		}
+		if (count) {
+			if (UNIQ) {
+				++owners;
+				sole_gid = gid;
+			}
+			if (!(ignore && re - rb > 1)) {
+				int m = MIN_INIT;
+				for (uint64_t x = i; x < s_end && m; ++x) {
+					const uint32_t oq = (uint32_t)((x == i ? key : a.keys[x]) & objmask);
+					for (uint32_t y = rb; y < re; ++y) {
+						const uint32_t diff = oq ^ (y == rb ? r.z : a.rec[y].z);
+						if (diff == 0) {
+							m = 0;
+							break;
+						}
+						const int pc = dna_popcount_dev(diff);
+						m = pc < m ? pc : m;
+					}
+				}
+				qj_add<UNIQ, DIRECT>(t, gid, m, false, spill_row, spilled);
+				if (UNIQ) {
+					sole_diff = m;
+					sole_allowed = true;
+				}
+			}
+		}
+		if (!more)
+			break;
+		r = nx;
+	}
+	if (UNIQ && owners == 1 && sole_allowed)
+		qj_add<UNIQ, DIRECT>(t, sole_gid, sole_diff, true, spill_row, spilled);
+}
+
+// Counts of query q into the table: the first key of each context group
+// does the work for the group.  Each thread takes KB of the query's keys at
+// a time (strided by the workgroup size, so loads of the keys coalesce) and
+// issues their fingerprint and bucket loads together before any scan.
+template <bool UNIQ, bool DIRECT>
+__device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned long long *spill_row,
+			 LDS uint32_t *spilled)
+{
+	const uint64_t kb = a.skidx[q], ke = a.skidx[q + 1];
+	const unsigned ob = a.obj_bits;
+	const int qgid = a.qry_gid_offset + (int)q;
+	const uint64_t B = blockDim.x;
+	for (uint64_t base = kb + threadIdx.x; base < ke; base += KB * B) {
+		uint64_t key[KB], c[KB];
+		uint32_t st[KB], en[KB], fp[KB];
+		bool on[KB];
+#pragma unroll
+		for (int j = 0; j < KB; ++j) {
+			const uint64_t i = base + j * B;
+			on[j] = i < ke;
+			key[j] = on[j] ? a.keys[i] : 0ull;
+		}
+#pragma unroll
+		for (int j = 0; j < KB; ++j) {
+			const uint64_t i = base + j * B;
+			c[j] = key[j] >> ob;
+			const uint64_t prev = on[j] && i > kb ? a.keys[i - 1] : ~0ull;
+			on[j] = on[j] && c[j] <= a.ctx_max && (i == kb || (prev >> ob) != c[j]);
+		}
+#pragma unroll
+		for (int j = 0; j < KB; ++j)
+			fp[j] = on[j] ? a.qfp[c[j] >> a.qshift] : 0u;
+#pragma unroll
+		for (int j = 0; j < KB; ++j) {
+			on[j] = on[j] && (fp[j] & fp_bit(c[j])) != 0u; // else no record of the bucket has context c
+			const uint32_t tb = (uint32_t)(c[j] >> a.qshift);
+			st[j] = on[j] ? a.qstart[tb] : 0u;
+			en[j] = on[j] ? a.qstart[tb + 1] : 0u;
+		}
+		// one instance of the group walk, fed from the batch by constant
+		// indexes (no private-array indexing)
+#pragma unroll 1
+		for (int j = 0; j < KB; ++j) {
+			bool o = on[0];
+			uint64_t kj = key[0], cj = c[0];
+			uint32_t sj = st[0], ej = en[0];
+#pragma unroll
+			for (int x = 1; x < KB; ++x) {
+				if (j == x) {
+					o = on[x];
+					kj = key[x];
+					cj = c[x];
+					sj = st[x];
+					ej = en[x];
+				}
+			}
+			if (o)
+				qj_group<UNIQ, DIRECT>(a, t, base + j * B, kj, cj, sj, ej, ke, qgid, spill_row, spilled);
+		}
+	}
+}
+
+struct Cell {
+	unsigned long long c, o, uc, uo;
+};
+
+// Final counters of genome g (direct rows or table slot, plus the spill row
+// when the query spilled; zero_spill clears the spill cells it read).
+template <bool UNIQ, bool DIRECT>
+__device__ __forceinline__ Cell cell_of_gid(const QTable &t, uint32_t g, bool spilled, unsigned long long *spill_row,
+					    bool zero_spill)
+{
+	Cell v = {0, 0, 0, 0};
+	const uint32_t s = DIRECT ? g : slot_find(t, g, false);
+	if (s != NONE) {
+		v.c = t.c[s];
+		v.o = t.o[s];
+		if (UNIQ) {
+			v.uc = t.uc[s];
+			v.uo = t.uo[s];
+		}
+	}
+	if (!DIRECT && spilled) {
+		unsigned long long *sp = spill_row + (uint64_t)g * (UNIQ ? 4u : 2u);
+		v.c += sp[0];
+		v.o += sp[1];
+		if (UNIQ) {
+			v.uc += sp[2];
+			v.uo += sp[3];
+		}
+		if (zero_spill) {
+			sp[0] = 0;
+			sp[1] = 0;
+			if (UNIQ) {
+				sp[2] = 0;
+				sp[3] = 0;
+			}
+		}
+	}
+	return v;
+}
+
+// The pass gate of the index path's ranking loop (command_ani.h:585-606);
+// rank: the ranking value.  With unique_rank the FP64 naive ANI is compared
+// 1e-9 below anicut, so that the host's own evaluation keeps a subset.
+template <bool UNIQ>
+__device__ __forceinline__ bool qj_gate(const QJArgs &a, uint32_t q, uint32_t g, const Cell &v, double *rank)
+{
+	const uint32_t xny = (uint32_t)v.c, dobj = (uint32_t)v.o, dsec = (uint32_t)(v.o >> 32);
+	if (xny < a.ctxcut)
+		return false;
+	const uint32_t qc = a.qry_ctx[q], rc = a.ref_ctx[g];
+	if (qc == 0 || rc == 0)
+		return false;
+	const double afq = (double)xny / (double)qc, afr = (double)xny / (double)rc;
+	if (!((afq > afr ? afq : afr) >= a.afcut))
+		return false;
+	const float dist = (float)dobj / (float)xny / 7.0f;
+	double ani = (double)(float)(1.0f - dist);
+	if (UNIQ && a.unique_rank) {
+		const uint32_t uxny = (uint32_t)v.uc;
+		const bool u = (uint64_t)uxny * 20 >= rc; // ani_use_unique_context, command_ani.h:437-440
+		ani = 1.0 - (u ? kssd_naive_dist(uxny, (uint32_t)v.uo, (uint32_t)(v.uo >> 32))
+			       : kssd_naive_dist(xny, dobj, dsec));
+		*rank = ani;
+		return ani >= (double)a.anicut - 1e-9;
+	}
+	*rank = ani;
+	return ani >= (double)a.anicut;
+}
+
+template <bool UNIQ, bool DIRECT>
+__device__ void qj_output(const QJArgs &a, const QTable &t, uint32_t q, bool spilled, unsigned long long *spill_row,
+			  LDS uint32_t *s_n, LDS uint32_t *s_base)
+{
+	const uint32_t R = a.R;
+	const uint64_t row = (uint64_t)q * R;
+	if (a.mode == QJ_DENSE) {
+		for (uint32_t g = threadIdx.x; g < R; g += blockDim.x) {
+			const Cell v = cell_of_gid<UNIQ, DIRECT>(t, g, spilled, spill_row, true);
+			a.ctx[row + g] = v.c;
+			a.objc[row + g] = v.o;
+			if (UNIQ) {
+				a.uctx[row + g] = v.uc;
+				a.uobj[row + g] = v.uo;
+			}
+		}
+		return;
+	}
+	// candidates: every genome when the table is direct or the query
+	// spilled, else the occupied slots
+	const bool by_gid = DIRECT || spilled;
+	const uint32_t ncand = by_gid ? R : a.slots;
+	if (a.mode == QJ_ACCUM) {
+		for (uint32_t k = threadIdx.x; k < ncand; k += blockDim.x) {
+			uint32_t g = k;
+			Cell v;
+			if (by_gid) {
+				v = cell_of_gid<UNIQ, DIRECT>(t, g, spilled, spill_row, true);
+			} else {
+				g = t.gid[k];
+				if (g == EMPTY)
+					continue;
+				v = Cell{t.c[k], t.o[k], UNIQ ? t.uc[k] : 0ull, UNIQ ? t.uo[k] : 0ull};
+			}
+			if (v.c)
+				atomicAdd(&a.ctx[row + g], v.c);
+			if (v.o)
+				atomicAdd(&a.objc[row + g], v.o);
+			if (UNIQ) {
+				if (v.uc)
+					atomicAdd(&a.uctx[row + g], v.uc);
+				if (v.uo)
+					atomicAdd(&a.uobj[row + g], v.uo);
+			}
+		}
+		return;
+	}
+	// QJ_SELECT: count the passing pairs, take a segment of the output, write
+	for (int phase = 0; phase < 2; ++phase) {
+		uint32_t mine = 0;
+		for (uint32_t k = threadIdx.x; k < ncand; k += blockDim.x) {
+			uint32_t g = k;
+			Cell v;
+			if (by_gid) {
+				v = cell_of_gid<UNIQ, DIRECT>(t, g, spilled, spill_row, phase == 1);
+			} else {
+				g = t.gid[k];
+				if (g == EMPTY)
+					continue;
+				v = Cell{t.c[k], t.o[k], UNIQ ? t.uc[k] : 0ull, UNIQ ? t.uo[k] : 0ull};
+			}
+			if (!v.c)
+				continue;
+			double rank;
+			if (!qj_gate<UNIQ>(a, q, g, v, &rank))
+				continue;
+			if (phase == 0) {
+				++mine;
+				continue;
+			}
+			const uint32_t at = *s_base + __hip_atomic_fetch_add(s_n, 1u, __ATOMIC_RELAXED,
+									   __HIP_MEMORY_SCOPE_WORKGROUP);
+			if (at >= a.cap)
+				continue;
+			kssdhip_ani_row_t w;
+			w.query = q;
+			w.ref = g;
+			w.ctx.num_ctx = (uint32_t)v.c;
+			w.ctx.num_mut2_ctx = (uint32_t)(v.c >> 32);
+			w.obj.diff_obj = (uint32_t)v.o;
+			w.obj.diff_obj_section = (uint32_t)(v.o >> 32);
+			w.unique_ctx.num_ctx = (uint32_t)v.uc;
+			w.unique_ctx.num_mut2_ctx = (uint32_t)(v.uc >> 32);
+			w.unique_obj.diff_obj = (uint32_t)v.uo;
+			w.unique_obj.diff_obj_section = (uint32_t)(v.uo >> 32);
+			w.rank_ani = rank;
+			w.value = 0.0; // k_row_values
+			a.rows[at] = w;
+		}
+		if (phase == 0) {
+			if (mine)
+				__hip_atomic_fetch_add(s_n, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				const uint32_t npass = *s_n;
+				const uint32_t base = npass ? atomicAdd(a.row_count, npass) : 0u;
+				a.q_first[q] = base;
+				a.q_count[q] = npass;
+				*s_base = base;
+				*s_n = 0;
+			}
+			__syncthreads();
+		}
+	}
+}
+
+// Persistent workgroups: each takes the next query of the block from the
+// work counter until none is left (the reference's guided schedule over
+// queries, command_ani.h:481).
+template <bool UNIQ, bool DIRECT>
+__global__ void k_qjoin(QJArgs a)
+{
+	extern __shared__ unsigned long long qj_lds[];
+	__shared__ uint32_t s_q, s_spilled, s_n, s_base;
+	const uint32_t ncell = DIRECT ? a.R : a.slots;
+	QTable t;
+	t.c = (LDS unsigned long long *)qj_lds;
+	t.o = t.c + ncell;
+	t.uc = UNIQ ? t.o + ncell : nullptr;
+	t.uo = UNIQ ? t.uc + ncell : nullptr;
+	t.gid = DIRECT ? nullptr : (LDS uint32_t *)(t.o + (UNIQ ? 3 : 1) * (uint64_t)ncell);
+	t.mask = a.slots - 1;
+	t.shift = 32 - a.sbits;
+	unsigned long long *spill_row = DIRECT ? nullptr : a.spill + (uint64_t)blockIdx.x * a.R * (UNIQ ? 4u : 2u);
+	for (;;) {
+		if (threadIdx.x == 0) {
+			const uint32_t k = atomicAdd(a.next, 1u);
+#ifndef QJ_NO_ORDER
+			s_q = k < a.nq ? a.order[k] : a.nq;
+#else
+			s_q = k;
+#endif
+		}
+		__syncthreads(); // also: every thread is done with the previous query's table
+		const uint32_t q = s_q;
+		if (q >= a.nq)
+			break;
+		for (uint32_t k = threadIdx.x; k < ncell; k += blockDim.x) {
+			t.c[k] = 0;
+			t.o[k] = 0;
+			if (UNIQ) {
+				t.uc[k] = 0;
+				t.uo[k] = 0;
+			}
+			if (!DIRECT)
+				t.gid[k] = EMPTY;
+		}
+		if (threadIdx.x == 0) {
+			s_spilled = 0;
+			s_n = 0;
+		}
+		__syncthreads();
+		qj_count<UNIQ, DIRECT>(a, t, q, spill_row, (LDS uint32_t *)&s_spilled);
+		__syncthreads();
+		const bool spilled = s_spilled != 0;
+		qj_output<UNIQ, DIRECT>(a, t, q, spilled, spill_row, (LDS uint32_t *)&s_n, (LDS uint32_t *)&s_base);
+	}
+}
+
+// FP64 value of each stored row (the number of rows is read on the device,
+// so the launch can sit in a graph)
+__global__ __launch_bounds__(256) void k_row_values(kssdhip_ani_row_t *rows, const uint32_t *count, uint32_t cap,
+						    const uint32_t *__restrict__ qry_ctx,
+						    const uint32_t *__restrict__ ref_ctx, kssdhip_epilogue_opts_t vo,
+						    int vmetric)
+{
+	const uint32_t n = *count < cap ? *count : cap;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		kssdhip_ani_row_t &w = rows[i];
+		w.value = epi_cell_value(vo, vmetric, w.ctx.num_ctx, w.ctx.num_mut2_ctx, w.obj.diff_obj,
+					 w.obj.diff_obj_section, qry_ctx[w.query], ref_ctx[w.ref]);
+	}
+}
+
+struct QJShape {
+	bool direct;
+	uint32_t slots;
+	int sbits;
+	int threads;
+	size_t lds;
+	unsigned grid;
+	size_t spill_bytes;
+};
+
+int ilog2(uint32_t v)
+{
+	int b = 0;
+	while ((1u << (b + 1)) <= v)
+		++b;
+	return b;
+}
+
+template <bool UNIQ, bool DIRECT>
+const void *qj_kernel()
+{
+	return (const void *)k_qjoin<UNIQ, DIRECT>;
+}
+
+const void *qj_fn(bool uniq, bool direct)
+{
+	return uniq ? (direct ? qj_kernel<true, true>() : qj_kernel<true, false>())
+		    : (direct ? qj_kernel<false, true>() : qj_kernel<false, false>());
+}
+
+void qj_layout(const kssdhip_index *ix, bool uniq, QJShape *sh)
+{
+	const uint32_t R = (uint32_t)ix->ref_gnum;
+	const uint32_t cell_bytes = uniq ? 32u : 16u;
+	sh->threads = ix->qthreads ? ix->qthreads : DEFAULT_THREADS;
+	sh->direct = (uint64_t)R * cell_bytes <= DIRECT_LDS_MAX;
+	if (sh->direct) {
+		sh->slots = 0;
+		sh->sbits = 0;
+		sh->lds = (size_t)R * cell_bytes;
+	} else {
+		sh->slots = ix->qslots ? (uint32_t)ix->qslots : DEFAULT_SLOTS;
+		sh->sbits = ilog2(sh->slots);
+		sh->lds = (size_t)sh->slots * (cell_bytes + 4u);
+	}
+}
+
+int qj_shape(const kssdhip_index *ix, uint32_t nq, bool uniq, QJShape *sh)
+{
+	qj_layout(ix, uniq, sh);
+	const int per_cu = ix->qj_per_cu[uniq ? 1 : 0];
+	if (per_cu < 1)
+		return kssdhip_fail(KSSDHIP_EINVAL, "qjoin: %d threads with %zu bytes of LDS do not fit a CU", sh->threads,
+				    sh->lds);
+	const uint32_t R = (uint32_t)ix->ref_gnum;
+	uint64_t g = (uint64_t)per_cu * (uint64_t)(ix->cus > 0 ? ix->cus : 256);
+	if (g > nq)
+		g = nq;
+	if (g < 1)
+		g = 1;
+	sh->grid = (unsigned)g;
+	sh->spill_bytes = sh->direct ? 0 : (size_t)sh->grid * R * (uniq ? 32u : 16u);
+	return 0;
+}
+
+// The call's workspace: the work counter and the order's histogram, the
+// work order, then the spill rows (zero at rest; zeroed when the buffer
+// grows).  Leaves the counter and histogram zeroed and the order built, all
+// by kernels on s.
+struct QJWork {
+	uint32_t *head;
+	uint32_t *order;
+	unsigned long long *spill;
+};
+
+int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_t nq, hipStream_t s, QJWork *w)
+{
+	const size_t order_bytes = ((size_t)nq * 4 + WS_ALIGN - 1) / WS_ALIGN * WS_ALIGN;
+	const size_t want = WS_HEAD + order_bytes + sh.spill_bytes;
+	if (want > ws->bytes) {
+		int rc = kssd_reserve_for_stream(ws, want, s, "query-major workspace");
+		if (rc)
+			return rc;
+		const uint64_t words = ws->bytes / 8;
+		hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 8192)), dim3(256), 0, s,
+				   (unsigned long long *)ws->ptr, words);
+	}
+	w->head = (uint32_t *)ws->ptr;
+	w->order = (uint32_t *)((char *)ws->ptr + WS_HEAD);
+	w->spill = (unsigned long long *)((char *)ws->ptr + WS_HEAD + order_bytes);
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s, w->head, WS_HEAD_WORDS);
+	const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nq + 255) / 256, 1024));
+	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, d_skidx, nq, w->head);
+	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, w->head);
+	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, d_skidx, nq, w->head, w->order);
+	return 0;
+}
+
+void fill_index_args(const kssdhip_index *ix, QJArgs &q)
+{
+	q.rec = ix->d_qrec;
+	q.qstart = ix->d_qstart;
+	q.qfp = ix->d_qfp;
+	q.n = ix->n;
+	q.qshift = ix->qshift;
+	q.ctx_max = ix->ctx_max;
+	q.R = (uint32_t)ix->ref_gnum;
+	q.obj_bits = ix->obj_bits;
+}
+
+int qj_launch(const QJShape &sh, bool uniq, const QJArgs &a, hipStream_t s)
+{
+	const dim3 grid(sh.grid), block(sh.threads);
+	if (uniq && sh.direct)
+		hipLaunchKernelGGL((k_qjoin<true, true>), grid, block, sh.lds, s, a);
+	else if (uniq)
+		hipLaunchKernelGGL((k_qjoin<true, false>), grid, block, sh.lds, s, a);
+	else if (sh.direct)
+		hipLaunchKernelGGL((k_qjoin<false, true>), grid, block, sh.lds, s, a);
+	else
+		hipLaunchKernelGGL((k_qjoin<false, false>), grid, block, sh.lds, s, a);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "qjoin launch: %s", hipGetErrorString(e));
+	return 0;
+}
+
+} // namespace
+
+int kssd_qjoin_build(kssdhip_index *ix)
+{
+	kssd_qjoin_free(ix);
+	int dev_cus = 0;
+	if (hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ix->device) == hipSuccess)
+		ix->cus = dev_cus;
+	const uint64_t n = ix->n;
+	// about 8-16 records per bucket, at most 2^24 buckets (128 MB)
+	int bw = 0;
+	while (bw < 64 && (n >> bw))
+		++bw;
+	int k = bw > 4 ? bw - 4 : 1;
+	if (k > 24)
+		k = 24;
+	int cbw = 0;
+	while (cbw < 64 && (ix->ctx_max >> cbw))
+		++cbw;
+	if (k > cbw)
+		k = cbw > 0 ? cbw : 1;
+	ix->qk = k;
+	ix->qshift = cbw - k > 0 ? cbw - k : 0;
+	const uint64_t nb = 1ull << k;
+	if (hipMalloc(&ix->d_qrec, (n + 1) * sizeof(uint4)) != hipSuccess ||
+	    hipMalloc(&ix->d_qstart, (nb + 1) * sizeof(uint32_t)) != hipSuccess ||
+	    hipMalloc(&ix->d_qfp, nb * sizeof(uint32_t)) != hipSuccess) {
+		kssd_qjoin_free(ix);
+		return kssdhip_fail(KSSDHIP_ENOMEM, "query-major index: device allocation failed");
+	}
+	const hipStream_t s = ix->stream;
+	auto grid = [](uint64_t items) { return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + 255) / 256, 1u << 20))); };
+	if (n)
+		hipLaunchKernelGGL(k_pack16, grid(n), dim3(256), 0, s, ix->d_ctxgid, ix->d_obj, n, ix->d_qrec);
+	hipLaunchKernelGGL(k_qbkt_start, grid(nb + 1), dim3(256), 0, s, ix->d_ctxgid, n, ix->qshift, nb, ix->d_qstart);
+	hipLaunchKernelGGL(k_qbkt_fp, grid(nb), dim3(256), 0, s, ix->d_ctxgid, nb, ix->d_qstart, ix->d_qfp);
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(s);
+	if (e != hipSuccess) {
+		kssd_qjoin_free(ix);
+		return kssdhip_fail(KSSDHIP_EHIP, "query-major index: %s", hipGetErrorString(e));
+	}
+	return kssd_qjoin_prepare(ix);
+}
+
+void kssd_qjoin_free(kssdhip_index *ix)
+{
+	if (ix->d_qrec)
+		(void)hipFree(ix->d_qrec);
+	if (ix->d_qstart)
+		(void)hipFree(ix->d_qstart);
+	if (ix->d_qfp)
+		(void)hipFree(ix->d_qfp);
+	ix->d_qrec = nullptr;
+	ix->d_qstart = nullptr;
+	ix->d_qfp = nullptr;
+}
+
+int kssd_qjoin_prepare(kssdhip_index *ix)
+{
+	for (int u = 0; u < 2; ++u) {
+		QJShape sh;
+		qj_layout(ix, u != 0, &sh);
+		const void *fn = qj_fn(u != 0, sh.direct);
+		ix->qj_per_cu[u] = 0;
+		if (sh.lds > 65536 &&
+		    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds) != hipSuccess)
+			continue; // does not fit: launches with this tiling fail with EINVAL
+		int per_cu = 0;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sh.threads, sh.lds) == hipSuccess)
+			ix->qj_per_cu[u] = per_cu;
+	}
+	(void)hipGetLastError();
+	return 0;
+}
+
+bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total)
+{
+	// the partitioned merge-join wins when each query brings about two or
+	// more keys to every index partition; the query-major join when the
+	// index is so much larger than a query that most (query, partition)
+	// pairs are empty; blocks of fewer queries than compute units go to the
+	// one-thread-per-key lookup
+	if (!ix->d_qrec || nq < (ix->cus > 0 ? ix->cus : 256))
+		return false;
+	const double parts = ix->P ? (double)ix->P : 1.0;
+	return (double)total < 2.0 * parts * (double)nq;
+}
+
+int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevBuf *ws)
+{
+	const bool uniq = (ja.flags & KSSDHIP_UNIQUE) && !(ja.flags & KSSDHIP_LOWER);
+	if (!ix->d_qrec)
+		return kssdhip_fail(KSSDHIP_EINVAL, "qjoin: the index has no query-major tables");
+	QJShape sh;
+	int rc = qj_shape(ix, (uint32_t)ja.nq, uniq, &sh);
+	if (rc)
+		return rc;
+	QJWork w;
+	if ((rc = qj_workspace(ws, sh, ja.skidx, (uint32_t)ja.nq, s, &w)))
+		return rc;
+	QJArgs a;
+	memset(&a, 0, sizeof a);
+	fill_index_args(ix, a);
+	a.keys = ja.keys;
+	a.skidx = ja.skidx;
+	a.nq = (uint32_t)ja.nq;
+	a.qry_gid_offset = ja.qry_gid_offset;
+	a.flags = ja.flags;
+	a.slots = sh.slots;
+	a.sbits = sh.sbits;
+	a.next = w.head;
+	a.order = w.order;
+	a.spill = w.spill;
+	a.mode = (ja.flags & KSSDHIP_ACCUMULATE) ? QJ_ACCUM : QJ_DENSE;
+	a.ctx = ja.ctx;
+	a.objc = ja.objc;
+	a.uctx = uniq ? ja.uctx : nullptr;
+	a.uobj = uniq ? ja.uobj : nullptr;
+	return qj_launch(sh, uniq, a, s);
+}
+
+extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq,
+					uint64_t total, uint32_t flags, const uint32_t *d_qry_ctx,
+					const uint32_t *d_ref_ctx, const kssdhip_select_opts_t *opts,
+					kssdhip_ani_row_t *d_rows, uint32_t cap, uint32_t *d_row_count,
+					uint32_t *d_q_first, uint32_t *d_q_count, void *stream)
+{
+	if (!ix || !opts || nq < 0 || !d_row_count)
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: bad arguments");
+	if (flags & ~(KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT))
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: flags 0x%x (UNIQUE and IGNORECONFLICT only)", flags);
+	if (opts->ctxcut < 1)
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: ctxcut must be >= 1");
+	if (opts->unique_rank && !(flags & KSSDHIP_UNIQUE))
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: unique_rank needs KSSDHIP_UNIQUE");
+	if (!epi_metric_supported(opts->value))
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: value metric %d needs the calibration model (use 3..9)",
+				    opts->value.metric);
+	if (nq > 0 && (!d_skidx || !d_qry_ctx || !d_ref_ctx || !d_q_first || !d_q_count || (cap && !d_rows) ||
+		       (total && !d_keys)))
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: null device pointer");
+	if (!ix->d_qrec)
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: the index has no query-major tables");
+	hipStream_t s = (hipStream_t)stream;
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_row_count, 1u);
+	if (nq == 0) {
+		hipError_t e = hipGetLastError();
+		return e == hipSuccess ? 0 : kssdhip_fail(KSSDHIP_EHIP, "select_block: %s", hipGetErrorString(e));
+	}
+	const bool uniq = (flags & KSSDHIP_UNIQUE) != 0;
+	QJShape sh;
+	int rc = qj_shape(ix, (uint32_t)nq, uniq, &sh);
+	if (rc)
+		return rc;
+	QJWork w;
+	if ((rc = qj_workspace(&ix->qws, sh, d_skidx, (uint32_t)nq, s, &w)))
+		return rc;
+	QJArgs a;
+	memset(&a, 0, sizeof a);
+	fill_index_args(ix, a);
+	a.keys = d_keys;
+	a.skidx = d_skidx;
+	a.nq = (uint32_t)nq;
+	a.flags = flags;
+	a.slots = sh.slots;
+	a.sbits = sh.sbits;
+	a.next = w.head;
+	a.order = w.order;
+	a.spill = w.spill;
+	a.mode = QJ_SELECT;
+	a.qry_ctx = d_qry_ctx;
+	a.ref_ctx = d_ref_ctx;
+	a.ctxcut = opts->ctxcut;
+	a.afcut = (double)opts->afcut;
+	a.anicut = opts->anicut;
+	a.unique_rank = opts->unique_rank;
+	a.vo = opts->value;
+	a.vmetric = 0;
+	{
+		int m = a.vo.metric;
+		if (m < 1 || m > 9)
+			m = 1;
+		a.vmetric = (a.vo.unassembled && m <= 4) ? 4 : m;
+	}
+	a.rows = d_rows;
+	a.cap = cap;
+	a.row_count = d_row_count;
+	a.q_first = d_q_first;
+	a.q_count = d_q_count;
+	if ((rc = qj_launch(sh, uniq, a, s)))
+		return rc;
+	if (cap) {
+		const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((cap + 255) / 256, 1024));
+		hipLaunchKernelGGL(k_row_values, dim3(g), dim3(256), 0, s, d_rows, d_row_count, cap, d_qry_ctx, d_ref_ctx,
+				   a.vo, a.vmetric);
+		hipError_t e = hipGetLastError();
+		if (e != hipSuccess)
+			return kssdhip_fail(KSSDHIP_EHIP, "select_block values: %s", hipGetErrorString(e));
+	}
+	return 0;
+}
+
+extern "C" int kssdhip_select_block(kssdhip_index *ix, const uint64_t *section_sk, const uint64_t *section_skidx,
+				    int nq, uint32_t flags, const uint32_t *qry_ctx, const uint32_t *ref_ctx,
+				    const kssdhip_select_opts_t *opts, kssdhip_ani_row_t *rows, size_t cap,
+				    size_t *n_rows)
+{
+	if (!ix || !opts || nq < 0 || !n_rows || (nq > 0 && (!section_skidx || !qry_ctx || !ref_ctx)) ||
+	    (cap && !rows))
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: bad arguments");
+	*n_rows = 0;
+	if (nq == 0)
+		return 0;
+	const uint64_t base = section_skidx[0];
+	std::vector<uint64_t> rel((size_t)nq + 1);
+	for (int i = 0; i <= nq; ++i) {
+		if (section_skidx[i] < base || (i && section_skidx[i] < section_skidx[i - 1]))
+			return kssdhip_fail(KSSDHIP_EINVAL, "select_block: section_skidx not ascending");
+		rel[i] = section_skidx[i] - base;
+	}
+	const uint64_t total = rel[nq];
+	if (total && !section_sk)
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: null keys");
+	const uint32_t dcap = (uint32_t)std::min<size_t>(cap, 0xffffffffu);
+	if (hipSetDevice(ix->device) != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "select_block: device not available");
+	int rc;
+	const size_t R = (size_t)ix->ref_gnum;
+	if ((rc = ix->keys.reserve((total ? total : 1) * 8)) || (rc = ix->skidx.reserve(rel.size() * 8)) ||
+	    (rc = ix->qctxb.reserve((size_t)nq * 4)) || (rc = ix->rctxb.reserve(R * 4)) ||
+	    (rc = ix->rows.reserve((dcap ? dcap : 1) * sizeof(kssdhip_ani_row_t))) ||
+	    (rc = ix->qmeta.reserve(((size_t)nq * 2 + 1) * 4)))
+		return rc;
+	hipStream_t s = ix->stream;
+	if (total && copy_h2d(ix->keys.ptr, section_sk, total * 8, s) != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "select_block: key upload failed");
+	if (hipMemcpyAsync(ix->skidx.ptr, rel.data(), rel.size() * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+	    hipMemcpyAsync(ix->qctxb.ptr, qry_ctx, (size_t)nq * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+	    hipMemcpyAsync(ix->rctxb.ptr, ref_ctx, R * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "select_block: upload failed");
+	uint32_t *meta = (uint32_t *)ix->qmeta.ptr; // [0] row count, [1, nq] first, [nq + 1, 2 nq] count
+	rc = kssdhip_select_block_dev(ix, (const uint64_t *)ix->keys.ptr, (const uint64_t *)ix->skidx.ptr, nq, total,
+				      flags, (const uint32_t *)ix->qctxb.ptr, (const uint32_t *)ix->rctxb.ptr, opts,
+				      (kssdhip_ani_row_t *)ix->rows.ptr, dcap, meta, meta + 1, meta + 1 + nq, s);
+	if (rc)
+		return rc;
+	uint32_t count = 0;
+	if (hipMemcpyAsync(&count, meta, 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "select_block: %s", hipGetErrorString(hipGetLastError()));
+	*n_rows = count;
+	if (count > cap)
+		return kssdhip_fail(KSSDHIP_ENOSPC, "select_block: %u passing pairs exceed the %zu-row buffer", count, cap);
+	if (count) {
+		if (copy_d2h(rows, ix->rows.ptr, (size_t)count * sizeof(kssdhip_ani_row_t), s) != hipSuccess)
+			return kssdhip_fail(KSSDHIP_EHIP, "select_block: row download failed");
+		std::sort(rows, rows + count, [](const kssdhip_ani_row_t &x, const kssdhip_ani_row_t &y) {
+			return x.query != y.query ? x.query < y.query : x.ref < y.ref;
+		});
+	}
+	return 0;
+}

@@ -146,7 +146,7 @@ int launch(kssdhip_server *s, Slot &sl)
 	int rc = kssd_count_dev(s->ix, (const uint64_t *)d, (const uint64_t *)(d + b.off_skidx), nq, b.total, 0,
 				s->opts.flags, (kssd_ctx_mut2_t *)(d + b.off_ctx), (kssd_obj_section_t *)(d + b.off_obj),
 				s->unique ? (kssd_ctx_mut2_t *)(d + b.off_uctx) : nullptr,
-				s->unique ? (kssd_obj_section_t *)(d + b.off_uobj) : nullptr, sl.stream, &sl.split);
+				s->unique ? (kssd_obj_section_t *)(d + b.off_uobj) : nullptr, sl.stream, &sl.split, nullptr);
 	if (rc)
 		return rc;
 	if (hipMemcpyAsync(h + b.off_ctx, d + b.off_ctx, b.end - b.off_ctx, hipMemcpyDeviceToHost, sl.stream) !=

@@ -42,11 +42,28 @@ class EpilogueOpts(ctypes.Structure):
 
 
 class Tiling(ctypes.Structure):
-    """kssdhip_tiling_t: mode 0 auto, 1 lookup, 2 partitioned merge-join;
-    group_partitions > 1 lets one workgroup of the join walk that many
-    consecutive partitions, summing hits per cell in LDS first."""
+    """kssdhip_tiling_t: mode 0 auto, 1 lookup, 2 partitioned merge-join,
+    3 query-major join; group_partitions > 1 lets one workgroup of the
+    partitioned join walk that many consecutive partitions, summing hits per
+    cell in LDS first; query_threads / query_slots shape the query-major
+    kernel (0 = defaults)."""
     _fields_ = [("mode", I), ("chunk_records", I), ("block_threads", I), ("auto_min_keys", D),
-                ("group_partitions", I)]
+                ("group_partitions", I), ("query_threads", I), ("query_slots", I)]
+
+
+class SelectOpts(ctypes.Structure):
+    """kssdhip_select_opts_t: the index path's pass gate and the FP64 value
+    computed for each passing pair."""
+    _fields_ = [("ctxcut", U32), ("afcut", ctypes.c_float), ("anicut", ctypes.c_float), ("unique_rank", I),
+                ("value", EpilogueOpts)]
+
+
+ROW_DTYPE = np.dtype([("query", "<u4"), ("ref", "<u4"), ("num_ctx", "<u4"), ("num_mut2_ctx", "<u4"),
+                      ("diff_obj", "<u4"), ("diff_obj_section", "<u4"), ("u_num_ctx", "<u4"),
+                      ("u_num_mut2_ctx", "<u4"), ("u_diff_obj", "<u4"), ("u_diff_obj_section", "<u4"),
+                      ("rank_ani", "<f8"), ("value", "<f8")])
+assert ROW_DTYPE.itemsize == 56
+ENOSPC = -6
 
 
 class KssdStat(ctypes.Structure):
@@ -141,10 +158,14 @@ class Hip:
         self.check(self.lib.hipStreamBeginCapture(stream, 0))  # hipStreamCaptureModeGlobal
         try:
             fn()
-        finally:
+        except BaseException:
             g = V()
-            rc = self.lib.hipStreamEndCapture(stream, ctypes.byref(g))
-        self.check(rc)
+            self.lib.hipStreamEndCapture(stream, ctypes.byref(g))
+            if g:
+                self.lib.hipGraphDestroy(g)
+            raise
+        g = V()
+        self.check(self.lib.hipStreamEndCapture(stream, ctypes.byref(g)))
         ex = V()
         self.check(self.lib.hipGraphInstantiate(ctypes.byref(ex), g, None, None, 0))
         return g, ex
@@ -268,6 +289,10 @@ def load_kssdhip():
     lib.kssdhip_index_get_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
     lib.kssdhip_epilogue_dev.argtypes = [V, V, I, I, V, V, ctypes.POINTER(EpilogueOpts), V, V]
     lib.kssdhip_select_hits_dev.argtypes = [V, I, I, ctypes.c_double, U32, V, V, V]
+    lib.kssdhip_select_block.argtypes = [V, V, V, I, U32, V, V, ctypes.POINTER(SelectOpts), V, S,
+                                         ctypes.POINTER(S)]
+    lib.kssdhip_select_block_dev.argtypes = [V, V, V, I, U64, U32, V, V, ctypes.POINTER(SelectOpts), V, U32,
+                                             V, V, V, V]
     lib.kssdhip_serve_open.argtypes = [V, ctypes.POINTER(ServeOpts), ctypes.POINTER(V)]
     lib.kssdhip_serve_submit.argtypes = [V, V, S]
     lib.kssdhip_serve_submit.restype = ctypes.c_int64
@@ -318,8 +343,12 @@ class KssdHip:
         return rec, (bits[:(n + 7) // 8].tobytes() if unique else None)
 
     def set_tiling(self, h, mode=None, chunk_records=None, block_threads=None, auto_min_keys=None,
-                   group_partitions=None):
+                   group_partitions=None, query_threads=None, query_slots=None):
         t = self.get_tiling(h)
+        if query_threads is not None:
+            t.query_threads = query_threads
+        if query_slots is not None:
+            t.query_slots = query_slots
         if group_partitions is not None:
             t.group_partitions = group_partitions
         if mode is not None:
@@ -408,6 +437,33 @@ class KssdHip:
     def epilogue_dev(self, d_ctx, d_obj, nq, nr, d_qctx, d_rctx, opts, d_out, stream=None):
         self.check(self.lib.kssdhip_epilogue_dev(d_ctx, d_obj, nq, nr, d_qctx, d_rctx,
                                                  ctypes.byref(opts), d_out, stream))
+
+    def select_block(self, h, keys, skidx_abs, qry_ctx, ref_ctx, opts, flags=0, cap=None):
+        """Passing pairs of the index path's gate as a ROW_DTYPE array
+        ordered by (query, ref); the buffer grows on KSSDHIP_ENOSPC."""
+        nq = len(skidx_abs) - 1
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        skidx_abs = np.ascontiguousarray(skidx_abs, dtype=np.uint64)
+        qry_ctx = np.ascontiguousarray(qry_ctx, dtype=np.uint32)
+        ref_ctx = np.ascontiguousarray(ref_ctx, dtype=np.uint32)
+        cap = max(1, 64 * nq) if cap is None else cap
+        while True:
+            rows = np.zeros(max(cap, 1), dtype=ROW_DTYPE)
+            n = S()
+            rc = self.lib.kssdhip_select_block(h, _p(keys) if len(keys) else None, _p(skidx_abs), nq, flags,
+                                               _p(qry_ctx), _p(ref_ctx), ctypes.byref(opts), _p(rows), cap,
+                                               ctypes.byref(n))
+            if rc == ENOSPC:
+                cap = int(n.value)
+                continue
+            self.check(rc)
+            return rows[: int(n.value)].copy()
+
+    def select_block_dev(self, h, d_keys, d_skidx, nq, total, flags, d_qctx, d_rctx, opts, d_rows, cap,
+                         d_count, d_first, d_qcount, stream=None):
+        self.check(self.lib.kssdhip_select_block_dev(h, d_keys, d_skidx, nq, total, flags, d_qctx, d_rctx,
+                                                     ctypes.byref(opts), d_rows, cap, d_count, d_first, d_qcount,
+                                                     stream))
 
     def select_hits_dev(self, d_values, nq, nr, min_value, cap, d_hits, d_count, stream=None):
         self.check(self.lib.kssdhip_select_hits_dev(d_values, nq, nr, float(min_value), cap, d_hits, d_count,

@@ -262,23 +262,6 @@ void orc_pair_features(const uint64_t *a, size_t n, const uint64_t *b, size_t m,
 	}
 }
 
-void orc_pair_features_many(const uint64_t *qkeys, const uint64_t *qidx,
-			    const uint64_t *rkeys, const uint64_t *ridx,
-			    const int32_t *pq, const int32_t *pr, size_t npairs,
-			    unsigned obj_bits, int ignore_b_conflict,
-			    orc_features_t *out, int nthreads)
-{
-	if (nthreads <= 0)
-		nthreads = 1;
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
-	for (size_t p = 0; p < npairs; ++p) {
-		const int q = pq[p], r = pr[p];
-		orc_pair_features(qkeys + qidx[q], (size_t)(qidx[q + 1] - qidx[q]),
-				  rkeys + ridx[r], (size_t)(ridx[r + 1] - ridx[r]),
-				  obj_bits, ignore_b_conflict, &out[p]);
-	}
-}
-
 static int orc_rec_cmp(const void *pa, const void *pb)
 {
 	/* kssdlib_sort.c:145-148: order by ctxgid, then obj */
@@ -502,4 +485,147 @@ void orc_matrix_values(const orc_ctx_t *ctx, const orc_obj_t *obj, int nq, int n
 		}
 		out[c] = values_ani ? 1.0 - d : d;
 	}
+}
+
+/* ------------------------------------------------------------ index-path ranking */
+typedef struct {
+	uint32_t id;
+	float ani;
+} orc_idani_t; /* idani_t, command_ani.h:353-356 */
+
+static int orc_cmp_idani_desc(const void *a, const void *b)
+{
+	/* compare_idani_desc, command_ani.h:361-366 */
+	const orc_idani_t *x = a, *y = b;
+	return (x->ani < y->ani) - (x->ani > y->ani);
+}
+
+static void orc_heap_down(orc_idani_t *h, int n, int i)
+{
+	/* heap_sift_down_min_idani, command_ani.h:369-383 */
+	for (;;) {
+		const int l = 2 * i + 1, r = l + 1;
+		int s = i;
+		if (l < n && h[l].ani < h[s].ani)
+			s = l;
+		if (r < n && h[r].ani < h[s].ani)
+			s = r;
+		if (s == i)
+			break;
+		const orc_idani_t t = h[i];
+		h[i] = h[s];
+		h[s] = t;
+		i = s;
+	}
+}
+
+static int orc_topN(orc_idani_t *arr, int m, int topN)
+{
+	/* idani_select_topN_sorted + partial_topN_inplace_idani,
+	 * command_ani.h:391-433; qsort is the C library's, as the reference's */
+	if (m <= 0 || topN <= 0)
+		return 0;
+	if (topN > m)
+		topN = m;
+	if (m <= 128 || topN >= m / 2) {
+		qsort(arr, (size_t)m, sizeof *arr, orc_cmp_idani_desc);
+		return topN;
+	}
+	if (m > 1) {
+		for (int i = (topN >> 1) - 1; i >= 0; --i)
+			orc_heap_down(arr, topN, i);
+		for (int i = topN; i < m; ++i) {
+			if (arr[i].ani <= arr[0].ani)
+				continue;
+			const orc_idani_t t = arr[i];
+			arr[i] = arr[0];
+			arr[0] = t;
+			orc_heap_down(arr, topN, 0);
+		}
+		qsort(arr, (size_t)topN, sizeof *arr, orc_cmp_idani_desc);
+	}
+	return topN;
+}
+
+typedef struct {
+	double ani;
+	uint32_t id;
+} orc_urank_t;
+
+static int orc_cmp_urank(const void *a, const void *b)
+{
+	/* compare_unique_ctx_rank, command_ani.h:456-461 */
+	const orc_urank_t *x = a, *y = b;
+	if (x->ani != y->ani)
+		return x->ani < y->ani ? 1 : -1;
+	return (x->id > y->id) - (x->id < y->id);
+}
+
+static double orc_naive_of(orc_ctx_t c, orc_obj_t o)
+{
+	/* ani_context_naive_dist, command_ani.h:442-450 */
+	orc_features_t f = {0};
+	f.XnY_ctx = c.num_ctx;
+	f.N_mut2_ctx = c.num_mut2_ctx;
+	f.N_diff_obj = o.diff_obj;
+	f.N_diff_obj_section = o.diff_obj_section;
+	return orc_naive_dist(&f);
+}
+
+void orc_index_rank(const orc_ctx_t *ctx, const orc_obj_t *obj, const orc_ctx_t *uctx, const orc_obj_t *uobj,
+		    int nq, int R, const uint32_t *ref_ctx, const uint32_t *qry_ctx, int ctxcut_opt, float afcut,
+		    float anicut, int ntop_opt, int unique_ctx, uint32_t *ids, uint32_t *pass_ids, double *rank,
+		    uint32_t *n_pass, uint32_t *n_out)
+{
+	/* the fmt == 0 branch of count_ctx_obj_frm_comb_sketch_section_with_unique,
+	 * command_ani.h:577-621 */
+	const uint32_t ctxcut = (uint32_t)ctxcut_opt;
+	const int ntop = ntop_opt > 0 ? ntop_opt : R;
+	orc_idani_t *ida = malloc((size_t)(R > 0 ? R : 1) * sizeof *ida);
+	orc_urank_t *ranks = unique_ctx ? malloc((size_t)(R > 0 ? R : 1) * sizeof *ranks) : NULL;
+	for (int i = 0; i < nq; ++i) {
+		const size_t row = (size_t)i * (size_t)R;
+		int np = 0;
+		for (int j = 0; j < R; ++j) {
+			const uint32_t shared = ctx[row + j].num_ctx;
+			if (shared < ctxcut)
+				continue;
+			if (ref_ctx[j] == 0 || qry_ctx[i] == 0)
+				continue;
+			const double afq = (double)shared / (double)qry_ctx[i], afr = (double)shared / (double)ref_ctx[j];
+			if (!((afq > afr ? afq : afr) >= (double)afcut))
+				continue;
+			const float dist = (float)obj[row + j].diff_obj / ctx[row + j].num_ctx / 7;
+			double ani = (float)(1 - dist);
+			if (ranks) {
+				/* ani_use_unique_context, command_ani.h:437-440 */
+				const int use = (uint64_t)uctx[row + j].num_ctx * 20 >= ref_ctx[j];
+				ani = 1.0 - orc_naive_of(use ? uctx[row + j] : ctx[row + j], use ? uobj[row + j] : obj[row + j]);
+			}
+			if (ani < anicut)
+				continue;
+			if (ranks)
+				ranks[np] = (orc_urank_t){ani, (uint32_t)j};
+			ida[np].id = (uint32_t)j;
+			ida[np].ani = (float)ani;
+			rank[row + (size_t)np] = ani;
+			pass_ids[row + (size_t)np] = (uint32_t)j;
+			np++;
+		}
+		n_pass[i] = (uint32_t)np;
+		int out;
+		if (ranks) {
+			qsort(ranks, (size_t)np, sizeof *ranks, orc_cmp_urank);
+			out = np < ntop ? np : ntop;
+			for (int r = 0; r < out; ++r)
+				ida[r].id = ranks[r].id;
+		} else {
+			out = orc_topN(ida, np, ntop);
+		}
+		for (int r = 0; r < out; ++r)
+			ids[row + (size_t)r] = ida[r].id;
+		n_out[i] = (uint32_t)out;
+	}
+	free(ida);
+	free(ranks);
 }

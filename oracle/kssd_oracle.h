@@ -71,13 +71,6 @@ void orc_count_block(const orc_rec_t *index, size_t n_index, int ref_gnum,
 void orc_pair_features(const uint64_t *a, size_t n, const uint64_t *b, size_t m,
 		       unsigned obj_bits, int ignore_b_conflict, orc_features_t *out);
 
-/* Many pairs: out[p] for (qa[p], rb[p]) using sketch offsets; OpenMP. */
-void orc_pair_features_many(const uint64_t *qkeys, const uint64_t *qidx,
-			    const uint64_t *rkeys, const uint64_t *ridx,
-			    const int32_t *pq, const int32_t *pr, size_t npairs,
-			    unsigned obj_bits, int ignore_b_conflict,
-			    orc_features_t *out, int nthreads);
-
 /*
  * Index build: ctxobj64_2ctxgidobj (sketch_rearrange.c:65-78) followed by a
  * sort on (ctxgid, obj) (kssdlib_sort.c:145-148).  out has skidx[ng] records.
@@ -105,6 +98,20 @@ void orc_matrix_values(const orc_ctx_t *ctx, const orc_obj_t *obj, int nq, int n
 		       const uint32_t *qctx, const uint32_t *rctx, int metric, int values_ani,
 		       double afcut, double e, int drfold, unsigned obj_bits, unsigned ctx_bits,
 		       double *out, int nthreads);
+
+/*
+ * The pass gate and ranking of the index path's block function for fmt == 0
+ * (command_ani.h:577-621), on dense counts [nq x R] (uctx/uobj with
+ * unique_ctx).  Per query row i: pass_ids[i*R + k] / rank[i*R + k] are the
+ * k-th passing reference in ascending id order and its ranking value
+ * (k < n_pass[i]), and
+ * ids[i*R + r] the printed order after the top-N cut (r < n_out[i]).
+ * ctxcut_opt / afcut / anicut / ntop_opt as ani_opt_t holds them.
+ */
+void orc_index_rank(const orc_ctx_t *ctx, const orc_obj_t *obj, const orc_ctx_t *uctx, const orc_obj_t *uobj,
+		    int nq, int R, const uint32_t *ref_ctx, const uint32_t *qry_ctx, int ctxcut_opt, float afcut,
+		    float anicut, int ntop_opt, int unique_ctx, uint32_t *ids, uint32_t *pass_ids, double *rank,
+		    uint32_t *n_pass, uint32_t *n_out);
 
 #ifdef __cplusplus
 }

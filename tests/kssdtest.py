@@ -22,7 +22,8 @@ ROOT = Path(__file__).resolve().parents[1]
 PKG = ROOT / "kssd3a-public_amd"
 sys.path.insert(0, str(PKG))
 from kssdhip import (Features, Host, Hip, KssdHip, EpilogueOpts, SimSpec, HIT_DTYPE,  # noqa: E402,F401
-                     load_kssdhip, load_kssdhost)
+                     SelectOpts, ROW_DTYPE, LOWER, UNIQUE, IGNORECONFLICT, ACCUMULATE, load_kssdhip,
+                     load_kssdhost)
 GOLDEN = ROOT / "tests" / "golden"
 KSSDHIP_EINVAL, KSSDHIP_EAGAIN = -1, -5
 STAT = struct.Struct("<I??2x6i")
@@ -172,8 +173,9 @@ def load_oracle():
     lib.orc_count_block.restype = None
     lib.orc_pair_features.argtypes = [V, S, V, S, U, I, ctypes.POINTER(Features)]
     lib.orc_pair_features.restype = None
-    lib.orc_pair_features_many.argtypes = [V, V, V, V, V, V, S, U, I, V, I]
-    lib.orc_pair_features_many.restype = None
+    lib.orc_index_rank.argtypes = [V, V, V, V, I, I, V, V, I, ctypes.c_float, ctypes.c_float, I, I, V, V, V, V,
+                                   V]
+    lib.orc_index_rank.restype = None
     lib.orc_build_index.argtypes = [V, V, I, U, V]
     lib.orc_build_index.restype = None
     lib.orc_ctx_runs.argtypes = [V, S, U, I]
@@ -225,6 +227,27 @@ class Oracle:
         self.lib.orc_pair_features(_p(a), len(a), _p(b), len(b), obj_bits, int(ignore_b_conflict),
                                    ctypes.byref(f))
         return f.tup()
+
+    def index_rank(self, ctx, obj, uctx, uobj, ref_ctx, qry_ctx, ctxcut=3, afcut=0.0, anicut=0.0, ntop=0,
+                   unique_ctx=False):
+        """The index path's gate and top-N (command_ani.h:577-621) on dense
+        counts: per query (passing ids ascending, their ranking values, the
+        printed order after the top-N cut)."""
+        nq, R = ctx.shape[0], ctx.shape[1]
+        ids = np.zeros((nq, R), np.uint32)
+        pass_ids = np.zeros((nq, R), np.uint32)
+        rank = np.zeros((nq, R), np.float64)
+        n_pass = np.zeros(nq, np.uint32)
+        n_out = np.zeros(nq, np.uint32)
+        ref_ctx = np.ascontiguousarray(ref_ctx, dtype=np.uint32)
+        qry_ctx = np.ascontiguousarray(qry_ctx, dtype=np.uint32)
+        self.lib.orc_index_rank(_p(np.ascontiguousarray(ctx)), _p(np.ascontiguousarray(obj)),
+                                _p(np.ascontiguousarray(uctx)) if unique_ctx else None,
+                                _p(np.ascontiguousarray(uobj)) if unique_ctx else None, nq, R, _p(ref_ctx),
+                                _p(qry_ctx), int(ctxcut), float(afcut), float(anicut), int(ntop), int(unique_ctx),
+                                _p(ids), _p(pass_ids), _p(rank), _p(n_pass), _p(n_out))
+        return [(pass_ids[i, :n_pass[i]].copy(), rank[i, :n_pass[i]].copy(), ids[i, :n_out[i]].copy())
+                for i in range(nq)]
 
     def find_first(self, records, keys, obj_bits, k=17):
         F = np.zeros((1 << k) + 1, dtype=np.uint64)

@@ -1,0 +1,286 @@
+"""GPU parity of the query-major join (tiling mode 3) and of the device-side
+pass gate (kssdhip_select_block), through the C ABI, against the CPU oracle:
+dense counts bit-exact for every flag set, with the LDS table in its direct
+form (few references), its hashed form, and with queries that touch more
+genomes than the table holds (spill rows); the gate's passing pairs equal the
+oracle's restatement of command_ani.h:585-606 pair for pair."""
+import numpy as np
+import pytest
+
+import kssdtest as kt
+
+pytestmark = pytest.mark.gpu
+OBJ_BITS = 20
+
+
+def _block(samples):
+    keys = np.concatenate(samples) if samples else np.zeros(0, np.uint64)
+    idx = np.zeros(len(samples) + 1, np.uint64)
+    idx[1:] = np.cumsum([len(s) for s in samples])
+    return keys, idx
+
+
+def fast_samples(rng, n, pool, base_obj, mean, conflict_p=0.2, obj_bits=OBJ_BITS):
+    """Samples of about `mean` contexts drawn from `pool` (vectorised);
+    objects are the pool's base object with 1-3 lanes changed for 30% of the
+    entries, and a share of the contexts carries a second object."""
+    out = []
+    for _ in range(n):
+        k = min(int(rng.integers(0, 2 * mean + 1)), len(pool))
+        ci = rng.choice(len(pool), size=k, replace=False)
+        obj = base_obj[ci].copy()
+        noisy = rng.random(k) < 0.3
+        lanes = rng.integers(0, obj_bits // 2, size=k).astype(np.uint64)
+        obj[noisy] ^= rng.integers(1, 4, size=int(noisy.sum())).astype(np.uint64) << (np.uint64(2) * lanes[noisy])
+        keys = (pool[ci] << np.uint64(obj_bits)) | obj
+        conf = rng.random(k) < conflict_p
+        extra = (pool[ci[conf]] << np.uint64(obj_bits)) | (obj[conf] ^ np.uint64(3))
+        out.append(np.unique(np.concatenate([keys, extra])))
+    return out
+
+
+def _wide(seed, n_ref, n_qry, pool_n, mean):
+    rng = np.random.default_rng(seed)
+    pool = rng.integers(1, 1 << 43, size=pool_n, dtype=np.uint64)
+    base = rng.integers(0, 1 << OBJ_BITS, size=pool_n, dtype=np.uint64)
+    refs = fast_samples(rng, n_ref, pool, base, mean)
+    qrys = fast_samples(rng, n_qry, pool, base, 4 * mean)
+    qrys[1] = np.zeros(0, np.uint64)  # an empty query
+    return refs, qrys
+
+
+# (references, queries, context pool, mean entries, query_slots): the LDS
+# table in direct form; hashed without spilling; hashed with most queries
+# touching more genomes than its 256 slots
+SHAPES = {
+    "direct": (40, 70, 3000, 200, 0),
+    "hashed": (4400, 40, 200000, 60, 0),
+    "spill": (4400, 24, 2500, 40, 256),
+}
+SEEDS = {"direct": 11, "hashed": 12, "spill": 13}
+
+
+@pytest.fixture(scope="module")
+def wide_cases():
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            n_ref, n_qry, pool_n, mean, slots = SHAPES[name]
+            refs, qrys = _wide(SEEDS[name], n_ref, n_qry, pool_n, mean)
+            rk, ri = _block(refs)
+            qk, qi = _block(qrys)
+            cache[name] = (rk, ri, qk, qi, kt.build_index_np(rk, ri, OBJ_BITS), slots)
+        return cache[name]
+
+    return get
+
+
+@pytest.mark.parametrize("shape", list(SHAPES))
+@pytest.mark.parametrize("flags", [0, 2, 4, 6])
+def test_query_major_dense_matches_oracle(hip, oracle, wide_cases, shape, flags):
+    rk, ri, qk, qi, rec, slots = wide_cases(shape)
+    R = len(ri) - 1
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=3, query_slots=slots, query_threads=256)
+        got = hip.count_block(h, R, qk, qi, flags=flags)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, R, OBJ_BITS, qk, qi, flags=flags)
+    for name, g, w in zip(("ctx", "obj", "unique_ctx", "unique_obj"), got, want):
+        assert np.array_equal(g, w), (shape, flags, name)
+    if shape == "spill":  # the shape must really overflow the table
+        assert (got[0][:, :, 0] > 0).sum(1).max() > 256
+
+
+@pytest.mark.parametrize("shape", ["direct", "spill"])
+def test_query_major_lower_blocks(hip, oracle, wide_cases, shape):
+    rk, ri, _, _, rec, slots = wide_cases(shape)
+    R = len(ri) - 1
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=3, query_slots=slots)
+        for off in (0, 7, R - 9):
+            end = min(off + 9, R)
+            got = hip.count_block(h, R, rk[ri[off]:ri[end]], ri[off:end + 1], qry_gid_offset=off, flags=1 | 4)
+            want = oracle.count_block(rec, R, OBJ_BITS, rk[ri[off]:ri[end]], ri[off:end + 1],
+                                      qry_gid_offset=off, flags=1 | 4)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), off
+    finally:
+        hip.close_index(h)
+
+
+def test_query_major_accumulate_and_auto(hip, oracle, wide_cases):
+    """ACCUMULATE adds into the outputs (twice the counts after two calls);
+    a plain call overwrites garbage in the outputs; auto mode agrees."""
+    rt = kt.Hip(0)
+    rk, ri, qk, qi, rec, _ = wide_cases("hashed")
+    R, Q = len(ri) - 1, len(qi) - 1
+    h = hip.open_index(rec, R, OBJ_BITS)
+    s = rt.stream()
+    d_k, d_i = rt.upload(qk), rt.upload(qi)
+    cells = Q * R
+    d_c, d_o = rt.malloc(8 * cells), rt.malloc(8 * cells)
+    try:
+        want = oracle.count_block(rec, R, OBJ_BITS, qk, qi)
+        hip.set_tiling(h, mode=3)
+        rt.check(rt.lib.hipMemsetAsync(d_c, 0, 8 * cells, s))
+        rt.check(rt.lib.hipMemsetAsync(d_o, 0, 8 * cells, s))
+        for _ in range(2):
+            hip.count_block_dev(h, d_k, d_i, Q, int(qi[-1]), 0, kt.ACCUMULATE, d_c, d_o, None, None, s)
+        rt.sync(s)
+        c = rt.download(d_c, np.uint32, 2 * cells).reshape(Q, R, 2)
+        o = rt.download(d_o, np.uint32, 2 * cells).reshape(Q, R, 2)
+        assert np.array_equal(c, 2 * want[0]) and np.array_equal(o, 2 * want[1])
+        hip.set_tiling(h, mode=0)
+        rt.check(rt.lib.hipMemsetAsync(d_c, 0x7F, 8 * cells, s))  # garbage: the kernel must overwrite
+        hip.count_block_dev(h, d_k, d_i, Q, int(qi[-1]), 0, 0, d_c, d_o, None, None, s)
+        rt.sync(s)
+        assert np.array_equal(rt.download(d_c, np.uint32, 2 * cells).reshape(Q, R, 2), want[0])
+    finally:
+        for d in (d_k, d_i, d_c, d_o):
+            rt.free(d)
+        hip.close_index(h)
+
+
+def _ctx_totals(keys, idx):
+    return np.diff(idx).astype(np.uint32)
+
+
+def _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, ctxcut, afcut, anicut, slots=0):
+    qctx = _ctx_totals(qk, qi)
+    rctx = np.maximum(np.bincount((rec["ctxgid"] & np.uint64((1 << 20) - 1)).astype(np.int64),
+                                  minlength=R).astype(np.uint32), 0)
+    rctx[::7] += 3  # totals need not equal the index's entries
+    opts = kt.SelectOpts(ctxcut, afcut, anicut, int(unique_rank),
+                         kt.EpilogueOpts(3, 1, 0, 0, 0, 0, 8, OBJ_BITS, 44, 0.5, 1.0))
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        hip.set_tiling(h, query_slots=slots)
+        rows = hip.select_block(h, qk, qi, qctx, rctx, opts, flags=flags)
+    finally:
+        hip.close_index(h)
+    ctx, obj, uctx, uobj = oracle.count_block(rec, R, OBJ_BITS, qk, qi, flags=flags)
+    ranked = oracle.index_rank(ctx, obj, uctx, uobj, rctx, qctx, ctxcut=ctxcut, afcut=afcut, anicut=anicut,
+                               unique_ctx=unique_rank)
+    ofs = 0
+    for q, (pass_ids, rank, _) in enumerate(ranked):
+        mine = rows[rows["query"] == q]
+        if unique_rank:
+            # the device keeps pairs up to 1e-9 below the cut; the oracle's set
+            # must be contained, and the extras must lie in that margin
+            assert set(pass_ids.tolist()) <= set(mine["ref"].tolist()), q
+            extra = ~np.isin(mine["ref"], pass_ids)
+            assert np.all(mine["rank_ani"][extra] >= np.float64(np.float32(anicut)) - 1e-9)
+            keep = ~extra
+            mine = mine[keep]
+        assert np.array_equal(mine["ref"], pass_ids), q
+        r = mine["ref"].astype(np.int64)
+        assert np.array_equal(mine["num_ctx"], ctx[q, r, 0]) and np.array_equal(mine["num_mut2_ctx"], ctx[q, r, 1])
+        assert np.array_equal(mine["diff_obj"], obj[q, r, 0])
+        assert np.array_equal(mine["diff_obj_section"], obj[q, r, 1])
+        if flags & kt.UNIQUE:
+            assert np.array_equal(mine["u_num_ctx"], uctx[q, r, 0])
+            assert np.array_equal(mine["u_diff_obj_section"], uobj[q, r, 1])
+        if unique_rank:
+            assert np.allclose(mine["rank_ani"], rank, rtol=1e-12, atol=0)
+        else:
+            assert np.array_equal(mine["rank_ani"], rank), q
+        ofs += len(mine)
+    # values: the FP64 CtxMoE value of each kept pair
+    if len(rows):
+        qsel, rsel = rows["query"].astype(np.int64), rows["ref"].astype(np.int64)
+        out = np.zeros((len(qi) - 1) * R, np.float64)
+        oracle.lib.orc_matrix_values(kt._p(ctx), kt._p(obj), len(qi) - 1, R, kt._p(qctx), kt._p(rctx), 3, 1,
+                                     0.5, 1.0, 8, OBJ_BITS, 44, kt._p(out), 4)
+        want = out.reshape(-1, R)[qsel, rsel]
+        rel = np.abs(rows["value"] - want) / np.maximum(np.abs(want), 1e-300)
+        assert float(rel.max()) <= 1e-12
+    return rows
+
+
+@pytest.mark.parametrize("shape", list(SHAPES))
+@pytest.mark.parametrize("flags,unique_rank", [(0, False), (4, False), (2, True), (6, True), (2, False)])
+def test_select_matches_oracle_gate(hip, oracle, wide_cases, shape, flags, unique_rank):
+    rk, ri, qk, qi, rec, slots = wide_cases(shape)
+    R = len(ri) - 1
+    rows = _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, 1, 0.0, 0.0, slots)
+    assert len(rows) > 0
+
+
+@pytest.mark.parametrize("ctxcut,afcut,anicut", [(3, 0.0, 0.0), (1, 0.05, 0.0), (2, 0.0, 0.93), (1, 0.02, 0.97)])
+def test_select_gate_cuts(hip, oracle, wide_cases, ctxcut, afcut, anicut):
+    rk, ri, qk, qi, rec, slots = wide_cases("direct")
+    _check_select(hip, oracle, rec, len(ri) - 1, qk, qi, 2, True, ctxcut, afcut, anicut)
+    _check_select(hip, oracle, rec, len(ri) - 1, qk, qi, 0, False, ctxcut, afcut, anicut)
+
+
+def test_select_capacity_and_empty(hip, oracle, wide_cases):
+    """A buffer too small reports the count needed (KSSDHIP_ENOSPC) and the
+    binding retries; an all-empty block passes nothing; ctxcut 0 is refused."""
+    rk, ri, qk, qi, rec, _ = wide_cases("direct")
+    R = len(ri) - 1
+    qctx, rctx = _ctx_totals(qk, qi), np.diff(ri).astype(np.uint32)
+    opts = kt.SelectOpts(1, 0.0, 0.0, 0, kt.EpilogueOpts(3, 1, 0, 0, 0, 0, 8, OBJ_BITS, 44, 0.5, 1.0))
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        full = hip.select_block(h, qk, qi, qctx, rctx, opts)
+        small = hip.select_block(h, qk, qi, qctx, rctx, opts, cap=3)
+        assert len(full) > 3 and np.array_equal(full, small)
+        none = hip.select_block(h, np.zeros(0, np.uint64), np.zeros(4, np.uint64), np.zeros(3, np.uint32), rctx,
+                                opts)
+        assert len(none) == 0
+        bad = kt.SelectOpts(0, 0.0, 0.0, 0, opts.value)
+        with pytest.raises(RuntimeError):
+            hip.select_block(h, qk, qi, qctx, rctx, bad)
+    finally:
+        hip.close_index(h)
+
+
+def test_select_graph_capture(hip, oracle, wide_cases):
+    """Capturing the select before its workspace exists is refused (no
+    allocation inside a capture); after one eager call every replay of the
+    captured call equals the eager rows, with eager work in between."""
+    rt = kt.Hip(0)
+    rk, ri, qk, qi, rec, _ = wide_cases("spill")
+    R, Q = len(ri) - 1, len(qi) - 1
+    qctx, rctx = _ctx_totals(qk, qi), np.diff(ri).astype(np.uint32)
+    opts = kt.SelectOpts(1, 0.0, 0.0, 0, kt.EpilogueOpts(3, 1, 0, 0, 0, 0, 8, OBJ_BITS, 44, 0.5, 1.0))
+    h = hip.open_index(rec, R, OBJ_BITS)
+    s = rt.stream()
+    cap = Q * R
+    bufs = [rt.upload(qk), rt.upload(qi), rt.upload(qctx), rt.upload(rctx), rt.malloc(56 * cap), rt.malloc(4),
+            rt.malloc(4 * Q), rt.malloc(4 * Q)]
+    d_k, d_i, d_qc, d_rc, d_rows, d_n, d_first, d_cnt = bufs
+    try:
+        hip.set_tiling(h, query_slots=256)
+
+        def call():
+            hip.select_block_dev(h, d_k, d_i, Q, int(qi[-1]), 0, d_qc, d_rc, opts, d_rows, cap, d_n, d_first, d_cnt,
+                                 s)
+
+        def rows_now():
+            rt.sync(s)
+            n = int(rt.download(d_n, np.uint32, 1)[0])
+            r = rt.download(d_rows, np.uint8, 56 * n).view(kt.ROW_DTYPE).copy()
+            return np.sort(r, order=["query", "ref"])
+
+        with pytest.raises(RuntimeError, match="captured"):
+            rt.capture(s, call)
+        call()
+        eager = rows_now()
+        assert len(eager) > 0
+        g, ex = rt.capture(s, call)
+        try:
+            for rep in range(3):
+                rt.check(rt.lib.hipMemsetAsync(d_rows, 0x33 + rep, 56 * cap, s))
+                hip.count_block(h, R, qk[: int(qi[3])], qi[:4])  # eager work on the handle between replays
+                rt.graph_launch(ex, s)
+                assert np.array_equal(rows_now(), eager), rep
+        finally:
+            rt.graph_destroy(g, ex)
+    finally:
+        for b in bufs:
+            rt.free(b)
+        hip.close_index(h)
