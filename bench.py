@@ -1,21 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark: ANI pairs/s of the batched ref x query comparison on one MI355X.
+"""Benchmark: ANI pairs/s of the sketch comparison on one MI355X.
 
-Workload (batched ref x query with uneven sizes): 1,000 reference and 1,000
-query genomes, lengths log-normal (median 4 Mbp, sigma 0.5) clipped to
-1-12 Mbp, uniform ACGT.  Each query copies a random reference and carries
-0.1-10% substitutions plus indels touching 1% of its length (events of
-1-50 bp).  Everything is sketched at -f8 with the coden pattern by the
-package's sketcher (bit-exact with the reference's), the references are
-turned into the inverted index, and all 1M pairs are counted and given an
-FP64 ANI (CtxMoE, `--format matrix --values ani -s3` semantics) per step.
+Headline workload (C3): a 10,000-genome reference index (500 clades of 20;
+each clade a random ancestor, lognormal length median 4 Mbp, sigma 0.5,
+clipped to 1-12 Mbp, whose members carry U(1%, 15%) substitutions) against
+100,000 distinct query genomes (new members of uniformly chosen clades),
+everything sketched at -f8 with the coden pattern by the package's sketcher
+(bit-exact with the reference's).  One step = one query-major select over
+all 100,000 queries with inputs resident in HBM: every one of the 1e9
+(query, reference) pairs is counted and the index path's pass gate
+(ctxcut 3, afcut 0.5, anicut 0.95: the reference's defaults) is applied on
+the device; each passing pair is emitted with its counts and FP64 CtxMoE
+ANI.  Beside it: the HIP-event duration of the launch against the HBM
+roofline, the CPU oracle (a C/OpenMP restatement of the reference's block
+loop and gate) on the first 4,096 queries, and bit-exact parity of the
+passing pairs on that sample.
 
-A step = zero the dense count block + index-join over all query keys + FP64
-epilogue, with inputs resident in HBM.  Also reported: the plain
-one-pair-per-launch kernel on a sample of pairs, per-query latency of the
-batched path, the HIP-event duration of the join kernel against the HBM
-roofline, and the CPU oracle (a C/OpenMP restatement of the reference's
-block loop) timed on the host cores on a bounded sample.
+Side figures in the same line: C2 (1,000 x 1,000 genomes, dense output,
+partitioned join) with the plain one-pair-per-launch kernel and per-query
+latency; dense C3 blocks and C4-like read sets with unique-owner counters;
+the C3 queries streamed from host memory (PCIe-inclusive); the serving loop
+under Poisson load on a 20,000-genome index.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -354,11 +359,209 @@ def readset_bench(args, host, threads):
     return out
 
 
-def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
-    """Large resident index: args.index_refs genomes (clades of 20, as above),
-    built into the inverted index; two query sets counted in one batched
-    launch each, with inputs resident in HBM:
-      assemblies  args.index_queries new members of random clades (one
+def clade_queries(rng, n, nclade, cseed, clen, seed):
+    """n new members of uniformly chosen clades: the clade's ancestor with
+    U(1%, 15%) substitutions of its own."""
+    clades = rng.integers(0, nclade, size=n)
+    divs = rng.uniform(0.01, 0.15, size=n)
+    return [kh.SimSpec(cseed[c], int(clen[c]), int(seed * 4_000_037 + i + 1), float(divs[i]), 0.0, 0, 0.0)
+            for i, c in enumerate(clades)]
+
+
+def simsketch_chunked(host, specs, threads, chunk=10000):
+    keys, sizes = [], []
+    for b in range(0, len(specs), chunk):
+        k, idx = host.simsketch(specs[b:b + chunk], threads=threads)
+        keys.append(k)
+        sizes.append(np.diff(idx))
+    qi = np.zeros(len(specs) + 1, np.uint64)
+    qi[1:] = np.cumsum(np.concatenate(sizes)) if sizes else []
+    out = np.concatenate(keys) if keys else np.zeros(0, np.uint64)
+    return out, qi
+
+
+def measured_profile(name):
+    """Figures of a committed rocprofv3 summary (profiles/<name>.md):
+    key=value lines; (dict or None, file name)."""
+    import re
+    f = ROOT / "profiles" / f"{name}.md"
+    if not f.exists():
+        return None, None
+    vals = {k: float(v) for k, v in re.findall(r"^(\w+)=([0-9.eE+-]+)\s*$", f.read_text(), re.M)}
+    return vals, f"profiles/{f.name}"
+
+
+def c3_bench(args, host, threads):
+    """The headline workload: a 10,000-genome clade index (500 clades of 20)
+    against 100,000 distinct query genomes (new members of random clades),
+    all sketched at -f8 by the package's sketcher; inputs resident in HBM.
+    A step = one kssdhip_select_block_dev over all 100,000 queries: the
+    query-major join counts every (query, reference) pair (1e9 pairs) and
+    applies the index path's pass gate with the reference's defaults
+    (ctxcut 3, afcut 0.5, anicut 0.95), emitting each passing pair with its
+    counts and FP64 CtxMoE ANI."""
+    rng = np.random.default_rng(args.seed + 200)
+    nref = args.index_refs - args.index_refs % 20
+    nclade = nref // 20
+    specs, cseed, clen = clade_specs(rng, nref, args.seed + 200)
+    t = time.time()
+    rk, ri = host.simsketch(specs, threads=threads)
+    t_ref = time.time() - t
+    qspecs = clade_queries(rng, args.c3_queries, nclade, cseed, clen, args.seed)
+    t = time.time()
+    qk, qi = simsketch_chunked(host, qspecs, threads)
+    t_qry = time.time() - t
+    nq, total = len(qi) - 1, int(qi[-1])
+    log(f"C3: {nref} reference genomes sketched in {t_ref:.1f}s ({len(rk)} entries); {nq} queries in {t_qry:.1f}s "
+        f"({total} entries, {total * 8 / 1e9:.1f} GB)")
+    hip = kh.KssdHip(0)
+    rt = kh.Hip(0)
+    rec, build = index_build_bench(host, hip, rk, ri, threads)
+    rctx = host.ctx_counts(rk, ri, 20)
+    qctx = host.ctx_counts(qk, qi, 20)
+    h = hip.open_index(rec, nref, 20)
+    stream = rt.stream()
+    d_k, d_i = rt.upload(qk), rt.upload(qi)
+    d_qc, d_rc = rt.upload(qctx), rt.upload(rctx)
+    cap = 1 << 24
+    d_rows, d_n = rt.malloc(56 * cap), rt.malloc(4)
+    d_first, d_cnt = rt.malloc(4 * nq), rt.malloc(4 * nq)
+    value = kh.EpilogueOpts(3, 1, 0, 1, 0, 0, 8, 20, 44, 0.5, 1.0)  # CtxMoE ANI, float AFs (index path)
+    sopts = kh.SelectOpts(3, 0.5, 0.95, 0, value)
+
+    def step():
+        hip.select_block_dev(h, d_k, d_i, nq, total, 0, d_qc, d_rc, sopts, d_rows, cap, d_n, d_first, d_cnt,
+                             stream)
+
+    step()  # sizes the workspace before any capture
+    rt.sync(stream)
+    for _ in range(args.warmup):
+        step()
+    rt.sync(stream)
+    evs = [(rt.event(), rt.event()) for _ in range(args.steps)]
+    e0, e1 = rt.event(), rt.event()
+    t0 = time.perf_counter()
+    rt.record(e0, stream)
+    for a_, b_ in evs:
+        rt.record(a_, stream)
+        step()
+        rt.record(b_, stream)
+    rt.record(e1, stream)
+    rt.sync(stream)
+    wall = time.perf_counter() - t0
+    eager_ms = rt.elapsed_ms(e0, e1) / args.steps
+    launch_ms = [rt.elapsed_ms(a_, b_) for a_, b_ in evs]
+    n_rows = int(rt.download(d_n, np.uint32, 1)[0])
+    rows = rt.download(d_rows, np.uint8, 56 * min(n_rows, cap)).view(kh.ROW_DTYPE).copy()
+    q_cnt = rt.download(d_cnt, np.uint32, nq)
+    # the same step as a HIP graph
+    g, gx = rt.capture(stream, step)
+    for _ in range(args.warmup):
+        rt.graph_launch(gx, stream)
+    rt.sync(stream)
+    rt.record(e0, stream)
+    for _ in range(args.steps):
+        rt.graph_launch(gx, stream)
+    rt.record(e1, stream)
+    rt.sync(stream)
+    graph_ms = rt.elapsed_ms(e0, e1) / args.steps
+    g_rows = rt.download(d_rows, np.uint8, 56 * min(n_rows, cap)).view(kh.ROW_DTYPE)
+    graph_ok = int(rt.download(d_n, np.uint32, 1)[0]) == n_rows and np.array_equal(
+        np.sort(g_rows, order=["query", "ref"]), np.sort(rows, order=["query", "ref"]))
+    rt.graph_destroy(g, gx)
+    pairs = nq * nref
+    ms = graph_ms if graph_ok else eager_ms
+    log(f"C3 step: {eager_ms:.2f} ms eager, {graph_ms:.2f} ms graph (agrees {graph_ok}), "
+        f"{pairs / (ms / 1e3):.3e} pairs/s; {n_rows} passing pairs")
+    props = {"rows_fit": n_rows <= cap, "segments_sum": int(q_cnt.sum()) == n_rows,
+             "rows_pass_ctxcut": bool((rows["num_ctx"] >= 3).all()),
+             "rows_pass_anicut": bool((rows["rank_ani"] >= np.float64(np.float32(0.95))).all())}
+    # dense counts of one reference-sized block by both joins agree cell for cell
+    nb = min(1024, nq)
+    d_c1, d_o1, d_c2, d_o2 = (rt.malloc(8 * nb * nref) for _ in range(4))
+    hip.set_tiling(h, mode=3)
+    hip.count_block_dev(h, d_k, d_i, nb, int(qi[nb]), 0, 0, d_c1, d_o1, None, None, stream)
+    hip.set_tiling(h, mode=2, chunk_records=5696, block_threads=1024, group_partitions=4)
+    t_p = rt.event(), rt.event()
+    rt.record(t_p[0], stream)
+    hip.count_block_dev(h, d_k, d_i, nb, int(qi[nb]), 0, 0, d_c2, d_o2, None, None, stream)
+    rt.record(t_p[1], stream)
+    rt.sync(stream)
+    props["query_major_equals_partitioned_on_a_block"] = bool(
+        np.array_equal(rt.download(d_c1, np.uint64, nb * nref), rt.download(d_c2, np.uint64, nb * nref))
+        and np.array_equal(rt.download(d_o1, np.uint64, nb * nref), rt.download(d_o2, np.uint64, nb * nref)))
+    hip.set_tiling(h, mode=0)
+    for d in (d_c1, d_o1, d_c2, d_o2):
+        rt.free(d)
+    # roofline: compulsory bytes of the launch, B = 8 N_Q + 12 M + 16 P_out
+    M = hip.matched_records(h, d_k, d_i, nq, total)
+    algo = 8 * total + 12 * M + 16 * n_rows
+    prof, prof_src = measured_profile("qjoin_c3_10k_x_100k")
+    kern_ms = prof.get("k_qjoin_avg_ms") if prof else None
+    t_launch = float(np.mean(launch_ms)) / 1e3
+    traffic = int(prof["traffic_bytes_per_launch"]) if prof and "traffic_bytes_per_launch" in prof else None
+    roof = {"bound": "hbm", "achieved": algo / t_launch / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": algo / t_launch / 1e9 / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": prof_src,
+            "traffic_GBs": traffic / t_launch / 1e9 if traffic else None, "kernel": "k_qjoin (select)",
+            "algorithmic_bytes": algo, "bytes_model": "8*N_Q + 12*M + 16*P_out", "N_Q": total, "M": M,
+            "P_out": n_rows, "index_records": int(len(rec)), "avg_launch_ms": t_launch * 1e3,
+            "avg_launch_what": "HIP events around kssdhip_select_block_dev on its stream (order + k_qjoin + "
+                               "row values)",
+            "k_qjoin_avg_ms_rocprof": kern_ms}
+    # CPU baseline (the oracle's block count with per-block fenceposts and
+    # guided OpenMP, then its restatement of the gate) on the first block
+    sys.path.insert(0, str(ROOT / "tests"))
+    import kssdtest as kt
+    orc = kt.Oracle()
+    ncpu = min(args.c3_cpu_queries, nq)
+    c_ctx = np.zeros((ncpu, nref, 2), np.uint32)
+    c_obj = np.zeros((ncpu, nref, 2), np.uint32)
+    sub = np.ascontiguousarray(qk[: int(qi[ncpu])])
+    t = time.perf_counter()
+    orc.lib.orc_count_block(kt._p(rec), len(rec), nref, 20, kt._p(sub), kt._p(qi[: ncpu + 1]), ncpu, 0, 0,
+                            kt._p(c_ctx), kt._p(c_obj), None, None, threads)
+    ranked = orc.index_rank(c_ctx, c_obj, c_ctx, c_obj, rctx, qctx[:ncpu], ctxcut=3, afcut=0.5, anicut=0.95)
+    cpu_s = time.perf_counter() - t
+    mine = np.sort(rows[rows["query"] < ncpu], order=["query", "ref"])
+    want_q = np.concatenate([np.full(len(p[0]), q, np.uint32) for q, p in enumerate(ranked)])
+    want_r = np.concatenate([p[0] for p in ranked])
+    exact = bool(np.array_equal(mine["query"], want_q) and np.array_equal(mine["ref"], want_r))
+    if exact and len(mine):
+        qq, rr = mine["query"].astype(np.int64), mine["ref"].astype(np.int64)
+        exact = bool(np.array_equal(mine["num_ctx"], c_ctx[qq, rr, 0])
+                     and np.array_equal(mine["num_mut2_ctx"], c_ctx[qq, rr, 1])
+                     and np.array_equal(mine["diff_obj"], c_obj[qq, rr, 0])
+                     and np.array_equal(mine["diff_obj_section"], c_obj[qq, rr, 1])
+                     and np.array_equal(mine["rank_ani"], np.concatenate([p[1] for p in ranked])))
+    cpu_pps = ncpu * nref / cpu_s
+    log(f"C3 CPU oracle: {ncpu} queries in {cpu_s:.2f}s = {cpu_pps:.3e} pairs/s; passing pairs equal: {exact} "
+        f"({len(mine)} pairs)")
+    res = {
+        "value": pairs / (ms / 1e3), "ms_per_step": ms, "pairs": pairs, "queries": nq, "refs": nref,
+        "query_entries": total, "index_records": int(len(rec)), "passing_pairs": n_rows,
+        "step": {"timed": "HIP graph of one kssdhip_select_block_dev over all queries" if graph_ok
+                 else "eager launches", "graph_ms": graph_ms, "eager_ms": eager_ms, "graph_agrees_with_eager": graph_ok,
+                 "wall_ms_per_step_eager": wall / args.steps * 1e3},
+        "gate": {"ctxcut": 3, "afcut": 0.5, "anicut": 0.95, "value": "CtxMoE ANI (FP64, float AFs)"},
+        "parity": {"cpu_sample_queries": ncpu, "passing_pairs_bit_exact": exact, "properties": props},
+        "partitioned_join_ms_per_1024_queries": rt.elapsed_ms(*t_p),
+        "roofline": roof,
+        "cpu_baseline": {"value": cpu_pps, "unit": "pairs/s", "cores": threads, "kind": "port",
+                         "sample": f"first {ncpu} queries x {nref} refs: oracle block count (fenceposts per block, "
+                                   "guided OpenMP) + its restatement of the pass gate (command_ani.h:577-621)"},
+        "sketch_s": {"refs": t_ref, "queries": t_qry}, "index_build": build,
+    }
+    for d in (d_k, d_i, d_qc, d_rc, d_rows, d_n, d_first, d_cnt):
+        rt.free(d)
+    shared = {"hip": hip, "rt": rt, "h": h, "rec": rec, "rctx": rctx, "specs": specs, "qk": qk, "qi": qi,
+              "nref": nref, "cseed": cseed, "clen": clen}
+    return res, shared
+
+
+def index_bench(args, host, threads, plain_s_per_pair, shared):
+    """The C3 index (resident) against two query sets counted in one batched
+    launch each into dense blocks, with inputs resident in HBM:
+      assemblies  the first args.index_queries of the C3 queries (one
                   reference-style block), dense counts + FP64 CtxMoE ANI;
       read sets   args.index_readsets samples (a reference + 0.5%
                   substitutions, read at U(10, 30)x, see readset_like),
@@ -366,27 +569,12 @@ def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
     Beside each: the oracle (C/OpenMP restatement of the reference's block
     loop) on a bounded prefix of the same queries, and the plain kernel's
     measured per-pair time."""
-    rng = np.random.default_rng(args.seed + 200)
-    nref = args.index_refs - args.index_refs % 20
-    specs, cseed, clen = clade_specs(rng, nref, args.seed + 200)
-    t = time.time()
-    rk, ri = host.simsketch(specs, threads=threads)
-    t_sk = time.time() - t
-    rec, build = index_build_bench(host, hip, rk, ri, threads)
-    host_build_s = build["host_ms"] / 1e3
-    gpu_build_s = build["gpu_ms_host_buffers"] / 1e3
-    rctx = host.ctx_counts(rk, ri, 20)
-    del rk
-    log(f"index bench: {nref} genomes, {len(rec)} records; sketched in {t_sk:.1f}s; index on the host "
-        f"{host_build_s * 1e3:.0f} ms, on the GPU {gpu_build_s * 1e3:.0f} ms (identical: {build['records_identical']})")
-    nclade = nref // 20
-    nq = args.index_queries
-    qspecs = []
-    for i in range(nq):
-        c = int(rng.integers(0, nclade))
-        qspecs.append(kh.SimSpec(cseed[c], int(clen[c]), int(args.seed * 4_000_037 + i + 1),
-                                 float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0))
-    qk, qi = host.simsketch(qspecs, threads=threads)
+    rng = np.random.default_rng(args.seed + 300)
+    hip, rt, h, rec, rctx, specs = (shared[k] for k in ("hip", "rt", "h", "rec", "rctx", "specs"))
+    nref = shared["nref"]
+    nq = min(args.index_queries, len(shared["qi"]) - 1)
+    qi = shared["qi"][: nq + 1].copy()
+    qk = shared["qk"][: int(qi[-1])]
     nrs = args.index_readsets
     rs = []
     for i in range(nrs):
@@ -399,9 +587,8 @@ def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
     rkeys = np.concatenate(reads)
     ridx = np.zeros(nrs + 1, np.uint64)
     ridx[1:] = np.cumsum([len(x) for x in reads])
-    h = hip.open_index(rec, nref, 20)
     stream = rt.stream()
-    out = {"index_genomes": nref, "index_records": int(len(rec)), "index_build": build}
+    out = {"index_genomes": nref, "index_records": int(len(rec))}
     orc = None
     for name, keys, idx, flags in (("assemblies", qk, qi, 0), ("read_sets", rkeys, ridx, kh.UNIQUE)):
         n = len(idx) - 1
@@ -474,112 +661,100 @@ def index_bench(args, host, hip, rt, threads, plain_s_per_pair):
         for d in (d_k, d_i, d_qc, d_rc, d_c, d_o, d_out, d_uc, d_uo):
             if d:
                 rt.free(d)
-        if not unique and args.index_stream_queries:
-            hip.set_tiling(h, **tiling)
-            out["streamed"] = stream_bench(args, host, hip, rt, h, qk, qi, nref, rctx)
-            hip.set_tiling(h, mode=0)
-    hip.close_index(h)
+        hip.set_tiling(h, mode=0)
+    if args.index_stream_queries:
+        out["streamed"] = stream_bench(args, host, hip, rt, h, shared["qk"], shared["qi"], nref, rctx)
     out["cpu_threads"] = threads
     return out
 
 
 def stream_bench(args, host, hip, rt, h, qk, qi, nref, rctx):
-    """The resident-index workload at its full query count: args.index_stream_queries
-    queries against the index, in blocks of the sketched query set (reused
-    cyclically: every block is a prefix of it), streamed from pinned host
-    memory.  Per block: host-to-device copy of the keys on a copy stream,
-    double-buffered against the count + FP64 CtxMoE epilogue + anicut hit
-    selection (one HIP graph per buffer and block size) on the compute stream,
-    and the hits copied back.  Timed from the first copy to the last hits on
-    the host; the hits of the first block are checked against the dense
-    values of an eager run."""
-    nq_set = len(qi) - 1
-    total_q = args.index_stream_queries
-    anicut = 0.95
-    blocks = [min(nq_set, total_q - b) for b in range(0, total_q, nq_set)]
-    nbytes = int(qi[nq_set]) * 8
-    pin = rt.host_malloc(nbytes)
-    ctypes.memmove(pin, qk.ctypes.data, nbytes)
-    d_keys = [rt.malloc(nbytes), rt.malloc(nbytes)]
-    d_i = rt.upload(qi)
-    qctx = host.ctx_counts(qk, qi, 20)
-    d_qc, d_rc = rt.upload(qctx), rt.upload(rctx)
-    cells = nq_set * nref
-    d_c, d_o, d_out = rt.malloc(8 * cells), rt.malloc(8 * cells), rt.malloc(8 * cells)
+    """The C3 queries with their keys in host memory (PCIe-inclusive): in
+    reference-sized blocks of 4,096 queries, each block's keys go over PCIe
+    from pinned memory on a copy stream, double-buffered against the select
+    (count + pass gate + FP64 values, graph-captured per buffer and block
+    size) on the compute stream; the passing pairs come back to the host.
+    Timed from the first copy to the last rows on the host."""
+    nq_all = min(args.index_stream_queries, len(qi) - 1)
+    bq = 4096
+    blocks = [(b, min(nq_all, b + bq)) for b in range(0, nq_all, bq)]
+    maxkeys = max(int(qi[e] - qi[b]) for b, e in blocks)
+    pin = rt.host_malloc(8 * int(qi[nq_all]))
+    ctypes.memmove(pin, qk.ctypes.data, 8 * int(qi[nq_all]))
+    qctx = host.ctx_counts(qk[: int(qi[nq_all])], qi[: nq_all + 1], 20)
+    d_keys = [rt.malloc(8 * maxkeys), rt.malloc(8 * maxkeys)]
+    d_idx = [rt.upload((qi[b:e + 1] - qi[b]).astype(np.uint64)) for b, e in blocks]
+    d_qc = [rt.upload(qctx[b:e]) for b, e in blocks]
+    d_rc = rt.upload(rctx)
     cap = 1 << 20
-    d_hits = [rt.malloc(16 * cap), rt.malloc(16 * cap)]
+    d_rows = [rt.malloc(56 * cap), rt.malloc(56 * cap)]
     d_n = [rt.malloc(4), rt.malloc(4)]
-    h_hits = [rt.host_malloc(16 * cap), rt.host_malloc(16 * cap)]
+    d_first, d_cnt = rt.malloc(4 * bq), rt.malloc(4 * bq)
+    h_rows = [rt.host_malloc(56 * cap), rt.host_malloc(56 * cap)]
     h_n = [rt.host_malloc(4), rt.host_malloc(4)]
-    opts = kh.EpilogueOpts(3, 1, 0, 0, 0, 0, 8, 20, 44, 0.5, 1.0)
+    value = kh.EpilogueOpts(3, 1, 0, 1, 0, 0, 8, 20, 44, 0.5, 1.0)
+    sopts = kh.SelectOpts(3, 0.5, 0.95, 0, value)
     cs, ks = rt.stream(), rt.stream()
     copied = [rt.event(), rt.event()]
     done = [rt.event(), rt.event()]
 
-    def compute(buf, n):
-        hip.count_block_dev(h, d_keys[buf], d_i, n, int(qi[n]), 0, 0, d_c, d_o, None, None, ks)
-        hip.epilogue_dev(d_c, d_o, n, nref, d_qc, d_rc, opts, d_out, ks)
-        hip.select_hits_dev(d_out, n, nref, anicut, cap, d_hits[buf], d_n[buf], ks)
+    def compute(buf, k):
+        b, e = blocks[k]
+        hip.select_block_dev(h, d_keys[buf], d_idx[k], e - b, int(qi[e] - qi[b]), 0, d_qc[k], d_rc, sopts,
+                             d_rows[buf], cap, d_n[buf], d_first, d_cnt, ks)
 
-    # eager reference for the first block, then one graph per (buffer, block size)
-    rt.copy_async(d_keys[0], pin, nbytes, rt.H2D, ks)
-    compute(0, blocks[0])
+    # eager warm-up sizes the workspace; graphs per (buffer, block)
+    rt.copy_async(d_keys[0], ctypes.c_void_p(pin + 8 * int(qi[0])), 8 * int(qi[blocks[0][1]]), rt.H2D, ks)
+    compute(0, 0)
     rt.sync(ks)
-    dense = rt.download(d_out, np.float64, blocks[0] * nref).reshape(blocks[0], nref)
-    n0 = int(rt.download(d_n[0], np.uint32, 1)[0])
-    first = rt.download(d_hits[0], np.uint8, 16 * min(n0, cap)).view(kh.HIT_DTYPE)
-    want = np.argwhere(dense >= anicut)
-    order = np.lexsort((first["ref"], first["query"]))
-    hits_ok = bool(n0 == len(want) and np.array_equal(
-        np.stack([first["query"][order], first["ref"][order]], 1), want))
     graphs = {}
-    for buf in (0, 1):
-        for n in sorted(set(blocks)):
-            graphs[(buf, n)] = rt.capture(ks, lambda: compute(buf, n))
+    for k in range(len(blocks)):
+        for buf in (0, 1):
+            graphs[(buf, k)] = rt.capture(ks, lambda: compute(buf, k))
     t0 = time.perf_counter()
-    total_hits = 0
-    overflow = False
-    for b, n in enumerate(blocks):
-        buf = b % 2
-        if b >= 2:
-            rt.wait_event(cs, done[buf])  # block b - 2 has finished with this buffer
-        rt.copy_async(d_keys[buf], pin, int(qi[n]) * 8, rt.H2D, cs)
+    total_rows, overflow = 0, False
+
+    def collect(k):
+        nonlocal total_rows, overflow
+        pb = k % 2
+        rt.lib.hipEventSynchronize(done[pb])
+        n = ctypes.cast(h_n[pb], ctypes.POINTER(ctypes.c_uint32))[0]
+        overflow |= n > cap
+        if n:
+            rt.copy_async(h_rows[pb], d_rows[pb], 56 * min(n, cap), rt.D2H, cs)
+        total_rows += n
+
+    for k, (b, e) in enumerate(blocks):
+        buf = k % 2
+        if k >= 2:
+            rt.wait_event(cs, done[buf])  # block k - 2 is done with this buffer
+        rt.copy_async(d_keys[buf], ctypes.c_void_p(pin + 8 * int(qi[b])), 8 * int(qi[e] - qi[b]), rt.H2D, cs)
         rt.record(copied[buf], cs)
         rt.wait_event(ks, copied[buf])
-        rt.graph_launch(graphs[(buf, n)][1], ks)
+        rt.graph_launch(graphs[(buf, k)][1], ks)
         rt.copy_async(h_n[buf], d_n[buf], 4, rt.D2H, ks)
         rt.record(done[buf], ks)
-        if b >= 1:  # collect block b - 1's hits while block b runs
-            pb = (b - 1) % 2
-            rt.lib.hipEventSynchronize(done[pb])
-            k = ctypes.cast(h_n[pb], ctypes.POINTER(ctypes.c_uint32))[0]
-            overflow |= k > cap
-            rt.copy_async(h_hits[pb], d_hits[pb], 16 * min(k, cap), rt.D2H, cs)
-            total_hits += k
-    rt.lib.hipEventSynchronize(done[(len(blocks) - 1) % 2])
-    k = ctypes.cast(h_n[(len(blocks) - 1) % 2], ctypes.POINTER(ctypes.c_uint32))[0]
-    rt.copy_async(h_hits[(len(blocks) - 1) % 2], d_hits[(len(blocks) - 1) % 2], 16 * min(k, cap), rt.D2H, cs)
-    total_hits += k
+        if k >= 1:  # block k - 1's rows while block k runs
+            collect(k - 1)
+    collect(len(blocks) - 1)
     rt.sync(cs)
     rt.sync(ks)
     wall = time.perf_counter() - t0
-    pairs = total_q * nref
-    key_bytes = sum(int(qi[n]) * 8 for n in blocks)
-    res = {"queries": total_q, "blocks": len(blocks), "block_queries": nq_set, "pairs": pairs,
-           "wall_s": wall, "pairs_per_s": pairs / wall, "h2d_bytes": key_bytes,
-           "h2d_GBs_sustained": key_bytes / wall / 1e9, "anicut": anicut, "hits": int(total_hits),
-           "hits_overflowed_cap": bool(overflow), "first_block_hits_match_dense": hits_ok,
-           "what": "PCIe-inclusive: keys streamed from pinned host memory, double-buffered; count + CtxMoE "
-                   "epilogue + anicut hit selection graph-captured per block; the query set is reused "
-                   "cyclically"}
-    log(f"streamed: {total_q} queries x {nref} refs in {len(blocks)} blocks, {wall:.2f} s = "
-        f"{pairs / wall:.3e} pairs/s incl. {key_bytes / 1e9:.1f} GB of H2D keys; {total_hits} hits >= {anicut}; "
-        f"first block hits match dense: {hits_ok}")
+    pairs = nq_all * nref
+    key_bytes = 8 * int(qi[nq_all])
+    res = {"queries": nq_all, "blocks": len(blocks), "block_queries": bq, "pairs": pairs, "wall_s": wall,
+           "pairs_per_s": pairs / wall, "h2d_bytes": key_bytes, "h2d_GBs_sustained": key_bytes / wall / 1e9,
+           "passing_pairs": int(total_rows), "rows_overflowed_cap": bool(overflow),
+           "what": "PCIe-inclusive: distinct queries' keys streamed from pinned host memory in blocks of 4096, "
+                   "double-buffered; select (count + pass gate + FP64 values) graph-captured per block; passing "
+                   "pairs copied back"}
+    log(f"streamed: {nq_all} queries x {nref} refs in {len(blocks)} blocks, {wall:.2f} s = {pairs / wall:.3e} "
+        f"pairs/s incl. {key_bytes / 1e9:.1f} GB of H2D keys; {total_rows} passing pairs")
     for g, ex in graphs.values():
         rt.graph_destroy(g, ex)
-    for p in (*d_keys, d_i, d_qc, d_rc, d_c, d_o, d_out, *d_hits, *d_n):
+    for p in (*d_keys, *d_idx, *d_qc, d_rc, *d_rows, *d_n, d_first, d_cnt):
         rt.free(p)
-    for p in (pin, *h_hits, *h_n):
+    for p in (pin, *h_rows, *h_n):
         rt.host_free(p)
     return res
 
@@ -743,7 +918,13 @@ def main():
     ap.add_argument("--index-queries", type=int, default=4096)
     ap.add_argument("--index-readsets", type=int, default=200)
     ap.add_argument("--index-stream-queries", type=int, default=100000,
-                    help="queries streamed against the resident index in blocks (0: skip)")
+                    help="C3 queries streamed from host memory against the resident index in blocks (0: skip)")
+    ap.add_argument("--c3-queries", type=int, default=100000,
+                    help="distinct query genomes of the headline workload (C3)")
+    ap.add_argument("--c3-cpu-queries", type=int, default=4096,
+                    help="C3 queries timed on the CPU oracle (the first reference block)")
+    ap.add_argument("--c3-only", action="store_true",
+                    help="run only the headline workload (for profiling runs)")
     ap.add_argument("--serve-refs", type=int, default=20000,
                     help="reference genomes of the serving index (1000 clades of 20); 0 skips serving")
     ap.add_argument("--serve-queries", type=int, default=2000)
@@ -772,6 +953,28 @@ def main():
         rk, ri = host.simsketch(specs, threads=threads)
         _, build = index_build_bench(host, kh.KssdHip(0), rk, ri, threads, reps=3)
         print(json.dumps({"index_build": build, "index_genomes": nref}), flush=True)
+        return
+    c3, shared = c3_bench(args, host, threads)
+    headline = {
+        "metric": "ANI pairs/sec on 1 MI355X, batched vs plain kernel; p50/p99 query latency",
+        "value": c3["value"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": c3["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 counts, f32 pass gate, f64 ANI",
+        "data": "synthetic genomes (seeded simulator) sketched at -f8 by the package sketcher",
+        "config": {"workload": f"C3: {c3['refs']}-genome clade index (500 clades of 20, lognormal median 4 Mbp, "
+                               f"members U(1%,15%) substitutions from the ancestor) x {c3['queries']} distinct query "
+                               "genomes (new members of random clades); every pair counted, pass gate ctxcut 3 / "
+                               "afcut 0.5 / anicut 0.95, passing pairs with FP64 CtxMoE ANI",
+                   "pairs": c3["pairs"], "queries": c3["queries"], "refs": c3["refs"],
+                   "query_entries": c3["query_entries"], "index_records": c3["index_records"],
+                   "output": "passing (query, ref) pairs: counts + ranking ANI + FP64 CtxMoE ANI"},
+        "roofline": c3["roofline"], "cpu_baseline": c3["cpu_baseline"], "gpu_over_cpu": c3["value"] / c3[
+            "cpu_baseline"]["value"],
+        "c3": c3,
+    }
+    if args.c3_only:
+        shared["hip"].close_index(shared["h"])
+        print(json.dumps(headline), flush=True)
         return
     rkeys, ridx, qkeys, qidx = make_workload(args.nref, args.nqry, args.seed, threads, host)
     obj_bits = 20
@@ -927,7 +1130,7 @@ def main():
 
     # ---------------- roofline of the join kernel
     M = matched_records(records, qkeys, obj_bits)
-    algo_bytes = 8 * total + 12 * M + 16 * p_touched
+    algo_bytes = 8 * total + 12 * M + 16 * cells  # dense output: P_out = Q x R
     join_s = float(np.mean(join_ms)) / 1e3
     achieved = algo_bytes / join_s / 1e9
     traffic, traffic_src = measured_traffic(tiling, nref, nqry)
@@ -947,26 +1150,20 @@ def main():
         f"= {cpu_pairs_per_s:.3e} pairs/s; counts bit-exact {counts_exact}, max rel ANI err {max_rel:.2e}")
 
     # ---------------- serving loop on a large resident index
-    big_index = index_bench(args, host, hip, rt, threads, plain_ms / npl / 1e3) if args.index_refs else None
+    big_index = index_bench(args, host, threads, plain_ms / npl / 1e3, shared)
+    shared["hip"].close_index(shared["h"])
+    del shared
     serving = serve_bench(args, host, hip, threads, plain_ms / npl / 1e3) if args.serve_refs else None
 
     eager_pairs_per_s = pairs_per_s
     if graph_ok:
         ms_per_step = graph_ms
         pairs_per_s = cells / (graph_ms / 1e3)
-    rec = {
-        "metric": "ANI pairs/sec on 1 MI355X, batched vs plain kernel; p50/p99 query latency",
+    c2 = {
+        "what": "C2: batched ref x query with dense output, side figure",
         "value": pairs_per_s,
         "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32 counts, f64 ANI",
-        "data": "synthetic genomes (seeded simulator) sketched at -f8 by the package sketcher",
         "config": {
             "workload": f"batched ref x query, {nref} refs x {nqry} queries, lognormal median 4 Mbp "
                         "(sigma 0.5, clip 1-12 Mbp), queries = ref + 0.1-10% subs + 1% indels, -f8",
@@ -990,8 +1187,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "traffic_GBs": traffic / join_s / 1e9 if traffic else None,
                      "kernel": "k_split+k_pjoin" if tiling["mode"] == 2 else "k_join", "algorithmic_bytes": algo_bytes,
-                     "bytes_model": "8*N_Q + 12*M + 16*P_touched",
-                     "N_Q": total, "M": M, "P_touched": p_touched,
+                     "bytes_model": "8*N_Q + 12*M + 16*P_out, P_out = Q*R (dense output)",
+                     "N_Q": total, "M": M, "P_out": cells, "P_touched": p_touched,
                      "avg_launch_ms": float(np.mean(join_ms))},
         "lds": {"events": H, "events_per_s": H / join_s,
                 "events_what": "(query key, index record) pairs sharing a context, each one object-mismatch "
@@ -1010,6 +1207,12 @@ def main():
         "index_10k": big_index,
         "serving": serving,
     }
+    rec = dict(headline)
+    rec["c2"] = c2
+    rec["batched_over_plain"] = c3["value"] / c2["plain_kernel"]["pairs_per_s"]
+    rec["latency_ms"] = c2["latency_ms"]
+    rec["index_10k"] = c2.pop("index_10k")
+    rec["serving"] = c2.pop("serving")
     hip.close_index(h)
     print(json.dumps(rec), flush=True)
 

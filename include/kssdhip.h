@@ -368,6 +368,16 @@ int kssdhip_select_block_dev(kssdhip_index *index, const uint64_t *d_keys, const
 			     uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, void *stream);
 
 /*
+ * Diagnostic: the number of index records whose context occurs in a
+ * device-resident block (relative offsets as kssdhip_count_block_dev),
+ * i.e. the records a join of the block must read at least once.  No
+ * reference counterpart; used to size work and to state the compulsory
+ * traffic of a block.  Synchronous (allocates and frees a byte per record).
+ */
+int kssdhip_index_matched_records(kssdhip_index *index, const uint64_t *d_keys, const uint64_t *d_skidx,
+				  int section_gnum, uint64_t total_keys, uint64_t *matched);
+
+/*
  * Serving loop (no reference counterpart: the reference answers one request
  * per process, stream_ref_sketches_one_qraw_lookup, command_ani.c:2490-2666).
  * A server keeps the index resident and answers query sketches submitted

@@ -115,13 +115,11 @@ struct kssdhip_index {
 	int qthreads = 0; // query-major workgroup size (0: default)
 	int qslots = 0;   // query-major LDS slots (0: default)
 	// query-major join (kssdhip_qjoin.hip): the records again as 16-byte
-	// {ctxgid, obj, 0} entries, the first record of each of 2^qk context
-	// buckets (bucket t = ctx >> qshift; qstart[2^qk] = n) and a 32-bit
-	// fingerprint mask of the contexts in each bucket
+	// {ctxgid, obj, 0} entries, and an open-addressed table of 2^ck slots,
+	// one per distinct context: {first record, 16-bit fingerprint, run length}
 	uint4 *d_qrec = nullptr;
-	uint32_t *d_qstart = nullptr;
-	uint32_t *d_qfp = nullptr;
-	int qk = 0, qshift = 0;
+	unsigned long long *d_ctab = nullptr;
+	int ck = 0;
 	int cus = 0; // compute units of the device
 	int qj_per_cu[2] = {0, 0}; // query-major workgroups per CU (plain, unique counters) at the tiling
 	// grow-only scratch: host-pointer entry points, the split table, the

@@ -54,12 +54,27 @@ constexpr int KB = QJ_KB; // keys per thread whose lookups are in flight togethe
 
 enum { QJ_DENSE = 0, QJ_ACCUM = 1, QJ_SELECT = 2 };
 
-// One bit of a bucket's fingerprint mask per context: the top 5 bits of a
-// multiplicative hash of the whole context (the bucket index uses only the
-// context's top bits).
-__host__ __device__ __forceinline__ uint32_t fp_bit(uint64_t c)
+// The context table: open addressing over 2^k 64-bit slots, one per
+// distinct index context c: bits 0..31 the first record of c's run, bits
+// 32..47 a 16-bit fingerprint of c, bits 48..63 the run's length (65535:
+// at least that many; the walk then stops where the context changes).  The
+// home slot is the top k bits of a 64-bit mix of c, the fingerprint its low
+// 16 bits, and collisions probe the following slots (load factor <= 1/2,
+// so a probe sequence rarely leaves its 64-byte segment).  A context absent
+// from the index costs one segment; a present one that segment and the
+// records of its run.
+constexpr unsigned long long SLOT_EMPTY = ~0ull;
+constexpr uint32_t RUN_CAP = 0xffffu;
+
+__host__ __device__ __forceinline__ uint64_t ctx_mix(uint64_t c)
 {
-	return 1u << (uint32_t)((c * 0x9E3779B97F4A7C15ull) >> 59);
+	// the murmur3 64-bit finalizer (a published bijective mixer)
+	c ^= c >> 33;
+	c *= 0xff51afd7ed558ccdull;
+	c ^= c >> 33;
+	c *= 0xc4ceb9fe1a85ec53ull;
+	c ^= c >> 33;
+	return c;
 }
 
 // 16-byte records {ctxgid low, ctxgid high, obj, 0}
@@ -72,37 +87,40 @@ __global__ void k_pack16(const uint64_t *__restrict__ ctxgid, const uint32_t *__
 	}
 }
 
-// start[t] = first record whose context >> shift >= t (t = 0 .. nb; nb = 2^k,
-// start[nb] = n)
-__global__ void k_qbkt_start(const uint64_t *__restrict__ ctxgid, uint64_t n, int shift, uint64_t nb, uint32_t *start)
+__device__ __forceinline__ bool run_start(const uint64_t *__restrict__ ctxgid, uint64_t i)
 {
-	for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t <= nb; t += (uint64_t)gridDim.x * blockDim.x) {
-		uint64_t lo = 0, hi = n;
-		if (t == nb) {
-			lo = n;
-		} else {
-			while (lo < hi) {
-				const uint64_t mid = lo + ((hi - lo) >> 1);
-				if ((ctxgid[mid] >> (GID_NBITS + shift)) < t)
-					lo = mid + 1;
-				else
-					hi = mid;
-			}
-		}
-		start[t] = (uint32_t)lo;
-	}
+	return i == 0 || (ctxgid[i] >> GID_NBITS) != (ctxgid[i - 1] >> GID_NBITS);
 }
 
-// fp[t] = OR of fp_bit over the contexts of bucket t's records
-__global__ void k_qbkt_fp(const uint64_t *__restrict__ ctxgid, uint64_t nb, const uint32_t *__restrict__ start,
-			  uint32_t *fp)
+// number of distinct contexts (record runs)
+__global__ __launch_bounds__(256) void k_count_runs(const uint64_t *__restrict__ ctxgid, uint64_t n,
+						    unsigned long long *count)
 {
-	for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < nb; t += (uint64_t)gridDim.x * blockDim.x) {
-		const uint32_t b = start[t], e = start[t + 1];
-		uint32_t m = 0;
-		for (uint32_t d = b; d < e; ++d)
-			m |= fp_bit(ctxgid[d] >> GID_NBITS);
-		fp[t] = m;
+	uint32_t k = 0;
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		k += run_start(ctxgid, i);
+	for (int off = 32; off > 0; off >>= 1)
+		k += __shfl_xor(k, off, 64);
+	if ((threadIdx.x & 63) == 0 && k)
+		atomicAdd(count, (unsigned long long)k);
+}
+
+__global__ void k_ctab_insert(const uint64_t *__restrict__ ctxgid, uint64_t n, int k, unsigned long long *tab)
+{
+	const uint64_t mask = (1ull << k) - 1;
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+		if (!run_start(ctxgid, i))
+			continue;
+		const uint64_t c = ctxgid[i] >> GID_NBITS;
+		uint64_t len = 1;
+		while (len < RUN_CAP && i + len < n && (ctxgid[i + len] >> GID_NBITS) == c)
+			++len;
+		const uint64_t h = ctx_mix(c);
+		const unsigned long long entry = (uint64_t)(uint32_t)i | ((h & 0xffffull) << 32) | (len << 48);
+		for (uint64_t s = h >> (64 - k);; s = (s + 1) & mask) {
+			if (atomicCAS(&tab[s], SLOT_EMPTY, entry) == SLOT_EMPTY)
+				break;
+		}
 	}
 }
 
@@ -153,10 +171,9 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t n)
 struct QJArgs {
 	// index
 	const uint4 *rec;
-	const uint32_t *qstart; // 2^k + 1 bucket starts
-	const uint32_t *qfp;    // 2^k fingerprint masks
+	const unsigned long long *ctab; // the context table, 2^ck slots
+	int ck;
 	uint64_t n;
-	int qshift;
 	uint64_t ctx_max;
 	uint32_t R; // ref_gnum
 	unsigned obj_bits;
@@ -251,61 +268,46 @@ __device__ __forceinline__ uint64_t rec_ctxgid(const uint4 &r)
 	return ((uint64_t)r.y << 32) | r.x;
 }
 
-// Four consecutive records held in registers, loaded together (one 64-byte
-// span), so that a bucket scan or a run walk waits for memory about once
-// per four records.  Only records below n are ever returned.  Accesses move
-// forward only; a window that starts empty has base = (first index) - 4, so
-// that i - base >= 4 (mod 2^32) reloads it on the first access.
-struct RecWin {
-	uint32_t base;
-	uint4 r[4];
-};
-
-#ifndef QJ_WIN
-#define QJ_WIN 1
-#endif
-__device__ __forceinline__ uint4 rw_get(RecWin &w, const QJArgs &a, uint32_t i)
+// Resolves the probe that began at slot s with entry e for context c
+// (h = ctx_mix(c)): the first record of c's run (loaded into *r), its
+// length, or false when the index has no record of c.
+__device__ __forceinline__ bool ctab_resolve(const QJArgs &a, uint64_t c, uint64_t h, uint64_t s,
+					     unsigned long long e, uint32_t *d, uint32_t *len, uint4 *r)
 {
-	if (QJ_WIN == 1)
-		return a.rec[i];
-	if (i - w.base >= 4u) {
-		w.base = i;
-#pragma unroll
-		for (int k = 0; k < 4; ++k)
-			w.r[k] = (uint64_t)i + k < a.n ? a.rec[i + k] : make_uint4(0, 0, 0, 0);
+	const uint64_t mask = (1ull << a.ck) - 1;
+	const uint32_t fp = (uint32_t)(h & 0xffffull);
+	for (;;) {
+		if (e == SLOT_EMPTY)
+			return false;
+		if ((uint32_t)((e >> 32) & 0xffffull) == fp) {
+			const uint32_t st = (uint32_t)e;
+			const uint4 x = a.rec[st];
+			if ((rec_ctxgid(x) >> GID_NBITS) == c) { // else another context with this fingerprint
+				*d = st;
+				*len = (uint32_t)(e >> 48);
+				*r = x;
+				return true;
+			}
+		}
+		s = (s + 1) & mask;
+		e = a.ctab[s];
 	}
-	const uint32_t k = i - w.base;
-	return k == 0 ? w.r[0] : k == 1 ? w.r[1] : k == 2 ? w.r[2] : w.r[3];
 }
 
 // One context group of the query (keys [i, s_end) share context c, whose
-// bucket holds records [st, en)): find the context's first record and walk
-// its per-genome runs (command_ani.h:521-575; LOWER: :668-716).
+// run starts at record d with `len` records, r = record d): walk the
+// context's per-genome runs (command_ani.h:521-575; LOWER: :668-716).
 template <bool UNIQ, bool DIRECT>
 __device__ __forceinline__ void qj_group(const QJArgs &a, const QTable &t, uint64_t i, uint64_t key, uint64_t c,
-					 uint32_t st, uint32_t en, uint64_t ke, int qgid, unsigned long long *spill_row,
-					 LDS uint32_t *spilled)
+					 uint32_t d, uint32_t len, uint4 r, uint64_t ke, int qgid,
+					 unsigned long long *spill_row, LDS uint32_t *spilled)
 {
 	const unsigned ob = a.obj_bits;
 	const uint64_t objmask = ob >= 64 ? ~0ull : ((1ull << ob) - 1ull);
 	const bool lower = (a.flags & KSSDHIP_LOWER) != 0;
 	const bool ignore = (a.flags & KSSDHIP_IGNORECONFLICT) != 0;
-	const uint64_t target = c << GID_NBITS;
-	RecWin w;
-	w.base = st - 4u;
-	uint32_t d = st;
-	bool found = false;
-	uint4 r = make_uint4(0, 0, 0, 0);
-	for (; d < en; ++d) {
-		r = rw_get(w, a, d);
-		const uint64_t cg = rec_ctxgid(r);
-		if (cg >= target) {
-			found = (cg >> GID_NBITS) == c;
-			break;
-		}
-	}
-	if (!found)
-		return;
+	const bool capped = len >= RUN_CAP; // a longer run: walk until the context changes
+	const uint64_t end = capped ? a.n : (uint64_t)d + len;
 	uint64_t s_end = i + 1;
 	while (s_end < ke && (a.keys[s_end] >> ob) == c)
 		++s_end;
@@ -320,12 +322,13 @@ __device__ __forceinline__ void qj_group(const QJArgs &a, const QTable &t, uint6
 		bool more;
 		do {
 			++d;
-			more = d < a.n;
+			more = d < end;
 			if (more)
-				nx = rw_get(w, a, d);
+				nx = a.rec[d];
 		} while (more && rec_ctxgid(nx) == cg);
 		const uint32_t re = d;
-		more = more && (rec_ctxgid(nx) >> GID_NBITS) == c;
+		if (capped)
+			more = more && (rec_ctxgid(nx) >> GID_NBITS) == c;
 		bool count = true;
 		if (lower && (int)gid >= qgid) {
 			if ((int)gid > qgid)
@@ -369,7 +372,7 @@ __device__ __forceinline__ void qj_group(const QJArgs &a, const QTable &t, uint6
 // Counts of query q into the table: the first key of each context group
 // does the work for the group.  Each thread takes KB of the query's keys at
 // a time (strided by the workgroup size, so loads of the keys coalesce) and
-// issues their fingerprint and bucket loads together before any scan.
+// issues their context-table loads together before resolving any.
 template <bool UNIQ, bool DIRECT>
 __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned long long *spill_row,
 			 LDS uint32_t *spilled)
@@ -379,8 +382,8 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned 
 	const int qgid = a.qry_gid_offset + (int)q;
 	const uint64_t B = blockDim.x;
 	for (uint64_t base = kb + threadIdx.x; base < ke; base += KB * B) {
-		uint64_t key[KB], c[KB];
-		uint32_t st[KB], en[KB], fp[KB];
+		uint64_t key[KB], c[KB], hs[KB];
+		unsigned long long e[KB];
 		bool on[KB];
 #pragma unroll
 		for (int j = 0; j < KB; ++j) {
@@ -394,36 +397,28 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned 
 			c[j] = key[j] >> ob;
 			const uint64_t prev = on[j] && i > kb ? a.keys[i - 1] : ~0ull;
 			on[j] = on[j] && c[j] <= a.ctx_max && (i == kb || (prev >> ob) != c[j]);
+			hs[j] = ctx_mix(c[j]);
+			e[j] = on[j] ? a.ctab[hs[j] >> (64 - a.ck)] : SLOT_EMPTY;
 		}
-#pragma unroll
-		for (int j = 0; j < KB; ++j)
-			fp[j] = on[j] ? a.qfp[c[j] >> a.qshift] : 0u;
-#pragma unroll
-		for (int j = 0; j < KB; ++j) {
-			on[j] = on[j] && (fp[j] & fp_bit(c[j])) != 0u; // else no record of the bucket has context c
-			const uint32_t tb = (uint32_t)(c[j] >> a.qshift);
-			st[j] = on[j] ? a.qstart[tb] : 0u;
-			en[j] = on[j] ? a.qstart[tb + 1] : 0u;
-		}
-		// one instance of the group walk, fed from the batch by constant
-		// indexes (no private-array indexing)
+		// one instance of the resolve and the group walk, fed from the batch
+		// by constant indexes (no private-array indexing)
 #pragma unroll 1
 		for (int j = 0; j < KB; ++j) {
-			bool o = on[0];
-			uint64_t kj = key[0], cj = c[0];
-			uint32_t sj = st[0], ej = en[0];
+			uint64_t kj = key[0], cj = c[0], hj = hs[0];
+			unsigned long long ej = e[0];
 #pragma unroll
 			for (int x = 1; x < KB; ++x) {
 				if (j == x) {
-					o = on[x];
 					kj = key[x];
 					cj = c[x];
-					sj = st[x];
-					ej = en[x];
+					hj = hs[x];
+					ej = e[x];
 				}
 			}
-			if (o)
-				qj_group<UNIQ, DIRECT>(a, t, base + j * B, kj, cj, sj, ej, ke, qgid, spill_row, spilled);
+			uint32_t d, len;
+			uint4 r;
+			if (ej != SLOT_EMPTY && ctab_resolve(a, cj, hj, hj >> (64 - a.ck), ej, &d, &len, &r))
+				qj_group<UNIQ, DIRECT>(a, t, base + j * B, kj, cj, d, len, r, ke, qgid, spill_row, spilled);
 		}
 	}
 }
@@ -671,6 +666,40 @@ __global__ __launch_bounds__(256) void k_row_values(kssdhip_ani_row_t *rows, con
 	}
 }
 
+// Marks the records of every index context that occurs in the block
+// (marks: one byte per record, zeroed before), then k_count_marks adds them.
+__global__ __launch_bounds__(256) void k_mark_matched(QJArgs a, uint8_t *marks)
+{
+	const unsigned ob = a.obj_bits;
+	const uint64_t total = a.skidx[a.nq];
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+		const uint64_t c = a.keys[i] >> ob;
+		if (c > a.ctx_max)
+			continue;
+		if (i > 0 && (a.keys[i - 1] >> ob) == c)
+			continue; // the previous key (of this or the previous query) marked this context
+		const uint64_t h = ctx_mix(c);
+		const uint64_t slot = h >> (64 - a.ck);
+		uint32_t d, len;
+		uint4 r;
+		if (!ctab_resolve(a, c, h, slot, a.ctab[slot], &d, &len, &r))
+			continue;
+		for (uint32_t x = d; x < a.n && (rec_ctxgid(a.rec[x]) >> GID_NBITS) == c; ++x)
+			marks[x] = 1;
+	}
+}
+
+__global__ __launch_bounds__(256) void k_count_marks(const uint8_t *marks, uint64_t n, unsigned long long *count)
+{
+	uint32_t k = 0;
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		k += marks[i];
+	for (int off = 32; off > 0; off >>= 1)
+		k += __shfl_xor(k, off, 64);
+	if ((threadIdx.x & 63) == 0 && k)
+		atomicAdd(count, (unsigned long long)k);
+}
+
 struct QJShape {
 	bool direct;
 	uint32_t slots;
@@ -772,10 +801,9 @@ int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_
 void fill_index_args(const kssdhip_index *ix, QJArgs &q)
 {
 	q.rec = ix->d_qrec;
-	q.qstart = ix->d_qstart;
-	q.qfp = ix->d_qfp;
+	q.ctab = ix->d_ctab;
+	q.ck = ix->ck;
 	q.n = ix->n;
-	q.qshift = ix->qshift;
 	q.ctx_max = ix->ctx_max;
 	q.R = (uint32_t)ix->ref_gnum;
 	q.obj_bits = ix->obj_bits;
@@ -800,6 +828,12 @@ int qj_launch(const QJShape &sh, bool uniq, const QJArgs &a, hipStream_t s)
 
 } // namespace
 
+__global__ void k_fill_empty(unsigned long long *p, uint64_t n)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		p[i] = SLOT_EMPTY;
+}
+
 int kssd_qjoin_build(kssdhip_index *ix)
 {
 	kssd_qjoin_free(ix);
@@ -807,34 +841,39 @@ int kssd_qjoin_build(kssdhip_index *ix)
 	if (hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ix->device) == hipSuccess)
 		ix->cus = dev_cus;
 	const uint64_t n = ix->n;
-	// about 8-16 records per bucket, at most 2^24 buckets (128 MB)
-	int bw = 0;
-	while (bw < 64 && (n >> bw))
-		++bw;
-	int k = bw > 4 ? bw - 4 : 1;
-	if (k > 24)
-		k = 24;
-	int cbw = 0;
-	while (cbw < 64 && (ix->ctx_max >> cbw))
-		++cbw;
-	if (k > cbw)
-		k = cbw > 0 ? cbw : 1;
-	ix->qk = k;
-	ix->qshift = cbw - k > 0 ? cbw - k : 0;
-	const uint64_t nb = 1ull << k;
+	const hipStream_t s = ix->stream;
+	auto grid = [](uint64_t items) {
+		return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + 255) / 256, 1u << 20)));
+	};
+	unsigned long long runs = 0, *d_runs = nullptr;
+	if (hipMalloc(&d_runs, 8) != hipSuccess)
+		return kssdhip_fail(KSSDHIP_ENOMEM, "context table: device allocation failed");
+	hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, d_runs, (uint64_t)1);
+	if (n)
+		hipLaunchKernelGGL(k_count_runs, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, 4096)), dim3(256), 0, s,
+				   ix->d_ctxgid, n, d_runs);
+	hipError_t e = hipMemcpyAsync(&runs, d_runs, 8, hipMemcpyDeviceToHost, s);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(s);
+	(void)hipFree(d_runs);
+	if (e != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "context table: %s", hipGetErrorString(e));
+	int k = 10; // at least twice as many slots as contexts
+	while ((1ull << k) < 2 * (uint64_t)runs)
+		++k;
+	ix->ck = k;
+	const uint64_t slots = 1ull << k;
 	if (hipMalloc(&ix->d_qrec, (n + 1) * sizeof(uint4)) != hipSuccess ||
-	    hipMalloc(&ix->d_qstart, (nb + 1) * sizeof(uint32_t)) != hipSuccess ||
-	    hipMalloc(&ix->d_qfp, nb * sizeof(uint32_t)) != hipSuccess) {
+	    hipMalloc(&ix->d_ctab, slots * sizeof(unsigned long long)) != hipSuccess) {
 		kssd_qjoin_free(ix);
 		return kssdhip_fail(KSSDHIP_ENOMEM, "query-major index: device allocation failed");
 	}
-	const hipStream_t s = ix->stream;
-	auto grid = [](uint64_t items) { return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + 255) / 256, 1u << 20))); };
-	if (n)
+	hipLaunchKernelGGL(k_fill_empty, grid(slots), dim3(256), 0, s, ix->d_ctab, slots);
+	if (n) {
 		hipLaunchKernelGGL(k_pack16, grid(n), dim3(256), 0, s, ix->d_ctxgid, ix->d_obj, n, ix->d_qrec);
-	hipLaunchKernelGGL(k_qbkt_start, grid(nb + 1), dim3(256), 0, s, ix->d_ctxgid, n, ix->qshift, nb, ix->d_qstart);
-	hipLaunchKernelGGL(k_qbkt_fp, grid(nb), dim3(256), 0, s, ix->d_ctxgid, nb, ix->d_qstart, ix->d_qfp);
-	hipError_t e = hipGetLastError();
+		hipLaunchKernelGGL(k_ctab_insert, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, k, ix->d_ctab);
+	}
+	e = hipGetLastError();
 	if (e == hipSuccess)
 		e = hipStreamSynchronize(s);
 	if (e != hipSuccess) {
@@ -848,13 +887,10 @@ void kssd_qjoin_free(kssdhip_index *ix)
 {
 	if (ix->d_qrec)
 		(void)hipFree(ix->d_qrec);
-	if (ix->d_qstart)
-		(void)hipFree(ix->d_qstart);
-	if (ix->d_qfp)
-		(void)hipFree(ix->d_qfp);
+	if (ix->d_ctab)
+		(void)hipFree(ix->d_ctab);
 	ix->d_qrec = nullptr;
-	ix->d_qstart = nullptr;
-	ix->d_qfp = nullptr;
+	ix->d_ctab = nullptr;
 }
 
 int kssd_qjoin_prepare(kssdhip_index *ix)
@@ -1059,5 +1095,49 @@ extern "C" int kssdhip_select_block(kssdhip_index *ix, const uint64_t *section_s
 			return x.query != y.query ? x.query < y.query : x.ref < y.ref;
 		});
 	}
+	return 0;
+}
+
+extern "C" int kssdhip_index_matched_records(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx,
+					     int nq, uint64_t total, uint64_t *matched)
+{
+	if (!ix || nq < 0 || !matched || (nq > 0 && !d_skidx) || (total && !d_keys))
+		return kssdhip_fail(KSSDHIP_EINVAL, "matched_records: bad arguments");
+	*matched = 0;
+	if (!nq || !total || !ix->n)
+		return 0;
+	if (!ix->d_qrec)
+		return kssdhip_fail(KSSDHIP_EINVAL, "matched_records: the index has no query-major tables");
+	if (hipSetDevice(ix->device) != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "matched_records: device not available");
+	DevBuf marks;
+	const uint64_t words = (ix->n + 7) / 8 + 1; // the marks, then the count (8-byte aligned)
+	int rc = marks.reserve(words * 8);
+	if (rc)
+		return rc;
+	hipStream_t s = ix->stream;
+	unsigned long long *d_count = (unsigned long long *)marks.ptr + (words - 1);
+	QJArgs a;
+	memset(&a, 0, sizeof a);
+	fill_index_args(ix, a);
+	a.keys = d_keys;
+	a.skidx = d_skidx;
+	a.nq = (uint32_t)nq;
+	hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 8192)), dim3(256), 0, s,
+			   (unsigned long long *)marks.ptr, words);
+	hipLaunchKernelGGL(k_mark_matched, dim3((unsigned)std::min<uint64_t>((total + 255) / 256, 1u << 16)), dim3(256),
+			   0, s, a, (uint8_t *)marks.ptr);
+	hipLaunchKernelGGL(k_count_marks, dim3((unsigned)std::min<uint64_t>((ix->n + 255) / 256, 4096)), dim3(256), 0, s,
+			   (const uint8_t *)marks.ptr, (uint64_t)ix->n, d_count);
+	unsigned long long count = 0;
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess)
+		e = hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, s);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(s);
+	marks.release();
+	if (e != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "matched_records: %s", hipGetErrorString(e));
+	*matched = count;
 	return 0;
 }

@@ -289,6 +289,7 @@ def load_kssdhip():
     lib.kssdhip_index_get_tiling.argtypes = [V, ctypes.POINTER(Tiling)]
     lib.kssdhip_epilogue_dev.argtypes = [V, V, I, I, V, V, ctypes.POINTER(EpilogueOpts), V, V]
     lib.kssdhip_select_hits_dev.argtypes = [V, I, I, ctypes.c_double, U32, V, V, V]
+    lib.kssdhip_index_matched_records.argtypes = [V, V, V, I, U64, ctypes.POINTER(U64)]
     lib.kssdhip_select_block.argtypes = [V, V, V, I, U32, V, V, ctypes.POINTER(SelectOpts), V, S,
                                          ctypes.POINTER(S)]
     lib.kssdhip_select_block_dev.argtypes = [V, V, V, I, U64, U32, V, V, ctypes.POINTER(SelectOpts), V, U32,
@@ -464,6 +465,11 @@ class KssdHip:
         self.check(self.lib.kssdhip_select_block_dev(h, d_keys, d_skidx, nq, total, flags, d_qctx, d_rctx,
                                                      ctypes.byref(opts), d_rows, cap, d_count, d_first, d_qcount,
                                                      stream))
+
+    def matched_records(self, h, d_keys, d_skidx, nq, total):
+        m = U64()
+        self.check(self.lib.kssdhip_index_matched_records(h, d_keys, d_skidx, nq, total, ctypes.byref(m)))
+        return int(m.value)
 
     def select_hits_dev(self, d_values, nq, nr, min_value, cap, d_hits, d_count, stream=None):
         self.check(self.lib.kssdhip_select_hits_dev(d_values, nq, nr, float(min_value), cap, d_hits, d_count,

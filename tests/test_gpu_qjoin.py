@@ -284,3 +284,22 @@ def test_select_graph_capture(hip, oracle, wide_cases):
         for b in bufs:
             rt.free(b)
         hip.close_index(h)
+
+
+@pytest.mark.parametrize("shape", ["direct", "hashed"])
+def test_matched_records(hip, wide_cases, shape):
+    """kssdhip_index_matched_records equals the count of index records whose
+    context occurs in some query of the block (numpy)."""
+    rt = kt.Hip(0)
+    rk, ri, qk, qi, rec, _ = wide_cases(shape)
+    R, Q = len(ri) - 1, len(qi) - 1
+    want = int(np.isin(rec["ctxgid"] >> np.uint64(20), np.unique(qk >> np.uint64(OBJ_BITS))).sum())
+    h = hip.open_index(rec, R, OBJ_BITS)
+    d_k, d_i = rt.upload(qk), rt.upload(qi)
+    try:
+        assert hip.matched_records(h, d_k, d_i, Q, int(qi[-1])) == want
+        assert hip.matched_records(h, d_k, d_i, 0, 0) == 0
+    finally:
+        rt.free(d_k)
+        rt.free(d_i)
+        hip.close_index(h)
