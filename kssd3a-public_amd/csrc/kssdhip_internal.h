@@ -120,6 +120,9 @@ struct kssdhip_index {
 	uint4 *d_qrec = nullptr;
 	unsigned long long *d_ctab = nullptr;
 	int ck = 0;
+	// a word-blocked Bloom filter of the contexts (2^fk 64-bit words)
+	unsigned long long *d_filt = nullptr;
+	int fk = 0;
 	int cus = 0; // compute units of the device
 	int qj_per_cu[2] = {0, 0}; // query-major workgroups per CU (plain, unique counters) at the tiling
 	// grow-only scratch: host-pointer entry points, the split table, the

@@ -124,6 +124,30 @@ __global__ void k_ctab_insert(const uint64_t *__restrict__ ctxgid, uint64_t n, i
 	}
 }
 
+// A word-blocked Bloom filter of the index's contexts in front of the
+// context table: each context sets 4 bits of one 64-bit word (all from a
+// second mix of the context), about 10-20 bits per context, so that most
+// absent contexts are rejected by one word that stays in the device's
+// last-level cache instead of a probe of the table in HBM.
+#ifndef QJ_FILTER
+#define QJ_FILTER 1
+#endif
+__host__ __device__ __forceinline__ uint64_t filt_mix(uint64_t c) { return ctx_mix(c ^ 0x9e3779b97f4a7c15ull); }
+__host__ __device__ __forceinline__ unsigned long long filt_bits(uint64_t g)
+{
+	return (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63)) | (1ull << ((g >> 18) & 63));
+}
+
+__global__ void k_filt_insert(const uint64_t *__restrict__ ctxgid, uint64_t n, int fk, unsigned long long *filt)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+		if (!run_start(ctxgid, i))
+			continue;
+		const uint64_t g = filt_mix(ctxgid[i] >> GID_NBITS);
+		atomicOr(&filt[g >> (64 - fk)], filt_bits(g));
+	}
+}
+
 // Work order of a block: queries grouped by the bit length of their key
 // count, longest first (so the largest queries do not end the launch
 // alone).  head[1 .. 32] histogram, head[33 .. 64] cursors (zeroed before).
@@ -133,10 +157,20 @@ __device__ __forceinline__ uint32_t size_class(const uint64_t *skidx, uint32_t q
 	return k ? 63u - (uint32_t)__clzll((long long)k) : 0u; // 0 .. 31 for k < 2^32
 }
 
-__global__ void k_order_hist(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t *head)
+// per workgroup: an LDS histogram of its queries' classes, then one global
+// atomic per class (the classes of a block are few, so per-query global
+// atomics would queue on the same words)
+__global__ __launch_bounds__(256) void k_order_hist(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t *head)
 {
+	__shared__ uint32_t h[32];
+	if (threadIdx.x < 32)
+		h[threadIdx.x] = 0;
+	__syncthreads();
 	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x)
-		atomicAdd(&head[1 + (size_class(skidx, q) & 31u)], 1u);
+		atomicAdd(&h[size_class(skidx, q) & 31u], 1u);
+	__syncthreads();
+	if (threadIdx.x < 32 && h[threadIdx.x])
+		atomicAdd(&head[1 + threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void k_order_scan(uint32_t *head)
@@ -150,10 +184,27 @@ __global__ void k_order_scan(uint32_t *head)
 	}
 }
 
-__global__ void k_order_scatter(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t *head, uint32_t *order)
+// per workgroup: its queries' ranks within their class in LDS, one global
+// atomic per class to reserve the workgroup's range of each
+__global__ __launch_bounds__(256) void k_order_scatter(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t *head,
+						       uint32_t *order)
 {
-	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x)
-		order[atomicAdd(&head[33 + (size_class(skidx, q) & 31u)], 1u)] = q;
+	__shared__ uint32_t cnt[32], base[32];
+	for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < nq; q0 += gridDim.x * blockDim.x) {
+		if (threadIdx.x < 32)
+			cnt[threadIdx.x] = 0;
+		__syncthreads();
+		const uint32_t q = q0 + threadIdx.x;
+		const uint32_t c = q < nq ? size_class(skidx, q) & 31u : 0u;
+		const uint32_t r = q < nq ? atomicAdd(&cnt[c], 1u) : 0u;
+		__syncthreads();
+		if (threadIdx.x < 32 && cnt[threadIdx.x])
+			base[threadIdx.x] = atomicAdd(&head[33 + threadIdx.x], cnt[threadIdx.x]);
+		__syncthreads();
+		if (q < nq)
+			order[base[c] + r] = q;
+		__syncthreads();
+	}
 }
 
 __global__ void k_zero_u32(uint32_t *p, uint32_t n)
@@ -173,6 +224,8 @@ struct QJArgs {
 	const uint4 *rec;
 	const unsigned long long *ctab; // the context table, 2^ck slots
 	int ck;
+	const unsigned long long *filt; // the context filter, 2^fk words
+	int fk;
 	uint64_t n;
 	uint64_t ctx_max;
 	uint32_t R; // ref_gnum
@@ -397,6 +450,18 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned 
 			c[j] = key[j] >> ob;
 			const uint64_t prev = on[j] && i > kb ? a.keys[i - 1] : ~0ull;
 			on[j] = on[j] && c[j] <= a.ctx_max && (i == kb || (prev >> ob) != c[j]);
+		}
+		if (QJ_FILTER) {
+#pragma unroll
+			for (int j = 0; j < KB; ++j) {
+				const uint64_t g = filt_mix(c[j]);
+				const unsigned long long want = filt_bits(g);
+				const unsigned long long w = on[j] ? a.filt[g >> (64 - a.fk)] : 0ull;
+				on[j] = on[j] && (w & want) == want; // else the index has no record of c[j]
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < KB; ++j) {
 			hs[j] = ctx_mix(c[j]);
 			e[j] = on[j] ? a.ctab[hs[j] >> (64 - a.ck)] : SLOT_EMPTY;
 		}
@@ -791,7 +856,7 @@ int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_
 	w->order = (uint32_t *)((char *)ws->ptr + WS_HEAD);
 	w->spill = (unsigned long long *)((char *)ws->ptr + WS_HEAD + order_bytes);
 	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s, w->head, WS_HEAD_WORDS);
-	const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nq + 255) / 256, 1024));
+	const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nq + 255) / 256, 2048));
 	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, d_skidx, nq, w->head);
 	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, w->head);
 	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, d_skidx, nq, w->head, w->order);
@@ -803,6 +868,8 @@ void fill_index_args(const kssdhip_index *ix, QJArgs &q)
 	q.rec = ix->d_qrec;
 	q.ctab = ix->d_ctab;
 	q.ck = ix->ck;
+	q.filt = ix->d_filt;
+	q.fk = ix->fk;
 	q.n = ix->n;
 	q.ctx_max = ix->ctx_max;
 	q.R = (uint32_t)ix->ref_gnum;
@@ -863,15 +930,23 @@ int kssd_qjoin_build(kssdhip_index *ix)
 		++k;
 	ix->ck = k;
 	const uint64_t slots = 1ull << k;
+	int fk = 6; // at least 10 bits per context
+	while ((64ull << fk) < 10 * (uint64_t)runs)
+		++fk;
+	ix->fk = fk;
+	const uint64_t fwords = 1ull << fk;
 	if (hipMalloc(&ix->d_qrec, (n + 1) * sizeof(uint4)) != hipSuccess ||
-	    hipMalloc(&ix->d_ctab, slots * sizeof(unsigned long long)) != hipSuccess) {
+	    hipMalloc(&ix->d_ctab, slots * sizeof(unsigned long long)) != hipSuccess ||
+	    hipMalloc(&ix->d_filt, fwords * sizeof(unsigned long long)) != hipSuccess) {
 		kssd_qjoin_free(ix);
 		return kssdhip_fail(KSSDHIP_ENOMEM, "query-major index: device allocation failed");
 	}
 	hipLaunchKernelGGL(k_fill_empty, grid(slots), dim3(256), 0, s, ix->d_ctab, slots);
+	hipLaunchKernelGGL(k_zero_u64, grid(fwords), dim3(256), 0, s, ix->d_filt, fwords);
 	if (n) {
 		hipLaunchKernelGGL(k_pack16, grid(n), dim3(256), 0, s, ix->d_ctxgid, ix->d_obj, n, ix->d_qrec);
 		hipLaunchKernelGGL(k_ctab_insert, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, k, ix->d_ctab);
+		hipLaunchKernelGGL(k_filt_insert, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, fk, ix->d_filt);
 	}
 	e = hipGetLastError();
 	if (e == hipSuccess)
@@ -889,8 +964,11 @@ void kssd_qjoin_free(kssdhip_index *ix)
 		(void)hipFree(ix->d_qrec);
 	if (ix->d_ctab)
 		(void)hipFree(ix->d_ctab);
+	if (ix->d_filt)
+		(void)hipFree(ix->d_filt);
 	ix->d_qrec = nullptr;
 	ix->d_ctab = nullptr;
+	ix->d_filt = nullptr;
 }
 
 int kssd_qjoin_prepare(kssdhip_index *ix)
