@@ -80,6 +80,8 @@ static struct {
 			   kssd_obj_section_t *, kssd_ctx_mut2_t *, kssd_obj_section_t *);
 	int (*count_votes)(kssdhip_index *, const uint64_t *, const uint64_t *, int, int, uint32_t, uint32_t,
 			   uint32_t *);
+	int (*select_block)(kssdhip_index *, const uint64_t *, const uint64_t *, int, uint32_t, const uint32_t *,
+			    const uint32_t *, const kssdhip_select_opts_t *, kssdhip_ani_row_t *, size_t, size_t *);
 } g_hip;
 
 static void *hip_sym(void *lib, const char *name)
@@ -110,6 +112,7 @@ static typeof(g_hip) *hip_load(void)
 	g_hip.build_index = (typeof(g_hip.build_index))hip_sym(lib, "kssdhip_build_index");
 	g_hip.count_block = (typeof(g_hip.count_block))hip_sym(lib, "kssdhip_count_block");
 	g_hip.count_votes = (typeof(g_hip.count_votes))hip_sym(lib, "kssdhip_count_votes");
+	g_hip.select_block = (typeof(g_hip.select_block))hip_sym(lib, "kssdhip_select_block");
 	g_hip.init = (int (*)(int))hip_sym(lib, "kssdhip_init");
 	return &g_hip;
 }
@@ -163,6 +166,7 @@ static typeof(g_hip) *hip_api(void)
 #define kssdhip_build_index(...) (hip_api()->build_index(__VA_ARGS__))
 #define kssdhip_count_block(...) (hip_api()->count_block(__VA_ARGS__))
 #define kssdhip_count_votes(...) (hip_api()->count_votes(__VA_ARGS__))
+#define kssdhip_select_block(...) (hip_api()->select_block(__VA_ARGS__))
 
 /* ------------------------------------------------------------ small helpers */
 static const char *g_cmd = "";
@@ -253,6 +257,14 @@ static void *xmalloc(size_t n)
 	if (!p)
 		err(EXIT_FAILURE, "out of memory (%zu bytes)", n);
 	return p;
+}
+
+static void *xrealloc(void *p, size_t n)
+{
+	void *q = realloc(p, n ? n : 1);
+	if (!q)
+		err(EXIT_FAILURE, "out of memory (%zu bytes)", n);
+	return q;
 }
 
 static void *xcalloc(size_t n, size_t sz)
@@ -869,6 +881,85 @@ static void emit_rows(FILE *fp, const ani_args_t *a, detail_path_t path, const k
 	free(bufs);
 }
 
+/* The index path's detail rows of a block from the pairs that passed the
+ * device-side gate (rows ordered by query, then reference): the host ranks
+ * and prints each query's candidates exactly as from dense counts. */
+static void emit_index_rows(FILE *fp, const ani_args_t *a, const kssd_sketchdir_t *ref, const kssd_sketchdir_t *qry,
+			    int q0, int nq, const kssdhip_ani_row_t *rows, size_t n, const uint32_t *qry_ctx,
+			    const uint32_t *ref_ctx, const cov_inputs_t *cv)
+{
+	const int R = ref->n;
+	uint32_t *ids = xmalloc((n ? n : 1) * sizeof *ids);
+	kssd_ctx_mut2_t *ctx = xmalloc((n ? n : 1) * sizeof *ctx), *uctx = xmalloc((n ? n : 1) * sizeof *uctx);
+	kssd_obj_section_t *obj = xmalloc((n ? n : 1) * sizeof *obj), *uobj = xmalloc((n ? n : 1) * sizeof *uobj);
+	size_t *off = xcalloc((size_t)nq + 1, sizeof *off);
+	for (size_t k = 0; k < n; ++k) {
+		ids[k] = rows[k].ref;
+		ctx[k] = rows[k].ctx;
+		obj[k] = rows[k].obj;
+		uctx[k] = rows[k].unique_ctx;
+		uobj[k] = rows[k].unique_obj;
+		off[rows[k].query + 1]++;
+	}
+	for (int i = 0; i < nq; ++i)
+		off[i + 1] += off[i];
+	kssd_buf_t *bufs = xcalloc((size_t)(nq ? nq : 1), sizeof *bufs);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(a->p)
+	for (int i = 0; i < nq; ++i) {
+		const int q = q0 + i;
+		const kssd_coverage_src_t src = {
+			.qkeys = qry->keys + qry->index[q],
+			.qcounts = cv->qcounts ? cv->qcounts + qry->index[q] : NULL,
+			.qn = (size_t)(qry->index[q + 1] - qry->index[q]),
+			.ref_keys = ref->keys,
+			.ref_index = ref->index,
+			.ref_records = cv->bits ? ref->records : NULL,
+			.ref_unique_bits = cv->bits,
+			.ref_nrec = (size_t)ref->index[R],
+			.obj_bits = ref->obj_bits,
+		};
+		const size_t b = off[i];
+		kssd_query_counts_t c = {
+			.cov = a->r.estimate_coverage ? &src : NULL,
+			.ctx = ctx + b,
+			.obj = obj + b,
+			.uctx = a->r.unique_ctx ? uctx + b : NULL,
+			.uobj = a->r.unique_ctx ? uobj + b : NULL,
+			.qry_ctx = qry_ctx[q],
+			.ref_ctx = ref_ctx,
+			.nref = R,
+			.qry_meta = qry->meta ? &qry->meta[q] : NULL,
+			.ref_meta = ref->meta,
+			.ids = ids + b,
+			.ncand = (int)(off[i + 1] - b),
+		};
+		kssd_emit_index_detail(&bufs[i], &a->r, &c, qry->names[q], ref->names, ref->anno);
+	}
+	for (int i = 0; i < nq; ++i) {
+		put_buf(fp, &bufs[i]);
+		kssd_buf_free(&bufs[i]);
+	}
+	free(bufs);
+	free(off);
+	free(ids);
+	free(ctx);
+	free(obj);
+	free(uctx);
+	free(uobj);
+}
+
+/* Queries per block of the device-side gate: the block's keys stay below
+ * 2^28 (2 GiB on the device) and its queries below 2^16. */
+static int select_block_queries(const kssd_sketchdir_t *qry, int q0)
+{
+	const uint64_t key_cap = (uint64_t)1 << 28;
+	int q = q0;
+	while (q < qry->n && q - q0 < 65536 &&
+	       (q == q0 || qry->index[q + 1] - qry->index[q0] <= key_cap))
+		++q;
+	return q - q0;
+}
+
 /* mem_eff_sorted_ctxgidobj_arrXcomb_sortedsketch64, command_ani.c:494-692 */
 static int run_index_path(const ani_args_t *a)
 {
@@ -902,6 +993,52 @@ static int run_index_path(const ani_args_t *a)
 		put_detail_header(fp, a);
 	const bool unique = a->r.unique_ctx;
 	const uint32_t flags = (unique ? KSSDHIP_UNIQUE : 0) | (a->r.ignoreconflict ? KSSDHIP_IGNORECONFLICT : 0);
+	if (!a->r.fmt && a->r.ctxcut >= 1 && R > 0) {
+		/* detail output: the pass gate of command_ani.h:585-606 on the
+		 * device, only the passing pairs come back */
+		kssdhip_select_opts_t so;
+		memset(&so, 0, sizeof so);
+		so.ctxcut = (uint32_t)a->r.ctxcut;
+		so.afcut = a->r.afcut;
+		so.anicut = a->r.anicut;
+		so.unique_rank = unique;
+		so.value.metric = 3;
+		so.value.values_ani = 1;
+		so.value.drfold = a->r.drfold;
+		so.value.obj_bits = ref.obj_bits;
+		so.value.ctx_bits = 44;
+		so.value.exception = 1.0;
+		size_t cap = 1 << 16;
+		kssdhip_ani_row_t *rows = xmalloc(cap * sizeof *rows);
+		for (int q0 = 0; q0 < Q;) {
+			const int nb = select_block_queries(&qry, q0);
+			size_t n = 0;
+			int rc = kssdhip_select_block(ix, qry.keys + qry.index[q0], qry.index + q0, nb, flags, qry_ctx + q0,
+						      ref_ctx, &so, rows, cap, &n);
+			if (rc == KSSDHIP_ENOSPC) {
+				cap = n;
+				rows = xrealloc(rows, cap * sizeof *rows);
+				rc = kssdhip_select_block(ix, qry.keys + qry.index[q0], qry.index + q0, nb, flags,
+							  qry_ctx + q0, ref_ctx, &so, rows, cap, &n);
+			}
+			hip_check(rc, "GPU counting failed");
+			stage("block counted and gated");
+			emit_index_rows(fp, a, &ref, &qry, q0, nb, rows, n, qry_ctx, ref_ctx, &cv);
+			stage("block ranked and formatted");
+			q0 += nb;
+		}
+		free(rows);
+		close_out(fp);
+		kssdhip_index_close(ix);
+		stage("index released");
+		free(cv.qcounts);
+		free(cv.bits);
+		free(ref_ctx);
+		free(qry_ctx);
+		kssd_sketchdir_close(&ref);
+		kssd_sketchdir_close(&qry);
+		return 0;
+	}
 	const int blk = block_queries(Q, R, unique);
 	const size_t cells = (size_t)blk * (size_t)(R ? R : 1);
 	kssd_ctx_mut2_t *ctx = xmalloc(cells * sizeof *ctx);

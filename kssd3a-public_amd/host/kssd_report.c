@@ -800,30 +800,34 @@ void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const 
 	const int ntop = o->ntop > 0 ? o->ntop : R;
 	if (R <= 0)
 		return;
-	/* ranking, command_ani.h:577-621 */
-	idani_t *ids = malloc((size_t)R * sizeof *ids);
-	unique_rank_t *ranks = o->unique_ctx ? malloc((size_t)R * sizeof *ranks) : NULL;
+	/* ranking, command_ani.h:577-621; the ids stored while ranking are
+	 * candidate positions k, ascending with the reference ids j = cand(k),
+	 * so every comparison on them orders as on the ids */
+	const int ncand = c->ids ? c->ncand : R;
+	idani_t *ids = malloc((size_t)(ncand > 0 ? ncand : 1) * sizeof *ids);
+	unique_rank_t *ranks = o->unique_ctx ? malloc((size_t)(ncand > 0 ? ncand : 1) * sizeof *ranks) : NULL;
 	if (!ids || (o->unique_ctx && !ranks))
 		abort();
 	int npass = 0;
-	for (int j = 0; j < R; ++j) {
-		const uint32_t shared = c->ctx[j].num_ctx;
+	for (int k = 0; k < ncand; ++k) {
+		const uint32_t j = c->ids ? c->ids[k] : (uint32_t)k;
+		const uint32_t shared = c->ctx[k].num_ctx;
 		if (shared < ctxcut)
 			continue;
 		if (c->ref_ctx[j] == 0 || c->qry_ctx == 0 ||
 		    !af_pass(o, (double)shared / (double)c->qry_ctx, (double)shared / (double)c->ref_ctx[j]))
 			continue;
-		const float dist = (float)c->obj[j].diff_obj / c->ctx[j].num_ctx / 7;
+		const float dist = (float)c->obj[k].diff_obj / c->ctx[k].num_ctx / 7;
 		double ani = (float)(1 - dist);
 		if (ranks) {
-			const bool u = use_unique(c->uctx[j].num_ctx, c->ref_ctx[j]);
-			ani = 1.0 - naive_of(u ? c->uctx[j] : c->ctx[j], u ? c->uobj[j] : c->obj[j]);
+			const bool u = use_unique(c->uctx[k].num_ctx, c->ref_ctx[j]);
+			ani = 1.0 - naive_of(u ? c->uctx[k] : c->ctx[k], u ? c->uobj[k] : c->obj[k]);
 		}
 		if (ani < o->anicut)
 			continue;
 		if (ranks)
-			ranks[npass] = (unique_rank_t){ani, (uint32_t)j};
-		ids[npass].id = (uint32_t)j;
+			ranks[npass] = (unique_rank_t){ani, (uint32_t)k};
+		ids[npass].id = (uint32_t)k;
 		ids[npass].ani = (float)ani;
 		npass++;
 	}
@@ -845,21 +849,22 @@ void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const 
 		abort();
 	const int qsz = (int)c->qry_ctx;
 	for (int n = 0; n < out_n; ++n) {
-		const uint32_t j = ids[n].id;
+		const uint32_t k = ids[n].id;
+		const uint32_t j = c->ids ? c->ids[k] : k;
 		const int rsz = (int)c->ref_ctx[j];
 		if (qsz == 0 || rsz == 0)
 			continue;
-		const uint32_t xny = c->ctx[j].num_ctx, mut2 = c->ctx[j].num_mut2_ctx;
-		const uint32_t dobj = c->obj[j].diff_obj, dsec = c->obj[j].diff_obj_section;
+		const uint32_t xny = c->ctx[k].num_ctx, mut2 = c->ctx[k].num_mut2_ctx;
+		const uint32_t dobj = c->obj[k].diff_obj, dsec = c->obj[k].diff_obj_section;
 		const float afq = (float)xny / qsz, afr = (float)xny / rsz;
 		const float bq = (float)kssd_af_anib(xny, dobj, dsec, mut2, (uint32_t)qsz, o->drfold);
 		const float br = (float)kssd_af_anib(xny, dobj, dsec, mut2, (uint32_t)rsz, o->drfold);
 		kssd_row_t row = kssd_make_row(j, xny, dobj, dsec, mut2, o, (uint32_t)qsz, (uint32_t)rsz, afq, bq,
 					       afr, br, c->qry_meta, c->ref_meta ? &c->ref_meta[j] : NULL);
 		if (o->unique_ctx) {
-			row.unique_ctx_selected = use_unique(c->uctx[j].num_ctx, c->ref_ctx[j]);
-			row.naive_dist = naive_of(row.unique_ctx_selected ? c->uctx[j] : c->ctx[j],
-						  row.unique_ctx_selected ? c->uobj[j] : c->obj[j]);
+			row.unique_ctx_selected = use_unique(c->uctx[k].num_ctx, c->ref_ctx[j]);
+			row.naive_dist = naive_of(row.unique_ctx_selected ? c->uctx[k] : c->ctx[k],
+						  row.unique_ctx_selected ? c->uobj[k] : c->obj[k]);
 			kssd_finalize_row(&row, o);
 		}
 		if (row.selected_ani < o->anicut)

@@ -142,7 +142,10 @@ typedef struct {
  * it holds; abundance fractions are shares of the rows' summed counts. */
 void kssd_populate_coverage(kssd_row_t *rows, size_t n, const kssd_coverage_src_t *src);
 
-/* Rows with qry/ref context totals and counts for one query against all refs. */
+/* Rows with qry/ref context totals and counts for one query against all
+ * refs -- or, when ids is not NULL, against the ncand references ids[0 ..
+ * ncand) (ascending) only, ctx/obj/uctx/uobj then holding ncand cells (the
+ * pairs a device-side gate let through; every other pair reads as zero). */
 typedef struct {
 	const kssd_ctx_mut2_t *ctx;  /* [nref] */
 	const kssd_obj_section_t *obj;
@@ -154,10 +157,15 @@ typedef struct {
 	const kssd_infile_meta_t *qry_meta;  /* may be NULL */
 	const kssd_infile_meta_t *ref_meta;  /* [nref] or NULL */
 	const kssd_coverage_src_t *cov;      /* --estimate-coverage inputs (NULL: not computed) */
+	const uint32_t *ids;                 /* candidate references (index path only) or NULL */
+	int ncand;
 } kssd_query_counts_t;
 
 /* Index path (mem_eff) detail rows of one query, ranked and filtered as the
- * reference does; appends the printed rows to out. */
+ * reference does; appends the printed rows to out.  With c->ids the pass
+ * gate is re-evaluated on the candidates only; that gives the reference's
+ * rows when no other pair could pass it (ctxcut >= 1: a pair with no shared
+ * context is never a candidate). */
 void kssd_emit_index_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const kssd_query_counts_t *c,
 			    const char *qry_name, char (*ref_names)[KSSD_PATHLEN],
 			    char (*ref_anno)[KSSD_PATHLEN]);

@@ -105,7 +105,7 @@ class ReportOpts(ctypes.Structure):
 
 class QueryCounts(ctypes.Structure):
     _fields_ = [("ctx", V), ("obj", V), ("uctx", V), ("uobj", V), ("qry_ctx", U32), ("ref_ctx", V),
-                ("nref", I), ("qry_meta", V), ("ref_meta", V), ("cov", V)]
+                ("nref", I), ("qry_meta", V), ("ref_meta", V), ("cov", V), ("ids", V), ("ncand", I)]
 
 
 class Buf(ctypes.Structure):

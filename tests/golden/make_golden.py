@@ -394,6 +394,13 @@ COVERAGE_CASES = {  # golden: (ani arguments, environment), sketches from READSE
     "cov_unique": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "--unique-ctx", "-n", "0.5"], {}),
     "cov_top1": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5", "-N", "1"],
                  {"KSSD3A_FORCE_REF_INDEX": "1"}),
+    # --asone --conflict -A read sets queried with --qraw --unique-ctx
+    # (the C4 combination), on the stream path and the index path
+    "asone_unique": (["-r", "refs", "--qraw", "rs_asone", "--unique-ctx", "-n0", "-f0", "-t1"], {}),
+    "asone_unique_index": (["-r", "refs", "--qraw", "rs_asone", "--unique-ctx", "-n0", "-f0", "-t1"],
+                           {"KSSD3A_FORCE_REF_INDEX": "1"}),
+    "A_unique_index": (["-r", "refs", "--qraw", "rs_A", "--unique-ctx", "-n0", "-f0", "-t1"],
+                       {"KSSD3A_FORCE_REF_INDEX": "1"}),
 }
 
 
@@ -437,6 +444,116 @@ def make_readsets():
     (out / "sketch_digests.json").write_text(json.dumps(digests, indent=1, sort_keys=True) + "\n")
 
 
+def indel(rng, seq, frac):
+    """Insertions and deletions of 1-50 bases (half each) whose lengths add
+    up to about `frac` of the sequence."""
+    s = seq.copy()
+    budget = int(frac * len(s))
+    while budget > 0:
+        n = int(rng.integers(1, 51))
+        at = int(rng.integers(0, max(1, len(s) - n)))
+        if rng.random() < 0.5:
+            s = np.concatenate([s[:at], rng.integers(0, 4, size=n, dtype=np.uint8), s[at:]])
+        else:
+            s = np.concatenate([s[:at], s[at + n:]])
+        budget -= n
+    return s
+
+
+def sketch_fastas(out, name, seqs, prefix):
+    """FASTA files of `seqs` under out/_fasta, sketched by the reference into
+    out/name (names relative to out, as make_genomes does)."""
+    tmp = out / "_fasta"
+    tmp.mkdir(parents=True, exist_ok=True)
+    paths = []
+    for i, g in enumerate(seqs):
+        p = tmp / f"{prefix}{i}.fa"
+        write_fasta(p, f"{prefix}{i}", g)
+        paths.append(p.relative_to(out))
+    run(["sketch", "-o", name, *paths], cwd=out)
+
+
+SCALE_CASES = {  # scale/<name>.tsv: (reference dir, query flag, query dir, arguments)
+    # C2 recipe (lognormal lengths, 0.1-10% substitutions + 1% indels), 50 x 50
+    "c2_best_index": ("c2/ref", "-q", "c2/qry", []),
+    "c2_s3_index": ("c2/ref", "-q", "c2/qry", ["-s3", "-n0", "-f0", "-t1"]),
+    "c2_best_noindex": ("c2/ref_noidx", "-q", "c2/qry", []),
+    # C3 recipe: 10 clades of 20 references (members U(1%, 15%) from the
+    # ancestor) x 300 new clade members; the index path with its defaults,
+    # looser cuts, and unique contexts
+    "c3_best": ("c3/ref", "-q", "c3/qry", []),
+    "c3_s3_loose": ("c3/ref", "-q", "c3/qry", ["-s3", "-n", "0.9", "-f", "0.1"]),
+    "c3_s4_all": ("c3/ref", "-q", "c3/qry", ["-s4", "-n0", "-f0", "-t1"]),
+    "c3_qraw_unique": ("c3/ref", "--qraw", "c3/qry", ["--unique-ctx", "-n0", "-f0", "-t1"]),
+    # heap top-N with many tied ANIs: 200 references (40 variants x 5 copies)
+    # all sharing contexts with every query, -N 3 (m > 128, N < m / 2)
+    "p2_top3": ("p2/ref", "-q", "p2/qry", ["-N", "3", "-n0", "-f0", "-t1"]),
+    "p2_top3_s3": ("p2/ref", "-q", "p2/qry", ["-s3", "-N", "3", "-n0", "-f0", "-t1"]),
+    "p2_top7_s4": ("p2/ref", "-q", "p2/qry", ["-s4", "-N", "7", "-n0", "-f0", "-t1"]),
+    "p2_top120": ("p2/ref", "-q", "p2/qry", ["-N", "120", "-n0", "-f0", "-t1"]),
+    "p2_all_s3": ("p2/ref", "-q", "p2/qry", ["-s3", "-n0", "-f0", "-t1"]),
+    # the reference's block boundary (tests/test_unique_context.py:216-223):
+    # 4096 one-context references, 310 queries, every third one empty
+    "boundary_unique": ("boundary/ref", "--qraw", "boundary/qry", ["--unique-ctx", "-n0", "-f0", "-t1", "-p4"]),
+    "boundary_detail": ("boundary/ref", "-q", "boundary/qry", ["-n0", "-f0", "-t1"]),
+}
+
+
+def make_scale():
+    """Down-scaled slices of the benchmark configurations and the ranking
+    and block-boundary cases, sketched and answered by the reference."""
+    out = HERE / "scale"
+    shutil.rmtree(out, ignore_errors=True)
+    out.mkdir(parents=True)
+    rng = np.random.default_rng(20261204)
+    # C2 slice
+    lens = np.clip(rng.lognormal(np.log(1.5e5), 0.5, size=50), 5e4, 4.5e5).astype(int)
+    refs = [rng.integers(0, 4, size=n, dtype=np.uint8) for n in lens]
+    qrys = [indel(rng, mutate(rng, refs[int(rng.integers(0, 50))], rng.uniform(0.001, 0.10)), 0.01)
+            for _ in range(50)]
+    (out / "c2").mkdir()
+    sketch_fastas(out / "c2", "ref", refs, "r")
+    sketch_fastas(out / "c2", "qry", qrys, "q")
+    shutil.copytree(out / "c2" / "ref", out / "c2" / "ref_noidx")
+    run(["sketch", "-i", "ref"], cwd=out / "c2")
+    # C3 slice
+    anc = [rng.integers(0, 4, size=int(rng.uniform(4e4, 8e4)), dtype=np.uint8) for _ in range(10)]
+    refs = [mutate(rng, anc[c], rng.uniform(0.01, 0.15)) for c in range(10) for _ in range(20)]
+    qrys = [mutate(rng, anc[int(rng.integers(0, 10))], rng.uniform(0.01, 0.15)) for _ in range(300)]
+    (out / "c3").mkdir()
+    sketch_fastas(out / "c3", "ref", refs, "r")
+    sketch_fastas(out / "c3", "qry", qrys, "q")
+    run(["sketch", "-i", "ref"], cwd=out / "c3")
+    # ties and the heap top-N
+    a = rng.integers(0, 4, size=50_000, dtype=np.uint8)
+    variants = [mutate(rng, a, [0.005, 0.01, 0.02][v % 3]) for v in range(40)]
+    order = rng.permutation(200)
+    refs = [variants[i % 40] for i in order]
+    qrys = [mutate(rng, a, rng.uniform(0.005, 0.03)) for _ in range(12)]
+    (out / "p2").mkdir()
+    sketch_fastas(out / "p2", "ref", refs, "r")
+    sketch_fastas(out / "p2", "qry", qrys, "q")
+    run(["sketch", "-i", "ref"], cwd=out / "p2")
+    # block boundary (hand-built sketches, as the reference's test builds them)
+    kt.write_sketch_dir(out / "boundary" / "ref", [to_keys([(i + 1, 0)]) for i in range(4096)],
+                        names=[f"ref_{i}" for i in range(4096)])
+    kt.write_sketch_dir(out / "boundary" / "qry", [to_keys([(1, 0)] if i % 3 else []) for i in range(310)],
+                        names=[f"qry_{i}" for i in range(310)])
+    run(["sketch", "-i", "ref"], cwd=out / "boundary")
+    import gzip
+    import json
+    (out / "cases.json").write_text(json.dumps(SCALE_CASES, indent=1, sort_keys=True) + "\n")
+    for name, (r, qflag, q, args) in SCALE_CASES.items():
+        text = run(["ani", "-r", r, qflag, q, *args], cwd=out).encode()
+        if len(text) > 100_000:  # large reports kept compressed
+            with gzip.GzipFile(out / f"{name}.tsv.gz", "wb", mtime=0) as fh:
+                fh.write(text)
+        else:
+            (out / f"{name}.tsv").write_bytes(text)
+    for d in out.glob("*/_fasta"):
+        shutil.rmtree(d)
+
+
 def main():
     if not BIN.exists():
         raise SystemExit(f"{BIN} missing: run `make -C oracle ref` first")
@@ -454,6 +571,7 @@ def main():
     make_matrix_dedup()
     make_sketch()
     make_readsets()
+    make_scale()
     print("golden fixtures written under", HERE)
 
 

@@ -243,6 +243,33 @@ def test_cli_genomes_match_reference(tmp_path, golden, ref_index, args, env):
     assert out.read_text() == (GEN / golden).read_text()
 
 
+SCALE = kt.GOLDEN / "scale"
+
+
+def _golden_text(stem):
+    import gzip
+    f = SCALE / f"{stem}.tsv"
+    return f.read_text() if f.exists() else gzip.decompress((SCALE / f"{stem}.tsv.gz").read_bytes()).decode()
+
+
+SCALE_CASES = __import__("json").loads((SCALE / "cases.json").read_text())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(SCALE_CASES))
+def test_cli_scale_cases_match_reference(tmp_path, name):
+    """Down-scaled C2 and C3 workloads, the heap top-N with ties and the
+    block boundary with empty queries, each byte-identical to the reference's
+    report (tests/golden/make_golden.py, make_scale)."""
+    ref, qflag, qry, args = SCALE_CASES[name]
+    top = ref.split("/")[0]
+    shutil.copytree(SCALE / top, tmp_path / top)
+    p = subprocess.run([str(BIN), "ani", "-r", ref, qflag, qry, *args], capture_output=True, text=True,
+                       cwd=tmp_path, timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == _golden_text(name)
+
+
 @pytest.mark.gpu
 def test_cli_stdout_and_auto_index(tmp_path):
     """Without -o the rows go to stdout; the self full matrix above the

@@ -155,6 +155,13 @@ COVERAGE_CASES = {  # make_golden.py COVERAGE_CASES
     "cov_unique": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "--unique-ctx", "-n", "0.5"], {}),
     "cov_top1": (["-r", "refs", "--qraw", "rs_A", "--estimate-coverage", "-n", "0.5", "-N", "1"],
                  {"KSSD3A_FORCE_REF_INDEX": "1"}),
+    # --asone --conflict -A read sets queried with --qraw --unique-ctx
+    # (the C4 combination), on the stream path and the index path
+    "asone_unique": (["-r", "refs", "--qraw", "rs_asone", "--unique-ctx", "-n0", "-f0", "-t1"], {}),
+    "asone_unique_index": (["-r", "refs", "--qraw", "rs_asone", "--unique-ctx", "-n0", "-f0", "-t1"],
+                           {"KSSD3A_FORCE_REF_INDEX": "1"}),
+    "A_unique_index": (["-r", "refs", "--qraw", "rs_A", "--unique-ctx", "-n0", "-f0", "-t1"],
+                       {"KSSD3A_FORCE_REF_INDEX": "1"}),
 }
 
 
