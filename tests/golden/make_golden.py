@@ -473,29 +473,38 @@ def sketch_fastas(out, name, seqs, prefix):
     run(["sketch", "-o", name, *paths], cwd=out)
 
 
-SCALE_CASES = {  # scale/<name>.tsv: (reference dir, query flag, query dir, arguments)
+FORCE = {"KSSD3A_FORCE_REF_INDEX": "1"}
+SCALE_CASES = {  # scale/<name>.tsv: (reference dir, query flag, query dir, arguments, environment)
     # C2 recipe (lognormal lengths, 0.1-10% substitutions + 1% indels), 50 x 50
-    "c2_best_index": ("c2/ref", "-q", "c2/qry", []),
-    "c2_s3_index": ("c2/ref", "-q", "c2/qry", ["-s3", "-n0", "-f0", "-t1"]),
-    "c2_best_noindex": ("c2/ref_noidx", "-q", "c2/qry", []),
+    # (as many queries as references: the index path when the index exists)
+    "c2_best_index": ("c2/ref", "-q", "c2/qry", [], {}),
+    "c2_s3_index": ("c2/ref", "-q", "c2/qry", ["-s3", "-n0", "-f0", "-t1"], {}),
+    "c2_best_noindex": ("c2/ref_noidx", "-q", "c2/qry", [], {}),
     # C3 recipe: 10 clades of 20 references (members U(1%, 15%) from the
     # ancestor) x 300 new clade members; the index path with its defaults,
-    # looser cuts, and unique contexts
-    "c3_best": ("c3/ref", "-q", "c3/qry", []),
-    "c3_s3_loose": ("c3/ref", "-q", "c3/qry", ["-s3", "-n", "0.9", "-f", "0.1"]),
-    "c3_s4_all": ("c3/ref", "-q", "c3/qry", ["-s4", "-n0", "-f0", "-t1"]),
-    "c3_qraw_unique": ("c3/ref", "--qraw", "c3/qry", ["--unique-ctx", "-n0", "-f0", "-t1"]),
-    # heap top-N with many tied ANIs: 200 references (40 variants x 5 copies)
-    # all sharing contexts with every query, -N 3 (m > 128, N < m / 2)
-    "p2_top3": ("p2/ref", "-q", "p2/qry", ["-N", "3", "-n0", "-f0", "-t1"]),
-    "p2_top3_s3": ("p2/ref", "-q", "p2/qry", ["-s3", "-N", "3", "-n0", "-f0", "-t1"]),
-    "p2_top7_s4": ("p2/ref", "-q", "p2/qry", ["-s4", "-N", "7", "-n0", "-f0", "-t1"]),
-    "p2_top120": ("p2/ref", "-q", "p2/qry", ["-N", "120", "-n0", "-f0", "-t1"]),
-    "p2_all_s3": ("p2/ref", "-q", "p2/qry", ["-s3", "-n0", "-f0", "-t1"]),
+    # looser cuts, all pairs, and unique contexts
+    "c3_best": ("c3/ref", "-q", "c3/qry", [], {}),
+    "c3_s3_loose": ("c3/ref", "-q", "c3/qry", ["-s3", "-n", "0.9", "-f", "0.1"], {}),
+    "c3_s4_all": ("c3/ref", "-q", "c3/qry", ["-s4", "-n0", "-f0", "-t1"], {}),
+    "c3_qraw_unique": ("c3/ref", "--qraw", "c3/qry", ["--unique-ctx", "-n0", "-f0", "-t1"], {}),
+    # many tied ANIs: 200 references (40 variants x 5 copies) all sharing
+    # contexts with each of 12 queries.  Fewer queries than references take
+    # the multi-query stream path; with the index forced, -N 3 / 7 take the
+    # heap branch of idani_select_topN_sorted (m > 128, N < m / 2), -N 120
+    # and the full listing its stable sort
+    "p2_top3": ("p2/ref", "-q", "p2/qry", ["-N", "3", "-n0", "-f0", "-t1"], {}),
+    "p2_top3_s3": ("p2/ref", "-q", "p2/qry", ["-s3", "-N", "3", "-n0", "-f0", "-t1"], {}),
+    "p2_index_top3": ("p2/ref", "-q", "p2/qry", ["-N", "3", "-n0", "-f0", "-t1"], FORCE),
+    "p2_index_top3_s3": ("p2/ref", "-q", "p2/qry", ["-s3", "-N", "3", "-n0", "-f0", "-t1"], FORCE),
+    "p2_index_top7_s4": ("p2/ref", "-q", "p2/qry", ["-s4", "-N", "7", "-n0", "-f0", "-t1"], FORCE),
+    "p2_index_top120": ("p2/ref", "-q", "p2/qry", ["-N", "120", "-n0", "-f0", "-t1"], FORCE),
+    "p2_index_all_s3": ("p2/ref", "-q", "p2/qry", ["-s3", "-n0", "-f0", "-t1"], FORCE),
+    "p2_index_unique_top3": ("p2/ref", "--qraw", "p2/qry", ["--unique-ctx", "-N", "3", "-n0", "-f0", "-t1"], {}),
     # the reference's block boundary (tests/test_unique_context.py:216-223):
     # 4096 one-context references, 310 queries, every third one empty
-    "boundary_unique": ("boundary/ref", "--qraw", "boundary/qry", ["--unique-ctx", "-n0", "-f0", "-t1", "-p4"]),
-    "boundary_detail": ("boundary/ref", "-q", "boundary/qry", ["-n0", "-f0", "-t1"]),
+    "boundary_unique": ("boundary/ref", "--qraw", "boundary/qry", ["--unique-ctx", "-n0", "-f0", "-t1", "-p4"], {}),
+    "boundary_detail": ("boundary/ref", "-q", "boundary/qry", ["-n0", "-f0", "-t1"], {}),
+    "boundary_index_detail": ("boundary/ref", "-q", "boundary/qry", ["-n0", "-f0", "-t1"], FORCE),
 }
 
 
@@ -543,8 +552,8 @@ def make_scale():
     import gzip
     import json
     (out / "cases.json").write_text(json.dumps(SCALE_CASES, indent=1, sort_keys=True) + "\n")
-    for name, (r, qflag, q, args) in SCALE_CASES.items():
-        text = run(["ani", "-r", r, qflag, q, *args], cwd=out).encode()
+    for name, (r, qflag, q, args, env) in SCALE_CASES.items():
+        text = run(["ani", "-r", r, qflag, q, *args], env=env, cwd=out).encode()
         if len(text) > 100_000:  # large reports kept compressed
             with gzip.GzipFile(out / f"{name}.tsv.gz", "wb", mtime=0) as fh:
                 fh.write(text)

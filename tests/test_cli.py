@@ -261,11 +261,14 @@ def test_cli_scale_cases_match_reference(tmp_path, name):
     """Down-scaled C2 and C3 workloads, the heap top-N with ties and the
     block boundary with empty queries, each byte-identical to the reference's
     report (tests/golden/make_golden.py, make_scale)."""
-    ref, qflag, qry, args = SCALE_CASES[name]
+    ref, qflag, qry, args, env = SCALE_CASES[name]
     top = ref.split("/")[0]
     shutil.copytree(SCALE / top, tmp_path / top)
+    e = dict(os.environ)
+    e.pop("KSSD3A_FORCE_REF_INDEX", None)
+    e.update(env)
     p = subprocess.run([str(BIN), "ani", "-r", ref, qflag, qry, *args], capture_output=True, text=True,
-                       cwd=tmp_path, timeout=300)
+                       cwd=tmp_path, env=e, timeout=300)
     assert p.returncode == 0, p.stderr
     assert p.stdout == _golden_text(name)
 

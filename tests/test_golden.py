@@ -146,3 +146,33 @@ def test_self_matrix_paths_agree(oracle):
                 ani = 1.0 - 1.0  # af gate: moe distance is the exception value e=1
             scaled = int(ani * 1e6 + 0.5) if ani >= 0 else int(ani * 1e6 - 0.5)
             assert cells[1 + gi] == "%d.%06d" % divmod(scaled, 1000000), (qi, gi)
+
+
+SCALE = kt.GOLDEN / "scale"
+
+
+def _scale_text(stem):
+    import gzip
+    f = SCALE / f"{stem}.tsv"
+    return f.read_text() if f.exists() else gzip.decompress((SCALE / f"{stem}.tsv.gz").read_bytes()).decode()
+
+
+@pytest.mark.parametrize("stem,ntop", [("p2_index_top3", 3), ("p2_index_top3_s3", 3), ("p2_index_top7_s4", 7),
+                                       ("p2_index_top120", 120), ("p2_index_all_s3", 0)])
+def test_index_rank_against_reference_order(oracle, stem, ntop):
+    """The oracle's restatement of the index path's gate and top-N
+    (command_ani.h:577-621) gives, per query, exactly the references the
+    reference printed, in its order: 200 references of 40 tied variants,
+    so -N 3 / 7 take the heap branch (m > 128, N < m / 2) and -N 120 and
+    the full listing the stable sort; with -n0 every ranked row prints."""
+    ref = kt.read_sketch_dir(SCALE / "p2" / "ref")
+    qry = kt.read_sketch_dir(SCALE / "p2" / "qry")
+    ctx, obj, uctx, uobj = oracle.count_block(ref.records, ref.n, ref.obj_bits, qry.keys, qry.index, threads=4)
+    ranked = oracle.index_rank(ctx, obj, uctx, uobj, np.diff(ref.index), np.diff(qry.index), ctxcut=1, afcut=0.0,
+                               anicut=0.0, ntop=ntop)
+    rows = list(csv.DictReader(io.StringIO(_scale_text(stem)), delimiter="\t"))
+    for q, (_, _, printed) in enumerate(ranked):
+        got = [ref.names[i] for i in printed]
+        want = [r["Ref"] for r in rows if r["Qry"] == qry.names[q]]
+        assert got == want, (stem, q)
+    assert ntop == 0 or all(len(p[2]) == ntop for p in ranked)
