@@ -141,10 +141,10 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  *          chunk_records records staged in LDS; queries are split at the
  *          partition boundaries and merged against the staged records;
  *        3 query-major: persistent workgroups take whole queries from a
- *          device work queue, look each context up through the index's
- *          fingerprinted bucket table and sum the query's counters in an
- *          LDS table, then write the query's row once (no counter atomics
- *          in HBM).
+ *          device work queue (largest first), look each context up through
+ *          the index's context filter and open-addressed context table and
+ *          sum the query's counters in an LDS table, then write the query's
+ *          row once (no counter atomics in HBM).
  * chunk_records (partitions are rebuilt when it changes) must leave the
  * staged partition within the 160 KB LDS: 64 <= chunk_records <= 12288.
  * group_partitions (1..64): consecutive partitions one workgroup walks;
@@ -360,7 +360,9 @@ int kssdhip_select_block(kssdhip_index *index, const uint64_t *section_sk, const
  * (rows past cap are not stored).  The call zeroes its counters with
  * kernels and, once the handle's workspace has grown to the block's size,
  * allocates nothing, so it can be captured in a HIP graph (capturing a call
- * that would grow the workspace fails with KSSDHIP_EINVAL).
+ * that would grow the workspace fails with KSSDHIP_EINVAL).  A graph holds
+ * the workspace's address: after a call on the same handle that grows it
+ * (a larger block, another tiling) the graph must not be replayed again.
  */
 int kssdhip_select_block_dev(kssdhip_index *index, const uint64_t *d_keys, const uint64_t *d_skidx, int section_gnum,
 			     uint64_t total_keys, uint32_t flags, const uint32_t *d_qry_ctx, const uint32_t *d_ref_ctx,

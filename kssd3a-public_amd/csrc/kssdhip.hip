@@ -436,6 +436,7 @@ int DevBuf::reserve(size_t want)
 		(void)hipFree(ptr);
 	ptr = nullptr;
 	bytes = 0;
+	zero_bytes = 0;
 	HIP_TRY(hipMalloc(&ptr, want));
 	bytes = want;
 	return 0;
@@ -460,6 +461,7 @@ void DevBuf::release()
 		(void)hipFree(ptr);
 	ptr = nullptr;
 	bytes = 0;
+	zero_bytes = 0;
 }
 
 static int bit_width_u64(uint64_t v)

@@ -80,6 +80,9 @@ struct PairWorkspace {
 struct DevBuf {
 	void *ptr = nullptr;
 	size_t bytes = 0;
+	// bytes past a user's fixed header that the user keeps zero at rest
+	// (reset when the buffer is reallocated)
+	size_t zero_bytes = 0;
 	int reserve(size_t want);
 	void release();
 };

@@ -7,11 +7,12 @@
 // the context's per-genome runs and adds into the query's own row of the
 // dense counters (command_ani.h:481-575).  Here a persistent workgroup takes
 // whole queries from a device work queue.  Its threads stride over the
-// query's keys; the first key of each context group looks the context up in
-// a bucket table whose entries carry a 32-bit fingerprint mask of the
-// contexts in the bucket (most query contexts are absent from the index and
-// stop there), scans the bucket's 16-byte {ctxgid, obj} records, walks the
-// runs and adds each pair's counters into an LDS table of the query (one
+// query's keys; the first key of each context group tests the context
+// against a word-blocked Bloom filter of the index's contexts (most query
+// contexts are absent from the index and stop there), finds the context's
+// first record and run length in an open-addressed context table, walks the
+// context's 16-byte {ctxgid, obj} records genome run by genome run and adds
+// each pair's counters into an LDS table of the query (one
 // cell per genome when the reference set is small enough, else an
 // open-addressed table keyed by genome; a genome that finds no slot goes to
 // the workgroup's spill row in HBM).  When the query is done the workgroup
@@ -42,8 +43,8 @@ constexpr uint32_t DIRECT_LDS_MAX = 64u * 1024u; // direct per-genome rows up to
 constexpr int DEFAULT_THREADS = 512;
 constexpr uint32_t DEFAULT_SLOTS = 2048;
 // workspace: [0, WS_HEAD) work counter and the order's histogram and
-// cursors (65 words), then the work order (nq words, WS_ALIGN-rounded),
-// then the spill rows
+// cursors (65 words), then the spill rows, then the work order (nq words,
+// WS_ALIGN-rounded)
 constexpr size_t WS_HEAD = 512;
 constexpr uint32_t WS_HEAD_WORDS = 65;
 constexpr size_t WS_ALIGN = 256;
@@ -830,36 +831,65 @@ int qj_shape(const kssdhip_index *ix, uint32_t nq, bool uniq, QJShape *sh)
 	return 0;
 }
 
-// The call's workspace: the work counter and the order's histogram, the
-// work order, then the spill rows (zero at rest; zeroed when the buffer
-// grows).  Leaves the counter and histogram zeroed and the order built, all
-// by kernels on s.
+// The call's workspace: [0, WS_HEAD) the work counter and the order's
+// histogram and cursors, then the spill rows (zero at rest: a workgroup
+// clears the cells it spilled), then the work order.  The spill part never
+// moves and only grows (ws->zero_bytes), so the order of a later, smaller
+// block cannot land on cells that must read zero.  Leaves the counter and
+// histogram zeroed and the order built, all by kernels on s.
 struct QJWork {
 	uint32_t *head;
 	uint32_t *order;
 	unsigned long long *spill;
 };
 
+size_t qj_order_bytes(uint32_t nq)
+{
+	return ((size_t)nq * 4 + WS_ALIGN - 1) / WS_ALIGN * WS_ALIGN;
+}
+
+void qj_zero(void *p, size_t bytes, hipStream_t s)
+{
+	const uint64_t words = bytes / 8;
+	if (words)
+		hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 8192)), dim3(256), 0, s,
+				   (unsigned long long *)p, words);
+}
+
+// zeroes the head and builds the work order (largest queries first)
+void qj_order(const uint64_t *d_skidx, uint32_t nq, hipStream_t s, const QJWork &w)
+{
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s, w.head, WS_HEAD_WORDS);
+	const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nq + 255) / 256, 2048));
+	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, d_skidx, nq, w.head);
+	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, w.head);
+	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, d_skidx, nq, w.head, w.order);
+}
+
 int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_t nq, hipStream_t s, QJWork *w)
 {
-	const size_t order_bytes = ((size_t)nq * 4 + WS_ALIGN - 1) / WS_ALIGN * WS_ALIGN;
-	const size_t want = WS_HEAD + order_bytes + sh.spill_bytes;
+	const size_t spill = std::max(ws->zero_bytes, sh.spill_bytes);
+	const size_t want = WS_HEAD + spill + qj_order_bytes(nq);
 	if (want > ws->bytes) {
 		int rc = kssd_reserve_for_stream(ws, want, s, "query-major workspace");
 		if (rc)
 			return rc;
-		const uint64_t words = ws->bytes / 8;
-		hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 8192)), dim3(256), 0, s,
-				   (unsigned long long *)ws->ptr, words);
+	}
+	if (spill > ws->zero_bytes) {
+		// kernels, but the cleared state must outlive the call: not in a capture
+		hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+		if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+			return kssdhip_fail(KSSDHIP_EINVAL,
+					    "query-major workspace: %zu spill bytes to clear while the stream is being "
+					    "captured; run the call once outside the capture first",
+					    spill);
+		qj_zero((char *)ws->ptr + WS_HEAD, spill, s);
+		ws->zero_bytes = spill;
 	}
 	w->head = (uint32_t *)ws->ptr;
-	w->order = (uint32_t *)((char *)ws->ptr + WS_HEAD);
-	w->spill = (unsigned long long *)((char *)ws->ptr + WS_HEAD + order_bytes);
-	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s, w->head, WS_HEAD_WORDS);
-	const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nq + 255) / 256, 2048));
-	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, d_skidx, nq, w->head);
-	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, w->head);
-	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, d_skidx, nq, w->head, w->order);
+	w->spill = (unsigned long long *)((char *)ws->ptr + WS_HEAD);
+	w->order = (uint32_t *)((char *)ws->ptr + WS_HEAD + spill);
+	qj_order(d_skidx, nq, s, *w);
 	return 0;
 }
 
@@ -1035,85 +1065,125 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 	return qj_launch(sh, uniq, a, s);
 }
 
-extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq,
-					uint64_t total, uint32_t flags, const uint32_t *d_qry_ctx,
-					const uint32_t *d_ref_ctx, const kssdhip_select_opts_t *opts,
-					kssdhip_ani_row_t *d_rows, uint32_t cap, uint32_t *d_row_count,
-					uint32_t *d_q_first, uint32_t *d_q_count, void *stream)
+namespace {
+
+struct SelectCall {
+	const uint64_t *keys, *skidx;
+	int nq;
+	uint64_t total;
+	uint32_t flags;
+	const uint32_t *qry_ctx, *ref_ctx;
+	const kssdhip_select_opts_t *opts;
+	kssdhip_ani_row_t *rows;
+	uint32_t cap;
+	uint32_t *row_count, *q_first, *q_count;
+	hipStream_t s;
+};
+
+int select_check(const kssdhip_index *ix, const SelectCall &c)
 {
-	if (!ix || !opts || nq < 0 || !d_row_count)
+	if (!ix || !c.opts || c.nq < 0 || !c.row_count)
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: bad arguments");
-	if (flags & ~(KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT))
-		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: flags 0x%x (UNIQUE and IGNORECONFLICT only)", flags);
-	if (opts->ctxcut < 1)
+	if (c.flags & ~(KSSDHIP_UNIQUE | KSSDHIP_IGNORECONFLICT))
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: flags 0x%x (UNIQUE and IGNORECONFLICT only)", c.flags);
+	if (c.opts->ctxcut < 1)
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: ctxcut must be >= 1");
-	if (opts->unique_rank && !(flags & KSSDHIP_UNIQUE))
+	if (c.opts->unique_rank && !(c.flags & KSSDHIP_UNIQUE))
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: unique_rank needs KSSDHIP_UNIQUE");
-	if (!epi_metric_supported(opts->value))
+	if (!epi_metric_supported(c.opts->value))
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: value metric %d needs the calibration model (use 3..9)",
-				    opts->value.metric);
-	if (nq > 0 && (!d_skidx || !d_qry_ctx || !d_ref_ctx || !d_q_first || !d_q_count || (cap && !d_rows) ||
-		       (total && !d_keys)))
+				    c.opts->value.metric);
+	if (c.nq > 0 && (!c.skidx || !c.qry_ctx || !c.ref_ctx || !c.q_first || !c.q_count || (c.cap && !c.rows) ||
+			 (c.total && !c.keys)))
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: null device pointer");
 	if (!ix->d_qrec)
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: the index has no query-major tables");
-	hipStream_t s = (hipStream_t)stream;
-	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_row_count, 1u);
-	if (nq == 0) {
-		hipError_t e = hipGetLastError();
-		return e == hipSuccess ? 0 : kssdhip_fail(KSSDHIP_EHIP, "select_block: %s", hipGetErrorString(e));
-	}
-	const bool uniq = (flags & KSSDHIP_UNIQUE) != 0;
-	QJShape sh;
-	int rc = qj_shape(ix, (uint32_t)nq, uniq, &sh);
-	if (rc)
-		return rc;
-	QJWork w;
-	if ((rc = qj_workspace(&ix->qws, sh, d_skidx, (uint32_t)nq, s, &w)))
-		return rc;
+	return 0;
+}
+
+// the join in select mode and the rows' values, with the work order built
+int select_launch(const kssdhip_index *ix, const SelectCall &c, const QJShape &sh, const QJWork &w)
+{
+	const bool uniq = (c.flags & KSSDHIP_UNIQUE) != 0;
 	QJArgs a;
 	memset(&a, 0, sizeof a);
 	fill_index_args(ix, a);
-	a.keys = d_keys;
-	a.skidx = d_skidx;
-	a.nq = (uint32_t)nq;
-	a.flags = flags;
+	a.keys = c.keys;
+	a.skidx = c.skidx;
+	a.nq = (uint32_t)c.nq;
+	a.flags = c.flags;
 	a.slots = sh.slots;
 	a.sbits = sh.sbits;
 	a.next = w.head;
 	a.order = w.order;
 	a.spill = w.spill;
 	a.mode = QJ_SELECT;
-	a.qry_ctx = d_qry_ctx;
-	a.ref_ctx = d_ref_ctx;
-	a.ctxcut = opts->ctxcut;
-	a.afcut = (double)opts->afcut;
-	a.anicut = opts->anicut;
-	a.unique_rank = opts->unique_rank;
-	a.vo = opts->value;
-	a.vmetric = 0;
+	a.qry_ctx = c.qry_ctx;
+	a.ref_ctx = c.ref_ctx;
+	a.ctxcut = c.opts->ctxcut;
+	a.afcut = (double)c.opts->afcut;
+	a.anicut = c.opts->anicut;
+	a.unique_rank = c.opts->unique_rank;
+	a.vo = c.opts->value;
 	{
 		int m = a.vo.metric;
 		if (m < 1 || m > 9)
 			m = 1;
 		a.vmetric = (a.vo.unassembled && m <= 4) ? 4 : m;
 	}
-	a.rows = d_rows;
-	a.cap = cap;
-	a.row_count = d_row_count;
-	a.q_first = d_q_first;
-	a.q_count = d_q_count;
-	if ((rc = qj_launch(sh, uniq, a, s)))
+	a.rows = c.rows;
+	a.cap = c.cap;
+	a.row_count = c.row_count;
+	a.q_first = c.q_first;
+	a.q_count = c.q_count;
+	int rc = qj_launch(sh, uniq, a, c.s);
+	if (rc)
 		return rc;
-	if (cap) {
-		const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((cap + 255) / 256, 1024));
-		hipLaunchKernelGGL(k_row_values, dim3(g), dim3(256), 0, s, d_rows, d_row_count, cap, d_qry_ctx, d_ref_ctx,
-				   a.vo, a.vmetric);
+	if (c.cap) {
+		const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((c.cap + 255) / 256, 1024));
+		hipLaunchKernelGGL(k_row_values, dim3(g), dim3(256), 0, c.s, c.rows, c.row_count, c.cap, c.qry_ctx,
+				   c.ref_ctx, a.vo, a.vmetric);
 		hipError_t e = hipGetLastError();
 		if (e != hipSuccess)
 			return kssdhip_fail(KSSDHIP_EHIP, "select_block values: %s", hipGetErrorString(e));
 	}
 	return 0;
+}
+
+// zeroes the row count; true when the block is empty (nothing else to do)
+int select_begin(const SelectCall &c, bool *done)
+{
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c.s, c.row_count, 1u);
+	*done = c.nq == 0;
+	if (*done) {
+		hipError_t e = hipGetLastError();
+		if (e != hipSuccess)
+			return kssdhip_fail(KSSDHIP_EHIP, "select_block: %s", hipGetErrorString(e));
+	}
+	return 0;
+}
+
+} // namespace
+
+extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq,
+					uint64_t total, uint32_t flags, const uint32_t *d_qry_ctx,
+					const uint32_t *d_ref_ctx, const kssdhip_select_opts_t *opts,
+					kssdhip_ani_row_t *d_rows, uint32_t cap, uint32_t *d_row_count,
+					uint32_t *d_q_first, uint32_t *d_q_count, void *stream)
+{
+	const SelectCall c{d_keys, d_skidx, nq, total, flags, d_qry_ctx, d_ref_ctx, opts,
+			   d_rows, cap, d_row_count, d_q_first, d_q_count, (hipStream_t)stream};
+	int rc = select_check(ix, c);
+	bool done = false;
+	if (rc || (rc = select_begin(c, &done)) || done)
+		return rc;
+	QJShape sh;
+	if ((rc = qj_shape(ix, (uint32_t)nq, (flags & KSSDHIP_UNIQUE) != 0, &sh)))
+		return rc;
+	QJWork w;
+	if ((rc = qj_workspace(&ix->qws, sh, d_skidx, (uint32_t)nq, c.s, &w)))
+		return rc;
+	return select_launch(ix, c, sh, w);
 }
 
 extern "C" int kssdhip_select_block(kssdhip_index *ix, const uint64_t *section_sk, const uint64_t *section_skidx,

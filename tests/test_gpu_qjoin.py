@@ -303,3 +303,38 @@ def test_matched_records(hip, wide_cases, shape):
         rt.free(d_k)
         rt.free(d_i)
         hip.close_index(h)
+
+
+@pytest.mark.parametrize("select", [False, True])
+def test_smaller_block_after_larger_on_one_handle(hip, oracle, wide_cases, select):
+    """The handle's workspace keeps its spill rows where they were when a
+    later block has fewer queries (a shorter work order): a 96-query block
+    (the spill shape's queries four times over), then its first 24 queries,
+    then 7, each against the oracle."""
+    rk, ri, qk, qi, rec, slots = wide_cases("spill")
+    R, Q = len(ri) - 1, len(qi) - 1
+    sizes = [np.diff(qi)] * 4
+    big_k = np.concatenate([qk] * 4)
+    big_i = np.zeros(4 * Q + 1, np.uint64)
+    big_i[1:] = np.cumsum(np.concatenate(sizes))
+    rctx = np.diff(ri).astype(np.uint32)
+    opts = kt.SelectOpts(1, 0.0, 0.0, 0, kt.EpilogueOpts(3, 1, 0, 0, 0, 0, 8, OBJ_BITS, 44, 0.5, 1.0))
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=3, query_slots=slots, query_threads=256)
+        for nq in (4 * Q, Q, 7):
+            bk, bi = big_k[: int(big_i[nq])], big_i[: nq + 1]
+            ctx, obj, _, _ = oracle.count_block(rec, R, OBJ_BITS, bk, bi)
+            if select:
+                qctx = np.diff(bi).astype(np.uint32)
+                rows = hip.select_block(h, bk, bi, qctx, rctx, opts)
+                ranked = oracle.index_rank(ctx, obj, ctx, obj, rctx, qctx, ctxcut=1, afcut=0.0, anicut=0.0)
+                q = np.concatenate([np.full(len(p[0]), i, np.uint32) for i, p in enumerate(ranked)])
+                r = np.concatenate([p[0] for p in ranked]).astype(np.int64)
+                assert np.array_equal(rows["query"], q) and np.array_equal(rows["ref"], r), nq
+                assert np.array_equal(rows["num_ctx"], ctx[q, r, 0]) and np.array_equal(rows["diff_obj"], obj[q, r, 0])
+            else:
+                got = hip.count_block(h, R, bk, bi)
+                assert np.array_equal(got[0], ctx) and np.array_equal(got[1], obj), nq
+    finally:
+        hip.close_index(h)
