@@ -370,6 +370,31 @@ int kssdhip_select_block_dev(kssdhip_index *index, const uint64_t *d_keys, const
 			     uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, void *stream);
 
 /*
+ * kssdhip_select_block_dev with no allocation at all: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results; the work queue, work order and spill rows live in
+ * one file-scope device array of KSSDHIP_NOALLOC_SCRATCH_BYTES per device
+ * (allocated when the library's device code loads and held until the
+ * process ends) instead of the handle's grow-only workspace.  Limits:
+ * section_gnum <= KSSDHIP_NOALLOC_MAX_QUERIES, and with reference sets too
+ * large for per-genome LDS rows (ref_gnum * 16 bytes, or 32 with
+ * KSSDHIP_UNIQUE, above 64 KiB) the kernel runs as many workgroups as
+ * spill rows of ref_gnum cells fit in the array's spill part, at least
+ * one; larger requests fail with KSSDHIP_EINVAL before anything is
+ * launched.  Every call on a device shares that device's array and clears
+ * what it needs zeroed with kernels; calls that overlap on one device (any
+ * handles or streams) race and must be serialised by the caller.
+ */
+#define KSSDHIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define KSSDHIP_NOALLOC_MAX_QUERIES (1u << 20)
+int kssdhip_select_block_dev_noalloc(kssdhip_index *index, const uint64_t *d_keys, const uint64_t *d_skidx,
+				     int section_gnum, uint64_t total_keys, uint32_t flags, const uint32_t *d_qry_ctx,
+				     const uint32_t *d_ref_ctx, const kssdhip_select_opts_t *opts,
+				     kssdhip_ani_row_t *d_rows, uint32_t cap, uint32_t *d_row_count, uint32_t *d_q_first,
+				     uint32_t *d_q_count, void *stream);
+
+/*
  * Diagnostic: the number of index records whose context occurs in a
  * device-resident block (relative offsets as kssdhip_count_block_dev),
  * i.e. the records a join of the block must read at least once.  No

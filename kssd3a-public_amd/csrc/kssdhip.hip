@@ -74,6 +74,9 @@ extern "C" int kssdhip_init(int device)
 	HIP_TRY(hipSetDevice(device));
 	g_device = device;
 	g_err[0] = '\0';
+	int rc = kssd_noalloc_setup(device);
+	if (rc)
+		return rc;
 	if (device < 64) {
 		std::lock_guard<std::mutex> lock(g_spare_mu);
 		if (!g_spare_made[device]) {

@@ -158,6 +158,9 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const kssd::JoinArgs &a, hipStrea
 // true when a block of this shape should take the query-major kernel in
 // auto mode
 bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total);
+// looks up the address of the no-allocation scratch array on `device`
+// (current on the calling thread) once per device
+int kssd_noalloc_setup(int device);
 // fails with KSSDHIP_EINVAL when `want` bytes would grow `b` while `s` is
 // being captured into a graph, else reserves them
 int kssd_reserve_for_stream(DevBuf *b, size_t want, hipStream_t s, const char *what);

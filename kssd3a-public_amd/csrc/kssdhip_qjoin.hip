@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "kssdhip.h"
@@ -1163,7 +1164,64 @@ int select_begin(const SelectCall &c, bool *done)
 	return 0;
 }
 
+// The no-allocation workspace: one array per device, at fixed offsets:
+// [0, WS_HEAD) the head, then the work order of up to
+// KSSDHIP_NOALLOC_MAX_QUERIES queries, then the spill rows.  Host code
+// never uses the array's name as an address: kssd_noalloc_setup looks the
+// device copy up once per device and the entry point passes that pointer.
+constexpr size_t NOALLOC_ORDER = (size_t)KSSDHIP_NOALLOC_MAX_QUERIES * 4;
+constexpr size_t NOALLOC_SPILL_AT = WS_HEAD + NOALLOC_ORDER;
+static_assert(NOALLOC_SPILL_AT % WS_ALIGN == 0, "spill rows stay aligned");
+static_assert(NOALLOC_SPILL_AT < KSSDHIP_NOALLOC_SCRATCH_BYTES, "scratch holds the head and the order");
+constexpr size_t NOALLOC_SPILL = KSSDHIP_NOALLOC_SCRATCH_BYTES - NOALLOC_SPILL_AT;
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch; // device copy of `scratch` per device (nullptr: not looked up)
+
 } // namespace
+
+static __device__ __align__(256) unsigned char scratch[KSSDHIP_NOALLOC_SCRATCH_BYTES];
+
+int kssd_noalloc_setup(int device)
+{
+	if (device < 0)
+		return kssdhip_fail(KSSDHIP_EINVAL, "scratch: device %d", device);
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if ((size_t)device < g_scratch.size() && g_scratch[device])
+		return 0;
+	void *ptr = nullptr;
+	hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+	if (e != hipSuccess || !ptr)
+		return kssdhip_fail(KSSDHIP_EHIP, "scratch: address lookup on device %d failed: %s", device,
+				    hipGetErrorString(e));
+	if ((size_t)device >= g_scratch.size())
+		g_scratch.resize((size_t)device + 1, nullptr);
+	g_scratch[device] = ptr;
+	return 0;
+}
+
+// the device copy of `scratch` on `device`, looked up on first use with
+// that device current (kssdhip_init does it for its device)
+static int noalloc_scratch(int device, void **out)
+{
+	{
+		std::lock_guard<std::mutex> lock(g_scratch_mu);
+		if ((size_t)device < g_scratch.size() && g_scratch[device]) {
+			*out = g_scratch[device];
+			return 0;
+		}
+	}
+	int prev = 0;
+	if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "scratch: device %d not available", device);
+	int rc = kssd_noalloc_setup(device);
+	(void)hipSetDevice(prev);
+	if (rc)
+		return rc;
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	*out = g_scratch[device];
+	return 0;
+}
 
 extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq,
 					uint64_t total, uint32_t flags, const uint32_t *d_qry_ctx,
@@ -1183,6 +1241,51 @@ extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_key
 	QJWork w;
 	if ((rc = qj_workspace(&ix->qws, sh, d_skidx, (uint32_t)nq, c.s, &w)))
 		return rc;
+	return select_launch(ix, c, sh, w);
+}
+
+extern "C" int kssdhip_select_block_dev_noalloc(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx,
+						int nq, uint64_t total, uint32_t flags, const uint32_t *d_qry_ctx,
+						const uint32_t *d_ref_ctx, const kssdhip_select_opts_t *opts,
+						kssdhip_ani_row_t *d_rows, uint32_t cap, uint32_t *d_row_count,
+						uint32_t *d_q_first, uint32_t *d_q_count, void *stream)
+{
+	const SelectCall c{d_keys, d_skidx, nq, total, flags, d_qry_ctx, d_ref_ctx, opts,
+			   d_rows, cap, d_row_count, d_q_first, d_q_count, (hipStream_t)stream};
+	int rc = select_check(ix, c);
+	if (rc)
+		return rc;
+	if ((uint32_t)nq > KSSDHIP_NOALLOC_MAX_QUERIES)
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block_noalloc: %d queries; the scratch array holds the work "
+				    "order of at most %u", nq, KSSDHIP_NOALLOC_MAX_QUERIES);
+	const bool uniq = (flags & KSSDHIP_UNIQUE) != 0;
+	QJShape sh;
+	if (nq > 0 && (rc = qj_shape(ix, (uint32_t)nq, uniq, &sh)))
+		return rc;
+	if (nq > 0 && !sh.direct) {
+		const size_t row = (size_t)ix->ref_gnum * (uniq ? 32u : 16u);
+		const size_t fit = NOALLOC_SPILL / row;
+		if (fit < 1)
+			return kssdhip_fail(KSSDHIP_EINVAL, "select_block_noalloc: a spill row of %d references (%zu bytes) "
+					    "exceeds the scratch array's %zu spill bytes", ix->ref_gnum, row,
+					    NOALLOC_SPILL);
+		if (sh.grid > fit)
+			sh.grid = (unsigned)fit;
+		sh.spill_bytes = (size_t)sh.grid * row;
+	}
+	void *base = nullptr;
+	if ((rc = noalloc_scratch(ix->device, &base)))
+		return rc;
+	bool done = false;
+	if ((rc = select_begin(c, &done)) || done)
+		return rc;
+	QJWork w;
+	w.head = (uint32_t *)base;
+	w.order = (uint32_t *)((char *)base + WS_HEAD);
+	w.spill = (unsigned long long *)((char *)base + NOALLOC_SPILL_AT);
+	if (!sh.direct)
+		qj_zero(w.spill, sh.spill_bytes, c.s); // the last call's contents stay in the array
+	qj_order(d_skidx, (uint32_t)nq, c.s, w);
 	return select_launch(ix, c, sh, w);
 }
 

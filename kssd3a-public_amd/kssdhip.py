@@ -64,6 +64,8 @@ ROW_DTYPE = np.dtype([("query", "<u4"), ("ref", "<u4"), ("num_ctx", "<u4"), ("nu
                       ("rank_ani", "<f8"), ("value", "<f8")])
 assert ROW_DTYPE.itemsize == 56
 ENOSPC = -6
+NOALLOC_SCRATCH_BYTES = 64 << 20
+NOALLOC_MAX_QUERIES = 1 << 20
 
 
 class KssdStat(ctypes.Structure):
@@ -294,6 +296,7 @@ def load_kssdhip():
                                          ctypes.POINTER(S)]
     lib.kssdhip_select_block_dev.argtypes = [V, V, V, I, U64, U32, V, V, ctypes.POINTER(SelectOpts), V, U32,
                                              V, V, V, V]
+    lib.kssdhip_select_block_dev_noalloc.argtypes = lib.kssdhip_select_block_dev.argtypes
     lib.kssdhip_serve_open.argtypes = [V, ctypes.POINTER(ServeOpts), ctypes.POINTER(V)]
     lib.kssdhip_serve_submit.argtypes = [V, V, S]
     lib.kssdhip_serve_submit.restype = ctypes.c_int64
@@ -465,6 +468,15 @@ class KssdHip:
         self.check(self.lib.kssdhip_select_block_dev(h, d_keys, d_skidx, nq, total, flags, d_qctx, d_rctx,
                                                      ctypes.byref(opts), d_rows, cap, d_count, d_first, d_qcount,
                                                      stream))
+
+    def select_block_dev_noalloc(self, h, d_keys, d_skidx, nq, total, flags, d_qctx, d_rctx, opts, d_rows, cap,
+                                 d_count, d_first, d_qcount, stream=None):
+        """select_block_dev with its workspace in the library's static
+        per-device scratch array (no allocation; calls on one device must
+        not overlap)."""
+        self.check(self.lib.kssdhip_select_block_dev_noalloc(h, d_keys, d_skidx, nq, total, flags, d_qctx, d_rctx,
+                                                             ctypes.byref(opts), d_rows, cap, d_count, d_first,
+                                                             d_qcount, stream))
 
     def matched_records(self, h, d_keys, d_skidx, nq, total):
         m = U64()

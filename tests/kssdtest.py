@@ -22,7 +22,7 @@ ROOT = Path(__file__).resolve().parents[1]
 PKG = ROOT / "kssd3a-public_amd"
 sys.path.insert(0, str(PKG))
 from kssdhip import (Features, Host, Hip, KssdHip, EpilogueOpts, SimSpec, HIT_DTYPE,  # noqa: E402,F401
-                     SelectOpts, ROW_DTYPE, LOWER, UNIQUE, IGNORECONFLICT, ACCUMULATE,
+                     SelectOpts, ROW_DTYPE, LOWER, UNIQUE, IGNORECONFLICT, ACCUMULATE, NOALLOC_MAX_QUERIES,
                      load_kssdhip, load_kssdhost)
 GOLDEN = ROOT / "tests" / "golden"
 KSSDHIP_EINVAL, KSSDHIP_EAGAIN = -1, -5
