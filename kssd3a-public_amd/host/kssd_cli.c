@@ -830,6 +830,30 @@ static void emit_rows(FILE *fp, const ani_args_t *a, detail_path_t path, const k
 {
 	const int R = ref->n;
 	kssd_buf_t *bufs = xcalloc((size_t)nq, sizeof *bufs);
+	uint64_t *last_ctx = NULL; /* one-query stream: the last context each reference's walk counts */
+	bool *has_last = NULL;
+	if (path == PATH_STREAM) {
+		last_ctx = xcalloc((size_t)(R ? R : 1), sizeof *last_ctx);
+		has_last = xcalloc((size_t)(R ? R : 1), sizeof *has_last);
+		const bool ignore = a->r.ignoreconflict && ref->stat.conflict;
+		const unsigned ob = ref->obj_bits;
+		for (int j = 0; j < R; ++j) {
+			size_t e = (size_t)ref->index[j + 1];
+			const size_t b = (size_t)ref->index[j];
+			while (e > b) {
+				const uint64_t c = ref->keys[e - 1] >> ob;
+				size_t s = e - 1;
+				while (s > b && (ref->keys[s - 1] >> ob) == c)
+					--s;
+				if (!ignore || e - s == 1) {
+					last_ctx[j] = c;
+					has_last[j] = true;
+					break;
+				}
+				e = s;
+			}
+		}
+	}
 #pragma omp parallel for schedule(dynamic, 1) num_threads(a->p)
 	for (int i = 0; i < nq; ++i) {
 		const int q = q0 + i;
@@ -857,6 +881,27 @@ static void emit_rows(FILE *fp, const ani_args_t *a, detail_path_t path, const k
 			.qry_meta = qry->meta ? &qry->meta[q] : NULL,
 			.ref_meta = ref->meta,
 		};
+		uint8_t *last_shared = NULL;
+		if (path == PATH_STREAM) {
+			const uint64_t *qk = qry->keys + qry->index[q];
+			const size_t qn = (size_t)(qry->index[q + 1] - qry->index[q]);
+			const unsigned ob = qry->obj_bits;
+			last_shared = xcalloc((size_t)(R ? R : 1), 1);
+			for (int j = 0; j < R; ++j) {
+				if (!has_last[j])
+					continue;
+				size_t lo = 0, hi = qn; /* first query key with context >= last_ctx[j] */
+				while (lo < hi) {
+					const size_t mid = lo + (hi - lo) / 2;
+					if ((qk[mid] >> ob) < last_ctx[j])
+						lo = mid + 1;
+					else
+						hi = mid;
+				}
+				last_shared[j] = lo < qn && (qk[lo] >> ob) == last_ctx[j];
+			}
+			c.last_shared = last_shared;
+		}
 		switch (path) {
 		case PATH_INDEX:
 			if (a->r.fmt)
@@ -873,7 +918,10 @@ static void emit_rows(FILE *fp, const ani_args_t *a, detail_path_t path, const k
 			kssd_emit_hash_detail(&bufs[i], &a->r, &c, qry->names[q], ref->names, ref->anno);
 			break;
 		}
+		free(last_shared);
 	}
+	free(last_ctx);
+	free(has_last);
 	for (int i = 0; i < nq; ++i) {
 		put_buf(fp, &bufs[i]);
 		kssd_buf_free(&bufs[i]);

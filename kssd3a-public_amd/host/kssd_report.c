@@ -951,6 +951,12 @@ void kssd_emit_stream_detail(kssd_buf_t *out, const kssd_report_opts_t *o, const
 		const uint32_t dobj = c->obj[j].diff_obj, dsec = c->obj[j].diff_obj_section;
 		if (single && (xny < need || close_filter_impossible(o, dsec, xny)))
 			continue;
+		/* the walk gives a reference up once the shared contexts so far plus
+		 * the contexts after the current one fall short of `need`, before it
+		 * looks the current one up (command_ani.c:2154-2156): a pair that
+		 * reaches `need` exactly on its last context is given up too */
+		if (single && need > 0 && xny == need && c->last_shared && c->last_shared[j])
+			continue;
 		if (xny < (uint32_t)o->ctxcut)
 			continue;
 		const double afq = (double)xny / (double)qc, afr = (double)xny / (double)rc;

@@ -159,6 +159,9 @@ typedef struct {
 	const kssd_coverage_src_t *cov;      /* --estimate-coverage inputs (NULL: not computed) */
 	const uint32_t *ids;                 /* candidate references (index path only) or NULL */
 	int ncand;
+	/* one-query stream path: [nref], 1 when the last context the reference's
+	 * walk counts is one of the query's (NULL: treated as 0) */
+	const uint8_t *last_shared;
 } kssd_query_counts_t;
 
 /* Index path (mem_eff) detail rows of one query, ranked and filtered as the

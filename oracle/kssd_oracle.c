@@ -25,6 +25,12 @@ int orc_dna_popcount(uint32_t x)
 	return __builtin_popcount(x);
 }
 
+void orc_dna_popcount_many(const uint32_t *x, size_t n, uint8_t *out)
+{
+	for (size_t i = 0; i < n; ++i)
+		out[i] = (uint8_t)orc_dna_popcount(x[i]);
+}
+
 int orc_fenceposts_build(const orc_rec_t *b, size_t n, int k, size_t *F)
 {
 	/* command_ani.c:3656-3678.  The bucket of a record is the top k bits of
@@ -628,4 +634,138 @@ void orc_index_rank(const orc_ctx_t *ctx, const orc_obj_t *obj, const orc_ctx_t 
 	}
 	free(ida);
 	free(ranks);
+}
+
+/* ------------------------------------------------------------------ one-query stream path */
+
+static inline uint64_t orc_fmix64(uint64_t k)
+{
+	k ^= k >> 33;
+	k *= 0xff51afd7ed558ccdULL;
+	k ^= k >> 33;
+	k *= 0xc4ceb9fe1a85ec53ULL;
+	k ^= k >> 33;
+	return k;
+}
+
+int orc_stream_one_query(const uint64_t *qry, size_t qlen, const uint64_t *ref_keys, const uint64_t *ref_idx, int R,
+			 unsigned obj_bits, int ignoreconflict, int ref_conflict, int ctxcut, float afcut,
+			 orc_ctx_t *ctx, orc_obj_t *obj, uint32_t *ref_ctx_out, uint8_t *pass, uint32_t *qry_ctx_out,
+			 int nthreads)
+{
+	const uint64_t objmask = obj_bits >= 64 ? ~0ULL : ((1ULL << obj_bits) - 1ULL);
+	memset(ctx, 0, (size_t)R * sizeof *ctx);
+	memset(obj, 0, (size_t)R * sizeof *obj);
+	memset(ref_ctx_out, 0, (size_t)R * sizeof *ref_ctx_out);
+	memset(pass, 0, (size_t)R);
+	/* build_qry_ctx_hash_lookup (command_ani.c:2024-2051): key ctx + 1,
+	 * value the run [beg, end), capacity the power of two >= 2 qlen */
+	uint32_t cap = 1;
+	while (cap < 2 * qlen)
+		cap <<= 1;
+	uint64_t *hk = calloc(cap, sizeof *hk);
+	uint64_t *hv = malloc((size_t)cap * sizeof *hv);
+	if (!hk || !hv)
+		abort();
+	const uint32_t mask = cap - 1;
+	uint32_t qry_ctx = 0;
+	for (size_t i = 0; i < qlen;) {
+		const uint64_t c = qry[i] >> obj_bits;
+		const size_t beg = i;
+		do {
+			++i;
+		} while (i < qlen && (qry[i] >> obj_bits) == c);
+		uint32_t pos = (uint32_t)orc_fmix64(c + 1) & mask;
+		while (hk[pos] && hk[pos] != c + 1)
+			pos = (pos + 1) & mask;
+		hk[pos] = c + 1;
+		hv[pos] = ((uint64_t)beg << 32) | (uint64_t)i;
+		++qry_ctx;
+	}
+	*qry_ctx_out = qry_ctx;
+	if (nthreads <= 0)
+		nthreads = 1;
+	/* compute_streamed_ref_row_one_qry (command_ani.c:2110-2201) per
+	 * reference, dynamic schedule as the block loop (:2607-2622) */
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+	for (int rn = 0; rn < R; ++rn) {
+		const uint64_t *ref = ref_keys + ref_idx[rn] - ref_idx[0];
+		const size_t ref_len = (size_t)(ref_idx[rn + 1] - ref_idx[rn]);
+		if (qry_ctx == 0 || ref_len == 0)
+			continue;
+		uint32_t total = 0; /* count_ctx_runs_sorted_ctxobj64_local (:337-349) */
+		if (ref_conflict) {
+			for (size_t j = 0; j < ref_len;) {
+				const uint64_t c = ref[j] >> obj_bits;
+				const size_t b = j;
+				do {
+					++j;
+				} while (j < ref_len && (ref[j] >> obj_bits) == c);
+				if (!ignoreconflict || j - b == 1)
+					++total;
+			}
+		} else {
+			total = (uint32_t)ref_len;
+		}
+		if (total == 0)
+			continue;
+		/* ani_report_af_needed_ctx (command_ani.h:80-90) */
+		const uint32_t denom = qry_ctx < total ? qry_ctx : total;
+		const double need_d = (double)afcut * (double)denom;
+		const uint32_t need = need_d <= 0.0 ? 0 : (uint32_t)ceil(need_d);
+		uint32_t xny = 0, mut2 = 0, dobj = 0, dsec = 0, walked = 0;
+		int gave_up = 0;
+		for (size_t j = 0; j < ref_len;) {
+			const uint64_t c = ref[j] >> obj_bits;
+			const size_t rb = j;
+			do {
+				++j;
+			} while (j < ref_len && (ref[j] >> obj_bits) == c);
+			if (ignoreconflict && j - rb > 1)
+				continue;
+			++walked;
+			if (xny + (total - walked) < need) { /* :2155-2156 */
+				gave_up = 1;
+				break;
+			}
+			uint32_t pos = (uint32_t)orc_fmix64(c + 1) & mask;
+			while (hk[pos] && hk[pos] != c + 1)
+				pos = (pos + 1) & mask;
+			if (!hk[pos])
+				continue;
+			const size_t qb = (size_t)(hv[pos] >> 32), qe = (size_t)(uint32_t)hv[pos];
+			++xny;
+			/* min_diff_sections_ctxobj64_runs (command_ani.h:283-299) */
+			int m = ORC_MIN_INIT;
+			for (size_t a = qb; a < qe && m; ++a)
+				for (size_t b = rb; b < j; ++b) {
+					const uint32_t x = (uint32_t)(qry[a] & objmask) ^ (uint32_t)(ref[b] & objmask);
+					if (!x) {
+						m = 0;
+						break;
+					}
+					const int d = orc_dna_popcount(x);
+					if (d < m)
+						m = d;
+				}
+			if (m > 0) {
+				++dobj;
+				dsec += (uint32_t)m;
+				if (m > 1)
+					++mut2;
+			}
+		}
+		ctx[rn].num_ctx = xny;
+		ctx[rn].num_mut2_ctx = mut2;
+		obj[rn].diff_obj = dobj;
+		obj[rn].diff_obj_section = dsec;
+		ref_ctx_out[rn] = walked;
+		if (gave_up || walked == 0 || xny < (uint32_t)ctxcut || xny < need)
+			continue;
+		const double afq = (double)xny / (double)qry_ctx, afr = (double)xny / (double)walked;
+		pass[rn] = (afq > afr ? afq : afr) >= (double)afcut; /* ani_report_af_pass (command_ani.h:75-78) */
+	}
+	free(hk);
+	free(hv);
+	return 0;
 }

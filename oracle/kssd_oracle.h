@@ -44,6 +44,7 @@ typedef struct {
 
 /* dna_popcount.h:45-49 */
 int orc_dna_popcount(uint32_t x);
+void orc_dna_popcount_many(const uint32_t *x, size_t n, uint8_t *out);
 
 /* command_ani.c:3656-3678 (F has 2^k + 1 entries) */
 int orc_fenceposts_build(const orc_rec_t *b, size_t n, int k, size_t *F);
@@ -112,6 +113,22 @@ void orc_index_rank(const orc_ctx_t *ctx, const orc_obj_t *obj, const orc_ctx_t 
 		    int nq, int R, const uint32_t *ref_ctx, const uint32_t *qry_ctx, int ctxcut_opt, float afcut,
 		    float anicut, int ntop_opt, int unique_ctx, uint32_t *ids, uint32_t *pass_ids, double *rank,
 		    uint32_t *n_pass, uint32_t *n_out);
+
+/*
+ * The reference's default path for one query (stream_ref_sketches_one_qraw_lookup,
+ * command_ani.c:2490-2666; compute_streamed_ref_row_one_qry, :2110-2201): the
+ * query's context runs in an open-addressed table, every reference streamed run
+ * by run against it.  Per reference r: ctx[r] / obj[r] the pair's counters (as
+ * a row of orc_count_block; a reference given up early by the AF-needed bound
+ * keeps the counts up to that point), ref_ctx[r] the contexts walked, pass[r]
+ * whether the row survives ctxcut, the AF-needed count and afcut (the
+ * selected-ANI cut and the close filters are left out).  qry_ctx: the query's
+ * context runs.
+ */
+int orc_stream_one_query(const uint64_t *qry, size_t qlen, const uint64_t *ref_keys, const uint64_t *ref_idx, int R,
+			 unsigned obj_bits, int ignoreconflict, int ref_conflict, int ctxcut, float afcut,
+			 orc_ctx_t *ctx, orc_obj_t *obj, uint32_t *ref_ctx, uint8_t *pass, uint32_t *qry_ctx,
+			 int nthreads);
 
 #ifdef __cplusplus
 }

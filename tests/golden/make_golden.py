@@ -505,7 +505,44 @@ SCALE_CASES = {  # scale/<name>.tsv: (reference dir, query flag, query dir, argu
     "boundary_unique": ("boundary/ref", "--qraw", "boundary/qry", ["--unique-ctx", "-n0", "-f0", "-t1", "-p4"], {}),
     "boundary_detail": ("boundary/ref", "-q", "boundary/qry", ["-n0", "-f0", "-t1"], {}),
     "boundary_index_detail": ("boundary/ref", "-q", "boundary/qry", ["-n0", "-f0", "-t1"], FORCE),
+    # the align-fraction bound of the one-query stream path: references that
+    # share exactly the needed number of contexts, the last of their walk
+    # shared or not (command_ani.c:2154-2156); three queries take the
+    # multi-query path, the forced index the index path, neither of which
+    # gives a reference up early
+    "afb_stream_one": ("afb/ref", "-q", "afb/qry1", ["-n0", "-t1"], {}),
+    "afb_stream_one_s3": ("afb/ref", "-q", "afb/qry1", ["-s3", "-n0", "-t1"], {}),
+    "afb_stream_one_ignoreconflict": ("afb/ref", "-q", "afb/qry1", ["-n0", "-t1", "--ignoreconflict"], {}),
+    "afb_stream_multi": ("afb/ref", "-q", "afb/qry3", ["-n0", "-t1"], {}),
+    "afb_index": ("afb/ref", "-q", "afb/qry3", ["-n0", "-t1"], FORCE),
 }
+
+
+def afb_sketches(out):
+    """Query contexts 1000..1099; references of 80 contexts (need = 40 at
+    afcut 0.5) sharing 40 of them with the last walked context shared or
+    not, 41, all, and a 10-context one sharing its last 5; one reference
+    whose last context holds two objects (walked or skipped under
+    --ignoreconflict)."""
+    q = [(c, 0) for c in range(1000, 1100)]
+    refs = {
+        "r_shared40_last_unshared": [(c, 0) for c in range(1000, 1040)] + [(c, 0) for c in range(2000, 2040)],
+        "r_shared40_last_shared": [(c, 0) for c in range(10, 50)] + [(c, 0) for c in range(1050, 1090)],
+        "r_shared41": [(c, 0) for c in range(10, 49)] + [(c, 0) for c in range(1050, 1091)],
+        "r_shared40_diffobj": [(c, 0) for c in range(10, 50)] + [(c, 3 * (c % 2)) for c in range(1050, 1090)],
+        "r_all": list(q),
+        "r_small_last_shared": [(c, 0) for c in range(5, 10)] + [(c, 0) for c in range(1095, 1100)],
+        "r_conflict_last": [(c, 0) for c in range(10, 49)] + [(c, 0) for c in range(1050, 1090)]
+        + [(1090, 0), (1090, 5)],
+        "r_none": [(c, 0) for c in range(3000, 3050)],
+    }
+    kt.write_sketch_dir(out / "afb" / "ref", [to_keys(v) for v in refs.values()], names=list(refs), conflict=True)
+    kt.write_sketch_dir(out / "afb" / "qry1", [to_keys(q)], names=["q_all"], conflict=True)
+    q2 = [(c, 0) for c in range(1000, 1060)]
+    q3 = [(c, 1) for c in range(1040, 1100)]
+    kt.write_sketch_dir(out / "afb" / "qry3", [to_keys(q), to_keys(q2), to_keys(q3)], names=["q_all", "q_low", "q_high"],
+                        conflict=True)
+    run(["sketch", "-i", "ref"], cwd=out / "afb")
 
 
 def make_scale():
@@ -549,6 +586,7 @@ def make_scale():
     kt.write_sketch_dir(out / "boundary" / "qry", [to_keys([(1, 0)] if i % 3 else []) for i in range(310)],
                         names=[f"qry_{i}" for i in range(310)])
     run(["sketch", "-i", "ref"], cwd=out / "boundary")
+    afb_sketches(out)
     import gzip
     import json
     (out / "cases.json").write_text(json.dumps(SCALE_CASES, indent=1, sort_keys=True) + "\n")

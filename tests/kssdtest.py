@@ -165,6 +165,7 @@ def load_oracle():
     lib = ctypes.CDLL(str(so))
     V, S, U, I, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint, ctypes.c_int, ctypes.c_double
     lib.orc_dna_popcount.argtypes = [ctypes.c_uint32]
+    lib.orc_dna_popcount_many.argtypes = [V, S, V]
     lib.orc_dna_popcount.restype = I
     lib.orc_fenceposts_build.argtypes = [V, S, I, V]
     lib.orc_find_first.argtypes = [V, S, V, S, V, I, U, V]
@@ -176,6 +177,8 @@ def load_oracle():
     lib.orc_index_rank.argtypes = [V, V, V, V, I, I, V, V, I, ctypes.c_float, ctypes.c_float, I, I, V, V, V, V,
                                    V]
     lib.orc_index_rank.restype = None
+    lib.orc_stream_one_query.argtypes = [V, S, V, V, I, U, I, I, I, ctypes.c_float, V, V, V, V,
+                                         ctypes.POINTER(ctypes.c_uint32), I]
     lib.orc_build_index.argtypes = [V, V, I, U, V]
     lib.orc_build_index.restype = None
     lib.orc_ctx_runs.argtypes = [V, S, U, I]
@@ -227,6 +230,24 @@ class Oracle:
         self.lib.orc_pair_features(_p(a), len(a), _p(b), len(b), obj_bits, int(ignore_b_conflict),
                                    ctypes.byref(f))
         return f.tup()
+
+    def stream_one_query(self, qry, ref_keys, ref_idx, obj_bits=20, ignoreconflict=False, ref_conflict=True,
+                         ctxcut=1, afcut=0.0, threads=8):
+        """The reference's one-query stream path: (ctx [R, 2], obj [R, 2],
+        ref_ctx [R], pass [R] bool, qry_ctx)."""
+        qry = np.ascontiguousarray(qry, np.uint64)
+        ref_keys = np.ascontiguousarray(ref_keys, np.uint64)
+        ref_idx = np.ascontiguousarray(ref_idx, np.uint64)
+        R = len(ref_idx) - 1
+        ctx = np.zeros((R, 2), np.uint32)
+        obj = np.zeros((R, 2), np.uint32)
+        rctx = np.zeros(R, np.uint32)
+        ok = np.zeros(R, np.uint8)
+        qc = ctypes.c_uint32()
+        self.lib.orc_stream_one_query(_p(qry), len(qry), _p(ref_keys), _p(ref_idx), R, obj_bits, int(ignoreconflict),
+                                      int(ref_conflict), ctxcut, afcut, _p(ctx), _p(obj), _p(rctx), _p(ok),
+                                      ctypes.byref(qc), threads)
+        return ctx, obj, rctx, ok.astype(bool), int(qc.value)
 
     def index_rank(self, ctx, obj, uctx, uobj, ref_ctx, qry_ctx, ctxcut=3, afcut=0.0, anicut=0.0, ntop=0,
                    unique_ctx=False):

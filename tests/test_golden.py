@@ -176,3 +176,26 @@ def test_index_rank_against_reference_order(oracle, stem, ntop):
         want = [r["Ref"] for r in rows if r["Qry"] == qry.names[q]]
         assert got == want, (stem, q)
     assert ntop == 0 or all(len(p[2]) == ntop for p in ranked)
+
+
+@pytest.mark.parametrize("stem,metric,values_ani", [
+    ("matrix_ani_rq", 3, 1), ("matrix_rq_s5", 5, 0), ("matrix_rq_s6", 6, 0), ("matrix_rq_s7", 7, 0),
+    ("matrix_rq_s8", 8, 0), ("matrix_rq_s-9", 9, 1)])
+def test_matrix_values_match_reference(oracle, stem, metric, values_ani):
+    """orc_matrix_values (the FP64 matrix epilogue restated from
+    command_ani.c:1321-1379, :1458-1496) on the oracle's counts, printed as
+    the reference prints matrix cells ("%f"), equals the reference binary's
+    `ani -r ref -q qry --format matrix` output byte for byte."""
+    g = kt.GOLDEN / "genomes"
+    ref, qry = kt.read_sketch_dir(g / "ref"), kt.read_sketch_dir(g / "qry")
+    R, Q = ref.n, qry.n
+    ctx, obj, _, _ = oracle.count_block(kt.build_index_np(ref.keys, ref.index, 20), R, 20, qry.keys, qry.index)
+    qctx = np.diff(qry.index).astype(np.uint32)
+    rctx = np.diff(ref.index).astype(np.uint32)
+    vals = np.zeros(Q * R, np.float64)
+    oracle.lib.orc_matrix_values(kt._p(ctx), kt._p(obj), Q, R, kt._p(qctx), kt._p(rctx), metric, values_ani, 0.5,
+                                 1.0, ref.drfold, 20, 44, kt._p(vals), 1)
+    v = vals.reshape(Q, R)
+    rows = ["".join("\t" + n for n in ref.names)]
+    rows += [qry.names[q] + "".join("\t%f" % x for x in v[q]) for q in range(Q)]
+    assert "\n".join(rows) + "\n" == (g / f"{stem}.tsv").read_text()

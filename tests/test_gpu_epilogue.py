@@ -165,3 +165,62 @@ def test_select_hits_matches_numpy(hip, rt, cut):
     assert all(v[q, r] >= cut for q, r in zip(part["query"], part["ref"]))
     for p in (d_v, d_hits, d_n):
         rt.free(p)
+
+
+def _random_tuples(rng, nq, nr):
+    """Dense count tuples that honour the counters' invariants (mut2 <=
+    diff_obj <= num_ctx <= min(qry_ctx, ref_ctx); diff_obj_section between
+    diff_obj + mut2 and 11 diff_obj), with edge cases: empty sketches,
+    totals of 1 and 2, no shared context, all contexts shared, no or all
+    objects different."""
+    qc = rng.integers(0, 300000, size=nq).astype(np.uint32)
+    rc = rng.integers(0, 300000, size=nr).astype(np.uint32)
+    qc[:8] = [0, 1, 2, 3, 0, 1, 50, 7]
+    rc[:8] = [0, 1, 2, 0, 5, 1, 50, 7]
+    top = np.minimum(qc[:, None], rc[None, :]).astype(np.int64)
+    frac = rng.random((nq, nr)) ** 3
+    xny = np.floor(frac * (top + 1)).astype(np.int64)
+    xny = np.minimum(xny, top)
+    edge = rng.integers(0, 6, size=(nq, nr))
+    xny = np.where(edge == 0, 0, np.where(edge == 1, top, xny))
+    dobj = np.floor(rng.random((nq, nr)) * (xny + 1)).astype(np.int64)
+    dobj = np.where(edge == 2, 0, np.where(edge == 3, xny, np.minimum(dobj, xny)))
+    mut2 = np.floor(rng.random((nq, nr)) * (dobj + 1)).astype(np.int64)
+    mut2 = np.minimum(mut2, dobj)
+    lo, hi = dobj + mut2, 11 * dobj
+    dsec = lo + np.floor(rng.random((nq, nr)) * (hi - lo + 1)).astype(np.int64)
+    dsec = np.minimum(dsec, hi)
+    ctx = np.stack([xny, mut2], -1).astype(np.uint32)
+    obj = np.stack([dobj, dsec], -1).astype(np.uint32)
+    return ctx, obj, qc, rc
+
+
+def test_epilogue_one_million_tuples(hip, oracle, rt):
+    """The device epilogue over 1,000 x 1,000 random count tuples for every
+    metric 3..9, distance and ANI, against the FP64 host epilogue of the
+    oracle (orc_matrix_values, itself equal to the reference's printed
+    matrices): within 1e-12 relative, and infinities / NaNs in the same
+    cells."""
+    nq = nr = 1000
+    ctx, obj, qc, rc = _random_tuples(np.random.default_rng(41), nq, nr)
+    d_ctx, d_obj, d_qc, d_rc = rt.upload(ctx), rt.upload(obj), rt.upload(qc), rt.upload(rc)
+    d_out = rt.malloc(8 * nq * nr)
+    try:
+        for s in range(3, 10):
+            for va in (0, 1):
+                want = np.zeros(nq * nr, np.float64)
+                oracle.lib.orc_matrix_values(kt._p(ctx), kt._p(obj), nq, nr, kt._p(qc), kt._p(rc), s, va, 0.5, 1.0,
+                                             8, 20, 44, kt._p(want), 8)
+                o = kt.EpilogueOpts(s, va, 0, 0, 0, 0, 8, 20, 44, 0.5, 1.0)
+                hip.epilogue_dev(d_ctx, d_obj, nq, nr, d_qc, d_rc, o, d_out)
+                got = rt.download(d_out, np.float64, nq * nr)
+                fin = np.isfinite(want)
+                assert np.array_equal(np.isnan(got), np.isnan(want)), (s, va)
+                assert np.array_equal(np.isposinf(got), np.isposinf(want)), (s, va)
+                assert np.array_equal(np.isneginf(got), np.isneginf(want)), (s, va)
+                rel = np.abs(got[fin] - want[fin]) / np.maximum(np.abs(want[fin]), 1e-300)
+                assert float(rel.max(initial=0.0)) <= REL_TOL, (s, va, float(rel.max()))
+                assert fin.sum() > nq * nr // 2
+    finally:
+        for p in (d_ctx, d_obj, d_qc, d_rc, d_out):
+            rt.free(p)

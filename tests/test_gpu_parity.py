@@ -288,3 +288,37 @@ def test_tiling_validation(hip):
                 hip.set_tiling(h, **bad)
     finally:
         hip.close_index(h)
+
+
+@pytest.mark.parametrize("obj_bits", [20, 32])
+@pytest.mark.parametrize("mode", [1, 3])
+def test_device_popcount_exhaustive(hip, obj_bits, mode):
+    """dna_popcount on the device over every 20-bit object difference (and
+    2^20 random 32-bit ones): 2^20 one-key queries against one reference
+    holding each query's context with object 0.  Per query the pair has one
+    shared context, and the minimum mismatch is the lane count of the
+    query's object, capped at 12 (NUM_CODENS + 1, command_ani.h:535)."""
+    n = 1 << 20
+    rng = np.random.default_rng(obj_bits)
+    objs = (np.arange(n, dtype=np.uint64) if obj_bits == 20
+            else rng.integers(0, 1 << 32, size=n, dtype=np.uint64))
+    ctxs = np.arange(1, n + 1, dtype=np.uint64)
+    ref_keys = ctxs << np.uint64(obj_bits)
+    ref_idx = np.array([0, n], np.uint64)
+    qry_keys = (ctxs << np.uint64(obj_bits)) | objs
+    qry_idx = np.arange(n + 1, dtype=np.uint64)
+    rec = kt.build_index_np(ref_keys, ref_idx, obj_bits)
+    lanes = np.zeros(n, np.uint32)
+    for bit in range(0, 32, 2):
+        lanes += ((objs >> np.uint64(bit)) & np.uint64(3)) != 0
+    m = np.minimum(lanes, 12)
+    h = hip.open_index(rec, 1, obj_bits)
+    try:
+        hip.set_tiling(h, mode=mode)
+        ctx, obj, _, _ = hip.count_block(h, 1, qry_keys, qry_idx)
+    finally:
+        hip.close_index(h)
+    assert np.all(ctx[:, 0, 0] == 1)
+    assert np.array_equal(ctx[:, 0, 1], (m > 1).astype(np.uint32))
+    assert np.array_equal(obj[:, 0, 0], (m > 0).astype(np.uint32))
+    assert np.array_equal(obj[:, 0, 1], m)

@@ -11,18 +11,25 @@ import kssdtest as kt
 OBJ_BITS = 20
 
 
-def test_dna_popcount_exhaustive(oracle):
-    # every 20-bit XOR value: number of 2-bit lanes that are nonzero
-    x = np.arange(1 << 20, dtype=np.uint32)
-    lanes = np.zeros_like(x)
+def _lanes(x):
+    """Number of nonzero 2-bit lanes of each u32, lane by lane."""
+    lanes = np.zeros(len(x), np.uint8)
     for bit in range(0, 32, 2):
-        lanes += ((x >> bit) & 3) != 0
-    sample = np.random.default_rng(0).choice(len(x), 4096, replace=False)
-    got = np.array([oracle.lib.orc_dna_popcount(int(v)) for v in x[sample]])
-    assert np.array_equal(got, lanes[sample])
-    # full 32-bit lanes at the top end
-    for v in (0xFFFFFFFF, 0x80000000, 0x40000000, 0xC0000001):
-        assert oracle.lib.orc_dna_popcount(v) == sum(((v >> b) & 3) != 0 for b in range(0, 32, 2))
+        lanes += (((x >> np.uint32(bit)) & np.uint32(3)) != 0).astype(np.uint8)
+    return lanes
+
+
+def test_dna_popcount_exhaustive(oracle):
+    """Every 20-bit XOR value (all 2^20), then 2^20 random 32-bit values and
+    the top-lane edge cases: the number of 2-bit lanes that are nonzero."""
+    rng = np.random.default_rng(0)
+    for x in (np.arange(1 << 20, dtype=np.uint32),
+              rng.integers(0, 1 << 32, size=1 << 20, dtype=np.uint64).astype(np.uint32),
+              np.array([0xFFFFFFFF, 0x80000000, 0x40000000, 0xC0000001, 0xAAAAAAAA, 0x55555555], np.uint32)):
+        got = np.zeros(len(x), np.uint8)
+        oracle.lib.orc_dna_popcount_many(kt._p(x), len(x), kt._p(got))
+        assert np.array_equal(got, _lanes(x))
+    assert oracle.lib.orc_dna_popcount(0xFFFFFFFF) == 16
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -135,3 +142,95 @@ def test_formula_special_cases(oracle):
     # Mash/AAF on identical sets: distance 0
     assert oracle.lib.orc_mashD(22, 500, 500, 500) == 0.0
     assert oracle.lib.orc_aafD(22, 500, 800, 500) == 0.0
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("ignore", [False, True])
+def test_stream_one_query_equals_index_counts(oracle, seed, ignore):
+    """The one-query stream restatement (hash of the query's contexts,
+    references streamed) gives each reference the counters the index path
+    gives the same pair (a different code path: fenceposts over the sorted
+    index), on sketches with conflicting objects; with afcut 0 every pair
+    with a shared context passes, and ref_ctx counts the walked contexts."""
+    rng = np.random.default_rng(300 + seed)
+    pool = rng.integers(1, 1 << 40, size=400, dtype=np.uint64)
+    refs = kt.random_samples(rng, 30, 400, 120, shared_pool=pool, conflict_p=0.2)
+    q = kt.random_samples(rng, 1, 400, 150, shared_pool=pool, conflict_p=0.2)[0]
+    rk = np.concatenate(refs)
+    ri = np.zeros(len(refs) + 1, np.uint64)
+    ri[1:] = np.cumsum([len(s) for s in refs])
+    rec = kt.build_index_np(rk, ri, OBJ_BITS)
+    flags = kt.FLAGS["IGNORECONFLICT"] if ignore else 0
+    want_c, want_o, _, _ = oracle.count_block(rec, len(refs), OBJ_BITS, q, np.array([0, len(q)], np.uint64),
+                                              flags=flags)
+    ctx, obj, rctx, ok, qc = oracle.stream_one_query(q, rk, ri, ignoreconflict=ignore)
+    assert np.array_equal(ctx, want_c[0]) and np.array_equal(obj, want_o[0])
+    assert qc == len(np.unique(q >> np.uint64(OBJ_BITS)))
+    for r, s in enumerate(refs):
+        c, n = np.unique(s >> np.uint64(OBJ_BITS), return_counts=True)
+        assert rctx[r] == ((n == 1).sum() if ignore else len(c))
+    assert np.array_equal(ok, ctx[:, 0] >= 1)
+
+
+@pytest.mark.parametrize("afcut", [0.05, 0.2, 0.5])
+def test_stream_one_query_af_gate(oracle, afcut):
+    """With afcut > 0 the stream path gives up a reference once the shared
+    contexts so far plus the contexts after the current one fall short of
+    the needed count, checked before the current context is looked up
+    (command_ani.c:2154-2156).  So a reference passes exactly when its full
+    counts meet ctxcut, that count and afcut (command_ani.h:75-90) and it
+    does not reach the count exactly on its last context (computed here from
+    the index path's counts and the sketches)."""
+    rng = np.random.default_rng(11)
+    pool = rng.integers(1, 1 << 40, size=300, dtype=np.uint64)
+    refs = kt.random_samples(rng, 40, 300, 100, shared_pool=pool)
+    q = kt.random_samples(rng, 1, 300, 100, shared_pool=pool)[0]
+    rk = np.concatenate(refs)
+    ri = np.zeros(len(refs) + 1, np.uint64)
+    ri[1:] = np.cumsum([len(s) for s in refs])
+    rec = kt.build_index_np(rk, ri, OBJ_BITS)
+    full, _, _, _ = oracle.count_block(rec, len(refs), OBJ_BITS, q, np.array([0, len(q)], np.uint64))
+    _, _, _, ok, qc = oracle.stream_one_query(q, rk, ri, ctxcut=3, afcut=afcut)
+    x = full[0, :, 0].astype(np.float64)
+    tot = np.array([len(np.unique(s >> np.uint64(OBJ_BITS))) for s in refs], np.float64)
+    need = np.ceil(np.float64(np.float32(afcut)) * np.minimum(qc, tot))
+    want = (x >= 3) & (x >= need) & (np.maximum(x / qc, x / np.maximum(tot, 1)) >= np.float64(np.float32(afcut)))
+    qctx = set((q >> np.uint64(OBJ_BITS)).tolist())
+    last_shared = np.array([len(s) > 0 and int(s[-1] >> np.uint64(OBJ_BITS)) in qctx for s in refs])
+    boundary = (x == need) & (need > 0) & last_shared
+    assert np.array_equal(ok, want & ~boundary)
+    assert 0 < (want & ~boundary).sum() < len(refs)
+
+
+def test_stream_one_query_matches_reference_rows(oracle):
+    """The reference binary's one-query stream output (stream_qry1_all_s3:
+    `ani -r ref -q qry1 -s3 -n0 -f0 -t1`) lists exactly the references the
+    restatement passes, with the same XnY_ctx, N_diff_obj,
+    N_diff_obj_section and N_mut2_ctx."""
+    g = kt.GOLDEN / "genomes"
+    ref, qry = kt.read_sketch_dir(g / "ref"), kt.read_sketch_dir(g / "qry1")
+    ctx, obj, _, ok, _ = oracle.stream_one_query(qry.keys, ref.keys, ref.index, ref_conflict=bool(ref.conflict))
+    lines = (g / "stream_qry1_all_s3.tsv").read_text().splitlines()
+    head = lines[0].split("\t")
+    col = {k: head.index(k) for k in ("Ref", "XnY_ctx", "N_diff_obj", "N_diff_obj_section", "N_mut2_ctx")}
+    rows = {c[col["Ref"]]: c for c in (ln.split("\t") for ln in lines[1:])}
+    assert set(rows) == {ref.names[r] for r in np.flatnonzero(ok)}
+    for name, c in rows.items():
+        r = ref.names.index(name)
+        assert [int(c[col[k]]) for k in ("XnY_ctx", "N_diff_obj", "N_diff_obj_section", "N_mut2_ctx")] == \
+            [int(ctx[r, 0]), int(obj[r, 0]), int(obj[r, 1]), int(ctx[r, 1])]
+
+
+@pytest.mark.parametrize("stem,ignore", [("afb_stream_one", False), ("afb_stream_one_ignoreconflict", True)])
+def test_stream_one_query_af_boundary_matches_reference(oracle, stem, ignore):
+    """References that share exactly the needed number of contexts: the
+    reference binary's one-query stream path keeps those whose last walked
+    context is not shared and drops the others (tests/golden/scale/afb_*);
+    the restatement passes the same set."""
+    g = kt.GOLDEN / "scale" / "afb"
+    ref, qry = kt.read_sketch_dir(g / "ref"), kt.read_sketch_dir(g / "qry1")
+    _, _, _, ok, _ = oracle.stream_one_query(qry.keys, ref.keys, ref.index, ignoreconflict=ignore,
+                                             ref_conflict=bool(ref.conflict), ctxcut=1, afcut=0.5)
+    lines = (kt.GOLDEN / "scale" / f"{stem}.tsv").read_text().splitlines()[1:]
+    assert {ln.split("\t")[1] for ln in lines} == {ref.names[r] for r in np.flatnonzero(ok)}
+    assert "r_shared40_last_shared" not in {ln.split("\t")[1] for ln in lines}
