@@ -413,8 +413,20 @@ int kssdhip_index_matched_records(kssdhip_index *index, const uint64_t *d_keys, 
  * into one launch while their keys and dense counter rows fit that slot's
  * share of `budget_bytes` of device memory (and pinned host memory of the
  * same size); requests arriving meanwhile go to the next free slot, so
- * batches overlap on the device.  Results are exactly those of
+ * batches overlap on the device.  Counter rows are exactly those of
  * kssdhip_count_block for the query alone.
+ *
+ * With `select` set the server returns results instead of counter rows:
+ * each batch runs the index path's pass gate and FP64 value on the device
+ * (kssdhip_select_block_dev with select_opts; the request's qry_ctx is the
+ * number of distinct contexts among its keys, ref_ctx the totals given at
+ * open), only the passing rows come back, and kssdhip_serve_wait_rows
+ * hands each request its rows ranked by value, highest first (ties by
+ * ascending reference id), cut to ntop when ntop > 0.  This replaces the
+ * per-query selection of the stream path (command_ani.c:2110-2201 rows,
+ * sorted and cut at :2634-2650) with the index path's gate
+ * (command_ani.h:585-606).  Admission then budgets keys plus one 56-byte
+ * row per reference per request.
  */
 typedef struct kssdhip_server kssdhip_server;
 typedef struct {
@@ -423,6 +435,10 @@ typedef struct {
 	int max_batch;       /* queries per launch */
 	int max_queued;      /* host queue bound; submit returns KSSDHIP_EAGAIN beyond it */
 	uint32_t flags;      /* KSSDHIP_UNIQUE, KSSDHIP_IGNORECONFLICT */
+	int select;          /* nonzero: results (kssdhip_serve_wait_rows), not counter rows */
+	kssdhip_select_opts_t select_opts; /* with select: gate and value (ctxcut >= 1) */
+	const uint32_t *ref_ctx;           /* with select: ref_gnum context totals, copied at open */
+	int ntop;                          /* with select: rows kept per request (0: all) */
 } kssdhip_serve_opts_t;
 typedef struct {
 	double queue_ms;    /* submit -> launch */
@@ -446,6 +462,12 @@ int64_t kssdhip_serve_submit(kssdhip_server *server, const uint64_t *keys, size_
 int kssdhip_serve_wait(kssdhip_server *server, int64_t ticket, kssd_ctx_mut2_t *ctx,
 		       kssd_obj_section_t *obj, kssd_ctx_mut2_t *unique_ctx,
 		       kssd_obj_section_t *unique_obj, int timeout_ms, kssdhip_serve_stats_t *stats);
+/* Results of a server opened with select: the request's passing rows
+ * (query 0), ranked; *n_rows receives their number.  When it exceeds cap
+ * nothing is copied, KSSDHIP_ENOSPC is returned and the ticket stays
+ * waitable, so the caller can retry with a larger buffer. */
+int kssdhip_serve_wait_rows(kssdhip_server *server, int64_t ticket, kssdhip_ani_row_t *rows, size_t cap,
+			    size_t *n_rows, int timeout_ms, kssdhip_serve_stats_t *stats);
 int kssdhip_serve_info(kssdhip_server *server, kssdhip_serve_info_t *info);
 /* Stop, drain the launched batches, fail the queued ones, free everything.
  * No call on the server may follow or run concurrently. */

@@ -1223,14 +1223,13 @@ static int noalloc_scratch(int device, void **out)
 	return 0;
 }
 
-extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq,
-					uint64_t total, uint32_t flags, const uint32_t *d_qry_ctx,
-					const uint32_t *d_ref_ctx, const kssdhip_select_opts_t *opts,
-					kssdhip_ani_row_t *d_rows, uint32_t cap, uint32_t *d_row_count,
-					uint32_t *d_q_first, uint32_t *d_q_count, void *stream)
+int kssd_select_dev_ws(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
+		       uint32_t flags, const uint32_t *d_qry_ctx, const uint32_t *d_ref_ctx,
+		       const kssdhip_select_opts_t *opts, kssdhip_ani_row_t *d_rows, uint32_t cap,
+		       uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, hipStream_t s, DevBuf *ws)
 {
 	const SelectCall c{d_keys, d_skidx, nq, total, flags, d_qry_ctx, d_ref_ctx, opts,
-			   d_rows, cap, d_row_count, d_q_first, d_q_count, (hipStream_t)stream};
+			   d_rows, cap, d_row_count, d_q_first, d_q_count, s};
 	int rc = select_check(ix, c);
 	bool done = false;
 	if (rc || (rc = select_begin(c, &done)) || done)
@@ -1239,9 +1238,21 @@ extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_key
 	if ((rc = qj_shape(ix, (uint32_t)nq, (flags & KSSDHIP_UNIQUE) != 0, &sh)))
 		return rc;
 	QJWork w;
-	if ((rc = qj_workspace(&ix->qws, sh, d_skidx, (uint32_t)nq, c.s, &w)))
+	if ((rc = qj_workspace(ws, sh, d_skidx, (uint32_t)nq, c.s, &w)))
 		return rc;
 	return select_launch(ix, c, sh, w);
+}
+
+extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq,
+					uint64_t total, uint32_t flags, const uint32_t *d_qry_ctx,
+					const uint32_t *d_ref_ctx, const kssdhip_select_opts_t *opts,
+					kssdhip_ani_row_t *d_rows, uint32_t cap, uint32_t *d_row_count,
+					uint32_t *d_q_first, uint32_t *d_q_count, void *stream)
+{
+	if (!ix)
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: bad arguments");
+	return kssd_select_dev_ws(ix, d_keys, d_skidx, nq, total, flags, d_qry_ctx, d_ref_ctx, opts, d_rows, cap,
+				  d_row_count, d_q_first, d_q_count, (hipStream_t)stream, &ix->qws);
 }
 
 extern "C" int kssdhip_select_block_dev_noalloc(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx,

@@ -251,7 +251,8 @@ HIT_DTYPE = np.dtype([("query", "<u4"), ("ref", "<u4"), ("value", "<f8")])  # ks
 
 
 class ServeOpts(ctypes.Structure):
-    _fields_ = [("budget_bytes", S), ("slots", I), ("max_batch", I), ("max_queued", I), ("flags", U32)]
+    _fields_ = [("budget_bytes", S), ("slots", I), ("max_batch", I), ("max_queued", I), ("flags", U32),
+                ("select", I), ("select_opts", SelectOpts), ("ref_ctx", V), ("ntop", I)]
 
 
 class ServeStats(ctypes.Structure):
@@ -301,6 +302,7 @@ def load_kssdhip():
     lib.kssdhip_serve_submit.argtypes = [V, V, S]
     lib.kssdhip_serve_submit.restype = ctypes.c_int64
     lib.kssdhip_serve_wait.argtypes = [V, ctypes.c_int64, V, V, V, V, I, ctypes.POINTER(ServeStats)]
+    lib.kssdhip_serve_wait_rows.argtypes = [V, ctypes.c_int64, V, S, ctypes.POINTER(S), I, ctypes.POINTER(ServeStats)]
     lib.kssdhip_serve_info.argtypes = [V, ctypes.POINTER(ServeInfo)]
     lib.kssdhip_serve_close.argtypes = [V]
     return lib
@@ -409,11 +411,33 @@ class KssdHip:
                                                       d_out, d_ws, stream))
 
     # ---- serving loop (kssdhip_serve_*)
-    def serve_open(self, h, budget_bytes, slots=2, max_batch=64, max_queued=4096, flags=0):
+    def serve_open(self, h, budget_bytes, slots=2, max_batch=64, max_queued=4096, flags=0, select_opts=None,
+                   ref_ctx=None, ntop=0):
+        """A server returning counter rows, or with select_opts (and the
+        references' context totals) ranked passing rows."""
         srv = V()
-        opts = ServeOpts(budget_bytes, slots, max_batch, max_queued, flags)
+        rc = None
+        if select_opts is not None:
+            rc = np.ascontiguousarray(ref_ctx, dtype=np.uint32)
+        opts = ServeOpts(budget_bytes, slots, max_batch, max_queued, flags, int(select_opts is not None),
+                         select_opts if select_opts is not None else SelectOpts(), _p(rc), ntop)
         self.check(self.lib.kssdhip_serve_open(h, ctypes.byref(opts), ctypes.byref(srv)))
         return srv
+
+    def serve_wait_rows(self, srv, ticket, timeout_ms=-1, cap=256):
+        """(ROW_DTYPE array of the request's ranked rows, ServeStats); the
+        buffer grows on KSSDHIP_ENOSPC."""
+        st = ServeStats()
+        n = S()
+        while True:
+            rows = np.zeros(max(cap, 1), dtype=ROW_DTYPE)
+            rc = self.lib.kssdhip_serve_wait_rows(srv, ticket, _p(rows), cap, ctypes.byref(n), timeout_ms,
+                                                  ctypes.byref(st))
+            if rc == ENOSPC:
+                cap = int(n.value)
+                continue
+            self.check(rc)
+            return rows[: int(n.value)].copy(), st
 
     def serve_submit(self, srv, keys):
         """Ticket (>= 0) or the negative error code (EAGAIN: queue full)."""
