@@ -256,12 +256,18 @@ def write_fastq_pair(path1, path2, rng, genome, coverage, read_len=150, insert=3
     flip = rng.random(n) < 0.5  # fragments from the reverse strand: swap the mates
     r1[flip], r2[flip] = r2[flip].copy(), r1[flip].copy()
     letters = np.frombuffer(b"ACGT", np.uint8)
+    digits = np.empty((n, 9), np.uint8)  # "@r%09d/%c\n" headers, vectorised
+    idx = np.arange(n, dtype=np.int64)
+    for k in range(9):
+        digits[:, 8 - k] = 48 + (idx // 10 ** k) % 10
     for path, reads, tag in ((path1, r1, b"1"), (path2, r2, b"2")):
-        hit = rng.random(reads.shape) < err
-        reads[hit] = (reads[hit] + rng.integers(1, 4, size=int(hit.sum()), dtype=np.uint8)) % 4
-        head = np.array([b"@r%09d/%s\n" % (i, tag) for i in range(n)], dtype="S14").view(np.uint8).reshape(n, 14)
+        flat = reads.reshape(-1)  # errors: a binomial count of distinct positions
+        pos = np.unique(rng.integers(0, flat.size, size=int(rng.binomial(flat.size, err)), dtype=np.int64))
+        flat[pos] = (flat[pos] + rng.integers(1, 4, size=len(pos), dtype=np.uint8)) % 4
         rec = np.empty((n, 14 + read_len + 3 + read_len + 1), np.uint8)
-        rec[:, :14] = head
+        rec[:, 0], rec[:, 1] = ord("@"), ord("r")
+        rec[:, 2:11] = digits
+        rec[:, 11], rec[:, 12], rec[:, 13] = ord("/"), tag[0], ord("\n")
         rec[:, 14:14 + read_len] = letters[reads]
         rec[:, 14 + read_len:17 + read_len] = np.frombuffer(b"\n+\n", np.uint8)
         rec[:, 17 + read_len:17 + 2 * read_len] = ord("I")

@@ -2638,23 +2638,13 @@ static void check_input(const char *path)
  * in order, the first path (or --sample-name) as the name. */
 static void sketch_asone(char **files, int n, const sketch_opts_t *o, sample_keys_t *sm, kssd_infile_meta_t *mt)
 {
-	kssd_u64vec_t *part = xcalloc((size_t)n, sizeof *part);
 	int bad = -1;
-	if (kssd_collect_files(files, n, o->drfold, o->threads, part, &bad, NULL, NULL) != 0)
-		err(EXIT_FAILURE, "%s", bad >= 0 ? files[bad] : "inputs");
-	size_t total = 0;
-	for (int i = 0; i < n; ++i)
-		total += part[i].n;
-	sm->keys.a = xmalloc((total ? total : 1) * sizeof *sm->keys.a);
-	sm->keys.cap = total ? total : 1;
-	for (int i = 0; i < n; ++i) {
-		if (part[i].n)
-			memcpy(sm->keys.a + sm->keys.n, part[i].a, part[i].n * sizeof *part[i].a);
-		sm->keys.n += part[i].n;
-		kssd_u64vec_free(&part[i]);
+	if (kssd_sketch_pooled(files, n, o->drfold, o->threads, o->conflict, &o->counts, &sm->keys,
+			       o->abundance ? &sm->counts : NULL, &sm->filter, &bad) != 0) {
+		if (bad < 0)
+			errx(EXIT_FAILURE, "out of memory sketching the pooled inputs");
+		err(EXIT_FAILURE, "%s", files[bad]);
 	}
-	free(part);
-	finish_sample(sm, o);
 	if (!o->meta)
 		return;
 	int fmt = kssd_infile_fmt_from_path(files[0]);

@@ -106,6 +106,18 @@ kssd_count_filter_t kssd_apply_count_filters(uint64_t *keys, uint32_t *counts, s
 void kssd_print_count_filter(FILE *fp, const kssd_count_opts_t *o, const char *name,
 			     const kssd_count_filter_t *r, int leading_newline);
 
+/* The keys of n files pooled as one sample, as `sketch --asone` makes them:
+ * every window of every file collected (kssd_collect_files), then, when the
+ * count options need counts, counted and filtered by count (the QC range
+ * recorded, or applied with reads_qc), else sorted and merged; contexts with
+ * several objects dropped unless keep_conflict (--conflict).  *counts
+ * (malloc'd, may be NULL when counts are not needed or keys is empty) is
+ * aligned with keys; *filter the count filter's result.  0, or -1 with *bad
+ * the failing file (or -1 for an allocation). */
+int kssd_sketch_pooled(char *const *paths, int n, int drfold, int threads, int keep_conflict,
+		       const kssd_count_opts_t *co, kssd_u64vec_t *keys, uint32_t **counts, kssd_count_filter_t *filter,
+		       int *bad);
+
 /* GET_SKETCHING_ID(hclen, holen, iolen, drfold, FILTER), command_sketch.h:49-52
  * and command_sketch.c:771.  The coden pattern passes (NUM_CODENS, 0, 0, drfold)
  * (command_sketch_wrapper.c:924). */
@@ -124,6 +136,15 @@ typedef struct {
 
 /* Generate genome `spec` into codes (malloc'd, *len set). */
 int kssd_sim_genome(const kssd_simspec_t *spec, uint8_t **codes, size_t *len);
+/* Paired-end reads of a simulated genome (2-bit codes) as two plain FASTQ
+ * files: coverage * len / (2 read_len) fragments of N(insert, sd) bp
+ * (clipped to [read_len, len - 1]) at uniform starts, read 1 the fragment's
+ * first read_len bases and read 2 the reverse complement of its last, the
+ * mates swapped for half the fragments (the reverse strand); i.i.d. per-base
+ * substitution errors at rate err; a constant quality line.  Deterministic
+ * in seed.  Returns the number of pairs, or -1. */
+long kssd_sim_fastq_pair(const uint8_t *codes, size_t len, double coverage, uint64_t seed, int read_len,
+			 int insert, int sd, double err, const char *path1, const char *path2);
 
 /*
  * Simulate and sketch n genomes in parallel (nthreads OpenMP threads).

@@ -459,3 +459,60 @@ void kssd_print_count_filter(FILE *fp, const kssd_count_opts_t *o, const char *n
 		fprintf(fp, "--npercentile %.6g count cutoff for %s: %u\n", o->percentile, name, r->cutoff);
 	}
 }
+
+int kssd_sketch_pooled(char *const *paths, int n, int drfold, int threads, int keep_conflict,
+		       const kssd_count_opts_t *co, kssd_u64vec_t *keys, uint32_t **counts, kssd_count_filter_t *filter,
+		       int *bad)
+{
+	/* sketch_few_files_with_intrafile_parallel (command_sketch.c:5435-5497)
+	 * with finalize_filter_conflict_and_write (:5194-5214) */
+	memset(keys, 0, sizeof *keys);
+	if (counts)
+		*counts = NULL;
+	if (filter)
+		memset(filter, 0, sizeof *filter);
+	kssd_u64vec_t *part = calloc((size_t)(n > 0 ? n : 1), sizeof *part);
+	if (!part)
+		return -1;
+	if (kssd_collect_files(paths, n, drfold, threads, part, bad, NULL, NULL) != 0) {
+		for (int i = 0; i < n; ++i)
+			kssd_u64vec_free(&part[i]);
+		free(part);
+		return -1;
+	}
+	size_t total = 0;
+	for (int i = 0; i < n; ++i)
+		total += part[i].n;
+	keys->a = malloc((total ? total : 1) * sizeof *keys->a);
+	if (!keys->a) {
+		for (int i = 0; i < n; ++i)
+			kssd_u64vec_free(&part[i]);
+		free(part);
+		return -1;
+	}
+	keys->cap = total ? total : 1;
+	for (int i = 0; i < n; ++i) {
+		if (part[i].n)
+			memcpy(keys->a + keys->n, part[i].a, part[i].n * sizeof *part[i].a);
+		keys->n += part[i].n;
+		kssd_u64vec_free(&part[i]);
+	}
+	free(part);
+	if (!kssd_count_opts_need_counts(co)) {
+		kssd_finalize_keys(keys, keep_conflict);
+		return 0;
+	}
+	uint32_t *c = NULL;
+	if (kssd_count_keys(keys, &c) != 0)
+		return -1;
+	const kssd_count_filter_t f = kssd_apply_count_filters(keys->a, c, &keys->n, co);
+	if (filter)
+		*filter = f;
+	if (!keep_conflict)
+		kssd_drop_conflicts(keys->a, c, &keys->n);
+	if (counts)
+		*counts = c;
+	else
+		free(c);
+	return 0;
+}

@@ -372,6 +372,94 @@ static void substitute(rng_t *r, uint8_t *g, size_t n, double rate)
 	}
 }
 
+long kssd_sim_fastq_pair(const uint8_t *codes, size_t len, double coverage, uint64_t seed, int read_len,
+			 int insert, int sd, double err, const char *path1, const char *path2)
+{
+	if (!codes || read_len < 1 || len < (size_t)read_len + 1 || coverage <= 0 || !path1 || !path2)
+		return -1;
+	const long n = (long)(coverage * (double)len / (2.0 * read_len));
+	FILE *f[2] = {fopen(path1, "w"), fopen(path2, "w")};
+	const size_t rec = 14 + (size_t)read_len + 3 + (size_t)read_len + 1;
+	const size_t chunk = 4096;
+	char *buf[2] = {malloc(rec * chunk), malloc(rec * chunk)};
+	uint8_t *mate = malloc((size_t)read_len * 2);
+	if (!f[0] || !f[1] || !buf[0] || !buf[1] || !mate) {
+		for (int m = 0; m < 2; ++m) {
+			if (f[m])
+				fclose(f[m]);
+			free(buf[m]);
+		}
+		free(mate);
+		return -1;
+	}
+	static const char acgt[4] = {'A', 'C', 'G', 'T'};
+	rng_t r;
+	rng_seed(&r, seed);
+	const double lq = err > 0 && err < 1 ? log1p(-err) : 0.0;
+	double next_err = -1; /* bases to the next error (geometric) */
+	size_t fill = 0;
+	int ok = 1;
+	for (long i = 0; i < n && ok; ++i) {
+		/* fragment length N(insert, sd) by Box-Muller, clipped */
+		const double u1 = rng_unif(&r), u2 = rng_unif(&r);
+		long fl = (long)(insert + sd * sqrt(-2.0 * log1p(-u1)) * cos(6.283185307179586 * u2));
+		if (fl < read_len)
+			fl = read_len;
+		if ((size_t)fl > len - 1)
+			fl = (long)len - 1;
+		const size_t start = (size_t)(rng_unif(&r) * (double)(len - (size_t)fl));
+		uint8_t *a = mate, *b = mate + read_len;
+		for (int k = 0; k < read_len; ++k) {
+			a[k] = codes[start + (size_t)k];
+			b[k] = (uint8_t)(3 - codes[start + (size_t)fl - 1 - (size_t)k]);
+		}
+		if (rng_next(&r) & 1) { /* a fragment of the reverse strand: mates swapped */
+			uint8_t *t = a;
+			a = b;
+			b = t;
+		}
+		uint8_t *m2[2] = {a, b};
+		for (int m = 0; m < 2; ++m) {
+			uint8_t *q = m2[m];
+			if (lq < 0) { /* i.i.d. per-base errors by geometric skipping */
+				for (int k = 0;;) {
+					if (next_err < 0)
+						next_err = floor(log1p(-rng_unif(&r)) / lq);
+					if (next_err >= (double)(read_len - k)) {
+						next_err -= (double)(read_len - k);
+						break;
+					}
+					k += (int)next_err;
+					q[k] = (uint8_t)((q[k] + 1 + rng_next(&r) % 3) & 3);
+					next_err = -1;
+					++k;
+				}
+			}
+			char *o = buf[m] + fill * rec;
+			snprintf(o, 15, "@r%09ld/%c", i % 1000000000L, m ? '2' : '1');
+			o[13] = '\n';
+			for (int k = 0; k < read_len; ++k)
+				o[14 + k] = acgt[q[k]];
+			memcpy(o + 14 + read_len, "\n+\n", 3);
+			memset(o + 17 + read_len, 'I', (size_t)read_len);
+			o[rec - 1] = '\n';
+		}
+		if (++fill == chunk || i + 1 == n) {
+			for (int m = 0; m < 2; ++m)
+				if (fwrite(buf[m], rec, fill, f[m]) != fill)
+					ok = 0;
+			fill = 0;
+		}
+	}
+	for (int m = 0; m < 2; ++m) {
+		if (fclose(f[m]) != 0)
+			ok = 0;
+		free(buf[m]);
+	}
+	free(mate);
+	return ok ? n : -1;
+}
+
 int kssd_sim_genome(const kssd_simspec_t *spec, uint8_t **codes, size_t *len)
 {
 	rng_t r;

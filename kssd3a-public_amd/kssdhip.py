@@ -568,6 +568,17 @@ def default_threads():
     return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
 
 
+class CountOpts(ctypes.Structure):
+    """kssd_count_opts_t: the sketcher's k-mer count options (-n,
+    --npercentile, --ncap, --readsQC, -A)."""
+    _fields_ = [("min_count", I), ("percentile", D), ("cap", I), ("reads_qc", I), ("record_qc", I)]
+
+
+class CountFilter(ctypes.Structure):
+    _fields_ = [("raw_cutoff", U32), ("cutoff", U32), ("upper", U32), ("mode", U32), ("used_reads_qc", I),
+                ("qc", U32 * 4)]
+
+
 class Host:
     """numpy-facing wrapper of libkssdhost.so."""
     _libc = ctypes.CDLL(None)
@@ -584,6 +595,41 @@ class Host:
         out = np.ctypeslib.as_array(v.a, shape=(v.n,)).copy() if v.n else np.zeros(0, np.uint64)
         self.lib.kssd_u64vec_free(ctypes.byref(v))
         return out
+
+    def sketch_pooled(self, paths, drfold=8, keep_conflict=True, record_qc=True, threads=None):
+        """The read-set sketch of `sketch --asone [--conflict] [-A]` over
+        FASTA/FASTQ files, in process: (sorted keys, per-key counts)."""
+        arr = (ctypes.c_char_p * len(paths))(*[str(p).encode() for p in paths])
+        v = U64Vec()
+        counts = ctypes.POINTER(U32)()
+        flt = CountFilter()
+        bad = I(-1)
+        co = CountOpts(1, 0.0, 0, 0, int(record_qc))
+        self.lib.kssd_sketch_pooled.argtypes = [V, I, I, I, I, ctypes.POINTER(CountOpts), ctypes.POINTER(U64Vec),
+                                                ctypes.POINTER(ctypes.POINTER(U32)), ctypes.POINTER(CountFilter),
+                                                ctypes.POINTER(I)]
+        if self.lib.kssd_sketch_pooled(ctypes.cast(arr, V), len(paths), drfold, int(threads or default_threads()),
+                                       int(keep_conflict), ctypes.byref(co), ctypes.byref(v), ctypes.byref(counts),
+                                       ctypes.byref(flt), ctypes.byref(bad)) != 0:
+            raise OSError(f"cannot sketch {paths[bad.value] if bad.value >= 0 else 'the inputs'}")
+        keys = np.ctypeslib.as_array(v.a, shape=(v.n,)).copy() if v.n else np.zeros(0, np.uint64)
+        cnt = np.ctypeslib.as_array(counts, shape=(v.n,)).copy() if (v.n and counts) else np.zeros(0, np.uint32)
+        self.lib.kssd_u64vec_free(ctypes.byref(v))
+        if counts:
+            self._libc.free(ctypes.cast(counts, V))
+        return keys, cnt
+
+    def sim_fastq_pair(self, codes, coverage, seed, path1, path2, read_len=150, insert=300, sd=30, err=0.01):
+        """Paired 2 x read_len FASTQ files of a genome (2-bit codes) at
+        `coverage` (kssd_sim_fastq_pair); returns the number of pairs."""
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        self.lib.kssd_sim_fastq_pair.argtypes = [V, S, D, U64, I, I, I, D, ctypes.c_char_p, ctypes.c_char_p]
+        self.lib.kssd_sim_fastq_pair.restype = ctypes.c_long
+        n = self.lib.kssd_sim_fastq_pair(_p(codes), len(codes), float(coverage), int(seed), read_len, insert, sd,
+                                         float(err), str(path1).encode(), str(path2).encode())
+        if n < 0:
+            raise OSError(f"cannot write {path1} / {path2}")
+        return int(n)
 
     def simsketch(self, specs, drfold=8, keep_conflict=False, threads=None):
         arr = (SimSpec * len(specs))(*specs)

@@ -65,3 +65,60 @@ def test_simulator_is_deterministic_and_mutation_scaled(host):
     shared = [len(np.intersect1d(a[ia[0]:ia[1]] >> np.uint64(20), a[ia[i]:ia[i + 1]] >> np.uint64(20)))
               for i in (1, 2)]
     assert n0 > shared[0] > shared[1] > 0
+
+
+def test_sim_fastq_pair_reads_come_from_the_genome(tmp_path):
+    """The paired-read simulator behind the read-set workloads: with no
+    errors every read is a substring of the genome or of its reverse
+    complement, mates face each other within the insert size; with 1%
+    errors about 1% of the bases differ from the best placement."""
+    host = kt.Host()
+    g = host.sim_genome(kt.SimSpec(77, 20000, 3, 0.0, 0.0, 0, 0.0))
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    fwd = letters[g].tobytes().decode()
+    rev = letters[3 - g[::-1]].tobytes().decode()
+
+    def reads(err):
+        a, b = tmp_path / f"r{err}_1.fq", tmp_path / f"r{err}_2.fq"
+        n = host.sim_fastq_pair(g, 10.0, 5, a, b, err=err)
+        ra = a.read_text().splitlines()[1::4]
+        rb = b.read_text().splitlines()[1::4]
+        assert len(ra) == len(rb) == n == int(10.0 * len(g) / 300)
+        assert all(len(x) == 150 for x in ra + rb)
+        return ra, rb
+
+    ra, rb = reads(0.0)
+    for x, y in zip(ra[:200], rb[:200]):
+        if x in fwd:
+            i, j = fwd.find(x), rev.find(y)
+            assert j >= 0 and 150 <= (len(g) - j) - i <= 600  # mate on the other strand, downstream
+        else:
+            assert x in rev and y in fwd
+    ra, rb = reads(0.01)
+    diff = total = 0
+    for x in ra[:300]:
+        place = [sum(p != q for p, q in zip(x, s[i:i + 150]))
+                 for s in (fwd, rev) for o in range(0, 121, 30) for i in [s.find(x[o:o + 30]) - o]
+                 if s.find(x[o:o + 30]) >= 0 and 0 <= i <= len(s) - 150]
+        if place:  # an error-free 30-mer anchors the read
+            diff += min(place)
+            total += 150
+    assert total >= 150 * 290 and 0.005 < diff / total < 0.02
+
+
+def test_pooled_sketch_equals_cli_asone(tmp_path):
+    """Host.sketch_pooled (kssd_sketch_pooled, in process) gives the keys and
+    counts `kssd3a-hip sketch --asone --conflict -A` writes to comblco and
+    comblco.a for the same read files."""
+    import subprocess
+    host = kt.Host()
+    g = host.sim_genome(kt.SimSpec(91, 60000, 4, 0.0, 0.0, 0, 0.0))
+    a, b = tmp_path / "s_1.fq", tmp_path / "s_2.fq"
+    host.sim_fastq_pair(g, 15.0, 9, a, b)
+    keys, counts = host.sketch_pooled([a, b], threads=2)
+    p = subprocess.run([str(kt.PKG / "kssd3a-hip"), "sketch", "--asone", "--conflict", "-A", "-p", "2", "-o",
+                        str(tmp_path / "R"), str(a), str(b)], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    assert np.array_equal(keys, np.fromfile(tmp_path / "R" / "comblco", "<u8"))
+    assert np.array_equal(counts, np.fromfile(tmp_path / "R" / "comblco.a", "<u4"))
+    assert len(keys) > 1000 and (np.diff(keys >> np.uint64(20)) == 0).any()  # conflicts kept
