@@ -147,24 +147,49 @@ def matched_events(records, qkeys, obj_bits):
     return int((np.searchsorted(rctx, qctx, "right") - np.searchsorted(rctx, qctx, "left")).sum())
 
 
-def readset_like(rng, keys, coverage, obj_bits=20):
-    """Key set shaped like a read-set sketch (`--asone --conflict`) of a genome
-    read at `coverage`x with 1% per-base read errors: the genome's keys plus
-    sequencing-error entries, about 0.27 per key per unit of coverage
-    ((1 - 0.99^32) of the kept 32-mers hold an error).  10 of the 32 bases are
-    object bases, so 10/32 of the error entries keep a true context with one
-    object lane changed (a conflict); the rest land in new contexts."""
-    n = len(keys)
-    ne = int(0.27 * coverage * n)
-    n_obj = int(ne * 10 / 32)
-    src = rng.integers(0, max(n, 1), size=n_obj)
-    lane = rng.integers(0, obj_bits // 2, size=n_obj).astype(np.uint64)
-    flip = rng.integers(1, 4, size=n_obj).astype(np.uint64)
-    e1 = keys[src] ^ (flip << (np.uint64(2) * lane)) if n else np.zeros(0, np.uint64)
-    m = ne - n_obj
-    e2 = ((rng.integers(1, 1 << 44, size=m, dtype=np.uint64) << np.uint64(obj_bits))
-          | rng.integers(0, 1 << obj_bits, size=m, dtype=np.uint64))
-    return np.unique(np.concatenate([keys, e1, e2]))
+def scratch_dir(need_bytes):
+    """A directory for large temporary files: memory-backed /dev/shm when it
+    has room for need_bytes, else /tmp."""
+    import shutil
+    for d in ("/dev/shm", "/tmp"):
+        if os.path.isdir(d) and shutil.disk_usage(d).free > need_bytes:
+            return d
+    return "/tmp"
+
+
+def real_readsets(host, specs, rng, threads, seed, cov_range=(10.0, 30.0)):
+    """Read-set sketches made from reads, as the reference makes them: for
+    each genome spec, 2 x 150 bp pairs (insert N(300, 30)) at U(cov_range)x
+    with 1% per-base errors are written as plain FASTQ to memory-backed files
+    (kssd_sim_fastq_pair) and sketched by the package's `sketch --asone
+    --conflict -A` code path in process (kssd_sketch_pooled, -f8), one
+    genome per worker thread.  Returns (keys, offsets, read bases)."""
+    import shutil
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+    covs = rng.uniform(*cov_range, size=len(specs))
+    tmp = tempfile.mkdtemp(dir=scratch_dir(threads * 400 << 20))
+
+    def one(i):
+        g = host.sim_genome(specs[i])
+        a, b = f"{tmp}/s{i}_1.fq", f"{tmp}/s{i}_2.fq"
+        n = host.sim_fastq_pair(g, covs[i], seed * 1_000_003 + i, a, b)
+        try:
+            k, _ = host.sketch_pooled([a, b], threads=1)
+        finally:
+            os.unlink(a)
+            os.unlink(b)
+        return k, 2 * 150 * n
+
+    try:
+        with ThreadPoolExecutor(max(1, threads)) as ex:
+            res = list(ex.map(one, range(len(specs))))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    keys = np.concatenate([k for k, _ in res]) if res else np.zeros(0, np.uint64)
+    idx = np.zeros(len(res) + 1, np.uint64)
+    idx[1:] = np.cumsum([len(k) for k, _ in res])
+    return keys, idx, int(sum(b for _, b in res))
 
 
 def clade_specs(rng, nref, seed):
@@ -502,17 +527,27 @@ def c3_bench(args, host, threads):
     # dense counts of one reference-sized block by both joins agree cell for cell
     nb = min(1024, nq)
     d_c1, d_o1, d_c2, d_o2 = (rt.malloc(8 * nb * nref) for _ in range(4))
-    hip.set_tiling(h, mode=3)
-    hip.count_block_dev(h, d_k, d_i, nb, int(qi[nb]), 0, 0, d_c1, d_o1, None, None, stream)
-    hip.set_tiling(h, mode=2, chunk_records=5696, block_threads=1024, group_partitions=4)
-    t_p = rt.event(), rt.event()
-    rt.record(t_p[0], stream)
-    hip.count_block_dev(h, d_k, d_i, nb, int(qi[nb]), 0, 0, d_c2, d_o2, None, None, stream)
-    rt.record(t_p[1], stream)
-    rt.sync(stream)
+    forms = {}
+    for form, tiling, dc, do in (("persistent_query_major", dict(mode=3), d_c1, d_o1),
+                                 ("static_partitioned", dict(mode=2, chunk_records=5696, block_threads=1024,
+                                                             group_partitions=4), d_c2, d_o2)):
+        hip.set_tiling(h, **tiling)
+        hip.count_block_dev(h, d_k, d_i, nb, int(qi[nb]), 0, 0, dc, do, None, None, stream)  # warm
+        t_p = rt.event(), rt.event()
+        rt.record(t_p[0], stream)
+        for _ in range(3):
+            hip.count_block_dev(h, d_k, d_i, nb, int(qi[nb]), 0, 0, dc, do, None, None, stream)
+        rt.record(t_p[1], stream)
+        rt.sync(stream)
+        forms[form] = {"ms": rt.elapsed_ms(*t_p) / 3, "tiling": tiling}
+    forms["what"] = (f"dense counts of the first {nb} queries x {nref} refs: the persistent query-major join "
+                     "(workgroups take queries from a device work queue, largest first) against the static "
+                     "partitioned merge-join grid (one workgroup per index-partition group)")
+    c1_cells = rt.download(d_c1, np.uint64, nb * nref)
     props["query_major_equals_partitioned_on_a_block"] = bool(
-        np.array_equal(rt.download(d_c1, np.uint64, nb * nref), rt.download(d_c2, np.uint64, nb * nref))
+        np.array_equal(c1_cells, rt.download(d_c2, np.uint64, nb * nref))
         and np.array_equal(rt.download(d_o1, np.uint64, nb * nref), rt.download(d_o2, np.uint64, nb * nref)))
+    p_touched_block = int(((c1_cells & np.uint64(0xffffffff)) > 0).sum())
     hip.set_tiling(h, mode=0)
     for d in (d_c1, d_o1, d_c2, d_o2):
         rt.free(d)
@@ -523,6 +558,18 @@ def c3_bench(args, host, threads):
     kern_ms = prof.get("k_qjoin_avg_ms") if prof else None
     t_launch = float(np.mean(launch_ms)) / 1e3
     traffic = int(prof["traffic_bytes_per_launch"]) if prof and "traffic_bytes_per_launch" in prof else None
+    dense_algo = 8 * total + 12 * M + 16 * pairs
+    line_rate = hip.probe_random_reads()
+    dram_req = prof.get("dram_read_requests_per_launch") if prof else None
+    random_access = {
+        "what": "k_qjoin is bound by independent random line fetches (a filter word, a context-table slot, a record "
+                "run per query key), not by streamed bytes: its DRAM line requests per second against the measured "
+                "rate of random 8-byte reads from a 2 GiB buffer (kssdhip_probe_random_reads, this run)",
+        "probe_reads_per_s": line_rate,
+        "k_qjoin_line_requests_per_launch": dram_req,
+        "k_qjoin_line_requests_per_s": dram_req / (kern_ms / 1e3) if dram_req and kern_ms else None,
+        "frac": dram_req / (kern_ms / 1e3) / line_rate if dram_req and kern_ms else None,
+        "source": prof_src}
     roof = {"bound": "hbm", "achieved": algo / t_launch / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": algo / t_launch / 1e9 / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": prof_src,
             "traffic_GBs": traffic / t_launch / 1e9 if traffic else None, "kernel": "k_qjoin (select)",
@@ -530,7 +577,12 @@ def c3_bench(args, host, threads):
             "P_out": n_rows, "index_records": int(len(rec)), "avg_launch_ms": t_launch * 1e3,
             "avg_launch_what": "HIP events around kssdhip_select_block_dev on its stream (order + k_qjoin + "
                                "row values)",
-            "k_qjoin_avg_ms_rocprof": kern_ms}
+            "k_qjoin_avg_ms_rocprof": kern_ms,
+            "P_out_note": "the select writes only passing pairs, so P_out = passing pairs; with a dense Q x R output "
+                          "(P_out = Q*R) the same launch would move dense_output_bytes",
+            "dense_output_bytes": dense_algo, "frac_dense_output_model": dense_algo / t_launch / 1e9 / HBM_PEAK_GBS,
+            "P_touched_first_block": p_touched_block, "P_touched_block_queries": nb,
+            "random_access": random_access}
     # CPU baseline (the oracle's block count with per-block fenceposts and
     # guided OpenMP, then its restatement of the gate) on the first block
     sys.path.insert(0, str(ROOT / "tests"))
@@ -567,7 +619,7 @@ def c3_bench(args, host, threads):
                  "wall_ms_per_step_eager": wall / args.steps * 1e3},
         "gate": {"ctxcut": 3, "afcut": 0.5, "anicut": 0.95, "value": "CtxMoE ANI (FP64, float AFs)"},
         "parity": {"cpu_sample_queries": ncpu, "passing_pairs_bit_exact": exact, "properties": props},
-        "partitioned_join_ms_per_1024_queries": rt.elapsed_ms(*t_p),
+        "forms": forms,
         "roofline": roof,
         "cpu_baseline": {"value": cpu_pps, "unit": "pairs/s", "cores": threads, "kind": "port",
                          "sample": f"first {ncpu} queries x {nref} refs: oracle block count (fenceposts per block, "
@@ -577,7 +629,7 @@ def c3_bench(args, host, threads):
     for d in (d_k, d_i, d_qc, d_rc, d_rows, d_n, d_first, d_cnt):
         rt.free(d)
     shared = {"hip": hip, "rt": rt, "h": h, "rec": rec, "rctx": rctx, "specs": specs, "qk": qk, "qi": qi,
-              "nref": nref, "cseed": cseed, "clen": clen}
+              "nref": nref, "cseed": cseed, "clen": clen, "rk": rk, "ri": ri, "rows": rows}
     return res, shared
 
 
@@ -586,9 +638,11 @@ def index_bench(args, host, threads, plain_s_per_pair, shared):
     launch each into dense blocks, with inputs resident in HBM:
       assemblies  the first args.index_queries of the C3 queries (one
                   reference-style block), dense counts + FP64 CtxMoE ANI;
-      read sets   args.index_readsets samples (a reference + 0.5%
-                  substitutions, read at U(10, 30)x, see readset_like),
-                  counted with the unique-owner counters (--qraw --unique-ctx).
+      read sets   C4: args.index_readsets samples, each a reference genome
+                  plus 0.5% substitutions read as 2 x 150 bp pairs at
+                  U(10, 30)x with 1% errors and sketched with the package's
+                  `sketch --asone --conflict -A` (see real_readsets), counted
+                  with the unique-owner counters (--qraw --unique-ctx).
     Beside each: the oracle (C/OpenMP restatement of the reference's block
     loop) on a bounded prefix of the same queries, and the plain kernel's
     measured per-pair time."""
@@ -604,14 +658,15 @@ def index_bench(args, host, threads, plain_s_per_pair, shared):
         r = specs[int(rng.integers(0, nref))]
         rs.append(kh.SimSpec(r.anc_seed, r.anc_len, int(args.seed * 6_000_011 + i + 1), 0.005, 0.0,
                              r.mut_seed, r.sub_rate))
-    sk, si = host.simsketch(rs, threads=threads)
-    reads = [readset_like(rng, sk[si[i]:si[i + 1]], rng.uniform(10, 30)) for i in range(nrs)]
-    del sk
-    rkeys = np.concatenate(reads)
-    ridx = np.zeros(nrs + 1, np.uint64)
-    ridx[1:] = np.cumsum([len(x) for x in reads])
+    t = time.time()
+    rkeys, ridx, read_bases = real_readsets(host, rs, rng, threads, args.seed + 300)
+    log(f"C4: {nrs} read sets ({read_bases / 1e9:.1f} Gbp of simulated reads) sketched in {time.time() - t:.1f}s, "
+        f"median {int(np.median(np.diff(ridx)))} entries")
     stream = rt.stream()
-    out = {"index_genomes": nref, "index_records": int(len(rec))}
+    out = {"index_genomes": nref, "index_records": int(len(rec)),
+           "read_sets": {"what": "C4: read sets sketched from simulated 2x150 bp reads (U(10,30)x, 1% errors) by "
+                                 "the package's sketch --asone --conflict -A path", "read_bases": read_bases,
+                         "sketch_s": time.time() - t}}
     orc = None
     for name, keys, idx, flags in (("assemblies", qk, qi, 0), ("read_sets", rkeys, ridx, kh.UNIQUE)):
         n = len(idx) - 1
@@ -674,7 +729,8 @@ def index_bench(args, host, threads, plain_s_per_pair, shared):
                                                   c_uc))
         cpu_pps = ncpu * nref / cdt
         plain_pps = 1.0 / plain_s_per_pair
-        out[name] = {"queries": n, "query_entries": total, "pairs": cells, "ms": ms, "pairs_per_s": pairs_per_s,
+        out[name] = {**out.get(name, {}), "queries": n, "query_entries": total, "pairs": cells, "ms": ms,
+                     "pairs_per_s": pairs_per_s,
                      "over_plain": pairs_per_s / plain_pps, "cpu_pairs_per_s": cpu_pps, "cpu_queries": ncpu,
                      "over_cpu": pairs_per_s / cpu_pps, "counts_bit_exact_on_cpu_sample": exact,
                      "flags": "unique-ctx" if unique else "none", "tiling": tiling}
@@ -727,7 +783,7 @@ def stream_bench(args, host, hip, rt, h, qk, qi, nref, rctx):
                              d_rows[buf], cap, d_n[buf], d_first, d_cnt, ks)
 
     # eager warm-up sizes the workspace; graphs per (buffer, block)
-    rt.copy_async(d_keys[0], ctypes.c_void_p(pin + 8 * int(qi[0])), 8 * int(qi[blocks[0][1]]), rt.H2D, ks)
+    rt.copy_async(d_keys[0], ctypes.c_void_p(pin.value + 8 * int(qi[0])), 8 * int(qi[blocks[0][1]]), rt.H2D, ks)
     compute(0, 0)
     rt.sync(ks)
     graphs = {}
@@ -751,7 +807,7 @@ def stream_bench(args, host, hip, rt, h, qk, qi, nref, rctx):
         buf = k % 2
         if k >= 2:
             rt.wait_event(cs, done[buf])  # block k - 2 is done with this buffer
-        rt.copy_async(d_keys[buf], ctypes.c_void_p(pin + 8 * int(qi[b])), 8 * int(qi[e] - qi[b]), rt.H2D, cs)
+        rt.copy_async(d_keys[buf], ctypes.c_void_p(pin.value + 8 * int(qi[b])), 8 * int(qi[e] - qi[b]), rt.H2D, cs)
         rt.record(copied[buf], cs)
         rt.wait_event(ks, copied[buf])
         rt.graph_launch(graphs[(buf, k)][1], ks)
@@ -782,17 +838,164 @@ def stream_bench(args, host, hip, rt, h, qk, qi, nref, rctx):
     return res
 
 
+def cli_e2e(args, host, shared, threads):
+    """`kssd3a-hip ani -r REF -q QRY` end to end on the C3 sketches written as
+    sketch directories (the reference index with its sorted index file, the
+    first args.cli_queries queries): the drop-in CLI reads the files, picks
+    the index path, runs the device pass gate block by block and ranks and
+    prints on the host with the reference's defaults.  The printed pairs must
+    be among the device gate's passing pairs of the headline step (the host
+    adds the selected-ANI cut, command_ani.c:3517)."""
+    import shutil
+    import subprocess
+    import tempfile
+    nq = min(args.cli_queries, len(shared["qi"]) - 1)
+    qi = shared["qi"][: nq + 1]
+    qk = shared["qk"][: int(qi[-1])]
+    rk, ri, rec, nref = shared["rk"], shared["ri"], shared["rec"], shared["nref"]
+    need = 8 * (len(qk) + len(rk)) + 14 * len(rec) + (4 << 30)
+    d = Path(tempfile.mkdtemp(dir=scratch_dir(need)))
+    try:
+        t = time.time()
+        host.write_sketch(d / "ref", rk, ri, [f"ref{i:05d}" for i in range(nref)], records=rec)
+        host.write_sketch(d / "qry", qk, qi, [f"qry{i:06d}" for i in range(nq)])
+        write_s = time.time() - t
+        argv = [str(kh.PKG / "kssd3a-hip"), "ani", "-r", str(d / "ref"), "-q", str(d / "qry"), "-p", str(threads),
+                "-o", str(d / "out.tsv")]
+        t = time.perf_counter()
+        p = subprocess.run(argv, capture_output=True, text=True, env=dict(os.environ, KSSDHIP_TIMING="1"),
+                           timeout=900)
+        wall = time.perf_counter() - t
+        if p.returncode:
+            raise RuntimeError(f"kssd3a-hip ani: {p.stderr[-500:]}")
+        stages = [ln for ln in p.stderr.splitlines() if ln.startswith("[kssd3a-hip")]
+        lines = (d / "out.tsv").read_text().splitlines()
+        rows = shared["rows"]
+        keep = rows["query"] < nq
+        gate = set(zip(rows["query"][keep].tolist(), rows["ref"][keep].tolist()))
+        printed = {(int(c[0][3:]), int(c[1][3:])) for c in (ln.split("\t") for ln in lines[1:])}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    res = {"command": "kssd3a-hip ani -r REF -q QRY -p THREADS -o OUT (reference defaults: -s1 Best, ctxcut 3, "
+                      "afcut 0.5, anicut 0.95)",
+           "queries": nq, "refs": nref, "pairs": nq * nref, "wall_s": wall, "pairs_per_s": nq * nref / wall,
+           "files_written_s": write_s, "printed_rows": len(printed),
+           "printed_within_device_gate": printed <= gate, "device_gate_pairs": len(gate),
+           "stages": stages[-12:]}
+    log(f"CLI end to end: {nq} queries x {nref} refs in {wall:.2f} s = {nq * nref / wall:.3e} pairs/s; "
+        f"{len(printed)} rows printed, within the device gate's {len(gate)}: {printed <= gate}")
+    return res
+
+
+def c1_bench(host, hip, rt, threads, seed):
+    """C1: the all-vs-all self matrix of 64 genomes (8 clades of 8, 2 Mbp
+    ancestors, members U(1%, 15%) substitutions), by the indexed path
+    (lower-triangle counts against the self index + FP64 epilogue,
+    command_ani.c:1657-1760) and the direct path (one merge per pair,
+    command_ani.c:3314-3365), each on the GPU and in the oracle."""
+    rng = np.random.default_rng(seed)
+    n = 64
+    specs = [kh.SimSpec(7000 + c, 2_000_000, 8000 + 8 * c + m, float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0)
+             for c in range(8) for m in range(8)]
+    k, i = host.simsketch(specs, threads=threads)
+    rec = host.build_index(k, i, 20, threads=threads)
+    cnt = np.diff(i).astype(np.uint32)
+    total, cells = int(i[-1]), n * n
+    h = hip.open_index(rec, n, 20)
+    s = rt.stream()
+    d_k, d_i, d_cnt = rt.upload(k), rt.upload(i), rt.upload(cnt)
+    d_c, d_o, d_out = rt.malloc(8 * cells), rt.malloc(8 * cells), rt.malloc(8 * cells)
+    opts = kh.EpilogueOpts(3, 1, 0, 0, 1, 0, 8, 20, 44, 0.5, 1.0)
+    e0, e1 = rt.event(), rt.event()
+
+    def indexed():
+        hip.count_block_dev(h, d_k, d_i, n, total, 0, kh.LOWER, d_c, d_o, None, None, s)
+        hip.epilogue_dev(d_c, d_o, n, n, d_cnt, d_cnt, opts, d_out, s)
+
+    indexed()
+    rt.record(e0, s)
+    for _ in range(5):
+        indexed()
+    rt.record(e1, s)
+    rt.sync(s)
+    gpu_indexed_ms = rt.elapsed_ms(e0, e1) / 5
+    g_ctx = rt.download(d_c, np.uint32, 2 * cells).reshape(n, n, 2)
+    g_obj = rt.download(d_o, np.uint32, 2 * cells).reshape(n, n, 2)
+    pairs = [(q, r) for q in range(n) for r in range(q)]
+    d_pout = rt.malloc(20 * len(pairs))
+    d_ws = rt.malloc(kh.PAIR_WORKSPACE_BYTES)
+    hip.check(hip.lib.kssdhip_pair_workspace_init_dev(d_ws, s))
+
+    def direct():
+        for j, (q, r) in enumerate(pairs):
+            hip.pair_dev(d_k.value + 8 * int(i[q]), int(cnt[q]), d_k.value + 8 * int(i[r]), int(cnt[r]), 20, False,
+                         d_pout.value + 20 * j, d_ws, s)
+
+    direct()
+    rt.sync(s)
+    rt.record(e0, s)
+    direct()
+    rt.record(e1, s)
+    rt.sync(s)
+    gpu_direct_ms = rt.elapsed_ms(e0, e1)
+    pres = rt.download(d_pout, np.uint32, 5 * len(pairs)).reshape(len(pairs), 5)
+    qq = np.array([q for q, _ in pairs])
+    rr = np.array([r for _, r in pairs])
+    direct_ok = bool(np.array_equal(pres[:, 0], g_ctx[qq, rr, 0]) and np.array_equal(pres[:, 4], g_obj[qq, rr, 0])
+                     and np.array_equal(pres[:, 2], g_obj[qq, rr, 1]) and np.array_equal(pres[:, 3], g_ctx[qq, rr, 1]))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import kssdtest as kt
+    orc = kt.Oracle()
+    c_ctx = np.zeros((n, n, 2), np.uint32)
+    c_obj = np.zeros((n, n, 2), np.uint32)
+    vals = np.zeros(cells, np.float64)
+    t = time.perf_counter()
+    orc.lib.orc_count_block(kt._p(rec), len(rec), n, 20, kt._p(k), kt._p(i), n, 0, kt.FLAGS["LOWER"], kt._p(c_ctx),
+                            kt._p(c_obj), None, None, threads)
+    orc.lib.orc_matrix_values(kt._p(c_ctx), kt._p(c_obj), n, n, kt._p(cnt), kt._p(cnt), 3, 1, 0.5, 1.0, 8, 20, 44,
+                              kt._p(vals), threads)
+    cpu_indexed_ms = (time.perf_counter() - t) * 1e3
+    f = kt.Features()
+    t = time.perf_counter()
+    for q, r in pairs:
+        orc.lib.orc_pair_features(kt._p(k[i[q]:i[q + 1]]), int(cnt[q]), kt._p(k[i[r]:i[r + 1]]), int(cnt[r]), 20, 0,
+                                  ctypes.byref(f))
+    cpu_direct_ms = (time.perf_counter() - t) * 1e3
+    low = np.tril(np.ones((n, n), bool), -1)
+    exact = bool(np.array_equal(g_ctx[low], c_ctx[low]) and np.array_equal(g_obj[low], c_obj[low]))
+    for p_ in (d_k, d_i, d_cnt, d_c, d_o, d_out, d_pout, d_ws):
+        rt.free(p_)
+    hip.close_index(h)
+    res = {"genomes": n, "pairs": len(pairs), "entries": total,
+           "gpu_indexed_ms": gpu_indexed_ms, "gpu_direct_ms": gpu_direct_ms,
+           "cpu_indexed_ms": cpu_indexed_ms, "cpu_direct_ms": cpu_direct_ms, "cpu_threads": threads,
+           "cpu_direct_threads": 1,
+           "what": "indexed: lower-triangle counts against the self index + FP64 CtxMoE epilogue (one launch each); "
+                   "direct: one merge launch per pair (GPU) / one oracle merge per pair (CPU, one thread)",
+           "counts_bit_exact": exact, "direct_equals_indexed": direct_ok}
+    log(f"C1: indexed {gpu_indexed_ms:.3f} ms GPU vs {cpu_indexed_ms:.1f} ms CPU; direct {gpu_direct_ms:.1f} ms GPU "
+        f"vs {cpu_direct_ms:.1f} ms CPU; bit-exact {exact}, direct == indexed {direct_ok}")
+    return res
+
+
 def serve_bench(args, host, hip, threads, plain_s_per_pair):
     """Serving loop under Poisson load against a large resident index.
 
     Index: args.serve_refs genomes = clades of 20, each clade a random
     ancestor (lognormal length as above) whose members carry U(1%, 15%)
     substitutions.  Requests: 70% assemblies (a new member of a random clade)
-    and 30% read sets (a reference genome plus 0.5% substitutions, read at
-    U(5, 30)x, see readset_like), arriving as a Poisson stream at
-    args.serve_rate per second.  Latency = arrival -> counter rows on the
-    host, measured per request.  The plain kernel's per-query figure is its
-    measured per-pair time times the index size."""
+    and 30% read sets (drawn from 64 read-set sketches of reference genomes
+    plus 0.5% substitutions, read as 2 x 150 bp pairs at U(5, 30)x and
+    sketched with the package's `sketch --asone --conflict -A`, see
+    real_readsets), arriving as a Poisson stream at args.serve_rate per
+    second.  The server returns results: each batch runs the pass gate
+    (ctxcut 3, afcut 0.2, anicut 0.95) and the FP64 CtxMoE ANI on the device
+    and each request gets its ranked passing rows.  Latency = arrival ->
+    ranked hits on the host, measured per request.  CPU baselines per
+    request: the oracle's restatement of the reference's default one-query
+    path (stream_ref_sketches_one_qraw_lookup) and its forced-index path;
+    the plain kernel's per-query figure is its measured per-pair time times
+    the index size."""
     import threading
     rng = np.random.default_rng(args.seed + 100)
     nref = args.serve_refs - args.serve_refs % 20
@@ -801,38 +1004,46 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
     t = time.time()
     rk, ri = host.simsketch(specs, threads=threads)
     rec = host.build_index(rk, ri, 20, threads=threads)
-    del rk
+    rctx = host.ctx_counts(rk, ri, 20)
     log(f"serving index: {nref} genomes ({nclade} clades of 20), {len(rec)} records, built in "
         f"{time.time() - t:.1f}s")
     h = hip.open_index(rec, nref, 20)
     nq = args.serve_queries
     is_asm = rng.random(nq) < 0.7
-    qspecs, cov = [], np.zeros(nq)
+    asm_specs = []
+    for i in range(nq):
+        c = int(rng.integers(0, nclade))
+        asm_specs.append(kh.SimSpec(cseed[c], int(clen[c]), int(args.seed * 3_000_017 + i + 1),
+                                    float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0))
+    qk, qi = host.simsketch([asm_specs[i] for i in range(nq) if is_asm[i]], threads=threads)
+    n_rs = 64
+    rs_specs = []
+    for k in range(n_rs):  # a reference genome itself, then 0.5% sample substitutions
+        r = specs[int(rng.integers(0, nref))]
+        rs_specs.append(kh.SimSpec(r.anc_seed, r.anc_len, int(args.seed * 7_000_001 + k + 1), 0.005, 0.0,
+                                   r.mut_seed, r.sub_rate))
+    t = time.time()
+    sk, si, _ = real_readsets(host, rs_specs, rng, threads, args.seed + 100, cov_range=(5.0, 30.0))
+    log(f"serving read sets: {n_rs} sketched from simulated reads in {time.time() - t:.1f}s")
+    queries, a = [], 0
+    pick = rng.integers(0, n_rs, size=nq)
     for i in range(nq):
         if is_asm[i]:
-            c = int(rng.integers(0, nclade))
-            qspecs.append(kh.SimSpec(cseed[c], int(clen[c]), int(args.seed * 3_000_017 + i + 1),
-                                     float(rng.uniform(0.01, 0.15)), 0.0, 0, 0.0))
+            queries.append(qk[qi[a]:qi[a + 1]])
+            a += 1
         else:
-            r = specs[int(rng.integers(0, nref))]
-            # the reference genome itself, then 0.5% sample substitutions
-            qspecs.append(kh.SimSpec(r.anc_seed, r.anc_len, int(args.seed * 3_000_017 + i + 1), 0.005, 0.0,
-                                     r.mut_seed, r.sub_rate))
-            cov[i] = rng.uniform(5, 30)
-    qk, qi = host.simsketch(qspecs, threads=threads)
-    queries = []
-    for i in range(nq):
-        k = qk[qi[i]:qi[i + 1]]
-        queries.append(k if is_asm[i] else readset_like(rng, k, cov[i]))
-    del qk
+            queries.append(sk[si[pick[i]]:si[pick[i] + 1]])
     sizes = np.array([len(q) for q in queries])
     log(f"serving requests: {nq} ({int(is_asm.sum())} assemblies, median {int(np.median(sizes[is_asm]))} keys; "
         f"{int((~is_asm).sum())} read sets, median {int(np.median(sizes[~is_asm])) if (~is_asm).any() else 0} keys)")
     budget = 2 << 30
-    srv = hip.serve_open(h, budget_bytes=budget, slots=4, max_batch=32, max_queued=1 << 20)
+    value = kh.EpilogueOpts(3, 1, 0, 0, 0, 0, 8, 20, 44, 0.2, 1.0)
+    sel = kh.SelectOpts(3, 0.2, 0.95, 0, value)
+    srv = hip.serve_open(h, budget_bytes=budget, slots=4, max_batch=32, max_queued=1 << 20, select_opts=sel,
+                         ref_ctx=rctx)
     # warm the server (first launches load kernels)
     for q in queries[:8]:
-        hip.serve_wait(srv, hip.serve_submit(srv, q), nref)
+        hip.serve_wait_rows(srv, hip.serve_submit(srv, q))
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=nq))
     tickets = [None] * nq
     late = np.zeros(nq)
@@ -850,14 +1061,16 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
     th.start()
     lat = np.zeros(nq)
     bsz = np.zeros(nq, np.int64)
+    hits = np.zeros(nq, np.int64)
     for i in range(nq):
         while tickets[i] is None:
             time.sleep(0.0002)
         if tickets[i] < 0:
             raise RuntimeError(f"serve_submit failed: {tickets[i]}")
-        _, _, _, _, st = hip.serve_wait(srv, tickets[i], nref)
+        rows, st = hip.serve_wait_rows(srv, tickets[i], cap=4096)
         lat[i] = st.total_ms + late[i] * 1e3
         bsz[i] = st.batch_queries
+        hits[i] = len(rows)
     t_end = time.perf_counter()
     th.join()
     info = hip.serve_info(srv)
@@ -867,7 +1080,7 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
     burst_t = [hip.serve_submit(srv, q) for q in queries]
     blat, bbsz = np.zeros(nq), np.zeros(nq, np.int64)
     for i, tk in enumerate(burst_t):
-        _, _, _, _, st = hip.serve_wait(srv, tk, nref)
+        _, st = hip.serve_wait_rows(srv, tk, cap=4096)
         blat[i], bbsz[i] = st.total_ms, st.batch_queries
     burst_wall = time.perf_counter() - tb
     hip.serve_close(srv)
@@ -876,8 +1089,8 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
     sys.path.insert(0, str(ROOT / "tests"))
     import kssdtest as kt
     orc = kt.Oracle()
-    cpu_ms = []
-    for i in range(2):
+    cpu_ms, cpu_stream_ms = [], []
+    for i in (int(np.flatnonzero(is_asm)[0]), int(np.flatnonzero(~is_asm)[0]) if (~is_asm).any() else 1):
         q = queries[i]
         ctx = np.zeros((1, nref, 2), np.uint32)
         obj = np.zeros((1, nref, 2), np.uint32)
@@ -886,6 +1099,10 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
                                 kt._p(np.array([0, len(q)], np.uint64)), 1, 0, 0, kt._p(ctx), kt._p(obj),
                                 None, None, threads)
         cpu_ms.append((time.perf_counter() - t) * 1e3)
+        t = time.perf_counter()
+        orc.stream_one_query(q, rk, ri, ctxcut=3, afcut=0.2, threads=threads)
+        cpu_stream_ms.append((time.perf_counter() - t) * 1e3)
+    del rk
     hip.close_index(h)
     res = {
         "index_genomes": nref, "index_records": int(len(rec)), "requests": nq,
@@ -893,7 +1110,12 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "p99_assembly": float(np.percentile(lat[is_asm], 99)),
                        "p99_readset": float(np.percentile(lat[~is_asm], 99)) if (~is_asm).any() else None,
-                       "max": float(lat.max()), "what": "arrival -> counter rows on the host"},
+                       "max": float(lat.max()),
+                       "what": "arrival -> ranked hits on the host (device pass gate ctxcut 3 / afcut 0.2 / anicut "
+                               "0.95 + FP64 CtxMoE ANI per batch, passing rows ranked per request)"},
+        "hits_per_request": {"mean": float(hits.mean()), "max": int(hits.max()),
+                             "requests_with_hits": int((hits > 0).sum())},
+        "read_set_requests": "drawn from 64 read-set sketches of simulated 2x150 bp reads (U(5,30)x, 1% errors)",
         "sustained_pairs_per_s": nq * nref / wall,
         "batches": int(info.batches), "mean_batch": float(bsz.mean()), "max_batch_seen": int(info.max_batch_seen),
         "saturated": {"what": "all requests submitted at once", "sustained_pairs_per_s": nq * nref / burst_wall,
@@ -902,11 +1124,16 @@ def serve_bench(args, host, hip, threads, plain_s_per_pair):
                       "latency_ms_p99": float(np.percentile(blat, 99))},
         "budget_bytes": budget, "slots": 4,
         "plain_kernel_per_query_ms": plain_s_per_pair * nref * 1e3,
-        "cpu_forced_index_per_query_ms": float(np.mean(cpu_ms)), "cpu_threads": threads,
+        "cpu_forced_index_per_query_ms": float(np.mean(cpu_ms)),
+        "cpu_default_path_per_query_ms": float(np.mean(cpu_stream_ms)),
+        "cpu_default_path_what": "oracle restatement of stream_ref_sketches_one_qraw_lookup (query hash, every "
+                                 "reference streamed) on one assembly and one read-set request",
+        "cpu_threads": threads,
     }
     log(f"serving: p50 {res['latency_ms']['p50']:.2f} ms, p99 {res['latency_ms']['p99']:.2f} ms, "
         f"{res['sustained_pairs_per_s']:.3e} pairs/s over {wall:.1f}s, mean batch {bsz.mean():.2f}; "
-        f"plain per query {res['plain_kernel_per_query_ms']:.0f} ms, CPU per query {np.mean(cpu_ms):.0f} ms; "
+        f"plain per query {res['plain_kernel_per_query_ms']:.0f} ms, CPU per query {np.mean(cpu_ms):.0f} ms "
+        f"(default path {np.mean(cpu_stream_ms):.0f} ms); "
         f"saturated {res['saturated']['sustained_pairs_per_s']:.3e} pairs/s, mean batch {bbsz.mean():.1f}")
     return res
 
@@ -950,6 +1177,8 @@ def main():
                     help="run only the headline workload (for profiling runs)")
     ap.add_argument("--data-cache", default="",
                     help="directory to keep the C3 sketches in between runs (profiling passes)")
+    ap.add_argument("--cli-queries", type=int, default=100000,
+                    help="C3 queries run end to end through kssd3a-hip ani (0 = skip)")
     ap.add_argument("--serve-refs", type=int, default=20000,
                     help="reference genomes of the serving index (1000 clades of 20); 0 skips serving")
     ap.add_argument("--serve-queries", type=int, default=2000)
@@ -1103,6 +1332,24 @@ def main():
     log(f"graph: {graph_ms:.3f} ms/step, agrees with eager: {graph_ok}")
     p_touched = int((ctx[:, :, 0] > 0).sum())
 
+    # ---------------- the persistent query-major form on the same block
+    hip.set_tiling(h, mode=3)
+    hip.count_block_dev(h, d_qkeys, d_qidx, nqry, total, 0, 0, d_ctx, d_obj, None, None, stream)
+    rt.record(e0, stream)
+    for _ in range(3):
+        hip.count_block_dev(h, d_qkeys, d_qidx, nqry, total, 0, 0, d_ctx, d_obj, None, None, stream)
+    rt.record(e1, stream)
+    rt.sync(stream)
+    qmajor_ms = rt.elapsed_ms(e0, e1) / 3
+    qmajor_ok = bool(np.array_equal(rt.download(d_ctx, np.uint32, 2 * cells).reshape(nqry, nref, 2), ctx)
+                     and np.array_equal(rt.download(d_obj, np.uint32, 2 * cells).reshape(nqry, nref, 2), obj))
+    hip.set_tiling(h, **tiling)
+    forms_c2 = {"static_partitioned": {"ms": float(np.mean(join_ms)), "tiling": tiling},
+                "persistent_query_major": {"ms": qmajor_ms, "tiling": {"mode": 3}, "counts_equal": qmajor_ok},
+                "what": "the counting call alone (split + join, or the query-major join) on the C2 block"}
+    log(f"C2 forms: static partitioned {np.mean(join_ms):.3f} ms, persistent query-major {qmajor_ms:.3f} ms "
+        f"(equal {qmajor_ok})")
+
     # ---------------- per-query latency of the batched path (one query per launch);
     # auto mode sends such a small block to the lookup kernel
     hip.set_tiling(h, mode=0)
@@ -1176,6 +1423,8 @@ def main():
 
     # ---------------- serving loop on a large resident index
     big_index = index_bench(args, host, threads, plain_ms / npl / 1e3, shared)
+    cli = cli_e2e(args, host, shared, threads) if args.cli_queries else None
+    c1 = c1_bench(host, hip, rt, threads, args.seed + 500)
     shared["hip"].close_index(shared["h"])
     del shared
     serving = serve_bench(args, host, hip, threads, plain_ms / npl / 1e3) if args.serve_refs else None
@@ -1202,7 +1451,7 @@ def main():
         "plain_kernel": {"pairs_per_s": plain_pairs_per_s, "sample_pairs": npl,
                          "us_per_pair": plain_ms / npl * 1e3, "agrees_with_batched": plain_ok},
         "batched_over_plain": pairs_per_s / plain_pairs_per_s,
-        "tiling": tiling, "tuner_ms_per_block": tune_ms,
+        "tiling": tiling, "tuner_ms_per_block": tune_ms, "forms": forms_c2,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "what": f"one query ({len(lat)} sampled) vs the {nref}-genome resident index, "
                                "join + epilogue, device time"},
@@ -1238,6 +1487,8 @@ def main():
     rec["latency_ms"] = c2["latency_ms"]
     rec["index_10k"] = c2.pop("index_10k")
     rec["serving"] = c2.pop("serving")
+    rec["c1"] = c1
+    rec["cli_end_to_end"] = cli
     hip.close_index(h)
     print(json.dumps(rec), flush=True)
 

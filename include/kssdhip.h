@@ -405,6 +405,16 @@ int kssdhip_index_matched_records(kssdhip_index *index, const uint64_t *d_keys, 
 				  int section_gnum, uint64_t total_keys, uint64_t *matched);
 
 /*
+ * Diagnostic: the device's rate of independent random 8-byte reads from a
+ * buffer of `bytes` (rounded down to a power of two; allocated and freed by
+ * the call), `reads` per timed launch, best of two after a warm-up.  No
+ * reference counterpart: it measures the bound of random-access kernels
+ * (every read fetches its own memory line), against which the query-major
+ * join's line requests are set.  Synchronous.
+ */
+int kssdhip_probe_random_reads(size_t bytes, uint64_t reads, double *reads_per_s);
+
+/*
  * Serving loop (no reference counterpart: the reference answers one request
  * per process, stream_ref_sketches_one_qraw_lookup, command_ani.c:2490-2666).
  * A server keeps the index resident and answers query sketches submitted

@@ -621,7 +621,7 @@ extern "C" int kssdhip_index_close(kssdhip_index *ix)
 	kssd_free_partitions(ix);
 	kssd_qjoin_free(ix);
 	DevBuf *bufs[] = {&ix->keys,  &ix->skidx, &ix->ctx,  &ix->obj,  &ix->uctx,  &ix->uobj,  &ix->idx,
-			  &ix->split, &ix->qws,   &ix->rows, &ix->qmeta, &ix->qctxb, &ix->rctxb};
+			  &ix->split, &ix->qws,   &ix->rows, &ix->qmeta, &ix->qctxb, &ix->rctxb, &ix->sdense};
 	for (DevBuf *b : bufs)
 		b->release();
 	if (ix->stream)
@@ -637,7 +637,7 @@ extern "C" size_t kssdhip_index_device_bytes(const kssdhip_index *ix)
 	size_t b = ((size_t)ix->n + INDEX_PAD) * 12 + ((((size_t)1) << ix->k) + 1) * 4;
 	b += ix->keys.bytes + ix->skidx.bytes + ix->ctx.bytes + ix->obj.bytes + ix->uctx.bytes +
 	     ix->uobj.bytes + ix->idx.bytes + ix->split.bytes + ((size_t)ix->P + 1) * 12;
-	b += ((size_t)ix->n + 1) * 16 + (((size_t)1) << ix->ck) * 8 + (((size_t)1) << ix->fk) * 8 + ix->qws.bytes + ix->rows.bytes +
+	b += ((size_t)ix->n + 1) * 16 + (((size_t)1) << ix->ck) * 8 + (((size_t)1) << ix->fk) * 8 + ix->qws.bytes + ix->sdense.bytes + ix->rows.bytes +
 	     ix->qmeta.bytes + ix->qctxb.bytes + ix->rctxb.bytes;
 	return b;
 }

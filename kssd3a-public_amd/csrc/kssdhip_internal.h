@@ -130,7 +130,7 @@ struct kssdhip_index {
 	int qj_per_cu[2] = {0, 0}; // query-major workgroups per CU (plain, unique counters) at the tiling
 	// grow-only scratch: host-pointer entry points, the split table, the
 	// query-major work counter and spill rows, select outputs
-	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx, split, qws, rows, qmeta, qctxb, rctxb;
+	DevBuf keys, skidx, ctx, obj, uctx, uobj, idx, split, qws, rows, qmeta, qctxb, rctxb, sdense;
 };
 
 // partitioned merge-join (kssdhip_pjoin.hip)
@@ -155,12 +155,15 @@ void kssd_qjoin_free(kssdhip_index *ix);
 // dense rows of a block by the query-major kernel; `ws` is the call's
 // workspace (work counter and spill rows)
 int kssd_launch_qjoin_dense(kssdhip_index *ix, const kssd::JoinArgs &a, hipStream_t s, DevBuf *ws);
-// kssdhip_select_block_dev with the caller's workspace `ws` (a server runs
-// one per slot, so concurrent batches never share it)
+// kssdhip_select_block_dev with the caller's workspaces (a server runs one
+// set per slot, so concurrent batches never share them): `ws` the
+// query-major join's, `split` and `dense` the dense-row path's that small
+// blocks take (nullptr: always the query-major join)
 int kssd_select_dev_ws(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
 		       uint32_t flags, const uint32_t *d_qry_ctx, const uint32_t *d_ref_ctx,
 		       const kssdhip_select_opts_t *opts, kssdhip_ani_row_t *d_rows, uint32_t cap,
-		       uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, hipStream_t s, DevBuf *ws);
+		       uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, hipStream_t s, DevBuf *ws,
+		       DevBuf *split, DevBuf *dense);
 // true when a block of this shape should take the query-major kernel in
 // auto mode
 bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total);

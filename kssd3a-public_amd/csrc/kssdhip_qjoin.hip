@@ -767,6 +767,79 @@ __global__ __launch_bounds__(256) void k_count_marks(const uint8_t *marks, uint6
 		atomicAdd(count, (unsigned long long)k);
 }
 
+// The pass gate over dense count rows (small blocks, counted by the
+// one-thread-per-key lookup): per-query pass counts, their prefix sums as
+// the segments' first rows, then the passing rows written into their
+// query's segment.
+template <bool UNIQ>
+__device__ __forceinline__ bool dense_cell(const QJArgs &a, uint64_t i, Cell *v)
+{
+	v->c = a.ctx[i];
+	if (!v->c)
+		return false;
+	v->o = a.objc[i];
+	v->uc = UNIQ ? a.uctx[i] : 0ull;
+	v->uo = UNIQ ? a.uobj[i] : 0ull;
+	return true;
+}
+
+template <bool UNIQ>
+__global__ __launch_bounds__(256) void k_gate_dense_count(QJArgs a)
+{
+	const uint64_t cells = (uint64_t)a.nq * a.R;
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cells; i += (uint64_t)gridDim.x * blockDim.x) {
+		Cell v;
+		double rank;
+		const uint32_t q = (uint32_t)(i / a.R), g = (uint32_t)(i % a.R);
+		if (dense_cell<UNIQ>(a, i, &v) && qj_gate<UNIQ>(a, q, g, v, &rank))
+			atomicAdd(&a.q_count[q], 1u);
+	}
+}
+
+// one thread: segments in query order, the total, cursors zeroed
+__global__ void k_gate_dense_scan(QJArgs a, uint32_t *cursor)
+{
+	if (blockIdx.x || threadIdx.x)
+		return;
+	uint32_t at = 0;
+	for (uint32_t q = 0; q < a.nq; ++q) {
+		a.q_first[q] = at;
+		at += a.q_count[q];
+		cursor[q] = 0;
+	}
+	*a.row_count = at;
+}
+
+template <bool UNIQ>
+__global__ __launch_bounds__(256) void k_gate_dense_write(QJArgs a, uint32_t *cursor)
+{
+	const uint64_t cells = (uint64_t)a.nq * a.R;
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cells; i += (uint64_t)gridDim.x * blockDim.x) {
+		Cell v;
+		double rank;
+		const uint32_t q = (uint32_t)(i / a.R), g = (uint32_t)(i % a.R);
+		if (!dense_cell<UNIQ>(a, i, &v) || !qj_gate<UNIQ>(a, q, g, v, &rank))
+			continue;
+		const uint32_t at = a.q_first[q] + atomicAdd(&cursor[q], 1u);
+		if (at >= a.cap)
+			continue;
+		kssdhip_ani_row_t w;
+		w.query = q;
+		w.ref = g;
+		w.ctx.num_ctx = (uint32_t)v.c;
+		w.ctx.num_mut2_ctx = (uint32_t)(v.c >> 32);
+		w.obj.diff_obj = (uint32_t)v.o;
+		w.obj.diff_obj_section = (uint32_t)(v.o >> 32);
+		w.unique_ctx.num_ctx = (uint32_t)v.uc;
+		w.unique_ctx.num_mut2_ctx = (uint32_t)(v.uc >> 32);
+		w.unique_obj.diff_obj = (uint32_t)v.uo;
+		w.unique_obj.diff_obj_section = (uint32_t)(v.uo >> 32);
+		w.rank_ani = rank;
+		w.value = 0.0; // k_row_values
+		a.rows[at] = w;
+	}
+}
+
 struct QJShape {
 	bool direct;
 	uint32_t slots;
@@ -1151,6 +1224,86 @@ int select_launch(const kssdhip_index *ix, const SelectCall &c, const QJShape &s
 	return 0;
 }
 
+// Small blocks (fewer queries than compute units) leave most of the chip to
+// a few query-major workgroups; they are counted into dense rows by the
+// lookup kernel instead (the handle's tiling unless it forces the
+// query-major join) and gated row by row.  Dense rows up to this size.
+constexpr uint64_t DENSE_SELECT_MAX_BYTES = 256ull << 20;
+
+bool select_dense_fits(const kssdhip_index *ix, int nq, bool uniq)
+{
+	const uint64_t cells = (uint64_t)nq * (uint64_t)ix->ref_gnum;
+	return ix->mode != 3 && nq < (ix->cus > 0 ? ix->cus : 256) && cells * (uniq ? 32u : 16u) <= DENSE_SELECT_MAX_BYTES;
+}
+
+int select_dense(kssdhip_index *ix, const SelectCall &c, DevBuf *split, DevBuf *dws)
+{
+	const bool uniq = (c.flags & KSSDHIP_UNIQUE) != 0;
+	const uint64_t cells = (uint64_t)c.nq * (uint64_t)ix->ref_gnum;
+	const size_t plane = (size_t)cells * 8;
+	const size_t want = plane * (uniq ? 4 : 2) + (size_t)c.nq * 4 + 256;
+	int rc = kssd_reserve_for_stream(dws, want, c.s, "select workspace");
+	if (rc)
+		return rc;
+	char *base = (char *)dws->ptr;
+	kssd_ctx_mut2_t *d_ctx = (kssd_ctx_mut2_t *)base;
+	kssd_obj_section_t *d_obj = (kssd_obj_section_t *)(base + plane);
+	kssd_ctx_mut2_t *d_uctx = uniq ? (kssd_ctx_mut2_t *)(base + 2 * plane) : nullptr;
+	kssd_obj_section_t *d_uobj = uniq ? (kssd_obj_section_t *)(base + 3 * plane) : nullptr;
+	uint32_t *cursor = (uint32_t *)(base + plane * (uniq ? 4 : 2));
+	if ((rc = kssd_count_dev(ix, c.keys, c.skidx, c.nq, c.total, 0, c.flags, d_ctx, d_obj, d_uctx, d_uobj, c.s,
+				 split, nullptr)))
+		return rc;
+	QJArgs a;
+	memset(&a, 0, sizeof a);
+	a.nq = (uint32_t)c.nq;
+	a.R = (uint32_t)ix->ref_gnum;
+	a.ctx = (unsigned long long *)d_ctx;
+	a.objc = (unsigned long long *)d_obj;
+	a.uctx = (unsigned long long *)d_uctx;
+	a.uobj = (unsigned long long *)d_uobj;
+	a.qry_ctx = c.qry_ctx;
+	a.ref_ctx = c.ref_ctx;
+	a.ctxcut = c.opts->ctxcut;
+	a.afcut = (double)c.opts->afcut;
+	a.anicut = c.opts->anicut;
+	a.unique_rank = c.opts->unique_rank;
+	a.rows = c.rows;
+	a.cap = c.cap;
+	a.row_count = c.row_count;
+	a.q_first = c.q_first;
+	a.q_count = c.q_count;
+	const unsigned g = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((cells + 255) / 256, 8192));
+	hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<uint32_t>(((uint32_t)c.nq + 255) / 256, 1024u)),
+			   dim3(256), 0, c.s, c.q_count, (uint32_t)c.nq);
+	if (uniq)
+		hipLaunchKernelGGL(k_gate_dense_count<true>, dim3(g), dim3(256), 0, c.s, a);
+	else
+		hipLaunchKernelGGL(k_gate_dense_count<false>, dim3(g), dim3(256), 0, c.s, a);
+	hipLaunchKernelGGL(k_gate_dense_scan, dim3(1), dim3(64), 0, c.s, a, cursor);
+	if (uniq)
+		hipLaunchKernelGGL(k_gate_dense_write<true>, dim3(g), dim3(256), 0, c.s, a, cursor);
+	else
+		hipLaunchKernelGGL(k_gate_dense_write<false>, dim3(g), dim3(256), 0, c.s, a, cursor);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess)
+		return kssdhip_fail(KSSDHIP_EHIP, "select_block gate: %s", hipGetErrorString(e));
+	if (c.cap) {
+		kssdhip_epilogue_opts_t vo = c.opts->value;
+		int m = vo.metric;
+		if (m < 1 || m > 9)
+			m = 1;
+		const int vmetric = (vo.unassembled && m <= 4) ? 4 : m;
+		const unsigned gr = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((c.cap + 255) / 256, 1024));
+		hipLaunchKernelGGL(k_row_values, dim3(gr), dim3(256), 0, c.s, c.rows, c.row_count, c.cap, c.qry_ctx,
+				   c.ref_ctx, vo, vmetric);
+		e = hipGetLastError();
+		if (e != hipSuccess)
+			return kssdhip_fail(KSSDHIP_EHIP, "select_block values: %s", hipGetErrorString(e));
+	}
+	return 0;
+}
+
 // zeroes the row count; true when the block is empty (nothing else to do)
 int select_begin(const SelectCall &c, bool *done)
 {
@@ -1226,7 +1379,8 @@ static int noalloc_scratch(int device, void **out)
 int kssd_select_dev_ws(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx, int nq, uint64_t total,
 		       uint32_t flags, const uint32_t *d_qry_ctx, const uint32_t *d_ref_ctx,
 		       const kssdhip_select_opts_t *opts, kssdhip_ani_row_t *d_rows, uint32_t cap,
-		       uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, hipStream_t s, DevBuf *ws)
+		       uint32_t *d_row_count, uint32_t *d_q_first, uint32_t *d_q_count, hipStream_t s, DevBuf *ws,
+		       DevBuf *split, DevBuf *dense)
 {
 	const SelectCall c{d_keys, d_skidx, nq, total, flags, d_qry_ctx, d_ref_ctx, opts,
 			   d_rows, cap, d_row_count, d_q_first, d_q_count, s};
@@ -1234,6 +1388,8 @@ int kssd_select_dev_ws(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t
 	bool done = false;
 	if (rc || (rc = select_begin(c, &done)) || done)
 		return rc;
+	if (split && dense && select_dense_fits(ix, nq, (flags & KSSDHIP_UNIQUE) != 0))
+		return select_dense(ix, c, split, dense);
 	QJShape sh;
 	if ((rc = qj_shape(ix, (uint32_t)nq, (flags & KSSDHIP_UNIQUE) != 0, &sh)))
 		return rc;
@@ -1252,7 +1408,7 @@ extern "C" int kssdhip_select_block_dev(kssdhip_index *ix, const uint64_t *d_key
 	if (!ix)
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: bad arguments");
 	return kssd_select_dev_ws(ix, d_keys, d_skidx, nq, total, flags, d_qry_ctx, d_ref_ctx, opts, d_rows, cap,
-				  d_row_count, d_q_first, d_q_count, (hipStream_t)stream, &ix->qws);
+				  d_row_count, d_q_first, d_q_count, (hipStream_t)stream, &ix->qws, &ix->split, &ix->sdense);
 }
 
 extern "C" int kssdhip_select_block_dev_noalloc(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_skidx,

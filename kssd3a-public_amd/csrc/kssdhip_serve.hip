@@ -70,7 +70,8 @@ struct Slot {
 	void *d_arena = nullptr;
 	void *h_arena = nullptr;
 	DevBuf split;
-	DevBuf qws; // select: the query-major join's workspace
+	DevBuf qws;    // select: the query-major join's workspace
+	DevBuf sdense; // select: dense rows of small batches
 	Batch batch;
 	bool busy = false;
 	int launch_err = 0;
@@ -188,7 +189,7 @@ int launch(kssdhip_server *s, Slot &sl)
 		int rc = kssd_select_dev_ws(s->ix, (const uint64_t *)d, (const uint64_t *)(d + b.off_skidx), nq, b.total,
 					    s->opts.flags, (const uint32_t *)(d + b.off_qctx), s->d_ref_ctx, &s->sel,
 					    (kssdhip_ani_row_t *)(d + b.off_rows), (uint32_t)cap, meta, meta + 1, meta + 1 + nq,
-					    sl.stream, &sl.qws);
+					    sl.stream, &sl.qws, &sl.split, &sl.sdense);
 		if (rc)
 			return rc;
 		if (hipMemcpyAsync(h + b.off_meta, d + b.off_meta, b.off_rows - b.off_meta, hipMemcpyDeviceToHost,
@@ -374,6 +375,7 @@ void free_slots(kssdhip_server *s)
 			(void)hipStreamSynchronize(sl.stream);
 		sl.split.release();
 		sl.qws.release();
+		sl.sdense.release();
 		if (sl.d_arena)
 			(void)hipFree(sl.d_arena);
 		if (sl.h_arena)

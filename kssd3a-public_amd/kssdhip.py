@@ -293,6 +293,7 @@ def load_kssdhip():
     lib.kssdhip_epilogue_dev.argtypes = [V, V, I, I, V, V, ctypes.POINTER(EpilogueOpts), V, V]
     lib.kssdhip_select_hits_dev.argtypes = [V, I, I, ctypes.c_double, U32, V, V, V]
     lib.kssdhip_index_matched_records.argtypes = [V, V, V, I, U64, ctypes.POINTER(U64)]
+    lib.kssdhip_probe_random_reads.argtypes = [S, U64, ctypes.POINTER(D)]
     lib.kssdhip_select_block.argtypes = [V, V, V, I, U32, V, V, ctypes.POINTER(SelectOpts), V, S,
                                          ctypes.POINTER(S)]
     lib.kssdhip_select_block_dev.argtypes = [V, V, V, I, U64, U32, V, V, ctypes.POINTER(SelectOpts), V, U32,
@@ -501,6 +502,12 @@ class KssdHip:
         self.check(self.lib.kssdhip_select_block_dev_noalloc(h, d_keys, d_skidx, nq, total, flags, d_qctx, d_rctx,
                                                              ctypes.byref(opts), d_rows, cap, d_count, d_first,
                                                              d_qcount, stream))
+
+    def probe_random_reads(self, nbytes=2 << 30, reads=400_000_000):
+        """Random 8-byte reads per second from an nbytes buffer."""
+        r = D()
+        self.check(self.lib.kssdhip_probe_random_reads(nbytes, reads, ctypes.byref(r)))
+        return float(r.value)
 
     def matched_records(self, h, d_keys, d_skidx, nq, total):
         m = U64()

@@ -148,7 +148,7 @@ def _ctx_totals(keys, idx):
     return np.diff(idx).astype(np.uint32)
 
 
-def _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, ctxcut, afcut, anicut, slots=0):
+def _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, ctxcut, afcut, anicut, slots=0, mode=3):
     qctx = _ctx_totals(qk, qi)
     rctx = np.maximum(np.bincount((rec["ctxgid"] & np.uint64((1 << 20) - 1)).astype(np.int64),
                                   minlength=R).astype(np.uint32), 0)
@@ -157,7 +157,7 @@ def _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, ctxcut, afcut
                          kt.EpilogueOpts(3, 1, 0, 0, 0, 0, 8, OBJ_BITS, 44, 0.5, 1.0))
     h = hip.open_index(rec, R, OBJ_BITS)
     try:
-        hip.set_tiling(h, query_slots=slots)
+        hip.set_tiling(h, mode=mode, query_slots=slots)
         rows = hip.select_block(h, qk, qi, qctx, rctx, opts, flags=flags)
     finally:
         hip.close_index(h)
@@ -202,18 +202,23 @@ def _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, ctxcut, afcut
 
 @pytest.mark.parametrize("shape", list(SHAPES))
 @pytest.mark.parametrize("flags,unique_rank", [(0, False), (4, False), (2, True), (6, True), (2, False)])
-def test_select_matches_oracle_gate(hip, oracle, wide_cases, shape, flags, unique_rank):
+@pytest.mark.parametrize("mode", [3, 0])
+def test_select_matches_oracle_gate(hip, oracle, wide_cases, shape, flags, unique_rank, mode):
+    """mode 3: the query-major join in select mode; mode 0: these blocks have
+    fewer queries than compute units, so they are counted into dense rows
+    by the lookup kernel and gated row by row."""
     rk, ri, qk, qi, rec, slots = wide_cases(shape)
     R = len(ri) - 1
-    rows = _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, 1, 0.0, 0.0, slots)
+    rows = _check_select(hip, oracle, rec, R, qk, qi, flags, unique_rank, 1, 0.0, 0.0, slots, mode)
     assert len(rows) > 0
 
 
 @pytest.mark.parametrize("ctxcut,afcut,anicut", [(3, 0.0, 0.0), (1, 0.05, 0.0), (2, 0.0, 0.93), (1, 0.02, 0.97)])
 def test_select_gate_cuts(hip, oracle, wide_cases, ctxcut, afcut, anicut):
     rk, ri, qk, qi, rec, slots = wide_cases("direct")
-    _check_select(hip, oracle, rec, len(ri) - 1, qk, qi, 2, True, ctxcut, afcut, anicut)
-    _check_select(hip, oracle, rec, len(ri) - 1, qk, qi, 0, False, ctxcut, afcut, anicut)
+    for mode in (3, 0):
+        _check_select(hip, oracle, rec, len(ri) - 1, qk, qi, 2, True, ctxcut, afcut, anicut, mode=mode)
+        _check_select(hip, oracle, rec, len(ri) - 1, qk, qi, 0, False, ctxcut, afcut, anicut, mode=mode)
 
 
 def test_select_capacity_and_empty(hip, oracle, wide_cases):
@@ -238,7 +243,8 @@ def test_select_capacity_and_empty(hip, oracle, wide_cases):
         hip.close_index(h)
 
 
-def test_select_graph_capture(hip, oracle, wide_cases):
+@pytest.mark.parametrize("mode", [3, 0])
+def test_select_graph_capture(hip, oracle, wide_cases, mode):
     """Capturing the select before its workspace exists is refused (no
     allocation inside a capture); after one eager call every replay of the
     captured call equals the eager rows, with eager work in between."""
@@ -254,7 +260,7 @@ def test_select_graph_capture(hip, oracle, wide_cases):
             rt.malloc(4 * Q), rt.malloc(4 * Q)]
     d_k, d_i, d_qc, d_rc, d_rows, d_n, d_first, d_cnt = bufs
     try:
-        hip.set_tiling(h, query_slots=256)
+        hip.set_tiling(h, mode=mode, query_slots=256)
 
         def call():
             hip.select_block_dev(h, d_k, d_i, Q, int(qi[-1]), 0, d_qc, d_rc, opts, d_rows, cap, d_n, d_first, d_cnt,
