@@ -171,6 +171,46 @@ int kssdhip_index_set_tiling(kssdhip_index *index, const kssdhip_tiling_t *tilin
 int kssdhip_index_get_tiling(const kssdhip_index *index, kssdhip_tiling_t *tiling);
 
 /*
+ * Autotuner (no reference counterpart; the reference's knobs are its fixed
+ * block sizes, command_ani.c:403, :1517-1529, command_matrix.c:689-699).
+ * kssdhip_index_autotune times every candidate kernel shape (the lookup
+ * kernel; the partitioned merge-join over chunk sizes, workgroup sizes and
+ * partition groups; the query-major join over workgroup sizes and LDS
+ * slots) on a device-resident block (relative offsets, as
+ * kssdhip_count_block_dev; the outputs go to scratch the call allocates),
+ * applies the fastest to the handle and records it in a JSON cache keyed
+ * by the device and the block's shape (index records, query keys, queries
+ * and references, each rounded down to a power of two, and the flags).
+ * With force == 0 a cached entry for the shape is applied without timing.
+ * cache_path NULL: $KSSDHIP_TUNE_CACHE, else ~/.cache/kssdhip/tuning.json
+ * (written atomically under an advisory lock).  Synchronous.
+ * kssdhip_index_autotune_host: the same for a host block with ABSOLUTE
+ * offsets (as kssdhip_count_block).
+ * kssdhip_index_tiling_from_cache: applies the cached shape for a block of
+ * this shape if there is one (returns 1), else leaves the handle as it is
+ * (returns 0); no timing.
+ * kssdhip_index_open starts a handle from the cached shape parameters of
+ * the largest block tuned for an index of its shape on its device (kernel
+ * still chosen per call), else from partition groups fitted to the compute
+ * units; KSSDHIP_NO_TUNE_CACHE set skips the cache.
+ */
+typedef struct {
+	kssdhip_tiling_t tiling; /* applied to the handle */
+	double ms;               /* the block's counting time with it */
+	int from_cache;          /* 1: taken from the cache, nothing timed */
+	int candidates;          /* shapes timed */
+	int cached;              /* 1: the result was written to the cache */
+} kssdhip_tune_result_t;
+int kssdhip_index_autotune(kssdhip_index *index, const uint64_t *d_keys, const uint64_t *d_skidx, int section_gnum,
+			   uint64_t total_keys, uint32_t flags, const char *cache_path, int force,
+			   kssdhip_tune_result_t *result);
+int kssdhip_index_autotune_host(kssdhip_index *index, const uint64_t *section_sk, const uint64_t *section_skidx,
+				int section_gnum, uint32_t flags, const char *cache_path, int force,
+				kssdhip_tune_result_t *result);
+int kssdhip_index_tiling_from_cache(kssdhip_index *index, int section_gnum, uint64_t total_keys, uint32_t flags,
+				    const char *cache_path, kssdhip_tune_result_t *result);
+
+/*
  * Count one block of queries against the index.
  * Replaces the counting part of
  *   count_ctx_obj_frm_comb_sketch_section_with_unique (command_ani.h:463-575)

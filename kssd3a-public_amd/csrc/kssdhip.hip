@@ -584,16 +584,26 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 			rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: bucket build failed");
 			break;
 		}
+		int dev_cus = 0;
+		if (hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ix->device) == hipSuccess)
+			ix->cus = dev_cus;
+		kssdhip_tiling_t t0;
+		kssd_default_tiling(ix, &t0);
 		ix->mode = 0;
-		ix->block = 256;
+		ix->block = t0.block_threads;
+		ix->group = t0.group_partitions;
 		ix->auto_min_keys = 512.0;
 		open_phase(nullptr, "bucket table");
-		if ((rc = kssd_build_partitions(ix, 4096)))
+		if ((rc = kssd_build_partitions(ix, t0.chunk_records)))
 			break;
 		open_phase(nullptr, "partition table");
 		if ((rc = kssd_qjoin_build(ix)))
 			break;
 		open_phase(nullptr, "query-major tables");
+		kssdhip_tiling_t tc;
+		if (kssd_cached_index_tiling(ix, &tc) && kssdhip_index_set_tiling(ix, &tc) != 0)
+			(void)kssdhip_index_set_tiling(ix, &t0); // an unusable cache entry: the default shape
+		open_phase(nullptr, "tiling");
 	} while (0);
 	if (d_raw)
 		(void)hipFree(d_raw);

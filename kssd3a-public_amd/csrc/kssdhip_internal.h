@@ -170,6 +170,10 @@ bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total);
 // looks up the address of the no-allocation scratch array on `device`
 // (current on the calling thread) once per device
 int kssd_noalloc_setup(int device);
+// autotuner (kssdhip_tune.hip): a new handle's shape when nothing is cached,
+// and the cached shape parameters for its index shape (1 when found)
+void kssd_default_tiling(const kssdhip_index *ix, kssdhip_tiling_t *t);
+int kssd_cached_index_tiling(const kssdhip_index *ix, kssdhip_tiling_t *t);
 // fails with KSSDHIP_EINVAL when `want` bytes would grow `b` while `s` is
 // being captured into a graph, else reserves them
 int kssd_reserve_for_stream(DevBuf *b, size_t want, hipStream_t s, const char *what);

@@ -82,6 +82,9 @@ static struct {
 			   uint32_t *);
 	int (*select_block)(kssdhip_index *, const uint64_t *, const uint64_t *, int, uint32_t, const uint32_t *,
 			    const uint32_t *, const kssdhip_select_opts_t *, kssdhip_ani_row_t *, size_t, size_t *);
+	int (*autotune_host)(kssdhip_index *, const uint64_t *, const uint64_t *, int, uint32_t, const char *, int,
+			     kssdhip_tune_result_t *);
+	int (*tiling_from_cache)(kssdhip_index *, int, uint64_t, uint32_t, const char *, kssdhip_tune_result_t *);
 } g_hip;
 
 static void *hip_sym(void *lib, const char *name)
@@ -113,6 +116,8 @@ static typeof(g_hip) *hip_load(void)
 	g_hip.count_block = (typeof(g_hip.count_block))hip_sym(lib, "kssdhip_count_block");
 	g_hip.count_votes = (typeof(g_hip.count_votes))hip_sym(lib, "kssdhip_count_votes");
 	g_hip.select_block = (typeof(g_hip.select_block))hip_sym(lib, "kssdhip_select_block");
+	g_hip.autotune_host = (typeof(g_hip.autotune_host))hip_sym(lib, "kssdhip_index_autotune_host");
+	g_hip.tiling_from_cache = (typeof(g_hip.tiling_from_cache))hip_sym(lib, "kssdhip_index_tiling_from_cache");
 	g_hip.init = (int (*)(int))hip_sym(lib, "kssdhip_init");
 	return &g_hip;
 }
@@ -286,6 +291,22 @@ static void hip_check(int rc, const char *what)
 {
 	if (rc != 0)
 		errx(EXIT_FAILURE, "%s: %s", what, kssdhip_last_error());
+}
+
+/* The kernel shape for a block of this shape: the tuning cache's entry
+ * (kssdhip_index_tiling_from_cache), or with KSSDHIP_TUNE=1 the library's
+ * autotuner timed on this block, which caches it for later runs.  Without
+ * either the handle keeps the shape it was opened with. */
+static void tune_block(kssdhip_index *ix, const uint64_t *keys, const uint64_t *skidx, int nb, uint32_t flags)
+{
+	kssdhip_tune_result_t r;
+	const char *e = getenv("KSSDHIP_TUNE");
+	if (nb < 1)
+		return;
+	if (e && e[0] == '1' && e[1] == '\0')
+		hip_check(hip_api()->autotune_host(ix, keys, skidx, nb, flags, NULL, 0, &r), "autotune");
+	else if (hip_api()->tiling_from_cache(ix, nb, skidx[nb] - skidx[0], flags, NULL, &r) < 0)
+		hip_check(-1, "tuning cache");
 }
 
 /* ------------------------------------------------------------ options */
@@ -604,6 +625,21 @@ static int block_queries(int nq, int nref, bool unique)
 	return b < (size_t)nq ? (int)b : nq;
 }
 
+/* Queries from q0 (of q_end) in one GPU block: at most blk (the dense rows'
+ * budget), the block's keys within 2^28 (2 GiB on the device) and the
+ * partitioned join's split table -- one 32-bit offset per query and index
+ * partition, at most index_records / 1024 + 2 partitions -- within 512 MiB;
+ * at least one query. */
+static int block_from(const uint64_t *index, int q0, int q_end, int blk, size_t index_records)
+{
+	const uint64_t key_cap = (uint64_t)1 << 28;
+	const size_t split_cap = ((size_t)512 << 20) / ((index_records / 1024 + 2) * 4);
+	int nb = 1;
+	while (q0 + nb < q_end && nb < blk && (size_t)nb < split_cap && index[q0 + nb + 1] - index[q0] <= key_cap)
+		++nb;
+	return nb;
+}
+
 static kssdhip_index *gpu_index(const kssd_ctxgidobj_t *rec, size_t n, int gnum, unsigned obj_bits)
 {
 	kssdhip_index *ix = NULL;
@@ -704,8 +740,9 @@ static void rect_counts(const kssd_sketchdir_t *ref, const kssd_sketchdir_t *qry
 		kssdhip_index *ix = gpu_index(rec, ref->index[R], R, ref->obj_bits);
 		free(rec);
 		const int blk = block_queries(Q, R, false);
-		for (int q0 = 0; q0 < Q; q0 += blk) {
-			const int nb = Q - q0 < blk ? Q - q0 : blk;
+		for (int q0 = 0, nb = 0; q0 < Q; q0 += nb) {
+			nb = block_from(qry->index, q0, Q, blk, (size_t)ref->index[R]);
+			tune_block(ix, qry->keys + qry->index[q0], qry->index + q0, nb, ignore ? KSSDHIP_IGNORECONFLICT : 0);
 			hip_check(kssdhip_count_block(ix, qry->keys + qry->index[q0], qry->index + q0, nb, q0,
 						      ignore ? KSSDHIP_IGNORECONFLICT : 0, ctx + (size_t)q0 * R,
 						      obj + (size_t)q0 * R, NULL, NULL),
@@ -726,8 +763,8 @@ static void rect_counts(const kssd_sketchdir_t *ref, const kssd_sketchdir_t *qry
 	const int blk = block_queries(R, Q, false);
 	kssd_ctx_mut2_t *tc = xmalloc((size_t)blk * Q * sizeof *tc);
 	kssd_obj_section_t *to = xmalloc((size_t)blk * Q * sizeof *to);
-	for (int r0 = 0; r0 < R; r0 += blk) {
-		const int nb = R - r0 < blk ? R - r0 : blk;
+	for (int r0 = 0, nb = 0; r0 < R; r0 += nb) {
+		nb = block_from(ridx, r0, R, blk, (size_t)qry->index[Q]);
 		hip_check(kssdhip_count_block(ix, rkeys + ridx[r0], ridx + r0, nb, r0, 0, tc, to, NULL, NULL),
 			  "GPU counting failed");
 		for (int i = 0; i < nb; ++i)
@@ -1061,6 +1098,7 @@ static int run_index_path(const ani_args_t *a)
 		for (int q0 = 0; q0 < Q;) {
 			const int nb = select_block_queries(&qry, q0);
 			size_t n = 0;
+			tune_block(ix, qry.keys + qry.index[q0], qry.index + q0, nb, flags);
 			int rc = kssdhip_select_block(ix, qry.keys + qry.index[q0], qry.index + q0, nb, flags, qry_ctx + q0,
 						      ref_ctx, &so, rows, cap, &n);
 			if (rc == KSSDHIP_ENOSPC) {
@@ -1093,8 +1131,10 @@ static int run_index_path(const ani_args_t *a)
 	kssd_obj_section_t *obj = xmalloc(cells * sizeof *obj);
 	kssd_ctx_mut2_t *uctx = unique ? xmalloc(cells * sizeof *uctx) : NULL;
 	kssd_obj_section_t *uobj = unique ? xmalloc(cells * sizeof *uobj) : NULL;
-	for (int q0 = 0; q0 < Q; q0 += blk) {
-		const int nb = Q - q0 < blk ? Q - q0 : blk;
+	for (int q0 = 0, nb = 0; q0 < Q; q0 += nb) {
+		nb = block_from(qry.index, q0, Q, blk, nrec);
+		if (R > 0)
+			tune_block(ix, qry.keys + qry.index[q0], qry.index + q0, nb, flags);
 		if (R > 0)
 			hip_check(kssdhip_count_block(ix, qry.keys + qry.index[q0], qry.index + q0, nb, q0, flags, ctx,
 						      obj, uctx, uobj),
@@ -1199,8 +1239,9 @@ static double *self_lower(const kssd_sketchdir_t *s, int threads, bool ignorecon
 	const int blk = block_queries(n, n, false);
 	kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * (n ? n : 1) * sizeof *ctx);
 	kssd_obj_section_t *obj = xmalloc((size_t)blk * (n ? n : 1) * sizeof *obj);
-	for (int q0 = 0; q0 < n; q0 += blk) {
-		const int nb = n - q0 < blk ? n - q0 : blk;
+	for (int q0 = 0, nb = 0; q0 < n; q0 += nb) {
+		nb = block_from(s->index, q0, n, blk, (size_t)s->index[n]);
+		tune_block(ix, s->keys + s->index[q0], s->index + q0, nb, flags);
 		hip_check(kssdhip_count_block(ix, s->keys + s->index[q0], s->index + q0, nb, q0, flags, ctx, obj, NULL,
 					      NULL),
 			  "GPU counting failed");
@@ -1973,8 +2014,8 @@ static edge_list_t *self_edges(const matrix_args_t *a, const kssd_sketchdir_t *s
 	kssd_ctx_mut2_t *ctx = xmalloc((size_t)blk * (n ? n : 1) * sizeof *ctx);
 	kssd_obj_section_t *obj = xmalloc((size_t)blk * (n ? n : 1) * sizeof *obj);
 	uint32_t *votes = indexed ? xmalloc((size_t)blk * (n ? n : 1) * sizeof *votes) : NULL;
-	for (int q0 = 0; q0 < n; q0 += blk) {
-		const int nb = n - q0 < blk ? n - q0 : blk;
+	for (int q0 = 0, nb = 0; q0 < n; q0 += nb) {
+		nb = block_from(s->index, q0, n, blk, nrec);
 		hip_check(kssdhip_count_block(ix, s->keys + s->index[q0], s->index + q0, nb, q0, KSSDHIP_LOWER, ctx, obj,
 					      NULL, NULL),
 			  "GPU counting failed");

@@ -51,6 +51,11 @@ class Tiling(ctypes.Structure):
                 ("group_partitions", I), ("query_threads", I), ("query_slots", I)]
 
 
+class TuneResult(ctypes.Structure):
+    """kssdhip_tune_result_t."""
+    _fields_ = [("tiling", Tiling), ("ms", D), ("from_cache", I), ("candidates", I), ("cached", I)]
+
+
 class SelectOpts(ctypes.Structure):
     """kssdhip_select_opts_t: the index path's pass gate and the FP64 value
     computed for each passing pair."""
@@ -294,6 +299,9 @@ def load_kssdhip():
     lib.kssdhip_select_hits_dev.argtypes = [V, I, I, ctypes.c_double, U32, V, V, V]
     lib.kssdhip_index_matched_records.argtypes = [V, V, V, I, U64, ctypes.POINTER(U64)]
     lib.kssdhip_probe_random_reads.argtypes = [S, U64, ctypes.POINTER(D)]
+    lib.kssdhip_index_autotune.argtypes = [V, V, V, I, U64, U32, ctypes.c_char_p, I, ctypes.POINTER(TuneResult)]
+    lib.kssdhip_index_autotune_host.argtypes = [V, V, V, I, U32, ctypes.c_char_p, I, ctypes.POINTER(TuneResult)]
+    lib.kssdhip_index_tiling_from_cache.argtypes = [V, I, U64, U32, ctypes.c_char_p, ctypes.POINTER(TuneResult)]
     lib.kssdhip_select_block.argtypes = [V, V, V, I, U32, V, V, ctypes.POINTER(SelectOpts), V, S,
                                          ctypes.POINTER(S)]
     lib.kssdhip_select_block_dev.argtypes = [V, V, V, I, U64, U32, V, V, ctypes.POINTER(SelectOpts), V, U32,
@@ -362,11 +370,41 @@ class KssdHip:
             t.mode = mode
         if chunk_records is not None:
             t.chunk_records = chunk_records
+            if chunk_records > 6144 and group_partitions is None:
+                t.group_partitions = 1  # partition groups need chunk_records <= 6144
         if block_threads is not None:
             t.block_threads = block_threads
         if auto_min_keys is not None:
             t.auto_min_keys = auto_min_keys
         self.check(self.lib.kssdhip_index_set_tiling(h, ctypes.byref(t)))
+
+    def autotune(self, h, d_keys, d_skidx, nq, total, flags=0, cache_path=None, force=False):
+        """Time the kernel shapes on a device-resident block (or take the
+        cached one), apply the fastest; returns a TuneResult."""
+        r = TuneResult()
+        self.check(self.lib.kssdhip_index_autotune(h, d_keys, d_skidx, nq, total, flags,
+                                                   str(cache_path).encode() if cache_path else None, int(force),
+                                                   ctypes.byref(r)))
+        return r
+
+    def autotune_host(self, h, keys, skidx_abs, flags=0, cache_path=None, force=False):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        skidx_abs = np.ascontiguousarray(skidx_abs, dtype=np.uint64)
+        r = TuneResult()
+        self.check(self.lib.kssdhip_index_autotune_host(h, _p(keys), _p(skidx_abs), len(skidx_abs) - 1, flags,
+                                                        str(cache_path).encode() if cache_path else None,
+                                                        int(force), ctypes.byref(r)))
+        return r
+
+    def tiling_from_cache(self, h, nq, total, flags=0, cache_path=None):
+        """(applied?, TuneResult)."""
+        r = TuneResult()
+        rc = self.lib.kssdhip_index_tiling_from_cache(h, nq, total, flags,
+                                                      str(cache_path).encode() if cache_path else None,
+                                                      ctypes.byref(r))
+        if rc < 0:
+            self.check(rc)
+        return rc == 1, r
 
     def get_tiling(self, h):
         t = Tiling()

@@ -10,6 +10,11 @@ sys.path.insert(0, str(HERE))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
+    # kernel shapes stay the handles' defaults: no tuning cache is read at
+    # index open, and cache writes go to a file of this session only
+    os.environ["KSSDHIP_NO_TUNE_CACHE"] = "1"
+    import tempfile
+    os.environ["KSSDHIP_TUNE_CACHE"] = str(Path(tempfile.mkdtemp(prefix="kssdhip_tune_")) / "tuning.json")
 
 
 @pytest.fixture(scope="session")
