@@ -139,6 +139,20 @@ def measured_lds(tiling, nref, nqry):
     return None, f"profiles/{files[0].name} (measured with other tilings)"
 
 
+def counters_note(tiling):
+    """What bounds k_pjoin at C2, from the committed counter summary when it
+    was measured at this run's tiling."""
+    vals, src = measured_profile("pjoin_c2_1000x1000_counters")
+    if not vals or (tiling["mode"], tiling["chunk_records"], tiling["block_threads"],
+                    tiling.get("group_partitions", 1)) != (2, 5696, 1024, 4):
+        return None
+    return {"source": src, "k_pjoin_us": vals.get("k_pjoin_avg_us"),
+            "dram_bytes_per_dispatch": vals.get("fetch_bytes_per_dispatch"),
+            "wave_cycles_waiting": vals.get("wait_any_frac"), "waves_per_cu": vals.get("mean_occupancy_per_cu"),
+            "reading": "latency of dependent phases (stage, scan, bucket build, per-key probe chains) with one "
+                       "16-wave workgroup per CU; DRAM at about 29% of peak"}
+
+
 def matched_events(records, qkeys, obj_bits):
     """H: (query key, index record) pairs sharing a context, the join's
     unit of work (one object-mismatch minimum and one counter update each)."""
@@ -1177,6 +1191,7 @@ def main():
                     help="run only the headline workload (for profiling runs)")
     ap.add_argument("--data-cache", default="",
                     help="directory to keep the C3 sketches in between runs (profiling passes)")
+    ap.add_argument("--c2-only", action="store_true", help="only the C2 step (for profiling runs)")
     ap.add_argument("--cli-queries", type=int, default=100000,
                     help="C3 queries run end to end through kssd3a-hip ani (0 = skip)")
     ap.add_argument("--serve-refs", type=int, default=20000,
@@ -1208,8 +1223,11 @@ def main():
         _, build = index_build_bench(host, kh.KssdHip(0), rk, ri, threads, reps=3)
         print(json.dumps({"index_build": build, "index_genomes": nref}), flush=True)
         return
-    c3, shared = c3_bench(args, host, threads)
-    headline = {
+    if args.c2_only:  # profiling runs of the C2 step alone
+        c3, shared, headline = None, None, {}
+    else:
+        c3, shared = c3_bench(args, host, threads)
+    headline = {} if args.c2_only else {
         "metric": "ANI pairs/sec on 1 MI355X, batched vs plain kernel; p50/p99 query latency",
         "value": c3["value"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": c3["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -1422,12 +1440,14 @@ def main():
         f"= {cpu_pairs_per_s:.3e} pairs/s; counts bit-exact {counts_exact}, max rel ANI err {max_rel:.2e}")
 
     # ---------------- serving loop on a large resident index
-    big_index = index_bench(args, host, threads, plain_ms / npl / 1e3, shared)
-    cli = cli_e2e(args, host, shared, threads) if args.cli_queries else None
-    c1 = c1_bench(host, hip, rt, threads, args.seed + 500)
-    shared["hip"].close_index(shared["h"])
-    del shared
-    serving = serve_bench(args, host, hip, threads, plain_ms / npl / 1e3) if args.serve_refs else None
+    big_index = cli = c1 = serving = None
+    if not args.c2_only:
+        big_index = index_bench(args, host, threads, plain_ms / npl / 1e3, shared)
+        cli = cli_e2e(args, host, shared, threads) if args.cli_queries else None
+        c1 = c1_bench(host, hip, rt, threads, args.seed + 500)
+        shared["hip"].close_index(shared["h"])
+        del shared
+        serving = serve_bench(args, host, hip, threads, plain_ms / npl / 1e3) if args.serve_refs else None
 
     eager_pairs_per_s = pairs_per_s
     if graph_ok:
@@ -1463,7 +1483,8 @@ def main():
                      "kernel": "k_split+k_pjoin" if tiling["mode"] == 2 else "k_join", "algorithmic_bytes": algo_bytes,
                      "bytes_model": "8*N_Q + 12*M + 16*P_out, P_out = Q*R (dense output)",
                      "N_Q": total, "M": M, "P_out": cells, "P_touched": p_touched,
-                     "avg_launch_ms": float(np.mean(join_ms))},
+                     "avg_launch_ms": float(np.mean(join_ms)),
+                     "bound_by": counters_note(tiling)},
         "lds": {"events": H, "events_per_s": H / join_s,
                 "events_what": "(query key, index record) pairs sharing a context, each one object-mismatch "
                                "minimum and one counter update",
@@ -1481,6 +1502,10 @@ def main():
         "index_10k": big_index,
         "serving": serving,
     }
+    if args.c2_only:
+        hip.close_index(h)
+        print(json.dumps({"c2": c2}), flush=True)
+        return
     rec = dict(headline)
     rec["c2"] = c2
     rec["batched_over_plain"] = c3["value"] / c2["plain_kernel"]["pairs_per_s"]

@@ -65,9 +65,9 @@ extern "C" int kssdhip_probe_random_reads(size_t bytes, uint64_t reads, double *
 	}
 	hipLaunchKernelGGL(k_fill_words, dim3(8192), dim3(256), 0, s, buf, words);
 	for (int rep = 0; rep < 3; ++rep) {
-		hipEventRecord(e0, s);
+		(void)hipEventRecord(e0, s);
 		hipLaunchKernelGGL(k_random_reads, dim3(8192), dim3(256), 0, s, buf, words - 1, reads, sink);
-		hipEventRecord(e1, s);
+		(void)hipEventRecord(e1, s);
 		float ms = 0;
 		if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
 			rc = kssdhip_fail(KSSDHIP_EHIP, "probe_random_reads: %s", hipGetErrorString(hipGetLastError()));

@@ -357,11 +357,11 @@ extern "C" int kssdhip_index_autotune(kssdhip_index *ix, const uint64_t *d_keys,
 			continue;
 		}
 		const int reps = 3;
-		hipEventRecord(e0, s);
+		(void)hipEventRecord(e0, s);
 		for (int r = 0; r < reps; ++r)
 			(void)kssd_count_dev(ix, d_keys, d_skidx, nq, total, 0, flags, c0, o0, uc, uo, s, &ix->split,
 					     &ix->qws);
-		hipEventRecord(e1, s);
+		(void)hipEventRecord(e1, s);
 		float ms = 0;
 		if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
 			rc = kssdhip_fail(KSSDHIP_EHIP, "autotune: %s", hipGetErrorString(hipGetLastError()));
