@@ -490,6 +490,17 @@ def c3_bench(args, host, threads):
         rt.free(b_)
     value = kh.EpilogueOpts(3, 1, 0, 1, 0, 0, 8, 20, 44, 0.5, 1.0)  # CtxMoE ANI, float AFs (index path)
     sopts = kh.SelectOpts(3, 0.5, 0.95, 0, value)
+    # the kernel shape: the library's autotuner on a reference-sized block of
+    # the queries (dense counts; the query-major shapes it tries are the
+    # select's)
+    nt = min(4096, nq)
+    t = time.time()
+    d_tc, d_to = rt.malloc(8 * nt * nref), rt.malloc(8 * nt * nref)
+    tiling, tune_sum = tune.tune(hip, rt, h, d_k, d_i, nt, int(qi[nt]), nref, 0, d_tc, d_to, stream,
+                                 force=args.retune, log=log)
+    rt.free(d_tc)
+    rt.free(d_to)
+    log(f"C3 tiling {tiling} ({time.time() - t:.1f}s)")
 
     def step():
         hip.select_block_dev(h, d_k, d_i, nq, total, 0, d_qc, d_rc, sopts, d_rows, cap, d_n, d_first, d_cnt,
@@ -634,6 +645,7 @@ def c3_bench(args, host, threads):
         "gate": {"ctxcut": 3, "afcut": 0.5, "anicut": 0.95, "value": "CtxMoE ANI (FP64, float AFs)"},
         "parity": {"cpu_sample_queries": ncpu, "passing_pairs_bit_exact": exact, "properties": props},
         "forms": forms,
+        "tiling": tiling, "tune": tune_sum,
         "roofline": roof,
         "cpu_baseline": {"value": cpu_pps, "unit": "pairs/s", "cores": threads, "kind": "port",
                          "sample": f"first {ncpu} queries x {nref} refs: oracle block count (fenceposts per block, "

@@ -153,8 +153,8 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  * it to the outputs once (hits on other genomes go to the outputs
  * directly), which needs chunk_records <= 6144.
  * query_threads (mode 3; 0 = default 512): threads per query-major
- * workgroup, 64..1024, multiple of 64.  query_slots (mode 3; 0 = default):
- * LDS counter slots per query when ref_gnum exceeds what a direct
+ * workgroup, 64..512, multiple of 64.  query_slots (mode 3; 0 = default
+ * 512): LDS counter slots per query when ref_gnum exceeds what a direct
  * per-genome LDS row holds, a power of two in 256..8192; a query that
  * touches more genomes spills the rest to a per-workgroup row in HBM.
  */
@@ -439,10 +439,21 @@ int kssdhip_select_block_dev_noalloc(kssdhip_index *index, const uint64_t *d_key
  * device-resident block (relative offsets as kssdhip_count_block_dev),
  * i.e. the records a join of the block must read at least once.  No
  * reference counterpart; used to size work and to state the compulsory
- * traffic of a block.  Synchronous (allocates and frees a byte per record).
+ * traffic of a block.  Synchronous (allocates and frees a byte per slot of
+ * the record table).
  */
 int kssdhip_index_matched_records(kssdhip_index *index, const uint64_t *d_keys, const uint64_t *d_skidx,
 				  int section_gnum, uint64_t total_keys, uint64_t *matched);
+
+/*
+ * Diagnostic: the layout of the handle's record table, the structure the
+ * query-major join reads one 128-byte line of per query context: lines in
+ * all (0: the handle has none, e.g. an index of 2^20 genomes), overflow
+ * lines among them, home-line bits (2^home_bits home lines) and context
+ * bits.  No reference counterpart.
+ */
+int kssdhip_index_table_info(const kssdhip_index *index, uint64_t *lines, uint64_t *overflow_lines, int *home_bits,
+			     int *ctx_bits);
 
 /*
  * Diagnostic: the device's rate of independent random 8-byte reads from a

@@ -128,12 +128,12 @@ __global__ void k_unpack_records(const uint32_t *raw, uint64_t n, uint64_t *ctxg
 // Flags records whose gid is out of range or that break the (ctxgid, obj)
 // order, so no later kernel can index a counter row out of bounds.
 __global__ void k_validate(const uint64_t *ctxgid, const uint32_t *obj, uint64_t n, uint32_t ref_gnum,
-			   uint32_t *bad)
+			   unsigned obj_bits, uint32_t *bad)
 {
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (uint64_t)gridDim.x * blockDim.x) {
 		const uint64_t c = ctxgid[i];
-		bool ok = (c & GID_MASK) < ref_gnum;
+		bool ok = (c & GID_MASK) < ref_gnum && (obj_bits >= 32 || (obj[i] >> obj_bits) == 0);
 		if (i) {
 			const uint64_t p = ctxgid[i - 1];
 			ok = ok && (p < c || (p == c && obj[i - 1] <= obj[i]));
@@ -565,7 +565,7 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 			uint32_t bad = 0;
 			hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ix->stream, d_bad, (uint64_t)1, 0u);
 			hipLaunchKernelGGL(k_validate, dim3(grid_for(n)), dim3(256), 0, ix->stream,
-					   ix->d_ctxgid, ix->d_obj, (uint64_t)n, (uint32_t)ref_gnum, d_bad);
+					   ix->d_ctxgid, ix->d_obj, (uint64_t)n, (uint32_t)ref_gnum, obj_bits, d_bad);
 			if (hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, ix->stream) != hipSuccess ||
 			    hipStreamSynchronize(ix->stream) != hipSuccess) {
 				rc = kssdhip_fail(KSSDHIP_EHIP, "index_open: validation failed to run");
@@ -573,7 +573,8 @@ extern "C" int kssdhip_index_open(const kssd_ctxgidobj_t *records, size_t n, int
 			}
 			if (bad) {
 				rc = kssdhip_fail(KSSDHIP_EINVAL,
-						  "index_open: records not sorted by (ctxgid, obj) or gid >= ref_gnum");
+						  "index_open: records not sorted by (ctxgid, obj), gid >= ref_gnum or an "
+						  "object wider than obj_bits");
 				break;
 			}
 		}
@@ -647,7 +648,7 @@ extern "C" size_t kssdhip_index_device_bytes(const kssdhip_index *ix)
 	size_t b = ((size_t)ix->n + INDEX_PAD) * 12 + ((((size_t)1) << ix->k) + 1) * 4;
 	b += ix->keys.bytes + ix->skidx.bytes + ix->ctx.bytes + ix->obj.bytes + ix->uctx.bytes +
 	     ix->uobj.bytes + ix->idx.bytes + ix->split.bytes + ((size_t)ix->P + 1) * 12;
-	b += ((size_t)ix->n + 1) * 16 + (((size_t)1) << ix->ck) * 8 + (((size_t)1) << ix->fk) * 8 + ix->qws.bytes + ix->sdense.bytes + ix->rows.bytes +
+	b += (size_t)ix->br_lines * 128 + ix->qws.bytes + ix->sdense.bytes + ix->rows.bytes +
 	     ix->qmeta.bytes + ix->qctxb.bytes + ix->rctxb.bytes;
 	return b;
 }
@@ -689,7 +690,7 @@ int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_
 	// into them under ACCUMULATE), so it needs no zeroed outputs
 	const bool work = total && nq && ix->n;
 	const bool voting = vote_max_freq || vote_step > 1;
-	const bool qjoin = work && qws && !voting && ix->d_qrec &&
+	const bool qjoin = work && qws && !voting && ix->d_brec &&
 			   (ix->mode == 3 || (ix->mode == 0 && kssd_qjoin_auto(ix, nq, total)));
 	if (cells && !(flags & KSSDHIP_ACCUMULATE) && !qjoin) {
 		hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(cells)), dim3(256), 0, s,

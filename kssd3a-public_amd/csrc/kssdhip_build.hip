@@ -498,6 +498,18 @@ int radix_sort(uint64_t *&k0, uint32_t *&v0, uint64_t *&k1, uint32_t *&v1, uint6
 
 } // namespace
 
+static_assert(SCAN_BLOCK == KSSD_SCAN_BLOCK, "scan block size shared with the callers");
+
+void kssd_scan_u32_excl(uint32_t *a, uint64_t n, uint32_t *sums, hipStream_t s)
+{
+	if (!n)
+		return;
+	const uint32_t nsum = (uint32_t)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+	hipLaunchKernelGGL(k_scan_sums, dim3(nsum), dim3(SCAN_THREADS), 0, s, a, n, sums);
+	hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, sums, nsum);
+	hipLaunchKernelGGL(k_scan_apply, dim3(nsum), dim3(SCAN_THREADS), 0, s, a, n, sums);
+}
+
 extern "C" int kssdhip_build_index(const uint64_t *keys, const uint64_t *skidx, int nsamples, unsigned obj_bits,
 				   kssd_ctxgidobj_t *records, uint8_t *unique_bits)
 {

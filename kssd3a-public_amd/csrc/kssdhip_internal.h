@@ -10,6 +10,7 @@ constexpr int GID_NBITS = 20;                         // sketch_rearrange.h:7
 constexpr uint64_t GID_MASK = (1ull << GID_NBITS) - 1ull;
 constexpr int MIN_INIT = 12;                          // NUM_CODENS + 1, command_ani.h:535
 constexpr size_t INDEX_PAD = 8;                       // device index arrays hold n + 8 records
+constexpr uint64_t KSSD_SCAN_BLOCK = 4096;            // elements per workgroup of kssd_scan_u32_excl
 
 // dna_popcount.h:45-49: one bit per differing 2-bit base lane
 __device__ __forceinline__ int dna_popcount_dev(uint32_t x)
@@ -117,15 +118,12 @@ struct kssdhip_index {
 	int group = 1;
 	int qthreads = 0; // query-major workgroup size (0: default)
 	int qslots = 0;   // query-major LDS slots (0: default)
-	// query-major join (kssdhip_qjoin.hip): the records again as 16-byte
-	// {ctxgid, obj, 0} entries, and an open-addressed table of 2^ck slots,
-	// one per distinct context: {first record, 16-bit fingerprint, run length}
-	uint4 *d_qrec = nullptr;
-	unsigned long long *d_ctab = nullptr;
-	int ck = 0;
-	// a word-blocked Bloom filter of the contexts (2^fk 64-bit words)
-	unsigned long long *d_filt = nullptr;
-	int fk = 0;
+	// query-major join (kssdhip_qjoin.hip): the record table, br_lines
+	// 128-byte lines (2^br_bits home lines, then br_ovf overflow lines) over
+	// contexts of br_w bits; nullptr when the index cannot have one
+	unsigned long long *d_brec = nullptr;
+	uint64_t br_lines = 0, br_ovf = 0;
+	int br_w = 0, br_bits = 0;
 	int cus = 0; // compute units of the device
 	int qj_per_cu[2] = {0, 0}; // query-major workgroups per CU (plain, unique counters) at the tiling
 	// grow-only scratch: host-pointer entry points, the split table, the
@@ -145,6 +143,10 @@ int kssd_count_dev(kssdhip_index *ix, const uint64_t *d_keys, const uint64_t *d_
 		   int qry_gid_offset, uint32_t flags, kssd_ctx_mut2_t *d_ctx, kssd_obj_section_t *d_obj,
 		   kssd_ctx_mut2_t *d_uctx, kssd_obj_section_t *d_uobj, hipStream_t s, DevBuf *split, DevBuf *qws,
 		   uint32_t vote_max_freq = 0, uint32_t vote_step = 1);
+
+// exclusive scan of a[0, n) in place (kssdhip_build.hip); sums: at least
+// ceil(n / KSSD_SCAN_BLOCK) words of scratch
+void kssd_scan_u32_excl(uint32_t *a, uint64_t n, uint32_t *sums, hipStream_t s);
 
 // query-major join (kssdhip_qjoin.hip)
 int kssd_qjoin_build(kssdhip_index *ix);

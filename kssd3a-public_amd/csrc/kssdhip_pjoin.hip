@@ -893,7 +893,7 @@ extern "C" int kssdhip_index_set_tiling(kssdhip_index *ix, const kssdhip_tiling_
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: null argument");
 	if (t->mode < 0 || t->mode > 3)
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: mode %d", t->mode);
-	if (t->query_threads && (t->query_threads < 64 || t->query_threads > 1024 || t->query_threads % 64))
+	if (t->query_threads && (t->query_threads < 64 || t->query_threads > 512 || t->query_threads % 64))
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: query_threads %d", t->query_threads);
 	if (t->query_slots && (t->query_slots < 256 || t->query_slots > 8192 || (t->query_slots & (t->query_slots - 1))))
 		return kssdhip_fail(KSSDHIP_EINVAL, "set_tiling: query_slots %d (a power of two in 256..8192)",

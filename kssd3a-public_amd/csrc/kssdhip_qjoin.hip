@@ -42,51 +42,67 @@ constexpr uint32_t NONE = 0xffffffffu;
 constexpr int MAX_PROBES = 32;
 constexpr uint32_t DIRECT_LDS_MAX = 64u * 1024u; // direct per-genome rows up to this many LDS bytes
 constexpr int DEFAULT_THREADS = 512;
-constexpr uint32_t DEFAULT_SLOTS = 2048;
+constexpr int QJ_MAX_THREADS = 512; // workgroup size bound of k_qjoin
+#ifndef QJ_WAVES_PER_SIMD
+#define QJ_WAVES_PER_SIMD 4 // register budget of k_qjoin: at most 128 VGPRs, four waves per SIMD
+#endif
+constexpr uint32_t DEFAULT_SLOTS = 512;
 // workspace: [0, WS_HEAD) work counter and the order's histogram and
 // cursors (65 words), then the spill rows, then the work order (nq words,
 // WS_ALIGN-rounded)
 constexpr size_t WS_HEAD = 512;
 constexpr uint32_t WS_HEAD_WORDS = 65;
 constexpr size_t WS_ALIGN = 256;
-#ifndef QJ_KB
-#define QJ_KB 2
-#endif
-constexpr int KB = QJ_KB; // keys per thread whose lookups are in flight together (1, 2 or 4)
 
 enum { QJ_DENSE = 0, QJ_ACCUM = 1, QJ_SELECT = 2 };
 
-// The context table: open addressing over 2^k 64-bit slots, one per
-// distinct index context c: bits 0..31 the first record of c's run, bits
-// 32..47 a 16-bit fingerprint of c, bits 48..63 the run's length (65535:
-// at least that many; the walk then stops where the context changes).  The
-// home slot is the top k bits of a 64-bit mix of c, the fingerprint its low
-// 16 bits, and collisions probe the following slots (load factor <= 1/2,
-// so a probe sequence rarely leaves its 64-byte segment).  A context absent
-// from the index costs one segment; a present one that segment and the
-// records of its run.
-constexpr unsigned long long SLOT_EMPTY = ~0ull;
-constexpr uint32_t RUN_CAP = 0xffffu;
+// The record table: the index's records bucketed by context, so that one
+// 128-byte memory line answers a lookup.  A context c of w = ctx_width bits
+// is mapped by a bijection p = br_perm(c) of w-bit values; the top bb bits
+// of p name c's home line, the low w - bb bits (the remainder) are stored
+// in each of c's slots.  As (home, remainder) determines c, a slot match is
+// exact: there is no fingerprint to confirm.  A line holds 16 u64 slots:
+//   record  remainder << (20 + obj_bits) | gid << obj_bits | obj  (< 2^63)
+//   long    remainder << (20 + obj_bits) | GID_MASK << obj_bits, then a
+//           payload slot 3 << 62 | count << 32 | first record: the
+//           context's records in the sorted index (contexts of more than
+//           BR_LMAX records)
+//   cont    1 << 63 | line (slot 15 only): the home's contexts continue in
+//           that overflow line
+//   empty   all ones
+// A context's slots are contiguous and never split between lines; its
+// records are in index order ((gid, obj) ascending).  bb >= 21 keeps every
+// record and header below 2^63 for object widths up to 32 bits (remainder
+// + gid + obj = 84 - bb bits), and gid GID_MASK is reserved for headers, so
+// an index of 2^20 genomes has no table.  Home lines are sized for about 6
+// slots each, so few homes need an overflow line: a context absent from
+// the index costs one line, a present one the same line (plus its run in
+// the sorted index when long).
+constexpr int BR_SLOTS = 16;
+constexpr uint32_t BR_LMAX = 12;
+constexpr int BR_MIN_BITS = 21;
+constexpr unsigned long long BR_EMPTY = ~0ull;
+constexpr unsigned long long BR_HIGH = 1ull << 63;
+constexpr unsigned long long BR_PAYLOAD = 3ull << 62;
+constexpr uint64_t BR_MAX_LONG = (1ull << 30) - 1; // records of one long context
 
-__host__ __device__ __forceinline__ uint64_t ctx_mix(uint64_t c)
+__host__ __device__ __forceinline__ int br_ctx_width(unsigned obj_bits)
 {
-	// the murmur3 64-bit finalizer (a published bijective mixer)
-	c ^= c >> 33;
-	c *= 0xff51afd7ed558ccdull;
-	c ^= c >> 33;
-	c *= 0xc4ceb9fe1a85ec53ull;
-	c ^= c >> 33;
-	return c;
+	return obj_bits >= 20 ? 64 - (int)obj_bits : 44;
 }
 
-// 16-byte records {ctxgid low, ctxgid high, obj, 0}
-__global__ void k_pack16(const uint64_t *__restrict__ ctxgid, const uint32_t *__restrict__ obj, uint64_t n,
-			 uint4 *rec)
+// A bijection of w-bit values (1 <= w <= 44): xor-shifts and odd
+// multipliers modulo 2^w, each invertible.
+__host__ __device__ __forceinline__ uint64_t br_perm(uint64_t x, int w)
 {
-	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-		const uint64_t c = ctxgid[i];
-		rec[i] = make_uint4((uint32_t)c, (uint32_t)(c >> 32), obj[i], 0u);
-	}
+	const uint64_t m = (1ull << w) - 1ull;
+	const int s = (w + 1) / 2;
+	x ^= x >> s;
+	x = (x * 0x9e3779b97f4a7c15ull) & m;
+	x ^= x >> s;
+	x = (x * 0xc2b2ae3d27d4eb4full) & m;
+	x ^= x >> s;
+	return x;
 }
 
 __device__ __forceinline__ bool run_start(const uint64_t *__restrict__ ctxgid, uint64_t i)
@@ -94,59 +110,147 @@ __device__ __forceinline__ bool run_start(const uint64_t *__restrict__ ctxgid, u
 	return i == 0 || (ctxgid[i] >> GID_NBITS) != (ctxgid[i - 1] >> GID_NBITS);
 }
 
-// number of distinct contexts (record runs)
-__global__ __launch_bounds__(256) void k_count_runs(const uint64_t *__restrict__ ctxgid, uint64_t n,
-						    unsigned long long *count)
+struct BrGeom {
+	int w, bb;          // context width, home bits
+	unsigned ob;        // object bits
+	uint64_t homes;     // 2^bb
+};
+
+__device__ __forceinline__ uint64_t br_home(const BrGeom &g, uint64_t c)
 {
-	uint32_t k = 0;
-	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-		k += run_start(ctxgid, i);
-	for (int off = 32; off > 0; off >>= 1)
-		k += __shfl_xor(k, off, 64);
-	if ((threadIdx.x & 63) == 0 && k)
-		atomicAdd(count, (unsigned long long)k);
+	return br_perm(c, g.w) >> (g.w - g.bb);
 }
 
-__global__ void k_ctab_insert(const uint64_t *__restrict__ ctxgid, uint64_t n, int k, unsigned long long *tab)
+// records of the context whose run starts at i (until the context changes)
+__device__ __forceinline__ uint64_t br_run_len(const uint64_t *__restrict__ ctxgid, uint64_t n, uint64_t i)
 {
-	const uint64_t mask = (1ull << k) - 1;
+	const uint64_t c = ctxgid[i] >> GID_NBITS;
+	uint64_t e = i + 1;
+	while (e < n && (ctxgid[e] >> GID_NBITS) == c)
+		++e;
+	return e - i;
+}
+
+__device__ __forceinline__ uint32_t br_need(uint64_t len)
+{
+	return len <= BR_LMAX ? (uint32_t)len : 2u;
+}
+
+// pass 1: contexts per home line (contexts wider than w cannot match a query
+// and are left out)
+__global__ void k_br_count(const uint64_t *__restrict__ ctxgid, uint64_t n, BrGeom g, uint32_t *cnt)
+{
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
 		if (!run_start(ctxgid, i))
 			continue;
 		const uint64_t c = ctxgid[i] >> GID_NBITS;
-		uint64_t len = 1;
-		while (len < RUN_CAP && i + len < n && (ctxgid[i + len] >> GID_NBITS) == c)
-			++len;
-		const uint64_t h = ctx_mix(c);
-		const unsigned long long entry = (uint64_t)(uint32_t)i | ((h & 0xffffull) << 32) | (len << 48);
-		for (uint64_t s = h >> (64 - k);; s = (s + 1) & mask) {
-			if (atomicCAS(&tab[s], SLOT_EMPTY, entry) == SLOT_EMPTY)
-				break;
-		}
+		if (c >> g.w)
+			continue;
+		atomicAdd(&cnt[br_home(g, c)], 1u);
 	}
 }
 
-// A word-blocked Bloom filter of the index's contexts in front of the
-// context table: each context sets 4 bits of one 64-bit word (all from a
-// second mix of the context), about 10-20 bits per context, so that most
-// absent contexts are rejected by one word that stays in the device's
-// last-level cache instead of a probe of the table in HBM.
-#ifndef QJ_FILTER
-#define QJ_FILTER 1
-#endif
-__host__ __device__ __forceinline__ uint64_t filt_mix(uint64_t c) { return ctx_mix(c ^ 0x9e3779b97f4a7c15ull); }
-__host__ __device__ __forceinline__ unsigned long long filt_bits(uint64_t g)
-{
-	return (1ull << (g & 63)) | (1ull << ((g >> 6) & 63)) | (1ull << ((g >> 12) & 63)) | (1ull << ((g >> 18) & 63));
-}
-
-__global__ void k_filt_insert(const uint64_t *__restrict__ ctxgid, uint64_t n, int fk, unsigned long long *filt)
+// pass 2: each context's first record into its home's list (off: exclusive
+// scan of the counts; cur: zeroed cursors)
+__global__ void k_br_scatter(const uint64_t *__restrict__ ctxgid, uint64_t n, BrGeom g, const uint32_t *off,
+			     uint32_t *cur, uint32_t *list)
 {
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
 		if (!run_start(ctxgid, i))
 			continue;
-		const uint64_t g = filt_mix(ctxgid[i] >> GID_NBITS);
-		atomicOr(&filt[g >> (64 - fk)], filt_bits(g));
+		const uint64_t c = ctxgid[i] >> GID_NBITS;
+		if (c >> g.w)
+			continue;
+		const uint64_t h = br_home(g, c);
+		list[off[h] + atomicAdd(&cur[h], 1u)] = (uint32_t)i;
+	}
+}
+
+// pass 3 (count) and 5 (write), one thread per home: its contexts in index
+// order (the list segment sorted in place), laid out line by line; a line
+// keeps slot 15 for the continuation while more than 16 slots remain.
+// Count: overflow lines per home into nov.  Write: the home line and its
+// overflow lines from ovf_base + ovoff[h] on (all lines prefilled empty).
+template <bool WRITE>
+__global__ void k_br_place(const uint64_t *__restrict__ ctxgid, const uint32_t *__restrict__ robj, uint64_t n,
+			   BrGeom g, const uint32_t *__restrict__ off, const uint32_t *__restrict__ cnt, uint32_t *list,
+			   uint32_t *nov, const uint32_t *__restrict__ ovoff, uint64_t ovf_base,
+			   unsigned long long *lines)
+{
+	const uint64_t objmask = g.ob >= 32 ? 0xffffffffull : ((1ull << g.ob) - 1ull);
+	const int rs = GID_NBITS + (int)g.ob;
+	const uint64_t remmask = (1ull << (g.w - g.bb)) - 1ull;
+	for (uint64_t h = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; h < g.homes;
+	     h += (uint64_t)gridDim.x * blockDim.x) {
+		const uint32_t k = cnt[h];
+		if (!k) {
+			if (!WRITE)
+				nov[h] = 0;
+			continue;
+		}
+		uint32_t *ls = list + off[h];
+		if (!WRITE) { // index order (insertion sort: homes hold a few contexts)
+			for (uint32_t x = 1; x < k; ++x) {
+				const uint32_t v = ls[x];
+				uint32_t y = x;
+				while (y > 0 && ls[y - 1] > v) {
+					ls[y] = ls[y - 1];
+					--y;
+				}
+				ls[y] = v;
+			}
+		}
+		uint64_t remaining = 0;
+		for (uint32_t x = 0; x < k; ++x)
+			remaining += br_need(br_run_len(ctxgid, n, ls[x]));
+		uint64_t line = h;
+		uint32_t used = 0, slot = 0;
+		uint32_t cap = remaining <= BR_SLOTS ? BR_SLOTS : BR_SLOTS - 1;
+		for (uint32_t x = 0; x < k; ++x) {
+			const uint64_t i = ls[x];
+			const uint64_t len = br_run_len(ctxgid, n, i);
+			const uint32_t need = br_need(len);
+			if (slot + need > cap) { // the rest goes to a new overflow line
+				const uint64_t next = ovf_base + (WRITE ? ovoff[h] : 0) + used;
+				if (WRITE)
+					lines[line * BR_SLOTS + (BR_SLOTS - 1)] = BR_HIGH | next;
+				++used;
+				line = next;
+				slot = 0;
+				cap = remaining <= BR_SLOTS ? BR_SLOTS : BR_SLOTS - 1;
+			}
+			remaining -= need;
+			if (WRITE) {
+				const uint64_t c = ctxgid[i] >> GID_NBITS;
+				const uint64_t head = (br_perm(c, g.w) & remmask) << rs;
+				unsigned long long *dst = lines + line * BR_SLOTS + slot;
+				if (len <= BR_LMAX) {
+					for (uint64_t y = 0; y < len; ++y) {
+						const uint64_t gid = ctxgid[i + y] & GID_MASK;
+						dst[y] = head | (gid << g.ob) | ((uint64_t)robj[i + y] & objmask);
+					}
+				} else {
+					dst[0] = head | (GID_MASK << g.ob);
+					dst[1] = BR_PAYLOAD | (len << 32) | i;
+				}
+			}
+			slot += need;
+		}
+		if (!WRITE)
+			nov[h] = used;
+	}
+}
+
+// longest context run (a long context's count must fit its payload field)
+__global__ __launch_bounds__(256) void k_br_longest(const uint64_t *__restrict__ ctxgid, uint64_t n,
+						    unsigned long long *longest)
+{
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+		if (!run_start(ctxgid, i))
+			continue;
+		const uint64_t len = br_run_len(ctxgid, n, i);
+		if (len > BR_LMAX)
+			atomicMax(longest, (unsigned long long)len);
 	}
 }
 
@@ -222,12 +326,12 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t n)
 }
 
 struct QJArgs {
-	// index
-	const uint4 *rec;
-	const unsigned long long *ctab; // the context table, 2^ck slots
-	int ck;
-	const unsigned long long *filt; // the context filter, 2^fk words
-	int fk;
+	// index: the record table (2^bb home lines, then overflow lines) and the
+	// sorted records (long contexts)
+	const unsigned long long *brec;
+	int w, bb;
+	const uint64_t *ctxgid;
+	const uint32_t *robj;
 	uint64_t n;
 	uint64_t ctx_max;
 	uint32_t R; // ref_gnum
@@ -318,175 +422,273 @@ __device__ __forceinline__ void qj_add(const QTable &t, uint32_t gid, int m, boo
 		atomicAdd(cell + 1, dob);
 }
 
-__device__ __forceinline__ uint64_t rec_ctxgid(const uint4 &r)
+// Lookups stage their lines in LDS: each wave owns 64 rows of 128 bytes,
+// row m holding the line of the wave's m-th key.  Eight lanes load one line
+// together (a 16-byte piece each, so one wave instruction reads eight whole
+// lines), and a lane then reads its own row.  Piece p of row m sits at
+// position p ^ (m & 7), so that both the eight lanes writing a row and
+// eight consecutive lanes reading their rows meet distinct banks.
+constexpr uint32_t STAGE_ROW = BR_SLOTS * 8;        // bytes per row
+constexpr uint32_t STAGE_WAVE = 64 * STAGE_ROW;     // bytes per wave
+constexpr uint32_t OWN_WAVE = 128;                  // words of sole-owner bookkeeping per wave
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned long long row_slot(const LDS unsigned long long *row, uint32_t lane, uint32_t j)
 {
-	return ((uint64_t)r.y << 32) | r.x;
+	return row[(((j >> 1) ^ (lane & 7u)) << 1) | (j & 1u)];
 }
 
-// Resolves the probe that began at slot s with entry e for context c
-// (h = ctx_mix(c)): the first record of c's run (loaded into *r), its
-// length, or false when the index has no record of c.
-__device__ __forceinline__ bool ctab_resolve(const QJArgs &a, uint64_t c, uint64_t h, uint64_t s,
-					     unsigned long long e, uint32_t *d, uint32_t *len, uint4 *r)
+// Slot x holds a record or header of the context with remainder rem
+// (rs = 20 + obj_bits).  Special slots (bit 63 set, empty) never match: for
+// home bits >= 21 their value above bit rs exceeds every remainder.
+__device__ __forceinline__ bool slot_is(unsigned long long x, uint32_t rem, int rs)
 {
-	const uint64_t mask = (1ull << a.ck) - 1;
-	const uint32_t fp = (uint32_t)(h & 0xffffull);
-	for (;;) {
-		if (e == SLOT_EMPTY)
-			return false;
-		if ((uint32_t)((e >> 32) & 0xffffull) == fp) {
-			const uint32_t st = (uint32_t)e;
-			const uint4 x = a.rec[st];
-			if ((rec_ctxgid(x) >> GID_NBITS) == c) { // else another context with this fingerprint
-				*d = st;
-				*len = (uint32_t)(e >> 48);
-				*r = x;
-				return true;
-			}
-		}
-		s = (s + 1) & mask;
-		e = a.ctab[s];
+	if (rs >= 32)
+		return ((uint32_t)(x >> 32) >> (rs - 32)) == rem;
+	return (x >> rs) == (uint64_t)rem;
+}
+
+// match bits of the 16 slots of a row (bit j: slot j holds the context) and
+// the row's last slot
+__device__ __forceinline__ uint32_t row_scan(const LDS unsigned long long *row, uint32_t lane, uint32_t rem, int rs,
+					     unsigned long long *last)
+{
+	uint32_t mm = 0;
+#pragma unroll
+	for (uint32_t p = 0; p < BR_SLOTS / 2; ++p) {
+		const u64x2 v = *(const LDS u64x2 *)(row + ((p ^ (lane & 7u)) << 1));
+		mm |= (uint32_t)slot_is(v.x, rem, rs) << (2 * p);
+		mm |= (uint32_t)slot_is(v.y, rem, rs) << (2 * p + 1);
+		if (p == BR_SLOTS / 2 - 1)
+			*last = v.y;
 	}
+	return mm;
 }
 
-// One context group of the query (keys [i, s_end) share context c, whose
-// run starts at record d with `len` records, r = record d): walk the
-// context's per-genome runs (command_ani.h:521-575; LOWER: :668-716).
-template <bool UNIQ, bool DIRECT>
-__device__ __forceinline__ void qj_group(const QJArgs &a, const QTable &t, uint64_t i, uint64_t key, uint64_t c,
-					 uint32_t d, uint32_t len, uint4 r, uint64_t ke, int qgid,
-					 unsigned long long *spill_row, LDS uint32_t *spilled)
+// One genome run's record j of key `owner`'s context: (gid << 32 | obj),
+// from the owner's row (inline contexts; src = first slot) or from the
+// sorted index (long contexts; src = first record).
+__device__ __forceinline__ uint64_t qj_record(const QJArgs &a, const LDS unsigned long long *wst, uint32_t owner,
+					      bool lng, uint32_t src, uint32_t j, uint64_t objmask)
 {
 	const unsigned ob = a.obj_bits;
-	const uint64_t objmask = ob >= 64 ? ~0ull : ((1ull << ob) - 1ull);
+	if (lng)
+		return ((a.ctxgid[(uint64_t)src + j] & GID_MASK) << 32) | ((uint64_t)a.robj[(uint64_t)src + j] & objmask);
+	const unsigned long long x = row_slot(wst + owner * BR_SLOTS, owner, src + j);
+	return (((x >> ob) & GID_MASK) << 32) | (x & objmask);
+}
+
+// The walk of the wave's found contexts with one record per lane
+// (command_ani.h:516-575; LOWER: :668-716, genomes below the query only).
+// Key m of the wave (found) has cnt records, from slot src of its row or,
+// for a long context, from record src of the sorted index; its group is
+// the query's keys [i, i + gs).  The wave numbers all records in lane
+// order; lane L takes record r0 + L of each round, finds its key by a
+// binary search of the inclusive ends, and the lane holding the first
+// record of a genome run counts the run: object-mismatch minimum over the
+// cross product of the group's and the run's objects, one add per run
+// (reading any further records of the run itself).  With unique counters
+// each key's runs are counted in `own` (per lane of the wave: runs, then
+// the last run's genome, minimum and whether it was counted), so that the
+// key's lane applies the sole-owner rule afterwards.
+template <bool UNIQ, bool DIRECT>
+__device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, const LDS unsigned long long *wst,
+					     LDS uint32_t *own, uint32_t lane, bool found, bool lng, uint32_t src,
+					     uint32_t cnt, uint64_t i, uint32_t gs, uint64_t key, int qgid,
+					     unsigned long long *spill_row, LDS uint32_t *spilled)
+{
+	const unsigned ob = a.obj_bits;
+	const uint64_t objmask = ob >= 32 ? 0xffffffffull : ((1ull << ob) - 1ull);
+	const uint32_t mycnt = found ? cnt : 0u;
+	uint32_t incl = mycnt;
+#pragma unroll
+	for (uint32_t d = 1; d < 64; d <<= 1) {
+		const uint32_t y = __shfl_up(incl, d);
+		if (lane >= d)
+			incl += y;
+	}
+	const uint32_t total = __shfl(incl, 63);
+	if (total == 0)
+		return;
+	const uint32_t first = incl - mycnt; // the key's first record number
+	const uint32_t qobj = (uint32_t)(key & objmask);
+	const uint32_t shape = (gs << 1) | (lng ? 1u : 0u);
+	const bool groups = __any(found && gs > 1); // some key has a group of several keys
 	const bool lower = (a.flags & KSSDHIP_LOWER) != 0;
 	const bool ignore = (a.flags & KSSDHIP_IGNORECONFLICT) != 0;
-	const bool capped = len >= RUN_CAP; // a longer run: walk until the context changes
-	const uint64_t end = capped ? a.n : (uint64_t)d + len;
-	uint64_t s_end = i + 1;
-	while (s_end < ke && (a.keys[s_end] >> ob) == c)
-		++s_end;
-	int owners = 0, sole_diff = 0;
-	uint32_t sole_gid = 0;
-	bool sole_allowed = false;
-	for (;;) { // r = record d, the first of a genome run of context c
-		const uint64_t cg = rec_ctxgid(r);
-		const uint32_t gid = (uint32_t)(cg & GID_MASK);
-		const uint32_t rb = d;
-		uint4 nx = r;
-		bool more;
-		do {
-			++d;
-			more = d < end;
-			if (more)
-				nx = a.rec[d];
-		} while (more && rec_ctxgid(nx) == cg);
-		const uint32_t re = d;
-		if (capped)
-			more = more && (rec_ctxgid(nx) >> GID_NBITS) == c;
-		bool count = true;
-		if (lower && (int)gid >= qgid) {
-			if ((int)gid > qgid)
+	uint32_t carry_gid = NONE, carry_owner = 64;
+	for (uint32_t r0 = 0; r0 < total; r0 += 64) { // wave-uniform
+		const uint32_t tt = r0 + lane;
+		const bool act = tt < total;
+		uint32_t owner = 0; // lanes whose inclusive end is <= tt
+#pragma unroll
+		for (uint32_t step = 32; step; step >>= 1) {
+			const uint32_t e = __shfl(incl, (int)(owner + step - 1));
+			if (e <= tt)
+				owner += step;
+		}
+		owner = owner < 64 ? owner : 63;
+		const uint32_t j = tt - __shfl(first, (int)owner);
+		const uint32_t osrc = __shfl(src, (int)owner);
+		const uint32_t oshape = __shfl(shape, (int)owner);
+		const uint32_t ocnt = __shfl(mycnt, (int)owner);
+		const uint32_t oobj = __shfl(qobj, (int)owner);
+		uint64_t oi = 0;
+		if (groups) {
+			oi = __shfl(i, (int)owner);
+		}
+		const bool olng = oshape & 1u;
+		const uint32_t ogs = oshape >> 1;
+		const uint64_t rec = act ? qj_record(a, wst, owner, olng, osrc, j, objmask) : 0ull;
+		const uint32_t gid = act ? (uint32_t)(rec >> 32) : NONE;
+		// minimum over the group's objects against this record's object
+		auto gmin = [&](uint32_t obj) -> int {
+			int mm = dna_popcount_dev(oobj ^ obj);
+			mm = mm < MIN_INIT ? mm : MIN_INIT;
+			for (uint32_t x = 1; x < ogs && mm; ++x) {
+				const int pc = dna_popcount_dev((uint32_t)(a.keys[oi + x] & objmask) ^ obj);
+				mm = pc < mm ? pc : mm;
+			}
+			return mm;
+		};
+		int m = act ? gmin((uint32_t)rec) : MIN_INIT;
+		uint32_t pg = __shfl_up(gid, 1), po = __shfl_up(owner, 1);
+		if (lane == 0) {
+			pg = carry_gid;
+			po = carry_owner;
+		}
+		const uint32_t ng = __shfl_down(gid, 1), no = __shfl_down(owner, 1);
+		carry_gid = __shfl(gid, 63);
+		carry_owner = __shfl(owner, 63);
+		if (!act || (po == owner && pg == gid))
+			continue; // not the first record of its run
+		uint32_t n = 1;
+		const bool more = lane < 63 ? (tt + 1 < total && no == owner && ng == gid) : j + 1 < ocnt;
+		for (uint32_t jj = j + 1; more && jj < ocnt; ++jj) {
+			const uint64_t y = qj_record(a, wst, owner, olng, osrc, jj, objmask);
+			if ((uint32_t)(y >> 32) != gid)
 				break;
-			count = false;
-		}
-		if (count) {
-			if (UNIQ) {
-				++owners;
-				sole_gid = gid;
-			}
-			if (!(ignore && re - rb > 1)) {
-				int m = MIN_INIT;
-				for (uint64_t x = i; x < s_end && m; ++x) {
-					const uint32_t oq = (uint32_t)((x == i ? key : a.keys[x]) & objmask);
-					for (uint32_t y = rb; y < re; ++y) {
-						const uint32_t diff = oq ^ (y == rb ? r.z : a.rec[y].z);
-						if (diff == 0) {
-							m = 0;
-							break;
-						}
-						const int pc = dna_popcount_dev(diff);
-						m = pc < m ? pc : m;
-					}
-				}
-				qj_add<UNIQ, DIRECT>(t, gid, m, false, spill_row, spilled);
-				if (UNIQ) {
-					sole_diff = m;
-					sole_allowed = true;
-				}
+			++n;
+			if (m) {
+				const int pc = gmin((uint32_t)y);
+				m = pc < m ? pc : m;
 			}
 		}
-		if (!more)
-			break;
-		r = nx;
+		if (lower && (int)gid >= qgid)
+			continue;
+		const bool counted = !(ignore && n > 1);
+		if (UNIQ) {
+			__hip_atomic_fetch_add(&own[owner], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+			own[64 + owner] = gid | ((uint32_t)m << 20) | ((uint32_t)counted << 24);
+		}
+		if (counted)
+			qj_add<UNIQ, DIRECT>(t, gid, m, false, spill_row, spilled);
 	}
-	if (UNIQ && owners == 1 && sole_allowed)
-		qj_add<UNIQ, DIRECT>(t, sole_gid, sole_diff, true, spill_row, spilled);
+	if (UNIQ) {
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		if (mycnt) {
+			const uint32_t runs = own[lane], info = own[64 + lane];
+			if (runs == 1 && (info >> 24))
+				qj_add<UNIQ, DIRECT>(t, info & GID_MASK, (int)((info >> 20) & 15u), true, spill_row, spilled);
+			own[lane] = 0;
+		}
+	}
 }
 
 // Counts of query q into the table: the first key of each context group
-// does the work for the group.  Each thread takes KB of the query's keys at
-// a time (strided by the workgroup size, so loads of the keys coalesce) and
-// issues their context-table loads together before resolving any.
+// does the work for the group.  Each wave takes 64 consecutive keys at a
+// time (strided by the workgroup size, so loads of the keys coalesce; the
+// next keys are loaded while the current ones are looked up), stages their
+// home lines in its LDS rows (`stage`: STAGE_WAVE bytes per wave; all eight
+// line loads in flight before the first row is written; lanes without a
+// lookup load the home line of context 0, a valid address), scans each row
+// for its key's context and walks the found contexts together; a home that
+// continues in an overflow line has that line loaded into its row and is
+// scanned again, until every lookup is resolved.
 template <bool UNIQ, bool DIRECT>
 __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned long long *spill_row,
-			 LDS uint32_t *spilled)
+			 LDS uint32_t *spilled, LDS unsigned long long *stage)
 {
 	const uint64_t kb = a.skidx[q], ke = a.skidx[q + 1];
 	const unsigned ob = a.obj_bits;
 	const int qgid = a.qry_gid_offset + (int)q;
-	const uint64_t B = blockDim.x;
-	for (uint64_t base = kb + threadIdx.x; base < ke; base += KB * B) {
-		uint64_t key[KB], c[KB], hs[KB];
-		unsigned long long e[KB];
-		bool on[KB];
+	const uint64_t B = blockDim.x; // a multiple of 64
+	const uint32_t lane = threadIdx.x & 63u, g = lane >> 3, piece = lane & 7u;
+	LDS unsigned long long *wst = stage + (threadIdx.x >> 6) * (STAGE_WAVE / 8);
+	LDS unsigned long long *row = wst + lane * BR_SLOTS;
+	LDS uint32_t *own = (LDS uint32_t *)(stage + (blockDim.x >> 6) * (STAGE_WAVE / 8)) + (threadIdx.x >> 6) * OWN_WAVE;
+	const int rs = GID_NBITS + (int)ob;
+	const int rsh = a.w - a.bb;
+	const uint64_t remmask = (1ull << rsh) - 1ull;
+	uint64_t wb = kb + (threadIdx.x & ~63u); // wave-uniform
+	uint64_t key = wb + lane < ke ? a.keys[wb + lane] : 0ull;
+	uint64_t before = lane == 0 && wb > kb && wb < ke ? a.keys[wb - 1] : ~0ull; // the key ahead of the wave's first
+	for (; wb < ke; wb += B) {
+		const uint64_t i = wb + lane;
+		const bool in = i < ke;
+		const uint64_t c = key >> ob;
+		uint64_t prev = __shfl_up(key, 1u);
+		if (lane == 0)
+			prev = before;
+		const bool on = in && c <= a.ctx_max && (c >> a.w) == 0 && (i == kb || (prev >> ob) != c);
+		const uint64_t p = br_perm(on ? c : 0ull, a.w);
+		const uint32_t rem = (uint32_t)(p & remmask);
+		const uint64_t line = p >> rsh;
+		u64x2 v[8];
 #pragma unroll
-		for (int j = 0; j < KB; ++j) {
-			const uint64_t i = base + j * B;
-			on[j] = i < ke;
-			key[j] = on[j] ? a.keys[i] : 0ull;
+		for (uint32_t k = 0; k < 8; ++k) { // rows 8k .. 8k+7, eight lanes per row
+			const uint64_t lm = __shfl(line, (int)(8 * k + g));
+			v[k] = ((const u64x2 *)(a.brec + lm * BR_SLOTS))[piece];
 		}
+		// the next keys, while the lines are in flight
+		const uint64_t nb = wb + B;
+		const uint64_t nkey = nb + lane < ke ? a.keys[nb + lane] : 0ull;
+		const uint64_t nbefore = lane == 0 && nb < ke ? a.keys[nb - 1] : ~0ull;
 #pragma unroll
-		for (int j = 0; j < KB; ++j) {
-			const uint64_t i = base + j * B;
-			c[j] = key[j] >> ob;
-			const uint64_t prev = on[j] && i > kb ? a.keys[i - 1] : ~0ull;
-			on[j] = on[j] && c[j] <= a.ctx_max && (i == kb || (prev >> ob) != c[j]);
-		}
-		if (QJ_FILTER) {
+		for (uint32_t k = 0; k < 8; ++k)
+			*(LDS u64x2 *)(wst + (8 * k + g) * BR_SLOTS + ((piece ^ g) << 1)) = v[k];
+		// the group: usually this key alone (the next lane's key has another
+		// context); else the keys that follow with the same context
+		const uint64_t after = __shfl_down(key, 1u);
+		uint32_t gs = 1;
+		if (on && !(lane < 63 ? (i + 1 >= ke || (after >> ob) != c) : i + 1 >= ke))
+			while (i + gs < ke && (a.keys[i + gs] >> ob) == c)
+				++gs;
+		bool pending = on;
+		for (;;) { // wave-uniform: until every lookup is resolved
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			unsigned long long last = 0;
+			const uint32_t mm = pending ? row_scan(row, lane, rem, rs, &last) : 0u;
+			const bool found = mm != 0;
+			const uint32_t lo = found ? (uint32_t)__builtin_ctz(mm) : 0u;
+			bool lng = false;
+			uint32_t src = lo, cnt = (uint32_t)__builtin_popcount(mm);
+			if (found && ((row_slot(row, lane, lo) >> ob) & GID_MASK) == GID_MASK) { // header + payload
+				const unsigned long long pay = row_slot(row, lane, lo + 1);
+				lng = true;
+				src = (uint32_t)pay;
+				cnt = (uint32_t)((pay >> 32) & BR_MAX_LONG);
+			}
+			qj_wave_walk<UNIQ, DIRECT>(a, t, wst, own, lane, found, lng, src, cnt, i, gs, key, qgid, spill_row,
+						   spilled);
+			pending = pending && !found && (last & BR_PAYLOAD) == BR_HIGH;
+			if (!__any(pending))
+				break;
+			__builtin_amdgcn_wave_barrier(); // rows read by the walk before any is replaced
+			if (pending) { // the home continues in this overflow line
+				const u64x2 *nsrc = (const u64x2 *)(a.brec + (last & ~BR_HIGH) * BR_SLOTS);
 #pragma unroll
-			for (int j = 0; j < KB; ++j) {
-				const uint64_t g = filt_mix(c[j]);
-				const unsigned long long want = filt_bits(g);
-				const unsigned long long w = on[j] ? a.filt[g >> (64 - a.fk)] : 0ull;
-				on[j] = on[j] && (w & want) == want; // else the index has no record of c[j]
+				for (uint32_t pp = 0; pp < BR_SLOTS / 2; ++pp)
+					*(LDS u64x2 *)(row + ((pp ^ (lane & 7u)) << 1)) = nsrc[pp];
 			}
 		}
-#pragma unroll
-		for (int j = 0; j < KB; ++j) {
-			hs[j] = ctx_mix(c[j]);
-			e[j] = on[j] ? a.ctab[hs[j] >> (64 - a.ck)] : SLOT_EMPTY;
-		}
-		// one instance of the resolve and the group walk, fed from the batch
-		// by constant indexes (no private-array indexing)
-#pragma unroll 1
-		for (int j = 0; j < KB; ++j) {
-			uint64_t kj = key[0], cj = c[0], hj = hs[0];
-			unsigned long long ej = e[0];
-#pragma unroll
-			for (int x = 1; x < KB; ++x) {
-				if (j == x) {
-					kj = key[x];
-					cj = c[x];
-					hj = hs[x];
-					ej = e[x];
-				}
-			}
-			uint32_t d, len;
-			uint4 r;
-			if (ej != SLOT_EMPTY && ctab_resolve(a, cj, hj, hj >> (64 - a.ck), ej, &d, &len, &r))
-				qj_group<UNIQ, DIRECT>(a, t, base + j * B, kj, cj, d, len, r, ke, qgid, spill_row, spilled);
-		}
+		__builtin_amdgcn_wave_barrier(); // every row read before the next keys' lines land
+		key = nkey;
+		before = nbefore;
 	}
 }
 
@@ -669,7 +871,7 @@ __device__ void qj_output(const QJArgs &a, const QTable &t, uint32_t q, bool spi
 // work counter until none is left (the reference's guided schedule over
 // queries, command_ani.h:481).
 template <bool UNIQ, bool DIRECT>
-__global__ void k_qjoin(QJArgs a)
+__global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(QJ_WAVES_PER_SIMD))) void k_qjoin(QJArgs a)
 {
 	extern __shared__ unsigned long long qj_lds[];
 	__shared__ uint32_t s_q, s_spilled, s_n, s_base;
@@ -680,9 +882,18 @@ __global__ void k_qjoin(QJArgs a)
 	t.uc = UNIQ ? t.o + ncell : nullptr;
 	t.uo = UNIQ ? t.uc + ncell : nullptr;
 	t.gid = DIRECT ? nullptr : (LDS uint32_t *)(t.o + (UNIQ ? 3 : 1) * (uint64_t)ncell);
+	// the lookup rows follow the table (16-byte aligned: cells are 8 bytes,
+	// slot counts multiples of 4)
+	LDS unsigned long long *stage =
+		(LDS unsigned long long *)qj_lds + (uint64_t)ncell * (UNIQ ? 4 : 2) + (DIRECT ? 0 : ncell / 2);
 	t.mask = a.slots - 1;
 	t.shift = 32 - a.sbits;
 	unsigned long long *spill_row = DIRECT ? nullptr : a.spill + (uint64_t)blockIdx.x * a.R * (UNIQ ? 4u : 2u);
+	if (UNIQ) { // the sole-owner counts start at zero (each key's lane clears its count after use)
+		LDS uint32_t *own = (LDS uint32_t *)(stage + (blockDim.x >> 6) * (STAGE_WAVE / 8));
+		for (uint32_t k = threadIdx.x; k < (blockDim.x >> 6) * OWN_WAVE; k += blockDim.x)
+			own[k] = 0;
+	}
 	for (;;) {
 		if (threadIdx.x == 0) {
 			const uint32_t k = atomicAdd(a.next, 1u);
@@ -711,7 +922,7 @@ __global__ void k_qjoin(QJArgs a)
 			s_n = 0;
 		}
 		__syncthreads();
-		qj_count<UNIQ, DIRECT>(a, t, q, spill_row, (LDS uint32_t *)&s_spilled);
+		qj_count<UNIQ, DIRECT>(a, t, q, spill_row, (LDS uint32_t *)&s_spilled, stage);
 		__syncthreads();
 		const bool spilled = s_spilled != 0;
 		qj_output<UNIQ, DIRECT>(a, t, q, spilled, spill_row, (LDS uint32_t *)&s_n, (LDS uint32_t *)&s_base);
@@ -733,38 +944,56 @@ __global__ __launch_bounds__(256) void k_row_values(kssdhip_ani_row_t *rows, con
 	}
 }
 
-// Marks the records of every index context that occurs in the block
-// (marks: one byte per record, zeroed before), then k_count_marks adds them.
+// Marks the table slots of every index context that occurs in the block
+// (marks: one byte per slot, zeroed before); k_count_marks then adds one
+// record per marked record slot and a long context's count per marked header.
 __global__ __launch_bounds__(256) void k_mark_matched(QJArgs a, uint8_t *marks)
 {
 	const unsigned ob = a.obj_bits;
+	const int rs = GID_NBITS + (int)ob;
+	const int rsh = a.w - a.bb;
 	const uint64_t total = a.skidx[a.nq];
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
 		const uint64_t c = a.keys[i] >> ob;
-		if (c > a.ctx_max)
+		if (c > a.ctx_max || (c >> a.w))
 			continue;
 		if (i > 0 && (a.keys[i - 1] >> ob) == c)
 			continue; // the previous key (of this or the previous query) marked this context
-		const uint64_t h = ctx_mix(c);
-		const uint64_t slot = h >> (64 - a.ck);
-		uint32_t d, len;
-		uint4 r;
-		if (!ctab_resolve(a, c, h, slot, a.ctab[slot], &d, &len, &r))
-			continue;
-		for (uint32_t x = d; x < a.n && (rec_ctxgid(a.rec[x]) >> GID_NBITS) == c; ++x)
-			marks[x] = 1;
+		const uint64_t p = br_perm(c, a.w);
+		const uint64_t rem = p & ((1ull << rsh) - 1ull);
+		uint64_t line = p >> rsh;
+		for (;;) {
+			const unsigned long long *ln = a.brec + line * BR_SLOTS;
+			bool hit = false;
+			for (int j = 0; j < BR_SLOTS; ++j) {
+				const unsigned long long x = ln[j];
+				if (x < BR_HIGH && (x >> rs) == rem) {
+					marks[line * BR_SLOTS + j] = 1;
+					hit = true;
+				}
+			}
+			const unsigned long long last = ln[BR_SLOTS - 1];
+			if (hit || (last & BR_PAYLOAD) != BR_HIGH)
+				break;
+			line = last & ~BR_HIGH;
+		}
 	}
 }
 
-__global__ __launch_bounds__(256) void k_count_marks(const uint8_t *marks, uint64_t n, unsigned long long *count)
+__global__ __launch_bounds__(256) void k_count_marks(const uint8_t *marks, const unsigned long long *__restrict__ brec,
+						     unsigned ob, uint64_t slots, unsigned long long *count)
 {
-	uint32_t k = 0;
-	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-		k += marks[i];
+	unsigned long long k = 0;
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots; i += (uint64_t)gridDim.x * blockDim.x) {
+		if (!marks[i])
+			continue;
+		const unsigned long long x = brec[i];
+		k += ((x >> ob) & GID_MASK) == GID_MASK ? ((brec[i + 1] >> 32) & BR_MAX_LONG) : 1ull;
+	}
 	for (int off = 32; off > 0; off >>= 1)
 		k += __shfl_xor(k, off, 64);
 	if ((threadIdx.x & 63) == 0 && k)
-		atomicAdd(count, (unsigned long long)k);
+		atomicAdd(count, k);
 }
 
 // The pass gate over dense count rows (small blocks, counted by the
@@ -885,6 +1114,7 @@ void qj_layout(const kssdhip_index *ix, bool uniq, QJShape *sh)
 		sh->sbits = ilog2(sh->slots);
 		sh->lds = (size_t)sh->slots * (cell_bytes + 4u);
 	}
+	sh->lds = (sh->lds + 15) / 16 * 16 + (size_t)(sh->threads / 64) * (STAGE_WAVE + OWN_WAVE * 4);
 }
 
 int qj_shape(const kssdhip_index *ix, uint32_t nq, bool uniq, QJShape *sh)
@@ -969,11 +1199,11 @@ int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_
 
 void fill_index_args(const kssdhip_index *ix, QJArgs &q)
 {
-	q.rec = ix->d_qrec;
-	q.ctab = ix->d_ctab;
-	q.ck = ix->ck;
-	q.filt = ix->d_filt;
-	q.fk = ix->fk;
+	q.brec = ix->d_brec;
+	q.w = ix->br_w;
+	q.bb = ix->br_bits;
+	q.ctxgid = ix->d_ctxgid;
+	q.robj = ix->d_obj;
 	q.n = ix->n;
 	q.ctx_max = ix->ctx_max;
 	q.R = (uint32_t)ix->ref_gnum;
@@ -1002,9 +1232,14 @@ int qj_launch(const QJShape &sh, bool uniq, const QJArgs &a, hipStream_t s)
 __global__ void k_fill_empty(unsigned long long *p, uint64_t n)
 {
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-		p[i] = SLOT_EMPTY;
+		p[i] = BR_EMPTY;
 }
 
+// The record table of the index (see BR_SLOTS): contexts counted per home
+// line, their first records listed per home, each home laid out by one
+// thread (a count pass, a scan for the overflow lines, a write pass).  An
+// index of 2^20 genomes (gid GID_MASK in use) or with a context of 2^30
+// records or more gets no table; the query-major kernels then refuse it.
 int kssd_qjoin_build(kssdhip_index *ix)
 {
 	kssd_qjoin_free(ix);
@@ -1016,63 +1251,97 @@ int kssd_qjoin_build(kssdhip_index *ix)
 	auto grid = [](uint64_t items) {
 		return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + 255) / 256, 1u << 20)));
 	};
-	unsigned long long runs = 0, *d_runs = nullptr;
-	if (hipMalloc(&d_runs, 8) != hipSuccess)
-		return kssdhip_fail(KSSDHIP_ENOMEM, "context table: device allocation failed");
-	hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, d_runs, (uint64_t)1);
-	if (n)
-		hipLaunchKernelGGL(k_count_runs, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, 4096)), dim3(256), 0, s,
-				   ix->d_ctxgid, n, d_runs);
-	hipError_t e = hipMemcpyAsync(&runs, d_runs, 8, hipMemcpyDeviceToHost, s);
-	if (e == hipSuccess)
-		e = hipStreamSynchronize(s);
-	(void)hipFree(d_runs);
-	if (e != hipSuccess)
-		return kssdhip_fail(KSSDHIP_EHIP, "context table: %s", hipGetErrorString(e));
-	int k = 10; // at least twice as many slots as contexts
-	while ((1ull << k) < 2 * (uint64_t)runs)
-		++k;
-	ix->ck = k;
-	const uint64_t slots = 1ull << k;
-	int fk = 6; // at least 10 bits per context
-	while ((64ull << fk) < 10 * (uint64_t)runs)
-		++fk;
-	ix->fk = fk;
-	const uint64_t fwords = 1ull << fk;
-	if (hipMalloc(&ix->d_qrec, (n + 1) * sizeof(uint4)) != hipSuccess ||
-	    hipMalloc(&ix->d_ctab, slots * sizeof(unsigned long long)) != hipSuccess ||
-	    hipMalloc(&ix->d_filt, fwords * sizeof(unsigned long long)) != hipSuccess) {
+	BrGeom g;
+	g.ob = ix->obj_bits;
+	g.w = br_ctx_width(ix->obj_bits);
+	g.bb = BR_MIN_BITS; // about 6 slots per home line
+	while (g.bb < g.w && (6ull << g.bb) < n)
+		++g.bb;
+	if ((uint64_t)ix->ref_gnum > GID_MASK || g.bb > g.w)
+		return kssd_qjoin_prepare(ix);
+	g.homes = 1ull << g.bb;
+	// temporaries: contexts per home, list offsets, cursors (then overflow
+	// lines per home), the lists, scan sums, and three read-back words
+	uint32_t *cnt = nullptr, *off = nullptr, *cur = nullptr, *list = nullptr, *sums = nullptr;
+	unsigned long long *misc = nullptr;
+	const uint64_t nsum = (g.homes + KSSD_SCAN_BLOCK - 1) / KSSD_SCAN_BLOCK;
+	int rc = 0;
+	do {
+		if (hipMalloc(&cnt, g.homes * 4) != hipSuccess || hipMalloc(&off, g.homes * 4) != hipSuccess ||
+		    hipMalloc(&cur, g.homes * 4) != hipSuccess ||
+		    hipMalloc(&list, std::max<uint64_t>(n, 1) * 4) != hipSuccess ||
+		    hipMalloc(&sums, nsum * 4) != hipSuccess || hipMalloc(&misc, 3 * 8) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_ENOMEM, "record table: device allocation failed");
+			break;
+		}
+		hipLaunchKernelGGL(k_zero_u32, grid(g.homes), dim3(256), 0, s, cnt, (uint32_t)g.homes);
+		hipLaunchKernelGGL(k_zero_u32, grid(g.homes), dim3(256), 0, s, cur, (uint32_t)g.homes);
+		hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, misc, (uint64_t)3);
+		if (n) {
+			hipLaunchKernelGGL(k_br_longest, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, misc);
+			hipLaunchKernelGGL(k_br_count, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, g, cnt);
+		}
+		if (hipMemcpyAsync(off, cnt, g.homes * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_EHIP, "record table: copy failed");
+			break;
+		}
+		kssd_scan_u32_excl(off, g.homes, sums, s);
+		if (n)
+			hipLaunchKernelGGL(k_br_scatter, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, g, off, cur, list);
+		// overflow lines per home into cur, their total: last count + its offset
+		hipLaunchKernelGGL((k_br_place<false>), grid(g.homes), dim3(256), 0, s, ix->d_ctxgid, ix->d_obj, n, g, off,
+				   cnt, list, cur, nullptr, (uint64_t)0, nullptr);
+		if (hipMemcpyAsync(&misc[1], cur + (g.homes - 1), 4, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_EHIP, "record table: copy failed");
+			break;
+		}
+		kssd_scan_u32_excl(cur, g.homes, sums, s);
+		unsigned long long back[3] = {0, 0, 0};
+		if (hipMemcpyAsync(&misc[2], cur + (g.homes - 1), 4, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+		    hipMemcpyAsync(back, misc, sizeof back, hipMemcpyDeviceToHost, s) != hipSuccess ||
+		    hipStreamSynchronize(s) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_EHIP, "record table: %s", hipGetErrorString(hipGetLastError()));
+			break;
+		}
+		if (back[0] > BR_MAX_LONG)
+			break; // a context too long for the payload: no table
+		const uint64_t ovf = (back[1] & 0xffffffffull) + (back[2] & 0xffffffffull);
+		const uint64_t lines = g.homes + ovf;
+		if (hipMalloc(&ix->d_brec, lines * BR_SLOTS * 8) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_ENOMEM, "record table: device allocation of %llu lines failed",
+					  (unsigned long long)lines);
+			break;
+		}
+		hipLaunchKernelGGL(k_fill_empty, grid(lines * BR_SLOTS), dim3(256), 0, s, ix->d_brec, lines * BR_SLOTS);
+		hipLaunchKernelGGL((k_br_place<true>), grid(g.homes), dim3(256), 0, s, ix->d_ctxgid, ix->d_obj, n, g, off,
+				   cnt, list, nullptr, cur, g.homes, ix->d_brec);
+		const hipError_t e = hipGetLastError();
+		if (e != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+			rc = kssdhip_fail(KSSDHIP_EHIP, "record table: %s", hipGetErrorString(e));
+			break;
+		}
+		ix->br_lines = lines;
+		ix->br_ovf = ovf;
+		ix->br_w = g.w;
+		ix->br_bits = g.bb;
+	} while (0);
+	for (void *p : {(void *)cnt, (void *)off, (void *)cur, (void *)list, (void *)sums, (void *)misc})
+		if (p)
+			(void)hipFree(p);
+	if (rc) {
 		kssd_qjoin_free(ix);
-		return kssdhip_fail(KSSDHIP_ENOMEM, "query-major index: device allocation failed");
-	}
-	hipLaunchKernelGGL(k_fill_empty, grid(slots), dim3(256), 0, s, ix->d_ctab, slots);
-	hipLaunchKernelGGL(k_zero_u64, grid(fwords), dim3(256), 0, s, ix->d_filt, fwords);
-	if (n) {
-		hipLaunchKernelGGL(k_pack16, grid(n), dim3(256), 0, s, ix->d_ctxgid, ix->d_obj, n, ix->d_qrec);
-		hipLaunchKernelGGL(k_ctab_insert, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, k, ix->d_ctab);
-		hipLaunchKernelGGL(k_filt_insert, grid(n), dim3(256), 0, s, ix->d_ctxgid, n, fk, ix->d_filt);
-	}
-	e = hipGetLastError();
-	if (e == hipSuccess)
-		e = hipStreamSynchronize(s);
-	if (e != hipSuccess) {
-		kssd_qjoin_free(ix);
-		return kssdhip_fail(KSSDHIP_EHIP, "query-major index: %s", hipGetErrorString(e));
+		return rc;
 	}
 	return kssd_qjoin_prepare(ix);
 }
 
 void kssd_qjoin_free(kssdhip_index *ix)
 {
-	if (ix->d_qrec)
-		(void)hipFree(ix->d_qrec);
-	if (ix->d_ctab)
-		(void)hipFree(ix->d_ctab);
-	if (ix->d_filt)
-		(void)hipFree(ix->d_filt);
-	ix->d_qrec = nullptr;
-	ix->d_ctab = nullptr;
-	ix->d_filt = nullptr;
+	if (ix->d_brec)
+		(void)hipFree(ix->d_brec);
+	ix->d_brec = nullptr;
+	ix->br_lines = 0;
+	ix->br_ovf = 0;
 }
 
 int kssd_qjoin_prepare(kssdhip_index *ix)
@@ -1100,7 +1369,7 @@ bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total)
 	// index is so much larger than a query that most (query, partition)
 	// pairs are empty; blocks of fewer queries than compute units go to the
 	// one-thread-per-key lookup
-	if (!ix->d_qrec || nq < (ix->cus > 0 ? ix->cus : 256))
+	if (!ix->d_brec || nq < (ix->cus > 0 ? ix->cus : 256))
 		return false;
 	const double parts = ix->P ? (double)ix->P : 1.0;
 	return (double)total < 2.0 * parts * (double)nq;
@@ -1109,8 +1378,8 @@ bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total)
 int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevBuf *ws)
 {
 	const bool uniq = (ja.flags & KSSDHIP_UNIQUE) && !(ja.flags & KSSDHIP_LOWER);
-	if (!ix->d_qrec)
-		return kssdhip_fail(KSSDHIP_EINVAL, "qjoin: the index has no query-major tables");
+	if (!ix->d_brec)
+		return kssdhip_fail(KSSDHIP_EINVAL, "qjoin: the index has no record table");
 	QJShape sh;
 	int rc = qj_shape(ix, (uint32_t)ja.nq, uniq, &sh);
 	if (rc)
@@ -1170,8 +1439,8 @@ int select_check(const kssdhip_index *ix, const SelectCall &c)
 	if (c.nq > 0 && (!c.skidx || !c.qry_ctx || !c.ref_ctx || !c.q_first || !c.q_count || (c.cap && !c.rows) ||
 			 (c.total && !c.keys)))
 		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: null device pointer");
-	if (!ix->d_qrec)
-		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: the index has no query-major tables");
+	if (!ix->d_brec)
+		return kssdhip_fail(KSSDHIP_EINVAL, "select_block: the index has no record table");
 	return 0;
 }
 
@@ -1524,12 +1793,13 @@ extern "C" int kssdhip_index_matched_records(kssdhip_index *ix, const uint64_t *
 	*matched = 0;
 	if (!nq || !total || !ix->n)
 		return 0;
-	if (!ix->d_qrec)
-		return kssdhip_fail(KSSDHIP_EINVAL, "matched_records: the index has no query-major tables");
+	if (!ix->d_brec)
+		return kssdhip_fail(KSSDHIP_EINVAL, "matched_records: the index has no record table");
 	if (hipSetDevice(ix->device) != hipSuccess)
 		return kssdhip_fail(KSSDHIP_EHIP, "matched_records: device not available");
 	DevBuf marks;
-	const uint64_t words = (ix->n + 7) / 8 + 1; // the marks, then the count (8-byte aligned)
+	const uint64_t slots = ix->br_lines * BR_SLOTS;
+	const uint64_t words = (slots + 7) / 8 + 1; // a mark per table slot, then the count (8-byte aligned)
 	int rc = marks.reserve(words * 8);
 	if (rc)
 		return rc;
@@ -1545,8 +1815,8 @@ extern "C" int kssdhip_index_matched_records(kssdhip_index *ix, const uint64_t *
 			   (unsigned long long *)marks.ptr, words);
 	hipLaunchKernelGGL(k_mark_matched, dim3((unsigned)std::min<uint64_t>((total + 255) / 256, 1u << 16)), dim3(256),
 			   0, s, a, (uint8_t *)marks.ptr);
-	hipLaunchKernelGGL(k_count_marks, dim3((unsigned)std::min<uint64_t>((ix->n + 255) / 256, 4096)), dim3(256), 0, s,
-			   (const uint8_t *)marks.ptr, (uint64_t)ix->n, d_count);
+	hipLaunchKernelGGL(k_count_marks, dim3((unsigned)std::min<uint64_t>((slots + 255) / 256, 8192)), dim3(256), 0, s,
+			   (const uint8_t *)marks.ptr, (const unsigned long long *)ix->d_brec, ix->obj_bits, slots, d_count);
 	unsigned long long count = 0;
 	hipError_t e = hipGetLastError();
 	if (e == hipSuccess)
@@ -1557,5 +1827,17 @@ extern "C" int kssdhip_index_matched_records(kssdhip_index *ix, const uint64_t *
 	if (e != hipSuccess)
 		return kssdhip_fail(KSSDHIP_EHIP, "matched_records: %s", hipGetErrorString(e));
 	*matched = count;
+	return 0;
+}
+
+extern "C" int kssdhip_index_table_info(const kssdhip_index *ix, uint64_t *lines, uint64_t *overflow_lines,
+					int *home_bits, int *ctx_bits)
+{
+	if (!ix || !lines || !overflow_lines || !home_bits || !ctx_bits)
+		return kssdhip_fail(KSSDHIP_EINVAL, "index_table_info: bad arguments");
+	*lines = ix->d_brec ? ix->br_lines : 0;
+	*overflow_lines = ix->d_brec ? ix->br_ovf : 0;
+	*home_bits = ix->d_brec ? ix->br_bits : 0;
+	*ctx_bits = ix->d_brec ? ix->br_w : 0;
 	return 0;
 }

@@ -299,6 +299,8 @@ def load_kssdhip():
     lib.kssdhip_select_hits_dev.argtypes = [V, I, I, ctypes.c_double, U32, V, V, V]
     lib.kssdhip_index_matched_records.argtypes = [V, V, V, I, U64, ctypes.POINTER(U64)]
     lib.kssdhip_probe_random_reads.argtypes = [S, U64, ctypes.POINTER(D)]
+    lib.kssdhip_index_table_info.argtypes = [V, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(I),
+                                             ctypes.POINTER(I)]
     lib.kssdhip_index_autotune.argtypes = [V, V, V, I, U64, U32, ctypes.c_char_p, I, ctypes.POINTER(TuneResult)]
     lib.kssdhip_index_autotune_host.argtypes = [V, V, V, I, U32, ctypes.c_char_p, I, ctypes.POINTER(TuneResult)]
     lib.kssdhip_index_tiling_from_cache.argtypes = [V, I, U64, U32, ctypes.c_char_p, ctypes.POINTER(TuneResult)]
@@ -546,6 +548,13 @@ class KssdHip:
         r = D()
         self.check(self.lib.kssdhip_probe_random_reads(nbytes, reads, ctypes.byref(r)))
         return float(r.value)
+
+    def table_info(self, h):
+        """(lines, overflow lines, home bits, context bits) of the handle's record table."""
+        lines, ovf, hb, cb = U64(), U64(), I(), I()
+        self.check(self.lib.kssdhip_index_table_info(h, ctypes.byref(lines), ctypes.byref(ovf), ctypes.byref(hb),
+                                                     ctypes.byref(cb)))
+        return int(lines.value), int(ovf.value), int(hb.value), int(cb.value)
 
     def matched_records(self, h, d_keys, d_skidx, nq, total):
         m = U64()

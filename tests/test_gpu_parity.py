@@ -282,7 +282,7 @@ def test_tiling_validation(hip):
         for bad in (dict(mode=4), dict(mode=-1), dict(chunk_records=10), dict(chunk_records=20000),
                     dict(block_threads=100), dict(group_partitions=0), dict(group_partitions=65),
                     dict(chunk_records=8192, group_partitions=2), dict(query_threads=32),
-                    dict(query_threads=1100), dict(query_slots=300), dict(query_slots=128),
+                    dict(query_threads=576), dict(query_threads=1024), dict(query_slots=300), dict(query_slots=128),
                     dict(query_slots=16384)):
             with pytest.raises(RuntimeError):
                 hip.set_tiling(h, **bad)

@@ -344,3 +344,130 @@ def test_smaller_block_after_larger_on_one_handle(hip, oracle, wide_cases, selec
                 assert np.array_equal(got[0], ctx) and np.array_equal(got[1], obj), nq
     finally:
         hip.close_index(h)
+
+
+# ---------------------------------------------------------------- record table
+# Home line of a context in the index's record table: the top home_bits bits
+# of a bijection of its ctx_bits-bit value (xor-shifts and odd multipliers
+# modulo 2^w).  Inverted here to place chosen contexts in one home line.
+_M1, _M2 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F
+
+
+def _xs_inv(y, s, w):
+    x = y
+    for _ in range(w // s + 1):
+        x = y ^ (x >> s)
+    return x
+
+
+def _perm_inv(p, w):
+    m, s = (1 << w) - 1, (w + 1) // 2
+    x = _xs_inv(p, s, w)
+    x = (x * pow(_M2, -1, 1 << w)) & m
+    x = _xs_inv(x, s, w)
+    x = (x * pow(_M1, -1, 1 << w)) & m
+    return _xs_inv(x, s, w)
+
+
+def _perm(x, w):
+    m, s = (1 << w) - 1, (w + 1) // 2
+    x ^= x >> s
+    x = (x * _M1) & m
+    x ^= x >> s
+    x = (x * _M2) & m
+    return x ^ (x >> s)
+
+
+def test_perm_inverse():
+    rng = np.random.default_rng(5)
+    for w in (32, 36, 44):
+        for x in rng.integers(0, 1 << w, size=200, dtype=np.uint64):
+            assert _perm_inv(_perm(int(x), w), w) == int(x)
+
+
+def _crowded_case(seed, obj_bits, n_ref=48):
+    """References whose contexts crowd a few home lines: one home with ~40
+    contexts (runs of 1-12 records and long runs of 13-40, so the home spills
+    over a chain of overflow lines), one with exactly 16 slots (one full line,
+    no continuation), one with 17 (a continuation for one record), plus
+    scattered contexts; queries hold most of them (objects kept or changed)
+    and absent contexts of the crowded homes (looked up through the whole
+    chain), some with a second object."""
+    rng = np.random.default_rng(seed)
+    w = 64 - obj_bits if obj_bits >= 20 else 44
+    bb = 21
+    rb = w - bb
+    omask = (1 << obj_bits) - 1
+
+    def ctx_in(home, count):
+        rems = rng.choice(1 << min(rb, 20), size=count, replace=False)
+        return [_perm_inv((home << rb) | int(r), w) for r in rems]
+
+    runs = {}  # context -> run length (genomes holding it)
+    for c in ctx_in(12345, 40):
+        runs[c] = int(rng.integers(13, 41)) if rng.random() < 0.2 else int(rng.integers(1, 13))
+    for c, ln in zip(ctx_in(777, 4), (4, 4, 4, 4)):  # 16 slots: one line exactly
+        runs[c] = ln
+    for c, ln in zip(ctx_in(4242, 3), (6, 6, 5)):  # 17 slots: a continuation
+        runs[c] = ln
+    for c in rng.integers(1, 1 << w, size=300, dtype=np.uint64):
+        runs[int(c)] = int(rng.integers(1, 6))
+    per_ref = [[] for _ in range(n_ref)]
+    base = {}
+    for c, ln in runs.items():
+        base[c] = int(rng.integers(0, omask + 1))
+        for g in rng.choice(n_ref, size=min(ln, n_ref), replace=False):
+            o = base[c] ^ (int(rng.integers(0, 4)) << (2 * int(rng.integers(0, obj_bits // 2))))
+            per_ref[g].append((c << obj_bits) | (o & omask))
+            if rng.random() < 0.1:  # a conflict: a second object in this genome
+                per_ref[g].append((c << obj_bits) | ((o ^ 3) & omask))
+    refs = [np.unique(np.array(k, dtype=np.uint64)) for k in per_ref]
+    absent = ctx_in(12345, 30) + ctx_in(4242, 5)
+    absent = [c for c in absent if c not in runs]
+    ctxs = list(runs)
+    qrys = []
+    for _ in range(300):
+        pick = rng.choice(len(ctxs), size=int(rng.integers(0, 60)), replace=False)
+        ks = []
+        for i in pick:
+            c = ctxs[i]
+            o = base[c] if rng.random() < 0.5 else int(rng.integers(0, omask + 1))
+            ks.append((c << obj_bits) | o)
+            if rng.random() < 0.1:
+                ks.append((c << obj_bits) | (o ^ 1))
+        for c in rng.choice(absent, size=min(5, len(absent)), replace=False):
+            ks.append((int(c) << obj_bits) | int(rng.integers(0, omask + 1)))
+        qrys.append(np.unique(np.array(ks, dtype=np.uint64)))
+    qrys[3] = np.zeros(0, np.uint64)
+    return refs, qrys
+
+
+@pytest.mark.parametrize("obj_bits", [20, 28, 32])
+def test_record_table_overflow_and_long_contexts(hip, oracle, obj_bits):
+    """Home lines that overflow into chains of overflow lines, a home of
+    exactly one full line, long contexts (records walked in the sorted
+    index), absent contexts that walk a whole chain: query-major counts
+    equal the oracle's for every flag set, also below the diagonal, and the
+    table really has overflow lines."""
+    refs, qrys = _crowded_case(31 + obj_bits, obj_bits)
+    rk, ri = _block(refs)
+    qk, qi = _block(qrys)
+    R = len(refs)
+    rec = kt.build_index_np(rk, ri, obj_bits)
+    h = hip.open_index(rec, R, obj_bits)
+    try:
+        lines, ovf, hb, cb = hip.table_info(h)
+        assert (hb, cb) == (21, 64 - obj_bits) and ovf >= 3 and lines == (1 << 21) + ovf
+        hip.set_tiling(h, mode=3, query_threads=256)
+        for flags in (0, 2, 4, 6):
+            got = hip.count_block(h, R, qk, qi, flags=flags)
+            want = oracle.count_block(rec, R, obj_bits, qk, qi, flags=flags)
+            for name, g, w in zip(("ctx", "obj", "unique_ctx", "unique_obj"), got, want):
+                assert np.array_equal(g, w), (obj_bits, flags, name)
+        for off in (0, 20, R - 9):
+            got = hip.count_block(h, R, rk[ri[off]:ri[off + 9]], ri[off:off + 10], qry_gid_offset=off, flags=1)
+            want = oracle.count_block(rec, R, obj_bits, rk[ri[off]:ri[off + 9]], ri[off:off + 10],
+                                      qry_gid_offset=off, flags=1)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), off
+    finally:
+        hip.close_index(h)
