@@ -41,12 +41,12 @@ constexpr uint32_t EMPTY = 0xffffffffu;
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr int MAX_PROBES = 32;
 constexpr uint32_t DIRECT_LDS_MAX = 64u * 1024u; // direct per-genome rows up to this many LDS bytes
-constexpr int DEFAULT_THREADS = 512;
+constexpr int DEFAULT_THREADS = 256;
 constexpr int QJ_MAX_THREADS = 512; // workgroup size bound of k_qjoin
 #ifndef QJ_WAVES_PER_SIMD
 #define QJ_WAVES_PER_SIMD 4 // register budget of k_qjoin: at most 128 VGPRs, four waves per SIMD
 #endif
-constexpr uint32_t DEFAULT_SLOTS = 512;
+constexpr uint32_t DEFAULT_SLOTS = 2048;
 // workspace: [0, WS_HEAD) work counter and the order's histogram and
 // cursors (65 words), then the spill rows, then the work order (nq words,
 // WS_ALIGN-rounded)
@@ -74,10 +74,10 @@ enum { QJ_DENSE = 0, QJ_ACCUM = 1, QJ_SELECT = 2 };
 // records are in index order ((gid, obj) ascending).  bb >= 21 keeps every
 // record and header below 2^63 for object widths up to 32 bits (remainder
 // + gid + obj = 84 - bb bits), and gid GID_MASK is reserved for headers, so
-// an index of 2^20 genomes has no table.  Home lines are sized for about 6
-// slots each, so few homes need an overflow line: a context absent from
-// the index costs one line, a present one the same line (plus its run in
-// the sorted index when long).
+// an index of 2^20 genomes has no table.  Home lines are sized for about 3
+// slots each, so few homes need an overflow line (0.1 % of the lines at
+// 171.7M records): a context absent from the index costs one line, a
+// present one the same line (plus its run in the sorted index when long).
 constexpr int BR_SLOTS = 16;
 constexpr uint32_t BR_LMAX = 12;
 constexpr int BR_MIN_BITS = 21;
@@ -1254,8 +1254,8 @@ int kssd_qjoin_build(kssdhip_index *ix)
 	BrGeom g;
 	g.ob = ix->obj_bits;
 	g.w = br_ctx_width(ix->obj_bits);
-	g.bb = BR_MIN_BITS; // about 6 slots per home line
-	while (g.bb < g.w && (6ull << g.bb) < n)
+	g.bb = BR_MIN_BITS; // about 3 slots per home line (few homes need an overflow line)
+	while (g.bb < g.w && (3ull << g.bb) < n)
 		++g.bb;
 	if ((uint64_t)ix->ref_gnum > GID_MASK || g.bb > g.w)
 		return kssd_qjoin_prepare(ix);
