@@ -717,12 +717,8 @@ def index_bench(args, host, threads, plain_s_per_pair, shared):
             hip.count_block_dev(h, d_k, d_i, n, total, 0, flags, d_c, d_o, d_uc, d_uo, stream)
             hip.epilogue_dev(d_c, d_o, n, nref, d_qc, d_rc, opts, d_out, stream)
 
-        if unique:
-            hip.set_tiling(h, mode=0)  # auto: lookup or partitioned join by block size
-            tiling = {"mode": 0}
-        else:
-            tiling, _ = tune.tune(hip, rt, h, d_k, d_i, n, total, nref, len(rec), d_c, d_o, stream,
-                                  flags=0, force=args.retune, log=log)
+        tiling, _ = tune.tune(hip, rt, h, d_k, d_i, n, total, nref, len(rec), d_c, d_o, stream,
+                              flags=flags, force=args.retune, log=log)
         step()
         rt.sync(stream)
         e0, e1 = rt.event(), rt.event()

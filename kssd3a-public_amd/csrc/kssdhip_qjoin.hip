@@ -41,12 +41,14 @@ constexpr uint32_t EMPTY = 0xffffffffu;
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr int MAX_PROBES = 32;
 constexpr uint32_t DIRECT_LDS_MAX = 64u * 1024u; // direct per-genome rows up to this many LDS bytes
-constexpr int DEFAULT_THREADS = 256;
+constexpr int DEFAULT_THREADS = 512;
 constexpr int QJ_MAX_THREADS = 512; // workgroup size bound of k_qjoin
 #ifndef QJ_WAVES_PER_SIMD
 #define QJ_WAVES_PER_SIMD 4 // register budget of k_qjoin: at most 128 VGPRs, four waves per SIMD
 #endif
-constexpr uint32_t DEFAULT_SLOTS = 2048;
+constexpr uint32_t DEFAULT_SLOTS = 512;
+constexpr uint64_t QJ_MIN_CHUNK = 1024; // keys per work item when queries are cut into key ranges
+constexpr uint64_t QJ_AUTO_MIN_KEYS = 1u << 20; // blocks of few queries take the query-major join from this many keys
 // workspace: [0, WS_HEAD) work counter and the order's histogram and
 // cursors (65 words), then the spill rows, then the work order (nq words,
 // WS_ALIGN-rounded)
@@ -313,6 +315,13 @@ __global__ __launch_bounds__(256) void k_order_scatter(const uint64_t *__restric
 	}
 }
 
+// items of `chunk` keys per query (a query without keys has none)
+__global__ void k_item_counts(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t chunk, uint32_t *cnt)
+{
+	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q <= nq; q += gridDim.x * blockDim.x)
+		cnt[q] = q < nq ? (uint32_t)((skidx[q + 1] - skidx[q] + chunk - 1) / chunk) : 0u;
+}
+
 __global__ void k_zero_u32(uint32_t *p, uint32_t n)
 {
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -348,6 +357,11 @@ struct QJArgs {
 	// work queue and spill rows (zero at rest: a workgroup zeroes what it spilled)
 	uint32_t *next;
 	const uint32_t *order; // queries in processing order
+	// key-range work items (chunk > 0): query q's keys in ranges of `chunk`,
+	// items qitems[q] .. qitems[q+1]-1, qitems[nq] in all; the outputs are
+	// added to (each item flushes its table), so they start zeroed
+	uint32_t chunk;
+	const uint32_t *qitems;
 	unsigned long long *spill; // gridDim.x rows of R cells x (2 | 4) u64
 	int mode;
 	// dense outputs (QJ_DENSE, QJ_ACCUM)
@@ -607,11 +621,16 @@ __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, c
 // for its key's context and walks the found contexts together; a home that
 // continues in an overflow line has that line loaded into its row and is
 // scanned again, until every lookup is resolved.
-template <bool UNIQ, bool DIRECT>
-__device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned long long *spill_row,
-			 LDS uint32_t *spilled, LDS unsigned long long *stage)
+template <bool UNIQ, bool DIRECT, bool ITEMS>
+__device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t k0, uint64_t k1,
+			 unsigned long long *spill_row, LDS uint32_t *spilled, LDS unsigned long long *stage)
 {
+	// the query's keys [kb, ke); groups headed in [k0, k1) (ITEMS) or all
 	const uint64_t kb = a.skidx[q], ke = a.skidx[q + 1];
+	if (!ITEMS) {
+		k0 = kb;
+		k1 = ke;
+	}
 	const unsigned ob = a.obj_bits;
 	const int qgid = a.qry_gid_offset + (int)q;
 	const uint64_t B = blockDim.x; // a multiple of 64
@@ -622,12 +641,12 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned 
 	const int rs = GID_NBITS + (int)ob;
 	const int rsh = a.w - a.bb;
 	const uint64_t remmask = (1ull << rsh) - 1ull;
-	uint64_t wb = kb + (threadIdx.x & ~63u); // wave-uniform
-	uint64_t key = wb + lane < ke ? a.keys[wb + lane] : 0ull;
-	uint64_t before = lane == 0 && wb > kb && wb < ke ? a.keys[wb - 1] : ~0ull; // the key ahead of the wave's first
-	for (; wb < ke; wb += B) {
+	uint64_t wb = k0 + (threadIdx.x & ~63u); // wave-uniform
+	uint64_t key = wb + lane < k1 ? a.keys[wb + lane] : 0ull;
+	uint64_t before = lane == 0 && wb > kb && wb < k1 ? a.keys[wb - 1] : ~0ull; // the key ahead of the wave's first
+	for (; wb < k1; wb += B) {
 		const uint64_t i = wb + lane;
-		const bool in = i < ke;
+		const bool in = i < k1;
 		const uint64_t c = key >> ob;
 		uint64_t prev = __shfl_up(key, 1u);
 		if (lane == 0)
@@ -644,8 +663,8 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, unsigned 
 		}
 		// the next keys, while the lines are in flight
 		const uint64_t nb = wb + B;
-		const uint64_t nkey = nb + lane < ke ? a.keys[nb + lane] : 0ull;
-		const uint64_t nbefore = lane == 0 && nb < ke ? a.keys[nb - 1] : ~0ull;
+		const uint64_t nkey = nb + lane < k1 ? a.keys[nb + lane] : 0ull;
+		const uint64_t nbefore = lane == 0 && nb < k1 ? a.keys[nb - 1] : ~0ull;
 #pragma unroll
 		for (uint32_t k = 0; k < 8; ++k)
 			*(LDS u64x2 *)(wst + (8 * k + g) * BR_SLOTS + ((piece ^ g) << 1)) = v[k];
@@ -870,11 +889,11 @@ __device__ void qj_output(const QJArgs &a, const QTable &t, uint32_t q, bool spi
 // Persistent workgroups: each takes the next query of the block from the
 // work counter until none is left (the reference's guided schedule over
 // queries, command_ani.h:481).
-template <bool UNIQ, bool DIRECT>
+template <bool UNIQ, bool DIRECT, bool ITEMS>
 __global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(QJ_WAVES_PER_SIMD))) void k_qjoin(QJArgs a)
 {
 	extern __shared__ unsigned long long qj_lds[];
-	__shared__ uint32_t s_q, s_spilled, s_n, s_base;
+	__shared__ uint32_t s_q, s_item, s_spilled, s_n, s_base;
 	const uint32_t ncell = DIRECT ? a.R : a.slots;
 	QTable t;
 	t.c = (LDS unsigned long long *)qj_lds;
@@ -897,16 +916,35 @@ __global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(
 	for (;;) {
 		if (threadIdx.x == 0) {
 			const uint32_t k = atomicAdd(a.next, 1u);
-#ifndef QJ_NO_ORDER
-			s_q = k < a.nq ? a.order[k] : a.nq;
-#else
-			s_q = k;
-#endif
+			if (ITEMS) { // item k: the query whose items include k, and the key range
+				if (k < a.qitems[a.nq]) {
+					uint32_t lo = 0, hi = a.nq; // largest q with qitems[q] <= k
+					while (hi - lo > 1) {
+						const uint32_t mid = (lo + hi) >> 1;
+						if (a.qitems[mid] <= k)
+							lo = mid;
+						else
+							hi = mid;
+					}
+					s_q = lo;
+					s_item = k - a.qitems[lo];
+				} else {
+					s_q = a.nq;
+				}
+			} else {
+				s_q = k < a.nq ? a.order[k] : a.nq;
+			}
 		}
 		__syncthreads(); // also: every thread is done with the previous query's table
 		const uint32_t q = s_q;
 		if (q >= a.nq)
 			break;
+		uint64_t k0 = 0, k1 = 0;
+		if (ITEMS) {
+			const uint64_t kb = a.skidx[q], ke = a.skidx[q + 1];
+			k0 = kb + (uint64_t)s_item * a.chunk;
+			k1 = min(ke, k0 + a.chunk);
+		}
 		for (uint32_t k = threadIdx.x; k < ncell; k += blockDim.x) {
 			t.c[k] = 0;
 			t.o[k] = 0;
@@ -922,7 +960,7 @@ __global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(
 			s_n = 0;
 		}
 		__syncthreads();
-		qj_count<UNIQ, DIRECT>(a, t, q, spill_row, (LDS uint32_t *)&s_spilled, stage);
+		qj_count<UNIQ, DIRECT, ITEMS>(a, t, q, k0, k1, spill_row, (LDS uint32_t *)&s_spilled, stage);
 		__syncthreads();
 		const bool spilled = s_spilled != 0;
 		qj_output<UNIQ, DIRECT>(a, t, q, spilled, spill_row, (LDS uint32_t *)&s_n, (LDS uint32_t *)&s_base);
@@ -1087,16 +1125,19 @@ int ilog2(uint32_t v)
 	return b;
 }
 
-template <bool UNIQ, bool DIRECT>
+template <bool UNIQ, bool DIRECT, bool ITEMS>
 const void *qj_kernel()
 {
-	return (const void *)k_qjoin<UNIQ, DIRECT>;
+	return (const void *)k_qjoin<UNIQ, DIRECT, ITEMS>;
 }
 
-const void *qj_fn(bool uniq, bool direct)
+const void *qj_fn(bool uniq, bool direct, bool items)
 {
-	return uniq ? (direct ? qj_kernel<true, true>() : qj_kernel<true, false>())
-		    : (direct ? qj_kernel<false, true>() : qj_kernel<false, false>());
+	if (items)
+		return uniq ? (direct ? qj_kernel<true, true, true>() : qj_kernel<true, false, true>())
+			    : (direct ? qj_kernel<false, true, true>() : qj_kernel<false, false, true>());
+	return uniq ? (direct ? qj_kernel<true, true, false>() : qj_kernel<true, false, false>())
+		    : (direct ? qj_kernel<false, true, false>() : qj_kernel<false, false, false>());
 }
 
 void qj_layout(const kssdhip_index *ix, bool uniq, QJShape *sh)
@@ -1145,6 +1186,8 @@ struct QJWork {
 	uint32_t *head;
 	uint32_t *order;
 	unsigned long long *spill;
+	uint32_t *qitems; // key-range items per query (nq + 1 words), then scan scratch
+	uint32_t *sums;
 };
 
 size_t qj_order_bytes(uint32_t nq)
@@ -1173,7 +1216,8 @@ void qj_order(const uint64_t *d_skidx, uint32_t nq, hipStream_t s, const QJWork 
 int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_t nq, hipStream_t s, QJWork *w)
 {
 	const size_t spill = std::max(ws->zero_bytes, sh.spill_bytes);
-	const size_t want = WS_HEAD + spill + qj_order_bytes(nq);
+	const uint32_t nsum = (uint32_t)((nq + 1 + KSSD_SCAN_BLOCK - 1) / KSSD_SCAN_BLOCK);
+	const size_t want = WS_HEAD + spill + qj_order_bytes(nq) + qj_order_bytes(nq + 1) + qj_order_bytes(nsum);
 	if (want > ws->bytes) {
 		int rc = kssd_reserve_for_stream(ws, want, s, "query-major workspace");
 		if (rc)
@@ -1193,6 +1237,8 @@ int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_
 	w->head = (uint32_t *)ws->ptr;
 	w->spill = (unsigned long long *)((char *)ws->ptr + WS_HEAD);
 	w->order = (uint32_t *)((char *)ws->ptr + WS_HEAD + spill);
+	w->qitems = (uint32_t *)((char *)w->order + qj_order_bytes(nq));
+	w->sums = (uint32_t *)((char *)w->qitems + qj_order_bytes(nq + 1));
 	qj_order(d_skidx, nq, s, *w);
 	return 0;
 }
@@ -1213,14 +1259,24 @@ void fill_index_args(const kssdhip_index *ix, QJArgs &q)
 int qj_launch(const QJShape &sh, bool uniq, const QJArgs &a, hipStream_t s)
 {
 	const dim3 grid(sh.grid), block(sh.threads);
-	if (uniq && sh.direct)
-		hipLaunchKernelGGL((k_qjoin<true, true>), grid, block, sh.lds, s, a);
+	const bool items = a.chunk != 0;
+	if (items) {
+		if (uniq && sh.direct)
+			hipLaunchKernelGGL((k_qjoin<true, true, true>), grid, block, sh.lds, s, a);
+		else if (uniq)
+			hipLaunchKernelGGL((k_qjoin<true, false, true>), grid, block, sh.lds, s, a);
+		else if (sh.direct)
+			hipLaunchKernelGGL((k_qjoin<false, true, true>), grid, block, sh.lds, s, a);
+		else
+			hipLaunchKernelGGL((k_qjoin<false, false, true>), grid, block, sh.lds, s, a);
+	} else if (uniq && sh.direct)
+		hipLaunchKernelGGL((k_qjoin<true, true, false>), grid, block, sh.lds, s, a);
 	else if (uniq)
-		hipLaunchKernelGGL((k_qjoin<true, false>), grid, block, sh.lds, s, a);
+		hipLaunchKernelGGL((k_qjoin<true, false, false>), grid, block, sh.lds, s, a);
 	else if (sh.direct)
-		hipLaunchKernelGGL((k_qjoin<false, true>), grid, block, sh.lds, s, a);
+		hipLaunchKernelGGL((k_qjoin<false, true, false>), grid, block, sh.lds, s, a);
 	else
-		hipLaunchKernelGGL((k_qjoin<false, false>), grid, block, sh.lds, s, a);
+		hipLaunchKernelGGL((k_qjoin<false, false, false>), grid, block, sh.lds, s, a);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess)
 		return kssdhip_fail(KSSDHIP_EHIP, "qjoin launch: %s", hipGetErrorString(e));
@@ -1349,10 +1405,12 @@ int kssd_qjoin_prepare(kssdhip_index *ix)
 	for (int u = 0; u < 2; ++u) {
 		QJShape sh;
 		qj_layout(ix, u != 0, &sh);
-		const void *fn = qj_fn(u != 0, sh.direct);
+		const void *fn = qj_fn(u != 0, sh.direct, false);
+		const void *fn_items = qj_fn(u != 0, sh.direct, true);
 		ix->qj_per_cu[u] = 0;
 		if (sh.lds > 65536 &&
-		    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds) != hipSuccess)
+		    (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds) != hipSuccess ||
+		     hipFuncSetAttribute(fn_items, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds) != hipSuccess))
 			continue; // does not fit: launches with this tiling fail with EINVAL
 		int per_cu = 0;
 		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sh.threads, sh.lds) == hipSuccess)
@@ -1369,10 +1427,12 @@ bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total)
 	// index is so much larger than a query that most (query, partition)
 	// pairs are empty; blocks of fewer queries than compute units go to the
 	// one-thread-per-key lookup
-	if (!ix->d_brec || nq < (ix->cus > 0 ? ix->cus : 256))
+	if (!ix->d_brec || nq < 1)
 		return false;
 	const double parts = ix->P ? (double)ix->P : 1.0;
-	return (double)total < 2.0 * parts * (double)nq;
+	if ((double)total >= 2.0 * parts * (double)nq)
+		return false;
+	return nq >= (ix->cus > 0 ? ix->cus : 256) || total >= QJ_AUTO_MIN_KEYS;
 }
 
 int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevBuf *ws)
@@ -1384,6 +1444,18 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 	int rc = qj_shape(ix, (uint32_t)ja.nq, uniq, &sh);
 	if (rc)
 		return rc;
+	// fewer queries than twice the resident workgroups: the queries' keys
+	// are cut into ranges of `chunk` keys, enough items for every
+	// workgroup; each item adds its table into the rows
+	const uint64_t full = (uint64_t)ix->qj_per_cu[uniq ? 1 : 0] * (uint64_t)(ix->cus > 0 ? ix->cus : 256);
+	uint32_t chunk = 0;
+	if ((uint64_t)ja.nq < 2 * full && ja.total > 0) {
+		uint64_t c = (ja.total + 4 * full - 1) / (4 * full);
+		c = std::max<uint64_t>(QJ_MIN_CHUNK, (c + 63) / 64 * 64);
+		chunk = (uint32_t)std::min<uint64_t>(c, 1u << 30);
+		sh.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(full, (uint64_t)ja.nq + ja.total / chunk));
+		sh.spill_bytes = sh.direct ? 0 : (size_t)sh.grid * ix->ref_gnum * (uniq ? 32u : 16u);
+	}
 	QJWork w;
 	if ((rc = qj_workspace(ws, sh, ja.skidx, (uint32_t)ja.nq, s, &w)))
 		return rc;
@@ -1405,6 +1477,24 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 	a.objc = ja.objc;
 	a.uctx = uniq ? ja.uctx : nullptr;
 	a.uobj = uniq ? ja.uobj : nullptr;
+	if (chunk) {
+		const uint32_t n1 = (uint32_t)ja.nq + 1;
+		hipLaunchKernelGGL(k_item_counts, dim3(std::min<uint32_t>((n1 + 255) / 256, 1024u)), dim3(256), 0, s, ja.skidx,
+				   (uint32_t)ja.nq, chunk, w.qitems);
+		kssd_scan_u32_excl(w.qitems, n1, w.sums, s);
+		if (a.mode == QJ_DENSE) {
+			const size_t plane = (size_t)ja.nq * (size_t)ix->ref_gnum * 8;
+			qj_zero(a.ctx, plane, s);
+			qj_zero(a.objc, plane, s);
+			if (uniq) {
+				qj_zero(a.uctx, plane, s);
+				qj_zero(a.uobj, plane, s);
+			}
+		}
+		a.mode = QJ_ACCUM;
+		a.chunk = chunk;
+		a.qitems = w.qitems;
+	}
 	return qj_launch(sh, uniq, a, s);
 }
 

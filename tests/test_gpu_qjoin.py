@@ -471,3 +471,56 @@ def test_record_table_overflow_and_long_contexts(hip, oracle, obj_bits):
             assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), off
     finally:
         hip.close_index(h)
+
+
+@pytest.mark.parametrize("flags", [0, 2, 4, 6, 8 | 2])
+def test_query_major_key_range_items(hip, oracle, flags):
+    """Blocks of few large queries are cut into key-range items of 1,024 keys
+    that add into the rows: 6 queries of about 2,000-9,000 keys whose context
+    groups (several objects per context) straddle the item boundaries,
+    against the oracle for every flag set; with ACCUMULATE the rows add to
+    what they held."""
+    rng = np.random.default_rng(77)
+    pool = rng.integers(1, 1 << 43, size=40000, dtype=np.uint64)
+    base = rng.integers(0, 1 << OBJ_BITS, size=len(pool), dtype=np.uint64)
+    refs = fast_samples(rng, 300, pool, base, 600, conflict_p=0.3)
+    qrys = fast_samples(rng, 6, pool, base, 2500, conflict_p=0.5)
+    qrys[2] = np.zeros(0, np.uint64)
+    rk, ri = _block(refs)
+    qk, qi = _block(qrys)
+    R = len(refs)
+    rec = kt.build_index_np(rk, ri, OBJ_BITS)
+    assert max(np.diff(qi)) > 4 * 1024
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=3)
+        want = oracle.count_block(rec, R, OBJ_BITS, qk, qi, flags=flags & ~8)
+        got = hip.count_block(h, R, qk, qi, flags=flags & ~8)
+        for name, g, w in zip(("ctx", "obj", "unique_ctx", "unique_obj"), got, want):
+            assert np.array_equal(g, w), (flags, name)
+        if flags & 8:  # twice into the same device rows
+            rt = kt.Hip(0)
+            s = rt.stream()
+            d_k, d_i = rt.upload(qk), rt.upload(qi)
+            cells = (len(qi) - 1) * R
+            bufs = [rt.malloc(8 * cells) for _ in range(4)]
+            try:
+                for b in bufs:
+                    rt.check(rt.lib.hipMemsetAsync(b, 0, 8 * cells, s))
+                for _ in range(2):
+                    hip.count_block_dev(h, d_k, d_i, len(qi) - 1, int(qi[-1]), 0, flags, *bufs, stream=s)
+                rt.sync(s)
+                for name, b, w in zip(("ctx", "obj", "unique_ctx", "unique_obj"), bufs, want):
+                    assert np.array_equal(rt.download(b, np.uint32, 2 * cells).reshape(w.shape), 2 * w), name
+            finally:
+                for b in bufs:
+                    rt.free(b)
+                rt.free(d_k)
+                rt.free(d_i)
+        for off in (0, R - 40):  # below the diagonal
+            sub_k, sub_i = rk[ri[off]:ri[off + 40]], ri[off:off + 41]
+            got = hip.count_block(h, R, sub_k, sub_i, qry_gid_offset=off, flags=1)
+            want = oracle.count_block(rec, R, OBJ_BITS, sub_k, sub_i, qry_gid_offset=off, flags=1)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), off
+    finally:
+        hip.close_index(h)
