@@ -500,6 +500,27 @@ def c3_bench(args, host, threads):
                                  force=args.retune, log=log)
     rt.free(d_tc)
     rt.free(d_to)
+    if tiling["mode"] == 3 and nref * 16 > 64 * 1024:
+        # the select spills what a query's LDS table cannot hold: its table size
+        # timed on the select itself (the first 10,000 queries)
+        ns = min(10000, nq)
+        best = None
+        for qs in (256, 512, 1024):
+            hip.set_tiling(h, query_slots=qs)
+            for rep in range(4):
+                if rep == 1:
+                    e0, e1 = rt.event(), rt.event()
+                    rt.record(e0, stream)
+                hip.select_block_dev(h, d_k, d_i, ns, int(qi[ns]), 0, d_qc, d_rc, sopts, d_rows, cap, d_n, d_first,
+                                     d_cnt, stream)
+            rt.record(e1, stream)
+            rt.sync(stream)
+            ms_qs = rt.elapsed_ms(e0, e1) / 3
+            if best is None or ms_qs < best[1]:
+                best = (qs, ms_qs)
+        hip.set_tiling(h, query_slots=best[0])
+        tiling["query_slots"] = best[0]
+        tune_sum["select_query_slots_ms"] = best[1]
     log(f"C3 tiling {tiling} ({time.time() - t:.1f}s)")
 
     def step():
