@@ -634,6 +634,18 @@ def c3_bench(args, host, threads):
             "dense_output_bytes": dense_algo, "frac_dense_output_model": dense_algo / t_launch / 1e9 / HBM_PEAK_GBS,
             "P_touched_first_block": p_touched_block, "P_touched_block_queries": nb,
             "random_access": random_access}
+    # LDS use of the launch (rocprofv3 counters of the committed profile)
+    lds_prof, lds_src = measured_profile("qjoin_c3_10k_x_100k_lds")
+    lds_peak = 256 * 128 * 2.4e9 / 1e9  # GB/s: 256 CUs x 128 B/clk x 2.4 GHz
+    lds = {"what": "LDS bytes of one select launch (staged home lines written and scanned, walk reads, lane "
+                   "shuffles, counter atomics) over its HIP-event time, against the LDS peak",
+           "bytes_per_launch": lds_prof.get("lds_bytes_per_dispatch") if lds_prof else None,
+           "achieved_GBs": lds_prof["lds_bytes_per_dispatch"] / t_launch / 1e9 if lds_prof else None,
+           "peak_GBs": lds_peak,
+           "frac": lds_prof["lds_bytes_per_dispatch"] / t_launch / 1e9 / lds_peak if lds_prof else None,
+           "lds_util_pct": lds_prof.get("lds_util_pct") if lds_prof else None,
+           "bank_conflict": lds_prof.get("lds_bank_conflict") if lds_prof else None,
+           "source": lds_src}
     # CPU baseline (the oracle's block count with per-block fenceposts and
     # guided OpenMP, then its restatement of the gate) on the first block
     sys.path.insert(0, str(ROOT / "tests"))
@@ -672,7 +684,7 @@ def c3_bench(args, host, threads):
         "parity": {"cpu_sample_queries": ncpu, "passing_pairs_bit_exact": exact, "properties": props},
         "forms": forms,
         "tiling": tiling, "tune": tune_sum,
-        "roofline": roof,
+        "roofline": roof, "lds": lds,
         "cpu_baseline": {"value": cpu_pps, "unit": "pairs/s", "cores": threads, "kind": "port",
                          "sample": f"first {ncpu} queries x {nref} refs: oracle block count (fenceposts per block, "
                                    "guided OpenMP) + its restatement of the pass gate (command_ani.h:577-621)"},
@@ -1274,8 +1286,8 @@ def main():
                    "pairs": c3["pairs"], "queries": c3["queries"], "refs": c3["refs"],
                    "query_entries": c3["query_entries"], "index_records": c3["index_records"],
                    "output": "passing (query, ref) pairs: counts + ranking ANI + FP64 CtxMoE ANI"},
-        "roofline": c3["roofline"], "cpu_baseline": c3["cpu_baseline"], "gpu_over_cpu": c3["value"] / c3[
-            "cpu_baseline"]["value"],
+        "roofline": c3["roofline"], "lds": c3["lds"], "cpu_baseline": c3["cpu_baseline"],
+        "gpu_over_cpu": c3["value"] / c3["cpu_baseline"]["value"],
         "c3": c3,
     }
     if args.c3_only:
