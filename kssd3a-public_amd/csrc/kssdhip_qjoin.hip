@@ -93,18 +93,13 @@ __host__ __device__ __forceinline__ int br_ctx_width(unsigned obj_bits)
 	return obj_bits >= 20 ? 64 - (int)obj_bits : 44;
 }
 
-// A bijection of w-bit values (1 <= w <= 44): xor-shifts and odd
-// multipliers modulo 2^w, each invertible.
+// A bijection of w-bit values (1 <= w <= 44): an odd multiplier modulo
+// 2^w, then an xor-shift that folds the well-mixed high bits into the low
+// ones; each step is invertible.
 __host__ __device__ __forceinline__ uint64_t br_perm(uint64_t x, int w)
 {
-	const uint64_t m = (1ull << w) - 1ull;
-	const int s = (w + 1) / 2;
-	x ^= x >> s;
-	x = (x * 0x9e3779b97f4a7c15ull) & m;
-	x ^= x >> s;
-	x = (x * 0xc2b2ae3d27d4eb4full) & m;
-	x ^= x >> s;
-	return x;
+	x = (x * 0x9e3779b97f4a7c15ull) & ((1ull << w) - 1ull);
+	return x ^ (x >> ((w + 1) / 2));
 }
 
 __device__ __forceinline__ bool run_start(const uint64_t *__restrict__ ctxgid, uint64_t i)
@@ -343,6 +338,7 @@ struct QJArgs {
 	const uint32_t *robj;
 	uint64_t n;
 	uint64_t ctx_max;
+	uint64_t cmax; // min(ctx_max, 2^w - 1): contexts above it are in no index record
 	uint32_t R; // ref_gnum
 	unsigned obj_bits;
 	// block
@@ -651,15 +647,17 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 		uint64_t prev = __shfl_up(key, 1u);
 		if (lane == 0)
 			prev = before;
-		const bool on = in && c <= a.ctx_max && (c >> a.w) == 0 && (i == kb || (prev >> ob) != c);
+		// cmax: the largest context the table can hold (index contexts <= ctx_max, below 2^w)
+		const bool on = in && c <= a.cmax && (i == kb || (prev >> ob) != c);
 		const uint64_t p = br_perm(on ? c : 0ull, a.w);
 		const uint32_t rem = (uint32_t)(p & remmask);
-		const uint64_t line = p >> rsh;
+		const uint32_t line = (uint32_t)(p >> rsh); // tables hold fewer than 2^32 lines
+		const u64x2 *lines = (const u64x2 *)a.brec + piece;
 		u64x2 v[8];
 #pragma unroll
 		for (uint32_t k = 0; k < 8; ++k) { // rows 8k .. 8k+7, eight lanes per row
-			const uint64_t lm = __shfl(line, (int)(8 * k + g));
-			v[k] = ((const u64x2 *)(a.brec + lm * BR_SLOTS))[piece];
+			const uint32_t lm = __shfl(line, (int)(8 * k + g));
+			v[k] = lines[(uint64_t)lm * (BR_SLOTS / 2)];
 		}
 		// the next keys, while the lines are in flight
 		const uint64_t nb = wb + B;
@@ -1252,6 +1250,7 @@ void fill_index_args(const kssdhip_index *ix, QJArgs &q)
 	q.robj = ix->d_obj;
 	q.n = ix->n;
 	q.ctx_max = ix->ctx_max;
+	q.cmax = ix->br_w ? std::min<uint64_t>(ix->ctx_max, (1ull << ix->br_w) - 1ull) : 0ull;
 	q.R = (uint32_t)ix->ref_gnum;
 	q.obj_bits = ix->obj_bits;
 }

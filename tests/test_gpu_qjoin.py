@@ -348,9 +348,10 @@ def test_smaller_block_after_larger_on_one_handle(hip, oracle, wide_cases, selec
 
 # ---------------------------------------------------------------- record table
 # Home line of a context in the index's record table: the top home_bits bits
-# of a bijection of its ctx_bits-bit value (xor-shifts and odd multipliers
-# modulo 2^w).  Inverted here to place chosen contexts in one home line.
-_M1, _M2 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F
+# of a bijection of its ctx_bits-bit value (an odd multiplier modulo 2^w,
+# then an xor-shift).  Inverted here to place chosen contexts in one home
+# line.
+_M1 = 0x9E3779B97F4A7C15
 
 
 def _xs_inv(y, s, w):
@@ -363,18 +364,12 @@ def _xs_inv(y, s, w):
 def _perm_inv(p, w):
     m, s = (1 << w) - 1, (w + 1) // 2
     x = _xs_inv(p, s, w)
-    x = (x * pow(_M2, -1, 1 << w)) & m
-    x = _xs_inv(x, s, w)
-    x = (x * pow(_M1, -1, 1 << w)) & m
-    return _xs_inv(x, s, w)
+    return (x * pow(_M1, -1, 1 << w)) & m
 
 
 def _perm(x, w):
     m, s = (1 << w) - 1, (w + 1) // 2
-    x ^= x >> s
     x = (x * _M1) & m
-    x ^= x >> s
-    x = (x * _M2) & m
     return x ^ (x >> s)
 
 
