@@ -605,21 +605,25 @@ def c3_bench(args, host, threads):
     t_launch = float(np.mean(launch_ms)) / 1e3
     traffic = int(prof["traffic_bytes_per_launch"]) if prof and "traffic_bytes_per_launch" in prof else None
     dense_algo = 8 * total + 12 * M + 16 * pairs
-    line_rate = hip.probe_random_reads()
+    table_lines = hip.table_info(h)[0]
+    line_rate = hip.probe_random_reads(nbytes=table_lines * 128)
     dram_req = prof.get("dram_read_requests_per_launch") if prof else None
     random_access = {
         "what": "k_qjoin is bound by independent random line fetches: one 128-byte line of the record table per "
                 "query context looked up (plus overflow lines, long contexts and the streamed keys), not by "
                 "streamed bytes. Its lookups per second (every query key of an assembly sketch heads its own "
-                "context group) against the rate of random 8-byte reads from a 2 GiB buffer measured in this run "
+                "context group) against the rate of random 128-byte line reads, in the kernel's load form, from "
+                "a buffer of the record table's size (rounded down to a power of two) measured in this run "
                 "(kssdhip_probe_random_reads), and its DRAM line requests from the committed profile",
-        "probe_reads_per_s": line_rate,
+        "probe_lines_per_s": line_rate,
+        "probe_buffer_bytes": 1 << (int(table_lines * 128).bit_length() - 1),
         "lookups_per_launch": total,
         "lookups_per_s": total / t_launch,
         "frac": total / t_launch / line_rate,
         "k_qjoin_line_requests_per_launch": dram_req,
         "k_qjoin_lines_per_lookup": dram_req / total if dram_req else None,
         "k_qjoin_line_requests_per_s": dram_req / (kern_ms / 1e3) if dram_req and kern_ms else None,
+        "line_requests_frac": dram_req / (kern_ms / 1e3) / line_rate if dram_req and kern_ms else None,
         "source": prof_src}
     roof = {"bound": "hbm", "achieved": algo / t_launch / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": algo / t_launch / 1e9 / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": prof_src,

@@ -456,12 +456,14 @@ int kssdhip_index_table_info(const kssdhip_index *index, uint64_t *lines, uint64
 			     int *ctx_bits);
 
 /*
- * Diagnostic: the device's rate of independent random 8-byte reads from a
- * buffer of `bytes` (rounded down to a power of two; allocated and freed by
- * the call), `reads` per timed launch, best of two after a warm-up.  No
- * reference counterpart: it measures the bound of random-access kernels
- * (every read fetches its own memory line), against which the query-major
- * join's line requests are set.  Synchronous.
+ * Diagnostic: the device's rate of independent random 128-byte line reads
+ * from a buffer of `bytes` (rounded down to a power of two; allocated and
+ * freed by the call), `reads` lines (rounded up to a multiple of 64) per
+ * timed launch, best of two after a warm-up.  Lines are read as the
+ * query-major join reads its record table (eight lanes per line, 16-byte
+ * nontemporal loads).  No reference counterpart: it measures the bound of
+ * the join's lookups (one home line per query context), against which its
+ * line requests are set.  Synchronous.
  */
 int kssdhip_probe_random_reads(size_t bytes, uint64_t reads, double *reads_per_s);
 

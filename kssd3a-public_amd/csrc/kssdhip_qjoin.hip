@@ -63,9 +63,11 @@ enum { QJ_DENSE = 0, QJ_ACCUM = 1, QJ_SELECT = 2 };
 // is mapped by a bijection p = br_perm(c) of w-bit values; the top bb bits
 // of p name c's home line, the low w - bb bits (the remainder) are stored
 // in each of c's slots.  As (home, remainder) determines c, a slot match is
-// exact: there is no fingerprint to confirm.  A line holds 16 u64 slots:
-//   record  remainder << (20 + obj_bits) | gid << obj_bits | obj  (< 2^63)
-//   long    remainder << (20 + obj_bits) | GID_MASK << obj_bits, then a
+// exact: there is no fingerprint to confirm.  A line holds 16 u64 slots
+// (rs = max(32, 20 + obj_bits): the remainder lies in the slot's high word,
+// so that a lookup compares high words only):
+//   record  remainder << rs | gid << obj_bits | obj  (< 2^63)
+//   long    remainder << rs | GID_MASK << obj_bits, then a
 //           payload slot 3 << 62 | count << 32 | first record: the
 //           context's records in the sorted index (contexts of more than
 //           BR_LMAX records)
@@ -74,8 +76,9 @@ enum { QJ_DENSE = 0, QJ_ACCUM = 1, QJ_SELECT = 2 };
 //   empty   all ones
 // A context's slots are contiguous and never split between lines; its
 // records are in index order ((gid, obj) ascending).  bb >= 21 keeps every
-// record and header below 2^63 for object widths up to 32 bits (remainder
-// + gid + obj = 84 - bb bits), and gid GID_MASK is reserved for headers, so
+// record and header below 2^63 for object widths up to 32 bits (rs plus the
+// remainder's w - bb bits: at most 84 - bb), and gid GID_MASK is reserved
+// for headers, so
 // an index of 2^20 genomes has no table.  Home lines are sized for about 3
 // slots each, so few homes need an overflow line (0.1 % of the lines at
 // 171.7M records): a context absent from the index costs one line, a
@@ -91,6 +94,12 @@ constexpr uint64_t BR_MAX_LONG = (1ull << 30) - 1; // records of one long contex
 __host__ __device__ __forceinline__ int br_ctx_width(unsigned obj_bits)
 {
 	return obj_bits >= 20 ? 64 - (int)obj_bits : 44;
+}
+
+// bit position of the remainder in record and header slots
+__host__ __device__ __forceinline__ int br_rem_shift(unsigned obj_bits)
+{
+	return GID_NBITS + (int)obj_bits > 32 ? GID_NBITS + (int)obj_bits : 32;
 }
 
 // A bijection of w-bit values (1 <= w <= 44): an odd multiplier modulo
@@ -175,7 +184,7 @@ __global__ void k_br_place(const uint64_t *__restrict__ ctxgid, const uint32_t *
 			   unsigned long long *lines)
 {
 	const uint64_t objmask = g.ob >= 32 ? 0xffffffffull : ((1ull << g.ob) - 1ull);
-	const int rs = GID_NBITS + (int)g.ob;
+	const int rs = br_rem_shift(g.ob);
 	const uint64_t remmask = (1ull << (g.w - g.bb)) - 1ull;
 	for (uint64_t h = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; h < g.homes;
 	     h += (uint64_t)gridDim.x * blockDim.x) {
@@ -437,54 +446,122 @@ __device__ __forceinline__ void qj_add(const QTable &t, uint32_t gid, int m, boo
 // together (a 16-byte piece each, so one wave instruction reads eight whole
 // lines), and a lane then reads its own row.  Piece p of row m sits at
 // position p ^ (m & 7), so that both the eight lanes writing a row and
-// eight consecutive lanes reading their rows meet distinct banks.
+// eight consecutive lanes reading their rows meet distinct banks.  Per wave
+// the rows are followed by OWN_WAVE words: the sole-owner bookkeeping of
+// unique counters (128 words), then the walk's 64 one-byte key marks.
 constexpr uint32_t STAGE_ROW = BR_SLOTS * 8;        // bytes per row
 constexpr uint32_t STAGE_WAVE = 64 * STAGE_ROW;     // bytes per wave
-constexpr uint32_t OWN_WAVE = 128;                  // words of sole-owner bookkeeping per wave
+constexpr uint32_t OWN_WAVE = 128 + 16;             // words of walk bookkeeping per wave
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// Cross-lane moves within the 64-lane wave (DPP controls): the whole wave
+// shifted by one lane (the lane left without a source takes `edge`), row
+// shifts within rows of 16 lanes, and the row broadcasts that carry a scan
+// from one row to the next.  They need every lane of the wave active.
+constexpr int DPP_WAVE_SHL1 = 0x130;  // lane i takes lane i + 1
+constexpr int DPP_WAVE_SHR1 = 0x138;  // lane i takes lane i - 1
+constexpr int DPP_ROW_SHR = 0x110;    // + n: lane i takes lane i - n of its row
+constexpr int DPP_ROW_BCAST15 = 0x142; // lane 15 of each row to the next row
+constexpr int DPP_ROW_BCAST31 = 0x143; // lane 31 to rows 2 and 3
+
+__device__ __forceinline__ uint32_t from_prev_lane(uint32_t x, uint32_t edge)
+{
+	return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)x, DPP_WAVE_SHR1, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t x, uint32_t edge)
+{
+	return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)x, DPP_WAVE_SHL1, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ uint64_t from_prev_lane64(uint64_t x, uint64_t edge)
+{
+	return ((uint64_t)from_prev_lane((uint32_t)(x >> 32), (uint32_t)(edge >> 32)) << 32) |
+	       from_prev_lane((uint32_t)x, (uint32_t)edge);
+}
+
+__device__ __forceinline__ uint64_t from_next_lane64(uint64_t x, uint64_t edge)
+{
+	return ((uint64_t)from_next_lane((uint32_t)(x >> 32), (uint32_t)(edge >> 32)) << 32) |
+	       from_next_lane((uint32_t)x, (uint32_t)edge);
+}
+
+// inclusive sum over lanes 0..i (row steps, then rows 1 and 3 take the row
+// below, rows 2 and 3 take rows 0-1)
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
+{
+	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 1, 0xf, 0xf, false);
+	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 2, 0xf, 0xf, false);
+	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 4, 0xf, 0xf, false);
+	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 8, 0xf, 0xf, false);
+	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST15, 0xa, 0xf, false);
+	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST31, 0xc, 0xf, false);
+	return x;
+}
+
+// inclusive maximum over lanes 0..i
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t x)
+{
+	x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 1, 0xf, 0xf, false));
+	x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 2, 0xf, 0xf, false));
+	x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 4, 0xf, 0xf, false));
+	x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_SHR + 8, 0xf, 0xf, false));
+	x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST15, 0xa, 0xf, false));
+	x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST31, 0xc, 0xf, false));
+	return x;
+}
+
+// lane 8 * (i / 8) + K's value (each group of eight lanes takes its K-th)
+template <uint32_t K>
+__device__ __forceinline__ uint32_t group8_lane(uint32_t x)
+{
+	return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x18 | (K << 5));
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ unsigned long long row_slot(const LDS unsigned long long *row, uint32_t lane, uint32_t j)
 {
 	return row[(((j >> 1) ^ (lane & 7u)) << 1) | (j & 1u)];
 }
 
-// Slot x holds a record or header of the context with remainder rem
-// (rs = 20 + obj_bits).  Special slots (bit 63 set, empty) never match: for
-// home bits >= 21 their value above bit rs exceeds every remainder.
-__device__ __forceinline__ bool slot_is(unsigned long long x, uint32_t rem, int rs)
-{
-	if (rs >= 32)
-		return ((uint32_t)(x >> 32) >> (rs - 32)) == rem;
-	return (x >> rs) == (uint64_t)rem;
-}
-
-// match bits of the 16 slots of a row (bit j: slot j holds the context) and
-// the row's last slot
-__device__ __forceinline__ uint32_t row_scan(const LDS unsigned long long *row, uint32_t lane, uint32_t rem, int rs,
-					     unsigned long long *last)
+// Match bits of the 16 slots of a row, slot j at bit 15 - j, and the high
+// word of the row's last slot.  Slot j holds a record or header of the
+// context with remainder rem when its high word, cleared below the
+// remainder (hmask = ~0 << (rs - 32)), equals remhi = rem << (rs - 32).
+// Special slots (bit 63 set, empty) never match: rs plus the remainder's
+// bits stay below 64, so remhi < 2^31.  Only the high words are read.
+__device__ __forceinline__ uint32_t row_scan(const LDS unsigned long long *row, uint32_t lane, uint32_t remhi,
+					     uint32_t hmask, uint32_t *lasthi)
 {
 	uint32_t mm = 0;
 #pragma unroll
 	for (uint32_t p = 0; p < BR_SLOTS / 2; ++p) {
-		const u64x2 v = *(const LDS u64x2 *)(row + ((p ^ (lane & 7u)) << 1));
-		mm |= (uint32_t)slot_is(v.x, rem, rs) << (2 * p);
-		mm |= (uint32_t)slot_is(v.y, rem, rs) << (2 * p + 1);
+		const LDS uint32_t *w = (const LDS uint32_t *)(row + ((p ^ (lane & 7u)) << 1));
+		const uint32_t hx = w[1], hy = w[3];
+		mm = 2u * mm + (uint32_t)((hx & hmask) == remhi);
+		mm = 2u * mm + (uint32_t)((hy & hmask) == remhi);
 		if (p == BR_SLOTS / 2 - 1)
-			*last = v.y;
+			*lasthi = hy;
 	}
 	return mm;
 }
 
-// One genome run's record j of key `owner`'s context: (gid << 32 | obj),
-// from the owner's row (inline contexts; src = first slot) or from the
-// sorted index (long contexts; src = first record).
+// Record at position pos of key `owner`'s context: (gid << 32 | obj), from
+// slot pos of the owner's row (inline contexts) or record pos of the sorted
+// index (long contexts).
 __device__ __forceinline__ uint64_t qj_record(const QJArgs &a, const LDS unsigned long long *wst, uint32_t owner,
-					      bool lng, uint32_t src, uint32_t j, uint64_t objmask)
+					      bool lng, uint32_t pos, uint64_t objmask)
 {
 	const unsigned ob = a.obj_bits;
 	if (lng)
-		return ((a.ctxgid[(uint64_t)src + j] & GID_MASK) << 32) | ((uint64_t)a.robj[(uint64_t)src + j] & objmask);
-	const unsigned long long x = row_slot(wst + owner * BR_SLOTS, owner, src + j);
+		return ((a.ctxgid[pos] & GID_MASK) << 32) | ((uint64_t)a.robj[pos] & objmask);
+	const unsigned long long x = row_slot(wst + owner * BR_SLOTS, owner, pos);
 	return (((x >> ob) & GID_MASK) << 32) | (x & objmask);
 }
 
@@ -492,96 +569,93 @@ __device__ __forceinline__ uint64_t qj_record(const QJArgs &a, const LDS unsigne
 // (command_ani.h:516-575; LOWER: :668-716, genomes below the query only).
 // Key m of the wave (found) has cnt records, from slot src of its row or,
 // for a long context, from record src of the sorted index; its group is
-// the query's keys [i, i + gs).  The wave numbers all records in lane
-// order; lane L takes record r0 + L of each round, finds its key by a
-// binary search of the inclusive ends, and the lane holding the first
-// record of a genome run counts the run: object-mismatch minimum over the
-// cross product of the group's and the run's objects, one add per run
+// the query's keys [wb + m, wb + m + gs).  The wave numbers all records in
+// lane order (an inclusive sum of the counts); lane L takes record r0 + L
+// of each round.  Its key: each key with records marks the position of its
+// first record in this round with its lane, and a running maximum over the
+// marks carries the key to the records that follow.  The lane holding the
+// first record of a genome run counts the run: object-mismatch minimum over
+// the cross product of the group's and the run's objects, one add per run
 // (reading any further records of the run itself).  With unique counters
 // each key's runs are counted in `own` (per lane of the wave: runs, then
 // the last run's genome, minimum and whether it was counted), so that the
-// key's lane applies the sole-owner rule afterwards.
+// key's lane applies the sole-owner rule afterwards.  Called by the whole
+// wave.
 template <bool UNIQ, bool DIRECT>
 __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, const LDS unsigned long long *wst,
 					     LDS uint32_t *own, uint32_t lane, bool found, bool lng, uint32_t src,
-					     uint32_t cnt, uint64_t i, uint32_t gs, uint64_t key, int qgid,
+					     uint32_t cnt, uint64_t wb, uint32_t gs, uint64_t key, int qgid,
 					     unsigned long long *spill_row, LDS uint32_t *spilled)
 {
 	const unsigned ob = a.obj_bits;
 	const uint64_t objmask = ob >= 32 ? 0xffffffffull : ((1ull << ob) - 1ull);
 	const uint32_t mycnt = found ? cnt : 0u;
-	uint32_t incl = mycnt;
-#pragma unroll
-	for (uint32_t d = 1; d < 64; d <<= 1) {
-		const uint32_t y = __shfl_up(incl, d);
-		if (lane >= d)
-			incl += y;
-	}
-	const uint32_t total = __shfl(incl, 63);
+	const uint32_t incl = wave_incl_sum(mycnt);
+	const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
 	if (total == 0)
 		return;
 	const uint32_t first = incl - mycnt; // the key's first record number
+	// read by the lanes of the key's records: record number r sits at
+	// position base + r (a slot of the row or a record of the index), the
+	// key's positions end before `end`
+	const uint32_t base = src - first, end = src + mycnt;
 	const uint32_t qobj = (uint32_t)(key & objmask);
 	const uint32_t shape = (gs << 1) | (lng ? 1u : 0u);
 	const bool groups = __any(found && gs > 1); // some key has a group of several keys
 	const bool lower = (a.flags & KSSDHIP_LOWER) != 0;
 	const bool ignore = (a.flags & KSSDHIP_IGNORECONFLICT) != 0;
+	LDS uint8_t *mark = (LDS uint8_t *)(own + 128);
 	uint32_t carry_gid = NONE, carry_owner = 64;
 	for (uint32_t r0 = 0; r0 < total; r0 += 64) { // wave-uniform
 		const uint32_t tt = r0 + lane;
 		const bool act = tt < total;
-		uint32_t owner = 0; // lanes whose inclusive end is <= tt
-#pragma unroll
-		for (uint32_t step = 32; step; step >>= 1) {
-			const uint32_t e = __shfl(incl, (int)(owner + step - 1));
-			if (e <= tt)
-				owner += step;
-		}
-		owner = owner < 64 ? owner : 63;
-		const uint32_t j = tt - __shfl(first, (int)owner);
-		const uint32_t osrc = __shfl(src, (int)owner);
+		if (mycnt && first - r0 < 64u)
+			mark[first - r0] = (uint8_t)(lane + 1u);
+		wave_sync();
+		uint32_t m = mark[lane];
+		mark[lane] = 0; // clear for the next round (the marks start at zero)
+		if (lane == 0 && m == 0)
+			m = carry_owner + 1u; // record r0 continues the previous round's last key
+		const uint32_t owner = (wave_incl_max(m) - 1u) & 63u;
 		const uint32_t oshape = __shfl(shape, (int)owner);
-		const uint32_t ocnt = __shfl(mycnt, (int)owner);
+		const uint32_t pos = __shfl(base, (int)owner) + tt;
+		const uint32_t oend = __shfl(end, (int)owner);
 		const uint32_t oobj = __shfl(qobj, (int)owner);
-		uint64_t oi = 0;
-		if (groups) {
-			oi = __shfl(i, (int)owner);
-		}
+		const uint64_t oi = wb + owner;
 		const bool olng = oshape & 1u;
 		const uint32_t ogs = oshape >> 1;
-		const uint64_t rec = act ? qj_record(a, wst, owner, olng, osrc, j, objmask) : 0ull;
-		const uint32_t gid = act ? (uint32_t)(rec >> 32) : NONE;
+		const bool ok = act && pos < oend;
+		const uint64_t rec = ok ? qj_record(a, wst, owner, olng, pos, objmask) : 0ull;
+		const uint32_t gid = ok ? (uint32_t)(rec >> 32) : NONE;
 		// minimum over the group's objects against this record's object
 		auto gmin = [&](uint32_t obj) -> int {
 			int mm = dna_popcount_dev(oobj ^ obj);
 			mm = mm < MIN_INIT ? mm : MIN_INIT;
-			for (uint32_t x = 1; x < ogs && mm; ++x) {
-				const int pc = dna_popcount_dev((uint32_t)(a.keys[oi + x] & objmask) ^ obj);
-				mm = pc < mm ? pc : mm;
+			if (groups) {
+				for (uint32_t x = 1; x < ogs && mm; ++x) {
+					const int pc = dna_popcount_dev((uint32_t)(a.keys[oi + x] & objmask) ^ obj);
+					mm = pc < mm ? pc : mm;
+				}
 			}
 			return mm;
 		};
-		int m = act ? gmin((uint32_t)rec) : MIN_INIT;
-		uint32_t pg = __shfl_up(gid, 1), po = __shfl_up(owner, 1);
-		if (lane == 0) {
-			pg = carry_gid;
-			po = carry_owner;
-		}
-		const uint32_t ng = __shfl_down(gid, 1), no = __shfl_down(owner, 1);
-		carry_gid = __shfl(gid, 63);
-		carry_owner = __shfl(owner, 63);
-		if (!act || (po == owner && pg == gid))
+		int mmin = ok ? gmin((uint32_t)rec) : MIN_INIT;
+		const uint32_t pg = from_prev_lane(gid, carry_gid), po = from_prev_lane(owner, carry_owner);
+		const uint32_t ng = from_next_lane(gid, NONE), no = from_next_lane(owner, 64u);
+		carry_gid = (uint32_t)__builtin_amdgcn_readlane((int)gid, 63);
+		carry_owner = (uint32_t)__builtin_amdgcn_readlane((int)owner, 63);
+		if (!ok || (po == owner && pg == gid))
 			continue; // not the first record of its run
 		uint32_t n = 1;
-		const bool more = lane < 63 ? (tt + 1 < total && no == owner && ng == gid) : j + 1 < ocnt;
-		for (uint32_t jj = j + 1; more && jj < ocnt; ++jj) {
-			const uint64_t y = qj_record(a, wst, owner, olng, osrc, jj, objmask);
+		const bool more = lane < 63 ? (tt + 1 < total && no == owner && ng == gid) : pos + 1 < oend;
+		for (uint32_t pp = pos + 1; more && pp < oend; ++pp) {
+			const uint64_t y = qj_record(a, wst, owner, olng, pp, objmask);
 			if ((uint32_t)(y >> 32) != gid)
 				break;
 			++n;
-			if (m) {
+			if (mmin) {
 				const int pc = gmin((uint32_t)y);
-				m = pc < m ? pc : m;
+				mmin = pc < mmin ? pc : mmin;
 			}
 		}
 		if (lower && (int)gid >= qgid)
@@ -589,15 +663,13 @@ __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, c
 		const bool counted = !(ignore && n > 1);
 		if (UNIQ) {
 			__hip_atomic_fetch_add(&own[owner], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-			own[64 + owner] = gid | ((uint32_t)m << 20) | ((uint32_t)counted << 24);
+			own[64 + owner] = gid | ((uint32_t)mmin << 20) | ((uint32_t)counted << 24);
 		}
 		if (counted)
-			qj_add<UNIQ, DIRECT>(t, gid, m, false, spill_row, spilled);
+			qj_add<UNIQ, DIRECT>(t, gid, mmin, false, spill_row, spilled);
 	}
 	if (UNIQ) {
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-		__builtin_amdgcn_wave_barrier();
-		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		wave_sync();
 		if (mycnt) {
 			const uint32_t runs = own[lane], info = own[64 + lane];
 			if (runs == 1 && (info >> 24))
@@ -634,9 +706,10 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 	LDS unsigned long long *wst = stage + (threadIdx.x >> 6) * (STAGE_WAVE / 8);
 	LDS unsigned long long *row = wst + lane * BR_SLOTS;
 	LDS uint32_t *own = (LDS uint32_t *)(stage + (blockDim.x >> 6) * (STAGE_WAVE / 8)) + (threadIdx.x >> 6) * OWN_WAVE;
-	const int rs = GID_NBITS + (int)ob;
 	const int rsh = a.w - a.bb;
 	const uint64_t remmask = (1ull << rsh) - 1ull;
+	const int hs = br_rem_shift(ob) - 32;
+	const uint32_t hmask = ~0u << hs;
 	uint64_t wb = k0 + (threadIdx.x & ~63u); // wave-uniform
 	uint64_t key = wb + lane < k1 ? a.keys[wb + lane] : 0ull;
 	uint64_t before = lane == 0 && wb > kb && wb < k1 ? a.keys[wb - 1] : ~0ull; // the key ahead of the wave's first
@@ -644,31 +717,37 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 		const uint64_t i = wb + lane;
 		const bool in = i < k1;
 		const uint64_t c = key >> ob;
-		uint64_t prev = __shfl_up(key, 1u);
-		if (lane == 0)
-			prev = before;
+		const uint64_t prev = from_prev_lane64(key, before);
 		// cmax: the largest context the table can hold (index contexts <= ctx_max, below 2^w)
 		const bool on = in && c <= a.cmax && (i == kb || (prev >> ob) != c);
 		const uint64_t p = br_perm(on ? c : 0ull, a.w);
-		const uint32_t rem = (uint32_t)(p & remmask);
+		const uint32_t remhi = (uint32_t)(p & remmask) << hs;
 		const uint32_t line = (uint32_t)(p >> rsh); // tables hold fewer than 2^32 lines
+		// load k: the eight lanes of group g read row 8g + k's line.  The
+		// loads are nontemporal: lines of the table are rarely read twice,
+		// and keeping them out of the L2 leaves it to the page-table
+		// entries of the multi-gigabyte table (random line reads of an
+		// 8 GiB buffer: 52 G/s nontemporal, 48 G/s otherwise)
 		const u64x2 *lines = (const u64x2 *)a.brec + piece;
 		u64x2 v[8];
-#pragma unroll
-		for (uint32_t k = 0; k < 8; ++k) { // rows 8k .. 8k+7, eight lanes per row
-			const uint32_t lm = __shfl(line, (int)(8 * k + g));
-			v[k] = lines[(uint64_t)lm * (BR_SLOTS / 2)];
-		}
+		v[0] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<0>(line) * (BR_SLOTS / 2)]);
+		v[1] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<1>(line) * (BR_SLOTS / 2)]);
+		v[2] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<2>(line) * (BR_SLOTS / 2)]);
+		v[3] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<3>(line) * (BR_SLOTS / 2)]);
+		v[4] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<4>(line) * (BR_SLOTS / 2)]);
+		v[5] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<5>(line) * (BR_SLOTS / 2)]);
+		v[6] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<6>(line) * (BR_SLOTS / 2)]);
+		v[7] = __builtin_nontemporal_load(&lines[(uint64_t)group8_lane<7>(line) * (BR_SLOTS / 2)]);
 		// the next keys, while the lines are in flight
 		const uint64_t nb = wb + B;
 		const uint64_t nkey = nb + lane < k1 ? a.keys[nb + lane] : 0ull;
 		const uint64_t nbefore = lane == 0 && nb < k1 ? a.keys[nb - 1] : ~0ull;
 #pragma unroll
 		for (uint32_t k = 0; k < 8; ++k)
-			*(LDS u64x2 *)(wst + (8 * k + g) * BR_SLOTS + ((piece ^ g) << 1)) = v[k];
+			*(LDS u64x2 *)(wst + (8 * g + k) * BR_SLOTS + ((piece ^ k) << 1)) = v[k];
 		// the group: usually this key alone (the next lane's key has another
 		// context); else the keys that follow with the same context
-		const uint64_t after = __shfl_down(key, 1u);
+		const uint64_t after = from_next_lane64(key, 0ull);
 		uint32_t gs = 1;
 		if (on && !(lane < 63 ? (i + 1 >= ke || (after >> ob) != c) : i + 1 >= ke))
 			while (i + gs < ke && (a.keys[i + gs] >> ob) == c)
@@ -678,10 +757,10 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 			__builtin_amdgcn_wave_barrier();
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-			unsigned long long last = 0;
-			const uint32_t mm = pending ? row_scan(row, lane, rem, rs, &last) : 0u;
+			uint32_t lasthi = 0;
+			const uint32_t mm = pending ? row_scan(row, lane, remhi, hmask, &lasthi) : 0u;
 			const bool found = mm != 0;
-			const uint32_t lo = found ? (uint32_t)__builtin_ctz(mm) : 0u;
+			const uint32_t lo = found ? (uint32_t)__builtin_clz(mm) - 16u : 0u; // the first matching slot
 			bool lng = false;
 			uint32_t src = lo, cnt = (uint32_t)__builtin_popcount(mm);
 			if (found && ((row_slot(row, lane, lo) >> ob) & GID_MASK) == GID_MASK) { // header + payload
@@ -690,17 +769,18 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 				src = (uint32_t)pay;
 				cnt = (uint32_t)((pay >> 32) & BR_MAX_LONG);
 			}
-			qj_wave_walk<UNIQ, DIRECT>(a, t, wst, own, lane, found, lng, src, cnt, i, gs, key, qgid, spill_row,
+			qj_wave_walk<UNIQ, DIRECT>(a, t, wst, own, lane, found, lng, src, cnt, wb, gs, key, qgid, spill_row,
 						   spilled);
-			pending = pending && !found && (last & BR_PAYLOAD) == BR_HIGH;
+			pending = pending && !found && (lasthi >> 30) == 2u; // slot 15 continues the home
 			if (!__any(pending))
 				break;
+			const unsigned long long nline = pending ? row_slot(row, lane, BR_SLOTS - 1) & ~BR_HIGH : 0ull;
 			__builtin_amdgcn_wave_barrier(); // rows read by the walk before any is replaced
 			if (pending) { // the home continues in this overflow line
-				const u64x2 *nsrc = (const u64x2 *)(a.brec + (last & ~BR_HIGH) * BR_SLOTS);
+				const u64x2 *nsrc = (const u64x2 *)(a.brec + nline * BR_SLOTS);
 #pragma unroll
 				for (uint32_t pp = 0; pp < BR_SLOTS / 2; ++pp)
-					*(LDS u64x2 *)(row + ((pp ^ (lane & 7u)) << 1)) = nsrc[pp];
+					*(LDS u64x2 *)(row + ((pp ^ (lane & 7u)) << 1)) = __builtin_nontemporal_load(&nsrc[pp]);
 			}
 		}
 		__builtin_amdgcn_wave_barrier(); // every row read before the next keys' lines land
@@ -906,7 +986,7 @@ __global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(
 	t.mask = a.slots - 1;
 	t.shift = 32 - a.sbits;
 	unsigned long long *spill_row = DIRECT ? nullptr : a.spill + (uint64_t)blockIdx.x * a.R * (UNIQ ? 4u : 2u);
-	if (UNIQ) { // the sole-owner counts start at zero (each key's lane clears its count after use)
+	{ // the walk's marks and sole-owner counts start at zero (each lane clears what it used)
 		LDS uint32_t *own = (LDS uint32_t *)(stage + (blockDim.x >> 6) * (STAGE_WAVE / 8));
 		for (uint32_t k = threadIdx.x; k < (blockDim.x >> 6) * OWN_WAVE; k += blockDim.x)
 			own[k] = 0;
@@ -986,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_row_values(kssdhip_ani_row_t *rows, con
 __global__ __launch_bounds__(256) void k_mark_matched(QJArgs a, uint8_t *marks)
 {
 	const unsigned ob = a.obj_bits;
-	const int rs = GID_NBITS + (int)ob;
+	const int rs = br_rem_shift(ob);
 	const int rsh = a.w - a.bb;
 	const uint64_t total = a.skidx[a.nq];
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {

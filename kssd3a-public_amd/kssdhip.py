@@ -544,7 +544,7 @@ class KssdHip:
                                                              d_qcount, stream))
 
     def probe_random_reads(self, nbytes=2 << 30, reads=400_000_000):
-        """Random 8-byte reads per second from an nbytes buffer."""
+        """Random 128-byte line reads per second from an nbytes buffer (the join's load form)."""
         r = D()
         self.check(self.lib.kssdhip_probe_random_reads(nbytes, reads, ctypes.byref(r)))
         return float(r.value)
