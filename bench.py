@@ -103,37 +103,53 @@ def cpu_baseline(records, nref, obj_bits, qkeys, qidx, qctx, rctx, budget_s, thr
     return nq, dt, ctx, obj, out
 
 
-def measured_traffic(tiling, nref, nqry):
-    """HBM bytes per step of k_split + k_pjoin from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload, when it was measured
-    with the tiling this run uses; (bytes or None, summary file name)."""
+def tiling_key(tiling):
+    """The fields that define a join tiling: the query-major join (mode 3)
+    by its workgroup size and LDS slots, the other modes by chunk, workgroup
+    size and partition group."""
+    if tiling["mode"] == 3:
+        return (3, tiling.get("query_threads", 0), tiling.get("query_slots", 0))
+    return (tiling["mode"], tiling["chunk_records"], tiling["block_threads"], tiling.get("group_partitions", 1))
+
+
+def profile_tiling(text):
+    """The tiling a committed profile summary names (its 'mode ...' line)."""
     import re
-    files = sorted((ROOT / "profiles").glob(f"pjoin_split_c2_{nref}x{nqry}_traffic*.md"))
+    m = re.search(r"mode 3, query_threads (\d+), query_slots (\d+)", text)
+    if m:
+        return (3, int(m.group(1)), int(m.group(2)))
+    m = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
+    return tuple(map(int, m.groups())) if m else None
+
+
+def measured_traffic(tiling, nref, nqry):
+    """HBM bytes per step of the counting call (k_split + k_pjoin, or the
+    query-major join) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload, when it was measured with the tiling this run
+    uses; (bytes or None, summary file name)."""
+    import re
+    files = sorted((ROOT / "profiles").glob(f"*_c2_{nref}x{nqry}_traffic*.md"))
     if not files:
         return None, None
-    want = (tiling["mode"], tiling["chunk_records"], tiling["block_threads"], tiling.get("group_partitions", 1))
     for f in files:  # one summary per measured tiling
         text = f.read_text()
         m = re.search(r"traffic_bytes_per_step=(\d+)", text)
-        mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
-        if m and mt and tuple(map(int, mt.groups())) == want:
+        if m and profile_tiling(text) == tiling_key(tiling):
             return int(m.group(1)), f"profiles/{f.name}"
     return None, f"profiles/{files[0].name} (measured with other tilings)"
 
 
 def measured_lds(tiling, nref, nqry):
-    """k_pjoin's LDS figures per dispatch from the committed rocprofv3 summary
-    of this workload when measured with this run's tiling: (dict or None,
-    summary file name)."""
+    """The counting kernel's LDS figures per dispatch from the committed
+    rocprofv3 summary of this workload when measured with this run's tiling:
+    (dict or None, summary file name)."""
     import re
-    files = sorted((ROOT / "profiles").glob(f"pjoin_split_c2_{nref}x{nqry}_lds*.md"))
+    files = sorted((ROOT / "profiles").glob(f"*_c2_{nref}x{nqry}_lds*.md"))
     if not files:
         return None, None
-    want = (tiling["mode"], tiling["chunk_records"], tiling["block_threads"], tiling.get("group_partitions", 1))
     for f in files:  # one summary per measured tiling
         text = f.read_text()
-        mt = re.search(r"mode (\d+), chunk_records (\d+), block_threads (\d+), group_partitions (\d+)", text)
-        if mt and tuple(map(int, mt.groups())) == want:
+        if profile_tiling(text) == tiling_key(tiling):
             vals = {k: float(v) for k, v in re.findall(r"^(lds_\w+)=([0-9.eE+-]+)$", text, re.M)}
             return vals, f"profiles/{f.name}"
     return None, f"profiles/{files[0].name} (measured with other tilings)"
@@ -143,8 +159,7 @@ def counters_note(tiling):
     """What bounds k_pjoin at C2, from the committed counter summary when it
     was measured at this run's tiling."""
     vals, src = measured_profile("pjoin_c2_1000x1000_counters")
-    if not vals or (tiling["mode"], tiling["chunk_records"], tiling["block_threads"],
-                    tiling.get("group_partitions", 1)) != (2, 5696, 1024, 4):
+    if not vals or tiling_key(tiling) != (2, 5696, 1024, 4):
         return None
     return {"source": src, "k_pjoin_us": vals.get("k_pjoin_avg_us"),
             "dram_bytes_per_dispatch": vals.get("fetch_bytes_per_dispatch"),
@@ -1225,7 +1240,8 @@ def main():
     ap.add_argument("--index-build-only", type=int, default=0, metavar="GENOMES",
                     help="only time the index build of GENOMES clade genomes (host builder vs GPU) and exit")
     ap.add_argument("--tiling", default="",
-                    help="fixed join tiling MODE,CHUNK,BLOCK[,GROUP] instead of the autotuner (for profiling runs)")
+                    help="fixed join tiling MODE,CHUNK,BLOCK[,GROUP], or 3,THREADS[,SLOTS] for the query-major "
+                         "join, instead of the autotuner (for profiling runs)")
     ap.add_argument("--latency-queries", type=int, default=64)
     ap.add_argument("--index-refs", type=int, default=10000,
                     help="reference genomes of the large-index bench (500 clades of 20); 0 skips it")
@@ -1335,8 +1351,11 @@ def main():
     t = time.time()
     if args.tiling:
         v = [int(x) for x in args.tiling.split(",")]
-        tiling = {"mode": v[0], "chunk_records": v[1], "block_threads": v[2],
-                  "group_partitions": v[3] if len(v) > 3 else 1}
+        if v[0] == 3:
+            tiling = {"mode": 3, "query_threads": v[1], "query_slots": v[2] if len(v) > 2 else 0}
+        else:
+            tiling = {"mode": v[0], "chunk_records": v[1], "block_threads": v[2],
+                      "group_partitions": v[3] if len(v) > 3 else 1}
         hip.set_tiling(h, **tiling)
         tune_ms = {}
         log(f"fixed tiling {tiling}")
@@ -1514,8 +1533,9 @@ def main():
             "pairs": cells, "ref_entries": int(ridx[-1]), "query_entries": total,
             "index_records": len(records), "output": "dense counts + f64 CtxMoE ANI per pair",
         },
-        "step": {"timed": "HIP graph of one step (fill kernels + k_split + k_pjoin + FP64 epilogue)"
-                          if graph_ok else "eager launches",
+        "step": {"timed": ("HIP graph of one step (fill kernels + "
+                           + {2: "k_split + k_pjoin", 3: "k_qjoin"}.get(tiling["mode"], "k_join")
+                           + " + FP64 epilogue)") if graph_ok else "eager launches",
                  "graph_ms": graph_ms, "graph_agrees_with_eager": graph_ok,
                  "eager_ms": total_ms / args.steps, "eager_pairs_per_s": eager_pairs_per_s},
         "plain_kernel": {"pairs_per_s": plain_pairs_per_s, "sample_pairs": npl,
@@ -1530,7 +1550,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "traffic_GBs": traffic / join_s / 1e9 if traffic else None,
-                     "kernel": "k_split+k_pjoin" if tiling["mode"] == 2 else "k_join", "algorithmic_bytes": algo_bytes,
+                     "kernel": {2: "k_split+k_pjoin", 3: "k_qjoin (key-range items)"}.get(tiling["mode"], "k_join"),
+                     "algorithmic_bytes": algo_bytes,
                      "bytes_model": "8*N_Q + 12*M + 16*P_out, P_out = Q*R (dense output)",
                      "N_Q": total, "M": M, "P_out": cells, "P_touched": p_touched,
                      "avg_launch_ms": float(np.mean(join_ms)),

@@ -47,7 +47,7 @@ constexpr int QJ_MAX_THREADS = 512; // workgroup size bound of k_qjoin
 #define QJ_WAVES_PER_SIMD 4 // register budget of k_qjoin: at most 128 VGPRs, four waves per SIMD
 #endif
 constexpr uint32_t DEFAULT_SLOTS = 512;
-constexpr uint64_t QJ_MIN_CHUNK = 1024; // keys per work item when queries are cut into key ranges
+constexpr uint64_t QJ_MIN_CHUNK = 1024; // least keys per work item when queries are cut into key ranges
 constexpr uint64_t QJ_AUTO_MIN_KEYS = 1u << 20; // blocks of few queries take the query-major join from this many keys
 // workspace: [0, WS_HEAD) work counter and the order's histogram and
 // cursors (65 words), then the spill rows, then the work order (nq words,
@@ -319,13 +319,6 @@ __global__ __launch_bounds__(256) void k_order_scatter(const uint64_t *__restric
 	}
 }
 
-// items of `chunk` keys per query (a query without keys has none)
-__global__ void k_item_counts(const uint64_t *__restrict__ skidx, uint32_t nq, uint32_t chunk, uint32_t *cnt)
-{
-	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q <= nq; q += gridDim.x * blockDim.x)
-		cnt[q] = q < nq ? (uint32_t)((skidx[q + 1] - skidx[q] + chunk - 1) / chunk) : 0u;
-}
-
 __global__ void k_zero_u32(uint32_t *p, uint32_t n)
 {
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -362,11 +355,10 @@ struct QJArgs {
 	// work queue and spill rows (zero at rest: a workgroup zeroes what it spilled)
 	uint32_t *next;
 	const uint32_t *order; // queries in processing order
-	// key-range work items (chunk > 0): query q's keys in ranges of `chunk`,
-	// items qitems[q] .. qitems[q+1]-1, qitems[nq] in all; the outputs are
-	// added to (each item flushes its table), so they start zeroed
+	// key-range work items (chunk > 0): item k is a range of the block's
+	// keys (qj_item_range), counted query by query; the outputs are added
+	// to (each query's part flushes its table), so they start zeroed
 	uint32_t chunk;
-	const uint32_t *qitems;
 	unsigned long long *spill; // gridDim.x rows of R cells x (2 | 4) u64
 	int mode;
 	// dense outputs (QJ_DENSE, QJ_ACCUM)
@@ -746,10 +738,13 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 		for (uint32_t k = 0; k < 8; ++k)
 			*(LDS u64x2 *)(wst + (8 * g + k) * BR_SLOTS + ((piece ^ k) << 1)) = v[k];
 		// the group: usually this key alone (the next lane's key has another
-		// context); else the keys that follow with the same context
+		// context); else the keys that follow with the same context, read
+		// from memory (also past the item's end k1, where the next lane
+		// holds no key)
 		const uint64_t after = from_next_lane64(key, 0ull);
+		const bool ends = i + 1 >= ke || (lane < 63 && i + 1 < k1 && (after >> ob) != c);
 		uint32_t gs = 1;
-		if (on && !(lane < 63 ? (i + 1 >= ke || (after >> ob) != c) : i + 1 >= ke))
+		if (on && !ends)
 			while (i + gs < ke && (a.keys[i + gs] >> ob) == c)
 				++gs;
 		bool pending = on;
@@ -964,14 +959,72 @@ __device__ void qj_output(const QJArgs &a, const QTable &t, uint32_t q, bool spi
 	}
 }
 
-// Persistent workgroups: each takes the next query of the block from the
-// work counter until none is left (the reference's guided schedule over
-// queries, command_ani.h:481).
+// One query's keys [k0, k1) (ITEMS; else all its keys) by the workgroup:
+// the table cleared, the keys counted, the query's output written or added.
+template <bool UNIQ, bool DIRECT, bool ITEMS>
+__device__ __forceinline__ void qj_query(const QJArgs &a, const QTable &t, uint32_t q, uint64_t k0, uint64_t k1,
+					 unsigned long long *spill_row, LDS unsigned long long *stage,
+					 LDS uint32_t *s_spilled, LDS uint32_t *s_n, LDS uint32_t *s_base)
+{
+	const uint32_t ncell = DIRECT ? a.R : a.slots;
+	for (uint32_t k = threadIdx.x; k < ncell; k += blockDim.x) {
+		t.c[k] = 0;
+		t.o[k] = 0;
+		if (UNIQ) {
+			t.uc[k] = 0;
+			t.uo[k] = 0;
+		}
+		if (!DIRECT)
+			t.gid[k] = EMPTY;
+	}
+	if (threadIdx.x == 0) {
+		*s_spilled = 0;
+		*s_n = 0;
+	}
+	__syncthreads();
+	qj_count<UNIQ, DIRECT, ITEMS>(a, t, q, k0, k1, spill_row, s_spilled, stage);
+	__syncthreads();
+	const bool spilled = *s_spilled != 0;
+	qj_output<UNIQ, DIRECT>(a, t, q, spilled, spill_row, s_n, s_base);
+}
+
+// Keys [b, e) of item k, of a block of `total` keys shared by g workgroups
+// (the reference's guided schedule over queries, command_ani.h:481, taken
+// to keys): items come in rounds of g, round j's items of total / (g *
+// 2^(j+1)) keys (rounded up to 64), so each round takes about half of what
+// is left, until they would not exceed `chunk` keys; then items of `chunk`
+// keys.  A function of k alone: workgroups take item numbers from a counter.
+__device__ __forceinline__ void qj_item_range(uint64_t k, uint64_t total, uint32_t g, uint32_t chunk, uint64_t *b,
+					      uint64_t *e)
+{
+	uint64_t at = 0;
+	for (int j = 0; j < 62; ++j) {
+		const uint64_t size = ((total >> (j + 1)) / g + 63) / 64 * 64;
+		if (size <= chunk)
+			break;
+		if (k < g) {
+			at += k * size;
+			*b = min(at, total);
+			*e = min(at + size, total);
+			return;
+		}
+		at += g * size;
+		k -= g;
+	}
+	at += k * chunk;
+	*b = min(at, total);
+	*e = min(at + chunk, total);
+}
+
+// Persistent workgroups: each takes the next query of the block (largest
+// first: the reference's guided schedule over queries, command_ani.h:481)
+// or the next key-range item from the work counter until none is left.
 template <bool UNIQ, bool DIRECT, bool ITEMS>
 __global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(QJ_WAVES_PER_SIMD))) void k_qjoin(QJArgs a)
 {
 	extern __shared__ unsigned long long qj_lds[];
-	__shared__ uint32_t s_q, s_item, s_spilled, s_n, s_base;
+	__shared__ uint32_t s_q, s_spilled, s_n, s_base;
+	__shared__ uint64_t s_k0, s_k1;
 	const uint32_t ncell = DIRECT ? a.R : a.slots;
 	QTable t;
 	t.c = (LDS unsigned long long *)qj_lds;
@@ -994,54 +1047,48 @@ __global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(
 	for (;;) {
 		if (threadIdx.x == 0) {
 			const uint32_t k = atomicAdd(a.next, 1u);
-			if (ITEMS) { // item k: the query whose items include k, and the key range
-				if (k < a.qitems[a.nq]) {
-					uint32_t lo = 0, hi = a.nq; // largest q with qitems[q] <= k
+			if (ITEMS) { // item k's keys (absolute indices) and the first query they reach
+				const uint64_t base = a.skidx[0], total = a.skidx[a.nq] - base;
+				uint64_t b, e;
+				qj_item_range(k, total, gridDim.x, a.chunk, &b, &e);
+				s_q = a.nq;
+				if (b < total) {
+					uint32_t lo = 0, hi = a.nq; // largest q with skidx[q] <= key b
 					while (hi - lo > 1) {
 						const uint32_t mid = (lo + hi) >> 1;
-						if (a.qitems[mid] <= k)
+						if (a.skidx[mid] <= base + b)
 							lo = mid;
 						else
 							hi = mid;
 					}
 					s_q = lo;
-					s_item = k - a.qitems[lo];
-				} else {
-					s_q = a.nq;
+					s_k0 = base + b;
+					s_k1 = base + e;
 				}
 			} else {
 				s_q = k < a.nq ? a.order[k] : a.nq;
 			}
 		}
 		__syncthreads(); // also: every thread is done with the previous query's table
-		const uint32_t q = s_q;
+		uint32_t q = s_q;
 		if (q >= a.nq)
 			break;
-		uint64_t k0 = 0, k1 = 0;
-		if (ITEMS) {
-			const uint64_t kb = a.skidx[q], ke = a.skidx[q + 1];
-			k0 = kb + (uint64_t)s_item * a.chunk;
-			k1 = min(ke, k0 + a.chunk);
+		if (!ITEMS) {
+			qj_query<UNIQ, DIRECT, ITEMS>(a, t, q, 0, 0, spill_row, stage, (LDS uint32_t *)&s_spilled,
+							      (LDS uint32_t *)&s_n, (LDS uint32_t *)&s_base);
+			continue;
 		}
-		for (uint32_t k = threadIdx.x; k < ncell; k += blockDim.x) {
-			t.c[k] = 0;
-			t.o[k] = 0;
-			if (UNIQ) {
-				t.uc[k] = 0;
-				t.uo[k] = 0;
-			}
-			if (!DIRECT)
-				t.gid[k] = EMPTY;
+		// each query the key range reaches, in turn
+		const uint64_t i0 = s_k0, i1 = s_k1;
+		for (;;) {
+			const uint64_t k0 = max(i0, a.skidx[q]), k1 = min(i1, a.skidx[q + 1]);
+			if (k0 < k1)
+				qj_query<UNIQ, DIRECT, ITEMS>(a, t, q, k0, k1, spill_row, stage, (LDS uint32_t *)&s_spilled,
+							      (LDS uint32_t *)&s_n, (LDS uint32_t *)&s_base);
+			if (++q >= a.nq || a.skidx[q] >= i1)
+				break;
+			__syncthreads(); // every thread is done with the previous query's table
 		}
-		if (threadIdx.x == 0) {
-			s_spilled = 0;
-			s_n = 0;
-		}
-		__syncthreads();
-		qj_count<UNIQ, DIRECT, ITEMS>(a, t, q, k0, k1, spill_row, (LDS uint32_t *)&s_spilled, stage);
-		__syncthreads();
-		const bool spilled = s_spilled != 0;
-		qj_output<UNIQ, DIRECT>(a, t, q, spilled, spill_row, (LDS uint32_t *)&s_n, (LDS uint32_t *)&s_base);
 	}
 }
 
@@ -1264,8 +1311,6 @@ struct QJWork {
 	uint32_t *head;
 	uint32_t *order;
 	unsigned long long *spill;
-	uint32_t *qitems; // key-range items per query (nq + 1 words), then scan scratch
-	uint32_t *sums;
 };
 
 size_t qj_order_bytes(uint32_t nq)
@@ -1282,20 +1327,22 @@ void qj_zero(void *p, size_t bytes, hipStream_t s)
 }
 
 // zeroes the head and builds the work order (largest queries first)
-void qj_order(const uint64_t *d_skidx, uint32_t nq, hipStream_t s, const QJWork &w)
+void qj_order(const uint64_t *d_skidx, uint32_t nq, hipStream_t s, const QJWork &w, bool order)
 {
 	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s, w.head, WS_HEAD_WORDS);
+	if (!order)
+		return;
 	const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nq + 255) / 256, 2048));
 	hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, s, d_skidx, nq, w.head);
 	hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, w.head);
 	hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, s, d_skidx, nq, w.head, w.order);
 }
 
-int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_t nq, hipStream_t s, QJWork *w)
+int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_t nq, hipStream_t s, QJWork *w,
+		 bool order = true)
 {
 	const size_t spill = std::max(ws->zero_bytes, sh.spill_bytes);
-	const uint32_t nsum = (uint32_t)((nq + 1 + KSSD_SCAN_BLOCK - 1) / KSSD_SCAN_BLOCK);
-	const size_t want = WS_HEAD + spill + qj_order_bytes(nq) + qj_order_bytes(nq + 1) + qj_order_bytes(nsum);
+	const size_t want = WS_HEAD + spill + qj_order_bytes(nq);
 	if (want > ws->bytes) {
 		int rc = kssd_reserve_for_stream(ws, want, s, "query-major workspace");
 		if (rc)
@@ -1315,9 +1362,7 @@ int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_
 	w->head = (uint32_t *)ws->ptr;
 	w->spill = (unsigned long long *)((char *)ws->ptr + WS_HEAD);
 	w->order = (uint32_t *)((char *)ws->ptr + WS_HEAD + spill);
-	w->qitems = (uint32_t *)((char *)w->order + qj_order_bytes(nq));
-	w->sums = (uint32_t *)((char *)w->qitems + qj_order_bytes(nq + 1));
-	qj_order(d_skidx, nq, s, *w);
+	qj_order(d_skidx, nq, s, *w, order);
 	return 0;
 }
 
@@ -1524,19 +1569,18 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 	if (rc)
 		return rc;
 	// fewer queries than twice the resident workgroups: the queries' keys
-	// are cut into ranges of `chunk` keys, enough items for every
-	// workgroup; each item adds its table into the rows
+	// are cut into key ranges that the workgroups take from a cursor
+	// (guided sizes, at least QJ_MIN_CHUNK keys), so every workgroup has
+	// work until the end; each range adds its table into the rows
 	const uint64_t full = (uint64_t)ix->qj_per_cu[uniq ? 1 : 0] * (uint64_t)(ix->cus > 0 ? ix->cus : 256);
 	uint32_t chunk = 0;
 	if ((uint64_t)ja.nq < 2 * full && ja.total > 0) {
-		uint64_t c = (ja.total + 4 * full - 1) / (4 * full);
-		c = std::max<uint64_t>(QJ_MIN_CHUNK, (c + 63) / 64 * 64);
-		chunk = (uint32_t)std::min<uint64_t>(c, 1u << 30);
-		sh.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(full, (uint64_t)ja.nq + ja.total / chunk));
+		chunk = (uint32_t)QJ_MIN_CHUNK;
+		sh.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(full, (ja.total + chunk - 1) / chunk));
 		sh.spill_bytes = sh.direct ? 0 : (size_t)sh.grid * ix->ref_gnum * (uniq ? 32u : 16u);
 	}
 	QJWork w;
-	if ((rc = qj_workspace(ws, sh, ja.skidx, (uint32_t)ja.nq, s, &w)))
+	if ((rc = qj_workspace(ws, sh, ja.skidx, (uint32_t)ja.nq, s, &w, chunk == 0)))
 		return rc;
 	QJArgs a;
 	memset(&a, 0, sizeof a);
@@ -1557,10 +1601,6 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 	a.uctx = uniq ? ja.uctx : nullptr;
 	a.uobj = uniq ? ja.uobj : nullptr;
 	if (chunk) {
-		const uint32_t n1 = (uint32_t)ja.nq + 1;
-		hipLaunchKernelGGL(k_item_counts, dim3(std::min<uint32_t>((n1 + 255) / 256, 1024u)), dim3(256), 0, s, ja.skidx,
-				   (uint32_t)ja.nq, chunk, w.qitems);
-		kssd_scan_u32_excl(w.qitems, n1, w.sums, s);
 		if (a.mode == QJ_DENSE) {
 			const size_t plane = (size_t)ja.nq * (size_t)ix->ref_gnum * 8;
 			qj_zero(a.ctx, plane, s);
@@ -1572,7 +1612,6 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 		}
 		a.mode = QJ_ACCUM;
 		a.chunk = chunk;
-		a.qitems = w.qitems;
 	}
 	return qj_launch(sh, uniq, a, s);
 }
@@ -1890,7 +1929,7 @@ extern "C" int kssdhip_select_block_dev_noalloc(kssdhip_index *ix, const uint64_
 	w.spill = (unsigned long long *)((char *)base + NOALLOC_SPILL_AT);
 	if (!sh.direct)
 		qj_zero(w.spill, sh.spill_bytes, c.s); // the last call's contents stay in the array
-	qj_order(d_skidx, (uint32_t)nq, c.s, w);
+	qj_order(d_skidx, (uint32_t)nq, c.s, w, true);
 	return select_launch(ix, c, sh, w);
 }
 

@@ -469,18 +469,21 @@ def test_record_table_overflow_and_long_contexts(hip, oracle, obj_bits):
         hip.close_index(h)
 
 
+@pytest.mark.parametrize("qconf", [0.5, 1.0])
 @pytest.mark.parametrize("flags", [0, 2, 4, 6, 8 | 2])
-def test_query_major_key_range_items(hip, oracle, flags):
-    """Blocks of few large queries are cut into key-range items of 1,024 keys
-    that add into the rows: 6 queries of about 2,000-9,000 keys whose context
-    groups (several objects per context) straddle the item boundaries,
-    against the oracle for every flag set; with ACCUMULATE the rows add to
-    what they held."""
+def test_query_major_key_range_items(hip, oracle, flags, qconf):
+    """Blocks of few large queries are cut into key-range items (rounds of
+    halving sizes, then 1,024 keys) that may span queries and add into the
+    rows: 6 queries of about 2,000-9,000 keys, one empty, whose context
+    groups (a second object for a share qconf of the contexts; every context
+    at 1.0) straddle the item boundaries, so that a group headed before an
+    item's end reads its keys past it; against the oracle for every flag
+    set; with ACCUMULATE the rows add to what they held."""
     rng = np.random.default_rng(77)
     pool = rng.integers(1, 1 << 43, size=40000, dtype=np.uint64)
     base = rng.integers(0, 1 << OBJ_BITS, size=len(pool), dtype=np.uint64)
     refs = fast_samples(rng, 300, pool, base, 600, conflict_p=0.3)
-    qrys = fast_samples(rng, 6, pool, base, 2500, conflict_p=0.5)
+    qrys = fast_samples(rng, 6, pool, base, 2500, conflict_p=qconf)
     qrys[2] = np.zeros(0, np.uint64)
     rk, ri = _block(refs)
     qk, qi = _block(qrys)
