@@ -153,10 +153,13 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  * it to the outputs once (hits on other genomes go to the outputs
  * directly), which needs chunk_records <= 6144.
  * query_threads (mode 3; 0 = default 512): threads per query-major
- * workgroup, 64..512, multiple of 64.  query_slots (mode 3; 0 = default
- * 512): LDS counter slots per query when ref_gnum exceeds what a direct
- * per-genome LDS row holds, a power of two in 256..8192; a query that
- * touches more genomes spills the rest to a per-workgroup row in HBM.
+ * workgroup, 64..512, multiple of 64.  query_slots (mode 3): 0 = a direct
+ * per-genome LDS row when it fits (ref_gnum x 16 bytes, 32 with unique
+ * counters, up to 64 KiB), else 512 hashed slots; otherwise the slots of a
+ * hashed LDS table per query, a power of two in 256..8192, also where the
+ * direct row would fit (a smaller table lets more workgroups share a CU); a
+ * query that touches more genomes than the slots spills the rest to a
+ * per-workgroup row in HBM.
  */
 typedef struct {
 	int mode;

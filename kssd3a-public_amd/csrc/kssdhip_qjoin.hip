@@ -1270,7 +1270,9 @@ void qj_layout(const kssdhip_index *ix, bool uniq, QJShape *sh)
 	const uint32_t R = (uint32_t)ix->ref_gnum;
 	const uint32_t cell_bytes = uniq ? 32u : 16u;
 	sh->threads = ix->qthreads ? ix->qthreads : DEFAULT_THREADS;
-	sh->direct = (uint64_t)R * cell_bytes <= DIRECT_LDS_MAX;
+	// a direct per-genome row when it fits, unless slots were asked for (a
+	// smaller hashed table lets more workgroups share a CU)
+	sh->direct = !ix->qslots && (uint64_t)R * cell_bytes <= DIRECT_LDS_MAX;
 	if (sh->direct) {
 		sh->slots = 0;
 		sh->sbits = 0;

@@ -237,9 +237,12 @@ std::vector<kssdhip_tiling_t> candidates(uint64_t n, int cus, int R)
 			v.push_back(tiling(2, p.first, b, p.second));
 	for (int qt : {128, 256, 512})
 		v.push_back(tiling(3, 4096, 256, 1, qt, 0));
-	if ((uint64_t)R * 16 > 64 * 1024) // hashed LDS tables: their size sets the workgroups per CU
-		for (int qt : {256, 512})
-			for (int qs : {256, 1024, 2048})
+	// hashed LDS tables: their size sets the workgroups per CU (where a
+	// direct row fits, small hashed tables leave room for more workgroups)
+	const bool direct_fits = (uint64_t)R * 16 <= 64 * 1024;
+	for (int qt : {256, 512})
+		for (int qs : {256, 512, 1024, 2048})
+			if (direct_fits ? qs <= 512 : qs != 512)
 				v.push_back(tiling(3, 4096, 256, 1, qt, qs));
 	return v;
 }
