@@ -128,7 +128,8 @@ def measured_traffic(tiling, nref, nqry):
     summary of this workload, when it was measured with the tiling this run
     uses; (bytes or None, summary file name)."""
     import re
-    files = sorted((ROOT / "profiles").glob(f"*_c2_{nref}x{nqry}_traffic*.md"))
+    files = [f for f in sorted((ROOT / "profiles").glob(f"*_c2_{nref}x{nqry}*.md"))
+             if "traffic_bytes_per_step=" in f.read_text()]
     if not files:
         return None, None
     for f in files:  # one summary per measured tiling
@@ -144,7 +145,8 @@ def measured_lds(tiling, nref, nqry):
     rocprofv3 summary of this workload when measured with this run's tiling:
     (dict or None, summary file name)."""
     import re
-    files = sorted((ROOT / "profiles").glob(f"*_c2_{nref}x{nqry}_lds*.md"))
+    files = [f for f in sorted((ROOT / "profiles").glob(f"*_c2_{nref}x{nqry}*.md"))
+             if "lds_bytes_per_dispatch=" in f.read_text()]
     if not files:
         return None, None
     for f in files:  # one summary per measured tiling
@@ -156,8 +158,20 @@ def measured_lds(tiling, nref, nqry):
 
 
 def counters_note(tiling):
-    """What bounds k_pjoin at C2, from the committed counter summary when it
-    was measured at this run's tiling."""
+    """What bounds the counting kernel at C2, from the committed counter
+    summary when it was measured at this run's tiling."""
+    if tiling["mode"] == 3:
+        import re
+        for f in sorted((ROOT / "profiles").glob("qjoin_c2_1000x1000_*.md")):
+            text = f.read_text()
+            m = re.search(r"line requests per second over `k_qjoin` \| ([0-9.]+) G/s", text)
+            if m and profile_tiling(text) == tiling_key(tiling):
+                occ = re.search(r"MeanOccupancyPerCU \| ([0-9.]+)", text)
+                return {"source": f"profiles/{f.name}", "k_qjoin_line_requests_G_per_s": float(m.group(1)),
+                        "waves_per_cu": float(occ.group(1)) if occ else None,
+                        "reading": "random 128-byte line fetches of the record table (one home line per query "
+                                   "key); the line requests run near the probe's random-line rate"}
+        return None
     vals, src = measured_profile("pjoin_c2_1000x1000_counters")
     if not vals or tiling_key(tiling) != (2, 5696, 1024, 4):
         return None
