@@ -472,12 +472,6 @@ __device__ __forceinline__ uint64_t from_prev_lane64(uint64_t x, uint64_t edge)
 	       from_prev_lane((uint32_t)x, (uint32_t)edge);
 }
 
-__device__ __forceinline__ uint64_t from_next_lane64(uint64_t x, uint64_t edge)
-{
-	return ((uint64_t)from_next_lane((uint32_t)(x >> 32), (uint32_t)(edge >> 32)) << 32) |
-	       from_next_lane((uint32_t)x, (uint32_t)edge);
-}
-
 // inclusive sum over lanes 0..i (row steps, then rows 1 and 3 take the row
 // below, rows 2 and 3 take rows 0-1)
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
@@ -705,6 +699,12 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 	uint64_t wb = k0 + (threadIdx.x & ~63u); // wave-uniform
 	uint64_t key = wb + lane < k1 ? a.keys[wb + lane] : 0ull;
 	uint64_t before = lane == 0 && wb > kb && wb < k1 ? a.keys[wb - 1] : ~0ull; // the key ahead of the wave's first
+	// the key after the last one the wave holds (keys [wb, min(wb + 64, k1)))
+	auto beyond_of = [&](uint64_t b) -> uint64_t {
+		const uint64_t p = min(b + 64, k1);
+		return p < ke ? a.keys[p] : ~0ull;
+	};
+	uint64_t beyond = wb < k1 ? beyond_of(wb) : ~0ull;
 	for (; wb < k1; wb += B) {
 		const uint64_t i = wb + lane;
 		const bool in = i < k1;
@@ -734,19 +734,26 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 		const uint64_t nb = wb + B;
 		const uint64_t nkey = nb + lane < k1 ? a.keys[nb + lane] : 0ull;
 		const uint64_t nbefore = lane == 0 && nb < k1 ? a.keys[nb - 1] : ~0ull;
+		const uint64_t nbeyond = nb < k1 ? beyond_of(nb) : ~0ull;
 #pragma unroll
 		for (uint32_t k = 0; k < 8; ++k)
 			*(LDS u64x2 *)(wst + (8 * g + k) * BR_SLOTS + ((piece ^ k) << 1)) = v[k];
-		// the group: usually this key alone (the next lane's key has another
-		// context); else the keys that follow with the same context, read
-		// from memory (also past the item's end k1, where the next lane
-		// holds no key)
-		const uint64_t after = from_next_lane64(key, 0ull);
-		const bool ends = i + 1 >= ke || (lane < 63 && i + 1 < k1 && (after >> ob) != c);
+		// the group: this key and the keys that follow with the same
+		// context.  Bit L of `cont`: lane L's key continues the previous
+		// key's context; a head's group runs over the set bits after its
+		// lane, and when it reaches the last key the wave holds (lane 63, or
+		// the item's end k1) and the key after that (`beyond`) continues it,
+		// the rest is read from memory
+		const uint64_t cont = __ballot(in && i != kb && (prev >> ob) == c);
 		uint32_t gs = 1;
-		if (on && !ends)
-			while (i + gs < ke && (a.keys[i + gs] >> ob) == c)
-				++gs;
+		if (on) {
+			const uint32_t run = lane < 63 ? (uint32_t)__builtin_ctzll(~(cont >> (lane + 1))) : 0u;
+			gs += run;
+			const uint64_t p = min(wb + 64, k1);
+			if (i + run + 1 == p && p < ke && (beyond >> ob) == c)
+				while (i + gs < ke && (a.keys[i + gs] >> ob) == c)
+					++gs;
+		}
 		bool pending = on;
 		for (;;) { // wave-uniform: until every lookup is resolved
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -781,6 +788,7 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 		__builtin_amdgcn_wave_barrier(); // every row read before the next keys' lines land
 		key = nkey;
 		before = nbefore;
+		beyond = nbeyond;
 	}
 }
 

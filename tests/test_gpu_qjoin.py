@@ -50,14 +50,16 @@ def _wide(seed, n_ref, n_qry, pool_n, mean):
 
 
 # (references, queries, context pool, mean entries, query_slots): the LDS
-# table in direct form; hashed without spilling; hashed with most queries
+# table in direct form; hashed for few references (slots asked for where a
+# direct row would fit); hashed without spilling; hashed with most queries
 # touching more genomes than its 256 slots
 SHAPES = {
     "direct": (40, 70, 3000, 200, 0),
+    "small_hashed": (40, 70, 3000, 200, 256),
     "hashed": (4400, 40, 200000, 60, 0),
     "spill": (4400, 24, 2500, 40, 256),
 }
-SEEDS = {"direct": 11, "hashed": 12, "spill": 13}
+SEEDS = {"direct": 11, "small_hashed": 14, "hashed": 12, "spill": 13}
 
 
 @pytest.fixture(scope="module")
