@@ -471,6 +471,44 @@ def test_record_table_overflow_and_long_contexts(hip, oracle, obj_bits):
         hip.close_index(h)
 
 
+@pytest.mark.parametrize("mode_items", [False, True])
+@pytest.mark.parametrize("flags", [0, 2, 4, 6])
+def test_query_major_groups_longer_than_a_wave(hip, oracle, flags, mode_items):
+    """Query-side context groups of 1 to 300 keys (one context, many
+    objects): groups that fill whole 64-key waves, run past a wave's last
+    lane and, with key-range items (few queries), past item ends; the index
+    holds the group's contexts with objects near the group's, so the
+    minimum over the group matters.  Counts against the oracle."""
+    rng = np.random.default_rng(91)
+    pool = rng.integers(1, 1 << 43, size=6000, dtype=np.uint64)
+    base = rng.integers(0, 1 << OBJ_BITS, size=len(pool), dtype=np.uint64)
+    refs = fast_samples(rng, 120, pool, base, 700, conflict_p=0.3)
+    big = rng.choice(len(pool), size=24, replace=False)
+    qrys = []
+    nq = 6 if mode_items else 1100  # 1,100 queries: more than twice the resident workgroups, whole queries
+    for q in range(nq):
+        keys = [fast_samples(rng, 1, pool, base, 400, conflict_p=0.4)[0]]
+        for c in rng.choice(big, size=3, replace=False):
+            n = int(rng.choice([2, 63, 64, 65, 130, 300]))
+            objs = np.unique((base[c] ^ rng.integers(0, 1 << OBJ_BITS, size=n, dtype=np.uint64)
+                              & np.uint64((1 << OBJ_BITS) - 1)))
+            keys.append((pool[c] << np.uint64(OBJ_BITS)) | objs)
+        qrys.append(np.unique(np.concatenate(keys)))
+    rk, ri = _block(refs)
+    qk, qi = _block(qrys)
+    R = len(refs)
+    rec = kt.build_index_np(rk, ri, OBJ_BITS)
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=3)
+        got = hip.count_block(h, R, qk, qi, flags=flags)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, R, OBJ_BITS, qk, qi, flags=flags)
+    for name, g, w in zip(("ctx", "obj", "unique_ctx", "unique_obj"), got, want):
+        assert np.array_equal(g, w), (flags, mode_items, name)
+
+
 @pytest.mark.parametrize("qconf", [0.5, 1.0])
 @pytest.mark.parametrize("flags", [0, 2, 4, 6, 8 | 2])
 def test_query_major_key_range_items(hip, oracle, flags, qconf):
