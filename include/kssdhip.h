@@ -130,21 +130,23 @@ size_t kssdhip_index_device_bytes(const kssdhip_index *index);
  * Kernel shape of the counting path for a handle.  No reference counterpart
  * (the reference's block loop has no device tiling); the project's
  * autotuner sets it.
- *   mode 0 auto: per call, from the block's shape: the lookup kernel for
- *          small blocks (fewer than auto_min_keys keys per index
- *          partition), the partitioned merge-join when queries carry
- *          many keys per index partition (dense blocks), the query-major
- *          join otherwise;
+ *   mode 0 auto: the query-major join whenever the index has a record
+ *          table (indexes of fewer than 2^20 genomes); else, per call, the
+ *          lookup kernel for small blocks (fewer than auto_min_keys keys
+ *          per index partition) and the partitioned merge-join otherwise;
  *        1 lookup: one thread per query key, bucket-table lookup into HBM,
  *          counters updated with global atomics;
  *        2 partitioned: one workgroup per index partition of about
  *          chunk_records records staged in LDS; queries are split at the
  *          partition boundaries and merged against the staged records;
  *        3 query-major: persistent workgroups take whole queries from a
- *          device work queue (largest first), look each context up through
- *          the index's context filter and open-addressed context table and
- *          sum the query's counters in an LDS table, then write the query's
- *          row once (no counter atomics in HBM).
+ *          device work queue (largest first) or, when the block has fewer
+ *          queries than twice the resident workgroups, key-range items
+ *          (rounds of halving sizes, then 1,024 keys); each query context
+ *          is looked up in the index's record table (one 128-byte home
+ *          line), the query's counters are summed in an LDS table and
+ *          written once per query (items add theirs into the outputs; no
+ *          per-match counter atomics in HBM).
  * chunk_records (partitions are rebuilt when it changes) must leave the
  * staged partition within the 160 KB LDS: 64 <= chunk_records <= 12288.
  * group_partitions (1..64): consecutive partitions one workgroup walks;

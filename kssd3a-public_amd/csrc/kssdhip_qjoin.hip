@@ -48,7 +48,6 @@ constexpr int QJ_MAX_THREADS = 512; // workgroup size bound of k_qjoin
 #endif
 constexpr uint32_t DEFAULT_SLOTS = 512;
 constexpr uint64_t QJ_MIN_CHUNK = 1024; // least keys per work item when queries are cut into key ranges
-constexpr uint64_t QJ_AUTO_MIN_KEYS = 1u << 20; // blocks of few queries take the query-major join from this many keys
 // workspace: [0, WS_HEAD) work counter and the order's histogram and
 // cursors (65 words), then the spill rows, then the work order (nq words,
 // WS_ALIGN-rounded)
@@ -1556,17 +1555,13 @@ int kssd_qjoin_prepare(kssdhip_index *ix)
 
 bool kssd_qjoin_auto(const kssdhip_index *ix, int nq, uint64_t total)
 {
-	// the partitioned merge-join wins when each query brings about two or
-	// more keys to every index partition; the query-major join when the
-	// index is so much larger than a query that most (query, partition)
-	// pairs are empty; blocks of fewer queries than compute units go to the
-	// one-thread-per-key lookup
-	if (!ix->d_brec || nq < 1)
-		return false;
-	const double parts = ix->P ? (double)ix->P : 1.0;
-	if ((double)total >= 2.0 * parts * (double)nq)
-		return false;
-	return nq >= (ix->cus > 0 ? ix->cus : 256) || total >= QJ_AUTO_MIN_KEYS;
+	// the query-major join over the record table, at every block shape: at
+	// the handles' default shapes it counted the 60-genome self matrix in
+	// 45 us (partitioned merge-join 320 us), 1,000 x 1,000 clade genomes in
+	// 0.45 ms (2.4 ms) and one assembly against 10,000 genomes in 26 us
+	// (lookup kernel 375 us); indexes without a record table keep the
+	// other kernels
+	return ix->d_brec && nq >= 1 && total > 0;
 }
 
 int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s, DevBuf *ws)
