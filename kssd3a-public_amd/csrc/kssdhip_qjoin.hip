@@ -1,25 +1,26 @@
-// Query-major join: the counting path for blocks whose queries carry few
-// keys per index partition (indexes much larger than one query), and the
-// device-side pass gate of the index path.
+// Query-major join: the default counting path of an index with a record
+// table, and the device-side pass gate of the index path.
 //
 // The reference counts one query per OpenMP iteration: every context group
 // of the query finds its first index record through the fenceposts, walks
 // the context's per-genome runs and adds into the query's own row of the
 // dense counters (command_ani.h:481-575).  Here a persistent workgroup takes
-// whole queries from a device work queue.  Its threads stride over the
-// query's keys; the first key of each context group tests the context
-// against a word-blocked Bloom filter of the index's contexts (most query
-// contexts are absent from the index and stop there), finds the context's
-// first record and run length in an open-addressed context table, walks the
-// context's 16-byte {ctxgid, obj} records genome run by genome run and adds
-// each pair's counters into an LDS table of the query (one
-// cell per genome when the reference set is small enough, else an
-// open-addressed table keyed by genome; a genome that finds no slot goes to
-// the workgroup's spill row in HBM).  When the query is done the workgroup
-// writes its dense row once, or applies the pass gate of the ranking loop
-// (command_ani.h:585-606) to the touched pairs and appends the passing ones
-// to a compact list.  No counter is updated with a global atomic, and no
-// dense block has to be zeroed first.
+// whole queries from a device work queue (largest first), or, for blocks of
+// few queries, key-range items whose sizes halve round by round.  Each wave
+// takes 64 consecutive keys; the first key of each context group looks its
+// context up in the record table, where one 128-byte home line holds the
+// context's records (or a pointer into the sorted index for long ones): the
+// wave's 64 lines are loaded eight lanes to a line into LDS rows, each lane
+// compares its row's slots with its context's remainder, and the wave walks
+// the found records one per lane, adding each genome run's counters into an
+// LDS table of the query (one cell per genome when the reference set is
+// small enough and no slot count is asked for, else an open-addressed table
+// keyed by genome; a genome that finds no slot goes to the workgroup's
+// spill row in HBM).  When the query (or item) is done the workgroup writes
+// its dense row once, adds it into the outputs, or applies the pass gate of
+// the ranking loop (command_ani.h:585-606) to the touched pairs and appends
+// the passing ones to a compact list.  No counter is updated per match with
+// a global atomic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
