@@ -730,7 +730,48 @@ def c3_bench(args, host, threads):
     return res, shared
 
 
-def index_bench(args, host, threads, plain_s_per_pair, shared):
+def c4_profile(tiling):
+    """The committed rocprofv3 summary of the C4 count when it was measured
+    at this run's tiling: (figures or None, summary file name)."""
+    import re
+    for f in sorted((ROOT / "profiles").glob("qjoin_c4_*.md")):
+        text = f.read_text()
+        if profile_tiling(text) == tiling_key(tiling):
+            return ({k: float(v) for k, v in re.findall(r"^(\w+)=([0-9.eE+-]+)$", text, re.M)},
+                    f"profiles/{f.name}")
+    return None, None
+
+
+def c4_index(args, host, threads):
+    """The C3 reference index alone (10,000 clade genomes, no queries), for
+    runs of the C4 count by itself."""
+    rng = np.random.default_rng(args.seed + 200)
+    nref = args.index_refs - args.index_refs % 20
+    specs, cseed, clen = clade_specs(rng, nref, args.seed + 200)
+    cache = Path(args.data_cache) / f"c4idx_{nref}_{args.seed}" if args.data_cache else None
+    if cache is not None and (cache / "ri.npy").exists():
+        rk, ri = (np.load(cache / f"{x}.npy") for x in ("rk", "ri"))
+    else:
+        rk, ri = host.simsketch(specs, threads=threads)
+        if cache is not None:
+            cache.mkdir(parents=True, exist_ok=True)
+            np.save(cache / "rk.npy", rk)
+            np.save(cache / "ri.npy", ri)
+    hip = kh.KssdHip(0)
+    rt = kh.Hip(0)
+    rec = host.build_index(rk, ri, 20, threads=threads)
+    h = hip.open_index(rec, nref, 20)
+    if args.copy_probe_mib:  # a copy of known size: calibrates the HBM counters of a profiled run
+        nb_ = args.copy_probe_mib << 20
+        a_, b_ = rt.malloc(nb_), rt.malloc(nb_)
+        rt.check(rt.lib.hipMemcpy(b_, a_, nb_, rt.D2D))
+        rt.free(a_)
+        rt.free(b_)
+    return {"hip": hip, "rt": rt, "h": h, "rec": rec, "rctx": host.ctx_counts(rk, ri, 20), "specs": specs,
+            "qk": np.zeros(0, np.uint64), "qi": np.zeros(1, np.uint64), "nref": nref}
+
+
+def index_bench(args, host, threads, plain_s_per_pair, shared, names=("assemblies", "read_sets")):
     """The C3 index (resident) against two query sets counted in one batched
     launch each into dense blocks, with inputs resident in HBM:
       assemblies  the first args.index_queries of the C3 queries (one
@@ -766,6 +807,8 @@ def index_bench(args, host, threads, plain_s_per_pair, shared):
                          "sketch_s": time.time() - t}}
     orc = None
     for name, keys, idx, flags in (("assemblies", qk, qi, 0), ("read_sets", rkeys, ridx, kh.UNIQUE)):
+        if name not in names:
+            continue
         n = len(idx) - 1
         cells = n * nref
         unique = bool(flags & kh.UNIQUE)
@@ -795,6 +838,28 @@ def index_bench(args, host, threads, plain_s_per_pair, shared):
         rt.record(e1, stream)
         ms = rt.elapsed_ms(e0, e1) / reps
         pairs_per_s = cells / (ms / 1e3)
+        # the counting call alone: its roofline (B = 8 N_Q + 12 M + 16 P_out,
+        # 32 bytes per output cell with the unique copies)
+        rt.record(e0, stream)
+        for _ in range(reps):
+            hip.count_block_dev(h, d_k, d_i, n, total, 0, flags, d_c, d_o, d_uc, d_uo, stream)
+        rt.record(e1, stream)
+        rt.sync(stream)
+        count_ms = rt.elapsed_ms(e0, e1) / reps
+        M = hip.matched_records(h, d_k, d_i, n, total)
+        algo = 8 * total + 12 * M + (32 if unique else 16) * cells
+        prof, prof_src = c4_profile(tiling) if unique else (None, None)
+        traffic = int(prof["traffic_bytes_per_launch"]) if prof and "traffic_bytes_per_launch" in prof else None
+        roof = {"bound": "hbm", "kernel": "k_qjoin (key-range items)" if tiling["mode"] == 3 else "count",
+                "achieved": algo / (count_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": algo / (count_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": algo,
+                "bytes_model": "8*N_Q + 12*M + %d*P_out (dense output)" % (32 if unique else 16),
+                "N_Q": total, "M": M, "P_out": cells, "count_ms": count_ms,
+                "traffic": traffic, "traffic_source": prof_src,
+                "traffic_GBs": traffic / (count_ms / 1e3) / 1e9 if traffic else None,
+                "line_requests_per_launch": prof.get("dram_read_requests_per_launch") if prof else None,
+                "lines_per_key": prof["dram_read_requests_per_launch"] / total
+                if prof and "dram_read_requests_per_launch" in prof else None}
         # oracle on a bounded prefix (fenceposts rebuilt per block, as the reference)
         if orc is None:
             sys.path.insert(0, str(ROOT / "tests"))
@@ -821,20 +886,20 @@ def index_bench(args, host, threads, plain_s_per_pair, shared):
             exact = exact and bool(np.array_equal(rt.download(d_uc, np.uint32, 2 * ncpu * nref).reshape(ncpu, nref, 2),
                                                   c_uc))
         cpu_pps = ncpu * nref / cdt
-        plain_pps = 1.0 / plain_s_per_pair
+        plain_pps = 1.0 / plain_s_per_pair if plain_s_per_pair else None
         out[name] = {**out.get(name, {}), "queries": n, "query_entries": total, "pairs": cells, "ms": ms,
                      "pairs_per_s": pairs_per_s,
-                     "over_plain": pairs_per_s / plain_pps, "cpu_pairs_per_s": cpu_pps, "cpu_queries": ncpu,
-                     "over_cpu": pairs_per_s / cpu_pps, "counts_bit_exact_on_cpu_sample": exact,
-                     "flags": "unique-ctx" if unique else "none", "tiling": tiling}
+                     "over_plain": pairs_per_s / plain_pps if plain_pps else None, "cpu_pairs_per_s": cpu_pps,
+                     "cpu_queries": ncpu, "over_cpu": pairs_per_s / cpu_pps, "counts_bit_exact_on_cpu_sample": exact,
+                     "flags": "unique-ctx" if unique else "none", "tiling": tiling, "roofline": roof}
         log(f"index bench {name}: {n} x {nref} in {ms:.2f} ms = {pairs_per_s:.3e} pairs/s "
-            f"({pairs_per_s / plain_pps:.0f}x plain, {pairs_per_s / cpu_pps:.0f}x CPU on {ncpu} queries), "
-            f"bit-exact {exact}")
+            f"({pairs_per_s / plain_pps if plain_pps else 0:.0f}x plain, {pairs_per_s / cpu_pps:.0f}x CPU on "
+            f"{ncpu} queries), bit-exact {exact}; count {count_ms:.3f} ms, {roof['achieved']:.0f} GB/s algorithmic")
         for d in (d_k, d_i, d_qc, d_rc, d_c, d_o, d_out, d_uc, d_uo):
             if d:
                 rt.free(d)
         hip.set_tiling(h, mode=0)
-    if args.index_stream_queries:
+    if args.index_stream_queries and "assemblies" in names:
         out["streamed"] = stream_bench(args, host, hip, rt, h, shared["qk"], shared["qi"], nref, rctx)
     out["cpu_threads"] = threads
     return out
@@ -1272,6 +1337,8 @@ def main():
     ap.add_argument("--data-cache", default="",
                     help="directory to keep the C3 sketches in between runs (profiling passes)")
     ap.add_argument("--c2-only", action="store_true", help="only the C2 step (for profiling runs)")
+    ap.add_argument("--c4-only", action="store_true",
+                    help="only the C4 count (200 read sets vs the 10k index; for profiling runs)")
     ap.add_argument("--cli-queries", type=int, default=100000,
                     help="C3 queries run end to end through kssd3a-hip ani (0 = skip)")
     ap.add_argument("--serve-refs", type=int, default=20000,
@@ -1302,6 +1369,12 @@ def main():
         rk, ri = host.simsketch(specs, threads=threads)
         _, build = index_build_bench(host, kh.KssdHip(0), rk, ri, threads, reps=3)
         print(json.dumps({"index_build": build, "index_genomes": nref}), flush=True)
+        return
+    if args.c4_only:  # profiling runs of the C4 count alone
+        shared = c4_index(args, host, threads)
+        out = index_bench(args, host, threads, None, shared, names=("read_sets",))
+        shared["hip"].close_index(shared["h"])
+        print(json.dumps({"c4": out["read_sets"]}), flush=True)
         return
     if args.c2_only:  # profiling runs of the C2 step alone
         c3, shared, headline = None, None, {}
