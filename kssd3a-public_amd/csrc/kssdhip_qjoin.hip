@@ -590,6 +590,22 @@ __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, c
 	const bool groups = __any(found && gs > 1); // some key has a group of several keys
 	const bool lower = (a.flags & KSSDHIP_LOWER) != 0;
 	const bool ignore = (a.flags & KSSDHIP_IGNORECONFLICT) != 0;
+	if (!groups && !__any(mycnt > 1)) {
+		// every found context has one record and every group one key: each
+		// key's lane counts its own record (one genome run of one record)
+		if (mycnt) {
+			const uint64_t rec = qj_record(a, wst, lane, lng, src, objmask);
+			const uint32_t gid = (uint32_t)(rec >> 32);
+			int mm = dna_popcount_dev(qobj ^ (uint32_t)rec);
+			mm = mm < MIN_INIT ? mm : MIN_INIT;
+			if (!(lower && (int)gid >= qgid)) {
+				qj_add<UNIQ, DIRECT>(t, gid, mm, false, spill_row, spilled);
+				if (UNIQ) // the context's only run: its sole owner
+					qj_add<UNIQ, DIRECT>(t, gid, mm, true, spill_row, spilled);
+			}
+		}
+		return;
+	}
 	LDS uint8_t *mark = (LDS uint8_t *)(own + 128);
 	uint32_t carry_gid = NONE, carry_owner = 64;
 	for (uint32_t r0 = 0; r0 < total; r0 += 64) { // wave-uniform

@@ -471,6 +471,37 @@ def test_record_table_overflow_and_long_contexts(hip, oracle, obj_bits):
         hip.close_index(h)
 
 
+@pytest.mark.parametrize("flags", [0, 1, 2, 4, 6])
+def test_query_major_single_record_contexts(hip, oracle, flags):
+    """Sketches without second objects over a large context pool, so that most
+    contexts an index holds have one record and most waves find only such
+    contexts (each key's lane then counts its own record); a share of the
+    pool shared more widely keeps some waves on the wave-wide walk.  LOWER
+    calls count the block against itself below each query's gid."""
+    rng = np.random.default_rng(123)
+    pool = rng.integers(1, 1 << 43, size=4000000, dtype=np.uint64)  # 4M contexts: about 7 of 8 waves take the per-lane path
+    base = rng.integers(0, 1 << OBJ_BITS, size=len(pool), dtype=np.uint64)
+    refs = fast_samples(rng, 300, pool, base, 900, conflict_p=0.0)
+    hot = pool[:300]
+    refs = [np.unique(np.concatenate([r, (hot[rng.integers(0, 300, 20)] << np.uint64(OBJ_BITS)) |
+                                      base[:20]])) for r in refs]
+    qrys = refs[:40] if flags & 1 else fast_samples(rng, 600, pool, base, 900, conflict_p=0.0)
+    rk, ri = _block(refs)
+    qk, qi = _block(qrys)
+    R = len(refs)
+    rec = kt.build_index_np(rk, ri, OBJ_BITS)
+    h = hip.open_index(rec, R, OBJ_BITS)
+    try:
+        hip.set_tiling(h, mode=3)
+        got = hip.count_block(h, R, qk, qi, flags=flags)
+    finally:
+        hip.close_index(h)
+    want = oracle.count_block(rec, R, OBJ_BITS, qk, qi, flags=flags)
+    for name, g, w in zip(("ctx", "obj", "unique_ctx", "unique_obj"), got, want):
+        assert np.array_equal(g, w), (flags, name)
+    assert (got[0][:, :, 0] > 0).sum() > 0
+
+
 @pytest.mark.parametrize("mode_items", [False, True])
 @pytest.mark.parametrize("flags", [0, 2, 4, 6])
 def test_query_major_groups_longer_than_a_wave(hip, oracle, flags, mode_items):
