@@ -331,6 +331,15 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t n)
 		p[i] = 0;
 }
 
+// n words of each output plane (blockIdx.y: plane 0..3) in one launch
+__global__ __launch_bounds__(256) void k_zero_planes(unsigned long long *p0, unsigned long long *p1,
+						     unsigned long long *p2, unsigned long long *p3, uint64_t n)
+{
+	unsigned long long *p = blockIdx.y == 0 ? p0 : blockIdx.y == 1 ? p1 : blockIdx.y == 2 ? p2 : p3;
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		p[i] = 0;
+}
+
 struct QJArgs {
 	// index: the record table (2^bb home lines, then overflow lines) and the
 	// sorted records (long contexts)
@@ -1623,14 +1632,11 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 	a.uctx = uniq ? ja.uctx : nullptr;
 	a.uobj = uniq ? ja.uobj : nullptr;
 	if (chunk) {
-		if (a.mode == QJ_DENSE) {
-			const size_t plane = (size_t)ja.nq * (size_t)ix->ref_gnum * 8;
-			qj_zero(a.ctx, plane, s);
-			qj_zero(a.objc, plane, s);
-			if (uniq) {
-				qj_zero(a.uctx, plane, s);
-				qj_zero(a.uobj, plane, s);
-			}
+		if (a.mode == QJ_DENSE) { // the output planes, in one launch
+			const uint64_t words = (uint64_t)ja.nq * (uint64_t)ix->ref_gnum;
+			const unsigned gx = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((words + 1023) / 1024, 8192));
+			hipLaunchKernelGGL(k_zero_planes, dim3(gx, uniq ? 4u : 2u), dim3(256), 0, s, a.ctx, a.objc,
+					   uniq ? a.uctx : nullptr, uniq ? a.uobj : nullptr, words);
 		}
 		a.mode = QJ_ACCUM;
 		a.chunk = chunk;
