@@ -331,10 +331,15 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t n)
 		p[i] = 0;
 }
 
-// n words of each output plane (blockIdx.y: plane 0..3) in one launch
+// n words of each output plane (blockIdx.y: plane 0..3) in one launch, and
+// the workspace head's words
 __global__ __launch_bounds__(256) void k_zero_planes(unsigned long long *p0, unsigned long long *p1,
-						     unsigned long long *p2, unsigned long long *p3, uint64_t n)
+						     unsigned long long *p2, unsigned long long *p3, uint64_t n,
+						     uint32_t *head, uint32_t head_words)
 {
+	if (blockIdx.x == 0 && blockIdx.y == 0)
+		for (uint32_t k = threadIdx.x; k < head_words; k += blockDim.x)
+			head[k] = 0;
 	unsigned long long *p = blockIdx.y == 0 ? p0 : blockIdx.y == 1 ? p1 : blockIdx.y == 2 ? p2 : p3;
 	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
 		p[i] = 0;
@@ -1362,9 +1367,10 @@ void qj_zero(void *p, size_t bytes, hipStream_t s)
 }
 
 // zeroes the head and builds the work order (largest queries first)
-void qj_order(const uint64_t *d_skidx, uint32_t nq, hipStream_t s, const QJWork &w, bool order)
+void qj_order(const uint64_t *d_skidx, uint32_t nq, hipStream_t s, const QJWork &w, bool order, bool head = true)
 {
-	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s, w.head, WS_HEAD_WORDS);
+	if (head)
+		hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s, w.head, WS_HEAD_WORDS);
 	if (!order)
 		return;
 	const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nq + 255) / 256, 2048));
@@ -1374,7 +1380,7 @@ void qj_order(const uint64_t *d_skidx, uint32_t nq, hipStream_t s, const QJWork 
 }
 
 int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_t nq, hipStream_t s, QJWork *w,
-		 bool order = true)
+		 bool order = true, bool head = true)
 {
 	const size_t spill = std::max(ws->zero_bytes, sh.spill_bytes);
 	const size_t want = WS_HEAD + spill + qj_order_bytes(nq);
@@ -1397,7 +1403,7 @@ int qj_workspace(DevBuf *ws, const QJShape &sh, const uint64_t *d_skidx, uint32_
 	w->head = (uint32_t *)ws->ptr;
 	w->spill = (unsigned long long *)((char *)ws->ptr + WS_HEAD);
 	w->order = (uint32_t *)((char *)ws->ptr + WS_HEAD + spill);
-	qj_order(d_skidx, nq, s, *w, order);
+	qj_order(d_skidx, nq, s, *w, order, head);
 	return 0;
 }
 
@@ -1610,8 +1616,10 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 		sh.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(full, (ja.total + chunk - 1) / chunk));
 		sh.spill_bytes = sh.direct ? 0 : (size_t)sh.grid * ix->ref_gnum * (uniq ? 32u : 16u);
 	}
+	// key-range items into dense outputs: the head is cleared with the planes
+	const bool zero_planes = chunk && !(ja.flags & KSSDHIP_ACCUMULATE);
 	QJWork w;
-	if ((rc = qj_workspace(ws, sh, ja.skidx, (uint32_t)ja.nq, s, &w, chunk == 0)))
+	if ((rc = qj_workspace(ws, sh, ja.skidx, (uint32_t)ja.nq, s, &w, chunk == 0, !zero_planes)))
 		return rc;
 	QJArgs a;
 	memset(&a, 0, sizeof a);
@@ -1632,11 +1640,11 @@ int kssd_launch_qjoin_dense(kssdhip_index *ix, const JoinArgs &ja, hipStream_t s
 	a.uctx = uniq ? ja.uctx : nullptr;
 	a.uobj = uniq ? ja.uobj : nullptr;
 	if (chunk) {
-		if (a.mode == QJ_DENSE) { // the output planes, in one launch
+		if (zero_planes) { // the output planes and the head, in one launch
 			const uint64_t words = (uint64_t)ja.nq * (uint64_t)ix->ref_gnum;
 			const unsigned gx = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((words + 1023) / 1024, 8192));
 			hipLaunchKernelGGL(k_zero_planes, dim3(gx, uniq ? 4u : 2u), dim3(256), 0, s, a.ctx, a.objc,
-					   uniq ? a.uctx : nullptr, uniq ? a.uobj : nullptr, words);
+					   uniq ? a.uctx : nullptr, uniq ? a.uobj : nullptr, words, w.head, WS_HEAD_WORDS);
 		}
 		a.mode = QJ_ACCUM;
 		a.chunk = chunk;
