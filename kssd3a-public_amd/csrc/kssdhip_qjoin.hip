@@ -453,11 +453,14 @@ __device__ __forceinline__ void qj_add(const QTable &t, uint32_t gid, int m, boo
 // lines), and a lane then reads its own row.  Piece p of row m sits at
 // position p ^ (m & 7), so that both the eight lanes writing a row and
 // eight consecutive lanes reading their rows meet distinct banks.  Per wave
-// the rows are followed by OWN_WAVE words: the sole-owner bookkeeping of
-// unique counters (128 words), then the walk's 64 one-byte key marks.
+// the rows are followed by own_wave(uniq) words: the sole-owner bookkeeping
+// of unique counters (128 words), the walk's 64 one-byte key marks, and with
+// unique counters the objects of the wave's 64 keys (without them, the
+// objects take the first 64 words, which the bookkeeping leaves unused).
 constexpr uint32_t STAGE_ROW = BR_SLOTS * 8;        // bytes per row
 constexpr uint32_t STAGE_WAVE = 64 * STAGE_ROW;     // bytes per wave
-constexpr uint32_t OWN_WAVE = 128 + 16;             // words of walk bookkeeping per wave
+__host__ __device__ constexpr uint32_t own_wave(bool uniq) { return uniq ? 128 + 16 + 64 : 128 + 16; }
+__host__ __device__ constexpr uint32_t own_objs(bool uniq) { return uniq ? 128 + 16 : 0; }
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
 // Cross-lane moves within the 64-lane wave (DPP controls): the whole wave
@@ -579,12 +582,14 @@ __device__ __forceinline__ uint64_t qj_record(const QJArgs &a, const LDS unsigne
 // (reading any further records of the run itself).  With unique counters
 // each key's runs are counted in `own` (per lane of the wave: runs, then
 // the last run's genome, minimum and whether it was counted), so that the
-// key's lane applies the sole-owner rule afterwards.  Called by the whole
+// key's lane applies the sole-owner rule afterwards.  The group's other
+// objects come from the wave's copy of its keys' objects (the first nv
+// keys) and, past those, from the keys in memory.  Called by the whole
 // wave.
 template <bool UNIQ, bool DIRECT>
 __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, const LDS unsigned long long *wst,
 					     LDS uint32_t *own, uint32_t lane, bool found, bool lng, uint32_t src,
-					     uint32_t cnt, uint64_t wb, uint32_t gs, uint64_t key, int qgid,
+					     uint32_t cnt, uint64_t wb, uint32_t nv, uint32_t gs, uint64_t key, int qgid,
 					     unsigned long long *spill_row, LDS uint32_t *spilled)
 {
 	const unsigned ob = a.obj_bits;
@@ -621,6 +626,7 @@ __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, c
 		return;
 	}
 	LDS uint8_t *mark = (LDS uint8_t *)(own + 128);
+	const LDS uint32_t *qo = own + own_objs(UNIQ);
 	uint32_t carry_gid = NONE, carry_owner = 64;
 	for (uint32_t r0 = 0; r0 < total; r0 += 64) { // wave-uniform
 		const uint32_t tt = r0 + lane;
@@ -649,7 +655,9 @@ __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, c
 			mm = mm < MIN_INIT ? mm : MIN_INIT;
 			if (groups) {
 				for (uint32_t x = 1; x < ogs && mm; ++x) {
-					const int pc = dna_popcount_dev((uint32_t)(a.keys[oi + x] & objmask) ^ obj);
+					const uint32_t y = owner + x;
+					const uint32_t o = y < nv ? qo[y] : (uint32_t)(a.keys[oi + x] & objmask);
+					const int pc = dna_popcount_dev(o ^ obj);
 					mm = pc < mm ? pc : mm;
 				}
 			}
@@ -721,7 +729,7 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 	const uint32_t lane = threadIdx.x & 63u, g = lane >> 3, piece = lane & 7u;
 	LDS unsigned long long *wst = stage + (threadIdx.x >> 6) * (STAGE_WAVE / 8);
 	LDS unsigned long long *row = wst + lane * BR_SLOTS;
-	LDS uint32_t *own = (LDS uint32_t *)(stage + (blockDim.x >> 6) * (STAGE_WAVE / 8)) + (threadIdx.x >> 6) * OWN_WAVE;
+	LDS uint32_t *own = (LDS uint32_t *)(stage + (blockDim.x >> 6) * (STAGE_WAVE / 8)) + (threadIdx.x >> 6) * own_wave(UNIQ);
 	const int rsh = a.w - a.bb;
 	const uint64_t remmask = (1ull << rsh) - 1ull;
 	const int hs = br_rem_shift(ob) - 32;
@@ -784,6 +792,10 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 				while (i + gs < ke && (a.keys[i + gs] >> ob) == c)
 					++gs;
 		}
+		// the objects of the wave's keys [wb, wb + nv), for the walk's group minima
+		const uint32_t nv = (uint32_t)min((uint64_t)64, k1 - wb);
+		if (__any(gs > 1u) && lane < nv)
+			own[own_objs(UNIQ) + lane] = (uint32_t)key & (ob >= 32 ? 0xffffffffu : (1u << ob) - 1u);
 		bool pending = on;
 		for (;;) { // wave-uniform: until every lookup is resolved
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -801,8 +813,8 @@ __device__ void qj_count(const QJArgs &a, const QTable &t, uint32_t q, uint64_t 
 				src = (uint32_t)pay;
 				cnt = (uint32_t)((pay >> 32) & BR_MAX_LONG);
 			}
-			qj_wave_walk<UNIQ, DIRECT>(a, t, wst, own, lane, found, lng, src, cnt, wb, gs, key, qgid, spill_row,
-						   spilled);
+			qj_wave_walk<UNIQ, DIRECT>(a, t, wst, own, lane, found, lng, src, cnt, wb, nv, gs, key, qgid,
+						   spill_row, spilled);
 			pending = pending && !found && (lasthi >> 30) == 2u; // slot 15 continues the home
 			if (!__any(pending))
 				break;
@@ -1079,7 +1091,7 @@ __global__ __launch_bounds__(QJ_MAX_THREADS) __attribute__((amdgpu_waves_per_eu(
 	unsigned long long *spill_row = DIRECT ? nullptr : a.spill + (uint64_t)blockIdx.x * a.R * (UNIQ ? 4u : 2u);
 	{ // the walk's marks and sole-owner counts start at zero (each lane clears what it used)
 		LDS uint32_t *own = (LDS uint32_t *)(stage + (blockDim.x >> 6) * (STAGE_WAVE / 8));
-		for (uint32_t k = threadIdx.x; k < (blockDim.x >> 6) * OWN_WAVE; k += blockDim.x)
+		for (uint32_t k = threadIdx.x; k < (blockDim.x >> 6) * own_wave(UNIQ); k += blockDim.x)
 			own[k] = 0;
 	}
 	for (;;) {
@@ -1320,7 +1332,7 @@ void qj_layout(const kssdhip_index *ix, bool uniq, QJShape *sh)
 		sh->sbits = ilog2(sh->slots);
 		sh->lds = (size_t)sh->slots * (cell_bytes + 4u);
 	}
-	sh->lds = (sh->lds + 15) / 16 * 16 + (size_t)(sh->threads / 64) * (STAGE_WAVE + OWN_WAVE * 4);
+	sh->lds = (sh->lds + 15) / 16 * 16 + (size_t)(sh->threads / 64) * (STAGE_WAVE + own_wave(uniq) * 4);
 }
 
 int qj_shape(const kssdhip_index *ix, uint32_t nq, bool uniq, QJShape *sh)
