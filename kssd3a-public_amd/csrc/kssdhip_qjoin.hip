@@ -423,13 +423,22 @@ __device__ __forceinline__ uint32_t slot_find(const QTable &t, uint32_t gid, boo
 	return NONE;
 }
 
+// The counters of genome gid: its cell s of the table (the genome itself
+// in a direct row; NONE when the hashed table has no slot left for it).
+template <bool DIRECT>
+__device__ __forceinline__ uint32_t qj_slot(const QTable &t, uint32_t gid)
+{
+	return DIRECT ? gid : slot_find(t, gid, true);
+}
+
+// Adds one run with minimum m into cell s of genome gid (qj_slot), or into
+// the workgroup's spill row when s is NONE.
 template <bool UNIQ, bool DIRECT>
-__device__ __forceinline__ void qj_add(const QTable &t, uint32_t gid, int m, bool ucopy,
+__device__ __forceinline__ void qj_add(const QTable &t, uint32_t s, uint32_t gid, int m, bool ucopy,
 				       unsigned long long *spill_row, LDS uint32_t *spilled)
 {
 	const unsigned long long dc = 1ull | ((unsigned long long)(m > 1) << 32);
 	const unsigned long long dob = 1ull | ((unsigned long long)m << 32);
-	const uint32_t s = DIRECT ? gid : slot_find(t, gid, true);
 	if (s != NONE) {
 		LDS unsigned long long *c = ucopy ? t.uc : t.c;
 		LDS unsigned long long *o = ucopy ? t.uo : t.o;
@@ -618,9 +627,10 @@ __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, c
 			int mm = dna_popcount_dev(qobj ^ (uint32_t)rec);
 			mm = mm < MIN_INIT ? mm : MIN_INIT;
 			if (!(lower && (int)gid >= qgid)) {
-				qj_add<UNIQ, DIRECT>(t, gid, mm, false, spill_row, spilled);
+				const uint32_t s = qj_slot<DIRECT>(t, gid);
+				qj_add<UNIQ, DIRECT>(t, s, gid, mm, false, spill_row, spilled);
 				if (UNIQ) // the context's only run: its sole owner
-					qj_add<UNIQ, DIRECT>(t, gid, mm, true, spill_row, spilled);
+					qj_add<UNIQ, DIRECT>(t, s, gid, mm, true, spill_row, spilled);
 			}
 		}
 		return;
@@ -685,19 +695,21 @@ __device__ __forceinline__ void qj_wave_walk(const QJArgs &a, const QTable &t, c
 		if (lower && (int)gid >= qgid)
 			continue;
 		const bool counted = !(ignore && n > 1);
-		if (UNIQ) {
+		const uint32_t s = counted ? qj_slot<DIRECT>(t, gid) : NONE;
+		if (UNIQ) { // the run's cell (its genome, flagged, when it has none), minimum and whether counted
 			__hip_atomic_fetch_add(&own[owner], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-			own[64 + owner] = gid | ((uint32_t)mmin << 20) | ((uint32_t)counted << 24);
+			own[64 + owner] = (s != NONE ? s : gid | (1u << 25)) | ((uint32_t)mmin << 20) | ((uint32_t)counted << 24);
 		}
 		if (counted)
-			qj_add<UNIQ, DIRECT>(t, gid, mmin, false, spill_row, spilled);
+			qj_add<UNIQ, DIRECT>(t, s, gid, mmin, false, spill_row, spilled);
 	}
 	if (UNIQ) {
 		wave_sync();
 		if (mycnt) {
 			const uint32_t runs = own[lane], info = own[64 + lane];
-			if (runs == 1 && (info >> 24))
-				qj_add<UNIQ, DIRECT>(t, info & GID_MASK, (int)((info >> 20) & 15u), true, spill_row, spilled);
+			if (runs == 1 && ((info >> 24) & 1u))
+				qj_add<UNIQ, DIRECT>(t, (info >> 25) & 1u ? NONE : info & GID_MASK, info & GID_MASK,
+						     (int)((info >> 20) & 15u), true, spill_row, spilled);
 			own[lane] = 0;
 		}
 	}
